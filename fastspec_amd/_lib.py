@@ -1,0 +1,64 @@
+"""Loader of libfsgpu.so (the HIP library).  There is no CPU fallback: if the library is
+missing or no GPU is visible, every call raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+from .abi import CLines, CSetup, CStats
+
+LIB_PATH = Path(__file__).resolve().parent / "libfsgpu.so"
+
+# every symbol declared in include/fsgpu.h
+EXPORTS = (
+    "fsg_init", "fsg_finalize", "fsg_last_error", "fsg_lineop", "fsg_maglineop",
+    "fsg_get_lineop", "fsg_set_lineop", "fsg_get_maglineop", "fsg_radtran",
+    "fsg_magradtran", "fsg_flux", "fsg_lines_upload", "fsg_lines_free", "fsg_lineop_dev",
+    "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop",
+)
+
+_LIB = None
+
+
+class FsgError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise FsgError(f"{LIB_PATH} is not built: run `python -m fastspec_amd.build` "
+                       "(the HIP library is required; there is no CPU path)")
+    lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_LOCAL)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int32)
+    lib.fsg_last_error.restype = C.c_char_p
+    lib.fsg_init.argtypes = [C.POINTER(CSetup), C.POINTER(vp)]
+    lib.fsg_finalize.argtypes = [vp]
+    lib.fsg_lineop.argtypes = [vp, C.c_int32, C.POINTER(CLines), C.c_int64, fp, ip, ip, ip, C.POINTER(CStats)]
+    lib.fsg_maglineop.argtypes = [vp, C.POINTER(CLines), C.c_int64, dp, C.c_int32, fp, ip, ip, ip,
+                                  C.POINTER(CStats)]
+    lib.fsg_get_lineop.argtypes = [vp, dp]
+    lib.fsg_set_lineop.argtypes = [vp, dp]
+    lib.fsg_get_maglineop.argtypes = [vp, dp]
+    lib.fsg_radtran.argtypes = [vp, dp, dp]
+    lib.fsg_magradtran.argtypes = [vp, dp, C.c_int32, C.c_int32, dp, dp]
+    lib.fsg_flux.argtypes = [vp, dp, dp]
+    lib.fsg_lines_upload.argtypes = [vp, C.c_int32, C.POINTER(CLines), C.POINTER(vp)]
+    lib.fsg_lines_free.argtypes = [vp]
+    lib.fsg_lineop_dev.argtypes = [vp, vp, C.c_int64, C.POINTER(CStats)]
+    lib.fsg_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp)]
+    lib.fsg_stream.argtypes = [vp, C.POINTER(vp)]
+    lib.fsg_reset_lineop.argtypes = [vp]
+    _LIB = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise FsgError(load().fsg_last_error().decode())

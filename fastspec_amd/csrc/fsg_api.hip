@@ -1,0 +1,634 @@
+// fsg_api.hip -- the C-ABI of libfsgpu (include/fsgpu.h): device context, uploads, the
+// orchestration of the line-opacity stage (pass 1, width binning, coefficient fill,
+// tiled accumulation) and of the formal solution.
+//
+// The stage order follows LineopCalc (src/synth.f90:747-1095) and RadTranCalc
+// (src/synth.f90:1825-1901); IntegConvol's flux sums (src/convol.f90:233-234) are fused
+// into the formal-solution kernel.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fsgpu.h"
+#include "fsg_kernels.h"
+
+using namespace fsg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &msg) {
+  g_err = msg;
+  return 1;
+}
+
+#define CK(call)                                                                              \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_));       \
+  } while (0)
+
+// grow-only device buffer
+struct DBuf {
+  void *p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) n = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+};
+
+}  // namespace
+
+struct fsg_ctx {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  int precision = FSG_FP64, batched = 0;
+  DevSetup S{};
+  std::vector<void *> owned;
+  void *lineop = nullptr;   // [ntau][nwl] float or double
+  double *spl = nullptr, *spc = nullptr, *sfl = nullptr, *sfc = nullptr;
+  int have_rt = 0;
+  DBuf flag, il, ir, ic, key, key2, idx, idx2, coef, wlc, meta, rng, tmp, pos, wlcl, rflag;
+  int *bincnt = nullptr, *binhmax = nullptr, *binoff = nullptr;
+  unsigned long long *u64 = nullptr;
+  hipEvent_t ev[4] = {};
+};
+
+struct fsg_dlines {
+  fsg_ctx *ctx = nullptr;
+  int list_type = 0;
+  DevLines L{};
+  std::vector<void *> owned;
+};
+
+namespace {
+
+template <typename T>
+int upload(fsg_ctx *c, const T *h, size_t n, const T **dst, std::vector<void *> &owned) {
+  if (!h || n == 0) {
+    *dst = nullptr;
+    return 0;
+  }
+  void *p = nullptr;
+  CK(hipMalloc(&p, n * sizeof(T)));
+  owned.push_back(p);
+  CK(hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, c->st));
+  *dst = (const T *)p;
+  return 0;
+}
+
+int alloc_zero(fsg_ctx *c, size_t bytes, void **out) {
+  void *p = nullptr;
+  CK(hipMalloc(&p, bytes));
+  c->owned.push_back(p);
+  CK(hipMemsetAsync(p, 0, bytes, c->st));
+  *out = p;
+  return 0;
+}
+
+size_t lop_elem(const fsg_ctx *c) { return c->precision == FSG_FP32 ? sizeof(float) : sizeof(double); }
+
+}  // namespace
+
+extern "C" {
+
+const char *fsg_last_error(void) { return g_err.c_str(); }
+
+int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
+  g_err.clear();
+  if (!s || !ctx_out) return fail("fsg_init: NULL argument");
+  *ctx_out = nullptr;
+  if (s->ntau < 3 || s->ntau > DPW * 8) return fail("fsg_init: Ntau must be in 3..144");
+  if (s->nwl < 2) return fail("Init: the wavelength grid has fewer than two points");
+  if (s->nmu < 1 || s->nmu > 16) return fail("fsg_init: Nmu must be in 1..16");
+  if (s->precision != FSG_FP64 && s->precision != FSG_FP32) return fail("fsg_init: unknown precision");
+  if (!s->wave || !s->rhox || !s->temp || !s->xne || !s->rho || !s->contop || !s->fraction_pf ||
+      !s->pot_ion || !s->xamass || !s->frac_h1 || !s->frac_he1 || !s->frac_hmol || !s->spc_ion ||
+      !s->spc_slot || !s->xmu || !s->wmu)
+    return fail("fsg_init: required array is NULL");
+  if (s->motype == 1 && !s->contop_std) return fail("fsg_init: Motype 1 needs ContOpacStd");
+  if (s->nslot < 1 || s->nslot > s->pf_ld) return fail("fsg_init: EOS slot count exceeds FractionPF leading dimension");
+  for (int i = 0; i < s->nspc; i++)
+    if (s->spc_slot[i] < 1 || s->spc_slot[i] > s->nslot || s->spc_ion[i] < 1)
+      return fail("fsg_init: species table refers to an EOS slot out of range");
+  if (s->wmode != 0 && s->wmode != 1) return fail("fsg_init: unknown wavelength grid mode");
+  if (s->nhe < 0 || s->nhe > 255 || (s->nhe > 0 && (!s->he_wl || !s->he_elo || !s->he_gam || !s->he_shf)))
+    return fail("fsg_init: bad He I Stark tables");
+  int ndev = 0;
+  CK(hipGetDeviceCount(&ndev));
+  if (ndev <= 0) return fail("fsg_init: no HIP device");
+  if (s->device < 0 || s->device >= ndev) return fail("fsg_init: device ordinal out of range");
+
+  fsg_ctx *c = new fsg_ctx();
+  c->dev = s->device;
+  c->precision = s->precision;
+  c->batched = s->batched ? 1 : 0;
+  CK(hipSetDevice(c->dev));
+  CK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  for (auto &e : c->ev) CK(hipEventCreate(&e));
+
+  DevSetup &S = c->S;
+  const int N = s->ntau, Nw = s->nwl;
+  S.ntau = N;
+  S.nwl = Nw;
+  S.nmu = s->nmu;
+  S.wmode = s->wmode;
+  S.motype = s->motype;
+  S.pf_ld = s->pf_ld;
+  S.nslot = s->nslot;
+  S.nspc = s->nspc;
+  S.he1_slot = s->he1_slot;
+  S.nhe = s->nhe;
+  S.molbrd = s->molbrd;
+  S.wfirst = s->wfirst;
+  S.dstep = s->dstep;
+  S.rstep = s->rstep;
+  S.wstart = s->wstart;
+  S.eps_opac = s->eps_opac;
+  S.vwfudge = s->vwfudge;
+  S.h2brd = s->h2brd;
+  S.vwmol = s->vwmol;
+  S.vmicro = s->vmicro;
+  S.he_wl_tol = s->he_wl_tol;
+  S.he_elo_tol = s->he_elo_tol;
+  int rc = 0;
+  rc |= upload(c, s->wave, Nw, &S.wave, c->owned);
+  rc |= upload(c, s->temp, N, &S.temp, c->owned);
+  rc |= upload(c, s->rhox, N, &S.rhox, c->owned);
+  rc |= upload(c, s->xne, N, &S.xne, c->owned);
+  rc |= upload(c, s->rho, N, &S.rho, c->owned);
+  rc |= upload(c, s->fraction_pf, (size_t)N * s->pf_ld, &S.fpf, c->owned);
+  rc |= upload(c, s->pot_ion, s->nslot, &S.pot_ion, c->owned);
+  rc |= upload(c, s->xamass, s->nslot, &S.xamass, c->owned);
+  rc |= upload(c, s->spc_ion, s->nspc, &S.spc_ion, c->owned);
+  rc |= upload(c, s->spc_slot, s->nspc, &S.spc_slot, c->owned);
+  rc |= upload(c, s->xmu, s->nmu, &S.xmu, c->owned);
+  rc |= upload(c, s->wmu, s->nmu, &S.wmu, c->owned);
+  if (s->nhe > 0) {
+    rc |= upload(c, s->he_wl, s->nhe, &S.he_wl, c->owned);
+    rc |= upload(c, s->he_elo, s->nhe, &S.he_elo, c->owned);
+    rc |= upload(c, s->he_gam, (size_t)s->nhe * N, &S.he_gam, c->owned);
+    rc |= upload(c, s->he_shf, (size_t)s->nhe * N, &S.he_shf, c->owned);
+  }
+  std::vector<double> cstd(N, 1.0);
+  if (s->motype == 1) std::memcpy(cstd.data(), s->contop_std, sizeof(double) * N);
+  rc |= upload(c, cstd.data(), N, &S.cstd, c->owned);
+  // ContOpac(Ntau, Nwl) -> [ntau][nwl]: wavelength fastest for coalesced per-lambda lanes
+  {
+    std::vector<double> ct((size_t)N * Nw);
+    for (size_t iw = 0; iw < (size_t)Nw; iw++)
+      for (int k = 0; k < N; k++) ct[(size_t)k * Nw + iw] = s->contop[iw * N + k];
+    rc |= upload(c, ct.data(), ct.size(), &S.contT, c->owned);
+    CK(hipStreamSynchronize(c->st));
+  }
+  // per-depth perturber densities and XlineInit
+  const float *fh1, *fhe1, *fhmol;
+  rc |= upload(c, s->frac_h1, N, &fh1, c->owned);
+  rc |= upload(c, s->frac_he1, N, &fhe1, c->owned);
+  rc |= upload(c, s->frac_hmol, N, &fhmol, c->owned);
+  if (rc) {
+    fsg_finalize(c);
+    return 1;
+  }
+  void *xln = nullptr;
+  if (alloc_zero(c, sizeof(double) * 6 * N, &xln)) {
+    fsg_finalize(c);
+    return 1;
+  }
+  launch_xline_init(N, S.temp, S.xne, fh1, fhe1, fhmol, s->h2brd, (double *)xln, c->st);
+  S.xln = (const double *)xln;
+  S.vturb2 = 1e26 / kXlnC / kXlnC * (s->vmicro * s->vmicro);
+  S.xlnz = std::sqrt(kPi) * (kCe * kCe) / (kMe * (kVc * kVc)) * 1e-18;
+  // Voigt table nodes (src/voigt.f90:48-62) and the table itself on the device
+  {
+    double dv = 8.0 / (double)(NVV - 1), dt = std::sqrt(8.0) / (double)(NVA - 1);
+    std::vector<double> anode(NVA + 1), aida(NVA + 1);
+    for (int ia = 0; ia <= NVA; ia++) anode[ia] = ((double)ia * dt) * ((double)ia * dt);
+    for (int ia = 0; ia < NVA; ia++) aida[ia] = 1.0 / (anode[ia + 1] - anode[ia]);
+    aida[NVA] = 0.0;
+    S.idv = 1.0 / dv;
+    S.idt = 1.0 / dt;
+    S.a1 = anode[1];
+    S.ia1 = aida[0];
+    if (upload(c, anode.data(), anode.size(), &S.anode, c->owned) ||
+        upload(c, aida.data(), aida.size(), &S.aida, c->owned)) {
+      fsg_finalize(c);
+      return 1;
+    }
+    void *tab = nullptr;
+    if (alloc_zero(c, sizeof(float) * (NVA + 1) * TAB_LD, &tab)) {
+      fsg_finalize(c);
+      return 1;
+    }
+    launch_voigt_table((float *)tab, S.anode, dv, c->st);
+    S.vtab = (const float *)tab;
+    CK(hipStreamSynchronize(c->st));
+  }
+  if (alloc_zero(c, lop_elem(c) * (size_t)N * Nw, &c->lineop) ||
+      alloc_zero(c, sizeof(int) * 64, (void **)&c->bincnt) ||
+      alloc_zero(c, sizeof(int) * 64, (void **)&c->binhmax) ||
+      alloc_zero(c, sizeof(int) * 3 * MAX_BINS, (void **)&c->binoff) ||
+      alloc_zero(c, sizeof(unsigned long long) * 4, (void **)&c->u64)) {
+    fsg_finalize(c);
+    return 1;
+  }
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(c->st));
+  *ctx_out = c;
+  return 0;
+}
+
+int fsg_finalize(fsg_ctx *c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->dev);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (void *p : c->owned) (void)hipFree(p);
+  for (DBuf *b : {&c->flag, &c->il, &c->ir, &c->ic, &c->key, &c->key2, &c->idx, &c->idx2, &c->coef, &c->wlc,
+                  &c->meta, &c->rng, &c->tmp, &c->pos, &c->wlcl, &c->rflag})
+    b->release();
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return 0;
+}
+
+int fsg_lines_upload(fsg_ctx *c, int32_t list_type, const fsg_lines *h, fsg_dlines **out) {
+  g_err.clear();
+  if (!c || !h || !out) return fail("fsg_lines_upload: NULL argument");
+  *out = nullptr;
+  if (list_type < 0 || list_type > 2) return fail("fsg_lines_upload: unknown line-list type");
+  if (h->n < 0 || h->n > 2147483647LL) return fail("fsg_lines_upload: line count out of range");
+  if (h->n > 0 && (!h->wl || !h->elo || !h->gf || !h->grad || !h->spc))
+    return fail("fsg_lines_upload: required line column is NULL");
+  if (h->n > 0 && list_type != FSG_LIST_MOLECULAR && (!h->gqst || !h->gvdw))
+    return fail("fsg_lines_upload: atomic and mixed lists need Gqst and Gvdw");
+  for (int64_t i = 1; i < h->n; i++)
+    if (h->wl[i] < h->wl[i - 1]) return fail("ReadHDF: line list is not sorted by wavelength");
+  for (int64_t i = 0; i < h->n; i++)
+    if (h->spc[i] < 1 || h->spc[i] > c->S.nspc) return fail("LineopCalc: species index out of range");
+  CK(hipSetDevice(c->dev));
+  fsg_dlines *d = new fsg_dlines();
+  d->ctx = c;
+  d->list_type = list_type;
+  d->L.n = h->n;
+  size_t n = (size_t)h->n;
+  int rc = 0;
+  rc |= upload(c, h->wl, n, &d->L.wl, d->owned);
+  rc |= upload(c, h->elo, n, &d->L.elo, d->owned);
+  rc |= upload(c, h->gf, n, &d->L.gf, d->owned);
+  rc |= upload(c, h->grad, n, &d->L.grad, d->owned);
+  rc |= upload(c, h->spc, n, (const int32_t **)&d->L.spc, d->owned);
+  if (list_type != FSG_LIST_MOLECULAR) {
+    rc |= upload(c, h->gqst, n, &d->L.gqst, d->owned);
+    rc |= upload(c, h->gvdw, n, &d->L.gvdw, d->owned);
+  }
+  if (list_type == FSG_LIST_MIXED && h->is_mol) rc |= upload(c, h->is_mol, n, &d->L.is_mol, d->owned);
+  if (rc || hipStreamSynchronize(c->st) != hipSuccess) {
+    fsg_lines_free(d);
+    return 1;
+  }
+  *out = d;
+  return 0;
+}
+
+int fsg_lines_free(fsg_dlines *d) {
+  if (!d) return 0;
+  (void)hipSetDevice(d->ctx->dev);
+  (void)hipStreamSynchronize(d->ctx->st);
+  for (void *p : d->owned) (void)hipFree(p);
+  delete d;
+  return 0;
+}
+
+// The line-opacity stage on a device-resident list.  All retained lines are processed in
+// one pass: the coefficient arrays live in HBM (Ntau x 32 B per retained line in FP32).
+// When nlines_lim is smaller than the list, MarkWideLines and PACK follow the reference's
+// multi-block rules; the accumulated opacity does not depend on the block partition.
+int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_stats *stats) {
+  g_err.clear();
+  if (!c || !dl) return fail("fsg_lineop_dev: NULL argument");
+  if (nlines_lim < 1) return fail("fsg_lineop_dev: NlinesLim must be positive");
+  CK(hipSetDevice(c->dev));
+  fsg_lineop_stats st{};
+  const int64_t n = dl->L.n;
+  st.nlines_in = n;
+  st.nsegments = 1;
+  if (n == 0) {
+    if (stats) *stats = st;
+    return 0;
+  }
+  const DevSetup &S = c->S;
+  const int N = S.ntau;
+  hipStream_t s = c->st;
+  CK(c->flag.ensure(n * sizeof(int)));
+  CK(c->il.ensure(n * sizeof(int)));
+  CK(c->ir.ensure(n * sizeof(int)));
+  CK(c->ic.ensure(n * sizeof(int)));
+  CK(c->key.ensure(n * sizeof(unsigned)));
+  CK(c->key2.ensure(n * sizeof(unsigned)));
+  CK(c->idx.ensure(n * sizeof(int)));
+  CK(c->idx2.ensure(n * sizeof(int)));
+  CK(hipMemsetAsync(c->bincnt, 0, sizeof(int) * 64, s));
+  CK(hipMemsetAsync(c->binhmax, 0, sizeof(int) * 64, s));
+  CK(hipMemsetAsync(c->u64, 0, sizeof(unsigned long long) * 4, s));
+  CK(hipEventRecord(c->ev[0], s));
+  launch_pass1(S, dl->L, dl->list_type, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), c->ic.as<int>(),
+               c->key.as<unsigned>(), c->bincnt, c->binhmax, c->batched, s);
+  launch_count_evals(n, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), N, c->u64, s);
+  launch_iota(c->idx.as<int>(), n, s);
+  // stable bin-major order of the lines (retained lines keep their index order per bin)
+  size_t tbytes = 0;
+  CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, c->key.as<unsigned>(), c->key2.as<unsigned>(),
+                                        c->idx.as<int>(), c->idx2.as<int>(), (int)n, 0, 6, s));
+  CK(c->tmp.ensure(tbytes));
+  CK(hipcub::DeviceRadixSort::SortPairs(c->tmp.p, tbytes, c->key.as<unsigned>(), c->key2.as<unsigned>(),
+                                        c->idx.as<int>(), c->idx2.as<int>(), (int)n, 0, 6, s));
+  int hcnt[64], hhmax[64];
+  unsigned long long hu[4];
+  std::vector<int> packed;  // bin tables staged for upload; alive until the final sync
+  CK(hipMemcpyAsync(hcnt, c->bincnt, sizeof(hcnt), hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(hhmax, c->binhmax, sizeof(hhmax), hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
+  CK(hipEventRecord(c->ev[1], s));
+  CK(hipStreamSynchronize(s));
+  int64_t nret = 0;
+  int nbins = 0;
+  for (int b = 0; b < WEAK_KEY; b++) {
+    nret += hcnt[b];
+    if (hcnt[b] > 0) nbins = b + 1;
+  }
+  if (nbins > MAX_BINS) return fail("fsg_lineop_dev: too many window-width bins");
+  st.nlines_use = nret;
+  st.evals = (int64_t)hu[0];
+  // MarkWideLines over the packed (single block, atomic/mixed) or full list
+  const int64_t nstep = std::min<int64_t>(n, nlines_lim);
+  const int pack = (nstep == n) && (dl->list_type == FSG_LIST_ATOMIC || dl->list_type == FSG_LIST_MIXED);
+  CK(c->wlcl.ensure(n * sizeof(double)));
+  CK(c->rflag.ensure(n * sizeof(int)));
+  CK(c->pos.ensure(n * sizeof(int)));
+  if (pack) {
+    launch_retained_flag(n, c->flag.as<int>(), c->rflag.as<int>(), s);
+    size_t sb = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, c->rflag.as<int>(), c->pos.as<int>(), (int)n, s));
+    CK(c->tmp.ensure(sb));
+    CK(hipcub::DeviceScan::ExclusiveSum(c->tmp.p, sb, c->rflag.as<int>(), c->pos.as<int>(), (int)n, s));
+  }
+  launch_wlc_list(n, dl->L.wl, S.wfirst, c->flag.as<int>(), c->pos.as<int>(), c->wlcl.as<double>(), pack, s);
+  launch_mark_wide(S, n, dl->L.wl, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), c->wlcl.as<double>(),
+                   pack ? nret : n, nstep, c->u64 + 1, s);
+  if (nret > 0) {
+    // bin offsets, counts and maximal half-widths of the candidate search
+    int hoff[MAX_BINS * 3];
+    int o = 0;
+    for (int b = 0; b < nbins; b++) {
+      hoff[b] = o;
+      hoff[MAX_BINS + b] = hcnt[b];
+      hoff[2 * MAX_BINS + b] = hhmax[b];
+      o += hcnt[b];
+    }
+    packed.assign(3 * nbins, 0);
+    for (int b = 0; b < nbins; b++) {
+      packed[b] = hoff[b];
+      packed[nbins + b] = hoff[MAX_BINS + b];
+      packed[2 * nbins + b] = hoff[2 * MAX_BINS + b];
+    }
+    CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * 3 * nbins, hipMemcpyHostToDevice, s));
+    const size_t csz = c->precision == FSG_FP32 ? sizeof(Coef32) : sizeof(Coef64);
+    CK(c->coef.ensure((size_t)nret * N * csz));
+    CK(c->wlc.ensure((size_t)nret * sizeof(double)));
+    CK(c->meta.ensure((size_t)nret * sizeof(int4)));
+    if (c->precision == FSG_FP32)
+      launch_fill32(S, dl->L, dl->list_type, nret, c->idx2.as<int>(), c->flag.as<int>(), c->il.as<int>(),
+                    c->ir.as<int>(), c->ic.as<int>(), c->coef.as<Coef32>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
+    else
+      launch_fill64(S, dl->L, dl->list_type, nret, c->idx2.as<int>(), c->flag.as<int>(), c->il.as<int>(),
+                    c->ir.as<int>(), c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
+    const int ntiles = (S.nwl + TILE - 1) / TILE;
+    CK(c->rng.ensure((size_t)ntiles * nbins * sizeof(int2)));
+    launch_tile_ranges(ntiles, nbins, S.nwl, c->binoff, c->binoff + nbins, c->binoff + 2 * nbins,
+                       c->meta.as<int4>(), c->rng.as<int2>(), s);
+    CK(hipEventRecord(c->ev[2], s));
+    if (c->precision == FSG_FP32)
+      launch_opacity32(S, ntiles, nbins, c->rng.as<int2>(), c->coef.as<Coef32>(), c->wlc.as<double>(),
+                       c->meta.as<int4>(), (float *)c->lineop, s);
+    else
+      launch_opacity64(S, ntiles, nbins, c->rng.as<int2>(), c->coef.as<Coef64>(), c->wlc.as<double>(),
+                       c->meta.as<int4>(), (double *)c->lineop, s);
+    CK(hipEventRecord(c->ev[3], s));
+  } else {
+    CK(hipEventRecord(c->ev[2], s));
+    CK(hipEventRecord(c->ev[3], s));
+  }
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  st.nwide = (int64_t)hu[1];
+  float ms1 = 0.f, ms2 = 0.f;
+  CK(hipEventElapsedTime(&ms1, c->ev[0], c->ev[1]));
+  CK(hipEventElapsedTime(&ms2, c->ev[2], c->ev[3]));
+  st.ms_pass1 = ms1;
+  st.ms_opacity = ms2;
+  c->have_rt = 0;
+  if (stats) *stats = st;
+  return 0;
+}
+
+int fsg_lineop(fsg_ctx *c, int32_t list_type, const fsg_lines *lines, int64_t nlines_lim, float *linstr_out,
+               int32_t *flags_out, int32_t *ileft_out, int32_t *iright_out, fsg_lineop_stats *stats) {
+  g_err.clear();
+  if (!c || !lines) return fail("fsg_lineop: NULL argument");
+  if (linstr_out) return fail("fsg_lineop: line strengths (SAVE flag 1) are not available in this build");
+  fsg_dlines *dl = nullptr;
+  if (fsg_lines_upload(c, list_type, lines, &dl)) return 1;
+  int rc = fsg_lineop_dev(c, dl, nlines_lim, stats);
+  if (rc == 0 && lines->n > 0) {
+    size_t b = (size_t)lines->n * sizeof(int32_t);
+    if (flags_out && hipMemcpy(flags_out, c->flag.p, b, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("D2H flags");
+    if (ileft_out && hipMemcpy(ileft_out, c->il.p, b, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("D2H ileft");
+    if (iright_out && hipMemcpy(iright_out, c->ir.p, b, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("D2H iright");
+  }
+  fsg_lines_free(dl);
+  return rc;
+}
+
+int fsg_maglineop(fsg_ctx *c, const fsg_lines *lines, int64_t nlines_lim, const double *bfield, int32_t bframe,
+                  float *linstr_out, int32_t *flags_out, int32_t *ileft_out, int32_t *iright_out,
+                  fsg_lineop_stats *stats) {
+  (void)c;
+  (void)lines;
+  (void)nlines_lim;
+  (void)bfield;
+  (void)bframe;
+  (void)linstr_out;
+  (void)flags_out;
+  (void)ileft_out;
+  (void)iright_out;
+  (void)stats;
+  return fail("fsg_maglineop: magnetic line opacity is not available in this build");
+}
+
+int fsg_reset_lineop(fsg_ctx *c) {
+  if (!c) return fail("fsg_reset_lineop: NULL context");
+  CK(hipSetDevice(c->dev));
+  CK(hipMemsetAsync(c->lineop, 0, lop_elem(c) * (size_t)c->S.ntau * c->S.nwl, c->st));
+  CK(hipStreamSynchronize(c->st));
+  c->have_rt = 0;
+  return 0;
+}
+
+int fsg_get_lineop(fsg_ctx *c, double *out) {
+  g_err.clear();
+  if (!c || !out) return fail("fsg_get_lineop: NULL argument");
+  CK(hipSetDevice(c->dev));
+  const int N = c->S.ntau, Nw = c->S.nwl;
+  size_t tot = (size_t)N * Nw;
+  if (c->precision == FSG_FP32) {
+    std::vector<float> h(tot);
+    CK(hipMemcpyAsync(h.data(), c->lineop, tot * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    CK(hipStreamSynchronize(c->st));
+    for (int k = 0; k < N; k++)
+      for (size_t iw = 0; iw < (size_t)Nw; iw++) out[iw * N + k] = (double)h[(size_t)k * Nw + iw];
+  } else {
+    std::vector<double> h(tot);
+    CK(hipMemcpyAsync(h.data(), c->lineop, tot * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    CK(hipStreamSynchronize(c->st));
+    for (int k = 0; k < N; k++)
+      for (size_t iw = 0; iw < (size_t)Nw; iw++) out[iw * N + k] = h[(size_t)k * Nw + iw];
+  }
+  return 0;
+}
+
+int fsg_set_lineop(fsg_ctx *c, const double *in) {
+  g_err.clear();
+  if (!c || !in) return fail("fsg_set_lineop: NULL argument");
+  CK(hipSetDevice(c->dev));
+  const int N = c->S.ntau, Nw = c->S.nwl;
+  size_t tot = (size_t)N * Nw;
+  if (c->precision == FSG_FP32) {
+    std::vector<float> h(tot);
+    for (int k = 0; k < N; k++)
+      for (size_t iw = 0; iw < (size_t)Nw; iw++) h[(size_t)k * Nw + iw] = (float)in[iw * N + k];
+    CK(hipMemcpyAsync(c->lineop, h.data(), tot * sizeof(float), hipMemcpyHostToDevice, c->st));
+    CK(hipStreamSynchronize(c->st));
+  } else {
+    std::vector<double> h(tot);
+    for (int k = 0; k < N; k++)
+      for (size_t iw = 0; iw < (size_t)Nw; iw++) h[(size_t)k * Nw + iw] = in[iw * N + k];
+    CK(hipMemcpyAsync(c->lineop, h.data(), tot * sizeof(double), hipMemcpyHostToDevice, c->st));
+    CK(hipStreamSynchronize(c->st));
+  }
+  c->have_rt = 0;
+  return 0;
+}
+
+int fsg_get_maglineop(fsg_ctx *c, double *out) {
+  (void)c;
+  (void)out;
+  return fail("fsg_get_maglineop: magnetic line opacity is not available in this build");
+}
+
+static int ensure_rt_buffers(fsg_ctx *c) {
+  if (c->spl) return 0;
+  const size_t Nw = (size_t)c->S.nwl, M = (size_t)c->S.nmu;
+  if (alloc_zero(c, sizeof(double) * M * Nw, (void **)&c->spl) ||
+      alloc_zero(c, sizeof(double) * M * Nw, (void **)&c->spc) ||
+      alloc_zero(c, sizeof(double) * Nw, (void **)&c->sfl) || alloc_zero(c, sizeof(double) * Nw, (void **)&c->sfc))
+    return 1;
+  return 0;
+}
+
+static int copy_spec_out(fsg_ctx *c, const double *dev, double *host) {
+  const size_t Nw = (size_t)c->S.nwl, M = (size_t)c->S.nmu;
+  std::vector<double> h(M * Nw);
+  CK(hipMemcpyAsync(h.data(), dev, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  CK(hipStreamSynchronize(c->st));
+  for (size_t m = 0; m < M; m++)
+    for (size_t iw = 0; iw < Nw; iw++) host[iw * M + m] = h[m * Nw + iw];
+  return 0;
+}
+
+int fsg_radtran(fsg_ctx *c, double *spl_out, double *spc_out) {
+  g_err.clear();
+  if (!c) return fail("fsg_radtran: NULL context");
+  CK(hipSetDevice(c->dev));
+  if (ensure_rt_buffers(c)) return 1;
+  if (c->precision == FSG_FP32)
+    launch_radtran32(c->S, (const float *)c->lineop, c->spl, c->spc, c->sfl, c->sfc, c->st);
+  else
+    launch_radtran64(c->S, (const double *)c->lineop, c->spl, c->spc, c->sfl, c->sfc, c->st);
+  CK(hipGetLastError());
+  c->have_rt = 1;
+  if (spl_out && copy_spec_out(c, c->spl, spl_out)) return 1;
+  if (spc_out && copy_spec_out(c, c->spc, spc_out)) return 1;
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int fsg_magradtran(fsg_ctx *c, const double *bfield, int32_t nbfield, int32_t bframe, double *spz_out,
+                   double *spc_out) {
+  (void)c;
+  (void)bfield;
+  (void)nbfield;
+  (void)bframe;
+  (void)spz_out;
+  (void)spc_out;
+  return fail("fsg_magradtran: polarised transfer is not available in this build");
+}
+
+int fsg_flux(fsg_ctx *c, double *sfl_out, double *sfc_out) {
+  g_err.clear();
+  if (!c) return fail("fsg_flux: NULL context");
+  if (!c->have_rt) return fail("fsg_flux: no formal solution computed since the last opacity change");
+  CK(hipSetDevice(c->dev));
+  const size_t Nw = (size_t)c->S.nwl;
+  if (sfl_out) CK(hipMemcpyAsync(sfl_out, c->sfl, Nw * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  if (sfc_out) CK(hipMemcpyAsync(sfc_out, c->sfc, Nw * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int fsg_device_ptr(fsg_ctx *c, int32_t which, void **ptr_out) {
+  if (!c || !ptr_out) return fail("fsg_device_ptr: NULL argument");
+  if (which != 0 && ensure_rt_buffers(c)) return 1;
+  switch (which) {
+    case 0: *ptr_out = c->lineop; break;
+    case 1: *ptr_out = c->spl; break;
+    case 2: *ptr_out = c->spc; break;
+    case 3: *ptr_out = c->sfl; break;
+    case 4: *ptr_out = c->sfc; break;
+    default: return fail("fsg_device_ptr: unknown array");
+  }
+  return 0;
+}
+
+int fsg_stream(fsg_ctx *c, void **stream_out) {
+  if (!c || !stream_out) return fail("fsg_stream: NULL argument");
+  *stream_out = (void *)c->st;
+  return 0;
+}
+
+}  // extern "C"

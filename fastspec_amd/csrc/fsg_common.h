@@ -1,0 +1,90 @@
+// fsg_common.h -- constants and device helpers shared by the HIP kernels of libfsgpu.
+//
+// Physical constants follow the reference (src/shared.f90:19-33) and the line-opacity
+// kernel constants src/opacity.f90:34-38.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fsg {
+
+constexpr double kVc = 299792.458;
+constexpr double kHc = 6.62607015e-27;
+constexpr double kKbolt = 1.380649e-16;
+constexpr double kA0 = 5.29177210903e-11;
+constexpr double kM0 = 1.66053906660e-27;
+constexpr double kCe = 4.803204712e-10;
+constexpr double kMe = 9.1093837139e-28;
+constexpr double kMh = 1.673532499e-24;
+constexpr double kHie = 13.598434005;
+constexpr double kKz = 4.668644778e-13;
+constexpr double kPi = 3.14159265358979323846;  // acos(-1d0)
+constexpr double kErg2ev = 1e-7 / 1.602176634e-19;
+constexpr double kEv2Ang = 8065.544005e-8;
+constexpr double kDWstart = 0.2, kDWstep = 2.0, kDwide = 5e3;  // src/shared.f90:430
+
+constexpr double kXlnC = kVc * 1e13;                 // speed of light [A/s]
+constexpr double kXlnC4Pi = 4.0 * kXlnC * kPi;
+constexpr double kXlnKM0 = kKbolt * 1e-7 / kM0;
+constexpr double kXlnDop = 2.0 * kKbolt / (kM0 * 1e3) * 1e16;
+constexpr double kSqrtPiInv = 0.56418958354775628695;
+constexpr double kPlkC1 = kHc * kVc * 1e13 / kKbolt;
+
+// Voigt table geometry (src/voigt.f90:19-26): H(a,v) on v in [0,8] step 8/1000 and
+// sqrt-spaced a nodes a_j = ((j-1) sqrt(8)/400)^2, j = 1..402
+constexpr int NVV = 1001, NVA = 401;
+constexpr int TAB_LD = NVV + 1;  // 1002 entries per a column
+
+// Depths per wavefront in the opacity kernel; a block holds ceil(ntau/DPW) wavefronts.
+constexpr int DPW = 18;
+constexpr int TILE = 64;        // wavelength points per tile (one per lane)
+constexpr int MAX_BINS = 40;
+constexpr int WEAK_KEY = 63;    // sort key of rejected lines
+
+// Per-(line, depth) coefficients of the batched FP32 path, one 32-byte record:
+//   vv = 1/Doppler width, a, a^2, a^2 + 1/4, K = LinCop a / sqrt(pi), LinCop,
+//   ua (a-interpolation weight), ia (a-cell of the table, 0 = outside the table box)
+struct alignas(32) Coef32 {
+  float vv, a, a2, c1, ka, cop, ua;
+  int ia;
+};
+// FP64 plain path: the reference's quantities LinVvgt, LinAvgt, LinCop and the a-state
+struct alignas(64) Coef64 {
+  double vv, a, cop, ua;
+  int ia, pad0;
+  double pad1, pad2, pad3;
+};
+
+// Everything the kernels read from the set-up (device pointers + scalars).
+struct DevSetup {
+  int ntau, nwl, nmu, wmode, motype, pf_ld, nslot, nspc, he1_slot, nhe, molbrd;
+  double wfirst, dstep, rstep, wstart;
+  double eps_opac, vwfudge, h2brd, vwmol, vmicro, he_wl_tol, he_elo_tol;
+  const double *wave;        // [nwl]
+  const double *contT;       // [ntau][nwl] ContOpac, wavelength fastest
+  const double *cstd;        // [ntau] ContOpacStd (ones if Motype 0)
+  const double *temp, *rhox, *xne, *rho;
+  const float *fpf;          // [ntau][pf_ld]
+  const float *pot_ion, *xamass;
+  const int *spc_ion, *spc_slot;
+  const double *he_wl, *he_elo, *he_gam, *he_shf;  // [nhe], [nhe][ntau]
+  const double *xln;         // [6][ntau] XlineInit depth constants
+  double vturb2;
+  double xlnz;               // sqrt(pi) e^2/(m c^2) 1e-18 (src/opacity.f90:36)
+  const double *xmu, *wmu;
+  // Voigt table and its node arrays
+  const float *vtab;         // [NVA+1][TAB_LD]
+  const double *anode, *aida;
+  double idv, idt, a1, ia1;
+};
+
+struct DevLines {
+  int64_t n;
+  const double *wl;
+  const float *elo, *gf, *grad, *gqst, *gvdw;
+  const int *spc;
+  const uint8_t *is_mol;
+};
+
+}  // namespace fsg

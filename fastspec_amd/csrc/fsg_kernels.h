@@ -1,0 +1,40 @@
+// fsg_kernels.h -- launchers of the HIP kernels (defined in fsg_kernels.hip).
+#pragma once
+
+#include <type_traits>
+
+#include "fsg_common.h"
+
+namespace fsg {
+
+void launch_voigt_table(float *tab, const double *anode, double dv, hipStream_t st);
+void launch_xline_init(int ntau, const double *temp, const double *xne, const float *fh1, const float *fhe1,
+                       const float *fhmol, double h2brd, double *xln, hipStream_t st);
+void launch_pass1(const DevSetup &S, const DevLines &L, int list_type, int *flag, int *il, int *ir, int *ic,
+                  unsigned *key, int *bin_count, int *bin_hmax, int batched, hipStream_t st);
+void launch_iota(int *v, int64_t n, hipStream_t st);
+void launch_count_evals(int64_t n, const int *flag, const int *il, const int *ir, int ntau,
+                        unsigned long long *acc, hipStream_t st);
+void launch_retained_flag(int64_t n, const int *flag, int *out, hipStream_t st);
+void launch_wlc_list(int64_t n, const double *wl, double wfirst, const int *flag, const int *pos, double *out,
+                     int pack, hipStream_t st);
+void launch_mark_wide(const DevSetup &S, int64_t n, const double *wl, const int *flag, const int *il, const int *ir,
+                      const double *wlc_list, int64_t m, int64_t nstep, unsigned long long *nwide, hipStream_t st);
+void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
+                   const int *flag, const int *il, const int *ir, const int *ic, Coef32 *coef, double *wlc_s,
+                   int4 *meta_s, hipStream_t st);
+void launch_fill64(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
+                   const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
+                   int4 *meta_s, hipStream_t st);
+void launch_tile_ranges(int ntiles, int nbins, int nwl, const int *bin_off, const int *bin_cnt, const int *bin_hmax,
+                        const int4 *meta_s, int2 *rng, hipStream_t st);
+void launch_opacity32(const DevSetup &S, int ntiles, int nbins, const int2 *rng, const Coef32 *coef,
+                      const double *wlc_s, const int4 *meta_s, float *lineop, hipStream_t st);
+void launch_opacity64(const DevSetup &S, int ntiles, int nbins, const int2 *rng, const Coef64 *coef,
+                      const double *wlc_s, const int4 *meta_s, double *lineop, hipStream_t st);
+void launch_radtran32(const DevSetup &S, const float *lop, double *spl, double *spc, double *sfl, double *sfc,
+                      hipStream_t st);
+void launch_radtran64(const DevSetup &S, const double *lop, double *spl, double *spc, double *sfl, double *sfc,
+                      hipStream_t st);
+
+}  // namespace fsg

@@ -1,0 +1,834 @@
+// fsg_kernels.hip -- HIP kernels for gfx950: Voigt tables, line-opacity pass 1, width
+// binning support, coefficient fill, the tiled opacity accumulation and the scalar
+// DELO-Bezier formal solution with the flux sum.
+//
+// Reference routines restated here (file:line in the reference tree):
+//   HVoigtExact/VoigtInit      src/voigt.f90:42-72, 158-187
+//   HVoigt                     src/voigt.f90:76-110
+//   XlineInit/Setup/Depth      src/opacity.f90:65-293
+//   LineOpacity1/2             src/opacity.f90:297-386
+//   WaveIdx, AnalyticWindow    src/opacity.f90:594-608, 678-710
+//   pass 1 of LineopCalc       src/synth.f90:833-907
+//   RadTranCalc, LocPrfPP      src/synth.f90:1825-1871, src/solver.f90:123-179
+//   TauStep, CentDer           src/solver.f90:591-647
+//   flux sums of DiskInteg     src/convol.f90:233-234
+#include "fsg_kernels.h"
+
+namespace fsg {
+
+// ------------------------------------------------------------------------------------
+// Voigt table (VoigtInit): FP64 Chiarella-Reichel sums, stored FP32
+// ------------------------------------------------------------------------------------
+__device__ double hvoigt_exact(double A, double V) {
+  const double HH = 0.5;
+  double av = fabs(V);
+  if (A <= 0.0) return exp(-fmin(av * av, 700.0));
+  // z = av + i A;  s = 1/z + sum 2 z e^{-(nh)^2} / (z^2 - (nh)^2)
+  double zr = av, zi = A;
+  double z2r = zr * zr - zi * zi, z2i = 2.0 * zr * zi;
+  double den = zr * zr + zi * zi;
+  double sr = zr / den, si = -zi / den;
+  for (int n = 1; n <= 16; n++) {
+    double nh = n * HH;
+    double e = 2.0 * exp(-nh * nh);
+    double nr = e * zr, ni = e * zi;              // 2 z e^{-nh^2}
+    double dr = z2r - nh * nh, di = z2i;          // z^2 - nh^2
+    double dd = dr * dr + di * di;
+    sr += (nr * dr + ni * di) / dd;
+    si += (ni * dr - nr * di) / dd;
+  }
+  // s = i (HH/pi) s
+  double c = HH / kPi;
+  double tr = -c * si, ti = c * sr;
+  if (A < 6.0) {
+    double p = 2.0 * kPi * A / HH;
+    double th = 2.0 * kPi * (2.0 * av - rint(2.0 * av));
+    double sh = sin(0.5 * th);
+    double redd = -2.0 * exp(0.5 * p) * sinh(0.5 * p) + exp(p) * 2.0 * sh * sh;
+    double imdd = exp(p) * sin(th);
+    // 2 exp(A^2 - av^2 - 2 i A av) / (redd + i imdd)
+    double m = 2.0 * exp(A * A - av * av);
+    double er = m * cos(2.0 * A * av), ei = -m * sin(2.0 * A * av);
+    double dd = redd * redd + imdd * imdd;
+    tr += (er * redd + ei * imdd) / dd;
+    ti += (ei * redd - er * imdd) / dd;
+  }
+  (void)ti;
+  return tr;
+}
+
+__global__ void k_voigt_table(float *tab, const double *anode, double dv) {
+  int iv = blockIdx.x * blockDim.x + threadIdx.x;  // 0..NVV
+  int ia = blockIdx.y;                             // 0..NVA
+  if (iv > NVV) return;
+  tab[(size_t)ia * TAB_LD + iv] = (float)hvoigt_exact(anode[ia], (double)iv * dv);
+}
+
+// ------------------------------------------------------------------------------------
+// XlineInit: per-depth constants [6][ntau] (src/opacity.f90:65-93)
+// ------------------------------------------------------------------------------------
+__global__ void k_xline_init(int ntau, const double *temp, const double *xne, const float *fh1,
+                             const float *fhe1, const float *fhmol, double h2brd, double *xln) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ntau) return;
+  double T = temp[k], Xne = xne[k];
+  double Vh = sqrt(2.0 * kKbolt * T / kMh);
+  double np = (double)fh1[k] + 0.412265 * (double)fhe1[k];
+  if (h2brd > 0.0) np = np + h2brd * 0.874307 * (double)fhmol[k];
+  xln[0 * ntau + k] = np;                                // XlnNprtb
+  xln[1 * ntau + k] = 17.0 * pow(Vh, 0.6) * np;          // XlnGvwprt
+  xln[2 * ntau + k] = pow(T / 1e4, 1.0 / 6.0) * Xne;     // XlnTemp6
+  xln[3 * ntau + k] = pow(T / 1e4, 0.3) * np;            // XlnTemp3
+  xln[4 * ntau + k] = kKbolt * T;                        // XlnKbT
+  xln[5 * ntau + k] = kKbolt * kErg2ev * T;              // XlnKbeT
+}
+
+// ------------------------------------------------------------------------------------
+// per-line kernel coefficients
+// ------------------------------------------------------------------------------------
+struct XC {
+  double WaveL, Hnux, Elo, Gf, Grad, Xmass, Cqst, Qdiv, Cvdw, AboPre, AboVbc, AboExp;
+  int IxPF, MdQst, MdVdw, IxHe;
+};
+
+__device__ __forceinline__ double air2vac(double L) {  // src/utils.f90:1020-1032
+  double S = 1e4 / L, S2 = S * S;
+  double N = 1.0 + 8.336624212083e-5 + 2.408926869968e-2 / (1.301065924522e2 - S2) +
+             1.599740894897e-4 / (3.892568793293e1 - S2);
+  return (L > 1999.352) ? L * N : L;
+}
+
+// flag bits as LineopCalc pass 1 sets them before XlineSetup (src/synth.f90:839-844)
+__device__ __forceinline__ int initial_flag(const DevLines &L, int64_t i, int list_type) {
+  int f = 0;
+  if (list_type == 0) {
+    if (L.gvdw[i] > 10.0f) f |= 4;
+  } else if (list_type == 2) {
+    if (L.is_mol && L.is_mol[i]) f |= 2;
+    if (L.gvdw[i] > 10.0f) f |= 4;
+  }
+  return f;
+}
+
+// XlineSetup (src/opacity.f90:96-235)
+__device__ XC xline_setup(const DevSetup &S, const DevLines &L, int64_t i, double wlc, int flag,
+                          int list_type) {
+  XC xc;
+  double WaveL = wlc + S.wfirst;
+  double Elo = (double)L.elo[i];
+  xc.WaveL = WaveL;
+  xc.Elo = Elo;
+  xc.Gf = (double)L.gf[i];
+  double GradL = (double)L.grad[i];
+  int ispc = L.spc[i];
+  int IonL = S.spc_ion[ispc - 1];
+  xc.IxPF = S.spc_slot[ispc - 1];
+  double GqstL, GvdwL;
+  bool IsMol;
+  if (list_type == 1) {
+    GqstL = 0.0;
+    GvdwL = 0.0;
+    IsMol = true;
+  } else {
+    GqstL = (double)L.gqst[i];
+    GvdwL = (double)L.gvdw[i];
+    IsMol = (list_type == 2) ? ((flag >> 1) & 1) : false;
+  }
+  double xam = (double)S.xamass[xc.IxPF - 1];
+  xc.Hnux = kXlnC * kHc / WaveL;
+  xc.Xmass = kXlnDop / kXlnC / kXlnC / xam;
+  xc.Grad = (GradL > 0.0) ? GradL : 0.22233e16 / (WaveL * WaveL);
+  xc.Cqst = 0.0;
+  xc.Qdiv = 1.0;
+  xc.Cvdw = 0.0;
+  xc.AboPre = 0.0;
+  xc.AboVbc = 0.0;
+  xc.AboExp = 0.0;
+  xc.IxHe = 0;
+  if (IsMol && !S.molbrd) {
+    xc.MdQst = 9;
+    xc.MdVdw = 9;
+  } else {
+    double Wavevac = air2vac(WaveL);
+    double Eup = Elo + 1.0 / (Wavevac * kEv2Ang);
+    double pot = (double)S.pot_ion[xc.IxPF - 1];
+    double t1 = (double)(IonL * IonL) * kHie / (pot - Eup);
+    double t2 = (double)(IonL * IonL) * kHie / (pot - Elo);
+    double Enu4 = t1 * t1, Enl4 = t2 * t2;
+    if (xc.IxPF == S.he1_slot) {  // HeMatchIdx (src/hestark.f90:376-389)
+      for (int k = 1; k <= S.nhe; k++)
+        if (fabs(WaveL - S.he_wl[k - 1]) <= S.he_wl_tol && fabs(Elo - S.he_elo[k - 1]) <= S.he_elo_tol) {
+          xc.IxHe = k;
+          break;
+        }
+    }
+    if (xc.IxHe > 0) {
+      xc.MdQst = 3;
+    } else if (GqstL > 0.0) {
+      xc.MdQst = 0;
+      xc.Cqst = GqstL;
+    } else if (IsMol) {
+      xc.MdQst = 9;
+    } else if (IonL == 1) {
+      xc.MdQst = 1;
+      xc.Cqst = 2.26e-07 * Enu4;
+    } else {
+      xc.MdQst = 2;
+      xc.Cqst = 5.42e-07 * Enu4;
+      xc.Qdiv = (IonL + 1.0) * (IonL + 1.0);
+    }
+    if ((flag >> 2) & 1) {
+      xc.MdVdw = 2;
+      double Sigma = kA0 * kA0 * (double)(int)GvdwL;
+      double Alpha = GvdwL - (double)(int)GvdwL;
+      double X = 2.0 - Alpha * 0.5;
+      double Gx = X - 1.0;
+      double GammaF = 1.0 + (-0.5748646 + (0.9512363 + (-0.6998588 + (0.4245549 - 0.1010678 * Gx) * Gx) * Gx) * Gx) * Gx;
+      xc.AboPre = 2e6 * (pow(4.0 / kPi, Alpha * 0.5) * GammaF * 1e4 * Sigma);
+      xc.AboVbc = 1.0 / 1.00794 + 1.0 / xam;
+      xc.AboExp = 1.0 - Alpha;
+    } else if (GvdwL > 0.0) {
+      xc.MdVdw = 0;
+      xc.Cvdw = GvdwL;
+    } else if (IsMol) {
+      xc.MdVdw = 9;
+    } else {
+      xc.MdVdw = 1;
+      double Cw = 1.61e-33 * fabs(Enu4 - Enl4) / (double)(IonL * IonL);
+      xc.Cvdw = pow(Cw, 0.4);
+    }
+  }
+  if (IsMol && S.vwmol > 0.0) {
+    xc.MdVdw = 0;
+    xc.Cvdw = S.vwmol;
+  }
+  return xc;
+}
+
+// XlineDepth (src/opacity.f90:239-293), k 0-based
+__device__ __forceinline__ void xline_depth(const DevSetup &S, int k, const XC &xc, double &cop, double &vv,
+                                            double &av) {
+  const int N = S.ntau;
+  double T = S.temp[k];
+  double Xstim = 1.0 - exp(-xc.Hnux / S.xln[4 * N + k]);
+  double Efract = exp(-xc.Elo / S.xln[5 * N + k]);
+  double Dopl = sqrt(T * xc.Xmass + S.vturb2);
+  double Dldopl = xc.WaveL * Dopl;
+  double Dndopl = Dopl / xc.WaveL;
+  double Gqst, Gvdw;
+  switch (xc.MdQst) {
+    case 0: Gqst = xc.Cqst * S.xln[2 * N + k]; break;
+    case 1: Gqst = xc.Cqst * S.xne[k]; break;
+    case 2: Gqst = xc.Cqst * S.xne[k] / xc.Qdiv; break;
+    case 3: Gqst = S.he_gam[(size_t)(xc.IxHe - 1) * N + k]; break;
+    default: Gqst = 0.0;
+  }
+  switch (xc.MdVdw) {
+    case 0: Gvdw = xc.Cvdw * S.xln[3 * N + k]; break;
+    case 1: Gvdw = S.vwfudge * S.xln[1 * N + k] * xc.Cvdw; break;
+    case 2: {
+      double Vbar = sqrt(8.0 * kXlnKM0 * T / kPi * xc.AboVbc);
+      Gvdw = xc.AboPre * pow(Vbar / 1e4, xc.AboExp) * S.xln[0 * N + k];
+      break;
+    }
+    default: Gvdw = 0.0;
+  }
+  double Gamtot = xc.Grad + Gqst + Gvdw;
+  av = Gamtot / (Dndopl * kXlnC4Pi);
+  vv = 1.0 / Dldopl;
+  double c = S.xlnz * Efract * xc.Gf * Xstim / (S.rho[k] * Dndopl);
+  cop = c * (double)S.fpf[(size_t)k * S.pf_ld + (xc.IxPF - 1)];
+}
+
+// a-state of the table lookup (src/voigt.f90:86-93 and 113-123)
+__device__ __forceinline__ void voigt_astate(const DevSetup &S, double A, int &ia, double &ua) {
+  if (A < 8.0) {
+    if (A < S.a1) {
+      ia = 1;
+      ua = A * S.ia1;
+    } else {
+      ia = (int)(sqrt(A) * S.idt) + 1;
+      ua = (A - S.anode[ia - 1]) * S.aida[ia - 1];
+    }
+  } else {
+    ia = 0;
+    ua = 0.0;
+  }
+}
+
+// HVoigt in FP64 (src/voigt.f90:76-110) with a precomputed a-state
+__device__ __forceinline__ double hvoigt64(const DevSetup &S, double A, int ia, double ua, double V) {
+  double av = fabs(V);
+  if (ia > 0 && av < 8.0) {
+    double fv = av * S.idv;
+    int iv = (int)fv;
+    double tv = fv - (double)iv;
+    const float *t0 = S.vtab + (size_t)(ia - 1) * TAB_LD;
+    const float *t1 = t0 + TAB_LD;
+    double c0 = (double)t0[iv] + tv * ((double)t0[iv + 1] - (double)t0[iv]);
+    double c1 = (double)t1[iv] + tv * ((double)t1[iv + 1] - (double)t1[iv]);
+    return c0 + ua * (c1 - c0);
+  }
+  double a2 = A * A, v2 = av * av, s = v2 + a2;
+  if (s >= 196.0) return kSqrtPiInv * A * (s + 0.5) / (s * s - v2 + a2 + 0.25);
+  double p = v2 - a2 - 1.5, q = 2.0 * av * A;
+  double dr = av * p - A * q, di = av * q + A * p;
+  return kSqrtPiInv * ((p + 0.5) * di - q * dr) / (dr * dr + di * di);
+}
+
+__device__ __forceinline__ double hvoigt64_full(const DevSetup &S, double A, double V) {
+  int ia;
+  double ua;
+  voigt_astate(S, A, ia, ua);
+  return hvoigt64(S, A, ia, ua, V);
+}
+
+// WaveIdx (src/opacity.f90:594-608)
+__device__ __forceinline__ int wave_idx(const DevSetup &S, double W) {
+  if (S.wmode == 0) {
+    double X = fmin(fmax(W / S.dstep, 0.0), (double)(S.nwl - 2));
+    return (int)X + 1;
+  }
+  double X = 1.0 + log(fmax((W / S.wstart) + 1.0, 2.2250738585072014e-308)) / S.rstep;
+  return (int)fmin(fmax(X, 1.0), (double)(S.nwl - 1));
+}
+
+// ------------------------------------------------------------------------------------
+// pass 1 of LineopCalc (src/synth.f90:833-907): one lane per line.  The three depth
+// maxima (bound test, centre test, analytic window) are streamed in one depth loop; the
+// decisions are taken afterwards in the reference's order.
+// ------------------------------------------------------------------------------------
+__global__ void k_pass1(DevSetup S, DevLines L, int list_type, int *flag_out, int *il_out, int *ir_out,
+                        int *ic_out, unsigned *key_out, int *bin_count, int *bin_hmax, int batched) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n) return;
+  const int N = S.ntau;
+  int flag = initial_flag(L, i, list_type);
+  double wlc = L.wl[i] - S.wfirst;
+  XC xc = xline_setup(S, L, i, wlc, flag, list_type);
+  if (xc.IxHe > 0) flag |= (xc.IxHe << 8);
+  int ihe = xc.IxHe;
+  int iwl = wave_idx(S, wlc);
+  double w0 = S.wave[iwl - 1], w1 = S.wave[iwl];
+  double dW = (wlc - w0) / (w1 - w0);
+  const double iSqrtPi = 0.5641895835477563;
+  double maxb = -INFINITY, maxc = -INFINITY, win = 0.0;
+  for (int k = 0; k < N; k++) {
+    double cop, vv, av;
+    xline_depth(S, k, xc, cop, vv, av);
+    const double *ck = S.contT + (size_t)k * S.nwl;
+    double kc = ck[iwl - 1] * (1.0 - dW) + ck[iwl] * dW;
+    double cs = S.cstd[k];
+    double rb = (S.motype == 1) ? cop / (kc * cs) : cop / kc;
+    maxb = fmax(maxb, rb);
+    // LineOpacity2 at the line centre: dWl0 = 0, shifted by HeShf for He I lines
+    double dWl = 0.0;
+    if (ihe > 0) dWl = 0.0 - S.he_shf[(size_t)(ihe - 1) * N + k];
+    double lo = hvoigt64_full(S, av, dWl * vv) * cop;
+    if (S.motype == 1) lo = lo / cs;
+    maxc = fmax(maxc, lo / kc);
+    // AnalyticWindow (src/opacity.f90:678-710)
+    double r = cop / (S.eps_opac * kc);
+    if (S.motype == 1) r = r / cs;
+    if (r > 1.0) {
+      double w = sqrt(log(r) + fmax(0.0, av * (r * iSqrtPi - av))) / vv;
+      if (ihe > 0) w = w + fabs(S.he_shf[(size_t)(ihe - 1) * N + k]);
+      win = fmax(win, w);
+    }
+  }
+  int il = 1, ir = S.nwl;
+  bool weak = (maxb < S.eps_opac) || (maxc < S.eps_opac);
+  if (weak) {
+    flag |= 1;
+  } else {
+    il = wave_idx(S, wlc - win);
+    int r2 = wave_idx(S, wlc + win) + 1;
+    ir = r2 < S.nwl ? r2 : S.nwl;
+  }
+  flag_out[i] = flag;
+  il_out[i] = il;
+  ir_out[i] = ir;
+  ic_out[i] = iwl;
+  unsigned key;
+  if (weak) {
+    key = WEAK_KEY;
+  } else {
+    int h = max(iwl - il, ir - iwl);
+    key = batched ? (h == 0 ? 0u : (unsigned)(32 - __clz(h))) : 0u;
+    atomicMax(&bin_hmax[key], h);
+  }
+  atomicAdd(&bin_count[key], 1);
+  key_out[i] = key;
+}
+
+__global__ void k_iota(int *v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (int)i;
+}
+
+// Sum over retained lines of window pixels x Ntau: the metric's evaluation count
+__global__ void k_count_evals(int64_t n, const int *flag, const int *il, const int *ir, int ntau,
+                              unsigned long long *acc) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  if (i < n && !(flag[i] & 1)) v = (unsigned long long)(ir[i] - il[i] + 1) * (unsigned long long)ntau;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(acc, v);
+}
+
+// MarkWideLines criterion (src/synth.f90:709-744) over the (packed or full) list of
+// centre offsets wlc_list[0..m): WlcIdx = number of entries <= W.
+__device__ __forceinline__ int64_t upper_count(const double *a, int64_t m, double W) {
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= W) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_mark_wide(DevSetup S, int64_t n, const double *wl, const int *flag, const int *il,
+                            const int *ir, const double *wlc_list, int64_t m, int64_t nstep,
+                            unsigned long long *nwide) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int w = 0;
+  if (i < n && !(flag[i] & 1)) {
+    double wr = S.wave[ir[i] - 1], wL = S.wave[il[i] - 1];
+    int64_t nrec = upper_count(wlc_list, m, wr) - upper_count(wlc_list, m, wL) + 1;
+    double wlc = wl[i] - S.wfirst;
+    double dv = kVc * (wr - wL) / (wlc + S.wfirst);
+    if (nrec > nstep / 2 || dv > kDwide) w = 1;
+  }
+  unsigned long long v = w;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(nwide, v);
+}
+
+__global__ void k_wlc_list(int64_t n, const double *wl, double wfirst, const int *flag, const int *pos,
+                           double *out, int pack) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!pack) {
+    out[i] = wl[i] - wfirst;
+  } else if (!(flag[i] & 1)) {
+    out[pos[i]] = wl[i] - wfirst;
+  }
+}
+
+// exclusive positions of retained lines (inclusive scan result shifted)
+__global__ void k_retained_flag(int64_t n, const int *flag, int *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (flag[i] & 1) ? 0 : 1;
+}
+
+// ------------------------------------------------------------------------------------
+// coefficient fill: one lane per retained line in bin-sorted order
+// ------------------------------------------------------------------------------------
+template <typename CoefT>
+__global__ void k_fill(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order,
+                       const int *flag, const int *il, const int *ir, const int *ic, CoefT *coef,
+                       double *wlc_s, int4 *meta_s) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nret) return;
+  int64_t i = order[j];
+  const int N = S.ntau;
+  int f = flag[i];
+  double wlc = L.wl[i] - S.wfirst;
+  XC xc = xline_setup(S, L, i, wlc, f & 0x6, list_type);
+  wlc_s[j] = wlc;
+  meta_s[j] = make_int4(il[i], ir[i], ic[i], (f >> 8) & 0xff);
+  CoefT *c = coef + (size_t)j * N;
+  for (int k = 0; k < N; k++) {
+    double cop, vv, av, ua;
+    int ia;
+    xline_depth(S, k, xc, cop, vv, av);
+    voigt_astate(S, av, ia, ua);
+    CoefT r;
+    if constexpr (sizeof(CoefT) == sizeof(Coef32)) {
+      r.vv = (float)vv;
+      r.a = (float)av;
+      r.a2 = (float)(av * av);
+      r.c1 = (float)(av * av + 0.25);
+      r.ka = (float)(cop * av * kSqrtPiInv);
+      r.cop = (float)cop;
+      r.ua = (float)ua;
+      r.ia = ia;
+    } else {
+      r.vv = vv;
+      r.a = av;
+      r.cop = cop;
+      r.ua = ua;
+      r.ia = ia;
+      r.pad0 = 0;
+      r.pad1 = r.pad2 = r.pad3 = 0.0;
+    }
+    c[k] = r;
+  }
+}
+
+// candidate ranges: for tile t and bin b, lines of the bin whose centre pixel lies in
+// [p0 - hmax_b, p1 + hmax_b] (a contiguous run of the bin's index-ordered lines)
+__global__ void k_tile_ranges(int ntiles, int nbins, int nwl, const int *bin_off, const int *bin_cnt,
+                              const int *bin_hmax, const int4 *meta_s, int2 *rng) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = blockIdx.y;
+  if (t >= ntiles || b >= nbins) return;
+  int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, nwl);
+  int off = bin_off[b], cnt = bin_cnt[b], h = bin_hmax[b];
+  long lo = (long)p0 - h, hi = (long)p1 + h;
+  int a = 0, e = cnt;
+  while (a < e) {  // first with ic >= lo
+    int m = (a + e) >> 1;
+    if (meta_s[off + m].z < lo) a = m + 1;
+    else e = m;
+  }
+  int s = a;
+  e = cnt;
+  while (a < e) {  // first with ic > hi
+    int m = (a + e) >> 1;
+    if (meta_s[off + m].z <= hi) a = m + 1;
+    else e = m;
+  }
+  rng[(size_t)t * nbins + b] = make_int2(off + s, off + a);
+}
+
+// ------------------------------------------------------------------------------------
+// line opacity accumulation: one block per 64-point wavelength tile, one wavefront per
+// group of DPW depths, one lane per wavelength point.  Per-(line, depth) coefficients
+// are wave-uniform and come through scalar loads.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ float contrib32(const DevSetup &S, const Coef32 &c, float v) {
+  float av = fabsf(v);
+  if (c.ia > 0 && av < 8.0f) {
+    // table box: bilinear interpolation of the FP32 table (src/voigt.f90:81-95)
+    float fv = av * (float)S.idv;
+    int iv = (int)fv;
+    float tv = fv - (float)iv;
+    const float *t0 = S.vtab + (size_t)(c.ia - 1) * TAB_LD + iv;
+    const float *t1 = t0 + TAB_LD;
+    float a0 = t0[0], a1 = t0[1], b0 = t1[0], b1 = t1[1];
+    float c0 = fmaf(tv, a1 - a0, a0);
+    float c1 = fmaf(tv, b1 - b0, b0);
+    return fmaf(c.ua, c1 - c0, c0) * c.cop;
+  }
+  float v2 = av * av;
+  float s = v2 + c.a2;
+  if (s >= 196.0f) {
+    // one-pole continued fraction: a (s + 1/2) / (s^2 - v^2 + a^2 + 1/4) / sqrt(pi)
+    float den = fmaf(s, s, c.c1 - v2);
+    return c.ka * (s + 0.5f) * __builtin_amdgcn_rcpf(den);
+  }
+  // two-pole continued fraction, numerator and denominator rearranged without
+  // cancellation: a (p (p + 1/2) + q^2 + v^2) / ((v^2 + a^2)(p^2 + q^2)) / sqrt(pi)
+  float p = v2 - c.a2 - 1.5f;
+  float q = 2.0f * av * c.a;
+  float num = fmaf(p, p + 0.5f, fmaf(q, q, v2));
+  float den = s * fmaf(p, p, q * q);
+  return c.ka * num * __builtin_amdgcn_rcpf(den);
+}
+
+__device__ __forceinline__ double contrib64(const DevSetup &S, const Coef64 &c, double v) {
+  return hvoigt64(S, c.a, c.ia, c.ua, v) * c.cop;
+}
+
+// XCD-aware tile order: consecutive block ids go round-robin over the 8 XCDs, so map
+// them so that each XCD works on a contiguous run of tiles (shared lines stay in its L2).
+// The grid holds 8 * per blocks (per = ceil(ntiles / 8)), so bid -> x * per + r is a
+// bijection onto [0, 8 * per) and every tile below ntiles is visited exactly once.
+__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
+  const int NX = 8;
+  int per = nblk / NX;
+  int x = bid % NX, r = bid / NX;
+  return x * per + r;
+}
+
+template <typename CoefT, bool KAHAN>
+__global__ void __launch_bounds__(512) k_opacity(DevSetup S, int ntiles, int nbins, const int2 *rng,
+                                                  const CoefT *__restrict__ coef,
+                                                  const double *__restrict__ wlc_s,
+                                                  const int4 *__restrict__ meta_s, void *lineop_v) {
+  using Real = typename std::conditional<sizeof(CoefT) == sizeof(Coef32), float, double>::type;
+  int t = xcd_tile(blockIdx.x, gridDim.x);
+  // blocks beyond the tile count (from rounding the XCD split) have nothing to do
+  if (t < 0 || t >= ntiles) return;
+  const int N = S.ntau;
+  int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
+  int p = p0 + lane;
+  bool valid = p <= S.nwl;
+  double Wp = valid ? S.wave[p - 1] : 0.0;
+  int d0 = wave * DPW;
+  int nd = min(DPW, N - d0);
+  Real acc[DPW], cmp[DPW];
+#pragma unroll
+  for (int k = 0; k < DPW; k++) {
+    acc[k] = 0;
+    cmp[k] = 0;
+  }
+  for (int b = 0; b < nbins; b++) {
+    int2 r = rng[(size_t)t * nbins + b];
+    for (int j = r.x; j < r.y; j++) {
+      int4 m = meta_s[j];
+      if (m.y < p0 || m.x > p1) continue;  // window misses the tile (wave-uniform)
+      bool act = valid && p >= m.x && p <= m.y;
+      if (!act) continue;
+      double dl = Wp - wlc_s[j];
+      const CoefT *c = coef + (size_t)j * N + d0;
+      const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * N + d0 : nullptr;
+#pragma unroll
+      for (int k = 0; k < DPW; k++) {
+        if (k < nd) {
+          double dld = hs ? dl - hs[k] : dl;
+          Real val;
+          if constexpr (sizeof(CoefT) == sizeof(Coef32)) {
+            val = contrib32(S, c[k], (float)dld * c[k].vv);
+          } else {
+            val = contrib64(S, c[k], dld * c[k].vv);
+          }
+          if (KAHAN) {
+            Real y = val - cmp[k];
+            Real tt = acc[k] + y;
+            cmp[k] = (tt - acc[k]) - y;
+            acc[k] = tt;
+          } else {
+            acc[k] += val;
+          }
+        }
+      }
+    }
+  }
+  if (!valid) return;
+  Real *lop = (Real *)lineop_v;
+#pragma unroll
+  for (int k = 0; k < DPW; k++) {
+    if (k < nd) {
+      int d = d0 + k;
+      Real v = KAHAN ? (acc[k] - cmp[k]) : acc[k];
+      if (S.motype == 1) v = v / (Real)S.cstd[d];
+      size_t o = (size_t)d * S.nwl + (p - 1);
+      lop[o] = lop[o] + v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// scalar plane-parallel DELO-Bezier (LocPrfPP) for the continuum and the line+continuum
+// opacity of one wavelength per lane, streamed from the bottom of the atmosphere up with
+// a sliding window of depth points; FP64 throughout.  Fused flux sums.
+// ------------------------------------------------------------------------------------
+constexpr int MAXMU = 16;
+
+template <typename LopT>
+struct RTCol {
+  const double *cont;
+  const LopT *lop;
+  int nwl, p;
+  __device__ __forceinline__ double k(int d, bool with_line) const {
+    double v = cont[(size_t)d * nwl + p];
+    if (with_line) v = v + (double)lop[(size_t)d * nwl + p];
+    return v;
+  }
+};
+
+// TauStep (src/solver.f90:591-615) element i (0-based, i <= N-2), Xmu = 1
+template <typename Col>
+__device__ __forceinline__ double tau_step_i(const double *rhox, const Col &col, bool wl, int i) {
+  if (i == 0) {
+    double Dx1 = (rhox[1] - rhox[0]) / 1.0;
+    double x0 = col.k(0, wl), x1 = col.k(1, wl);
+    double Yp = (x1 - x0) / Dx1;
+    return Dx1 * (x1 + 2.0 * x0 + Dx1 / 2.0 * Yp) / 3.0;
+  }
+  double Dx = (rhox[i] - rhox[i - 1]) / 1.0;
+  double Dx1 = (rhox[i + 1] - rhox[i]) / 1.0;
+  double xm = col.k(i - 1, wl), x0 = col.k(i, wl), xp = col.k(i + 1, wl);
+  double Der = (x0 - xm) / Dx, Der1 = (xp - x0) / Dx1, Yp;
+  if (Der * Der1 > 0.0) {
+    double Lambda = (1.0 + Dx1 / (Dx + Dx1)) / 3.0;
+    Yp = (Der / (Lambda * Der1 + (1.0 - Lambda) * Der)) * Der1;
+  } else {
+    Yp = 0.0;
+  }
+  return Dx1 * (xp + 2.0 * x0 + Dx1 / 2.0 * Yp) / 3.0;
+}
+
+// CentDer (src/solver.f90:618-647) interior element from the two adjacent steps
+__device__ __forceinline__ double cent_der_i(double Dx, double Dx1, double ym, double y0, double yp) {
+  double InvDx = 1.0 / Dx, InvDx1 = 1.0 / Dx1;
+  double Lambda = (1.0 + Dx1 / (Dx + Dx1)) / 3.0;
+  double Lambda1 = 1.0 - Lambda;
+  double Der = (y0 - ym) * InvDx, Der1 = (yp - y0) * InvDx1;
+  return (Der * Der1 > 0.0) ? (Der / (Lambda * Der1 + Lambda1 * Der)) * Der1 : 0.0;
+}
+
+// DELO coefficients (src/solver.f90:154-169)
+__device__ __forceinline__ void delo_coef(double Dtau, double &Eps, double &Alpha, double &Beta, double &Gamma) {
+  double Dtau2 = Dtau * Dtau;
+  double Idtau2 = 1.0 / Dtau2;
+  if (Dtau < 5e-3) {
+    Eps = 1.0 - Dtau + 0.5 * Dtau2 - Dtau2 * Dtau / 6.0 + Dtau2 * Dtau2 / 24.0;
+    Alpha = Dtau / 3.0 - Dtau2 / 12.0 + Dtau2 * Dtau / 60.0;
+    Beta = Dtau / 3.0 - Dtau2 * 0.25 + Dtau2 * Dtau * 0.1;
+    Gamma = Dtau / 3.0 - Dtau2 / 6.0 + Dtau2 * Dtau * 0.05;
+  } else {
+    double E = (Dtau < 100.0) ? exp(-Dtau) : 0.0;
+    Eps = E;
+    Alpha = (Dtau2 - 2.0 * Dtau + 2.0 - 2.0 * E) * Idtau2;
+    Beta = (2.0 - (2.0 + 2.0 * Dtau + Dtau2) * E) * Idtau2;
+    Gamma = (2.0 * Dtau - 4.0 + (2.0 * Dtau + 4.0) * E) * Idtau2;
+  }
+}
+
+__device__ __forceinline__ double planck_src(double C1, double C2, double T) { return C2 / (exp(C1 / T) - 1.0); }
+
+template <typename LopT>
+__global__ void __launch_bounds__(256) k_radtran(DevSetup S, const LopT *lop, double *spl, double *spc,
+                                                 double *sfl, double *sfc) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= S.nwl) return;
+  const int N = S.ntau, M = S.nmu;
+  RTCol<LopT> col{S.contT, lop, S.nwl, p};
+  double Wl = S.wave[p] + S.wfirst;
+  double C1 = kPlkC1 / Wl;
+  double C2 = exp(50.7649141 - 5.0 * log(Wl));
+  double imu[MAXMU];
+  for (int m = 0; m < M; m++) imu[m] = 1.0 / S.xmu[m];
+  for (int pass = 0; pass < 2; pass++) {  // 0: continuum (SpC), 1: line + continuum (SpL)
+    bool wl = pass == 1;
+    double Y[MAXMU];
+    // sliding window, 0-based depth i: steps dt_i = dtau[i], dt_im1 = dtau[i-1]
+    int i = N - 2;
+    double S_ip1 = planck_src(C1, C2, S.temp[N - 1]);
+    double S_i = planck_src(C1, C2, S.temp[N - 2]);
+    double S_im1 = (N >= 3) ? planck_src(C1, C2, S.temp[N - 3]) : 0.0;
+    double dt_i = tau_step_i(S.rhox, col, wl, i);
+    double dt_im1 = (i >= 1) ? tau_step_i(S.rhox, col, wl, i - 1) : 0.0;
+    double Sd_ip1 = (S_ip1 - S_i) / dt_i;  // Srd(N)
+    for (int m = 0; m < M; m++) Y[m] = S_ip1;
+    for (; i >= 0; i--) {
+      double Sd_i = (i == 0) ? (S_ip1 - S_i) / dt_i : cent_der_i(dt_im1, dt_i, S_im1, S_i, S_ip1);
+      double U = 0.5 * (S_i + 0.5 * dt_i * Sd_i + S_ip1 - 0.5 * dt_i * Sd_ip1);
+      for (int m = 0; m < M; m++) {
+        double Eps, Alpha, Beta, Gamma;
+        delo_coef(dt_i * imu[m], Eps, Alpha, Beta, Gamma);
+        Y[m] = Y[m] * Eps + Alpha * S_i + Beta * S_ip1 + Gamma * U;
+      }
+      // shift the window up by one depth point
+      Sd_ip1 = Sd_i;
+      S_ip1 = S_i;
+      S_i = S_im1;
+      dt_i = dt_im1;
+      if (i >= 2) {
+        S_im1 = (i >= 2) ? planck_src(C1, C2, S.temp[i - 2]) : 0.0;
+        dt_im1 = tau_step_i(S.rhox, col, wl, i - 2);
+      }
+    }
+    double *out = wl ? spl : spc;
+    double f = 0.0;
+    for (int m = 0; m < M; m++) {
+      out[(size_t)m * S.nwl + p] = Y[m];
+      f = f + Y[m] * S.wmu[m];
+    }
+    (wl ? sfl : sfc)[p] = f;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// host-side launchers (the only entry points the API translation unit uses)
+// ------------------------------------------------------------------------------------
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_voigt_table(float *tab, const double *anode, double dv, hipStream_t st) {
+  dim3 grid(nblk(NVV + 1, 256), NVA + 1);
+  k_voigt_table<<<grid, 256, 0, st>>>(tab, anode, dv);
+}
+
+void launch_xline_init(int ntau, const double *temp, const double *xne, const float *fh1, const float *fhe1,
+                       const float *fhmol, double h2brd, double *xln, hipStream_t st) {
+  k_xline_init<<<nblk(ntau, 64), 64, 0, st>>>(ntau, temp, xne, fh1, fhe1, fhmol, h2brd, xln);
+}
+
+void launch_pass1(const DevSetup &S, const DevLines &L, int list_type, int *flag, int *il, int *ir, int *ic,
+                  unsigned *key, int *bin_count, int *bin_hmax, int batched, hipStream_t st) {
+  if (L.n == 0) return;
+  k_pass1<<<nblk(L.n, 128), 128, 0, st>>>(S, L, list_type, flag, il, ir, ic, key, bin_count, bin_hmax, batched);
+}
+
+void launch_iota(int *v, int64_t n, hipStream_t st) {
+  if (n) k_iota<<<nblk(n, 256), 256, 0, st>>>(v, n);
+}
+
+void launch_count_evals(int64_t n, const int *flag, const int *il, const int *ir, int ntau,
+                        unsigned long long *acc, hipStream_t st) {
+  if (n) k_count_evals<<<nblk(n, 256), 256, 0, st>>>(n, flag, il, ir, ntau, acc);
+}
+
+void launch_retained_flag(int64_t n, const int *flag, int *out, hipStream_t st) {
+  if (n) k_retained_flag<<<nblk(n, 256), 256, 0, st>>>(n, flag, out);
+}
+
+void launch_wlc_list(int64_t n, const double *wl, double wfirst, const int *flag, const int *pos, double *out,
+                     int pack, hipStream_t st) {
+  if (n) k_wlc_list<<<nblk(n, 256), 256, 0, st>>>(n, wl, wfirst, flag, pos, out, pack);
+}
+
+void launch_mark_wide(const DevSetup &S, int64_t n, const double *wl, const int *flag, const int *il, const int *ir,
+                      const double *wlc_list, int64_t m, int64_t nstep, unsigned long long *nwide, hipStream_t st) {
+  if (n) k_mark_wide<<<nblk(n, 256), 256, 0, st>>>(S, n, wl, flag, il, ir, wlc_list, m, nstep, nwide);
+}
+
+void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
+                   const int *flag, const int *il, const int *ir, const int *ic, Coef32 *coef, double *wlc_s,
+                   int4 *meta_s, hipStream_t st) {
+  if (nret) k_fill<Coef32><<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, wlc_s, meta_s);
+}
+
+void launch_fill64(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
+                   const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
+                   int4 *meta_s, hipStream_t st) {
+  if (nret) k_fill<Coef64><<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, wlc_s, meta_s);
+}
+
+void launch_tile_ranges(int ntiles, int nbins, int nwl, const int *bin_off, const int *bin_cnt, const int *bin_hmax,
+                        const int4 *meta_s, int2 *rng, hipStream_t st) {
+  dim3 grid(nblk(ntiles, 256), nbins);
+  k_tile_ranges<<<grid, 256, 0, st>>>(ntiles, nbins, nwl, bin_off, bin_cnt, bin_hmax, meta_s, rng);
+}
+
+void launch_opacity32(const DevSetup &S, int ntiles, int nbins, const int2 *rng, const Coef32 *coef,
+                      const double *wlc_s, const int4 *meta_s, float *lineop, hipStream_t st) {
+  int nw = (S.ntau + DPW - 1) / DPW;
+  int nb = 8 * ((ntiles + 7) / 8);
+  k_opacity<Coef32, true><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rng, coef, wlc_s, meta_s, lineop);
+}
+
+void launch_opacity64(const DevSetup &S, int ntiles, int nbins, const int2 *rng, const Coef64 *coef,
+                      const double *wlc_s, const int4 *meta_s, double *lineop, hipStream_t st) {
+  int nw = (S.ntau + DPW - 1) / DPW;
+  int nb = 8 * ((ntiles + 7) / 8);
+  k_opacity<Coef64, false><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rng, coef, wlc_s, meta_s, lineop);
+}
+
+void launch_radtran32(const DevSetup &S, const float *lop, double *spl, double *spc, double *sfl, double *sfc,
+                      hipStream_t st) {
+  k_radtran<float><<<nblk(S.nwl, 256), 256, 0, st>>>(S, lop, spl, spc, sfl, sfc);
+}
+
+void launch_radtran64(const DevSetup &S, const double *lop, double *spl, double *spc, double *sfl, double *sfc,
+                      hipStream_t st) {
+  k_radtran<double><<<nblk(S.nwl, 256), 256, 0, st>>>(S, lop, spl, spc, sfl, sfc);
+}
+
+template __global__ void k_fill<Coef32>(DevSetup, DevLines, int, int64_t, const int *, const int *, const int *,
+                                        const int *, const int *, Coef32 *, double *, int4 *);
+template __global__ void k_fill<Coef64>(DevSetup, DevLines, int, int64_t, const int *, const int *, const int *,
+                                        const int *, const int *, Coef64 *, double *, int4 *);
+template __global__ void k_opacity<Coef32, true>(DevSetup, int, int, const int2 *, const Coef32 *, const double *,
+                                                 const int4 *, void *);
+template __global__ void k_opacity<Coef64, false>(DevSetup, int, int, const int2 *, const Coef64 *,
+                                                  const double *, const int4 *, void *);
+template __global__ void k_radtran<float>(DevSetup, const float *, double *, double *, double *, double *);
+template __global__ void k_radtran<double>(DevSetup, const double *, double *, double *, double *, double *);
+
+}  // namespace fsg

@@ -1,0 +1,151 @@
+"""Host-side mirror of FastSpec's hot-path drivers over libfsgpu.
+
+The method names follow the reference's module procedures so that a caller of the
+reference finds the same stage sequence (src/fastspec.f90:241-461):
+
+    LineopCalc(lines, NlinesLim)        src/synth.f90:747   (non-magnetic line opacity)
+    RadTranCalc()                       src/synth.f90:1825  (scalar DELO-Bezier, SpL/SpC)
+    IntegConvol(2) / DiskInteg flux     src/convol.f90:27   (flux sums, no convolution)
+
+Errors raised by the library carry the reference's ErrStop messages where it has one.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .abi import FSG_FP32, FSG_FP64, CStats, LineList, Setup, ptr
+
+
+class DeviceLines:
+    """A line list resident in HBM (fsg_lines_upload)."""
+
+    def __init__(self, synth: "FastSpecGPU", lines: LineList):
+        self._lib = _lib.load()
+        self.synth = synth
+        self.n = lines.n
+        self.list_type = lines.list_type
+        cl = lines.cstruct()
+        self.handle = C.c_void_p()
+        _lib.check(self._lib.fsg_lines_upload(synth.ctx, lines.list_type, C.byref(cl), C.byref(self.handle)))
+
+    def free(self):
+        if self.handle:
+            self._lib.fsg_lines_free(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class FastSpecGPU:
+    """One synthesis context on one GPU (fsg_init ... fsg_finalize)."""
+
+    def __init__(self, setup: Setup, precision: int = FSG_FP32, batched: Optional[int] = None,
+                 device: int = 0):
+        self._lib = _lib.load()
+        self.setup = setup
+        self.precision = precision
+        self._cs = setup.cstruct(precision=precision, batched=batched, device=device)
+        self.ctx = C.c_void_p()
+        _lib.check(self._lib.fsg_init(C.byref(self._cs), C.byref(self.ctx)))
+
+    def close(self):
+        if self.ctx:
+            self._lib.fsg_finalize(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- line opacity -------------------------------------------------------------------
+    def LineopCalc(self, lines: LineList, NlinesLim: Optional[int] = None, linstr: bool = False) -> dict:
+        """Accumulates one list's line opacity into the device-resident LineOpac."""
+        n = lines.n
+        nlim = n if NlinesLim is None else int(NlinesLim)
+        nlim = max(nlim, 1)
+        flags = np.zeros(n, np.int32)
+        il = np.zeros(n, np.int32)
+        ir = np.zeros(n, np.int32)
+        ls = np.zeros(n, np.float32) if linstr else None
+        st = CStats()
+        cl = lines.cstruct()
+        _lib.check(self._lib.fsg_lineop(self.ctx, lines.list_type, C.byref(cl), nlim, ptr(ls, C.c_float),
+                                        ptr(flags, C.c_int32), ptr(il, C.c_int32), ptr(ir, C.c_int32),
+                                        C.byref(st)))
+        return dict(flags=flags, ileft=il, iright=ir, linstr=ls, stats=st.as_dict())
+
+    lineop_calc = LineopCalc
+
+    def upload(self, lines: LineList) -> DeviceLines:
+        return DeviceLines(self, lines)
+
+    def lineop_dev(self, dlines: DeviceLines, NlinesLim: Optional[int] = None) -> dict:
+        nlim = dlines.n if NlinesLim is None else int(NlinesLim)
+        st = CStats()
+        _lib.check(self._lib.fsg_lineop_dev(self.ctx, dlines.handle, max(nlim, 1), C.byref(st)))
+        return st.as_dict()
+
+    def reset_lineop(self):
+        _lib.check(self._lib.fsg_reset_lineop(self.ctx))
+
+    def get_lineop(self) -> np.ndarray:
+        out = np.zeros((self.setup.nwl, self.setup.ntau))
+        _lib.check(self._lib.fsg_get_lineop(self.ctx, ptr(out, C.c_double)))
+        return out
+
+    def set_lineop(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.shape == (self.setup.nwl, self.setup.ntau)
+        _lib.check(self._lib.fsg_set_lineop(self.ctx, ptr(a, C.c_double)))
+
+    # --- formal solution and flux -----------------------------------------------------
+    def RadTranCalc(self, intensities: bool = True):
+        s = self.setup
+        spl = np.zeros((s.nwl, s.nmu)) if intensities else None
+        spc = np.zeros((s.nwl, s.nmu)) if intensities else None
+        _lib.check(self._lib.fsg_radtran(self.ctx, ptr(spl, C.c_double), ptr(spc, C.c_double)))
+        return spl, spc
+
+    radtran_calc = RadTranCalc
+
+    def flux(self):
+        s = self.setup
+        sfl = np.zeros(s.nwl)
+        sfc = np.zeros(s.nwl)
+        _lib.check(self._lib.fsg_flux(self.ctx, ptr(sfl, C.c_double), ptr(sfc, C.c_double)))
+        return sfl, sfc
+
+    def IntegConvol(self, CImode: int = 2):
+        """CImode 2 with no rotation, macroturbulence or instrumental profile: the flux sums."""
+        if CImode != 2:
+            raise _lib.FsgError("IntegConvol: only the disk integration (CImode 2) is on the device")
+        return self.flux()
+
+    def device_ptr(self, which: int) -> int:
+        p = C.c_void_p()
+        _lib.check(self._lib.fsg_device_ptr(self.ctx, which, C.byref(p)))
+        return p.value
+
+    def stream(self) -> int:
+        p = C.c_void_p()
+        _lib.check(self._lib.fsg_stream(self.ctx, C.byref(p)))
+        return p.value
+
+
+__all__ = ["FastSpecGPU", "DeviceLines", "FSG_FP32", "FSG_FP64"]

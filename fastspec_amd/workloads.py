@@ -1,0 +1,354 @@
+"""Seeded synthetic inputs for the hot path.
+
+The reference obtains the atmosphere's thermodynamic state from its EOS and the continuum
+opacity from its ATLAS12-style routines; both stay on the host and are outside this
+library.  For measurements and parity tests we therefore build those host-side inputs
+synthetically, with the same shapes, units and orders of magnitude:
+
+* a grey Eddington atmosphere, T^4 = 3/4 Teff^4 (tau + 2/3), on 72 depths with
+  log tau_5000 uniform in [-5, 2];
+* hydrostatic equilibrium dP/dtau = g / kappa_std with a smooth H^- like opacity law;
+* Saha ionisation with constant partition functions for H, He and the line species,
+  giving N/U per cm^3 (FractionPF), electron and perturber densities;
+* a continuum opacity kappa_c(lambda, depth) = kappa_std(depth) * phi(lambda), phi smooth;
+* line lists drawn from the distributions of the benchmark configurations, already in the
+  form the reference's HDF5 reader produces (gf = 10**log gf in REAL(4), damping
+  constants as linear REAL(4) values).
+
+Everything is deterministic for a given seed (numpy PCG64).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .abi import FSG_LIST_ATOMIC, FSG_LIST_MIXED, FSG_LIST_MOLECULAR, LineList, Setup
+
+KB = 1.380649e-16
+MH = 1.673532499e-24
+EV = 1.602176634e-12
+VC = 299792.458
+SEED_BASE = 20261200
+
+# species of the atomic configurations: (name, Z, ion, ionisation potential [eV] of this
+# stage, atomic mass, log10 N/N_H, U(this stage), U(next stage))
+ATOMIC_SPECIES = [
+    ("Fe1", 26, 1, 7.902, 55.845, -4.50, 25.0, 40.0),
+    ("Fe2", 26, 2, 16.199, 55.845, -4.50, 40.0, 10.0),
+    ("Ti1", 22, 1, 6.828, 47.867, -7.05, 30.0, 55.0),
+    ("Ca1", 20, 1, 6.113, 40.078, -5.66, 1.2, 2.2),
+    ("Cr1", 24, 1, 6.767, 51.996, -6.36, 10.0, 7.0),
+]
+
+
+def eddington_temperature(tau: np.ndarray, teff: float) -> np.ndarray:
+    return (0.75 * teff**4 * (tau + 2.0 / 3.0)) ** 0.25
+
+
+def _kappa_std(p: float, t: float) -> float:
+    """Smooth H^- like Rosseland-ish opacity law [cm^2/g]."""
+    return 0.35 * (p / 1e5) ** 0.55 * (t / 5800.0) ** 7.0 + 2e-4
+
+
+def saha_ratio(t: np.ndarray, ne: np.ndarray, chi_ev: float, u0: float, u1: float) -> np.ndarray:
+    """N(i+1)/N(i) from the Saha equation."""
+    return 2.0 * u1 / u0 * 2.4147e15 * t**1.5 * np.exp(-chi_ev * EV / (KB * t)) / ne
+
+
+@dataclass
+class Atmosphere:
+    tau: np.ndarray
+    temp: np.ndarray
+    pgas: np.ndarray
+    rhox: np.ndarray
+    rho: np.ndarray
+    xne: np.ndarray
+    n_h: np.ndarray
+    kappa_std: np.ndarray
+    teff: float
+    logg: float
+
+
+def grey_atmosphere(teff: float = 5777.0, logg: float = 4.44, ntau: int = 72,
+                    logtau=(-5.0, 2.0)) -> Atmosphere:
+    """Grey Eddington model in hydrostatic equilibrium on a column-mass scale."""
+    g = 10.0**logg
+    tau = np.logspace(logtau[0], logtau[1], ntau)
+    temp = eddington_temperature(tau, teff)
+    # integrate dP/dln tau = tau g / kappa(P, T) with RK4 in ln tau
+    lnt = np.log(tau)
+    p = np.empty(ntau)
+    p0 = tau[0] * g / _kappa_std(1.0, temp[0])
+    for _ in range(20):  # P = tau g / kappa(P) at the top layer
+        p0 = tau[0] * g / _kappa_std(p0, temp[0])
+    p[0] = p0
+
+    def rhs(lt, pp):
+        tt = float(np.interp(lt, lnt, temp))
+        return math.exp(lt) * g / _kappa_std(pp, tt)
+
+    for i in range(ntau - 1):
+        h = lnt[i + 1] - lnt[i]
+        k1 = rhs(lnt[i], p[i])
+        k2 = rhs(lnt[i] + 0.5 * h, p[i] + 0.5 * h * k1)
+        k3 = rhs(lnt[i] + 0.5 * h, p[i] + 0.5 * h * k2)
+        k4 = rhs(lnt[i] + h, p[i] + h * k3)
+        p[i + 1] = p[i] + h / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+    kstd = np.array([_kappa_std(pp, tt) for pp, tt in zip(p, temp)])
+    rhox = p / g
+    # electron density: H plus a lumped low-potential metal donor, fixed-point iteration
+    ntot = p / (KB * temp)
+    n_h = ntot / 1.1
+    ne = np.maximum(1e-4 * n_h, 1.0)
+    for _ in range(60):
+        fh = saha_ratio(temp, ne, 13.598, 2.0, 1.0)
+        fm = saha_ratio(temp, ne, 7.5, 20.0, 30.0)
+        ne_new = n_h * (fh / (1.0 + fh)) + 1e-4 * n_h * (fm / (1.0 + fm))
+        ne = np.sqrt(ne * np.maximum(ne_new, 1.0))
+    rho = (ntot - ne) * 1.3 * MH
+    return Atmosphere(tau=tau, temp=temp, pgas=p, rhox=rhox, rho=rho, xne=ne, n_h=n_h,
+                      kappa_std=kstd, teff=teff, logg=logg)
+
+
+def continuum_shape(lam: np.ndarray) -> np.ndarray:
+    """phi(lambda): smooth H^- bound-free + free-free like wavelength dependence."""
+    x = lam / 5000.0
+    bf = np.where(lam < 16400.0, x**1.3 * (1.0 - 0.6 * (lam / 16400.0) ** 2), 0.0)
+    ff = 0.15 * x**2
+    scat = 0.05 * x**-4
+    return 0.55 + bf + ff + scat
+
+
+def gauss_mu(n: int):
+    """Gauss-Legendre limb angles on [0,1] (descending) and flux weights 2 w mu."""
+    z, w = np.polynomial.legendre.leggauss(n)
+    z = z[::-1].copy()
+    w = w[::-1].copy()
+    x = 0.5 * (1.0 + z)
+    wg = 0.5 * w
+    return x, 2.0 * wg * x
+
+
+def wave_grid(wstart: float, wend: float, step: float, wmode: int):
+    """The reference's grid (src/synth.f90:56-74): offsets from Wfirst."""
+    eps = 1e-12
+    if wmode == 0:
+        nwl = int(math.floor((wend - wstart) / step + eps)) + 1
+        wave = np.arange(nwl, dtype=np.float64) * step
+        return wave, wstart, step, 0.0
+    rstep = step / VC
+    nwl = int(math.floor(math.log(wend / wstart) / rstep + eps)) + 1
+    wave = wstart * np.exp(np.arange(nwl, dtype=np.float64) * rstep) - wstart
+    return wave, wstart, step, rstep
+
+
+def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode: int,
+                species=ATOMIC_SPECIES, nmu: int = 7, vmicro: float = 1.0,
+                motype: int = 0, eps_opac: float = 1e-4, extra_slots: int = 4) -> Setup:
+    """Host-side inputs (EOS tables, continuum) on the given grid.
+
+    EOS slots: 1..len(species) are the line species, then H I, He I, H2, H II, He II."""
+    ntau = atm.temp.shape[0]
+    wave, wfirst, dstep, rstep = wave_grid(wstart, wend, step, wmode)
+    lam = wave + wfirst
+    t, ne, nh = atm.temp, atm.xne, atm.n_h
+    nsp = len(species)
+    nslot = nsp + 5
+    pf_ld = nslot + extra_slots
+    fraction_pf = np.zeros((ntau, pf_ld), np.float32)
+    pot_ion = np.zeros(nslot, np.float32)
+    xamass = np.zeros(nslot, np.float32)
+    for j, (_, _z, ion, chi, mass, logab, u0, u1) in enumerate(species):
+        if ion == 1:
+            r = saha_ratio(t, ne, chi, u0, u1)
+            n_stage = 10.0**logab * nh / (1.0 + r)
+        else:  # first ion: use the neutral stage's potential of the same element
+            chi0 = [s for s in species if s[1] == _z and s[2] == 1][0][3]
+            r = saha_ratio(t, ne, chi0, 25.0, u0)
+            n_stage = 10.0**logab * nh * r / (1.0 + r)
+        fraction_pf[:, j] = (n_stage / u0).astype(np.float32)
+        pot_ion[j] = chi
+        xamass[j] = mass
+    fh = saha_ratio(t, ne, 13.598, 2.0, 1.0)
+    n_h1 = nh / (1.0 + fh)
+    n_hp = nh - n_h1
+    n_he = 0.085 * nh
+    fhe = saha_ratio(t, ne, 24.587, 1.0, 2.0)
+    n_he1 = n_he / (1.0 + fhe)
+    n_he2 = n_he - n_he1
+    n_h2 = 1e-6 * nh * np.clip(4000.0 / t, 0.0, 2.0) ** 8
+    for j, (n_s, chi, mass, u) in enumerate([(n_h1, 13.598, 1.008, 2.0), (n_he1, 24.587, 4.0026, 1.0),
+                                             (n_h2, 15.43, 2.016, 1.0), (n_hp, 0.0, 1.008, 1.0),
+                                             (n_he2, 54.418, 4.0026, 2.0)]):
+        fraction_pf[:, nsp + j] = (n_s / u).astype(np.float32)
+        pot_ion[nsp + j] = chi
+        xamass[nsp + j] = mass
+    contop = (atm.kappa_std[None, :] * continuum_shape(lam)[:, None]).astype(np.float64)
+    contop_std = None
+    rhox = atm.rhox
+    if motype == 1:
+        contop_std = atm.kappa_std * continuum_shape(np.array([5000.0]))[0]
+        contop = contop / contop_std[None, :]
+        rhox = atm.tau.copy()
+    xmu, wmu = gauss_mu(nmu)
+    return Setup(
+        wave=wave, wmode=wmode, wfirst=wfirst, dstep=dstep, rstep=rstep, wstart=wstart,
+        rhox=rhox, temp=t, xne=ne, rho=atm.rho, contop=contop, contop_std=contop_std,
+        fraction_pf=fraction_pf, pot_ion=pot_ion, xamass=xamass,
+        frac_h1=n_h1.astype(np.float32), frac_he1=n_he1.astype(np.float32),
+        frac_hmol=n_h2.astype(np.float32),
+        spc_ion=np.array([s[2] for s in species], np.int32),
+        spc_slot=np.arange(1, nsp + 1, dtype=np.int32),
+        he1_slot=nsp + 2, xmu=xmu, wmu=wmu, motype=motype, eps_opac=eps_opac, vmicro=vmicro,
+        meta=dict(atmosphere=atm, species=[s[0] for s in species]),
+    )
+
+
+def _f32pow10(x):
+    return np.power(np.float32(10.0), np.asarray(x, np.float32)).astype(np.float32)
+
+
+def atomic_lines(n: int, wl_lo: float, wl_hi: float, seed: int, log_uniform: bool = False,
+                 nspecies: int = len(ATOMIC_SPECIES)) -> LineList:
+    """Seeded atomic list with the benchmark distributions (sorted by wavelength)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if log_uniform:
+        wl = np.exp(rng.uniform(math.log(wl_lo), math.log(wl_hi), n))
+    else:
+        wl = rng.uniform(wl_lo, wl_hi, n)
+    wl.sort()
+    lggf = rng.uniform(-5.0, 0.0, n)
+    elo = rng.uniform(0.0, 6.0, n)
+    spc = rng.integers(1, nspecies + 1, n)
+    grad = rng.normal(8.0, 0.3, n)
+    gqst = rng.normal(-6.0, 0.5, n)
+    gvdw = rng.normal(-7.5, 0.3, n)
+    return LineList(
+        wl=wl, elo=elo.astype(np.float32), gf=_f32pow10(lggf), grad=_f32pow10(grad),
+        gqst=_f32pow10(gqst), gvdw=_f32pow10(gvdw), spc=spc.astype(np.int32),
+        list_type=FSG_LIST_ATOMIC,
+    )
+
+
+def molecular_lines(n: int, wl_lo: float, wl_hi: float, seed: int, spc: int = 1) -> LineList:
+    """Seeded single-species diatomic list: log gf ~ N(-4, 1.5), E_low ~ Exp(1 eV)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    wl = np.sort(np.exp(rng.uniform(math.log(wl_lo), math.log(wl_hi), n)))
+    lggf = rng.normal(-4.0, 1.5, n)
+    elo = rng.exponential(1.0, n)
+    grad = rng.normal(8.0, 0.3, n)
+    return LineList(wl=wl, elo=elo.astype(np.float32), gf=_f32pow10(lggf), grad=_f32pow10(grad),
+                    spc=np.full(n, spc, np.int32), list_type=FSG_LIST_MOLECULAR)
+
+
+def zeeman_pattern(jlo: float, jup: float, glo: float, gup: float):
+    """Anomalous Zeeman pattern in the reference's convention (src/utils.f90:901-972):
+    pi shifts/strengths and one sigma branch, strengths normalised per branch."""
+    tol = 1e-5
+    nm = int(round(2 * jup)) + 1
+    gp, rp, gs, rs = [], [], [], []
+    for i in range(nm):
+        m = jup - i
+        gp.append(m * (gup - glo))
+        gs.append(m * gup - (m + 1.0) * glo)
+        if jlo == jup:
+            rp.append(m * m)
+            rs.append(0.25 * (jup - m) * (jup + m + 1.0))
+        elif jlo > jup:
+            rp.append((jup + 1.0) ** 2 - m * m)
+            rs.append(0.25 * (jup + m + 1.0) * (jup + m + 2.0))
+        else:
+            rp.append(jup**2 - m * m)
+            rs.append(0.25 * (jup - m) * (jup - m - 1.0))
+    gp = [0.0 if abs(x) < tol else x for x in gp]
+    gs = [0.0 if abs(x) < tol else x for x in gs]
+    P = [(g, r) for g, r in zip(gp, rp) if abs(r) > tol]
+    S = [(-g, r) for g, r in zip(gs, rs) if abs(r) > tol]
+
+    def merge(comp):
+        out = []
+        for g, r in comp:
+            for k, (gm, rm) in enumerate(out):
+                if abs(g - gm) < tol:
+                    out[k] = (gm, rm + r)
+                    break
+            else:
+                out.append((g, r))
+        return out
+
+    P, S = merge(P), merge(S)
+    sp = sum(r for _, r in P)
+    ss = sum(r for _, r in S)
+    P = [(g, r / sp if sp > 0 else r) for g, r in P]
+    S = [(g, r / ss if ss > 0 else r) for g, r in S]
+    return P, S
+
+
+def magnetic_lines(n: int, wl_lo: float, wl_hi: float, seed: int) -> LineList:
+    """Seeded Zeeman list (mixed .z type): J <= 6, Lande g ~ U(0.5, 2.5); sigma components
+    stored blue (negated shifts) then red, as the reference's reader does
+    (src/ioutils.f90:1012-1031)."""
+    base = atomic_lines(n, wl_lo, wl_hi, seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 7))
+    pi_off, si_off = [0], [0]
+    spl_pi, str_pi, spl_si, str_si = [], [], [], []
+    for _ in range(n):
+        while True:
+            jlo = 0.5 * rng.integers(0, 13)
+            dj = rng.integers(-1, 2)
+            jup = jlo + dj
+            if jup < 0 or jup > 6 or (jlo == 0 and jup == 0):
+                continue
+            break
+        glo, gup = rng.uniform(0.5, 2.5, 2)
+        P, S = zeeman_pattern(float(jlo), float(jup), float(np.float32(glo)), float(np.float32(gup)))
+        for g, r in P:
+            spl_pi.append(g)
+            str_pi.append(r)
+        for g, r in S:
+            spl_si.append(-g)
+            str_si.append(r)
+        for g, r in S:
+            spl_si.append(g)
+            str_si.append(r)
+        pi_off.append(len(spl_pi))
+        si_off.append(len(spl_si))
+    return LineList(
+        wl=base.wl, elo=base.elo, gf=base.gf, grad=base.grad, gqst=base.gqst, gvdw=base.gvdw,
+        spc=base.spc, is_mol=np.zeros(n, np.uint8), list_type=FSG_LIST_MIXED,
+        pi_off=np.array(pi_off, np.int32), spl_pi=np.array(spl_pi, np.float32),
+        str_pi=np.array(str_pi, np.float32), si_off=np.array(si_off, np.int32),
+        spl_si=np.array(spl_si, np.float32), str_si=np.array(str_si, np.float32),
+    )
+
+
+# --- the benchmark configurations -------------------------------------------------------
+
+def config_c1(nlines: int = 20000):
+    """Plain FP64 scalar synthesis: 20,000 lines in 5000-5100 A, 0.01 A step, 7 mu."""
+    atm = grey_atmosphere(5777.0, 4.44)
+    setup = build_setup(atm, 5000.0, 5100.0, 0.01, 0)
+    lines = atomic_lines(nlines, 5000.0, 5100.0, SEED_BASE + 1)
+    return setup, lines
+
+
+def config_c2(nlines: int = 2_000_000, wl_lo: float = 3000.0, wl_hi: float = 10000.0):
+    """Large atomic run: 2,000,000 lines log-uniform in 3000-10000 A on an R=10^6 log grid."""
+    atm = grey_atmosphere(5777.0, 4.44)
+    setup = build_setup(atm, wl_lo, wl_hi, 0.299792458, 1)
+    # line density of the full configuration kept for sub-ranges
+    frac = math.log(wl_hi / wl_lo) / math.log(10000.0 / 3000.0)
+    n = int(round(nlines * frac)) if (wl_lo, wl_hi) != (3000.0, 10000.0) else nlines
+    lines = atomic_lines(n, wl_lo, wl_hi, SEED_BASE + 2, log_uniform=True)
+    return setup, lines
+
+
+def config_c4(nlines: int = 5000):
+    """Stokes run: 5,000 Zeeman lines in 6000-6100 A, 0.005 A step, B = 2 kG at 45/30 deg."""
+    atm = grey_atmosphere(5777.0, 4.44)
+    setup = build_setup(atm, 6000.0, 6100.0, 0.005, 0)
+    lines = magnetic_lines(nlines, 6000.0, 6100.0, SEED_BASE + 4)
+    bfield = np.array([2000.0, 45.0, 30.0])
+    return setup, lines, bfield, 1

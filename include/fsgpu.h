@@ -1,0 +1,189 @@
+/*
+ * fsgpu.h -- C-ABI of the MI355X line-opacity and formal-solution library (libfsgpu.so).
+ *
+ * The library replaces the hot-path module procedures of FastSpec with HIP kernels for
+ * gfx950.  Every entry point takes plain pointers and sizes; no C++ or torch types cross
+ * the boundary.  The Fortran host binds these symbols with INTERFACE ... BIND(C) blocks
+ * (fastspec_amd/fortran/fsgpu_host.f90), the Python side with ctypes
+ * (fastspec_amd/_lib.py).
+ *
+ * Conventions
+ *   - Return value 0 = success.  Non-zero = failure; fsg_last_error() then returns the
+ *     message (the reference's ErrStop text where the reference has one).
+ *   - Array layouts follow the Fortran host (column-major, depth fastest):
+ *       ContOpac/LineOpac(Ntau,Nwl)  -> C array [nwl][ntau]
+ *       FractionPF(Maxspl,Ntau)      -> C array [ntau][pf_ld]   (REAL(4))
+ *       SpL/SpC(Nmu,Nwl)             -> C array [nwl][nmu]
+ *       SpZ(Nmu,4,Nwl)               -> C array [nwl][4][nmu]
+ *       MagLineOpac(6,Ntau,Nwl)      -> C array [nwl][ntau][6]
+ *   - Species indices (LinSpc) and EOS slots (IdxSpc) are 1-based, as in the host.
+ *   - Wavelengths of the grid are offsets from Wfirst (the reference's Wave array);
+ *     line wavelengths are absolute air wavelengths in Angstrom (Lines%Wl).
+ *   - One context per process and device; calls on one context are not re-entrant.
+ *   - The host owns every host buffer; the library owns device memory.
+ */
+#ifndef FSGPU_H
+#define FSGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* arithmetic of the line-opacity stage */
+#define FSG_FP64 0   /* plain path: FP64 Voigt table interpolation and FP64 sums        */
+#define FSG_FP32 1   /* batched path: FP32 Voigt in registers, Kahan-compensated sums  */
+
+/* line-list types (LineListType in the reference, src/shared.f90:403) */
+#define FSG_LIST_ATOMIC    0
+#define FSG_LIST_MOLECULAR 1
+#define FSG_LIST_MIXED     2   /* .z / Zeeman lists: per-line molecule flag */
+
+/* Everything LineopCalc/RadTranCalc read from module globals (src/shared.f90:392-548,
+ * src/synth.f90:747-1095, src/synth.f90:1825-1901).  Pointers are host memory and are
+ * copied by fsg_init; they need not outlive the call. */
+typedef struct fsg_setup {
+  int32_t ntau, nwl, nmu;
+  /* wavelength grid (src/synth.f90:56-74) */
+  int32_t wmode;                 /* 0 linear, 1 logarithmic                          */
+  double wfirst, dstep, rstep, wstart;
+  const double *wave;            /* [nwl] offsets from wfirst                        */
+  /* atmosphere (AtmosData) */
+  int32_t motype;                /* 0 column mass, 1 tau scale                       */
+  const double *rhox, *temp, *xne, *rho;   /* [ntau]                                 */
+  const double *contop;          /* [nwl][ntau] ContOpac                             */
+  const double *contop_std;      /* [ntau] ContOpacStd (Motype 1) or NULL            */
+  /* EOS outputs (REAL(4) in the host) */
+  int32_t pf_ld;                 /* leading dimension of fraction_pf (Maxspl)         */
+  int32_t nslot;                 /* number of valid EOS slots (<= pf_ld)              */
+  const float *fraction_pf;      /* [ntau][pf_ld] FractionPF                         */
+  const float *pot_ion;          /* [nslot] PotIon                                   */
+  const float *xamass;           /* [nslot] Xamass                                   */
+  const float *frac_h1, *frac_he1, *frac_hmol;  /* [ntau] Fraction of H I, He I, H2  */
+  /* global species table: LinSpc -> Ion, IdxSpc (1-based) */
+  int32_t nspc;
+  const int32_t *spc_ion;        /* [nspc] Ion(ispc)                                 */
+  const int32_t *spc_slot;       /* [nspc] IdxSpc(ispc)                              */
+  int32_t he1_slot;              /* IdxSpc(Ix%He1), 0 if absent                      */
+  /* tabulated He I Stark lines (src/hestark.f90:43-60) */
+  int32_t nhe;                   /* 0..255                                           */
+  const double *he_wl, *he_elo;  /* [nhe] match table WlcHe, EloHe                   */
+  double he_wl_tol, he_elo_tol;  /* HeMatchTol, HeEloTol                             */
+  const double *he_gam, *he_shf; /* [nhe][ntau] HeGam(Ntau,NheTab), HeShf            */
+  /* Config scalars (src/shared.f90:416-425) */
+  double eps_opac, vwfudge, h2brd, vwmol, vmicro;
+  int32_t molbrd;
+  /* limb angles and flux weights (Xmu, Wmu from MuQuadrature) */
+  const double *xmu, *wmu;       /* [nmu]                                            */
+  /* library options */
+  int32_t precision;             /* FSG_FP64 or FSG_FP32                             */
+  int32_t batched;               /* 1: lines binned by window width (batched path);
+                                    0: one bin, the reference's index-range scan     */
+  int32_t device;                /* HIP device ordinal                               */
+} fsg_setup;
+
+/* One line list in structure-of-arrays form, after the host's reader conversions
+ * (src/ioutils.f90:1055-1076): gf, Grad, Gqst, Gvdw are linear REAL(4) values. */
+typedef struct fsg_lines {
+  int64_t n;
+  const double *wl;              /* [n] Lines%Wl, ascending                          */
+  const float *elo, *gf, *grad;  /* [n]                                              */
+  const float *gqst, *gvdw;      /* [n]; NULL for molecular lists                    */
+  const int32_t *spc;            /* [n] LinSpc (1-based global species index)        */
+  const uint8_t *is_mol;         /* [n] IsMolecule(El) for mixed lists, else NULL    */
+  /* Zeeman patterns in CSR form for magnetic lists (src/shared.f90:473-481),
+   * shifts in units of the reference (before the Bsplit*lambda^2 scaling) */
+  const int32_t *pi_off;         /* [n+1] offsets into spl_pi/str_pi                 */
+  const float *spl_pi, *str_pi;
+  const int32_t *si_off;         /* [n+1] offsets into spl_si/str_si (blue then red) */
+  const float *spl_si, *str_si;
+} fsg_lines;
+
+/* counters reported by the line-opacity stage */
+typedef struct fsg_lineop_stats {
+  int64_t nlines_in;             /* Nlines on entry                                  */
+  int64_t nlines_use;            /* NlinesUse: retained lines                        */
+  int64_t nwide;                 /* Nwide                                            */
+  int64_t nblocks;               /* coefficient block fills                          */
+  int64_t nsegments;             /* wavelength segments                              */
+  int64_t evals;                 /* sum over retained lines of window pixels x Ntau  */
+  double ms_pass1, ms_opacity;   /* device time of the two stages                    */
+} fsg_lineop_stats;
+
+typedef struct fsg_ctx fsg_ctx;
+
+/* Set-up for one synthesis: copies grid, atmosphere, EOS tables and ContOpac to the
+ * device, builds the Voigt table and the per-depth kernel constants (the device side of
+ * Init, src/synth.f90:22-311: VoigtInit src/voigt.f90:42, XlineInit src/opacity.f90:65).
+ * LineOpac starts at zero (src/synth.f90:91). */
+int fsg_init(const fsg_setup *setup, fsg_ctx **ctx_out);
+
+/* Releases all device memory of the context. */
+int fsg_finalize(fsg_ctx *ctx);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *fsg_last_error(void);
+
+/* Non-magnetic line opacity of one list, accumulated into the device-resident LineOpac:
+ * replaces LineopCalc(ifile, NlinesLim) (src/synth.f90:747-1095).
+ *   nlines_lim   block size NlinesMax(LimitRAM) computed by the host (src/opacity.f90:713)
+ *   linstr_out   [n] central line depths (LinStr, src/synth.f90:1774) or NULL
+ *   flags/ileft/iright_out  [n] LinFlag, Ileft, Iright of the input lines (1-based
+ *                indices, before PACK) or NULL
+ *   stats        or NULL */
+int fsg_lineop(fsg_ctx *ctx, int32_t list_type, const fsg_lines *lines, int64_t nlines_lim,
+               float *linstr_out, int32_t *flags_out, int32_t *ileft_out,
+               int32_t *iright_out, fsg_lineop_stats *stats);
+
+/* Magnetic line opacity of one Zeeman list into the device-resident MagLineOpac:
+ * replaces MagLineopCalc(ifile, NlinesLim) (src/synth.f90:1098-1455).
+ *   bfield[3], bframe  BFIELD / BFRAME (only |B| enters here, src/synth.f90:1150-1154) */
+int fsg_maglineop(fsg_ctx *ctx, const fsg_lines *lines, int64_t nlines_lim,
+                  const double *bfield, int32_t bframe, float *linstr_out,
+                  int32_t *flags_out, int32_t *ileft_out, int32_t *iright_out,
+                  fsg_lineop_stats *stats);
+
+/* Copies LineOpac (Ntau,Nwl) between host and device, e.g. for SAVE flag 2
+ * (src/ioutils.f90:1759-1771) and the OPACITY keyword (src/fastspec.f90:215). */
+int fsg_get_lineop(fsg_ctx *ctx, double *lineop_out);
+int fsg_set_lineop(fsg_ctx *ctx, const double *lineop_in);
+int fsg_get_maglineop(fsg_ctx *ctx, double *maglineop_out);
+
+/* Scalar plane-parallel formal solution: replaces RadTranCalc (src/synth.f90:1825-1901,
+ * LocPrfPP src/solver.f90:123-179).  spl_out/spc_out [nwl][nmu] SpL/SpC, or NULL to
+ * keep the intensities on the device for fsg_flux. */
+int fsg_radtran(fsg_ctx *ctx, double *spl_out, double *spc_out);
+
+/* Polarised plane-parallel formal solution: replaces MagRadTranCalc
+ * (src/synth.f90:1904-2111, StokesPrfPP src/solver.f90:265-394).
+ *   bfield [3] or [1+2*nmu] for bframe 2;  spz_out [nwl][4][nmu], spc_out [nwl][nmu] */
+int fsg_magradtran(fsg_ctx *ctx, const double *bfield, int32_t nbfield, int32_t bframe,
+                   double *spz_out, double *spc_out);
+
+/* Disk integration of the last formal solution, Sf = sum_mu Wmu I_mu: the flux sums of
+ * DiskInteg without convolution (src/convol.f90:67-246, lines 233-234).  Uses Stokes I
+ * after fsg_magradtran (src/fastspec.f90:437-440).  sfl_out/sfc_out [nwl]. */
+int fsg_flux(fsg_ctx *ctx, double *sfl_out, double *sfc_out);
+
+/* Device-resident variants for callers that keep data in HBM (the PyTorch wrapper and
+ * the benchmark).  A device line list is uploaded once and reused. */
+typedef struct fsg_dlines fsg_dlines;
+int fsg_lines_upload(fsg_ctx *ctx, int32_t list_type, const fsg_lines *lines,
+                     fsg_dlines **out);
+int fsg_lines_free(fsg_dlines *dl);
+int fsg_lineop_dev(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_stats *stats);
+/* Device pointers to the context's resident arrays (valid until fsg_finalize):
+ *   which = 0 LineOpac [ntau][nwl] (lambda fastest; FP32 or FP64 by precision)
+ *           1 SpL [nmu][nwl] FP64   2 SpC [nmu][nwl] FP64
+ *           3 SfL [nwl] FP64        4 SfC [nwl] FP64 */
+int fsg_device_ptr(fsg_ctx *ctx, int32_t which, void **ptr_out);
+/* Stream the library launches on (hipStream_t as void*). */
+int fsg_stream(fsg_ctx *ctx, void **stream_out);
+/* Zeroes the device-resident LineOpac (and MagLineOpac). */
+int fsg_reset_lineop(fsg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSGPU_H */
