@@ -1,0 +1,161 @@
+"""Benchmark of the line-opacity + formal-solution hot path on one MI355X.
+
+Workload: the large atomic configuration -- 2,000,000 seeded lines, lambda log-uniform in
+3000-10000 A, log-lambda grid at R = 10^6 (1,203,973 points), grey Eddington atmosphere
+(Teff 5777 K, log g 4.44, 72 depths), 7 mu angles.  Synthetic inputs (see
+fastspec_amd/workloads.py); lines and host tables are uploaded to HBM before timing.
+
+One step = one spectrum: LineOpac reset, pass 1 (rejection + windows), width binning,
+coefficient fill, tiled opacity accumulation (FP32 Voigt, Kahan sums), FP64 DELO-Bezier for
+continuum and line+continuum at 7 mu, flux sums.  The FP64 plain GPU path (table Voigt,
+FP64 sums, one bin) is timed on the same input and the FP32 flux is compared with it.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "line·depth·λ evals/s + spectra/s; FP32-batched vs FP64 GPU at ≤1e-4 flux err"
+PEAK_FP32_TFLOPS = 157.3
+FLOP_PER_EVAL = 14          # U_op cost of the table-lerp evaluation (fixed definition)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_path(setup, lines, precision, batched, steps, warmup):
+    from fastspec_amd.synth import FastSpecGPU
+
+    g = FastSpecGPU(setup, precision=precision, batched=batched)
+    dl = g.upload(lines)
+    st = None
+    for _ in range(warmup):
+        g.reset_lineop()
+        st = g.lineop_dev(dl)
+        g.RadTranCalc(intensities=False)
+    t0 = time.perf_counter()
+    kms, p1ms = [], []
+    for _ in range(steps):
+        g.reset_lineop()
+        st = g.lineop_dev(dl)
+        g.RadTranCalc(intensities=False)   # synchronises the library stream
+        kms.append(st["ms_opacity"])
+        p1ms.append(st["ms_pass1"])
+    t1 = time.perf_counter()
+    sfl, sfc = g.flux()
+    dl.free()
+    g.close()
+    return dict(ms_per_step=(t1 - t0) * 1e3 / steps, stats=st, opacity_ms=float(np.mean(kms)),
+                pass1_ms=float(np.mean(p1ms)), fnorm=sfl / sfc)
+
+
+def cpu_baseline(budget_s: float = 20.0):
+    """The reference's own OpenMP kernels (oracle/_ref, built from the reference sources)
+    on a bounded sample of the same workload: the C2 grid and line density restricted to
+    a wavelength range around 5000 A.  Times LineOpacity1 over the sample grid and the
+    RadTranCalc loop (LocPrfPP); counts evaluations from the windows."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from oracle_bridge import Oracle, Ref, ref_available
+
+    from fastspec_amd import workloads as W
+
+    if not ref_available():
+        return None
+    width = 150.0
+    setup, lines = W.config_c2(wl_lo=5000.0, wl_hi=5000.0 + width)
+    orc = Oracle(setup)
+    ro = orc.pass1(lines)
+    ref = Ref(setup)
+    ref.set_lines(lines)
+    cop, vv, av, _ = ref.xline()
+    ref.set_windows(ro["flags"] & ~8, ro["ileft"], ro["iright"])
+    ref.set_block(cop, vv, av)
+    nthreads = int(ref.lib.ref_nthreads())
+    evals = int(ro["stats"]["evals"])
+    t0 = time.perf_counter()
+    lop = ref.lineop1(1, setup.nwl, 1, lines.n)
+    ref.radtran(1, setup.nwl, lop)
+    dt = time.perf_counter() - t0
+    return {"value": evals / dt, "unit": "line·depth·λ evals/s", "cores": nthreads,
+            "kind": "reference",
+            "sample": f"C2 restricted to {5000:.0f}-{5000 + width:.0f} A ({setup.nwl} grid points, "
+                      f"{lines.n} lines, {evals:.3e} evals); reference LineOpacity1 + LocPrfPP "
+                      f"(amdflang -O2 -fopenmp, {nthreads} threads) in {dt:.2f} s",
+            "seconds": dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--nlines", type=int, default=2_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              "(multi-GPU lambda sharding is described in DESIGN.md, not implemented).", file=sys.stderr)
+        sys.exit(2)
+
+    from fastspec_amd import workloads as W
+    from fastspec_amd.abi import FSG_FP32, FSG_FP64
+
+    log("building the C2 workload")
+    setup, lines = W.config_c2(nlines=args.nlines)
+    log(f"nwl={setup.nwl} ntau={setup.ntau} nmu={setup.nmu} nlines={lines.n}")
+    r32 = run_path(setup, lines, FSG_FP32, 1, args.steps, args.warmup)
+    st = r32["stats"]
+    log(f"fp32 batched: {r32['ms_per_step']:.2f} ms/step, opacity kernel {r32['opacity_ms']:.2f} ms, "
+        f"evals {st['evals']:.3e}, retained {st['nlines_use']}")
+    evals = float(st["evals"])
+    value = evals / (r32["ms_per_step"] * 1e-3)
+    achieved = evals * FLOP_PER_EVAL / (r32["opacity_ms"] * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": value, "unit": "line·depth·λ evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 opacity (Kahan), f64 formal solution", "data": "synthetic",
+        "config": {"workload": "C2 large atomic: 2,000,000 lines log-uniform 3000-10000 A, R=1e6 "
+                               "log grid (1,203,973 points), 72 depths, 7 mu, grey Teff 5777 logg 4.44",
+                   "nlines": lines.n, "nwl": setup.nwl, "ntau": setup.ntau, "nmu": setup.nmu,
+                   "parallelism": "single GPU"},
+        "spectra_per_s": 1e3 / r32["ms_per_step"],
+        "evals_per_step": evals,
+        "retained_lines": st["nlines_use"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "k_opacity<Coef32,true>", "kernel_ms": r32["opacity_ms"],
+                     "flop_per_eval": FLOP_PER_EVAL},
+    }
+    if not args.no_fp64:
+        r64 = run_path(setup, lines, FSG_FP64, 0, args.fp64_steps, 0)
+        err = float(np.max(np.abs(r32["fnorm"] - r64["fnorm"])))
+        log(f"fp64 plain: {r64['ms_per_step']:.2f} ms/step; max |dFn| fp32 vs fp64 = {err:.3e}")
+        out["fp64_plain"] = {"ms_per_step": r64["ms_per_step"], "spectra_per_s": 1e3 / r64["ms_per_step"],
+                             "evals_per_s": evals / (r64["ms_per_step"] * 1e-3),
+                             "opacity_kernel_ms": r64["opacity_ms"]}
+        out["speedup_vs_fp64_plain"] = r64["ms_per_step"] / r32["ms_per_step"]
+        out["max_flux_err_vs_fp64"] = err
+    if not args.no_cpu_baseline:
+        log("timing the reference CPU kernels on a bounded sample")
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1185,7 +1185,7 @@ static void mag_motype_div(const fso_ctx *c, double *M6) {
  * ------------------------------------------------------------------------------------- */
 static int lineop_common(fso_ctx *c, int list_type, const fsg_lines *L, int64_t nlines_lim, int magnetic,
                          const double *bfield, int bframe, float *linstr_out, int32_t *flags_out,
-                         int32_t *ileft_out, int32_t *iright_out, fsg_lineop_stats *stats) {
+                         int32_t *ileft_out, int32_t *iright_out, fsg_lineop_stats *stats, int pass1_only) {
   int N = c->ntau, Nwl = c->nwl;
   int64_t Nlines = L->n;
   fsg_lineop_stats st;
@@ -1333,7 +1333,13 @@ static int lineop_common(fso_ctx *c, int list_type, const fsg_lines *L, int64_t 
     Nlines = n;
     ls.nlines = n;
   }
-  if (Nlines > 0) {
+  if (pass1_only) {
+    for (int64_t i = 1; i <= Nlines; i++)
+      if (!(ls.LinFlag[i] & 1)) {
+        st.nlines_use++;
+        st.evals += (int64_t)(ls.Iright[i] - ls.Ileft[i] + 1) * N;
+      }
+  } else if (Nlines > 0) {
     int64_t Nwide = mark_wide_lines(c, &ls, NlinesStep);
     st.nwide = Nwide;
     int *Lstart = malloc(sizeof(int) * (Nwl + 1)), *Lstop = malloc(sizeof(int) * (Nwl + 1));
@@ -1466,14 +1472,21 @@ static int lineop_common(fso_ctx *c, int list_type, const fsg_lines *L, int64_t 
 EXPORT int fso_lineop(fso_ctx *c, int32_t list_type, const fsg_lines *L, int64_t nlines_lim, float *linstr_out,
                       int32_t *flags_out, int32_t *ileft_out, int32_t *iright_out, fsg_lineop_stats *stats) {
   return lineop_common(c, list_type, L, nlines_lim, 0, NULL, 0, linstr_out, flags_out, ileft_out, iright_out,
-                       stats);
+                       stats, 0);
+}
+
+/* pass 1 of LineopCalc only: flags, windows and the evaluation count, no accumulation */
+EXPORT int fso_lineop_pass1(fso_ctx *c, int32_t list_type, const fsg_lines *L, int32_t *flags_out,
+                            int32_t *ileft_out, int32_t *iright_out, fsg_lineop_stats *stats) {
+  return lineop_common(c, list_type, L, L->n > 0 ? L->n : 1, 0, NULL, 0, NULL, flags_out, ileft_out, iright_out,
+                       stats, 1);
 }
 
 EXPORT int fso_maglineop(fso_ctx *c, const fsg_lines *L, int64_t nlines_lim, const double *bfield, int32_t bframe,
                          float *linstr_out, int32_t *flags_out, int32_t *ileft_out, int32_t *iright_out,
                          fsg_lineop_stats *stats) {
   return lineop_common(c, FSG_LIST_MIXED, L, nlines_lim, 1, bfield, bframe, linstr_out, flags_out, ileft_out,
-                       iright_out, stats);
+                       iright_out, stats, 0);
 }
 
 /* Per-line coefficients for tests: XlineSetup + XlineDepth of line i (0-based) with the

@@ -35,6 +35,7 @@ def _load_oracle() -> C.CDLL:
     lib.fso_finalize.argtypes = [vp]
     lib.fso_lineop.argtypes = [vp, C.c_int32, C.POINTER(CLines), C.c_int64, _fp, _ip, _ip, _ip,
                                C.POINTER(CStats)]
+    lib.fso_lineop_pass1.argtypes = [vp, C.c_int32, C.POINTER(CLines), _ip, _ip, _ip, C.POINTER(CStats)]
     lib.fso_maglineop.argtypes = [vp, C.POINTER(CLines), C.c_int64, _dp, C.c_int32, _fp, _ip, _ip,
                                   _ip, C.POINTER(CStats)]
     lib.fso_radtran.argtypes = [vp, _dp, _dp]
@@ -116,6 +117,19 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(self.lib.fso_last_error().decode())
         return dict(flags=flags, ileft=il, iright=ir, linstr=ls, stats=st.as_dict())
+
+    def pass1(self, lines: LineList):
+        n = lines.n
+        cl = lines.cstruct()
+        flags = np.zeros(n, np.int32)
+        il = np.zeros(n, np.int32)
+        ir = np.zeros(n, np.int32)
+        st = CStats()
+        rc = self.lib.fso_lineop_pass1(self.ctx, lines.list_type, C.byref(cl), ptr(flags, C.c_int32),
+                                       ptr(il, C.c_int32), ptr(ir, C.c_int32), C.byref(st))
+        if rc != 0:
+            raise RuntimeError(self.lib.fso_last_error().decode())
+        return dict(flags=flags, ileft=il, iright=ir, stats=st.as_dict())
 
     def maglineop(self, lines: LineList, bfield, bframe: int, nlines_lim: int | None = None,
                   linstr: bool = False):

@@ -49,7 +49,7 @@ def _check(ref, g, opac_rel, flux_abs, windows_exact=True):
     assert rel <= opac_rel, rel
     spl, spc = g.RadTranCalc()
     sfl, sfc = g.flux()
-    assert np.max(np.abs(spc - ref["spc"]) / ref["spc"]) <= 1e-12
+    assert np.max(np.abs(spc - ref["spc"]) / ref["spc"]) <= 1e-9
     fe = np.max(np.abs(sfl / sfc - ref["sfl"] / ref["sfc"]))
     assert fe <= flux_abs, fe
     ie = np.max(np.abs(spl - ref["spl"]) / ref["spl"])
