@@ -187,12 +187,16 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
   rc |= upload(c, s->spc_slot, s->nspc, &S.spc_slot, c->owned);
   rc |= upload(c, s->xmu, s->nmu, &S.xmu, c->owned);
   rc |= upload(c, s->wmu, s->nmu, &S.wmu, c->owned);
+  S.ntp = DPW * ((N + DPW - 1) / DPW);
+  std::vector<double> shf((size_t)std::max(s->nhe, 1) * S.ntp, 0.0);  // zero-padded to ntp depths
   if (s->nhe > 0) {
     rc |= upload(c, s->he_wl, s->nhe, &S.he_wl, c->owned);
     rc |= upload(c, s->he_elo, s->nhe, &S.he_elo, c->owned);
     rc |= upload(c, s->he_gam, (size_t)s->nhe * N, &S.he_gam, c->owned);
-    rc |= upload(c, s->he_shf, (size_t)s->nhe * N, &S.he_shf, c->owned);
+    for (int e = 0; e < s->nhe; e++)
+      for (int k = 0; k < N; k++) shf[(size_t)e * S.ntp + k] = s->he_shf[(size_t)e * N + k];
   }
+  rc |= upload(c, shf.data(), shf.size(), &S.he_shf, c->owned);
   std::vector<double> cstd(N, 1.0);
   if (s->motype == 1) std::memcpy(cstd.data(), s->contop_std, sizeof(double) * N);
   rc |= upload(c, cstd.data(), N, &S.cstd, c->owned);
@@ -250,7 +254,7 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
   if (alloc_zero(c, lop_elem(c) * (size_t)N * Nw, &c->lineop) ||
       alloc_zero(c, sizeof(int) * 64, (void **)&c->bincnt) ||
       alloc_zero(c, sizeof(int) * 64, (void **)&c->binhmax) ||
-      alloc_zero(c, sizeof(int) * 3 * MAX_BINS, (void **)&c->binoff) ||
+      alloc_zero(c, sizeof(int) * 3 * (MAX_BINS + 1), (void **)&c->binoff) ||
       alloc_zero(c, sizeof(unsigned long long) * 4, (void **)&c->u64)) {
     fsg_finalize(c);
     return 1;
@@ -375,12 +379,15 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
   CK(hipEventRecord(c->ev[1], s));
   CK(hipStreamSynchronize(s));
-  int64_t nret = 0;
+  // retained lines: width bins 0..nbins-1, then the He I Stark lines (HE_KEY)
+  int64_t nmain = 0;
   int nbins = 0;
-  for (int b = 0; b < WEAK_KEY; b++) {
-    nret += hcnt[b];
+  for (int b = 0; b < HE_KEY; b++) {
+    nmain += hcnt[b];
     if (hcnt[b] > 0) nbins = b + 1;
   }
+  const int64_t nhe = hcnt[HE_KEY];
+  const int64_t nret = nmain + nhe;
   if (nbins > MAX_BINS) return fail("fsg_lineop_dev: too many window-width bins");
   st.nlines_use = nret;
   st.evals = (int64_t)hu[0];
@@ -401,24 +408,20 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   launch_mark_wide(S, n, dl->L.wl, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), c->wlcl.as<double>(),
                    pack ? nret : n, nstep, c->u64 + 1, s);
   if (nret > 0) {
-    // bin offsets, counts and maximal half-widths of the candidate search
-    int hoff[MAX_BINS * 3];
-    int o = 0;
-    for (int b = 0; b < nbins; b++) {
-      hoff[b] = o;
-      hoff[MAX_BINS + b] = hcnt[b];
-      hoff[2 * MAX_BINS + b] = hhmax[b];
-      o += hcnt[b];
+    // per group: offsets, counts and maximal half-widths of the candidate search;
+    // groups 0..nbins-1 are the width bins, group nbins the He I lines
+    int64_t o = 0;
+    packed.assign(3 * (nbins + 1), 0);
+    for (int b = 0; b <= nbins; b++) {
+      int key = b < nbins ? b : HE_KEY;
+      packed[b] = (int)o;
+      packed[(nbins + 1) + b] = hcnt[key];
+      packed[2 * (nbins + 1) + b] = hhmax[key];
+      o += hcnt[key];
     }
-    packed.assign(3 * nbins, 0);
-    for (int b = 0; b < nbins; b++) {
-      packed[b] = hoff[b];
-      packed[nbins + b] = hoff[MAX_BINS + b];
-      packed[2 * nbins + b] = hoff[2 * MAX_BINS + b];
-    }
-    CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * 3 * nbins, hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * packed.size(), hipMemcpyHostToDevice, s));
     const size_t csz = c->precision == FSG_FP32 ? sizeof(Coef32) : sizeof(Coef64);
-    CK(c->coef.ensure((size_t)nret * N * csz));
+    CK(c->coef.ensure((size_t)nret * S.ntp * csz));
     CK(c->wlc.ensure((size_t)nret * sizeof(double)));
     CK(c->meta.ensure((size_t)nret * sizeof(int4)));
     if (c->precision == FSG_FP32)
@@ -428,16 +431,28 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       launch_fill64(S, dl->L, dl->list_type, nret, c->idx2.as<int>(), c->flag.as<int>(), c->il.as<int>(),
                     c->ir.as<int>(), c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
     const int ntiles = (S.nwl + TILE - 1) / TILE;
-    CK(c->rng.ensure((size_t)ntiles * nbins * sizeof(int2)));
-    launch_tile_ranges(ntiles, nbins, S.nwl, c->binoff, c->binoff + nbins, c->binoff + 2 * nbins,
-                       c->meta.as<int4>(), c->rng.as<int2>(), s);
+    const int ng = nbins + 1;
+    CK(c->rng.ensure((size_t)ntiles * ng * sizeof(int2)));
+    launch_tile_ranges(ntiles, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng, c->meta.as<int4>(),
+                       c->rng.as<int2>(), s);
     CK(hipEventRecord(c->ev[2], s));
-    if (c->precision == FSG_FP32)
-      launch_opacity32(S, ntiles, nbins, c->rng.as<int2>(), c->coef.as<Coef32>(), c->wlc.as<double>(),
-                       c->meta.as<int4>(), (float *)c->lineop, s);
-    else
-      launch_opacity64(S, ntiles, nbins, c->rng.as<int2>(), c->coef.as<Coef64>(), c->wlc.as<double>(),
-                       c->meta.as<int4>(), (double *)c->lineop, s);
+    // the range table has ng groups per tile: the first nbins are read by the main launch
+    // (stride ng), the last one by the He launch (offset nbins, stride ng)
+    if (c->precision == FSG_FP32) {
+      if (nmain > 0)
+        launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), c->coef.as<Coef32>(), c->wlc.as<double>(),
+                         c->meta.as<int4>(), (float *)c->lineop, 0, s);
+      if (nhe > 0)
+        launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, c->coef.as<Coef32>(), c->wlc.as<double>(),
+                         c->meta.as<int4>(), (float *)c->lineop, 1, s);
+    } else {
+      if (nmain > 0)
+        launch_opacity64(S, ntiles, nbins, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), c->wlc.as<double>(),
+                         c->meta.as<int4>(), (double *)c->lineop, s);
+      if (nhe > 0)
+        launch_opacity64(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, c->coef.as<Coef64>(), c->wlc.as<double>(),
+                         c->meta.as<int4>(), (double *)c->lineop, s);
+    }
     CK(hipEventRecord(c->ev[3], s));
   } else {
     CK(hipEventRecord(c->ev[2], s));

@@ -36,11 +36,13 @@ constexpr double kPlkC1 = kHc * kVc * 1e13 / kKbolt;
 constexpr int NVV = 1001, NVA = 401;
 constexpr int TAB_LD = NVV + 1;  // 1002 entries per a column
 
-// Depths per wavefront in the opacity kernel; a block holds ceil(ntau/DPW) wavefronts.
+// Depths per wavefront in the opacity kernel; a block holds ceil(ntau/DPW) wavefronts and
+// per-line coefficient records are padded to ntp = DPW * ceil(ntau/DPW) depths.
 constexpr int DPW = 18;
 constexpr int TILE = 64;        // wavelength points per tile (one per lane)
 constexpr int MAX_BINS = 40;
 constexpr int WEAK_KEY = 63;    // sort key of rejected lines
+constexpr int HE_KEY = 62;      // sort key of retained lines with a tabulated He I Stark shift
 
 // Per-(line, depth) coefficients of the batched FP32 path, one 32-byte record:
 //   vv = 1/Doppler width, a, a^2, a^2 + 1/4, K = LinCop a / sqrt(pi), LinCop,
@@ -58,7 +60,7 @@ struct alignas(64) Coef64 {
 
 // Everything the kernels read from the set-up (device pointers + scalars).
 struct DevSetup {
-  int ntau, nwl, nmu, wmode, motype, pf_ld, nslot, nspc, he1_slot, nhe, molbrd;
+  int ntau, ntp, nwl, nmu, wmode, motype, pf_ld, nslot, nspc, he1_slot, nhe, molbrd;
   double wfirst, dstep, rstep, wstart;
   double eps_opac, vwfudge, h2brd, vwmol, vmicro, he_wl_tol, he_elo_tol;
   const double *wave;        // [nwl]
@@ -68,7 +70,8 @@ struct DevSetup {
   const float *fpf;          // [ntau][pf_ld]
   const float *pot_ion, *xamass;
   const int *spc_ion, *spc_slot;
-  const double *he_wl, *he_elo, *he_gam, *he_shf;  // [nhe], [nhe][ntau]
+  const double *he_wl, *he_elo, *he_gam;  // [nhe], [nhe][ntau]
+  const double *he_shf;      // [nhe][ntp]: HeShf padded with zeros to ntp depths
   const double *xln;         // [6][ntau] XlineInit depth constants
   double vturb2;
   double xlnz;               // sqrt(pi) e^2/(m c^2) 1e-18 (src/opacity.f90:36)

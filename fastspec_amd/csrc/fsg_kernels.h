@@ -28,9 +28,9 @@ void launch_fill64(const DevSetup &S, const DevLines &L, int list_type, int64_t 
                    int4 *meta_s, hipStream_t st);
 void launch_tile_ranges(int ntiles, int nbins, int nwl, const int *bin_off, const int *bin_cnt, const int *bin_hmax,
                         const int4 *meta_s, int2 *rng, hipStream_t st);
-void launch_opacity32(const DevSetup &S, int ntiles, int nbins, const int2 *rng, const Coef32 *coef,
-                      const double *wlc_s, const int4 *meta_s, float *lineop, hipStream_t st);
-void launch_opacity64(const DevSetup &S, int ntiles, int nbins, const int2 *rng, const Coef64 *coef,
+void launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef32 *coef,
+                      const double *wlc_s, const int4 *meta_s, float *lineop, int he, hipStream_t st);
+void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
                       const double *wlc_s, const int4 *meta_s, double *lineop, hipStream_t st);
 void launch_radtran32(const DevSetup &S, const float *lop, double *spl, double *spc, double *sfl, double *sfc,
                       hipStream_t st);

@@ -323,7 +323,7 @@ __global__ void k_pass1(DevSetup S, DevLines L, int list_type, int *flag_out, in
     maxb = fmax(maxb, rb);
     // LineOpacity2 at the line centre: dWl0 = 0, shifted by HeShf for He I lines
     double dWl = 0.0;
-    if (ihe > 0) dWl = 0.0 - S.he_shf[(size_t)(ihe - 1) * N + k];
+    if (ihe > 0) dWl = 0.0 - S.he_shf[(size_t)(ihe - 1) * S.ntp + k];
     double lo = hvoigt64_full(S, av, dWl * vv) * cop;
     if (S.motype == 1) lo = lo / cs;
     maxc = fmax(maxc, lo / kc);
@@ -332,7 +332,7 @@ __global__ void k_pass1(DevSetup S, DevLines L, int list_type, int *flag_out, in
     if (S.motype == 1) r = r / cs;
     if (r > 1.0) {
       double w = sqrt(log(r) + fmax(0.0, av * (r * iSqrtPi - av))) / vv;
-      if (ihe > 0) w = w + fabs(S.he_shf[(size_t)(ihe - 1) * N + k]);
+      if (ihe > 0) w = w + fabs(S.he_shf[(size_t)(ihe - 1) * S.ntp + k]);
       win = fmax(win, w);
     }
   }
@@ -354,7 +354,8 @@ __global__ void k_pass1(DevSetup S, DevLines L, int list_type, int *flag_out, in
     key = WEAK_KEY;
   } else {
     int h = max(iwl - il, ir - iwl);
-    key = batched ? (h == 0 ? 0u : (unsigned)(32 - __clz(h))) : 0u;
+    if (ihe > 0) key = HE_KEY;
+    else key = batched ? (h == 0 ? 0u : (unsigned)(32 - __clz(h))) : 0u;
     atomicMax(&bin_hmax[key], h);
   }
   atomicAdd(&bin_count[key], 1);
@@ -438,7 +439,17 @@ __global__ void k_fill(DevSetup S, DevLines L, int list_type, int64_t nret, cons
   XC xc = xline_setup(S, L, i, wlc, f & 0x6, list_type);
   wlc_s[j] = wlc;
   meta_s[j] = make_int4(il[i], ir[i], ic[i], (f >> 8) & 0xff);
-  CoefT *c = coef + (size_t)j * N;
+  CoefT *c = coef + (size_t)j * S.ntp;
+  for (int k = N; k < S.ntp; k++) {  // padding depths contribute exactly zero
+    CoefT r{};
+    if constexpr (sizeof(CoefT) == sizeof(Coef32)) {
+      r.a2 = 1e4f;
+      r.c1 = 1e4f + 0.25f;
+    } else {
+      r.ia = 1;
+    }
+    c[k] = r;
+  }
   for (int k = 0; k < N; k++) {
     double cop, vv, av, ua;
     int ia;
@@ -543,70 +554,130 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
   return x * per + r;
 }
 
-template <typename CoefT, bool KAHAN>
-__global__ void __launch_bounds__(512) k_opacity(DevSetup S, int ntiles, int nbins, const int2 *rng,
-                                                  const CoefT *__restrict__ coef,
-                                                  const double *__restrict__ wlc_s,
-                                                  const int4 *__restrict__ meta_s, void *lineop_v) {
-  using Real = typename std::conditional<sizeof(CoefT) == sizeof(Coef32), float, double>::type;
+typedef float f8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ Coef32 as_coef(const f8v &x) {
+  Coef32 c;
+  c.vv = x[0];
+  c.a = x[1];
+  c.a2 = x[2];
+  c.c1 = x[3];
+  c.ka = x[4];
+  c.cop = x[5];
+  c.ua = x[6];
+  c.ia = __float_as_int(x[7]);
+  return c;
+}
+
+__device__ __forceinline__ void kahan_add(float &acc, float &cmp, float val) {
+  float y = val - cmp;
+  float t = acc + y;
+  cmp = (t - acc) - y;
+  acc = t;
+}
+
+// FP32 batched path.  Per (line, depth) the 32-byte coefficient record is read with one
+// scalar load (wave-uniform address) issued one depth ahead of its use.  Lines of the
+// tabulated He I Stark table (depth-dependent centre shift) come in a separate launch
+// (HE = true) so that the common path carries no per-depth shift.
+template <bool HE>
+__global__ void __launch_bounds__(512) k_opacity32(DevSetup S, int ntiles, int nbins, int rstride, const int2 *__restrict__ rng,
+                                                   const Coef32 *__restrict__ coef, const double *__restrict__ wlc_s,
+                                                   const int4 *__restrict__ meta_s, float *__restrict__ lop) {
   int t = xcd_tile(blockIdx.x, gridDim.x);
-  // blocks beyond the tile count (from rounding the XCD split) have nothing to do
-  if (t < 0 || t >= ntiles) return;
+  if (t >= ntiles) return;
   const int N = S.ntau;
-  int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
-  int p = p0 + lane;
-  bool valid = p <= S.nwl;
-  double Wp = valid ? S.wave[p - 1] : 0.0;
-  int d0 = wave * DPW;
-  int nd = min(DPW, N - d0);
-  Real acc[DPW], cmp[DPW];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
+  const int p = p0 + lane;
+  const bool valid = p <= S.nwl;
+  const double Wp = valid ? S.wave[p - 1] : 0.0;
+  const int d0 = wave * DPW;
+  const int nd = min(DPW, N - d0);
+  float acc[DPW], cmp[DPW];
 #pragma unroll
   for (int k = 0; k < DPW; k++) {
-    acc[k] = 0;
-    cmp[k] = 0;
+    acc[k] = 0.f;
+    cmp[k] = 0.f;
   }
   for (int b = 0; b < nbins; b++) {
-    int2 r = rng[(size_t)t * nbins + b];
+    const int2 r = rng[(size_t)t * rstride + b];
     for (int j = r.x; j < r.y; j++) {
-      int4 m = meta_s[j];
+      const int4 m = meta_s[j];
       if (m.y < p0 || m.x > p1) continue;  // window misses the tile (wave-uniform)
-      bool act = valid && p >= m.x && p <= m.y;
-      if (!act) continue;
-      double dl = Wp - wlc_s[j];
-      const CoefT *c = coef + (size_t)j * N + d0;
-      const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * N + d0 : nullptr;
+      if (!(valid && p >= m.x && p <= m.y)) continue;
+      const double dl = Wp - wlc_s[j];
+      const f8v *cp = reinterpret_cast<const f8v *>(coef + (size_t)j * S.ntp + d0);
+      const float dlf = (float)dl;
+      const double *hs = HE ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
+      f8v nx = cp[0];
 #pragma unroll
       for (int k = 0; k < DPW; k++) {
-        if (k < nd) {
-          double dld = hs ? dl - hs[k] : dl;
-          Real val;
-          if constexpr (sizeof(CoefT) == sizeof(Coef32)) {
-            val = contrib32(S, c[k], (float)dld * c[k].vv);
-          } else {
-            val = contrib64(S, c[k], dld * c[k].vv);
-          }
-          if (KAHAN) {
-            Real y = val - cmp[k];
-            Real tt = acc[k] + y;
-            cmp[k] = (tt - acc[k]) - y;
-            acc[k] = tt;
-          } else {
-            acc[k] += val;
-          }
-        }
+        const f8v cu = nx;
+        if (k + 1 < DPW) nx = cp[k + 1];
+        const Coef32 c = as_coef(cu);
+        const float x = HE ? (float)(dl - hs[k]) : dlf;
+        kahan_add(acc[k], cmp[k], contrib32(S, c, x * c.vv));
+        // keep the next depth's load from being hoisted further (bounds live SGPRs)
+        __asm__ volatile("" ::: "memory");
       }
     }
   }
   if (!valid) return;
-  Real *lop = (Real *)lineop_v;
 #pragma unroll
   for (int k = 0; k < DPW; k++) {
     if (k < nd) {
-      int d = d0 + k;
-      Real v = KAHAN ? (acc[k] - cmp[k]) : acc[k];
-      if (S.motype == 1) v = v / (Real)S.cstd[d];
-      size_t o = (size_t)d * S.nwl + (p - 1);
+      const int d = d0 + k;
+      float v = acc[k] - cmp[k];
+      if (S.motype == 1) v = v / (float)S.cstd[d];
+      const size_t o = (size_t)d * S.nwl + (p - 1);
+      lop[o] = lop[o] + v;
+    }
+  }
+}
+
+// FP64 plain path: the reference's arithmetic (table lookup in the box, continued
+// fractions outside, plain FP64 sums in line order within a bin).
+__global__ void __launch_bounds__(512) k_opacity64(DevSetup S, int ntiles, int nbins, int rstride, const int2 *__restrict__ rng,
+                                                   const Coef64 *__restrict__ coef, const double *__restrict__ wlc_s,
+                                                   const int4 *__restrict__ meta_s, double *__restrict__ lop) {
+  int t = xcd_tile(blockIdx.x, gridDim.x);
+  if (t >= ntiles) return;
+  const int N = S.ntau;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
+  const int p = p0 + lane;
+  const bool valid = p <= S.nwl;
+  const double Wp = valid ? S.wave[p - 1] : 0.0;
+  const int d0 = wave * DPW;
+  const int nd = min(DPW, N - d0);
+  double acc[DPW];
+#pragma unroll
+  for (int k = 0; k < DPW; k++) acc[k] = 0.0;
+  for (int b = 0; b < nbins; b++) {
+    const int2 r = rng[(size_t)t * rstride + b];
+    for (int j = r.x; j < r.y; j++) {
+      const int4 m = meta_s[j];
+      if (m.y < p0 || m.x > p1) continue;
+      if (!(valid && p >= m.x && p <= m.y)) continue;
+      const double dl = Wp - wlc_s[j];
+      const Coef64 *c = coef + (size_t)j * S.ntp + d0;
+      const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
+#pragma unroll
+      for (int k = 0; k < DPW; k++) {
+        const double dld = hs ? dl - hs[k] : dl;
+        acc[k] += contrib64(S, c[k], dld * c[k].vv);
+      }
+    }
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int k = 0; k < DPW; k++) {
+    if (k < nd) {
+      const int d = d0 + k;
+      double v = acc[k];
+      if (S.motype == 1) v = v / S.cstd[d];
+      const size_t o = (size_t)d * S.nwl + (p - 1);
       lop[o] = lop[o] + v;
     }
   }
@@ -796,18 +867,21 @@ void launch_tile_ranges(int ntiles, int nbins, int nwl, const int *bin_off, cons
   k_tile_ranges<<<grid, 256, 0, st>>>(ntiles, nbins, nwl, bin_off, bin_cnt, bin_hmax, meta_s, rng);
 }
 
-void launch_opacity32(const DevSetup &S, int ntiles, int nbins, const int2 *rng, const Coef32 *coef,
-                      const double *wlc_s, const int4 *meta_s, float *lineop, hipStream_t st) {
+void launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef32 *coef,
+                      const double *wlc_s, const int4 *meta_s, float *lineop, int he, hipStream_t st) {
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
-  k_opacity<Coef32, true><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rng, coef, wlc_s, meta_s, lineop);
+  if (he)
+    k_opacity32<true><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, lineop);
+  else
+    k_opacity32<false><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, lineop);
 }
 
-void launch_opacity64(const DevSetup &S, int ntiles, int nbins, const int2 *rng, const Coef64 *coef,
+void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
                       const double *wlc_s, const int4 *meta_s, double *lineop, hipStream_t st) {
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
-  k_opacity<Coef64, false><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rng, coef, wlc_s, meta_s, lineop);
+  k_opacity64<<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, lineop);
 }
 
 void launch_radtran32(const DevSetup &S, const float *lop, double *spl, double *spc, double *sfl, double *sfc,
@@ -824,10 +898,6 @@ template __global__ void k_fill<Coef32>(DevSetup, DevLines, int, int64_t, const 
                                         const int *, const int *, Coef32 *, double *, int4 *);
 template __global__ void k_fill<Coef64>(DevSetup, DevLines, int, int64_t, const int *, const int *, const int *,
                                         const int *, const int *, Coef64 *, double *, int4 *);
-template __global__ void k_opacity<Coef32, true>(DevSetup, int, int, const int2 *, const Coef32 *, const double *,
-                                                 const int4 *, void *);
-template __global__ void k_opacity<Coef64, false>(DevSetup, int, int, const int2 *, const Coef64 *,
-                                                  const double *, const int4 *, void *);
 template __global__ void k_radtran<float>(DevSetup, const float *, double *, double *, double *, double *);
 template __global__ void k_radtran<double>(DevSetup, const double *, double *, double *, double *, double *);
 
