@@ -121,7 +121,7 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
   g_err.clear();
   if (!s || !ctx_out) return fail("fsg_init: NULL argument");
   *ctx_out = nullptr;
-  if (s->ntau < 3 || s->ntau > DPW * 8) return fail("fsg_init: Ntau must be in 3..144");
+  if (s->ntau < 3 || s->ntau > DPW * 16) return fail("fsg_init: Ntau must be in 3..144");
   if (s->nwl < 2) return fail("Init: the wavelength grid has fewer than two points");
   if (s->nmu < 1 || s->nmu > 16) return fail("fsg_init: Nmu must be in 1..16");
   if (s->precision != FSG_FP64 && s->precision != FSG_FP32) return fail("fsg_init: unknown precision");
@@ -420,13 +420,15 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       o += hcnt[key];
     }
     CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * packed.size(), hipMemcpyHostToDevice, s));
-    const size_t csz = c->precision == FSG_FP32 ? sizeof(Coef32) : sizeof(Coef64);
+    const size_t csz = c->precision == FSG_FP32 ? sizeof(CoefA) + sizeof(CoefB) : sizeof(Coef64);
     CK(c->coef.ensure((size_t)nret * S.ntp * csz));
+    CoefA *ca = c->coef.as<CoefA>();
+    CoefB *cb = reinterpret_cast<CoefB *>(ca + (size_t)nret * S.ntp);
     CK(c->wlc.ensure((size_t)nret * sizeof(double)));
     CK(c->meta.ensure((size_t)nret * sizeof(int4)));
     if (c->precision == FSG_FP32)
       launch_fill32(S, dl->L, dl->list_type, nret, c->idx2.as<int>(), c->flag.as<int>(), c->il.as<int>(),
-                    c->ir.as<int>(), c->ic.as<int>(), c->coef.as<Coef32>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
+                    c->ir.as<int>(), c->ic.as<int>(), ca, cb, c->wlc.as<double>(), c->meta.as<int4>(), s);
     else
       launch_fill64(S, dl->L, dl->list_type, nret, c->idx2.as<int>(), c->flag.as<int>(), c->il.as<int>(),
                     c->ir.as<int>(), c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
@@ -440,10 +442,10 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
     // (stride ng), the last one by the He launch (offset nbins, stride ng)
     if (c->precision == FSG_FP32) {
       if (nmain > 0)
-        launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), c->coef.as<Coef32>(), c->wlc.as<double>(),
+        launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, c->wlc.as<double>(),
                          c->meta.as<int4>(), (float *)c->lineop, 0, s);
       if (nhe > 0)
-        launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, c->coef.as<Coef32>(), c->wlc.as<double>(),
+        launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, c->wlc.as<double>(),
                          c->meta.as<int4>(), (float *)c->lineop, 1, s);
     } else {
       if (nmain > 0)

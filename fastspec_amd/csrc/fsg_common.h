@@ -36,18 +36,34 @@ constexpr double kPlkC1 = kHc * kVc * 1e13 / kKbolt;
 constexpr int NVV = 1001, NVA = 401;
 constexpr int TAB_LD = NVV + 1;  // 1002 entries per a column
 
-// Depths per wavefront in the opacity kernel; a block holds ceil(ntau/DPW) wavefronts and
+// Depths per wavefront in the opacity kernels; a block holds ceil(ntau/DPW) wavefronts and
 // per-line coefficient records are padded to ntp = DPW * ceil(ntau/DPW) depths.
-constexpr int DPW = 18;
-constexpr int TILE = 64;        // wavelength points per tile (one per lane)
+constexpr int DPW = 9;
+constexpr int PPL = 2;          // wavelength points per lane
+constexpr int TILE = 64 * PPL;  // wavelength points per tile
 constexpr int MAX_BINS = 40;
 constexpr int WEAK_KEY = 63;    // sort key of rejected lines
 constexpr int HE_KEY = 62;      // sort key of retained lines with a tabulated He I Stark shift
 
-// Per-(line, depth) coefficients of the batched FP32 path, one 32-byte record:
-//   vv = 1/Doppler width, a, a^2, a^2 + 1/4, K = LinCop a / sqrt(pi), LinCop,
-//   ua (a-interpolation weight), ia (a-cell of the table, 0 = outside the table box)
-struct alignas(32) Coef32 {
+// Per-(line, depth) coefficients of the batched FP32 path, two 32-byte records kept in
+// separate arrays (CoefA for every evaluation, CoefB only on the general path).
+//   Far-wing expansion (used when every lane of the wavefront has v^2 >= 400 (a^2 + 1/2)):
+//     with u = 1/v^2 the reference's one-pole fraction a (s + 1/2)/(s^2 - v^2 + a^2 + 1/4)/sqrt(pi)
+//     equals K u (1 + b u)/(1 + al u + b^2 u^2), K = LinCop a/sqrt(pi), al = 2a^2 - 1,
+//     b = a^2 + 1/2; it is expanded to K u (f0 + f1 u + f2 u^2 + f3 u^3), f0 = 1.
+//     w = 1/vv^2 (u = w / dl^2), k0..k3 = K f0..f3, dthr = the |dl| above which the
+//     expansion applies.
+//   General path: vv = 1/Doppler width, a, a^2, a^2 + 1/4, K, LinCop, ua (a-interpolation
+//   weight), ia (a-cell of the table, 0 = outside the table box).
+struct alignas(32) CoefA {
+  float w, k0, k1, k2, k3, dthr, vv, a;
+};
+struct alignas(32) CoefB {
+  float a2, c1, ka, cop, ua;
+  int ia;
+  float pad0, pad1;
+};
+struct Coef32 {  // the general path's view (assembled from both records)
   float vv, a, a2, c1, ka, cop, ua;
   int ia;
 };

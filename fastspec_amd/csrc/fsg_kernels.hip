@@ -426,28 +426,81 @@ __global__ void k_retained_flag(int64_t n, const int *flag, int *out) {
 // ------------------------------------------------------------------------------------
 // coefficient fill: one lane per retained line in bin-sorted order
 // ------------------------------------------------------------------------------------
-template <typename CoefT>
-__global__ void k_fill(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order,
-                       const int *flag, const int *il, const int *ir, const int *ic, CoefT *coef,
-                       double *wlc_s, int4 *meta_s) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nret) return;
+__device__ __forceinline__ void line_meta(const DevSetup &S, const DevLines &L, int list_type, int64_t j,
+                                          const int *order, const int *flag, const int *il, const int *ir,
+                                          const int *ic, XC &xc, double *wlc_s, int4 *meta_s) {
   int64_t i = order[j];
-  const int N = S.ntau;
   int f = flag[i];
   double wlc = L.wl[i] - S.wfirst;
-  XC xc = xline_setup(S, L, i, wlc, f & 0x6, list_type);
+  xc = xline_setup(S, L, i, wlc, f & 0x6, list_type);
   wlc_s[j] = wlc;
   meta_s[j] = make_int4(il[i], ir[i], ic[i], (f >> 8) & 0xff);
-  CoefT *c = coef + (size_t)j * S.ntp;
+}
+
+// FP32 batched path: far-wing expansion records (CoefA) and general-path records (CoefB)
+__global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+                         const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, double *wlc_s,
+                         int4 *meta_s) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nret) return;
+  XC xc;
+  line_meta(S, L, list_type, j, order, flag, il, ir, ic, xc, wlc_s, meta_s);
+  const int N = S.ntau;
+  CoefA *A = ca + (size_t)j * S.ntp;
+  CoefB *B = cb + (size_t)j * S.ntp;
+  for (int k = N; k < S.ntp; k++) {  // padding depths contribute exactly zero on either path
+    CoefA a{};
+    CoefB b{};
+    b.a2 = 1e4f;
+    b.c1 = 1e4f + 0.25f;
+    A[k] = a;
+    B[k] = b;
+  }
+  for (int k = 0; k < N; k++) {
+    double cop, vv, av, ua;
+    int ia;
+    xline_depth(S, k, xc, cop, vv, av);
+    voigt_astate(S, av, ia, ua);
+    const double a2 = av * av, K = cop * av * kSqrtPiInv;
+    const double al = 2.0 * a2 - 1.0, be = a2 + 0.5;
+    // series of (1 + be u)/(1 + al u + be^2 u^2): f_n = -al f_{n-1} - be^2 f_{n-2}
+    const double f1 = be - al;
+    const double f2 = -al * f1 - be * be;
+    const double f3 = -al * f2 - be * be * f1;
+    CoefA a;
+    a.w = (float)(1.0 / (vv * vv));
+    a.k0 = (float)K;
+    a.k1 = (float)(K * f1);
+    a.k2 = (float)(K * f2);
+    a.k3 = (float)(K * f3);
+    a.dthr = (float)(sqrt(fmax(196.0, 400.0 * be)) / vv * (1.0 + 1e-4));
+    a.vv = (float)vv;
+    a.a = (float)av;
+    CoefB b;
+    b.a2 = (float)a2;
+    b.c1 = (float)(a2 + 0.25);
+    b.ka = (float)K;
+    b.cop = (float)cop;
+    b.ua = (float)ua;
+    b.ia = ia;
+    b.pad0 = b.pad1 = 0.f;
+    A[k] = a;
+    B[k] = b;
+  }
+}
+
+// FP64 plain path: the reference's LinVvgt, LinAvgt, LinCop and the table a-state
+__global__ void k_fill64(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+                         const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s, int4 *meta_s) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nret) return;
+  XC xc;
+  line_meta(S, L, list_type, j, order, flag, il, ir, ic, xc, wlc_s, meta_s);
+  const int N = S.ntau;
+  Coef64 *c = coef + (size_t)j * S.ntp;
   for (int k = N; k < S.ntp; k++) {  // padding depths contribute exactly zero
-    CoefT r{};
-    if constexpr (sizeof(CoefT) == sizeof(Coef32)) {
-      r.a2 = 1e4f;
-      r.c1 = 1e4f + 0.25f;
-    } else {
-      r.ia = 1;
-    }
+    Coef64 r{};
+    r.ia = 1;
     c[k] = r;
   }
   for (int k = 0; k < N; k++) {
@@ -455,25 +508,12 @@ __global__ void k_fill(DevSetup S, DevLines L, int list_type, int64_t nret, cons
     int ia;
     xline_depth(S, k, xc, cop, vv, av);
     voigt_astate(S, av, ia, ua);
-    CoefT r;
-    if constexpr (sizeof(CoefT) == sizeof(Coef32)) {
-      r.vv = (float)vv;
-      r.a = (float)av;
-      r.a2 = (float)(av * av);
-      r.c1 = (float)(av * av + 0.25);
-      r.ka = (float)(cop * av * kSqrtPiInv);
-      r.cop = (float)cop;
-      r.ua = (float)ua;
-      r.ia = ia;
-    } else {
-      r.vv = vv;
-      r.a = av;
-      r.cop = cop;
-      r.ua = ua;
-      r.ia = ia;
-      r.pad0 = 0;
-      r.pad1 = r.pad2 = r.pad3 = 0.0;
-    }
+    Coef64 r{};
+    r.vv = vv;
+    r.a = av;
+    r.cop = cop;
+    r.ua = ua;
+    r.ia = ia;
     c[k] = r;
   }
 }
@@ -556,19 +596,6 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
 
 typedef float f8v __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ Coef32 as_coef(const f8v &x) {
-  Coef32 c;
-  c.vv = x[0];
-  c.a = x[1];
-  c.a2 = x[2];
-  c.c1 = x[3];
-  c.ka = x[4];
-  c.cop = x[5];
-  c.ua = x[6];
-  c.ia = __float_as_int(x[7]);
-  return c;
-}
-
 __device__ __forceinline__ void kahan_add(float &acc, float &cmp, float val) {
   float y = val - cmp;
   float t = acc + y;
@@ -576,109 +603,183 @@ __device__ __forceinline__ void kahan_add(float &acc, float &cmp, float val) {
   acc = t;
 }
 
-// FP32 batched path.  Per (line, depth) the 32-byte coefficient record is read with one
-// scalar load (wave-uniform address) issued one depth ahead of its use.  Lines of the
-// tabulated He I Stark table (depth-dependent centre shift) come in a separate launch
-// (HE = true) so that the common path carries no per-depth shift.
+// FP32 batched path.  A block owns a tile of TILE = 64 x PPL wavelength points (PPL per
+// lane, 64 apart) and all depths, one wavefront per DPW depths.  Per (line, depth) the
+// 32-byte CoefA record is read with one scalar load (wave-uniform address), issued while
+// the previous depth is computed.  Lines of the tabulated He I Stark table come in a
+// separate launch (HE = true) so that the common path carries no per-depth shift.
 template <bool HE>
-__global__ void __launch_bounds__(512) k_opacity32(DevSetup S, int ntiles, int nbins, int rstride, const int2 *__restrict__ rng,
-                                                   const Coef32 *__restrict__ coef, const double *__restrict__ wlc_s,
-                                                   const int4 *__restrict__ meta_s, float *__restrict__ lop) {
-  int t = xcd_tile(blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int nbins, int rstride,
+                                                    const int2 *__restrict__ rng, const CoefA *__restrict__ ca,
+                                                    const CoefB *__restrict__ cb, const double *__restrict__ wlc_s,
+                                                    const int4 *__restrict__ meta_s, float *__restrict__ lop) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
   const int N = S.ntau;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
-  const int p = p0 + lane;
-  const bool valid = p <= S.nwl;
-  const double Wp = valid ? S.wave[p - 1] : 0.0;
-  const int d0 = wave * DPW;
-  const int nd = min(DPW, N - d0);
-  float acc[DPW], cmp[DPW];
+  int pp[PPL];
+  bool valid[PPL];
+  double Wp[PPL];
 #pragma unroll
-  for (int k = 0; k < DPW; k++) {
-    acc[k] = 0.f;
-    cmp[k] = 0.f;
+  for (int q = 0; q < PPL; q++) {
+    pp[q] = p0 + lane + 64 * q;
+    valid[q] = pp[q] <= S.nwl;
+    Wp[q] = valid[q] ? S.wave[pp[q] - 1] : 0.0;
   }
+  const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];  // tile edges
+  const int d0 = wave * DPW;
+  float acc[PPL][DPW], cmp[PPL][DPW];
+#pragma unroll
+  for (int q = 0; q < PPL; q++)
+#pragma unroll
+    for (int k = 0; k < DPW; k++) {
+      acc[q][k] = 0.f;
+      cmp[q][k] = 0.f;
+    }
   for (int b = 0; b < nbins; b++) {
     const int2 r = rng[(size_t)t * rstride + b];
     for (int j = r.x; j < r.y; j++) {
       const int4 m = meta_s[j];
       if (m.y < p0 || m.x > p1) continue;  // window misses the tile (wave-uniform)
-      if (!(valid && p >= m.x && p <= m.y)) continue;
-      const double dl = Wp - wlc_s[j];
-      const f8v *cp = reinterpret_cast<const f8v *>(coef + (size_t)j * S.ntp + d0);
-      const float dlf = (float)dl;
+      bool act[PPL];
+      bool any = false;
+#pragma unroll
+      for (int q = 0; q < PPL; q++) {
+        act[q] = valid[q] && pp[q] >= m.x && pp[q] <= m.y;
+        any = any || act[q];
+      }
+      if (!any) continue;
+      const double wc = wlc_s[j];
+      float dlf[PPL], U[PPL];
+      double dl[PPL];
+#pragma unroll
+      for (int q = 0; q < PPL; q++) {
+        dl[q] = Wp[q] - wc;
+        dlf[q] = (float)dl[q];
+        // U = 1/dl^2 on points inside the window, 0 outside (the expansion then adds 0)
+        U[q] = act[q] ? 1.0f / (dlf[q] * dlf[q]) : 0.0f;
+      }
+      const f8v *cp = reinterpret_cast<const f8v *>(ca + (size_t)j * S.ntp + d0);
+      const CoefB *cbp = cb + (size_t)j * S.ntp + d0;
       const double *hs = HE ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
-      f8v nx = cp[0];
+      // smallest |lambda - lambda_c| over the tile (0 if the centre lies inside it); the
+      // far-wing expansion applies at depth k for the whole wavefront when it exceeds dthr
+      const double dmin = (wc < Wt0) ? (Wt0 - wc) : (wc > Wt1 ? (wc - Wt1) : 0.0);
+      const unsigned dmb = HE ? 0u : __builtin_amdgcn_readfirstlane(__float_as_uint((float)dmin));
+      f8v cu = cp[0];
 #pragma unroll
       for (int k = 0; k < DPW; k++) {
-        const f8v cu = nx;
+        // Scalar loads complete out of order, so a wait covers every load in flight: take
+        // depth k's record (one wait, only it outstanding) before issuing depth k+1's load,
+        // whose latency then overlaps depth k's arithmetic.
+        __asm__ volatile("" ::"s"(cu[5]) : "memory");
+        f8v nx = cu;
         if (k + 1 < DPW) nx = cp[k + 1];
-        const Coef32 c = as_coef(cu);
-        const float x = HE ? (float)(dl - hs[k]) : dlf;
-        kahan_add(acc[k], cmp[k], contrib32(S, c, x * c.vv));
-        // keep the next depth's load from being hoisted further (bounds live SGPRs)
-        __asm__ volatile("" ::: "memory");
+        if (!HE && dmb >= __float_as_uint(cu[5])) {
+#pragma unroll
+          for (int q = 0; q < PPL; q++) {
+            const float u = U[q] * cu[0];
+            const float val = u * fmaf(fmaf(fmaf(cu[4], u, cu[3]), u, cu[2]), u, cu[1]);
+            kahan_add(acc[q][k], cmp[q][k], val);
+          }
+        } else {
+          const CoefB B = cbp[k];
+          Coef32 c;
+          c.vv = cu[6];
+          c.a = cu[7];
+          c.a2 = B.a2;
+          c.c1 = B.c1;
+          c.ka = B.ka;
+          c.cop = B.cop;
+          c.ua = B.ua;
+          c.ia = B.ia;
+#pragma unroll
+          for (int q = 0; q < PPL; q++) {
+            if (act[q]) {
+              const float x = HE ? (float)(dl[q] - hs[k]) : dlf[q];
+              kahan_add(acc[q][k], cmp[q][k], contrib32(S, c, x * c.vv));
+            }
+          }
+        }
+        cu = nx;
       }
     }
   }
-  if (!valid) return;
 #pragma unroll
-  for (int k = 0; k < DPW; k++) {
-    if (k < nd) {
+  for (int q = 0; q < PPL; q++) {
+    if (!valid[q]) continue;
+#pragma unroll
+    for (int k = 0; k < DPW; k++) {
       const int d = d0 + k;
-      float v = acc[k] - cmp[k];
-      if (S.motype == 1) v = v / (float)S.cstd[d];
-      const size_t o = (size_t)d * S.nwl + (p - 1);
-      lop[o] = lop[o] + v;
+      if (d < N) {
+        float v = acc[q][k] - cmp[q][k];
+        if (S.motype == 1) v = v / (float)S.cstd[d];
+        const size_t o = (size_t)d * S.nwl + (pp[q] - 1);
+        lop[o] = lop[o] + v;
+      }
     }
   }
 }
 
 // FP64 plain path: the reference's arithmetic (table lookup in the box, continued
 // fractions outside, plain FP64 sums in line order within a bin).
-__global__ void __launch_bounds__(512) k_opacity64(DevSetup S, int ntiles, int nbins, int rstride, const int2 *__restrict__ rng,
-                                                   const Coef64 *__restrict__ coef, const double *__restrict__ wlc_s,
-                                                   const int4 *__restrict__ meta_s, double *__restrict__ lop) {
-  int t = xcd_tile(blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(1024) k_opacity64(DevSetup S, int ntiles, int nbins, int rstride,
+                                                    const int2 *__restrict__ rng, const Coef64 *__restrict__ coef,
+                                                    const double *__restrict__ wlc_s,
+                                                    const int4 *__restrict__ meta_s, double *__restrict__ lop) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
   const int N = S.ntau;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
-  const int p = p0 + lane;
-  const bool valid = p <= S.nwl;
-  const double Wp = valid ? S.wave[p - 1] : 0.0;
-  const int d0 = wave * DPW;
-  const int nd = min(DPW, N - d0);
-  double acc[DPW];
+  int pp[PPL];
+  bool valid[PPL];
+  double Wp[PPL];
 #pragma unroll
-  for (int k = 0; k < DPW; k++) acc[k] = 0.0;
+  for (int q = 0; q < PPL; q++) {
+    pp[q] = p0 + lane + 64 * q;
+    valid[q] = pp[q] <= S.nwl;
+    Wp[q] = valid[q] ? S.wave[pp[q] - 1] : 0.0;
+  }
+  const int d0 = wave * DPW;
+  double acc[PPL][DPW];
+#pragma unroll
+  for (int q = 0; q < PPL; q++)
+#pragma unroll
+    for (int k = 0; k < DPW; k++) acc[q][k] = 0.0;
   for (int b = 0; b < nbins; b++) {
     const int2 r = rng[(size_t)t * rstride + b];
     for (int j = r.x; j < r.y; j++) {
       const int4 m = meta_s[j];
       if (m.y < p0 || m.x > p1) continue;
-      if (!(valid && p >= m.x && p <= m.y)) continue;
-      const double dl = Wp - wlc_s[j];
+      const double wc = wlc_s[j];
       const Coef64 *c = coef + (size_t)j * S.ntp + d0;
       const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
 #pragma unroll
-      for (int k = 0; k < DPW; k++) {
-        const double dld = hs ? dl - hs[k] : dl;
-        acc[k] += contrib64(S, c[k], dld * c[k].vv);
+      for (int q = 0; q < PPL; q++) {
+        if (!(valid[q] && pp[q] >= m.x && pp[q] <= m.y)) continue;
+        const double dl = Wp[q] - wc;
+#pragma unroll
+        for (int k = 0; k < DPW; k++) {
+          const double dld = hs ? dl - hs[k] : dl;
+          acc[q][k] += contrib64(S, c[k], dld * c[k].vv);
+        }
       }
     }
   }
-  if (!valid) return;
 #pragma unroll
-  for (int k = 0; k < DPW; k++) {
-    if (k < nd) {
+  for (int q = 0; q < PPL; q++) {
+    if (!valid[q]) continue;
+#pragma unroll
+    for (int k = 0; k < DPW; k++) {
       const int d = d0 + k;
-      double v = acc[k];
-      if (S.motype == 1) v = v / S.cstd[d];
-      const size_t o = (size_t)d * S.nwl + (p - 1);
-      lop[o] = lop[o] + v;
+      if (d < N) {
+        double v = acc[q][k];
+        if (S.motype == 1) v = v / S.cstd[d];
+        const size_t o = (size_t)d * S.nwl + (pp[q] - 1);
+        lop[o] = lop[o] + v;
+      }
     }
   }
 }
@@ -850,15 +951,15 @@ void launch_mark_wide(const DevSetup &S, int64_t n, const double *wl, const int 
 }
 
 void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
-                   const int *flag, const int *il, const int *ir, const int *ic, Coef32 *coef, double *wlc_s,
+                   const int *flag, const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, double *wlc_s,
                    int4 *meta_s, hipStream_t st) {
-  if (nret) k_fill<Coef32><<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, wlc_s, meta_s);
+  if (nret) k_fill32<<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, ca, cb, wlc_s, meta_s);
 }
 
 void launch_fill64(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                    const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
                    int4 *meta_s, hipStream_t st) {
-  if (nret) k_fill<Coef64><<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, wlc_s, meta_s);
+  if (nret) k_fill64<<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, wlc_s, meta_s);
 }
 
 void launch_tile_ranges(int ntiles, int nbins, int nwl, const int *bin_off, const int *bin_cnt, const int *bin_hmax,
@@ -867,14 +968,14 @@ void launch_tile_ranges(int ntiles, int nbins, int nwl, const int *bin_off, cons
   k_tile_ranges<<<grid, 256, 0, st>>>(ntiles, nbins, nwl, bin_off, bin_cnt, bin_hmax, meta_s, rng);
 }
 
-void launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef32 *coef,
-                      const double *wlc_s, const int4 *meta_s, float *lineop, int he, hipStream_t st) {
+void launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
+                      const CoefB *cb, const double *wlc_s, const int4 *meta_s, float *lineop, int he, hipStream_t st) {
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
   if (he)
-    k_opacity32<true><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, lineop);
+    k_opacity32<true><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, cb, wlc_s, meta_s, lineop);
   else
-    k_opacity32<false><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, lineop);
+    k_opacity32<false><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, cb, wlc_s, meta_s, lineop);
 }
 
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
@@ -894,10 +995,6 @@ void launch_radtran64(const DevSetup &S, const double *lop, double *spl, double 
   k_radtran<double><<<nblk(S.nwl, 256), 256, 0, st>>>(S, lop, spl, spc, sfl, sfc);
 }
 
-template __global__ void k_fill<Coef32>(DevSetup, DevLines, int, int64_t, const int *, const int *, const int *,
-                                        const int *, const int *, Coef32 *, double *, int4 *);
-template __global__ void k_fill<Coef64>(DevSetup, DevLines, int, int64_t, const int *, const int *, const int *,
-                                        const int *, const int *, Coef64 *, double *, int4 *);
 template __global__ void k_radtran<float>(DevSetup, const float *, double *, double *, double *, double *);
 template __global__ void k_radtran<double>(DevSetup, const double *, double *, double *, double *, double *);
 
