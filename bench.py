@@ -139,7 +139,7 @@ def main():
         "retained_lines": st["nlines_use"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_opacity<Coef32,true>", "kernel_ms": r32["opacity_ms"],
+                     "kernel": "k_opacity32<false>", "kernel_ms": r32["opacity_ms"],
                      "flop_per_eval": FLOP_PER_EVAL},
     }
     if not args.no_fp64:

@@ -478,12 +478,24 @@ int fsg_lineop(fsg_ctx *c, int32_t list_type, const fsg_lines *lines, int64_t nl
                int32_t *flags_out, int32_t *ileft_out, int32_t *iright_out, fsg_lineop_stats *stats) {
   g_err.clear();
   if (!c || !lines) return fail("fsg_lineop: NULL argument");
-  if (linstr_out) return fail("fsg_lineop: line strengths (SAVE flag 1) are not available in this build");
   fsg_dlines *dl = nullptr;
   if (fsg_lines_upload(c, list_type, lines, &dl)) return 1;
   int rc = fsg_lineop_dev(c, dl, nlines_lim, stats);
   if (rc == 0 && lines->n > 0) {
     size_t b = (size_t)lines->n * sizeof(int32_t);
+    if (linstr_out) {
+      // central line depths of the retained lines (pass 1 flags are still resident)
+      DBuf ls;
+      if (ls.ensure(lines->n * sizeof(float)) != hipSuccess) {
+        fsg_lines_free(dl);
+        return fail("LineopCalc: out of device memory allocating the line strength array");
+      }
+      launch_linstr(c->S, dl->L, dl->list_type, c->flag.as<int>(), ls.as<float>(), c->st);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess ||
+          hipMemcpy(linstr_out, ls.p, lines->n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail("fsg_lineop: line strength kernel failed");
+      ls.release();
+    }
     if (flags_out && hipMemcpy(flags_out, c->flag.p, b, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("D2H flags");
     if (ileft_out && hipMemcpy(ileft_out, c->il.p, b, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("D2H ileft");
     if (iright_out && hipMemcpy(iright_out, c->ir.p, b, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("D2H iright");

@@ -32,6 +32,8 @@ void launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, con
                       const CoefB *cb, const double *wlc_s, const int4 *meta_s, float *lineop, int he, hipStream_t st);
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
                       const double *wlc_s, const int4 *meta_s, double *lineop, hipStream_t st);
+void launch_linstr(const DevSetup &S, const DevLines &L, int list_type, const int *flag, float *linstr,
+                   hipStream_t st);
 void launch_radtran32(const DevSetup &S, const float *lop, double *spl, double *spc, double *sfl, double *sfc,
                       hipStream_t st);
 void launch_radtran64(const DevSetup &S, const double *lop, double *spl, double *spc, double *sfl, double *sfc,

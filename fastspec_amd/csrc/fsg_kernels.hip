@@ -791,30 +791,18 @@ __global__ void __launch_bounds__(1024) k_opacity64(DevSetup S, int ntiles, int 
 // ------------------------------------------------------------------------------------
 constexpr int MAXMU = 16;
 
-template <typename LopT>
-struct RTCol {
-  const double *cont;
-  const LopT *lop;
-  int nwl, p;
-  __device__ __forceinline__ double k(int d, bool with_line) const {
-    double v = cont[(size_t)d * nwl + p];
-    if (with_line) v = v + (double)lop[(size_t)d * nwl + p];
-    return v;
-  }
-};
-
 // TauStep (src/solver.f90:591-615) element i (0-based, i <= N-2), Xmu = 1
 template <typename Col>
-__device__ __forceinline__ double tau_step_i(const double *rhox, const Col &col, bool wl, int i) {
+__device__ __forceinline__ double tau_step_i(const double *rhox, const Col &col, int i) {
   if (i == 0) {
     double Dx1 = (rhox[1] - rhox[0]) / 1.0;
-    double x0 = col.k(0, wl), x1 = col.k(1, wl);
+    double x0 = col.k(0), x1 = col.k(1);
     double Yp = (x1 - x0) / Dx1;
     return Dx1 * (x1 + 2.0 * x0 + Dx1 / 2.0 * Yp) / 3.0;
   }
   double Dx = (rhox[i] - rhox[i - 1]) / 1.0;
   double Dx1 = (rhox[i + 1] - rhox[i]) / 1.0;
-  double xm = col.k(i - 1, wl), x0 = col.k(i, wl), xp = col.k(i + 1, wl);
+  double xm = col.k(i - 1), x0 = col.k(i), xp = col.k(i + 1);
   double Der = (x0 - xm) / Dx, Der1 = (xp - x0) / Dx1, Yp;
   if (Der * Der1 > 0.0) {
     double Lambda = (1.0 + Dx1 / (Dx + Dx1)) / 3.0;
@@ -854,56 +842,135 @@ __device__ __forceinline__ void delo_coef(double Dtau, double &Eps, double &Alph
 
 __device__ __forceinline__ double planck_src(double C1, double C2, double T) { return C2 / (exp(C1 / T) - 1.0); }
 
+// Emergent intensities at all mu of one opacity column (LocPrfPP, src/solver.f90:123-179),
+// streamed from the bottom up; col.k(d) is the opacity at depth d.
+template <typename Col>
+__device__ __forceinline__ void locprf_pp(const DevSetup &S, const Col &col, double C1, double C2, const double *imu,
+                                          double *Y) {
+  const int N = S.ntau, M = S.nmu;
+  int i = N - 2;
+  double S_ip1 = planck_src(C1, C2, S.temp[N - 1]);
+  double S_i = planck_src(C1, C2, S.temp[N - 2]);
+  double S_im1 = (N >= 3) ? planck_src(C1, C2, S.temp[N - 3]) : 0.0;
+  double dt_i = tau_step_i(S.rhox, col, i);
+  double dt_im1 = (i >= 1) ? tau_step_i(S.rhox, col, i - 1) : 0.0;
+  double Sd_ip1 = (S_ip1 - S_i) / dt_i;  // Srd(N)
+  for (int m = 0; m < M; m++) Y[m] = S_ip1;
+  for (; i >= 0; i--) {
+    double Sd_i = (i == 0) ? (S_ip1 - S_i) / dt_i : cent_der_i(dt_im1, dt_i, S_im1, S_i, S_ip1);
+    double U = 0.5 * (S_i + 0.5 * dt_i * Sd_i + S_ip1 - 0.5 * dt_i * Sd_ip1);
+    for (int m = 0; m < M; m++) {
+      double Eps, Alpha, Beta, Gamma;
+      delo_coef(dt_i * imu[m], Eps, Alpha, Beta, Gamma);
+      Y[m] = Y[m] * Eps + Alpha * S_i + Beta * S_ip1 + Gamma * U;
+    }
+    // shift the window up by one depth point
+    Sd_ip1 = Sd_i;
+    S_ip1 = S_i;
+    S_i = S_im1;
+    dt_i = dt_im1;
+    if (i >= 2) {
+      S_im1 = planck_src(C1, C2, S.temp[i - 2]);
+      dt_im1 = tau_step_i(S.rhox, col, i - 2);
+    }
+  }
+}
+
+// opacity column of one grid point: continuum, or continuum + accumulated line opacity
+template <typename LopT>
+struct GridCol {
+  const double *cont;
+  const LopT *lop;
+  int nwl, p;
+  bool with_line;
+  __device__ __forceinline__ double k(int d) const {
+    double v = cont[(size_t)d * nwl + p];
+    if (with_line) v = v + (double)lop[(size_t)d * nwl + p];
+    return v;
+  }
+};
+
 template <typename LopT>
 __global__ void __launch_bounds__(256) k_radtran(DevSetup S, const LopT *lop, double *spl, double *spc,
                                                  double *sfl, double *sfc) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= S.nwl) return;
-  const int N = S.ntau, M = S.nmu;
-  RTCol<LopT> col{S.contT, lop, S.nwl, p};
+  const int M = S.nmu;
   double Wl = S.wave[p] + S.wfirst;
   double C1 = kPlkC1 / Wl;
   double C2 = exp(50.7649141 - 5.0 * log(Wl));
-  double imu[MAXMU];
+  double imu[MAXMU], Y[MAXMU];
   for (int m = 0; m < M; m++) imu[m] = 1.0 / S.xmu[m];
   for (int pass = 0; pass < 2; pass++) {  // 0: continuum (SpC), 1: line + continuum (SpL)
-    bool wl = pass == 1;
-    double Y[MAXMU];
-    // sliding window, 0-based depth i: steps dt_i = dtau[i], dt_im1 = dtau[i-1]
-    int i = N - 2;
-    double S_ip1 = planck_src(C1, C2, S.temp[N - 1]);
-    double S_i = planck_src(C1, C2, S.temp[N - 2]);
-    double S_im1 = (N >= 3) ? planck_src(C1, C2, S.temp[N - 3]) : 0.0;
-    double dt_i = tau_step_i(S.rhox, col, wl, i);
-    double dt_im1 = (i >= 1) ? tau_step_i(S.rhox, col, wl, i - 1) : 0.0;
-    double Sd_ip1 = (S_ip1 - S_i) / dt_i;  // Srd(N)
-    for (int m = 0; m < M; m++) Y[m] = S_ip1;
-    for (; i >= 0; i--) {
-      double Sd_i = (i == 0) ? (S_ip1 - S_i) / dt_i : cent_der_i(dt_im1, dt_i, S_im1, S_i, S_ip1);
-      double U = 0.5 * (S_i + 0.5 * dt_i * Sd_i + S_ip1 - 0.5 * dt_i * Sd_ip1);
-      for (int m = 0; m < M; m++) {
-        double Eps, Alpha, Beta, Gamma;
-        delo_coef(dt_i * imu[m], Eps, Alpha, Beta, Gamma);
-        Y[m] = Y[m] * Eps + Alpha * S_i + Beta * S_ip1 + Gamma * U;
-      }
-      // shift the window up by one depth point
-      Sd_ip1 = Sd_i;
-      S_ip1 = S_i;
-      S_i = S_im1;
-      dt_i = dt_im1;
-      if (i >= 2) {
-        S_im1 = (i >= 2) ? planck_src(C1, C2, S.temp[i - 2]) : 0.0;
-        dt_im1 = tau_step_i(S.rhox, col, wl, i - 2);
-      }
-    }
-    double *out = wl ? spl : spc;
+    GridCol<LopT> col{S.contT, lop, S.nwl, p, pass == 1};
+    locprf_pp(S, col, C1, C2, imu, Y);
+    double *out = pass == 1 ? spl : spc;
     double f = 0.0;
     for (int m = 0; m < M; m++) {
       out[(size_t)m * S.nwl + p] = Y[m];
       f = f + Y[m] * S.wmu[m];
     }
-    (wl ? sfl : sfc)[p] = f;
+    (pass == 1 ? sfl : sfc)[p] = f;
   }
+}
+
+// opacity column at a line centre for LinStrCalc: continuum interpolated to the centre and
+// the line's own opacity there (LineOpacity2 at the centre, src/synth.f90:867-888)
+struct LineCentreCol {
+  const DevSetup *S;
+  const XC *xc;
+  int iwl, ihe;
+  double dW;
+  bool with_line;
+  __device__ __forceinline__ double k(int d) const {
+    const double *ck = S->contT + (size_t)d * S->nwl;
+    double kc = ck[iwl - 1] * (1.0 - dW) + ck[iwl] * dW;
+    if (!with_line) return kc;
+    double cop, vv, av;
+    xline_depth(*S, d, *xc, cop, vv, av);
+    double dWl = (ihe > 0) ? 0.0 - S->he_shf[(size_t)(ihe - 1) * S->ntp + d] : 0.0;
+    double lo = hvoigt64_full(*S, av, dWl * vv) * cop;
+    if (S->motype == 1) lo = lo / S->cstd[d];
+    return kc + lo;
+  }
+};
+
+// LinStrCalc (src/synth.f90:1774-1822): central depth of each retained line, plane-parallel
+__global__ void __launch_bounds__(128) k_linstr(DevSetup S, DevLines L, int list_type, const int *flag,
+                                                float *linstr) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n) return;
+  int f = flag[i];
+  if (f & 1) {
+    linstr[i] = 0.0f;
+    return;
+  }
+  const int M = S.nmu;
+  double wlc = L.wl[i] - S.wfirst;
+  XC xc = xline_setup(S, L, i, wlc, f & 0x6, list_type);
+  int iwl = wave_idx(S, wlc);
+  double dW = (wlc - S.wave[iwl - 1]) / (S.wave[iwl] - S.wave[iwl - 1]);
+  double Wl = wlc + S.wfirst;
+  double C1 = kPlkC1 / Wl;
+  double C2 = exp(50.7649141 - 5.0 * log(Wl));
+  double imu[MAXMU], IntC[MAXMU], IntL[MAXMU];
+  for (int m = 0; m < M; m++) imu[m] = 1.0 / S.xmu[m];
+  LineCentreCol cc{&S, &xc, iwl, xc.IxHe, dW, false};
+  locprf_pp(S, cc, C1, C2, imu, IntC);
+  LineCentreCol cl{&S, &xc, iwl, xc.IxHe, dW, true};
+  locprf_pp(S, cl, C1, C2, imu, IntL);
+  float r;
+  if (M == 1) {
+    r = (float)(1.0 - IntL[0] / IntC[0]);
+  } else {
+    double FlxL = 0.0, FlxC = 0.0;
+    for (int m = 0; m < M; m++) {
+      FlxL = FlxL + S.wmu[m] * IntL[m];
+      FlxC = FlxC + S.wmu[m] * IntC[m];
+    }
+    r = (float)(1.0 - FlxL / FlxC);
+  }
+  linstr[i] = r;
 }
 
 // ------------------------------------------------------------------------------------
@@ -983,6 +1050,11 @@ void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, con
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
   k_opacity64<<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, lineop);
+}
+
+void launch_linstr(const DevSetup &S, const DevLines &L, int list_type, const int *flag, float *linstr,
+                   hipStream_t st) {
+  if (L.n) k_linstr<<<nblk(L.n, 128), 128, 0, st>>>(S, L, list_type, flag, linstr);
 }
 
 void launch_radtran32(const DevSetup &S, const float *lop, double *spl, double *spc, double *sfl, double *sfc,

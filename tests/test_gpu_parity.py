@@ -68,3 +68,131 @@ def test_fp32_matches_oracle(c1_small, batched):
     with _gpu(c1_small["setup"], FSG_FP32, batched) as g:
         e = _check(c1_small, g, FP32_OPAC_REL, FP32_FLUX_ABS)
         print("fp32", batched, e)
+
+
+# ---- more cases --------------------------------------------------------------------------
+from cases import mixed_list, molecular_list, with_he_lines  # noqa: E402
+
+
+def _oracle_run(setup, lines, nlim=None, linstr=False):
+    orc = Oracle(setup)
+    ro = orc.lineop(lines, nlines_lim=nlim, linstr=linstr)
+    lop = orc.get_lineop()
+    spl, spc = orc.radtran()
+    sfl, sfc = orc.flux()
+    return dict(setup=setup, lines=lines, ro=ro, lop=lop, spl=spl, spc=spc, sfl=sfl, sfc=sfc)
+
+
+@pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])
+def test_linstr_matches_oracle(prec):
+    setup, lines = W.config_c1(800)
+    ref = _oracle_run(setup, lines, linstr=True)
+    with _gpu(setup, prec) as g:
+        rg = g.LineopCalc(lines, linstr=True)
+        keep = (ref["ro"]["flags"] & 1) == 0
+        assert np.all(rg["linstr"][~keep] == 0)
+        d = np.abs(rg["linstr"] - ref["ro"]["linstr"])
+        assert d.max() <= 2e-6, d.max()
+
+
+@pytest.mark.parametrize("prec,tol", [(FSG_FP64, FP64_FLUX_ABS), (FSG_FP32, FP32_FLUX_ABS)])
+def test_motype1_tau_scale(prec, tol):
+    atm = W.grey_atmosphere()
+    setup = W.build_setup(atm, 5000.0, 5020.0, 0.01, 0, motype=1)
+    lines = W.atomic_lines(1500, 5000.0, 5020.0, 77)
+    ref = _oracle_run(setup, lines)
+    with _gpu(setup, prec) as g:
+        _check(ref, g, FP64_OPAC_REL if prec == FSG_FP64 else FP32_OPAC_REL, tol)
+
+
+@pytest.mark.parametrize("prec,tol", [(FSG_FP64, FP64_FLUX_ABS), (FSG_FP32, FP32_FLUX_ABS)])
+def test_he_stark_lines(prec, tol):
+    setup, lines = W.config_c1(600)
+    setup, lines = with_he_lines(setup, lines)
+    ref = _oracle_run(setup, lines)
+    assert np.any((ref["ro"]["flags"] >> 8) & 0xff), "no He I line matched the table"
+    with _gpu(setup, prec) as g:
+        _check(ref, g, FP64_OPAC_REL if prec == FSG_FP64 else FP32_OPAC_REL, tol)
+
+
+@pytest.mark.parametrize("kind", ["mixed", "molecular"])
+@pytest.mark.parametrize("prec,tol", [(FSG_FP64, FP64_FLUX_ABS), (FSG_FP32, FP32_FLUX_ABS)])
+def test_list_types(kind, prec, tol):
+    setup, lines = W.config_c1(1200)
+    lines = mixed_list(lines) if kind == "mixed" else molecular_list(lines)
+    ref = _oracle_run(setup, lines)
+    with _gpu(setup, prec) as g:
+        _check(ref, g, FP64_OPAC_REL if prec == FSG_FP64 else FP32_OPAC_REL, tol)
+
+
+@pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])
+def test_multi_block_windows_and_opacity(prec):
+    """A block size below the list length: the reference does not PACK and marks wide
+    lines against the block size; the accumulated opacity is unchanged."""
+    setup, lines = W.config_c1(2000)
+    nlim = 300
+    ref = _oracle_run(setup, lines, nlim=nlim)
+    with _gpu(setup, prec) as g:
+        rg = g.LineopCalc(lines, NlinesLim=nlim)
+        assert np.array_equal(rg["flags"], ref["ro"]["flags"])
+        assert np.array_equal(rg["ileft"], ref["ro"]["ileft"])
+        assert rg["stats"]["nwide"] == ref["ro"]["stats"]["nwide"]
+        lop = g.get_lineop()
+        tol = FP64_OPAC_REL if prec == FSG_FP64 else FP32_OPAC_REL
+        assert np.max(np.abs(lop - ref["lop"]) / (ref["lop"] + setup.contop)) <= tol
+
+
+@pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])
+def test_lists_accumulate_and_opacity_file_round_trip(prec):
+    setup, lines = W.config_c1(1000)
+    a = lines.take(np.arange(0, lines.n, 2))
+    b = lines.take(np.arange(1, lines.n, 2))
+    orc = Oracle(setup)
+    orc.lineop(a)
+    orc.lineop(b)
+    ref = orc.get_lineop()
+    tol = FP64_OPAC_REL if prec == FSG_FP64 else FP32_OPAC_REL
+    with _gpu(setup, prec) as g:
+        g.LineopCalc(a)
+        g.LineopCalc(b)
+        lop = g.get_lineop()
+        assert np.max(np.abs(lop - ref) / (ref + setup.contop)) <= tol
+        # OPACITY keyword: a saved table is loaded before further lists accumulate
+        g.reset_lineop()
+        assert np.all(g.get_lineop() == 0)
+        g.set_lineop(lop)
+        lop2 = g.get_lineop()
+        if prec == FSG_FP64:
+            assert np.array_equal(lop2, lop)
+        else:
+            assert np.allclose(lop2, lop, rtol=1e-7, atol=0)
+
+
+@pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])
+def test_empty_and_all_weak_lists(prec):
+    setup, lines = W.config_c1(50)
+    weak = lines.take(np.arange(30))
+    weak.gf[:] = np.float32(1e-30)
+    with _gpu(setup, prec) as g:
+        r = g.LineopCalc(lines.take(np.arange(0)))
+        assert r["stats"]["nlines_use"] == 0
+        r = g.LineopCalc(weak)
+        assert np.all(r["flags"] & 1) and r["stats"]["nlines_use"] == 0 and r["stats"]["evals"] == 0
+        assert np.all(g.get_lineop() == 0)
+        spl, spc = g.RadTranCalc()
+        assert np.allclose(spl, spc, rtol=0, atol=0)
+
+
+def test_error_paths():
+    from fastspec_amd._lib import FsgError
+    setup, lines = W.config_c1(50)
+    with _gpu(setup, FSG_FP32) as g:
+        with pytest.raises(FsgError, match="no formal solution"):
+            g.flux()
+        bad = lines.take(np.arange(10)[::-1])  # unsorted wavelengths
+        with pytest.raises(FsgError, match="not sorted"):
+            g.LineopCalc(bad)
+        bad = lines.take(np.arange(10))
+        bad.spc[3] = 99
+        with pytest.raises(FsgError, match="species"):
+            g.LineopCalc(bad)
