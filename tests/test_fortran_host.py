@@ -1,0 +1,58 @@
+"""The Fortran host binding (fastspec_amd/fortran/fsgpu_host.f90), compiled against the
+reference's own module definitions, drives libfsgpu.so through ISO_C_BINDING."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from fastspec_amd import workloads as W
+from fortran_case import read_out, write_case
+from oracle_bridge import Oracle, ref_available
+
+ROOT = Path(__file__).resolve().parents[1]
+DRIVER = ROOT / "tests" / "fortran" / "_build" / "host_driver"
+
+
+def _driver():
+    if not DRIVER.exists():
+        if not ref_available():
+            pytest.skip("reference modules (oracle/_ref) not built here")
+        subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "fortran")], check=True)
+    return DRIVER
+
+
+@pytest.mark.ref
+def test_driver_reports_library_errors_through_errstop(tmp_path):
+    """Without a usable device the binding stops with the library's message (ErrStop)."""
+    drv = _driver()
+    setup, lines = W.config_c1(50)
+    write_case(tmp_path / "case.bin", setup, lines)
+    import os
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1")
+    r = subprocess.run([str(drv), str(tmp_path / "case.bin")], cwd=tmp_path, capture_output=True, text=True,
+                       env=env, timeout=120)
+    assert "no HIP device" in r.stdout or "HIP error" in r.stdout, r.stdout + r.stderr
+    assert not (tmp_path / "out.bin").exists()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fortran_host_matches_oracle(tmp_path, prec):
+    drv = _driver()
+    setup, lines = W.config_c1(600)
+    write_case(tmp_path / "case.bin", setup, lines)
+    r = subprocess.run([str(drv), str(tmp_path / "case.bin"), str(prec)], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0 and "done" in r.stdout, r.stdout + r.stderr
+    out = read_out(tmp_path / "out.bin", setup.nmu)
+    orc = Oracle(setup)
+    ro = orc.lineop(lines, linstr=True)
+    spl, _ = orc.radtran()
+    sfl, sfc = orc.flux()
+    assert out["nlines_use"] == ro["stats"]["nlines_use"]
+    assert np.array_equal(out["flags"], ro["flags"])
+    assert np.max(np.abs(out["linstr"] - ro["linstr"])) <= 2e-6
+    tol = 1e-8 if prec == 0 else 1e-4
+    assert np.max(np.abs(out["sfl"] / out["sfc"] - sfl / sfc)) <= tol
+    assert np.max(np.abs(out["spl"] - spl) / spl) <= (1e-9 if prec == 0 else 1e-6)
