@@ -147,5 +147,5 @@ def test_zero_field_stokes_reduces_to_unpolarised(c4):
     orc.maglineop(lines, b0, 1)
     spz, spc = orc.magradtran(b0, 1)
     # Q, U, V vanish up to the rounding of the REAL(4) component strengths
-    assert np.max(np.abs(spz[:, 1:, :]) / spz[:, :1, :]) < 1e-9
+    assert np.max(np.abs(spz[:, 1:, :]) / spz[:, :1, :]) < 1e-7
     assert np.all(spz[:, 0, :] <= spc * (1 + 1e-12))
