@@ -15,7 +15,7 @@ EXPORTS = (
     "fsg_init", "fsg_finalize", "fsg_last_error", "fsg_lineop", "fsg_maglineop",
     "fsg_get_lineop", "fsg_set_lineop", "fsg_get_maglineop", "fsg_radtran",
     "fsg_magradtran", "fsg_flux", "fsg_lines_upload", "fsg_lines_free", "fsg_lineop_dev",
-    "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop",
+    "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop", "fsg_maglineop_dev",
 )
 
 _LIB = None
@@ -52,6 +52,7 @@ def load() -> C.CDLL:
     lib.fsg_lines_upload.argtypes = [vp, C.c_int32, C.POINTER(CLines), C.POINTER(vp)]
     lib.fsg_lines_free.argtypes = [vp]
     lib.fsg_lineop_dev.argtypes = [vp, vp, C.c_int64, C.POINTER(CStats)]
+    lib.fsg_maglineop_dev.argtypes = [vp, vp, C.c_int64, dp, C.c_int32, C.POINTER(CStats)]
     lib.fsg_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp)]
     lib.fsg_stream.argtypes = [vp, C.POINTER(vp)]
     lib.fsg_reset_lineop.argtypes = [vp]

@@ -13,7 +13,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libfsgpu.so"
-SOURCES = ["fsg_kernels.hip", "fsg_api.hip"]
+SOURCES = ["fsg_kernels.hip", "fsg_magnetic.hip", "fsg_api.hip"]
 ARCH = os.environ.get("FSG_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for all device code.
@@ -32,17 +32,22 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> Path:
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
-    objs = []
-    for src in SOURCES:
+    objs, procs = [], []
+    for src in SOURCES:  # the translation units compile concurrently
         obj = objdir / (Path(src).stem + ".o")
         cmd = [hipcc(), *HIPFLAGS, "-I", str(CSRC), "-c", str(CSRC / src), "-o", str(obj)]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"hipcc failed on {src}")
-        if verbose:
-            sys.stderr.write(r.stderr)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
         objs.append(str(obj))
+    failed = []
+    for src, p in procs:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out + err)
+            failed.append(src)
+        elif verbose:
+            sys.stderr.write(err)
+    if failed:
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

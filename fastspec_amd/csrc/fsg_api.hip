@@ -75,12 +75,21 @@ struct fsg_ctx {
   int *bincnt = nullptr, *binhmax = nullptr, *binoff = nullptr;
   unsigned long long *u64 = nullptr;
   hipEvent_t ev[4] = {};
+  // polarised path, allocated on first use
+  void *maglineop = nullptr;  // [6][ntau][nwl] MagLineOpac, wavelength fastest
+  float2 *ftab = nullptr;     // Faraday-Voigt table [NVA+1][TAB_LD]
+  double *spz = nullptr;      // [4][nmu][nwl] Stokes I, Q, U, V
+  double *geo = nullptr;      // [nmu][5] field geometry
+  int magnetic = 0;           // the last formal solution was the polarised one
+  DBuf comp;
 };
 
 struct fsg_dlines {
   fsg_ctx *ctx = nullptr;
   int list_type = 0;
   DevLines L{};
+  MagLine Z{};      // Zeeman patterns (magnetic lists only)
+  int zeeman = 0;
   std::vector<void *> owned;
 };
 
@@ -271,7 +280,7 @@ int fsg_finalize(fsg_ctx *c) {
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (void *p : c->owned) (void)hipFree(p);
   for (DBuf *b : {&c->flag, &c->il, &c->ir, &c->ic, &c->key, &c->key2, &c->idx, &c->idx2, &c->coef, &c->wlc,
-                  &c->meta, &c->rng, &c->tmp, &c->pos, &c->wlcl, &c->rflag})
+                  &c->meta, &c->rng, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp})
     b->release();
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -311,6 +320,39 @@ int fsg_lines_upload(fsg_ctx *c, int32_t list_type, const fsg_lines *h, fsg_dlin
     rc |= upload(c, h->gvdw, n, &d->L.gvdw, d->owned);
   }
   if (list_type == FSG_LIST_MIXED && h->is_mol) rc |= upload(c, h->is_mol, n, &d->L.is_mol, d->owned);
+  if (h->pi_off && h->si_off && n > 0 && !rc) {
+    // Zeeman patterns (CSR): offsets start at 0 and grow; sigma groups hold a blue and a
+    // red half of equal size
+    const char *bad = nullptr;
+    if (h->pi_off[0] != 0 || h->si_off[0] != 0) bad = "offsets must start at 0";
+    for (size_t i = 0; i < n && !bad; i++) {
+      int32_t np = h->pi_off[i + 1] - h->pi_off[i], ns = h->si_off[i + 1] - h->si_off[i];
+      if (np < 0 || ns < 0) bad = "offsets must not decrease";
+      else if (ns % 2) bad = "sigma components must come in blue/red pairs";
+    }
+    const int64_t npi = h->pi_off[n], nsi = h->si_off[n];
+    if (!bad && ((npi > 0 && (!h->spl_pi || !h->str_pi)) || (nsi > 0 && (!h->spl_si || !h->str_si))))
+      bad = "component arrays are NULL";
+    if (bad) {
+      fsg_lines_free(d);
+      return fail(std::string("fsg_lines_upload: Zeeman pattern: ") + bad);
+    }
+    float *sp = nullptr;
+    rc |= upload(c, h->pi_off, n + 1, &d->Z.pi_off, d->owned);
+    rc |= upload(c, h->si_off, n + 1, &d->Z.si_off, d->owned);
+    rc |= upload(c, h->spl_pi, (size_t)npi, &d->Z.spl_pi, d->owned);
+    rc |= upload(c, h->str_pi, (size_t)npi, &d->Z.str_pi, d->owned);
+    rc |= upload(c, h->spl_si, (size_t)nsi, &d->Z.spl_si, d->owned);
+    rc |= upload(c, h->str_si, (size_t)nsi, &d->Z.str_si, d->owned);
+    if (!rc && hipMalloc((void **)&sp, sizeof(float) * (size_t)std::max<int64_t>(npi + nsi, 1)) != hipSuccess)
+      rc = fail("fsg_lines_upload: out of device memory for the Zeeman shifts");
+    if (sp) {
+      d->owned.push_back(sp);
+      d->Z.sp_pi = sp;
+      d->Z.sp_si = sp + npi;
+    }
+    d->zeeman = 1;
+  }
   if (rc || hipStreamSynchronize(c->st) != hipSuccess) {
     fsg_lines_free(d);
     return 1;
@@ -435,7 +477,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
     const int ntiles = (S.nwl + TILE - 1) / TILE;
     const int ng = nbins + 1;
     CK(c->rng.ensure((size_t)ntiles * ng * sizeof(int2)));
-    launch_tile_ranges(ntiles, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng, c->meta.as<int4>(),
+    launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng, c->meta.as<int4>(),
                        c->rng.as<int2>(), s);
     CK(hipEventRecord(c->ev[2], s));
     // the range table has ng groups per tile: the first nbins are read by the main launch
@@ -474,6 +516,56 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+// host copies of pass 1's results for the input lines: LinFlag, Ileft, Iright and, on
+// request, the central line depths LinStr (src/synth.f90:1774-1822)
+int lineop_outputs(fsg_ctx *c, fsg_dlines *dl, int list_type, float *linstr_out, int32_t *flags_out,
+                   int32_t *ileft_out, int32_t *iright_out) {
+  const int64_t n = dl->L.n;
+  if (n == 0) return 0;
+  size_t b = (size_t)n * sizeof(int32_t);
+  if (linstr_out) {
+    DBuf ls;
+    if (ls.ensure(n * sizeof(float)) != hipSuccess)
+      return fail("LineopCalc: out of device memory allocating the line strength array");
+    launch_linstr(c->S, dl->L, list_type, c->flag.as<int>(), ls.as<float>(), c->st);
+    int rc = 0;
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess ||
+        hipMemcpy(linstr_out, ls.p, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail("line strength kernel failed");
+    ls.release();
+    if (rc) return rc;
+  }
+  if (flags_out && hipMemcpy(flags_out, c->flag.p, b, hipMemcpyDeviceToHost) != hipSuccess) return fail("D2H flags");
+  if (ileft_out && hipMemcpy(ileft_out, c->il.p, b, hipMemcpyDeviceToHost) != hipSuccess) return fail("D2H ileft");
+  if (iright_out && hipMemcpy(iright_out, c->ir.p, b, hipMemcpyDeviceToHost) != hipSuccess) return fail("D2H iright");
+  return 0;
+}
+
+// MagLineOpac, the Faraday-Voigt table and the Stokes buffers, on first use
+int ensure_mag_buffers(fsg_ctx *c) {
+  if (c->maglineop) return 0;
+  const size_t N = (size_t)c->S.ntau, Nw = (size_t)c->S.nwl, M = (size_t)c->S.nmu;
+  void *tab = nullptr;
+  if (alloc_zero(c, lop_elem(c) * 6 * N * Nw, &c->maglineop) ||
+      alloc_zero(c, sizeof(float2) * (NVA + 1) * TAB_LD, &tab) ||
+      alloc_zero(c, sizeof(double) * 4 * M * Nw, (void **)&c->spz) ||
+      alloc_zero(c, sizeof(double) * 5 * M, (void **)&c->geo))
+    return 1;
+  c->ftab = (float2 *)tab;
+  launch_fvoigt_table(c->ftab, c->S.anode, 8.0 / (double)(NVV - 1), c->st);
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int fsg_lineop(fsg_ctx *c, int32_t list_type, const fsg_lines *lines, int64_t nlines_lim, float *linstr_out,
                int32_t *flags_out, int32_t *ileft_out, int32_t *iright_out, fsg_lineop_stats *stats) {
   g_err.clear();
@@ -481,49 +573,158 @@ int fsg_lineop(fsg_ctx *c, int32_t list_type, const fsg_lines *lines, int64_t nl
   fsg_dlines *dl = nullptr;
   if (fsg_lines_upload(c, list_type, lines, &dl)) return 1;
   int rc = fsg_lineop_dev(c, dl, nlines_lim, stats);
-  if (rc == 0 && lines->n > 0) {
-    size_t b = (size_t)lines->n * sizeof(int32_t);
-    if (linstr_out) {
-      // central line depths of the retained lines (pass 1 flags are still resident)
-      DBuf ls;
-      if (ls.ensure(lines->n * sizeof(float)) != hipSuccess) {
-        fsg_lines_free(dl);
-        return fail("LineopCalc: out of device memory allocating the line strength array");
-      }
-      launch_linstr(c->S, dl->L, dl->list_type, c->flag.as<int>(), ls.as<float>(), c->st);
-      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess ||
-          hipMemcpy(linstr_out, ls.p, lines->n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
-        rc = fail("fsg_lineop: line strength kernel failed");
-      ls.release();
-    }
-    if (flags_out && hipMemcpy(flags_out, c->flag.p, b, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("D2H flags");
-    if (ileft_out && hipMemcpy(ileft_out, c->il.p, b, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("D2H ileft");
-    if (iright_out && hipMemcpy(iright_out, c->ir.p, b, hipMemcpyDeviceToHost) != hipSuccess) rc = fail("D2H iright");
-  }
+  if (rc == 0) rc = lineop_outputs(c, dl, list_type, linstr_out, flags_out, ileft_out, iright_out);
   fsg_lines_free(dl);
   return rc;
+}
+
+// The polarised line-opacity stage on a device-resident Zeeman list: pass 1 with the
+// window search, width binning, coefficient fill and the six-profile accumulation into
+// MagLineOpac.  MarkWideLines and PACK follow the reference's rules for the counters; the
+// accumulated opacity does not depend on the block partition.
+int fsg_maglineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, const double *bfield, int32_t bframe,
+                      fsg_lineop_stats *stats) {
+  g_err.clear();
+  if (!c || !dl || !bfield) return fail("fsg_maglineop_dev: NULL argument");
+  if (nlines_lim < 1) return fail("fsg_maglineop_dev: NlinesLim must be positive");
+  if (bframe < 0 || bframe > 2) return fail("fsg_maglineop_dev: BFRAME must be 0, 1 or 2");
+  const int64_t n = dl->L.n;
+  if (n > 0 && !dl->zeeman) return fail("MagLineopCalc: line list has no Zeeman patterns");
+  CK(hipSetDevice(c->dev));
+  if (ensure_mag_buffers(c)) return 1;
+  fsg_lineop_stats st{};
+  st.nlines_in = n;
+  st.nsegments = 1;
+  if (n == 0) {
+    if (stats) *stats = st;
+    return 0;
+  }
+  const DevSetup &S = c->S;
+  const int N = S.ntau;
+  hipStream_t s = c->st;
+  // Zeeman splitting per unit shift (src/synth.f90:1150-1154): |B| for BFRAME 0, the
+  // first BFIELD value (the field modulus) otherwise
+  const double bsplit = (bframe == 0)
+                            ? kKz * std::sqrt(bfield[0] * bfield[0] + bfield[1] * bfield[1] + bfield[2] * bfield[2])
+                            : kKz * std::fabs(bfield[0]);
+  CK(c->flag.ensure(n * sizeof(int)));
+  CK(c->il.ensure(n * sizeof(int)));
+  CK(c->ir.ensure(n * sizeof(int)));
+  CK(c->ic.ensure(n * sizeof(int)));
+  CK(c->key.ensure(n * sizeof(unsigned)));
+  CK(c->key2.ensure(n * sizeof(unsigned)));
+  CK(c->idx.ensure(n * sizeof(int)));
+  CK(c->idx2.ensure(n * sizeof(int)));
+  CK(hipMemsetAsync(c->bincnt, 0, sizeof(int) * 64, s));
+  CK(hipMemsetAsync(c->binhmax, 0, sizeof(int) * 64, s));
+  CK(hipMemsetAsync(c->u64, 0, sizeof(unsigned long long) * 4, s));
+  CK(hipEventRecord(c->ev[0], s));
+  launch_magpass1(S, dl->L, dl->Z, bsplit, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), c->ic.as<int>(),
+                  c->key.as<unsigned>(), c->bincnt, c->binhmax, c->batched, s);
+  launch_count_evals(n, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), N, c->u64, s);
+  launch_iota(c->idx.as<int>(), n, s);
+  size_t tbytes = 0;
+  CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, c->key.as<unsigned>(), c->key2.as<unsigned>(),
+                                        c->idx.as<int>(), c->idx2.as<int>(), (int)n, 0, 6, s));
+  CK(c->tmp.ensure(tbytes));
+  CK(hipcub::DeviceRadixSort::SortPairs(c->tmp.p, tbytes, c->key.as<unsigned>(), c->key2.as<unsigned>(),
+                                        c->idx.as<int>(), c->idx2.as<int>(), (int)n, 0, 6, s));
+  int hcnt[64], hhmax[64];
+  unsigned long long hu[4];
+  std::vector<int> packed;
+  CK(hipMemcpyAsync(hcnt, c->bincnt, sizeof(hcnt), hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(hhmax, c->binhmax, sizeof(hhmax), hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
+  CK(hipEventRecord(c->ev[1], s));
+  CK(hipStreamSynchronize(s));
+  int64_t nret = 0;
+  int nbins = 0;
+  for (int b = 0; b < WEAK_KEY; b++) {
+    nret += hcnt[b];
+    if (hcnt[b] > 0) nbins = b + 1;
+  }
+  if (nbins > MAX_BINS) return fail("fsg_maglineop_dev: too many window-width bins");
+  st.nlines_use = nret;
+  st.evals = (int64_t)hu[0];
+  // MarkWideLines (PACK happens when the whole list fits one block)
+  const int64_t nstep = std::min<int64_t>(n, nlines_lim);
+  const int pack = (nstep == n);
+  CK(c->wlcl.ensure(n * sizeof(double)));
+  CK(c->rflag.ensure(n * sizeof(int)));
+  CK(c->pos.ensure(n * sizeof(int)));
+  if (pack) {
+    launch_retained_flag(n, c->flag.as<int>(), c->rflag.as<int>(), s);
+    size_t sb = 0;
+    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, c->rflag.as<int>(), c->pos.as<int>(), (int)n, s));
+    CK(c->tmp.ensure(sb));
+    CK(hipcub::DeviceScan::ExclusiveSum(c->tmp.p, sb, c->rflag.as<int>(), c->pos.as<int>(), (int)n, s));
+  }
+  launch_wlc_list(n, dl->L.wl, S.wfirst, c->flag.as<int>(), c->pos.as<int>(), c->wlcl.as<double>(), pack, s);
+  launch_mark_wide(S, n, dl->L.wl, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), c->wlcl.as<double>(),
+                   pack ? nret : n, nstep, c->u64 + 1, s);
+  const int fp32 = c->precision == FSG_FP32;
+  if (nret > 0) {
+    int64_t o = 0;
+    packed.assign(3 * nbins, 0);
+    for (int b = 0; b < nbins; b++) {
+      packed[b] = (int)o;
+      packed[nbins + b] = hcnt[b];
+      packed[2 * nbins + b] = hhmax[b];
+      o += hcnt[b];
+    }
+    CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * packed.size(), hipMemcpyHostToDevice, s));
+    CK(c->coef.ensure((size_t)nret * S.ntp * magcoef_size(fp32)));
+    CK(c->wlc.ensure((size_t)nret * sizeof(double)));
+    CK(c->meta.ensure((size_t)nret * sizeof(int4)));
+    CK(c->comp.ensure((size_t)nret * sizeof(int4)));
+    launch_magfill(S, dl->L, dl->Z, nret, c->idx2.as<int>(), c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
+                   c->ic.as<int>(), c->coef.p, fp32, c->wlc.as<double>(), c->meta.as<int4>(), c->comp.as<int4>(), s);
+    const int ntiles = (S.nwl + MTILE - 1) / MTILE;
+    CK(c->rng.ensure((size_t)ntiles * nbins * sizeof(int2)));
+    launch_tile_ranges(ntiles, MTILE, nbins, S.nwl, c->binoff, c->binoff + nbins, c->binoff + 2 * nbins,
+                       c->meta.as<int4>(), c->rng.as<int2>(), s);
+    CK(hipEventRecord(c->ev[2], s));
+    launch_magopacity(S, c->ftab, ntiles, nbins, nbins, c->rng.as<int2>(), c->coef.p, c->wlc.as<double>(),
+                      c->meta.as<int4>(), c->comp.as<int4>(), dl->Z, c->maglineop, fp32, s);
+    CK(hipEventRecord(c->ev[3], s));
+  } else {
+    CK(hipEventRecord(c->ev[2], s));
+    CK(hipEventRecord(c->ev[3], s));
+  }
+  CK(hipGetLastError());
+  CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  st.nwide = (int64_t)hu[1];
+  float ms1 = 0.f, ms2 = 0.f;
+  CK(hipEventElapsedTime(&ms1, c->ev[0], c->ev[1]));
+  CK(hipEventElapsedTime(&ms2, c->ev[2], c->ev[3]));
+  st.ms_pass1 = ms1;
+  st.ms_opacity = ms2;
+  c->have_rt = 0;
+  if (stats) *stats = st;
+  return 0;
 }
 
 int fsg_maglineop(fsg_ctx *c, const fsg_lines *lines, int64_t nlines_lim, const double *bfield, int32_t bframe,
                   float *linstr_out, int32_t *flags_out, int32_t *ileft_out, int32_t *iright_out,
                   fsg_lineop_stats *stats) {
-  (void)c;
-  (void)lines;
-  (void)nlines_lim;
-  (void)bfield;
-  (void)bframe;
-  (void)linstr_out;
-  (void)flags_out;
-  (void)ileft_out;
-  (void)iright_out;
-  (void)stats;
-  return fail("fsg_maglineop: magnetic line opacity is not available in this build");
+  g_err.clear();
+  if (!c || !lines || !bfield) return fail("fsg_maglineop: NULL argument");
+  if (lines->n > 0 && (!lines->pi_off || !lines->si_off))
+    return fail("MagLineopCalc: line list has no Zeeman patterns");
+  fsg_dlines *dl = nullptr;
+  if (fsg_lines_upload(c, FSG_LIST_MIXED, lines, &dl)) return 1;
+  int rc = fsg_maglineop_dev(c, dl, nlines_lim, bfield, bframe, stats);
+  if (rc == 0) rc = lineop_outputs(c, dl, FSG_LIST_MIXED, linstr_out, flags_out, ileft_out, iright_out);
+  fsg_lines_free(dl);
+  return rc;
 }
 
 int fsg_reset_lineop(fsg_ctx *c) {
   if (!c) return fail("fsg_reset_lineop: NULL context");
   CK(hipSetDevice(c->dev));
   CK(hipMemsetAsync(c->lineop, 0, lop_elem(c) * (size_t)c->S.ntau * c->S.nwl, c->st));
+  if (c->maglineop) CK(hipMemsetAsync(c->maglineop, 0, lop_elem(c) * 6 * (size_t)c->S.ntau * c->S.nwl, c->st));
   CK(hipStreamSynchronize(c->st));
   c->have_rt = 0;
   return 0;
@@ -575,9 +776,29 @@ int fsg_set_lineop(fsg_ctx *c, const double *in) {
 }
 
 int fsg_get_maglineop(fsg_ctx *c, double *out) {
-  (void)c;
-  (void)out;
-  return fail("fsg_get_maglineop: magnetic line opacity is not available in this build");
+  g_err.clear();
+  if (!c || !out) return fail("fsg_get_maglineop: NULL argument");
+  CK(hipSetDevice(c->dev));
+  const size_t N = (size_t)c->S.ntau, Nw = (size_t)c->S.nwl, tot = 6 * N * Nw;
+  if (!c->maglineop) {
+    std::memset(out, 0, sizeof(double) * tot);
+    return 0;
+  }
+  // device [6][ntau][nwl] -> MagLineOpac(6, Ntau, Nwl)
+  std::vector<double> h(tot);
+  if (c->precision == FSG_FP32) {
+    std::vector<float> f(tot);
+    CK(hipMemcpyAsync(f.data(), c->maglineop, tot * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    CK(hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < tot; i++) h[i] = (double)f[i];
+  } else {
+    CK(hipMemcpyAsync(h.data(), c->maglineop, tot * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    CK(hipStreamSynchronize(c->st));
+  }
+  for (size_t q = 0; q < 6; q++)
+    for (size_t k = 0; k < N; k++)
+      for (size_t iw = 0; iw < Nw; iw++) out[(iw * N + k) * 6 + q] = h[(q * N + k) * Nw + iw];
+  return 0;
 }
 
 static int ensure_rt_buffers(fsg_ctx *c) {
@@ -611,6 +832,7 @@ int fsg_radtran(fsg_ctx *c, double *spl_out, double *spc_out) {
     launch_radtran64(c->S, (const double *)c->lineop, c->spl, c->spc, c->sfl, c->sfc, c->st);
   CK(hipGetLastError());
   c->have_rt = 1;
+  c->magnetic = 0;
   if (spl_out && copy_spec_out(c, c->spl, spl_out)) return 1;
   if (spc_out && copy_spec_out(c, c->spc, spc_out)) return 1;
   CK(hipStreamSynchronize(c->st));
@@ -619,13 +841,85 @@ int fsg_radtran(fsg_ctx *c, double *spl_out, double *spc_out) {
 
 int fsg_magradtran(fsg_ctx *c, const double *bfield, int32_t nbfield, int32_t bframe, double *spz_out,
                    double *spc_out) {
-  (void)c;
-  (void)bfield;
-  (void)nbfield;
-  (void)bframe;
-  (void)spz_out;
-  (void)spc_out;
-  return fail("fsg_magradtran: polarised transfer is not available in this build");
+  g_err.clear();
+  if (!c || !bfield) return fail("fsg_magradtran: NULL argument");
+  const int M = c->S.nmu;
+  if (bframe < 0 || bframe > 2) return fail("fsg_magradtran: BFRAME must be 0, 1 or 2");
+  if (bframe == 2 && nbfield < 1 + 2 * M) return fail("BFRAME 2 needs 1+2*Nmu BFIELD values");
+  if (bframe != 2 && nbfield < 3) return fail("fsg_magradtran: BFIELD needs 3 values");
+  CK(hipSetDevice(c->dev));
+  if (ensure_rt_buffers(c) || ensure_mag_buffers(c)) return 1;
+  // field geometry per mu (src/synth.f90:1935-1994): sin^2 and cos^2 of the inclination,
+  // its cosine, and sin/cos of twice the azimuth
+  std::vector<double> g((size_t)5 * M);
+  std::vector<double> xmu(M);
+  CK(hipMemcpy(xmu.data(), c->S.xmu, sizeof(double) * M, hipMemcpyDeviceToHost));
+  const double d2r = kPi / 180.0;
+  for (int im = 0; im < M; im++) {
+    double Sphi, Cphi, Spsi, Cpsi;
+    if (bframe == 1) {
+      Sphi = std::sin(bfield[1] * d2r);
+      Cphi = std::cos(bfield[1] * d2r);
+      Spsi = std::sin(bfield[2] * d2r);
+      Cpsi = std::cos(bfield[2] * d2r);
+    } else if (bframe == 2) {
+      Sphi = std::sin(bfield[2 * im + 1] * d2r);
+      Cphi = std::cos(bfield[2 * im + 1] * d2r);
+      Spsi = std::sin(bfield[2 * im + 2] * d2r);
+      Cpsi = std::cos(bfield[2 * im + 2] * d2r);
+    } else {
+      const double Fldmod = std::sqrt(bfield[0] * bfield[0] + bfield[1] * bfield[1] + bfield[2] * bfield[2]);
+      if (Fldmod > 0.0) {
+        const double Coslon = xmu[im], Sinlon = std::sqrt(1.0 - Coslon * Coslon);
+        const double Bx = bfield[0] * Sinlon + bfield[2] * Coslon, By = bfield[1];
+        const double Bz = bfield[0] * Coslon - bfield[2] * Sinlon;
+        const double Bt = std::sqrt(Bx * Bx + By * By);
+        Sphi = Bt / Fldmod;
+        Cphi = Bz / Fldmod;
+        if (Bt > 0.0) {
+          Spsi = By / Bt;
+          Cpsi = Bx / Bt;
+        } else {
+          Spsi = 0.0;
+          Cpsi = 1.0;
+        }
+      } else {
+        Sphi = 0.0;
+        Cphi = 1.0;
+        Spsi = 0.0;
+        Cpsi = 1.0;
+      }
+    }
+    double *gm = g.data() + (size_t)5 * im;
+    gm[0] = Sphi * Sphi;
+    gm[1] = Cphi * Cphi;
+    gm[2] = Cphi;
+    gm[3] = 2.0 * Spsi * Cpsi;
+    gm[4] = Cpsi * Cpsi - Spsi * Spsi;
+  }
+  CK(hipMemcpyAsync(c->geo, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, c->st));
+  // continuum intensities and their flux (the scalar solution of LineOpac + ContOpac is
+  // computed alongside and replaced by the Stokes I flux below)
+  const int fp32 = c->precision == FSG_FP32;
+  if (fp32)
+    launch_radtran32(c->S, (const float *)c->lineop, c->spl, c->spc, c->sfl, c->sfc, c->st);
+  else
+    launch_radtran64(c->S, (const double *)c->lineop, c->spl, c->spc, c->sfl, c->sfc, c->st);
+  launch_stokes(c->S, c->lineop, c->maglineop, fp32, c->geo, c->spz, c->sfl, c->st);
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(c->st));
+  c->have_rt = 1;
+  c->magnetic = 1;
+  if (spz_out) {
+    const size_t Nw = (size_t)c->S.nwl;
+    std::vector<double> h((size_t)4 * M * Nw);
+    CK(hipMemcpy(h.data(), c->spz, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < 4; q++)
+      for (size_t m = 0; m < (size_t)M; m++)
+        for (size_t iw = 0; iw < Nw; iw++) spz_out[(iw * 4 + q) * M + m] = h[(q * M + m) * Nw + iw];
+  }
+  if (spc_out && copy_spec_out(c, c->spc, spc_out)) return 1;
+  return 0;
 }
 
 int fsg_flux(fsg_ctx *c, double *sfl_out, double *sfc_out) {
@@ -643,12 +937,15 @@ int fsg_flux(fsg_ctx *c, double *sfl_out, double *sfc_out) {
 int fsg_device_ptr(fsg_ctx *c, int32_t which, void **ptr_out) {
   if (!c || !ptr_out) return fail("fsg_device_ptr: NULL argument");
   if (which != 0 && ensure_rt_buffers(c)) return 1;
+  if (which >= 5 && ensure_mag_buffers(c)) return 1;
   switch (which) {
     case 0: *ptr_out = c->lineop; break;
     case 1: *ptr_out = c->spl; break;
     case 2: *ptr_out = c->spc; break;
     case 3: *ptr_out = c->sfl; break;
     case 4: *ptr_out = c->sfc; break;
+    case 5: *ptr_out = c->maglineop; break;
+    case 6: *ptr_out = c->spz; break;
     default: return fail("fsg_device_ptr: unknown array");
   }
   return 0;

@@ -106,4 +106,17 @@ struct DevLines {
   const uint8_t *is_mol;
 };
 
+// Zeeman patterns of a device-resident list (src/ioutils.f90:1012-1031): per line i the
+// pi components [pi_off[i], pi_off[i+1]) and the sigma components [si_off[i], si_off[i+1]),
+// blue half first.  spl_* are shifts per unit Bsplit lambda^2; sp_* receive the scaled
+// shifts of the current field (REAL(4), src/synth.f90:1193-1197).
+struct MagLine {
+  const int *pi_off, *si_off;
+  const float *spl_pi, *str_pi, *spl_si, *str_si;
+  float *sp_pi, *sp_si;
+};
+
+// the polarised opacity kernel: one wavelength point per lane, DPW depths per wavefront
+constexpr int MTILE = 64;
+
 }  // namespace fsg

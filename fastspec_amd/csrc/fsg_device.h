@@ -1,0 +1,267 @@
+// fsg_device.h -- device helpers shared by the line-opacity and formal-solution kernels:
+// XlineSetup/XlineDepth, the HVoigt table lookup, WaveIdx, the XCD-aware tile order,
+// compensated FP32 sums and the DELO-Bezier step functions.
+#pragma once
+
+#include "fsg_common.h"
+
+namespace fsg {
+
+// ------------------------------------------------------------------------------------
+// per-line kernel coefficients
+// ------------------------------------------------------------------------------------
+struct XC {
+  double WaveL, Hnux, Elo, Gf, Grad, Xmass, Cqst, Qdiv, Cvdw, AboPre, AboVbc, AboExp;
+  int IxPF, MdQst, MdVdw, IxHe;
+};
+
+__device__ __forceinline__ double air2vac(double L) {  // src/utils.f90:1020-1032
+  double S = 1e4 / L, S2 = S * S;
+  double N = 1.0 + 8.336624212083e-5 + 2.408926869968e-2 / (1.301065924522e2 - S2) +
+             1.599740894897e-4 / (3.892568793293e1 - S2);
+  return (L > 1999.352) ? L * N : L;
+}
+
+// flag bits as LineopCalc pass 1 sets them before XlineSetup (src/synth.f90:839-844)
+__device__ __forceinline__ int initial_flag(const DevLines &L, int64_t i, int list_type) {
+  int f = 0;
+  if (list_type == 0) {
+    if (L.gvdw[i] > 10.0f) f |= 4;
+  } else if (list_type == 2) {
+    if (L.is_mol && L.is_mol[i]) f |= 2;
+    if (L.gvdw[i] > 10.0f) f |= 4;
+  }
+  return f;
+}
+
+// XlineSetup (src/opacity.f90:96-235)
+__device__ inline XC xline_setup(const DevSetup &S, const DevLines &L, int64_t i, double wlc, int flag,
+                          int list_type) {
+  XC xc;
+  double WaveL = wlc + S.wfirst;
+  double Elo = (double)L.elo[i];
+  xc.WaveL = WaveL;
+  xc.Elo = Elo;
+  xc.Gf = (double)L.gf[i];
+  double GradL = (double)L.grad[i];
+  int ispc = L.spc[i];
+  int IonL = S.spc_ion[ispc - 1];
+  xc.IxPF = S.spc_slot[ispc - 1];
+  double GqstL, GvdwL;
+  bool IsMol;
+  if (list_type == 1) {
+    GqstL = 0.0;
+    GvdwL = 0.0;
+    IsMol = true;
+  } else {
+    GqstL = (double)L.gqst[i];
+    GvdwL = (double)L.gvdw[i];
+    IsMol = (list_type == 2) ? ((flag >> 1) & 1) : false;
+  }
+  double xam = (double)S.xamass[xc.IxPF - 1];
+  xc.Hnux = kXlnC * kHc / WaveL;
+  xc.Xmass = kXlnDop / kXlnC / kXlnC / xam;
+  xc.Grad = (GradL > 0.0) ? GradL : 0.22233e16 / (WaveL * WaveL);
+  xc.Cqst = 0.0;
+  xc.Qdiv = 1.0;
+  xc.Cvdw = 0.0;
+  xc.AboPre = 0.0;
+  xc.AboVbc = 0.0;
+  xc.AboExp = 0.0;
+  xc.IxHe = 0;
+  if (IsMol && !S.molbrd) {
+    xc.MdQst = 9;
+    xc.MdVdw = 9;
+  } else {
+    double Wavevac = air2vac(WaveL);
+    double Eup = Elo + 1.0 / (Wavevac * kEv2Ang);
+    double pot = (double)S.pot_ion[xc.IxPF - 1];
+    double t1 = (double)(IonL * IonL) * kHie / (pot - Eup);
+    double t2 = (double)(IonL * IonL) * kHie / (pot - Elo);
+    double Enu4 = t1 * t1, Enl4 = t2 * t2;
+    if (xc.IxPF == S.he1_slot) {  // HeMatchIdx (src/hestark.f90:376-389)
+      for (int k = 1; k <= S.nhe; k++)
+        if (fabs(WaveL - S.he_wl[k - 1]) <= S.he_wl_tol && fabs(Elo - S.he_elo[k - 1]) <= S.he_elo_tol) {
+          xc.IxHe = k;
+          break;
+        }
+    }
+    if (xc.IxHe > 0) {
+      xc.MdQst = 3;
+    } else if (GqstL > 0.0) {
+      xc.MdQst = 0;
+      xc.Cqst = GqstL;
+    } else if (IsMol) {
+      xc.MdQst = 9;
+    } else if (IonL == 1) {
+      xc.MdQst = 1;
+      xc.Cqst = 2.26e-07 * Enu4;
+    } else {
+      xc.MdQst = 2;
+      xc.Cqst = 5.42e-07 * Enu4;
+      xc.Qdiv = (IonL + 1.0) * (IonL + 1.0);
+    }
+    if ((flag >> 2) & 1) {
+      xc.MdVdw = 2;
+      double Sigma = kA0 * kA0 * (double)(int)GvdwL;
+      double Alpha = GvdwL - (double)(int)GvdwL;
+      double X = 2.0 - Alpha * 0.5;
+      double Gx = X - 1.0;
+      double GammaF = 1.0 + (-0.5748646 + (0.9512363 + (-0.6998588 + (0.4245549 - 0.1010678 * Gx) * Gx) * Gx) * Gx) * Gx;
+      xc.AboPre = 2e6 * (pow(4.0 / kPi, Alpha * 0.5) * GammaF * 1e4 * Sigma);
+      xc.AboVbc = 1.0 / 1.00794 + 1.0 / xam;
+      xc.AboExp = 1.0 - Alpha;
+    } else if (GvdwL > 0.0) {
+      xc.MdVdw = 0;
+      xc.Cvdw = GvdwL;
+    } else if (IsMol) {
+      xc.MdVdw = 9;
+    } else {
+      xc.MdVdw = 1;
+      double Cw = 1.61e-33 * fabs(Enu4 - Enl4) / (double)(IonL * IonL);
+      xc.Cvdw = pow(Cw, 0.4);
+    }
+  }
+  if (IsMol && S.vwmol > 0.0) {
+    xc.MdVdw = 0;
+    xc.Cvdw = S.vwmol;
+  }
+  return xc;
+}
+
+// XlineDepth (src/opacity.f90:239-293), k 0-based
+__device__ __forceinline__ void xline_depth(const DevSetup &S, int k, const XC &xc, double &cop, double &vv,
+                                            double &av) {
+  const int N = S.ntau;
+  double T = S.temp[k];
+  double Xstim = 1.0 - exp(-xc.Hnux / S.xln[4 * N + k]);
+  double Efract = exp(-xc.Elo / S.xln[5 * N + k]);
+  double Dopl = sqrt(T * xc.Xmass + S.vturb2);
+  double Dldopl = xc.WaveL * Dopl;
+  double Dndopl = Dopl / xc.WaveL;
+  double Gqst, Gvdw;
+  switch (xc.MdQst) {
+    case 0: Gqst = xc.Cqst * S.xln[2 * N + k]; break;
+    case 1: Gqst = xc.Cqst * S.xne[k]; break;
+    case 2: Gqst = xc.Cqst * S.xne[k] / xc.Qdiv; break;
+    case 3: Gqst = S.he_gam[(size_t)(xc.IxHe - 1) * N + k]; break;
+    default: Gqst = 0.0;
+  }
+  switch (xc.MdVdw) {
+    case 0: Gvdw = xc.Cvdw * S.xln[3 * N + k]; break;
+    case 1: Gvdw = S.vwfudge * S.xln[1 * N + k] * xc.Cvdw; break;
+    case 2: {
+      double Vbar = sqrt(8.0 * kXlnKM0 * T / kPi * xc.AboVbc);
+      Gvdw = xc.AboPre * pow(Vbar / 1e4, xc.AboExp) * S.xln[0 * N + k];
+      break;
+    }
+    default: Gvdw = 0.0;
+  }
+  double Gamtot = xc.Grad + Gqst + Gvdw;
+  av = Gamtot / (Dndopl * kXlnC4Pi);
+  vv = 1.0 / Dldopl;
+  double c = S.xlnz * Efract * xc.Gf * Xstim / (S.rho[k] * Dndopl);
+  cop = c * (double)S.fpf[(size_t)k * S.pf_ld + (xc.IxPF - 1)];
+}
+
+// a-state of the table lookup (src/voigt.f90:86-93 and 113-123)
+__device__ __forceinline__ void voigt_astate(const DevSetup &S, double A, int &ia, double &ua) {
+  if (A < 8.0) {
+    if (A < S.a1) {
+      ia = 1;
+      ua = A * S.ia1;
+    } else {
+      ia = (int)(sqrt(A) * S.idt) + 1;
+      ua = (A - S.anode[ia - 1]) * S.aida[ia - 1];
+    }
+  } else {
+    ia = 0;
+    ua = 0.0;
+  }
+}
+
+// HVoigt in FP64 (src/voigt.f90:76-110) with a precomputed a-state
+__device__ __forceinline__ double hvoigt64(const DevSetup &S, double A, int ia, double ua, double V) {
+  double av = fabs(V);
+  if (ia > 0 && av < 8.0) {
+    double fv = av * S.idv;
+    int iv = (int)fv;
+    double tv = fv - (double)iv;
+    const float *t0 = S.vtab + (size_t)(ia - 1) * TAB_LD;
+    const float *t1 = t0 + TAB_LD;
+    double c0 = (double)t0[iv] + tv * ((double)t0[iv + 1] - (double)t0[iv]);
+    double c1 = (double)t1[iv] + tv * ((double)t1[iv + 1] - (double)t1[iv]);
+    return c0 + ua * (c1 - c0);
+  }
+  double a2 = A * A, v2 = av * av, s = v2 + a2;
+  if (s >= 196.0) return kSqrtPiInv * A * (s + 0.5) / (s * s - v2 + a2 + 0.25);
+  double p = v2 - a2 - 1.5, q = 2.0 * av * A;
+  double dr = av * p - A * q, di = av * q + A * p;
+  return kSqrtPiInv * ((p + 0.5) * di - q * dr) / (dr * dr + di * di);
+}
+
+__device__ __forceinline__ double hvoigt64_full(const DevSetup &S, double A, double V) {
+  int ia;
+  double ua;
+  voigt_astate(S, A, ia, ua);
+  return hvoigt64(S, A, ia, ua, V);
+}
+
+// WaveIdx (src/opacity.f90:594-608)
+__device__ __forceinline__ int wave_idx(const DevSetup &S, double W) {
+  if (S.wmode == 0) {
+    double X = fmin(fmax(W / S.dstep, 0.0), (double)(S.nwl - 2));
+    return (int)X + 1;
+  }
+  double X = 1.0 + log(fmax((W / S.wstart) + 1.0, 2.2250738585072014e-308)) / S.rstep;
+  return (int)fmin(fmax(X, 1.0), (double)(S.nwl - 1));
+}
+
+// XCD-aware tile order: consecutive block ids go round-robin over the 8 XCDs, so map
+// them so that each XCD works on a contiguous run of tiles (shared lines stay in its L2).
+// The grid holds 8 * per blocks (per = ceil(ntiles / 8)), so bid -> x * per + r is a
+// bijection onto [0, 8 * per) and every tile below ntiles is visited exactly once.
+__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
+  const int NX = 8;
+  int per = nblk / NX;
+  int x = bid % NX, r = bid / NX;
+  return x * per + r;
+}
+
+__device__ __forceinline__ void kahan_add(float &acc, float &cmp, float val) {
+  float y = val - cmp;
+  float t = acc + y;
+  cmp = (t - acc) - y;
+  acc = t;
+}
+
+// CentDer (src/solver.f90:618-647) interior element from the two adjacent steps
+__device__ __forceinline__ double cent_der_i(double Dx, double Dx1, double ym, double y0, double yp) {
+  double InvDx = 1.0 / Dx, InvDx1 = 1.0 / Dx1;
+  double Lambda = (1.0 + Dx1 / (Dx + Dx1)) / 3.0;
+  double Lambda1 = 1.0 - Lambda;
+  double Der = (y0 - ym) * InvDx, Der1 = (yp - y0) * InvDx1;
+  return (Der * Der1 > 0.0) ? (Der / (Lambda * Der1 + Lambda1 * Der)) * Der1 : 0.0;
+}
+
+// DELO coefficients (src/solver.f90:154-169)
+__device__ __forceinline__ void delo_coef(double Dtau, double &Eps, double &Alpha, double &Beta, double &Gamma) {
+  double Dtau2 = Dtau * Dtau;
+  double Idtau2 = 1.0 / Dtau2;
+  if (Dtau < 5e-3) {
+    Eps = 1.0 - Dtau + 0.5 * Dtau2 - Dtau2 * Dtau / 6.0 + Dtau2 * Dtau2 / 24.0;
+    Alpha = Dtau / 3.0 - Dtau2 / 12.0 + Dtau2 * Dtau / 60.0;
+    Beta = Dtau / 3.0 - Dtau2 * 0.25 + Dtau2 * Dtau * 0.1;
+    Gamma = Dtau / 3.0 - Dtau2 / 6.0 + Dtau2 * Dtau * 0.05;
+  } else {
+    double E = (Dtau < 100.0) ? exp(-Dtau) : 0.0;
+    Eps = E;
+    Alpha = (Dtau2 - 2.0 * Dtau + 2.0 - 2.0 * E) * Idtau2;
+    Beta = (2.0 - (2.0 + 2.0 * Dtau + Dtau2) * E) * Idtau2;
+    Gamma = (2.0 * Dtau - 4.0 + (2.0 * Dtau + 4.0) * E) * Idtau2;
+  }
+}
+
+__device__ __forceinline__ double planck_src(double C1, double C2, double T) { return C2 / (exp(C1 / T) - 1.0); }
+
+}  // namespace fsg

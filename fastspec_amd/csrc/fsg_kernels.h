@@ -26,8 +26,8 @@ void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t 
 void launch_fill64(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                    const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
                    int4 *meta_s, hipStream_t st);
-void launch_tile_ranges(int ntiles, int nbins, int nwl, const int *bin_off, const int *bin_cnt, const int *bin_hmax,
-                        const int4 *meta_s, int2 *rng, hipStream_t st);
+void launch_tile_ranges(int ntiles, int tile, int nbins, int nwl, const int *bin_off, const int *bin_cnt,
+                        const int *bin_hmax, const int4 *meta_s, int2 *rng, hipStream_t st);
 void launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
                       const CoefB *cb, const double *wlc_s, const int4 *meta_s, float *lineop, int he, hipStream_t st);
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
@@ -38,5 +38,19 @@ void launch_radtran32(const DevSetup &S, const float *lop, double *spl, double *
                       hipStream_t st);
 void launch_radtran64(const DevSetup &S, const double *lop, double *spl, double *spc, double *sfl, double *sfc,
                       hipStream_t st);
+
+// polarised path (fsg_magnetic.hip)
+void launch_fvoigt_table(float2 *tab, const double *anode, double dv, hipStream_t st);
+void launch_magpass1(const DevSetup &S, const DevLines &L, const MagLine &Z, double bsplit, int *flag, int *il,
+                     int *ir, int *ic, unsigned *key, int *bin_count, int *bin_hmax, int batched, hipStream_t st);
+size_t magcoef_size(int fp32);
+void launch_magfill(const DevSetup &S, const DevLines &L, const MagLine &Z, int64_t nret, const int *order,
+                    const int *flag, const int *il, const int *ir, const int *ic, void *coef, int fp32, double *wlc_s,
+                    int4 *meta_s, int4 *comp_s, hipStream_t st);
+void launch_magopacity(const DevSetup &S, const float2 *ftab, int ntiles, int nbins, int rstride, const int2 *rng,
+                       const void *coef, const double *wlc_s, const int4 *meta_s, const int4 *comp_s,
+                       const MagLine &Z, void *mlop, int fp32, hipStream_t st);
+void launch_stokes(const DevSetup &S, const void *lop, const void *mlop, int fp32, const double *geo, double *spz,
+                   double *sfl, hipStream_t st);
 
 }  // namespace fsg

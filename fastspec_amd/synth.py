@@ -4,7 +4,9 @@ The method names follow the reference's module procedures so that a caller of th
 reference finds the same stage sequence (src/fastspec.f90:241-461):
 
     LineopCalc(lines, NlinesLim)        src/synth.f90:747   (non-magnetic line opacity)
+    MagLineopCalc(lines, NlinesLim, ..) src/synth.f90:1098  (Zeeman lists, MagLineOpac)
     RadTranCalc()                       src/synth.f90:1825  (scalar DELO-Bezier, SpL/SpC)
+    MagRadTranCalc(bfield, bframe)      src/synth.f90:1904  (Stokes DELO-Bezier, SpZ/SpC)
     IntegConvol(2) / DiskInteg flux     src/convol.f90:27   (flux sums, no convolution)
 
 Errors raised by the library carry the reference's ErrStop messages where it has one.
@@ -101,6 +103,39 @@ class FastSpecGPU:
         _lib.check(self._lib.fsg_lineop_dev(self.ctx, dlines.handle, max(nlim, 1), C.byref(st)))
         return st.as_dict()
 
+    def MagLineopCalc(self, lines: LineList, bfield, bframe: int, NlinesLim: Optional[int] = None,
+                      linstr: bool = False) -> dict:
+        """Accumulates one Zeeman list's six profiles into the device-resident MagLineOpac."""
+        n = lines.n
+        nlim = max(n if NlinesLim is None else int(NlinesLim), 1)
+        b = np.ascontiguousarray(bfield, np.float64)
+        flags = np.zeros(n, np.int32)
+        il = np.zeros(n, np.int32)
+        ir = np.zeros(n, np.int32)
+        ls = np.zeros(n, np.float32) if linstr else None
+        st = CStats()
+        cl = lines.cstruct()
+        _lib.check(self._lib.fsg_maglineop(self.ctx, C.byref(cl), nlim, ptr(b, C.c_double), int(bframe),
+                                           ptr(ls, C.c_float), ptr(flags, C.c_int32), ptr(il, C.c_int32),
+                                           ptr(ir, C.c_int32), C.byref(st)))
+        return dict(flags=flags, ileft=il, iright=ir, linstr=ls, stats=st.as_dict())
+
+    mag_lineop_calc = MagLineopCalc
+
+    def maglineop_dev(self, dlines: DeviceLines, bfield, bframe: int, NlinesLim: Optional[int] = None) -> dict:
+        nlim = dlines.n if NlinesLim is None else int(NlinesLim)
+        b = np.ascontiguousarray(bfield, np.float64)
+        st = CStats()
+        _lib.check(self._lib.fsg_maglineop_dev(self.ctx, dlines.handle, max(nlim, 1), ptr(b, C.c_double),
+                                               int(bframe), C.byref(st)))
+        return st.as_dict()
+
+    def get_maglineop(self) -> np.ndarray:
+        """MagLineOpac(6, Ntau, Nwl) as an array [nwl][ntau][6]."""
+        out = np.zeros((self.setup.nwl, self.setup.ntau, 6))
+        _lib.check(self._lib.fsg_get_maglineop(self.ctx, ptr(out, C.c_double)))
+        return out
+
     def reset_lineop(self):
         _lib.check(self._lib.fsg_reset_lineop(self.ctx))
 
@@ -123,6 +158,18 @@ class FastSpecGPU:
         return spl, spc
 
     radtran_calc = RadTranCalc
+
+    def MagRadTranCalc(self, bfield, bframe: int, intensities: bool = True):
+        """Stokes I, Q, U, V [nwl][4][nmu] and continuum intensities [nwl][nmu]."""
+        s = self.setup
+        b = np.ascontiguousarray(bfield, np.float64)
+        spz = np.zeros((s.nwl, 4, s.nmu)) if intensities else None
+        spc = np.zeros((s.nwl, s.nmu)) if intensities else None
+        _lib.check(self._lib.fsg_magradtran(self.ctx, ptr(b, C.c_double), b.shape[0], int(bframe),
+                                            ptr(spz, C.c_double), ptr(spc, C.c_double)))
+        return spz, spc
+
+    mag_radtran_calc = MagRadTranCalc
 
     def flux(self):
         s = self.setup

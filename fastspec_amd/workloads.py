@@ -286,12 +286,11 @@ def zeeman_pattern(jlo: float, jup: float, glo: float, gup: float):
     return P, S
 
 
-def magnetic_lines(n: int, wl_lo: float, wl_hi: float, seed: int) -> LineList:
-    """Seeded Zeeman list (mixed .z type): J <= 6, Lande g ~ U(0.5, 2.5); sigma components
-    stored blue (negated shifts) then red, as the reference's reader does
+def zeeman_patterns(n: int, seed: int):
+    """Seeded Zeeman patterns for n lines in CSR form: J <= 6, Lande g ~ U(0.5, 2.5); sigma
+    components stored blue (negated shifts) then red, as the reference's reader does
     (src/ioutils.f90:1012-1031)."""
-    base = atomic_lines(n, wl_lo, wl_hi, seed)
-    rng = np.random.Generator(np.random.PCG64(seed + 7))
+    rng = np.random.Generator(np.random.PCG64(seed))
     pi_off, si_off = [0], [0]
     spl_pi, str_pi, spl_si, str_si = [], [], [], []
     for _ in range(n):
@@ -315,12 +314,18 @@ def magnetic_lines(n: int, wl_lo: float, wl_hi: float, seed: int) -> LineList:
             str_si.append(r)
         pi_off.append(len(spl_pi))
         si_off.append(len(spl_si))
+    return dict(pi_off=np.array(pi_off, np.int32), spl_pi=np.array(spl_pi, np.float32),
+                str_pi=np.array(str_pi, np.float32), si_off=np.array(si_off, np.int32),
+                spl_si=np.array(spl_si, np.float32), str_si=np.array(str_si, np.float32))
+
+
+def magnetic_lines(n: int, wl_lo: float, wl_hi: float, seed: int) -> LineList:
+    """Seeded Zeeman list (mixed .z type) with the patterns of zeeman_patterns."""
+    base = atomic_lines(n, wl_lo, wl_hi, seed)
     return LineList(
         wl=base.wl, elo=base.elo, gf=base.gf, grad=base.grad, gqst=base.gqst, gvdw=base.gvdw,
         spc=base.spc, is_mol=np.zeros(n, np.uint8), list_type=FSG_LIST_MIXED,
-        pi_off=np.array(pi_off, np.int32), spl_pi=np.array(spl_pi, np.float32),
-        str_pi=np.array(str_pi, np.float32), si_off=np.array(si_off, np.int32),
-        spl_si=np.array(spl_si, np.float32), str_si=np.array(str_si, np.float32),
+        **zeeman_patterns(n, seed + 7),
     )
 
 

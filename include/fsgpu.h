@@ -173,10 +173,16 @@ int fsg_lines_upload(fsg_ctx *ctx, int32_t list_type, const fsg_lines *lines,
                      fsg_dlines **out);
 int fsg_lines_free(fsg_dlines *dl);
 int fsg_lineop_dev(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_stats *stats);
+/* MagLineopCalc (src/synth.f90:1098-1455) on a device list uploaded with its Zeeman
+ * patterns (lines->pi_off/si_off set); the list type is taken as FSG_LIST_MIXED. */
+int fsg_maglineop_dev(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, const double *bfield,
+                      int32_t bframe, fsg_lineop_stats *stats);
 /* Device pointers to the context's resident arrays (valid until fsg_finalize):
  *   which = 0 LineOpac [ntau][nwl] (lambda fastest; FP32 or FP64 by precision)
  *           1 SpL [nmu][nwl] FP64   2 SpC [nmu][nwl] FP64
- *           3 SfL [nwl] FP64        4 SfC [nwl] FP64 */
+ *           3 SfL [nwl] FP64        4 SfC [nwl] FP64 (Stokes I flux after fsg_magradtran)
+ *           5 MagLineOpac [6][ntau][nwl] (FP32 or FP64 by precision)
+ *           6 SpZ [4][nmu][nwl] FP64 */
 int fsg_device_ptr(fsg_ctx *ctx, int32_t which, void **ptr_out);
 /* Stream the library launches on (hipStream_t as void*). */
 int fsg_stream(fsg_ctx *ctx, void **stream_out);

@@ -57,3 +57,11 @@ def mixed_list(lines, seed=3):
 def molecular_list(lines):
     return LineList(wl=lines.wl, elo=lines.elo, gf=lines.gf, grad=lines.grad, spc=lines.spc,
                     list_type=FSG_LIST_MOLECULAR)
+
+
+def with_zeeman(lines, seed=11):
+    """The same lines as a Zeeman (.z-type) list with seeded patterns."""
+    z = W.zeeman_patterns(lines.n, seed)
+    is_mol = lines.is_mol if lines.is_mol is not None else np.zeros(lines.n, np.uint8)
+    return LineList(wl=lines.wl, elo=lines.elo, gf=lines.gf, grad=lines.grad, spc=lines.spc,
+                    gqst=lines.gqst, gvdw=lines.gvdw, is_mol=is_mol, list_type=FSG_LIST_MIXED, **z)
