@@ -120,6 +120,35 @@ int alloc_zero(fsg_ctx *c, size_t bytes, void **out) {
 
 size_t lop_elem(const fsg_ctx *c) { return c->precision == FSG_FP32 ? sizeof(float) : sizeof(double); }
 
+// Lines per accumulation chunk: NlinesLim, lowered so that the chunk's per-line device
+// buffers (per_line bytes each) fit in 85 % of the free device memory plus what the
+// context's reusable buffers already hold.
+int64_t chunk_lines(fsg_ctx *c, int64_t nret, int64_t nlim, size_t per_line, int magnetic) {
+  int64_t m = std::min(nret, nlim);
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+    size_t avail = fr + c->coef.n + c->wlc.n + c->meta.n + (magnetic ? c->comp.n : 0);
+    int64_t fit = (int64_t)((double)avail * 0.85 / (double)per_line);
+    m = std::min(m, fit);
+  }
+  return std::max<int64_t>(m, 1);
+}
+
+// group tables of one chunk [s0, s1) of the bin-sorted retained lines: per group the offset
+// of its overlap with the chunk (chunk-relative), the overlap's size and the group's
+// maximal window half-width
+void chunk_tables(const std::vector<int64_t> &goff, const std::vector<int64_t> &gcnt, const std::vector<int> &ghmax,
+                  int64_t s0, int64_t s1, std::vector<int> &out) {
+  const size_t ng = goff.size();
+  out.assign(3 * ng, 0);
+  for (size_t b = 0; b < ng; b++) {
+    const int64_t lo = std::max(goff[b], s0), hi = std::min(goff[b] + gcnt[b], s1);
+    out[b] = hi > lo ? (int)(lo - s0) : 0;
+    out[ng + b] = hi > lo ? (int)(hi - lo) : 0;
+    out[2 * ng + b] = ghmax[b];
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -415,7 +444,6 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
                                         c->idx.as<int>(), c->idx2.as<int>(), (int)n, 0, 6, s));
   int hcnt[64], hhmax[64];
   unsigned long long hu[4];
-  std::vector<int> packed;  // bin tables staged for upload; alive until the final sync
   CK(hipMemcpyAsync(hcnt, c->bincnt, sizeof(hcnt), hipMemcpyDeviceToHost, s));
   CK(hipMemcpyAsync(hhmax, c->binhmax, sizeof(hhmax), hipMemcpyDeviceToHost, s));
   CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
@@ -449,59 +477,73 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   launch_wlc_list(n, dl->L.wl, S.wfirst, c->flag.as<int>(), c->pos.as<int>(), c->wlcl.as<double>(), pack, s);
   launch_mark_wide(S, n, dl->L.wl, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), c->wlcl.as<double>(),
                    pack ? nret : n, nstep, c->u64 + 1, s);
+  // Accumulation over chunks of the bin-sorted retained lines: the coefficient records of
+  // at most `chunk` lines are resident at once (the device analogue of the reference's
+  // NlinesStep blocks, src/synth.f90:1011-1088).  Groups 0..nbins-1 are the width bins,
+  // group nbins the He I lines.
+  bool timed = false;  // ms_opacity runs from the first accumulation launch to the last
   if (nret > 0) {
-    // per group: offsets, counts and maximal half-widths of the candidate search;
-    // groups 0..nbins-1 are the width bins, group nbins the He I lines
+    const int ng = nbins + 1;
+    std::vector<int64_t> goff(ng), gcnt(ng);
+    std::vector<int> ghmax(ng);
     int64_t o = 0;
-    packed.assign(3 * (nbins + 1), 0);
-    for (int b = 0; b <= nbins; b++) {
+    for (int b = 0; b < ng; b++) {
       int key = b < nbins ? b : HE_KEY;
-      packed[b] = (int)o;
-      packed[(nbins + 1) + b] = hcnt[key];
-      packed[2 * (nbins + 1) + b] = hhmax[key];
+      goff[b] = o;
+      gcnt[b] = hcnt[key];
+      ghmax[b] = hhmax[key];
       o += hcnt[key];
     }
-    CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * packed.size(), hipMemcpyHostToDevice, s));
     const size_t csz = c->precision == FSG_FP32 ? sizeof(CoefA) + sizeof(CoefB) : sizeof(Coef64);
-    CK(c->coef.ensure((size_t)nret * S.ntp * csz));
-    CoefA *ca = c->coef.as<CoefA>();
-    CoefB *cb = reinterpret_cast<CoefB *>(ca + (size_t)nret * S.ntp);
-    CK(c->wlc.ensure((size_t)nret * sizeof(double)));
-    CK(c->meta.ensure((size_t)nret * sizeof(int4)));
-    if (c->precision == FSG_FP32)
-      launch_fill32(S, dl->L, dl->list_type, nret, c->idx2.as<int>(), c->flag.as<int>(), c->il.as<int>(),
-                    c->ir.as<int>(), c->ic.as<int>(), ca, cb, c->wlc.as<double>(), c->meta.as<int4>(), s);
-    else
-      launch_fill64(S, dl->L, dl->list_type, nret, c->idx2.as<int>(), c->flag.as<int>(), c->il.as<int>(),
-                    c->ir.as<int>(), c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
+    const int64_t chunk = chunk_lines(c, nret, nlines_lim, S.ntp * csz + sizeof(double) + sizeof(int4), 0);
+    CK(c->coef.ensure((size_t)chunk * S.ntp * csz));
+    CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
+    CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
     const int ntiles = (S.nwl + TILE - 1) / TILE;
-    const int ng = nbins + 1;
     CK(c->rng.ensure((size_t)ntiles * ng * sizeof(int2)));
-    launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng, c->meta.as<int4>(),
-                       c->rng.as<int2>(), s);
-    CK(hipEventRecord(c->ev[2], s));
-    // the range table has ng groups per tile: the first nbins are read by the main launch
-    // (stride ng), the last one by the He launch (offset nbins, stride ng)
-    if (c->precision == FSG_FP32) {
-      if (nmain > 0)
-        launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, c->wlc.as<double>(),
-                         c->meta.as<int4>(), (float *)c->lineop, 0, s);
-      if (nhe > 0)
-        launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, c->wlc.as<double>(),
-                         c->meta.as<int4>(), (float *)c->lineop, 1, s);
-    } else {
-      if (nmain > 0)
-        launch_opacity64(S, ntiles, nbins, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), c->wlc.as<double>(),
-                         c->meta.as<int4>(), (double *)c->lineop, s);
-      if (nhe > 0)
-        launch_opacity64(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, c->coef.as<Coef64>(), c->wlc.as<double>(),
-                         c->meta.as<int4>(), (double *)c->lineop, s);
+    std::vector<std::vector<int>> tables;  // staged for upload; alive until the final sync
+    for (int64_t s0 = 0; s0 < nret; s0 += chunk) {
+      const int64_t s1 = std::min(nret, s0 + chunk), m = s1 - s0;
+      tables.emplace_back();
+      std::vector<int> &packed = tables.back();
+      chunk_tables(goff, gcnt, ghmax, s0, s1, packed);
+      CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * packed.size(), hipMemcpyHostToDevice, s));
+      CoefA *ca = c->coef.as<CoefA>();
+      CoefB *cb = reinterpret_cast<CoefB *>(ca + (size_t)m * S.ntp);
+      const int *order = c->idx2.as<int>() + s0;
+      if (c->precision == FSG_FP32)
+        launch_fill32(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
+                      c->ic.as<int>(), ca, cb, c->wlc.as<double>(), c->meta.as<int4>(), s);
+      else
+        launch_fill64(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
+                      c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
+      launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
+                         c->meta.as<int4>(), c->rng.as<int2>(), s);
+      // the range table has ng groups per tile: the first nbins are read by the main launch
+      // (stride ng), the last one by the He launch (offset nbins, stride ng)
+      const bool has_main = s0 < goff[nbins], has_he = s1 > goff[nbins];
+      if (!timed) CK(hipEventRecord(c->ev[2], s));
+      timed = true;
+      if (c->precision == FSG_FP32) {
+        if (has_main && nbins > 0)
+          launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, c->wlc.as<double>(),
+                           c->meta.as<int4>(), (float *)c->lineop, 0, s);
+        if (has_he)
+          launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, c->wlc.as<double>(),
+                           c->meta.as<int4>(), (float *)c->lineop, 1, s);
+      } else {
+        if (has_main && nbins > 0)
+          launch_opacity64(S, ntiles, nbins, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), c->wlc.as<double>(),
+                           c->meta.as<int4>(), (double *)c->lineop, s);
+        if (has_he)
+          launch_opacity64(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, c->coef.as<Coef64>(),
+                           c->wlc.as<double>(), c->meta.as<int4>(), (double *)c->lineop, s);
+      }
+      st.nblocks++;
     }
-    CK(hipEventRecord(c->ev[3], s));
-  } else {
-    CK(hipEventRecord(c->ev[2], s));
-    CK(hipEventRecord(c->ev[3], s));
   }
+  if (!timed) CK(hipEventRecord(c->ev[2], s));
+  CK(hipEventRecord(c->ev[3], s));
   CK(hipGetLastError());
   CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
@@ -631,7 +673,6 @@ int fsg_maglineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, const doub
                                         c->idx.as<int>(), c->idx2.as<int>(), (int)n, 0, 6, s));
   int hcnt[64], hhmax[64];
   unsigned long long hu[4];
-  std::vector<int> packed;
   CK(hipMemcpyAsync(hcnt, c->bincnt, sizeof(hcnt), hipMemcpyDeviceToHost, s));
   CK(hipMemcpyAsync(hhmax, c->binhmax, sizeof(hhmax), hipMemcpyDeviceToHost, s));
   CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
@@ -663,34 +704,46 @@ int fsg_maglineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, const doub
   launch_mark_wide(S, n, dl->L.wl, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), c->wlcl.as<double>(),
                    pack ? nret : n, nstep, c->u64 + 1, s);
   const int fp32 = c->precision == FSG_FP32;
+  bool timed = false;
   if (nret > 0) {
+    std::vector<int64_t> goff(nbins), gcnt(nbins);
+    std::vector<int> ghmax(nbins);
     int64_t o = 0;
-    packed.assign(3 * nbins, 0);
     for (int b = 0; b < nbins; b++) {
-      packed[b] = (int)o;
-      packed[nbins + b] = hcnt[b];
-      packed[2 * nbins + b] = hhmax[b];
+      goff[b] = o;
+      gcnt[b] = hcnt[b];
+      ghmax[b] = hhmax[b];
       o += hcnt[b];
     }
-    CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * packed.size(), hipMemcpyHostToDevice, s));
-    CK(c->coef.ensure((size_t)nret * S.ntp * magcoef_size(fp32)));
-    CK(c->wlc.ensure((size_t)nret * sizeof(double)));
-    CK(c->meta.ensure((size_t)nret * sizeof(int4)));
-    CK(c->comp.ensure((size_t)nret * sizeof(int4)));
-    launch_magfill(S, dl->L, dl->Z, nret, c->idx2.as<int>(), c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
-                   c->ic.as<int>(), c->coef.p, fp32, c->wlc.as<double>(), c->meta.as<int4>(), c->comp.as<int4>(), s);
+    const size_t rec = magcoef_size(fp32);
+    const int64_t chunk = chunk_lines(c, nret, nlines_lim, S.ntp * rec + sizeof(double) + 2 * sizeof(int4), 1);
+    CK(c->coef.ensure((size_t)chunk * S.ntp * rec));
+    CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
+    CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
+    CK(c->comp.ensure((size_t)chunk * sizeof(int4)));
     const int ntiles = (S.nwl + MTILE - 1) / MTILE;
     CK(c->rng.ensure((size_t)ntiles * nbins * sizeof(int2)));
-    launch_tile_ranges(ntiles, MTILE, nbins, S.nwl, c->binoff, c->binoff + nbins, c->binoff + 2 * nbins,
-                       c->meta.as<int4>(), c->rng.as<int2>(), s);
-    CK(hipEventRecord(c->ev[2], s));
-    launch_magopacity(S, c->ftab, ntiles, nbins, nbins, c->rng.as<int2>(), c->coef.p, c->wlc.as<double>(),
-                      c->meta.as<int4>(), c->comp.as<int4>(), dl->Z, c->maglineop, fp32, s);
-    CK(hipEventRecord(c->ev[3], s));
-  } else {
-    CK(hipEventRecord(c->ev[2], s));
-    CK(hipEventRecord(c->ev[3], s));
+    std::vector<std::vector<int>> tables;  // staged for upload; alive until the final sync
+    for (int64_t s0 = 0; s0 < nret; s0 += chunk) {
+      const int64_t s1 = std::min(nret, s0 + chunk), m = s1 - s0;
+      tables.emplace_back();
+      chunk_tables(goff, gcnt, ghmax, s0, s1, tables.back());
+      CK(hipMemcpyAsync(c->binoff, tables.back().data(), sizeof(int) * tables.back().size(), hipMemcpyHostToDevice,
+                        s));
+      launch_magfill(S, dl->L, dl->Z, m, c->idx2.as<int>() + s0, c->flag.as<int>(), c->il.as<int>(),
+                     c->ir.as<int>(), c->ic.as<int>(), c->coef.p, fp32, c->wlc.as<double>(), c->meta.as<int4>(),
+                     c->comp.as<int4>(), s);
+      launch_tile_ranges(ntiles, MTILE, nbins, S.nwl, c->binoff, c->binoff + nbins, c->binoff + 2 * nbins,
+                         c->meta.as<int4>(), c->rng.as<int2>(), s);
+      if (!timed) CK(hipEventRecord(c->ev[2], s));
+      timed = true;
+      launch_magopacity(S, c->ftab, ntiles, nbins, nbins, c->rng.as<int2>(), c->coef.p, c->wlc.as<double>(),
+                        c->meta.as<int4>(), c->comp.as<int4>(), dl->Z, c->maglineop, fp32, s);
+      st.nblocks++;
+    }
   }
+  if (!timed) CK(hipEventRecord(c->ev[2], s));
+  CK(hipEventRecord(c->ev[3], s));
   CK(hipGetLastError());
   CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));

@@ -105,8 +105,10 @@ typedef struct fsg_lineop_stats {
   int64_t nlines_in;             /* Nlines on entry                                  */
   int64_t nlines_use;            /* NlinesUse: retained lines                        */
   int64_t nwide;                 /* Nwide                                            */
-  int64_t nblocks;               /* coefficient block fills                          */
-  int64_t nsegments;             /* wavelength segments                              */
+  int64_t nblocks;               /* coefficient block fills: the reference's block
+                                    refills; on the device the line chunks of at most
+                                    nlines_lim coefficient records                   */
+  int64_t nsegments;             /* wavelength segments (1 on the device)            */
   int64_t evals;                 /* sum over retained lines of window pixels x Ntau  */
   double ms_pass1, ms_opacity;   /* device time of the two stages                    */
 } fsg_lineop_stats;

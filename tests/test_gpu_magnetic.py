@@ -156,6 +156,12 @@ def test_mag_blocks(small_setup_lines):
     ref = _oracle(setup, lines, B_INCLINED, 1, nlim=lines.n // 4)
     with _gpu(setup, FSG_FP64) as g:
         print(_check(ref, g, FP64_OPAC_REL, FP64_STOKES, FP64_FLUX_ABS))
+        # a second list in chunks of 7 lines adds the same opacity again
+        a = g.get_maglineop()
+        r = g.MagLineopCalc(lines, B_INCLINED, 1, NlinesLim=7)
+        assert r["stats"]["nblocks"] == -(-r["stats"]["nlines_use"] // 7)
+        b = g.get_maglineop()
+        assert np.max(np.abs(b - 2 * a) / (np.abs(a) + setup.contop[:, :, None])) <= 1e-11
 
 
 @pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])

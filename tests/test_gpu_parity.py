@@ -137,8 +137,28 @@ def test_multi_block_windows_and_opacity(prec):
         assert np.array_equal(rg["flags"], ref["ro"]["flags"])
         assert np.array_equal(rg["ileft"], ref["ro"]["ileft"])
         assert rg["stats"]["nwide"] == ref["ro"]["stats"]["nwide"]
+        # at most NlinesLim lines have resident coefficients: the retained lines go through
+        # the accumulation in ceil(NlinesUse / NlinesLim) chunks
+        assert rg["stats"]["nblocks"] == -(-rg["stats"]["nlines_use"] // nlim)
         lop = g.get_lineop()
         tol = FP64_OPAC_REL if prec == FSG_FP64 else FP32_OPAC_REL
+        assert np.max(np.abs(lop - ref["lop"]) / (ref["lop"] + setup.contop)) <= tol
+
+
+@pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])
+def test_chunks_across_width_bins_and_he_lines(prec):
+    """Line chunks of 37 records cut through width bins and the He I group; every cut
+    leaves the accumulated opacity unchanged."""
+    setup, lines = W.config_c1(600)
+    setup, lines = with_he_lines(setup, lines, nhe_lines=20, seed=41)
+    ref = _oracle_run(setup, lines)
+    tol = FP64_OPAC_REL if prec == FSG_FP64 else FP32_OPAC_REL
+    with _gpu(setup, prec) as g:
+        rg = g.LineopCalc(lines, NlinesLim=37)
+        assert np.array_equal(rg["flags"], ref["ro"]["flags"])
+        assert rg["stats"]["evals"] == ref["ro"]["stats"]["evals"]
+        assert rg["stats"]["nblocks"] == -(-rg["stats"]["nlines_use"] // 37)
+        lop = g.get_lineop()
         assert np.max(np.abs(lop - ref["lop"]) / (ref["lop"] + setup.contop)) <= tol
 
 
