@@ -47,6 +47,7 @@ class CSetup(C.Structure):
         ("molbrd", C.c_int32),
         ("xmu", _dp), ("wmu", _dp),
         ("precision", C.c_int32), ("batched", C.c_int32), ("device", C.c_int32),
+        ("radius", C.c_double), ("height", _dp),
     ]
 
 
@@ -133,6 +134,8 @@ class Setup:
     vwmol: float = 0.0
     vmicro: float = 0.0
     molbrd: bool = False
+    radius: float = -1.0                  # spherical models: Radius [cm] (<= 0: plane-parallel)
+    height: Optional[np.ndarray] = None   # [ntau] Atmos%Height [cm]
     # extra host data that is not part of the C structure (kept for writers and tests)
     meta: dict = field(default_factory=dict)
 
@@ -146,7 +149,7 @@ class Setup:
             setattr(self, k, _f32(getattr(self, k)))
         self.spc_ion = _i32(self.spc_ion)
         self.spc_slot = _i32(self.spc_slot)
-        for k in ("he_wl", "he_elo", "he_gam", "he_shf"):
+        for k in ("he_wl", "he_elo", "he_gam", "he_shf", "height"):
             setattr(self, k, _f64(getattr(self, k)))
 
     @property
@@ -192,6 +195,7 @@ class Setup:
             vwmol=self.vwmol, vmicro=self.vmicro, molbrd=int(bool(self.molbrd)),
             xmu=ptr(self.xmu, C.c_double), wmu=ptr(self.wmu, C.c_double),
             precision=int(precision), batched=int(batched), device=int(device),
+            radius=float(self.radius), height=ptr(self.height, C.c_double),
         )
         return cs
 

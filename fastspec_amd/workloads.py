@@ -146,10 +146,13 @@ def wave_grid(wstart: float, wend: float, step: float, wmode: int):
 
 def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode: int,
                 species=ATOMIC_SPECIES, nmu: int = 7, vmicro: float = 1.0,
-                motype: int = 0, eps_opac: float = 1e-4, extra_slots: int = 4) -> Setup:
+                motype: int = 0, eps_opac: float = 1e-4, extra_slots: int = 4,
+                radius: float = -1.0) -> Setup:
     """Host-side inputs (EOS tables, continuum) on the given grid.
 
-    EOS slots: 1..len(species) are the line species, then H I, He I, H2, H II, He II."""
+    EOS slots: 1..len(species) are the line species, then H I, He I, H2, H II, He II.
+    radius > 0 makes a spherical model: layer heights from dh = -dm / rho, zero at
+    tau_5000 = 1."""
     ntau = atm.temp.shape[0]
     wave, wfirst, dstep, rstep = wave_grid(wstart, wend, step, wmode)
     lam = wave + wfirst
@@ -193,6 +196,13 @@ def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode:
         contop = contop / contop_std[None, :]
         rhox = atm.tau.copy()
     xmu, wmu = gauss_mu(nmu)
+    height = None
+    if radius > 0:
+        m, r = atm.rhox, atm.rho
+        h = np.zeros(ntau)
+        for k in range(1, ntau):
+            h[k] = h[k - 1] - (m[k] - m[k - 1]) / (0.5 * (r[k] + r[k - 1]))
+        height = h - np.interp(0.0, np.log10(atm.tau), h)
     return Setup(
         wave=wave, wmode=wmode, wfirst=wfirst, dstep=dstep, rstep=rstep, wstart=wstart,
         rhox=rhox, temp=t, xne=ne, rho=atm.rho, contop=contop, contop_std=contop_std,
@@ -202,6 +212,7 @@ def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode:
         spc_ion=np.array([s[2] for s in species], np.int32),
         spc_slot=np.arange(1, nsp + 1, dtype=np.int32),
         he1_slot=nsp + 2, xmu=xmu, wmu=wmu, motype=motype, eps_opac=eps_opac, vmicro=vmicro,
+        radius=radius, height=height,
         meta=dict(atmosphere=atm, species=[s[0] for s in species]),
     )
 

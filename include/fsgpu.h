@@ -81,6 +81,11 @@ typedef struct fsg_setup {
   int32_t batched;               /* 1: lines binned by window width (batched path);
                                     0: one bin, the reference's index-range scan     */
   int32_t device;                /* HIP device ordinal                               */
+  /* spherical models (Radius, Atmos%Height: src/shared.f90:520,527); the ray geometry
+   * of SphereInit (src/solver.f90:45-120) is built by fsg_init */
+  double radius;                 /* Radius [cm]; <= 0 selects plane-parallel transfer */
+  const double *height;          /* [ntau] layer heights [cm], decreasing inward;
+                                    NULL when radius <= 0                           */
 } fsg_setup;
 
 /* One line list in structure-of-arrays form, after the host's reader conversions

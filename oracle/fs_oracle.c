@@ -556,6 +556,11 @@ typedef struct fso_ctx {
   double *maglineop;                /* [nwl][ntau][6] MagLineOpac */
   double *spl, *spc, *spz;          /* [nwl][nmu], [nwl][nmu], [nwl][4][nmu] */
   int magnetic;
+  /* spherical ray geometry (SphereInit): per mu, points 1..ntau_mu along the ray */
+  double radius;
+  int *ntau_mu, *graz_mu;           /* [nmu] */
+  int *map_mu;                      /* [nmu][2 ntau] layer index (1-based) of each point */
+  double *rhox_mu, *muloc;          /* [nmu][2 ntau] */
 } fso_ctx;
 
 static char g_err[512];
@@ -606,6 +611,94 @@ static void xline_init(fso_ctx *c) {
     c->XlnKbeT[k] = kKbolt * kErg2ev * Temp;
   }
 }
+
+/* SphereInit (src/solver.f90:45-120): rays with impact parameter p = R_top sqrt(1 - mu^2);
+ * the column-mass steps are scaled by (r_k + r_k-1) / (z_k + z_k-1), z = sqrt(r^2 - p^2);
+ * grazing rays (p above the deepest layer) cross the tangent shell and revisit the crossed
+ * layers in reverse order */
+static int sphere_init(fso_ctx *c, const double *height) {
+  const int N = c->ntau, M = c->nmu, L = 2 * N;
+  double *Rk = malloc(sizeof(double) * N);
+  for (int k = 0; k < N; k++) Rk[k] = c->radius + height[k];
+  c->ntau_mu = calloc(M, sizeof(int));
+  c->graz_mu = calloc(M, sizeof(int));
+  c->map_mu = calloc((size_t)M * L, sizeof(int));
+  c->rhox_mu = calloc((size_t)M * L, sizeof(double));
+  c->muloc = calloc((size_t)M * L, sizeof(double));
+  int rc = 0;
+  for (int im = 0; im < M && !rc; im++) {
+    double xmu = c->xmu[im];
+    if (xmu <= 0.0) {
+      snprintf(g_err, sizeof(g_err), "SphereInit: mu = 0 rays are unsupported for spherical models");
+      rc = 1;
+      break;
+    }
+    double P = Rk[0] * sqrt((1.0 - xmu) * (1.0 + xmu));
+    int graz = P > Rk[N - 1];
+    int n = N;
+    if (graz) {
+      int kt;
+      for (kt = 2; kt <= N; kt++)
+        if (P >= Rk[kt - 1]) break;
+      n = kt - 1;
+      if (n < 2) {
+        snprintf(g_err, sizeof(g_err),
+                 "SphereInit: grazing ray tangent in the top model layer; remove extreme mu values close to 0");
+        rc = 1;
+        break;
+      }
+    }
+    c->graz_mu[im] = graz;
+    int *map = c->map_mu + (size_t)im * L;
+    double *rm = c->rhox_mu + (size_t)im * L, *ml = c->muloc + (size_t)im * L;
+    rm[0] = c->rhox[0] / xmu;
+    double Zk1 = sqrt((Rk[0] - P) * (Rk[0] + P));
+    ml[0] = Zk1 / Rk[0];
+    map[0] = 1;
+    for (int k = 2; k <= n; k++) {
+      double Zk = sqrt((Rk[k - 1] - P) * (Rk[k - 1] + P));
+      rm[k - 1] = rm[k - 2] + (c->rhox[k - 1] - c->rhox[k - 2]) * (Rk[k - 1] + Rk[k - 2]) / (Zk + Zk1);
+      ml[k - 1] = Zk / Rk[k - 1];
+      map[k - 1] = k;
+      Zk1 = Zk;
+    }
+    if (!graz) {
+      c->ntau_mu[im] = N;
+      continue;
+    }
+    double Chord = 2.0 * sqrt((Rk[n - 1] - P) * (Rk[n - 1] + P));
+    rm[n] = rm[n - 1] + Chord * (c->rhox[n] - c->rhox[n - 1]) / (Rk[n - 1] - Rk[n]);
+    map[n] = n;
+    ml[n] = -ml[n - 1];
+    for (int k = n + 2; k <= 2 * n; k++) {
+      rm[k - 1] = rm[k - 2] + (rm[2 * n - k + 1] - rm[2 * n - k]);
+      map[k - 1] = 2 * n + 1 - k;
+      ml[k - 1] = -ml[2 * n - k];
+    }
+    c->ntau_mu[im] = 2 * n;
+  }
+  free(Rk);
+  return rc;
+}
+
+/* ray geometry for tests: ntau_mu/graz [nmu], map/rhox_mu/muloc [nmu][2 ntau] */
+EXPORT int fso_sphere_geometry(fso_ctx *c, int32_t *ntau_mu, int32_t *graz, int32_t *map, double *rhox_mu,
+                               double *muloc) {
+  if (c->radius <= 0.0) return 1;
+  size_t L = (size_t)2 * c->ntau;
+  for (int im = 0; im < c->nmu; im++) {
+    ntau_mu[im] = c->ntau_mu[im];
+    graz[im] = c->graz_mu[im];
+  }
+  for (size_t i = 0; i < (size_t)c->nmu * L; i++) {
+    map[i] = c->map_mu[i];
+    rhox_mu[i] = c->rhox_mu[i];
+    muloc[i] = c->muloc[i];
+  }
+  return 0;
+}
+
+EXPORT int fso_finalize(fso_ctx *c);
 
 EXPORT int fso_init(const fsg_setup *s, fso_ctx **out) {
   init_constants();
@@ -658,6 +751,18 @@ EXPORT int fso_init(const fsg_setup *s, fso_ctx **out) {
   c->wmu = dup_d(s->wmu, s->nmu);
   c->lineop = calloc((size_t)s->nwl * s->ntau, sizeof(double));
   xline_init(c);
+  c->radius = s->radius;
+  if (c->radius > 0.0 && s->nmu > 0) {
+    if (!s->height) {
+      snprintf(g_err, sizeof(g_err), "fso_init: a spherical model needs layer heights");
+      fso_finalize(c);
+      return 1;
+    }
+    if (sphere_init(c, s->height)) {
+      fso_finalize(c);
+      return 1;
+    }
+  }
   *out = c;
   return 0;
 }
@@ -669,7 +774,8 @@ EXPORT int fso_finalize(fso_ctx *c) {
                   c->frac_hmol, c->spc_ion,   c->spc_slot,  c->he_wl,     c->he_elo,    c->he_gam,
                   c->he_shf,    c->xmu,       c->wmu,       c->XlnNprtb,  c->XlnGvwprt, c->XlnTemp6,
                   c->XlnTemp3,  c->XlnKbT,    c->XlnKbeT,   c->lineop,    c->maglineop, c->spl,
-                  c->spc,       c->spz};
+                  c->spc,       c->spz,       c->ntau_mu,   c->graz_mu,   c->map_mu,    c->rhox_mu,
+                  c->muloc};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
   free(c);
   return 0;
@@ -985,15 +1091,60 @@ EXPORT void fso_locprfpp(double *Ploc, const double *Xmu, int Nmu, const double 
   free(U);
 }
 
-/* LinStrCalc (src/synth.f90:1774-1822), plane-parallel */
-static float lin_str_calc(const fso_ctx *c, double Wl, const double *LineOpacL, const double *ContOpacL) {
+/* DeloScalar (src/solver.f90:219-261): DELO-Bezier integration along one ray from the
+ * boundary intensity Y0 */
+static double delo_scalar(const double *Deltau, const double *Src, int N, double Y0) {
+  double *Srd = malloc(sizeof(double) * N);
+  cent_der(Deltau, Src, 1, N, Srd);
+  double Y = Y0;
+  for (int k = N - 1; k >= 1; k--) {
+    double Dtau = Deltau[k - 1], Eps, Alpha, Beta, Gamma;
+    delo_coef(Dtau, &Eps, &Alpha, &Beta, &Gamma);
+    double U = 0.5 * (Src[k - 1] + 0.5 * Dtau * Srd[k - 1] + Src[k] - 0.5 * Dtau * Srd[k]);
+    Y = Y * Eps + Alpha * Src[k - 1] + Beta * Src[k] + Gamma * U;
+  }
+  free(Srd);
+  return Y;
+}
+
+/* LocPrfSph (src/solver.f90:182-215): per mu, opacity and source gathered along the ray,
+ * optical depth steps on the ray's column mass (no division by mu); grazing rays start
+ * from zero, the others from S at the deepest point */
+EXPORT void fso_locprfsph(fso_ctx *c, double *Ploc, const double *Xki, const double *Src) {
+  int L = 2 * c->ntau;
+  double *XkiR = malloc(sizeof(double) * L), *SrcR = malloc(sizeof(double) * L);
+  double *Deltau = malloc(sizeof(double) * L);
+  for (int im = 0; im < c->nmu; im++) {
+    int N = c->ntau_mu[im];
+    const int *map = c->map_mu + (size_t)im * L;
+    for (int k = 0; k < N; k++) {
+      XkiR[k] = Xki[map[k] - 1];
+      SrcR[k] = Src[map[k] - 1];
+    }
+    tau_step(c->rhox_mu + (size_t)im * L, XkiR, 1.0, N, Deltau);
+    double Y = c->graz_mu[im] ? 0.0 : SrcR[N - 1];
+    Ploc[im] = delo_scalar(Deltau, SrcR, N, Y);
+  }
+  free(XkiR);
+  free(SrcR);
+  free(Deltau);
+}
+
+/* emergent intensities of one opacity column in the model's geometry */
+static void locprf(fso_ctx *c, double *Ploc, const double *Xki, const double *Src) {
+  if (c->radius > 0.0) fso_locprfsph(c, Ploc, Xki, Src);
+  else fso_locprfpp(Ploc, c->xmu, c->nmu, Xki, Src, c->rhox, c->ntau);
+}
+
+/* LinStrCalc (src/synth.f90:1774-1822) */
+static float lin_str_calc(fso_ctx *c, double Wl, const double *LineOpacL, const double *ContOpacL) {
   int N = c->ntau, M = c->nmu;
   double *Src = malloc(sizeof(double) * N), *X = malloc(sizeof(double) * N);
   double *IntL = malloc(sizeof(double) * M), *IntC = malloc(sizeof(double) * M);
   planck(c, Wl, Src);
-  fso_locprfpp(IntC, c->xmu, M, ContOpacL, Src, c->rhox, N);
+  locprf(c, IntC, ContOpacL, Src);
   for (int k = 0; k < N; k++) X[k] = ContOpacL[k] + LineOpacL[k];
-  fso_locprfpp(IntL, c->xmu, M, X, Src, c->rhox, N);
+  locprf(c, IntL, X, Src);
   float res;
   if (M == 1) {
     res = (float)(1.0 - IntL[0] / IntC[0]);
@@ -1529,9 +1680,9 @@ EXPORT int fso_radtran(fso_ctx *c, double *spl_out, double *spc_out) {
   for (int iwl = 1; iwl <= Nwl; iwl++) {
     planck(c, WAVE(iwl) + c->wfirst, Src);
     for (int k = 1; k <= N; k++) Xki[k - 1] = CONT(k, iwl);
-    fso_locprfpp(c->spc + (size_t)(iwl - 1) * M, c->xmu, M, Xki, Src, c->rhox, N);
+    locprf(c, c->spc + (size_t)(iwl - 1) * M, Xki, Src);
     for (int k = 1; k <= N; k++) Xki[k - 1] = CONT(k, iwl) + LOP(k, iwl);
-    fso_locprfpp(c->spl + (size_t)(iwl - 1) * M, c->xmu, M, Xki, Src, c->rhox, N);
+    locprf(c, c->spl + (size_t)(iwl - 1) * M, Xki, Src);
   }
   free(Src);
   free(Xki);
@@ -1595,90 +1746,120 @@ static void mat_vec(const double *A, const double *x, double *y) {
   }
 }
 
-/* StokesPrfPP (src/solver.f90:265-394) for one wavelength; Stokes[4][nmu] */
-EXPORT void fso_stokes_prf_pp(fso_ctx *c, int iwl, const double *Src, const double *SPhi2, const double *CPhi2,
-                              const double *CPhi, const double *S2Psi, const double *C2Psi, double *Stokes) {
-  int N = c->ntau, Nmu = c->nmu;
+/* The polarised DELO-Bezier integration along one ray of N points (the body shared by
+ * StokesPrfPP src/solver.f90:265-394 and StokesPrfSph src/solver.f90:402-538).  Point k
+ * (0-based) samples layer map[k] (1-based; identity when map is NULL) at column mass
+ * rhoxR[k]; the optical depth steps use Xmu (1 on spherical rays, whose column mass is
+ * already measured along the ray); geometry g[k * gs + 0..4] = SPhi2, CPhi2, CPhi, S2Psi,
+ * C2Psi (gs = 0 for depth-independent angles).  Grazing rays start from a zero Stokes
+ * vector, the others from the unpolarised S at the deepest point.  out4 = I, Q, U, V. */
+static void stokes_ray(fso_ctx *c, int iwl, int N, const int *map, const double *rhoxR, double Xmu, const double *g,
+                       size_t gs, int graz, const double *Src, double *out4) {
+  const int Nt = c->ntau;
   double *XK = malloc(sizeof(double) * 16 * N), *XKD = malloc(sizeof(double) * 16 * N);
   double *XKI = malloc(sizeof(double) * N), *Deltau = malloc(sizeof(double) * N);
   double *SRC4 = malloc(sizeof(double) * 4 * N), *SRD = malloc(sizeof(double) * 4 * N);
   /* XK(i,j,k) column-major -> XK[k*16 + (j)*4 + i]; kept in that layout so that CentDer over
    * the 16 components follows the reference's component order */
-  for (int im = 0; im < Nmu; im++) {
-    for (int it = 0; it < N; it++) {
-      const double *m = c->maglineop + ((size_t)(iwl - 1) * N + it) * 6;
-      double Php = m[0], Phb = m[1], Phr = m[2], Psp = m[3], Psb = m[4], Psr = m[5];
-      double PhiI = 0.5 * (Php * SPhi2[im] + 0.5 * (Phr + Phb) * (1.0 + CPhi2[im])) + LOP(it + 1, iwl) +
-                    CONT(it + 1, iwl);
-      double InvPhiI = 1.0 / PhiI;
-      double Sabs = 0.5 * (Php - 0.5 * (Phr + Phb)) * SPhi2[im];
-      double PhiQ = Sabs * C2Psi[im] * InvPhiI;
-      double PhiU = Sabs * S2Psi[im] * InvPhiI;
-      double PhiV = 0.5 * (Phr - Phb) * CPhi[im] * InvPhiI;
-      double Sdis = 0.5 * (Psp - 0.5 * (Psr + Psb)) * SPhi2[im];
-      double PsiQ = Sdis * C2Psi[im] * InvPhiI;
-      double PsiU = Sdis * S2Psi[im] * InvPhiI;
-      double PsiV = 0.5 * (Psr - Psb) * CPhi[im] * InvPhiI;
-      XKI[it] = PhiI;
-      double *x = XK + (size_t)it * 16; /* x[col*4 + row] */
-      x[0 * 4 + 0] = 0.0;   x[1 * 4 + 0] = PhiQ;  x[2 * 4 + 0] = PhiU;  x[3 * 4 + 0] = PhiV;
-      x[0 * 4 + 1] = PhiQ;  x[1 * 4 + 1] = 0.0;   x[2 * 4 + 1] = PsiV;  x[3 * 4 + 1] = -PsiU;
-      x[0 * 4 + 2] = PhiU;  x[1 * 4 + 2] = -PsiV; x[2 * 4 + 2] = 0.0;   x[3 * 4 + 2] = PsiQ;
-      x[0 * 4 + 3] = PhiV;  x[1 * 4 + 3] = PsiU;  x[2 * 4 + 3] = -PsiQ; x[3 * 4 + 3] = 0.0;
-      SRC4[it * 4 + 0] = Src[it];
-      SRC4[it * 4 + 1] = Src[it] * PhiQ;
-      SRC4[it * 4 + 2] = Src[it] * PhiU;
-      SRC4[it * 4 + 3] = Src[it] * PhiV;
-    }
-    tau_step(c->rhox, XKI, c->xmu[im], N, Deltau);
-    cent_der(Deltau, SRC4, 4, N, SRD);
-    cent_der(Deltau, XK, 16, N, XKD);
-    /* row-major copies for the algebra below */
+  for (int it = 0; it < N; it++) {
+    const int ia = map ? map[it] - 1 : it;
+    const double *gk = g + (size_t)it * gs;
+    const double SPhi2 = gk[0], CPhi2 = gk[1], CPhi = gk[2], S2Psi = gk[3], C2Psi = gk[4];
+    const double *m = c->maglineop + ((size_t)(iwl - 1) * Nt + ia) * 6;
+    double Php = m[0], Phb = m[1], Phr = m[2], Psp = m[3], Psb = m[4], Psr = m[5];
+    double PhiI = 0.5 * (Php * SPhi2 + 0.5 * (Phr + Phb) * (1.0 + CPhi2)) + LOP(ia + 1, iwl) + CONT(ia + 1, iwl);
+    double InvPhiI = 1.0 / PhiI;
+    double Sabs = 0.5 * (Php - 0.5 * (Phr + Phb)) * SPhi2;
+    double PhiQ = Sabs * C2Psi * InvPhiI;
+    double PhiU = Sabs * S2Psi * InvPhiI;
+    double PhiV = 0.5 * (Phr - Phb) * CPhi * InvPhiI;
+    double Sdis = 0.5 * (Psp - 0.5 * (Psr + Psb)) * SPhi2;
+    double PsiQ = Sdis * C2Psi * InvPhiI;
+    double PsiU = Sdis * S2Psi * InvPhiI;
+    double PsiV = 0.5 * (Psr - Psb) * CPhi * InvPhiI;
+    XKI[it] = PhiI;
+    double *x = XK + (size_t)it * 16; /* x[col*4 + row] */
+    x[0 * 4 + 0] = 0.0;   x[1 * 4 + 0] = PhiQ;  x[2 * 4 + 0] = PhiU;  x[3 * 4 + 0] = PhiV;
+    x[0 * 4 + 1] = PhiQ;  x[1 * 4 + 1] = 0.0;   x[2 * 4 + 1] = PsiV;  x[3 * 4 + 1] = -PsiU;
+    x[0 * 4 + 2] = PhiU;  x[1 * 4 + 2] = -PsiV; x[2 * 4 + 2] = 0.0;   x[3 * 4 + 2] = PsiQ;
+    x[0 * 4 + 3] = PhiV;  x[1 * 4 + 3] = PsiU;  x[2 * 4 + 3] = -PsiQ; x[3 * 4 + 3] = 0.0;
+    SRC4[it * 4 + 0] = Src[ia];
+    SRC4[it * 4 + 1] = Src[ia] * PhiQ;
+    SRC4[it * 4 + 2] = Src[ia] * PhiU;
+    SRC4[it * 4 + 3] = Src[ia] * PhiV;
+  }
+  tau_step(rhoxR, XKI, Xmu, N, Deltau);
+  cent_der(Deltau, SRC4, 4, N, SRD);
+  cent_der(Deltau, XK, 16, N, XKD);
+  /* row-major copies for the algebra below */
 #define RM(src, k, out)                                                  \
   for (int r_ = 0; r_ < 4; r_++)                                         \
     for (int c_ = 0; c_ < 4; c_++) (out)[r_ * 4 + c_] = (src)[(size_t)(k)*16 + c_ * 4 + r_];
-    double K[16], K1[16], KD[16], XKA[16], XKB[16], QA[16], QB[16], AA[16], T1[16];
-    double XINT1[4], XINT2[4], v1[4], v2[4], v3[4], v4[4];
-    XINT2[0] = SRC4[(N - 1) * 4 + 0];
-    XINT2[1] = XINT2[2] = XINT2[3] = 0.0;
-    RM(XK, N - 1, K);
-    RM(XKD, N - 1, KD);
+  double K[16], K1[16], KD[16], XKA[16], XKB[16], QA[16], QB[16], AA[16], T1[16];
+  double XINT1[4], XINT2[4], v1[4], v2[4], v3[4], v4[4];
+  XINT2[0] = graz ? 0.0 : SRC4[(N - 1) * 4 + 0];
+  XINT2[1] = XINT2[2] = XINT2[3] = 0.0;
+  RM(XK, N - 1, K);
+  RM(XKD, N - 1, KD);
+  mat_mul(K, K, T1);
+  for (int q = 0; q < 16; q++) XKB[q] = T1[q] + KD[q];
+  for (int k = N - 1; k >= 1; k--) {
+    int k1 = k + 1;
+    double Dtau = Deltau[k - 1], Dtau05 = 0.5 * Dtau, Eps, Alpha, Beta, Gamma;
+    delo_coef(Dtau, &Eps, &Alpha, &Beta, &Gamma);
+    Gamma = Gamma * 0.5;
+    RM(XK, k - 1, K);
+    RM(XK, k1 - 1, K1);
+    RM(XKD, k - 1, KD);
+    for (int q = 0; q < 16; q++) XKA[q] = XKB[q];
     mat_mul(K, K, T1);
     for (int q = 0; q < 16; q++) XKB[q] = T1[q] + KD[q];
-    for (int k = N - 1; k >= 1; k--) {
-      int k1 = k + 1;
-      double Dtau = Deltau[k - 1], Dtau05 = 0.5 * Dtau, Eps, Alpha, Beta, Gamma;
-      delo_coef(Dtau, &Eps, &Alpha, &Beta, &Gamma);
-      Gamma = Gamma * 0.5;
-      RM(XK, k - 1, K);
-      RM(XK, k1 - 1, K1);
-      RM(XKD, k - 1, KD);
-      for (int q = 0; q < 16; q++) XKA[q] = XKB[q];
-      mat_mul(K, K, T1);
-      for (int q = 0; q < 16; q++) XKB[q] = T1[q] + KD[q];
-      for (int q = 0; q < 16; q++) QA[q] = Dtau05 * XKA[q] + (Dtau05 - 1.0) * K1[q];
-      for (int q = 0; q < 16; q++) QB[q] = -Dtau05 * XKB[q] - (Dtau05 + 1.0) * K[q];
-      for (int q = 0; q < 16; q++) AA[q] = ((q % 5) == 0 ? 1.0 : 0.0) + Alpha * K[q] - Gamma * QB[q];
-      double GD = Gamma * Dtau05;
-      mat_vec(K1, XINT2, v1);
-      mat_vec(QA, XINT2, v2);
-      mat_vec(K1, SRC4 + (size_t)(k1 - 1) * 4, v3);
-      mat_vec(K, SRC4 + (size_t)(k - 1) * 4, v4);
-      for (int q = 0; q < 4; q++)
-        XINT1[q] = Eps * XINT2[q] - Beta * v1[q] + Gamma * v2[q] + (Beta + Gamma) * SRC4[(k1 - 1) * 4 + q] -
-                   GD * v3[q] + (Alpha + Gamma) * SRC4[(k - 1) * 4 + q] + GD * v4[q] +
-                   GD * (SRD[(k - 1) * 4 + q] - SRD[(k1 - 1) * 4 + q]);
-      fso_solve4x4(AA, XINT1, XINT2);
-    }
-#undef RM
-    for (int q = 0; q < 4; q++) Stokes[q * Nmu + im] = XINT2[q];
+    for (int q = 0; q < 16; q++) QA[q] = Dtau05 * XKA[q] + (Dtau05 - 1.0) * K1[q];
+    for (int q = 0; q < 16; q++) QB[q] = -Dtau05 * XKB[q] - (Dtau05 + 1.0) * K[q];
+    for (int q = 0; q < 16; q++) AA[q] = ((q % 5) == 0 ? 1.0 : 0.0) + Alpha * K[q] - Gamma * QB[q];
+    double GD = Gamma * Dtau05;
+    mat_vec(K1, XINT2, v1);
+    mat_vec(QA, XINT2, v2);
+    mat_vec(K1, SRC4 + (size_t)(k1 - 1) * 4, v3);
+    mat_vec(K, SRC4 + (size_t)(k - 1) * 4, v4);
+    for (int q = 0; q < 4; q++)
+      XINT1[q] = Eps * XINT2[q] - Beta * v1[q] + Gamma * v2[q] + (Beta + Gamma) * SRC4[(k1 - 1) * 4 + q] -
+                 GD * v3[q] + (Alpha + Gamma) * SRC4[(k - 1) * 4 + q] + GD * v4[q] +
+                 GD * (SRD[(k - 1) * 4 + q] - SRD[(k1 - 1) * 4 + q]);
+    fso_solve4x4(AA, XINT1, XINT2);
   }
+#undef RM
+  for (int q = 0; q < 4; q++) out4[q] = XINT2[q];
   free(XK);
   free(XKD);
   free(XKI);
   free(Deltau);
   free(SRC4);
   free(SRD);
+}
+
+/* StokesPrfPP (src/solver.f90:265-394) for one wavelength; Stokes[4][nmu] */
+EXPORT void fso_stokes_prf_pp(fso_ctx *c, int iwl, const double *Src, const double *SPhi2, const double *CPhi2,
+                              const double *CPhi, const double *S2Psi, const double *C2Psi, double *Stokes) {
+  int Nmu = c->nmu;
+  for (int im = 0; im < Nmu; im++) {
+    double g[5] = {SPhi2[im], CPhi2[im], CPhi[im], S2Psi[im], C2Psi[im]}, out[4];
+    stokes_ray(c, iwl, c->ntau, NULL, c->rhox, c->xmu[im], g, 0, 0, Src, out);
+    for (int q = 0; q < 4; q++) Stokes[q * Nmu + im] = out[q];
+  }
+}
+
+/* StokesPrfSph (src/solver.f90:402-538) for one wavelength along the rays of SphereInit;
+ * geo[(im * 2 ntau + k) * 5 + 0..4] = SPhi2S, CPhi2S, CPhiS, S2PsiS, C2PsiS at ray point k;
+ * Stokes[4][nmu] */
+EXPORT void fso_stokes_prf_sph(fso_ctx *c, int iwl, const double *Src, const double *geo, double *Stokes) {
+  int Nmu = c->nmu, L = 2 * c->ntau;
+  for (int im = 0; im < Nmu; im++) {
+    double out[4];
+    stokes_ray(c, iwl, c->ntau_mu[im], c->map_mu + (size_t)im * L, c->rhox_mu + (size_t)im * L, 1.0,
+               geo + (size_t)im * L * 5, 5, c->graz_mu[im], Src, out);
+    for (int q = 0; q < 4; q++) Stokes[q * Nmu + im] = out[q];
+  }
 }
 
 /* Field geometry per mu (src/synth.f90:1935-1994) */
@@ -1743,6 +1924,66 @@ EXPORT int fso_field_geometry(int nmu, const double *xmu, const double *bfield, 
   return 0;
 }
 
+/* Field geometry along the spherical rays (src/synth.f90:2017-2090): BFRAME 1 and 2 as in
+ * the plane-parallel case; BFRAME 0 projects the stellar-frame field with the local
+ * ray-to-radial cosine MuLoc of each ray point.  geo[(im * 2 ntau + k) * 5 + 0..4] =
+ * SPhi2, CPhi2, CPhi, S2Psi, C2Psi. */
+EXPORT int fso_sph_field_geometry(fso_ctx *c, const double *bfield, int nbfield, int bframe, double *geo) {
+  if (kPi == 0.0) init_constants();
+  const double d2r = kPi / 180.0;
+  const int M = c->nmu, L = 2 * c->ntau;
+  if (bframe == 2 && nbfield < 1 + 2 * M) {
+    snprintf(g_err, sizeof(g_err), "BFRAME 2 needs 1+2*Nmu BFIELD values");
+    return 1;
+  }
+  for (int im = 0; im < M; im++) {
+    int np = (bframe == 1) ? L : c->ntau_mu[im];
+    for (int k = 0; k < np; k++) {
+      double Sphi, Cgam, Spsi, Cpsi;
+      if (bframe == 1) {
+        Sphi = sin(bfield[1] * d2r);
+        Spsi = sin(bfield[2] * d2r);
+        Cpsi = cos(bfield[2] * d2r);
+        Cgam = cos(bfield[1] * d2r);
+      } else if (bframe == 2) {
+        Sphi = sin(bfield[2 * im + 1] * d2r);
+        Spsi = sin(bfield[2 * im + 2] * d2r);
+        Cpsi = cos(bfield[2 * im + 2] * d2r);
+        Cgam = cos(bfield[2 * im + 1] * d2r);
+      } else {
+        double Fldmod = sqrt(bfield[0] * bfield[0] + bfield[1] * bfield[1] + bfield[2] * bfield[2]);
+        if (Fldmod > 0.0) {
+          double Coslon = c->muloc[(size_t)im * L + k], Sinlon = sqrt(1.0 - Coslon * Coslon);
+          double Bx = bfield[0] * Sinlon + bfield[2] * Coslon, By = bfield[1];
+          double Bz = bfield[0] * Coslon - bfield[2] * Sinlon;
+          double Bt = sqrt(Bx * Bx + By * By);
+          Sphi = Bt / Fldmod;
+          Cgam = Bz / Fldmod;
+          if (Bt > 0.0) {
+            Spsi = By / Bt;
+            Cpsi = Bx / Bt;
+          } else {
+            Spsi = 0.0;
+            Cpsi = 1.0;
+          }
+        } else {
+          Sphi = 0.0;
+          Cgam = 1.0;
+          Spsi = 0.0;
+          Cpsi = 1.0;
+        }
+      }
+      double *g = geo + ((size_t)im * L + k) * 5;
+      g[0] = Sphi * Sphi;
+      g[1] = Cgam * Cgam;
+      g[2] = Cgam;
+      g[3] = 2.0 * Spsi * Cpsi;
+      g[4] = Cpsi * Cpsi - Spsi * Spsi;
+    }
+  }
+  return 0;
+}
+
 EXPORT int fso_magradtran(fso_ctx *c, const double *bfield, int32_t nbfield, int32_t bframe, double *spz_out,
                           double *spc_out) {
   int N = c->ntau, M = c->nmu, Nwl = c->nwl;
@@ -1755,14 +1996,22 @@ EXPORT int fso_magradtran(fso_ctx *c, const double *bfield, int32_t nbfield, int
   double *SPhi2 = malloc(sizeof(double) * M), *CPhi2 = malloc(sizeof(double) * M);
   double *CPhi = malloc(sizeof(double) * M), *S2Psi = malloc(sizeof(double) * M);
   double *C2Psi = malloc(sizeof(double) * M);
-  if (fso_field_geometry(M, c->xmu, bfield, nbfield, bframe, SPhi2, CPhi2, CPhi, S2Psi, C2Psi)) return 1;
+  double *geo = NULL;
+  if (c->radius > 0.0) {
+    geo = calloc((size_t)M * 2 * N * 5, sizeof(double));
+    if (fso_sph_field_geometry(c, bfield, nbfield, bframe, geo)) return 1;
+  } else if (fso_field_geometry(M, c->xmu, bfield, nbfield, bframe, SPhi2, CPhi2, CPhi, S2Psi, C2Psi)) {
+    return 1;
+  }
   double *Src = malloc(sizeof(double) * N), *Xki = malloc(sizeof(double) * N);
   for (int iwl = 1; iwl <= Nwl; iwl++) {
     planck(c, WAVE(iwl) + c->wfirst, Src);
     for (int k = 1; k <= N; k++) Xki[k - 1] = CONT(k, iwl);
-    fso_locprfpp(c->spc + (size_t)(iwl - 1) * M, c->xmu, M, Xki, Src, c->rhox, N);
-    fso_stokes_prf_pp(c, iwl, Src, SPhi2, CPhi2, CPhi, S2Psi, C2Psi, c->spz + (size_t)(iwl - 1) * 4 * M);
+    locprf(c, c->spc + (size_t)(iwl - 1) * M, Xki, Src);
+    if (geo) fso_stokes_prf_sph(c, iwl, Src, geo, c->spz + (size_t)(iwl - 1) * 4 * M);
+    else fso_stokes_prf_pp(c, iwl, Src, SPhi2, CPhi2, CPhi, S2Psi, C2Psi, c->spz + (size_t)(iwl - 1) * 4 * M);
   }
+  free(geo);
   free(Src);
   free(Xki);
   free(SPhi2);

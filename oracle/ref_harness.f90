@@ -434,6 +434,46 @@ CONTAINS
     CALL StokesPrfPP(iwl, src, Atmos%Rhox, sphi2, cphi2, cphi, s2psi, c2psi, stokes)
   END SUBROUTINE ref_stokesprfpp
 
+  ! SphereInit on the current set-up for Radius and layer heights; returns the ray geometry
+  SUBROUTINE ref_sphere(radius_, height_, ntmu, graz, map, rhoxmu_, muloc_) BIND(C, NAME='ref_sphere')
+    REAL(C_DOUBLE), VALUE :: radius_
+    REAL(C_DOUBLE), INTENT(IN) :: height_(Ntau)
+    INTEGER(C_INT), INTENT(OUT) :: ntmu(Nmu), graz(Nmu), map(2*Ntau, Nmu)
+    REAL(C_DOUBLE), INTENT(OUT) :: rhoxmu_(2*Ntau, Nmu), muloc_(2*Ntau, Nmu)
+    Radius = radius_
+    Atmos%Height = height_
+    IF (ALLOCATED(NtauMu)) DEALLOCATE(NtauMu, GrazMu, MapMu, RhoxMu, MuLoc)
+    CALL SphereInit
+    ntmu = NtauMu
+    graz = MERGE(1, 0, GrazMu)
+    map = MapMu
+    rhoxmu_ = RhoxMu
+    muloc_ = MuLoc
+  END SUBROUTINE ref_sphere
+
+  ! LocPrfSph for one opacity column on the rays of the last ref_sphere call
+  SUBROUTINE ref_locprfsph(ploc, xki, src) BIND(C, NAME='ref_locprfsph')
+    REAL(C_DOUBLE), INTENT(OUT) :: ploc(Nmu)
+    REAL(C_DOUBLE), INTENT(IN) :: xki(Ntau), src(Ntau)
+    CALL LocPrfSph(ploc, xki, src)
+  END SUBROUTINE ref_locprfsph
+
+  ! StokesPrfSph at wavelength iwl with the given MagLineOpac/LineOpac columns and the
+  ! per-point field geometry (2*Ntau, Nmu) of each angle
+  SUBROUTINE ref_stokesprfsph(iwl, mlop, lop, src, sphi2, cphi2, cphi, s2psi, c2psi, stokes) &
+                              BIND(C, NAME='ref_stokesprfsph')
+    INTEGER(C_INT), VALUE :: iwl
+    REAL(C_DOUBLE), INTENT(IN) :: mlop(6, Ntau), lop(Ntau), src(Ntau)
+    REAL(C_DOUBLE), INTENT(IN), DIMENSION(2*Ntau, Nmu) :: sphi2, cphi2, cphi, s2psi, c2psi
+    REAL(C_DOUBLE), INTENT(OUT) :: stokes(Nmu, 4)
+    IF (ALLOCATED(MagLineOpac)) DEALLOCATE(MagLineOpac)
+    ALLOCATE(MagLineOpac(6, Ntau, Nwl))
+    MagLineOpac = 0d0
+    MagLineOpac(:, :, iwl) = mlop
+    LineOpac(:, iwl) = lop
+    CALL StokesPrfSph(iwl, src, sphi2, cphi2, cphi, s2psi, c2psi, stokes)
+  END SUBROUTINE ref_stokesprfsph
+
   SUBROUTINE ref_solve4x4(a, b, x) BIND(C, NAME='ref_solve4x4')
     REAL(C_DOUBLE), INTENT(IN) :: a(4, 4), b(4)
     REAL(C_DOUBLE), INTENT(OUT) :: x(4)

@@ -66,6 +66,10 @@ def _load_oracle() -> C.CDLL:
     lib.fso_zeeman_pattern.argtypes = [C.c_float] * 4 + [_ip, _fp, _fp, _ip, _fp, _fp]
     lib.fso_gauss_mu.argtypes = [C.c_int, C.c_int, _dp, _dp]
     lib.fso_mu_quadrature.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
+    lib.fso_sphere_geometry.argtypes = [vp, _ip, _ip, _ip, _dp, _dp]
+    lib.fso_locprfsph.argtypes = [vp, _dp, _dp, _dp]
+    lib.fso_sph_field_geometry.argtypes = [vp, _dp, C.c_int, C.c_int, _dp]
+    lib.fso_stokes_prf_sph.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     lib.fso_last_error.restype = C.c_char_p
     return lib
 
@@ -89,7 +93,9 @@ class Oracle:
         self._cs = setup.cstruct()
         self.ctx = C.c_void_p()
         rc = self.lib.fso_init(C.byref(self._cs), C.byref(self.ctx))
-        assert rc == 0
+        if rc != 0:
+            self.ctx = C.c_void_p()
+            raise RuntimeError(self.lib.fso_last_error().decode())
 
     def close(self):
         if self.ctx:
@@ -186,6 +192,35 @@ class Oracle:
         sfc = np.zeros(s.nwl)
         self.lib.fso_flux(self.ctx, ptr(sfl, C.c_double), ptr(sfc, C.c_double))
         return sfl, sfc
+
+    def sphere_geometry(self) -> dict:
+        """SphereInit's rays: ntau_mu, graz [nmu]; map (1-based layers), rhox_mu, muloc
+        [nmu][2 ntau]."""
+        m, L = self.setup.nmu, 2 * self.setup.ntau
+        nt, gz = np.zeros(m, np.int32), np.zeros(m, np.int32)
+        mp = np.zeros((m, L), np.int32)
+        rm, ml = np.zeros((m, L)), np.zeros((m, L))
+        rc = self.lib.fso_sphere_geometry(self.ctx, ptr(nt, C.c_int32), ptr(gz, C.c_int32), ptr(mp, C.c_int32),
+                                          ptr(rm, C.c_double), ptr(ml, C.c_double))
+        assert rc == 0
+        return dict(ntau_mu=nt, graz=gz, map=mp, rhox_mu=rm, muloc=ml)
+
+    def locprfsph(self, xki, src) -> np.ndarray:
+        out = np.zeros(self.setup.nmu)
+        a = np.ascontiguousarray(xki, np.float64)
+        b = np.ascontiguousarray(src, np.float64)
+        self.lib.fso_locprfsph(self.ctx, ptr(out, C.c_double), ptr(a, C.c_double), ptr(b, C.c_double))
+        return out
+
+    def sph_field_geometry(self, bfield, bframe: int) -> np.ndarray:
+        """[nmu][2 ntau][5]: SPhi2, CPhi2, CPhi, S2Psi, C2Psi at each ray point."""
+        b = np.ascontiguousarray(bfield, np.float64)
+        geo = np.zeros((self.setup.nmu, 2 * self.setup.ntau, 5))
+        rc = self.lib.fso_sph_field_geometry(self.ctx, ptr(b, C.c_double), b.shape[0], bframe,
+                                             ptr(geo, C.c_double))
+        if rc != 0:
+            raise RuntimeError(self.lib.fso_last_error().decode())
+        return geo
 
     def xline(self, lines: LineList, i: int):
         nt = self.setup.ntau
@@ -358,6 +393,31 @@ class Ref:
         a = np.ascontiguousarray(xki, np.float64)
         b = np.ascontiguousarray(src, np.float64)
         self.lib.ref_locprfpp(ptr(out, C.c_double), ptr(a, C.c_double), ptr(b, C.c_double))
+        return out
+
+    def sphere(self, radius: float, height) -> dict:
+        m, nt = self.setup.nmu, self.setup.ntau
+        h = np.ascontiguousarray(height, np.float64)
+        ntmu, graz = np.zeros(m, np.int32), np.zeros(m, np.int32)
+        mp = np.zeros((m, 2 * nt), np.int32)  # (2 Ntau, Nmu) column-major
+        rm, ml = np.zeros((m, 2 * nt)), np.zeros((m, 2 * nt))
+        self.lib.ref_sphere(C.c_double(radius), ptr(h, C.c_double), ptr(ntmu, C.c_int32), ptr(graz, C.c_int32),
+                            ptr(mp, C.c_int32), ptr(rm, C.c_double), ptr(ml, C.c_double))
+        return dict(ntau_mu=ntmu, graz=graz, map=mp, rhox_mu=rm, muloc=ml)
+
+    def locprfsph(self, xki, src):
+        out = np.zeros(self.setup.nmu)
+        a = np.ascontiguousarray(xki, np.float64)
+        b = np.ascontiguousarray(src, np.float64)
+        self.lib.ref_locprfsph(ptr(out, C.c_double), ptr(a, C.c_double), ptr(b, C.c_double))
+        return out
+
+    def stokesprfsph(self, iwl, mlop, lop, src, geo):
+        """geo [nmu][2 ntau][5] as the oracle's sph_field_geometry."""
+        out = np.zeros((4, self.setup.nmu))
+        g5 = [np.ascontiguousarray(geo[:, :, q]) for q in range(5)]
+        arrs = [np.ascontiguousarray(x, np.float64) for x in (mlop, lop, src)] + g5
+        self.lib.ref_stokesprfsph(C.c_int(iwl), *(ptr(x, C.c_double) for x in arrs), ptr(out, C.c_double))
         return out
 
     def stokesprfpp(self, iwl, mlop, lop, src, geo):
