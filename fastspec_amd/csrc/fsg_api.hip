@@ -82,6 +82,9 @@ struct fsg_ctx {
   double *geo = nullptr;      // [nmu][5] field geometry
   int magnetic = 0;           // the last formal solution was the polarised one
   DBuf comp;
+  // spherical rays on the host (field geometry of the polarised solution)
+  std::vector<int> ntau_mu;
+  std::vector<double> muloc;  // [nmu][2 ntau]
 };
 
 struct fsg_dlines {
@@ -119,6 +122,72 @@ int alloc_zero(fsg_ctx *c, size_t bytes, void **out) {
 }
 
 size_t lop_elem(const fsg_ctx *c) { return c->precision == FSG_FP32 ? sizeof(float) : sizeof(double); }
+
+// The rays of SphereInit (src/solver.f90:45-120).  A ray with impact parameter
+// p = R_top sqrt(1 - mu^2) crosses the layers r_k = Radius + Height_k; its column-mass steps
+// are the radial ones scaled by (r_k + r_k-1) / (z_k + z_k-1), z = sqrt((r - p)(r + p)).
+// A ray with p above the deepest layer (grazing) is tangent below layer n: the tangent
+// shell adds chord * dm/dr, and the far half revisits layers n..1 with mirrored steps.
+// Outputs per mu (row stride 2 ntau): point count, grazing flag, 0-based layer of each
+// point, column mass and the local ray-to-radial cosine (sign flipped beyond the tangent).
+struct SphereRays {
+  std::vector<int> ntau_mu, graz, map;
+  std::vector<double> rhox, muloc;
+};
+
+int sphere_rays(int N, int M, double radius, const double *height, const double *rhox, const double *xmu,
+                SphereRays &R) {
+  const int L = 2 * N;
+  std::vector<double> Rk(N);
+  for (int k = 0; k < N; k++) Rk[k] = radius + height[k];
+  R.ntau_mu.assign(M, 0);
+  R.graz.assign(M, 0);
+  R.map.assign((size_t)M * L, 0);
+  R.rhox.assign((size_t)M * L, 0.0);
+  R.muloc.assign((size_t)M * L, 0.0);
+  for (int im = 0; im < M; im++) {
+    const double mu = xmu[im];
+    if (mu <= 0.0) return fail("SphereInit: mu = 0 rays are unsupported for spherical models");
+    const double P = Rk[0] * std::sqrt((1.0 - mu) * (1.0 + mu));
+    const bool graz = P > Rk[N - 1];
+    int n = N;
+    if (graz) {
+      n = 1;
+      while (n < N && P < Rk[n]) n++;  // n = number of layers above the tangent point
+      if (n < 2)
+        return fail("SphereInit: grazing ray tangent in the top model layer; remove extreme mu values close to 0");
+    }
+    int *mp = R.map.data() + (size_t)im * L;
+    double *rm = R.rhox.data() + (size_t)im * L, *ml = R.muloc.data() + (size_t)im * L;
+    rm[0] = rhox[0] / mu;
+    double z1 = std::sqrt((Rk[0] - P) * (Rk[0] + P));
+    ml[0] = z1 / Rk[0];
+    mp[0] = 0;
+    for (int k = 1; k < n; k++) {
+      const double z = std::sqrt((Rk[k] - P) * (Rk[k] + P));
+      rm[k] = rm[k - 1] + (rhox[k] - rhox[k - 1]) * (Rk[k] + Rk[k - 1]) / (z + z1);
+      ml[k] = z / Rk[k];
+      mp[k] = k;
+      z1 = z;
+    }
+    R.graz[im] = graz ? 1 : 0;
+    if (!graz) {
+      R.ntau_mu[im] = N;
+      continue;
+    }
+    const double chord = 2.0 * std::sqrt((Rk[n - 1] - P) * (Rk[n - 1] + P));
+    rm[n] = rm[n - 1] + chord * (rhox[n] - rhox[n - 1]) / (Rk[n - 1] - Rk[n]);
+    mp[n] = n - 1;
+    ml[n] = -ml[n - 1];
+    for (int k = n + 1; k < 2 * n; k++) {  // point k mirrors point 2n - 1 - k
+      rm[k] = rm[k - 1] + (rm[2 * n - k] - rm[2 * n - k - 1]);
+      mp[k] = 2 * n - 1 - k;
+      ml[k] = -ml[2 * n - 1 - k];
+    }
+    R.ntau_mu[im] = 2 * n;
+  }
+  return 0;
+}
 
 // Lines per accumulation chunk: NlinesLim, lowered so that the chunk's per-line device
 // buffers (per_line bytes each) fit in 85 % of the free device memory plus what the
@@ -245,6 +314,26 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
       for (int k = 0; k < N; k++) ct[(size_t)k * Nw + iw] = s->contop[iw * N + k];
     rc |= upload(c, ct.data(), ct.size(), &S.contT, c->owned);
     CK(hipStreamSynchronize(c->st));
+  }
+  // spherical models: the rays of SphereInit
+  if (s->radius > 0.0 && s->nmu > 0) {
+    if (!s->height) {
+      fsg_finalize(c);
+      return fail("fsg_init: a spherical model (radius > 0) needs the layer heights");
+    }
+    SphereRays R;
+    if (sphere_rays(N, s->nmu, s->radius, s->height, s->rhox, s->xmu, R)) {
+      std::string e = g_err;
+      fsg_finalize(c);
+      return fail(e);
+    }
+    rc |= upload(c, R.ntau_mu.data(), R.ntau_mu.size(), &S.ntau_mu, c->owned);
+    rc |= upload(c, R.graz.data(), R.graz.size(), &S.graz_mu, c->owned);
+    rc |= upload(c, R.map.data(), R.map.size(), &S.map_mu, c->owned);
+    rc |= upload(c, R.rhox.data(), R.rhox.size(), &S.rhox_mu, c->owned);
+    S.sph = 1;
+    c->ntau_mu = R.ntau_mu;
+    c->muloc = R.muloc;
   }
   // per-depth perturber densities and XlineInit
   const float *fh1, *fhe1, *fhmol;
@@ -595,7 +684,7 @@ int ensure_mag_buffers(fsg_ctx *c) {
   if (alloc_zero(c, lop_elem(c) * 6 * N * Nw, &c->maglineop) ||
       alloc_zero(c, sizeof(float2) * (NVA + 1) * TAB_LD, &tab) ||
       alloc_zero(c, sizeof(double) * 4 * M * Nw, (void **)&c->spz) ||
-      alloc_zero(c, sizeof(double) * 5 * M, (void **)&c->geo))
+      alloc_zero(c, sizeof(double) * 5 * M * 2 * N, (void **)&c->geo))
     return 1;
   c->ftab = (float2 *)tab;
   launch_fvoigt_table(c->ftab, c->S.anode, 8.0 / (double)(NVV - 1), c->st);
@@ -902,13 +991,13 @@ int fsg_magradtran(fsg_ctx *c, const double *bfield, int32_t nbfield, int32_t bf
   if (bframe != 2 && nbfield < 3) return fail("fsg_magradtran: BFIELD needs 3 values");
   CK(hipSetDevice(c->dev));
   if (ensure_rt_buffers(c) || ensure_mag_buffers(c)) return 1;
-  // field geometry per mu (src/synth.f90:1935-1994): sin^2 and cos^2 of the inclination,
-  // its cosine, and sin/cos of twice the azimuth
-  std::vector<double> g((size_t)5 * M);
+  // field geometry (src/synth.f90:1935-1994, 2017-2090): sin^2 and cos^2 of the
+  // inclination gamma, its cosine, and sin/cos of twice the azimuth chi; per mu in
+  // plane-parallel geometry, per ray point on spherical rays
   std::vector<double> xmu(M);
   CK(hipMemcpy(xmu.data(), c->S.xmu, sizeof(double) * M, hipMemcpyDeviceToHost));
   const double d2r = kPi / 180.0;
-  for (int im = 0; im < M; im++) {
+  auto angles = [&](int im, double coslon, double *gm) {
     double Sphi, Cphi, Spsi, Cpsi;
     if (bframe == 1) {
       Sphi = std::sin(bfield[1] * d2r);
@@ -921,9 +1010,11 @@ int fsg_magradtran(fsg_ctx *c, const double *bfield, int32_t nbfield, int32_t bf
       Spsi = std::sin(bfield[2 * im + 2] * d2r);
       Cpsi = std::cos(bfield[2 * im + 2] * d2r);
     } else {
+      // stellar frame (Br, Bm, Bp) projected with the cosine between the line of sight
+      // and the local radial direction
       const double Fldmod = std::sqrt(bfield[0] * bfield[0] + bfield[1] * bfield[1] + bfield[2] * bfield[2]);
       if (Fldmod > 0.0) {
-        const double Coslon = xmu[im], Sinlon = std::sqrt(1.0 - Coslon * Coslon);
+        const double Coslon = coslon, Sinlon = std::sqrt(1.0 - Coslon * Coslon);
         const double Bx = bfield[0] * Sinlon + bfield[2] * Coslon, By = bfield[1];
         const double Bz = bfield[0] * Coslon - bfield[2] * Sinlon;
         const double Bt = std::sqrt(Bx * Bx + By * By);
@@ -943,12 +1034,22 @@ int fsg_magradtran(fsg_ctx *c, const double *bfield, int32_t nbfield, int32_t bf
         Cpsi = 1.0;
       }
     }
-    double *gm = g.data() + (size_t)5 * im;
     gm[0] = Sphi * Sphi;
     gm[1] = Cphi * Cphi;
     gm[2] = Cphi;
     gm[3] = 2.0 * Spsi * Cpsi;
     gm[4] = Cpsi * Cpsi - Spsi * Spsi;
+  };
+  std::vector<double> g;
+  if (c->S.sph) {
+    const int L = 2 * c->S.ntau;
+    g.assign((size_t)5 * M * L, 0.0);
+    for (int im = 0; im < M; im++)
+      for (int k = 0; k < c->ntau_mu[im]; k++)
+        angles(im, c->muloc[(size_t)im * L + k], g.data() + ((size_t)im * L + k) * 5);
+  } else {
+    g.assign((size_t)5 * M, 0.0);
+    for (int im = 0; im < M; im++) angles(im, xmu[im], g.data() + (size_t)5 * im);
   }
   CK(hipMemcpyAsync(c->geo, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, c->st));
   // continuum intensities and their flux (the scalar solution of LineOpac + ContOpac is

@@ -96,6 +96,11 @@ struct DevSetup {
   const float *vtab;         // [NVA+1][TAB_LD]
   const double *anode, *aida;
   double idv, idt, a1, ia1;
+  // spherical models: the rays of SphereInit, per mu up to 2 ntau points
+  int sph;
+  const int *ntau_mu, *graz_mu;  // [nmu] points on the ray, ray misses the deepest layer
+  const int *map_mu;             // [nmu][2 ntau] model layer (0-based) of each ray point
+  const double *rhox_mu;         // [nmu][2 ntau] column mass along the ray
 };
 
 struct DevLines {

@@ -264,4 +264,26 @@ __device__ __forceinline__ void delo_coef(double Dtau, double &Eps, double &Alph
 
 __device__ __forceinline__ double planck_src(double C1, double C2, double T) { return C2 / (exp(C1 / T) - 1.0); }
 
+// TauStep (src/solver.f90:591-615) element i (0-based) on the column-mass scale rhox with
+// the steps divided by Xmu (1 on spherical rays); km, k0, kp = opacity at points i-1, i,
+// i+1 (km unused for i = 0)
+__device__ __forceinline__ double tau_step_mu(const double *rhox, double xmu, int i, double km, double k0,
+                                              double kp) {
+  const double Dx1 = (rhox[i + 1] - rhox[i]) / xmu;
+  double Yp;
+  if (i == 0) {
+    Yp = (kp - k0) / Dx1;
+  } else {
+    const double Dx = (rhox[i] - rhox[i - 1]) / xmu;
+    const double Der = (k0 - km) / Dx, Der1 = (kp - k0) / Dx1;
+    if (Der * Der1 > 0.0) {
+      const double Lambda = (1.0 + Dx1 / (Dx + Dx1)) / 3.0;
+      Yp = (Der / (Lambda * Der1 + (1.0 - Lambda) * Der)) * Der1;
+    } else {
+      Yp = 0.0;
+    }
+  }
+  return Dx1 * (kp + 2.0 * k0 + Dx1 / 2.0 * Yp) / 3.0;
+}
+
 }  // namespace fsg

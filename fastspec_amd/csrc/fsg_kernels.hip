@@ -9,7 +9,8 @@
 //   LineOpacity1/2             src/opacity.f90:297-386
 //   WaveIdx, AnalyticWindow    src/opacity.f90:594-608, 678-710
 //   pass 1 of LineopCalc       src/synth.f90:833-907
-//   RadTranCalc, LocPrfPP      src/synth.f90:1825-1871, src/solver.f90:123-179
+//   RadTranCalc, LocPrfPP      src/synth.f90:1825-1895, src/solver.f90:123-179
+//   LocPrfSph, DeloScalar      src/solver.f90:182-261
 //   TauStep, CentDer           src/solver.f90:591-647
 //   flux sums of DiskInteg     src/convol.f90:233-234
 #include "fsg_kernels.h"
@@ -634,6 +635,72 @@ struct GridCol {
   }
 };
 
+// DeloScalar along ray im (LocPrfSph, src/solver.f90:182-261) for NC opacity columns that
+// share the source function: points gathered through the ray map, optical depth steps on
+// the ray's column mass (Xmu = 1), streamed from the deepest point upward.  Grazing rays
+// enter with zero intensity, the others with S at the deepest point.
+template <int NC, typename Col>
+__device__ void delo_ray(const DevSetup &S, const Col *cols, int im, double C1, double C2, double *Y) {
+  const int L = 2 * S.ntau, N = S.ntau_mu[im];
+  const int *map = S.map_mu + (size_t)im * L;
+  const double *rm = S.rhox_mu + (size_t)im * L;
+  auto src = [&](int j) { return planck_src(C1, C2, S.temp[map[j]]); };
+  auto tau = [&](int c, int j) {  // Deltau(j) of column c
+    const double km = (j >= 1) ? cols[c].k(map[j - 1]) : 0.0;
+    return tau_step_mu(rm, 1.0, j, km, cols[c].k(map[j]), cols[c].k(map[j + 1]));
+  };
+  int i = N - 2;
+  double S_ip1 = src(N - 1), S_i = src(N - 2), S_im1 = src(N - 3);
+  double dt_i[NC], dt_im1[NC], Sd_ip1[NC];
+  for (int c = 0; c < NC; c++) {
+    dt_i[c] = tau(c, i);
+    dt_im1[c] = tau(c, i - 1);
+    Sd_ip1[c] = (S_ip1 - S_i) / dt_i[c];
+    Y[c] = S.graz_mu[im] ? 0.0 : S_ip1;
+  }
+  for (; i >= 0; i--) {
+    for (int c = 0; c < NC; c++) {
+      const double Sd_i = (i == 0) ? (S_ip1 - S_i) / dt_i[c] : cent_der_i(dt_im1[c], dt_i[c], S_im1, S_i, S_ip1);
+      double Eps, Alpha, Beta, Gamma;
+      delo_coef(dt_i[c], Eps, Alpha, Beta, Gamma);
+      const double U = 0.5 * (S_i + 0.5 * dt_i[c] * Sd_i + S_ip1 - 0.5 * dt_i[c] * Sd_ip1[c]);
+      Y[c] = Y[c] * Eps + Alpha * S_i + Beta * S_ip1 + Gamma * U;
+      Sd_ip1[c] = Sd_i;
+      dt_i[c] = dt_im1[c];
+    }
+    S_ip1 = S_i;
+    S_i = S_im1;
+    if (i >= 2) {
+      S_im1 = src(i - 2);
+      for (int c = 0; c < NC; c++) dt_im1[c] = tau(c, i - 2);
+    }
+  }
+}
+
+// spherical RadTranCalc (src/synth.f90:1873-1895): continuum and line+continuum along each
+// ray, one lane per wavelength point, flux sums fused
+template <typename LopT>
+__global__ void __launch_bounds__(128) k_radtran_sph(DevSetup S, const LopT *lop, double *spl, double *spc,
+                                                     double *sfl, double *sfc) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= S.nwl) return;
+  const double Wl = S.wave[p] + S.wfirst;
+  const double C1 = kPlkC1 / Wl;
+  const double C2 = exp(50.7649141 - 5.0 * log(Wl));
+  const GridCol<LopT> cols[2] = {{S.contT, lop, S.nwl, p, false}, {S.contT, lop, S.nwl, p, true}};
+  double fc = 0.0, fl = 0.0;
+  for (int m = 0; m < S.nmu; m++) {
+    double Y[2];
+    delo_ray<2>(S, cols, m, C1, C2, Y);
+    spc[(size_t)m * S.nwl + p] = Y[0];
+    spl[(size_t)m * S.nwl + p] = Y[1];
+    fc = fc + Y[0] * S.wmu[m];
+    fl = fl + Y[1] * S.wmu[m];
+  }
+  sfc[p] = fc;
+  sfl[p] = fl;
+}
+
 template <typename LopT>
 __global__ void __launch_bounds__(256) k_radtran(DevSetup S, const LopT *lop, double *spl, double *spc,
                                                  double *sfl, double *sfc) {
@@ -679,7 +746,7 @@ struct LineCentreCol {
   }
 };
 
-// LinStrCalc (src/synth.f90:1774-1822): central depth of each retained line, plane-parallel
+// LinStrCalc (src/synth.f90:1774-1822): central depth of each retained line
 __global__ void __launch_bounds__(128) k_linstr(DevSetup S, DevLines L, int list_type, const int *flag,
                                                 float *linstr) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -699,10 +766,18 @@ __global__ void __launch_bounds__(128) k_linstr(DevSetup S, DevLines L, int list
   double C2 = exp(50.7649141 - 5.0 * log(Wl));
   double imu[MAXMU], IntC[MAXMU], IntL[MAXMU];
   for (int m = 0; m < M; m++) imu[m] = 1.0 / S.xmu[m];
-  LineCentreCol cc{&S, &xc, iwl, xc.IxHe, dW, false};
-  locprf_pp(S, cc, C1, C2, imu, IntC);
-  LineCentreCol cl{&S, &xc, iwl, xc.IxHe, dW, true};
-  locprf_pp(S, cl, C1, C2, imu, IntL);
+  const LineCentreCol cols[2] = {{&S, &xc, iwl, xc.IxHe, dW, false}, {&S, &xc, iwl, xc.IxHe, dW, true}};
+  if (S.sph) {
+    for (int m = 0; m < M; m++) {
+      double Y[2];
+      delo_ray<2>(S, cols, m, C1, C2, Y);
+      IntC[m] = Y[0];
+      IntL[m] = Y[1];
+    }
+  } else {
+    locprf_pp(S, cols[0], C1, C2, imu, IntC);
+    locprf_pp(S, cols[1], C1, C2, imu, IntL);
+  }
   float r;
   if (M == 1) {
     r = (float)(1.0 - IntL[0] / IntC[0]);
@@ -803,12 +878,14 @@ void launch_linstr(const DevSetup &S, const DevLines &L, int list_type, const in
 
 void launch_radtran32(const DevSetup &S, const float *lop, double *spl, double *spc, double *sfl, double *sfc,
                       hipStream_t st) {
-  k_radtran<float><<<nblk(S.nwl, 256), 256, 0, st>>>(S, lop, spl, spc, sfl, sfc);
+  if (S.sph) k_radtran_sph<float><<<nblk(S.nwl, 128), 128, 0, st>>>(S, lop, spl, spc, sfl, sfc);
+  else k_radtran<float><<<nblk(S.nwl, 256), 256, 0, st>>>(S, lop, spl, spc, sfl, sfc);
 }
 
 void launch_radtran64(const DevSetup &S, const double *lop, double *spl, double *spc, double *sfl, double *sfc,
                       hipStream_t st) {
-  k_radtran<double><<<nblk(S.nwl, 256), 256, 0, st>>>(S, lop, spl, spc, sfl, sfc);
+  if (S.sph) k_radtran_sph<double><<<nblk(S.nwl, 128), 128, 0, st>>>(S, lop, spl, spc, sfl, sfc);
+  else k_radtran<double><<<nblk(S.nwl, 256), 256, 0, st>>>(S, lop, spl, spc, sfl, sfc);
 }
 
 template __global__ void k_radtran<float>(DevSetup, const float *, double *, double *, double *, double *);

@@ -7,8 +7,8 @@
 //   WVoigtExact                           src/voigt.f90:356-393
 //   MagLineopCalc pass 1                  src/synth.f90:1182-1273
 //   MagLineOpacity1/2, RefineIdx          src/opacity.f90:391-520, 611-675
-//   MagRadTranCalc field geometry         src/synth.f90:1935-1994
-//   StokesPrfPP, Solve4x4                 src/solver.f90:265-394, 541-588
+//   MagRadTranCalc field geometry         src/synth.f90:1935-2090
+//   StokesPrfPP, StokesPrfSph, Solve4x4   src/solver.f90:265-394, 402-538, 541-588
 #include "fsg_device.h"
 #include "fsg_kernels.h"
 
@@ -433,13 +433,15 @@ __global__ void __launch_bounds__(1024) k_magopacity(DevSetup S, const float2 *_
 }
 
 // ------------------------------------------------------------------------------------
-// Stokes DELO-Bezier (StokesPrfPP, src/solver.f90:265-394): one lane per (mu, lambda),
-// lambda fastest; FP64, the depth recursion streamed upward over a sliding window of
-// three depths.  The normalised propagation matrix K' has six distinct entries
-// (PhiQ, PhiU, PhiV, PsiQ, PsiU, PsiV) and zeros on the diagonal; CentDer is odd in its
-// data, so the derivative of K' is K' built from the derivatives of the six entries.
+// Stokes DELO-Bezier (StokesPrfPP src/solver.f90:265-394, StokesPrfSph 402-538): one lane
+// per (mu, lambda), lambda fastest; FP64, the recursion streamed from the deepest point
+// upward over a sliding window of three points.  A ray point samples one model layer: the
+// layer itself in plane-parallel geometry, the ray map of SphereInit on spherical rays.
+// The normalised propagation matrix K' has six distinct entries (PhiQ, PhiU, PhiV, PsiQ,
+// PsiU, PsiV) and zeros on the diagonal; CentDer is odd in its data, so the derivative of
+// K' is K' built from the derivatives of the six entries.
 // ------------------------------------------------------------------------------------
-struct Geo {  // per mu: sin^2 gamma, cos^2 gamma, cos gamma, sin 2 chi, cos 2 chi
+struct Geo {  // sin^2 gamma, cos^2 gamma, cos gamma, sin 2 chi, cos 2 chi
   double sphi2, cphi2, cphi, s2psi, c2psi;
 };
 
@@ -449,26 +451,35 @@ struct SDepth {
   double s4[4];  // source vector S (1, PhiQ, PhiU, PhiV)
 };
 
+// the opacities of one wavelength point seen along one ray
 template <typename LopT, typename MlopT>
-struct StokesCol {
+struct StokesRay {
   const DevSetup *S;
   const LopT *lop;
   const MlopT *mlop;
   size_t p;
-  Geo g;
   double C1, C2;
-  __device__ __forceinline__ double phi_i(int d, double m0, double m1, double m2) const {
-    return 0.5 * (m0 * g.sphi2 + 0.5 * (m2 + m1) * (1.0 + g.cphi2)) + (double)lop[(size_t)d * S->nwl + p] +
-           S->contT[(size_t)d * S->nwl + p];
-  }
+  const int *map;  // ray point -> layer (nullptr: identity)
+  const Geo *geo;  // geometry of point j at geo[j * gs]
+  int gs;
+  __device__ __forceinline__ int layer(int j) const { return map ? map[j] : j; }
   __device__ __forceinline__ double mval(int q, int d) const {
     return (double)mlop[((size_t)q * S->ntau + d) * S->nwl + p];
   }
-  __device__ __forceinline__ double phi(int d) const { return phi_i(d, mval(0, d), mval(1, d), mval(2, d)); }
-  __device__ __forceinline__ void at(int d, SDepth &o) const {
+  __device__ __forceinline__ double phi_i(int d, const Geo &g, double m0, double m1, double m2) const {
+    return 0.5 * (m0 * g.sphi2 + 0.5 * (m2 + m1) * (1.0 + g.cphi2)) + (double)lop[(size_t)d * S->nwl + p] +
+           S->contT[(size_t)d * S->nwl + p];
+  }
+  __device__ __forceinline__ double phi(int j) const {
+    const int d = layer(j);
+    return phi_i(d, geo[(size_t)j * gs], mval(0, d), mval(1, d), mval(2, d));
+  }
+  __device__ __forceinline__ void at(int j, SDepth &o) const {
+    const int d = layer(j);
+    const Geo g = geo[(size_t)j * gs];
     const double Php = mval(0, d), Phb = mval(1, d), Phr = mval(2, d);
     const double Psp = mval(3, d), Psb = mval(4, d), Psr = mval(5, d);
-    o.phi = phi_i(d, Php, Phb, Phr);
+    o.phi = phi_i(d, g, Php, Phb, Phr);
     const double InvPhiI = 1.0 / o.phi;
     const double Sabs = 0.5 * (Php - 0.5 * (Phr + Phb)) * g.sphi2;
     o.k6[0] = Sabs * g.c2psi * InvPhiI;
@@ -568,47 +579,40 @@ __device__ __forceinline__ void solve4x4(const double *A, const double *B, doubl
   X[0] = (R[0] - M[0][1] * X[1] - M[0][2] * X[2] - M[0][3] * X[3]) / M[0][0];
 }
 
-// TauStep (src/solver.f90:591-615) element i (0-based) along mu: Dx scaled by 1/Xmu;
-// km, k0, kp = PhiI at depths i-1, i, i+1 (km unused for i = 0)
-__device__ __forceinline__ double tau_step_mu(const double *rhox, double xmu, int i, double km, double k0,
-                                              double kp) {
-  const double Dx1 = (rhox[i + 1] - rhox[i]) / xmu;
-  double Yp;
-  if (i == 0) {
-    Yp = (kp - k0) / Dx1;
-  } else {
-    const double Dx = (rhox[i] - rhox[i - 1]) / xmu;
-    const double Der = (k0 - km) / Dx, Der1 = (kp - k0) / Dx1;
-    if (Der * Der1 > 0.0) {
-      const double Lambda = (1.0 + Dx1 / (Dx + Dx1)) / 3.0;
-      Yp = (Der / (Lambda * Der1 + (1.0 - Lambda) * Der)) * Der1;
-    } else {
-      Yp = 0.0;
-    }
-  }
-  return Dx1 * (kp + 2.0 * k0 + Dx1 / 2.0 * Yp) / 3.0;
-}
-
+// geo: [nmu] angles (plane-parallel) or [nmu][2 ntau] angles per ray point (spherical)
 template <typename LopT, typename MlopT>
 __global__ void __launch_bounds__(64) k_stokes(DevSetup S, const LopT *__restrict__ lop,
                                                const MlopT *__restrict__ mlop, const Geo *__restrict__ geo,
                                                double *__restrict__ spz) {
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int M = S.nmu, N = S.ntau, Nw = S.nwl;
+  const int M = S.nmu, Nw = S.nwl, L = 2 * S.ntau;
   if (gid >= (int64_t)Nw * M) return;
   const int im = (int)(gid / Nw), p = (int)(gid - (int64_t)im * Nw);
   const double Wl = S.wave[p] + S.wfirst;
-  const StokesCol<LopT, MlopT> col{&S, lop, mlop, (size_t)p, geo[im], kPlkC1 / Wl, exp(50.7649141 - 5.0 * log(Wl))};
-  const double xmu = S.xmu[im];
-  // window: w0 = depth i-1, w1 = depth i, w2 = depth i+1; dt_i = Deltau(i), dt_im1 = Deltau(i-1)
+  const bool sph = S.sph != 0;
+  // the ray: its points, column mass and the Xmu of the optical depth steps
+  const int N = sph ? S.ntau_mu[im] : S.ntau;
+  const double *rhox = sph ? S.rhox_mu + (size_t)im * L : S.rhox;
+  const double xmu = sph ? 1.0 : S.xmu[im];
+  const bool graz = sph && S.graz_mu[im] != 0;
+  const StokesRay<LopT, MlopT> col{&S,
+                                   lop,
+                                   mlop,
+                                   (size_t)p,
+                                   kPlkC1 / Wl,
+                                   exp(50.7649141 - 5.0 * log(Wl)),
+                                   sph ? S.map_mu + (size_t)im * L : nullptr,
+                                   sph ? geo + (size_t)im * L : geo + im,
+                                   sph ? 1 : 0};
+  // window: w0 = point i-1, w1 = point i, w2 = point i+1; dt_i = Deltau(i), dt_im1 = Deltau(i-1)
   SDepth w0, w1, w2;
   int i = N - 2;
   col.at(N - 1, w2);
   col.at(N - 2, w1);
   col.at(N - 3, w0);
-  double dt_i = tau_step_mu(S.rhox, xmu, i, w0.phi, w1.phi, w2.phi);
-  double dt_im1 = tau_step_mu(S.rhox, xmu, i - 1, (i >= 2) ? col.phi(i - 2) : 0.0, w0.phi, w1.phi);
-  // derivatives at the bottom (CentDer's one-sided last point)
+  double dt_i = tau_step_mu(rhox, xmu, i, w0.phi, w1.phi, w2.phi);
+  double dt_im1 = tau_step_mu(rhox, xmu, i - 1, (i >= 2) ? col.phi(i - 2) : 0.0, w0.phi, w1.phi);
+  // derivatives at the deepest point (CentDer's one-sided last point)
   double kd_ip1[6], sd_ip1[4];
 #pragma unroll
   for (int q = 0; q < 6; q++) kd_ip1[q] = (w2.k6[q] - w1.k6[q]) / dt_i;
@@ -616,7 +620,8 @@ __global__ void __launch_bounds__(64) k_stokes(DevSetup S, const LopT *__restric
   for (int q = 0; q < 4; q++) sd_ip1[q] = (w2.s4[q] - w1.s4[q]) / dt_i;
   double XKB[16];
   ksq_plus(w2.k6, kd_ip1, XKB);
-  double X2[4] = {w2.s4[0], 0.0, 0.0, 0.0};
+  // thermalised and unpolarised at the deepest layer; grazing rays enter with zero
+  double X2[4] = {graz ? 0.0 : w2.s4[0], 0.0, 0.0, 0.0};
   for (; i >= 0; i--) {
     double kd_i[6], sd_i[4];
     if (i == 0) {
@@ -657,7 +662,7 @@ __global__ void __launch_bounds__(64) k_stokes(DevSetup S, const LopT *__restric
       X1[q] = Eps * X2[q] - Beta * v1[q] + Gamma * v2[q] + (Beta + Gamma) * w2.s4[q] - GD * v3[q] +
               (Alpha + Gamma) * w1.s4[q] + GD * v4[q] + GD * (sd_i[q] - sd_ip1[q]);
     solve4x4(AA, X1, X2);
-    // move the window up one depth
+    // move the window up one point
 #pragma unroll
     for (int q = 0; q < 6; q++) kd_ip1[q] = kd_i[q];
 #pragma unroll
@@ -667,7 +672,7 @@ __global__ void __launch_bounds__(64) k_stokes(DevSetup S, const LopT *__restric
     dt_i = dt_im1;
     if (i >= 2) {
       col.at(i - 2, w0);
-      dt_im1 = tau_step_mu(S.rhox, xmu, i - 2, (i >= 3) ? col.phi(i - 3) : 0.0, w0.phi, w1.phi);
+      dt_im1 = tau_step_mu(rhox, xmu, i - 2, (i >= 3) ? col.phi(i - 3) : 0.0, w0.phi, w1.phi);
     }
   }
 #pragma unroll

@@ -50,6 +50,8 @@ MODULE FsgpuHost
     INTEGER(C_INT32_T) :: molbrd
     TYPE(C_PTR) :: xmu, wmu
     INTEGER(C_INT32_T) :: precision, batched, device
+    REAL(C_DOUBLE) :: radius
+    TYPE(C_PTR) :: height
   END TYPE FsgSetup
 
   ! mirror of fsg_lines
@@ -164,6 +166,7 @@ CONTAINS
   SUBROUTINE GpuInit(Precision, Device)
     INTEGER, INTENT(IN), OPTIONAL :: Precision, Device
     REAL(8), ALLOCATABLE, TARGET :: Rhox(:), T(:), Xne(:), Rho(:), Cstd(:), HeW(:), HeE(:), HeG(:,:), HeS(:,:)
+    REAL(8), ALLOCATABLE, TARGET :: Hgt(:)
     REAL(4), ALLOCATABLE, TARGET :: Fh1(:), Fhe1(:), Fhmol(:)
     INTEGER(C_INT32_T), ALLOCATABLE, TARGET :: SpcIon(:), SpcSlot(:)
     INTEGER :: k, Nspc_, Hmol
@@ -176,7 +179,9 @@ CONTAINS
     GpuPrecision = Prec
 
     ALLOCATE(Rhox(Ntau), T(Ntau), Xne(Ntau), Rho(Ntau), Cstd(Ntau), Fh1(Ntau), Fhe1(Ntau), Fhmol(Ntau))
+    ALLOCATE(Hgt(Ntau))
     DO k = 1, Ntau
+      Hgt(k) = Atmos(k)%Height
       Rhox(k) = Atmos(k)%Rhox
       T(k) = Atmos(k)%T
       Xne(k) = Atmos(k)%Xne
@@ -199,15 +204,16 @@ CONTAINS
     HeG = HeGam
     HeS = HeShf
     CALL GpuInitArrays(Wave, Rhox, T, Xne, Rho, ContOpac, Cstd, FractionPF, PotIon, Xamass, &
-                       Fh1, Fhe1, Fhmol, SpcIon, SpcSlot, HeW, HeE, HeG, HeS, Xmu, Wmu, Prec, Dev)
+                       Fh1, Fhe1, Fhmol, SpcIon, SpcSlot, HeW, HeE, HeG, HeS, Xmu, Wmu, Hgt, Prec, Dev)
 
   CONTAINS
 
     ! the C pointers are taken from TARGET dummies and used only during this call
     SUBROUTINE GpuInitArrays(Wave_, Rhox_, T_, Xne_, Rho_, Cont_, Cstd_, Fpf_, Pot_, Xam_, &
-                             Fh1_, Fhe1_, Fhmol_, Ion_, Slot_, HeW_, HeE_, HeG_, HeS_, Xmu_, Wmu_, Prec_, Dev_)
+                             Fh1_, Fhe1_, Fhmol_, Ion_, Slot_, HeW_, HeE_, HeG_, HeS_, Xmu_, Wmu_, Hgt_, &
+                             Prec_, Dev_)
       REAL(8), INTENT(IN), TARGET :: Wave_(*), Rhox_(*), T_(*), Xne_(*), Rho_(*), Cont_(*), Cstd_(*)
-      REAL(8), INTENT(IN), TARGET :: HeW_(*), HeE_(*), HeG_(*), HeS_(*), Xmu_(*), Wmu_(*)
+      REAL(8), INTENT(IN), TARGET :: HeW_(*), HeE_(*), HeG_(*), HeS_(*), Xmu_(*), Wmu_(*), Hgt_(*)
       REAL(4), INTENT(IN), TARGET :: Fpf_(*), Pot_(*), Xam_(*), Fh1_(*), Fhe1_(*), Fhmol_(*)
       INTEGER(C_INT32_T), INTENT(IN), TARGET :: Ion_(*), Slot_(*)
       INTEGER, INTENT(IN) :: Prec_, Dev_
@@ -260,6 +266,10 @@ CONTAINS
       S%precision = Prec_
       S%batched = MERGE(1, 0, Prec_ == FSG_FP32)
       S%device = Dev_
+      ! spherical models: Radius > 0 and the layer heights (src/solver.f90:51)
+      S%radius = Radius
+      S%height = C_NULL_PTR
+      IF (Radius > 0d0) S%height = C_LOC(Hgt_)
       CALL GpuCheck(fsg_init(S, GpuCtx))
     END SUBROUTINE GpuInitArrays
 
