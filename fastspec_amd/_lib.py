@@ -15,7 +15,8 @@ EXPORTS = (
     "fsg_init", "fsg_finalize", "fsg_last_error", "fsg_lineop", "fsg_maglineop",
     "fsg_get_lineop", "fsg_set_lineop", "fsg_get_maglineop", "fsg_radtran",
     "fsg_magradtran", "fsg_flux", "fsg_lines_upload", "fsg_lines_free", "fsg_lineop_dev",
-    "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop", "fsg_maglineop_dev",
+    "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop", "fsg_maglineop_dev", "fsg_lines_set_gf",
+    "fsg_flux_vjp", "fsg_lineop_vjp",
 )
 
 _LIB = None
@@ -53,6 +54,9 @@ def load() -> C.CDLL:
     lib.fsg_lines_free.argtypes = [vp]
     lib.fsg_lineop_dev.argtypes = [vp, vp, C.c_int64, C.POINTER(CStats)]
     lib.fsg_maglineop_dev.argtypes = [vp, vp, C.c_int64, dp, C.c_int32, C.POINTER(CStats)]
+    lib.fsg_lines_set_gf.argtypes = [vp, vp, C.c_int32]
+    lib.fsg_flux_vjp.argtypes = [vp, dp, dp, dp, dp, dp]
+    lib.fsg_lineop_vjp.argtypes = [vp, vp, C.c_int64, dp, dp, dp]
     lib.fsg_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp)]
     lib.fsg_stream.argtypes = [vp, C.POINTER(vp)]
     lib.fsg_reset_lineop.argtypes = [vp]

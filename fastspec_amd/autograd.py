@@ -1,0 +1,72 @@
+"""PyTorch autograd over libfsgpu: the normalised flux of one synthesis as a differentiable
+function of the lines' log gf, the temperature of each depth and the EOS number densities.
+
+    F/Fc = NormalisedFlux.apply(loggf, temp, lnfpf, synth, dlines, nlines_lim)
+
+Forward runs the same stages as the reference's synthesis for one list (LineopCalc,
+RadTranCalc, the flux sums of IntegConvol, src/fastspec.f90:241-449) on the GPU; backward
+runs the library's adjoint kernels (fsg_flux_vjp, fsg_lineop_vjp).  ``temp`` and ``lnfpf``
+(the log of FractionPF relative to the context's values, all zeros) are the points at
+which dL/dT and dL/d ln FractionPF are taken: the host's EOS and continuum, which map T and
+abundances onto the library's inputs, stay outside, so a new atmosphere needs a new
+context.  The element-abundance gradient of a trace species is the sum of its slots' rows:
+dL/d log10 A = ln 10 sum_depth dL/d ln FractionPF.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .synth import DeviceLines, FastSpecGPU
+
+
+class NormalisedFlux(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, loggf, temp, lnfpf, synth: FastSpecGPU, dlines: DeviceLines, nlines_lim=None):
+        s = synth.setup
+        t_now = torch.as_tensor(s.temp, dtype=torch.float64)
+        if not torch.equal(temp.detach().to("cpu", torch.float64), t_now):
+            raise ValueError("NormalisedFlux: temp differs from the context's atmosphere; build a new context")
+        if bool(torch.any(lnfpf.detach() != 0)):
+            raise ValueError("NormalisedFlux: lnfpf must be zero (FractionPF is fixed by the context)")
+        # gf = 10**log gf in REAL(4), as the host's reader converts it
+        gf = torch.pow(torch.tensor(10.0, dtype=torch.float32, device=loggf.device), loggf.detach().float())
+        if gf.is_cuda:
+            torch.cuda.current_stream(gf.device).synchronize()
+            synth.set_gf(dlines, gf.contiguous())
+        else:
+            synth.set_gf(dlines, gf.numpy())
+        synth.reset_lineop()
+        synth.lineop_dev(dlines, nlines_lim)
+        synth.RadTranCalc(intensities=False)
+        sfl, sfc = synth.flux()
+        ctx.synth, ctx.dlines, ctx.nlim = synth, dlines, nlines_lim
+        ctx.sfl, ctx.sfc = sfl, sfc
+        ctx.meta = [(x.dtype, x.device) for x in (loggf, temp, lnfpf)]
+        return torch.as_tensor(sfl / sfc, dtype=torch.float64, device=loggf.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        g = g.detach().to("cpu", torch.float64).numpy()
+        sfl, sfc = ctx.sfl, ctx.sfc
+        # d(F/Fc): dL/dSfL = g / Fc, dL/dSfC = -g F / Fc^2
+        _, _, gt_src = ctx.synth.flux_vjp(g / sfc, -g * sfl / (sfc * sfc))
+        g_loggf, g_fpf, gt_line = ctx.synth.lineop_vjp(ctx.dlines, ctx.nlim)
+        out = [torch.as_tensor(a).to(dtype=dt, device=dev)
+               for a, (dt, dev) in zip((g_loggf, gt_src + gt_line, g_fpf), ctx.meta)]
+        return (*out, None, None, None)
+
+
+def normalised_flux(synth: FastSpecGPU, dlines: DeviceLines, loggf: torch.Tensor, temp=None, lnfpf=None,
+                    nlines_lim=None) -> torch.Tensor:
+    """F/Fc [nwl] of one synthesis; differentiable in loggf (and temp / lnfpf when given
+    with requires_grad)."""
+    s = synth.setup
+    if temp is None:
+        temp = torch.as_tensor(np.array(s.temp), dtype=torch.float64)
+    if lnfpf is None:
+        lnfpf = torch.zeros(s.fraction_pf.shape, dtype=torch.float64)
+    return NormalisedFlux.apply(loggf, temp, lnfpf, synth, dlines, nlines_lim)
+
+
+__all__ = ["NormalisedFlux", "normalised_flux"]

@@ -13,7 +13,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libfsgpu.so"
-SOURCES = ["fsg_kernels.hip", "fsg_magnetic.hip", "fsg_api.hip"]
+SOURCES = ["fsg_kernels.hip", "fsg_magnetic.hip", "fsg_adjoint.hip", "fsg_api.hip"]
 ARCH = os.environ.get("FSG_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for all device code.

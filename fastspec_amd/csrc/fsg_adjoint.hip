@@ -1,0 +1,511 @@
+// fsg_adjoint.hip -- reverse mode of the plane-parallel formal solution with the flux sums
+// and of the line opacity: the gradients of a loss L(SfL, SfC) with respect to LineOpac,
+// ContOpac, the temperature of each depth, log gf of each line and the EOS number
+// densities FractionPF, at fixed host inputs (continuum, EOS, electron density).
+//
+// Forward routines differentiated here (file:line in the reference tree):
+//   LocPrfPP, TauStep, CentDer, DELO coefficients   src/solver.f90:123-179, 591-647
+//   flux sums of DiskInteg                          src/convol.f90:233-234
+//   XlineDepth, HVoigt, LineOpacity1                src/opacity.f90:239-346, src/voigt.f90:76-110
+#include "fsg_device.h"
+#include "fsg_kernels.h"
+
+namespace fsg {
+
+// ------------------------------------------------------------------------------------
+// derivatives of the forward pieces
+// ------------------------------------------------------------------------------------
+// DELO-Bezier coefficients and their derivatives with respect to Dtau (src/solver.f90:154-169)
+__device__ __forceinline__ void delo_coef_d(double x, double &E, double &A, double &B, double &G, double &dE,
+                                            double &dA, double &dB, double &dG) {
+  delo_coef(x, E, A, B, G);
+  const double x2 = x * x;
+  if (x < 5e-3) {
+    dE = -1.0 + x - 0.5 * x2 + x2 * x / 6.0;
+    dA = 1.0 / 3.0 - x / 6.0 + x2 / 20.0;
+    dB = 1.0 / 3.0 - 0.5 * x + 0.3 * x2;
+    dG = 1.0 / 3.0 - x / 3.0 + 0.15 * x2;
+  } else {
+    dE = (x < 100.0) ? -E : 0.0;
+    const double ix2 = 1.0 / x2;
+    dA = (2.0 * x - 2.0 - 2.0 * dE) * ix2 - 2.0 * A / x;
+    dB = (-(2.0 + 2.0 * x) * E - (2.0 + 2.0 * x + x2) * dE) * ix2 - 2.0 * B / x;
+    dG = (2.0 + 2.0 * E + (2.0 * x + 4.0) * dE) * ix2 - 2.0 * G / x;
+  }
+}
+
+// the monotone harmonic-mean slope Yp = Der Der1 / (l Der1 + (1 - l) Der) (zero unless the
+// one-sided slopes agree in sign) and its partial derivatives
+struct Slope {
+  double yp, d_der, d_der1, d_l;
+};
+__device__ __forceinline__ Slope mono_slope(double der, double der1, double l) {
+  Slope s{0.0, 0.0, 0.0, 0.0};
+  if (der * der1 > 0.0) {
+    const double Q = l * der1 + (1.0 - l) * der, iq2 = 1.0 / (Q * Q);
+    s.yp = (der / Q) * der1;
+    s.d_der = l * der1 * der1 * iq2;
+    s.d_der1 = (1.0 - l) * der * der * iq2;
+    s.d_l = -der * der1 * (der1 - der) * iq2;
+  }
+  return s;
+}
+
+// TauStep element j (Xmu = 1): d Deltau(j) / d kappa at points j-1, j, j+1
+__device__ __forceinline__ void tau_step_grad(const double *rhox, int j, double km, double k0, double kp, double &gm,
+                                              double &g0, double &gp) {
+  const double Dx1 = rhox[j + 1] - rhox[j];
+  if (j == 0) {  // Yp = (kp - k0) / Dx1: Deltau = Dx1 (kp + k0) / 2 up to rounding
+    gm = 0.0;
+    g0 = Dx1 * (2.0 - 0.5) / 3.0;
+    gp = Dx1 * (1.0 + 0.5) / 3.0;
+    return;
+  }
+  const double Dx = rhox[j] - rhox[j - 1];
+  const double der = (k0 - km) / Dx, der1 = (kp - k0) / Dx1;
+  const double l = (1.0 + Dx1 / (Dx + Dx1)) / 3.0;
+  const Slope s = mono_slope(der, der1, l);
+  const double h = Dx1 * Dx1 / 6.0;  // d Deltau / d Yp
+  gm = h * (-s.d_der / Dx);
+  g0 = 2.0 * Dx1 / 3.0 + h * (s.d_der / Dx - s.d_der1 / Dx1);
+  gp = Dx1 / 3.0 + h * (s.d_der1 / Dx1);
+}
+
+// CentDer interior element Sd(i) = f(Dx = Deltau(i-1), Dx1 = Deltau(i); y(i-1), y(i), y(i+1))
+// and its partial derivatives
+struct CentGrad {
+  double dx, dx1, ym, y0, yp;
+};
+__device__ __forceinline__ CentGrad cent_der_grad(double Dx, double Dx1, double ym, double y0, double yp) {
+  const double der = (y0 - ym) / Dx, der1 = (yp - y0) / Dx1;
+  const double l = (1.0 + Dx1 / (Dx + Dx1)) / 3.0;
+  const Slope s = mono_slope(der, der1, l);
+  const double ss = (Dx + Dx1) * (Dx + Dx1);
+  CentGrad g;
+  g.dx = s.d_der * (-der / Dx) + s.d_l * (-Dx1 / (3.0 * ss));
+  g.dx1 = s.d_der1 * (-der1 / Dx1) + s.d_l * (Dx / (3.0 * ss));
+  g.ym = -s.d_der / Dx;
+  g.y0 = s.d_der / Dx - s.d_der1 / Dx1;
+  g.yp = s.d_der1 / Dx1;
+  return g;
+}
+
+// ------------------------------------------------------------------------------------
+// reverse mode of LocPrfPP + flux for one opacity column of one wavelength point
+// ------------------------------------------------------------------------------------
+// Forward, from the deepest point up (as LocPrfPP): Deltau(i) by TauStep, Sd = CentDer(S),
+// U(i) = (S(i) + Deltau Sd(i) / 2 + S(i+1) - Deltau Sd(i+1) / 2) / 2 and per mu
+// Y <- Y eps + alpha S(i) + beta S(i+1) + gamma U(i) with the coefficients at Deltau / mu;
+// F = sum_mu Wmu Y.  The intensities entering each step are kept in scratch (the
+// checkpoint of the reverse sweep).  Reverse, from the top down: adjoints of Y (per mu), U,
+// Sd, Deltau, kappa and S, each emitted as soon as no later step can add to it.
+// Scratch rows (stride ld, one column per lane): Yin [(N-1) M], Deltau [N], Sd [N], S [N];
+// the adjoints of S go to gsrc (rows of stride ld).
+template <typename Col>
+__device__ void rt_vjp_column(const DevSetup &S, const Col &col, double gF, const double *imu, double C1, double C2,
+                              double *scr, size_t ld, double *gkap, size_t gstride, double *gsrc) {
+  const int N = S.ntau, M = S.nmu;
+  double *Yin = scr, *dts = scr + (size_t)(N - 1) * M * ld, *sds = dts + (size_t)N * ld, *srcs = sds + (size_t)N * ld;
+  // ---- forward sweep (streamed as locprf_pp), storing the checkpoint
+  {
+    int i = N - 2;
+    double S_ip1 = planck_src(C1, C2, S.temp[N - 1]);
+    double S_i = planck_src(C1, C2, S.temp[N - 2]);
+    double S_im1 = planck_src(C1, C2, S.temp[N - 3]);
+    srcs[(size_t)(N - 1) * ld] = S_ip1;
+    double dt_i = tau_step_mu(S.rhox, 1.0, i, col.k(i - 1), col.k(i), col.k(i + 1));
+    double dt_im1 = tau_step_mu(S.rhox, 1.0, i - 1, (i >= 2) ? col.k(i - 2) : 0.0, col.k(i - 1), col.k(i));
+    double Sd_ip1 = (S_ip1 - S_i) / dt_i;
+    sds[(size_t)(N - 1) * ld] = Sd_ip1;
+    double Y[MAXMU];
+    for (int m = 0; m < M; m++) Y[m] = S_ip1;
+    for (; i >= 0; i--) {
+      const double Sd_i = (i == 0) ? (S_ip1 - S_i) / dt_i : cent_der_i(dt_im1, dt_i, S_im1, S_i, S_ip1);
+      const double U = 0.5 * (S_i + 0.5 * dt_i * Sd_i + S_ip1 - 0.5 * dt_i * Sd_ip1);
+      dts[(size_t)i * ld] = dt_i;
+      sds[(size_t)i * ld] = Sd_i;
+      srcs[(size_t)i * ld] = S_i;
+      for (int m = 0; m < M; m++) {
+        Yin[((size_t)i * M + m) * ld] = Y[m];
+        double Eps, Alpha, Beta, Gamma;
+        delo_coef(dt_i * imu[m], Eps, Alpha, Beta, Gamma);
+        Y[m] = Y[m] * Eps + Alpha * S_i + Beta * S_ip1 + Gamma * U;
+      }
+      Sd_ip1 = Sd_i;
+      S_ip1 = S_i;
+      S_i = S_im1;
+      dt_i = dt_im1;
+      if (i >= 2) {
+        S_im1 = planck_src(C1, C2, S.temp[i - 2]);
+        dt_im1 = tau_step_mu(S.rhox, 1.0, i - 2, (i >= 3) ? col.k(i - 3) : 0.0, col.k(i - 2), col.k(i - 1));
+      }
+    }
+  }
+  // ---- reverse sweep
+  double yb[MAXMU];
+  for (int m = 0; m < M; m++) yb[m] = gF * S.wmu[m];
+  double sb_prev = 0.0, sb_cur = 0.0, sb_next = 0.0;  // S adjoints at i-1, i, i+1
+  double sdb_cur = 0.0, sdb_next = 0.0;               // Sd adjoints at i, i+1
+  double dtb_prev = 0.0, dtb_cur = 0.0;               // Deltau adjoints at i-1, i
+  double kb0 = 0.0, kb1 = 0.0, kb2 = 0.0;             // kappa adjoints at i-2, i-1, i
+  for (int i = 0; i <= N - 2; i++) {
+    const double dt = dts[(size_t)i * ld], Sd_i = sds[(size_t)i * ld], Sd_ip1 = sds[(size_t)(i + 1) * ld];
+    const double S_i = srcs[(size_t)i * ld], S_ip1 = srcs[(size_t)(i + 1) * ld];
+    const double U = 0.5 * (S_i + 0.5 * dt * Sd_i + S_ip1 - 0.5 * dt * Sd_ip1);
+    double ub = 0.0;
+    for (int m = 0; m < M; m++) {
+      double E, A, B, G, dE, dA, dB, dG;
+      delo_coef_d(dt * imu[m], E, A, B, G, dE, dA, dB, dG);
+      const double y = yb[m];
+      dtb_cur += y * (Yin[((size_t)i * M + m) * ld] * dE + dA * S_i + dB * S_ip1 + dG * U) * imu[m];
+      sb_cur += y * A;
+      sb_next += y * B;
+      ub += y * G;
+      yb[m] = y * E;
+    }
+    // U(i)
+    sb_cur += 0.5 * ub;
+    sb_next += 0.5 * ub;
+    sdb_cur += 0.25 * dt * ub;
+    sdb_next -= 0.25 * dt * ub;
+    dtb_cur += 0.25 * (Sd_i - Sd_ip1) * ub;
+    // Sd(i) is complete
+    if (i == 0) {
+      sb_next += sdb_cur / dt;
+      sb_cur -= sdb_cur / dt;
+      dtb_cur += sdb_cur * (-Sd_i / dt);
+    } else {
+      const double dtm = dts[(size_t)(i - 1) * ld], S_im1 = srcs[(size_t)(i - 1) * ld];
+      const CentGrad cg = cent_der_grad(dtm, dt, S_im1, S_i, S_ip1);
+      dtb_prev += sdb_cur * cg.dx;
+      dtb_cur += sdb_cur * cg.dx1;
+      sb_prev += sdb_cur * cg.ym;
+      sb_cur += sdb_cur * cg.y0;
+      sb_next += sdb_cur * cg.yp;
+      // Deltau(i-1) is complete: kappa at i-2, i-1, i
+      double gm, g0, gp;
+      tau_step_grad(S.rhox, i - 1, (i >= 2) ? col.k(i - 2) : 0.0, col.k(i - 1), col.k(i), gm, g0, gp);
+      kb0 += dtb_prev * gm;
+      kb1 += dtb_prev * g0;
+      kb2 += dtb_prev * gp;
+      if (i >= 2) gkap[(size_t)(i - 2) * gstride] = kb0;
+      gsrc[(size_t)(i - 1) * ld] = sb_prev;
+      kb0 = kb1;
+      kb1 = kb2;
+      kb2 = 0.0;
+    }
+    if (i == 0) {  // nothing below point 0 to retire: kappa window starts at points -1, 0, 1
+      kb0 = 0.0;
+      kb1 = 0.0;
+      kb2 = 0.0;
+    }
+    sb_prev = sb_cur;
+    sb_cur = sb_next;
+    sb_next = 0.0;
+    sdb_cur = sdb_next;
+    sdb_next = 0.0;
+    dtb_prev = dtb_cur;
+    dtb_cur = 0.0;
+  }
+  // deepest point: Sd(N-1) one-sided, the boundary Y = S(N-1), Deltau(N-2) complete
+  {
+    const double dt = dts[(size_t)(N - 2) * ld], Sd_last = sds[(size_t)(N - 1) * ld];
+    sb_cur += sdb_cur / dt;
+    sb_prev -= sdb_cur / dt;
+    dtb_prev += sdb_cur * (-Sd_last / dt);
+    for (int m = 0; m < M; m++) sb_cur += yb[m];
+    double gm, g0, gp;
+    tau_step_grad(S.rhox, N - 2, (N >= 3) ? col.k(N - 3) : 0.0, col.k(N - 2), col.k(N - 1), gm, g0, gp);
+    kb0 += dtb_prev * gm;
+    kb1 += dtb_prev * g0;
+    kb2 += dtb_prev * gp;
+    gkap[(size_t)(N - 3) * gstride] = kb0;
+    gkap[(size_t)(N - 2) * gstride] = kb1;
+    gkap[(size_t)(N - 1) * gstride] = kb2;
+    gsrc[(size_t)(N - 2) * ld] = sb_prev;
+    gsrc[(size_t)(N - 1) * ld] = sb_cur;
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One lane per wavelength point of [p_lo, p_lo + np): the line column (cont + LineOpac,
+// adjoint gsfl) and the continuum column (adjoint gsfc).  Outputs dL/dLineOpac and
+// dL/dContOpac [ntau][nwl] and the source-function part of dL/dT [ntau] (accumulated).
+template <typename LopT>
+__global__ void __launch_bounds__(64) k_rt_vjp(DevSetup S, const LopT *lop, const double *gsfl, const double *gsfc,
+                                               int p_lo, int np, double *scr, double *glop, double *gcont,
+                                               double *gtemp) {
+  const int lane_p = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = lane_p < np;
+  const int p = p_lo + (valid ? lane_p : 0);
+  const int N = S.ntau, M = S.nmu;
+  const double Wl = S.wave[p] + S.wfirst;
+  const double C1 = kPlkC1 / Wl, C2 = exp(50.7649141 - 5.0 * log(Wl));
+  double imu[MAXMU];
+  for (int m = 0; m < M; m++) imu[m] = 1.0 / S.xmu[m];
+  // scratch: the checkpoint rows, then the S adjoints of the line and continuum columns
+  const size_t ld = (size_t)np;
+  double *scr_l = scr + (valid ? lane_p : 0);
+  double *sL = scr_l + ((size_t)(N - 1) * M + 3 * (size_t)N) * ld, *sC = sL + (size_t)N * ld;
+  if (valid) {
+    // kappa adjoints go straight to the outputs; the line column's are added to the
+    // continuum's, since ContOpac enters both columns
+    GridCol<LopT> line{S.contT, lop, S.nwl, p, true};
+    rt_vjp_column(S, line, gsfl[p], imu, C1, C2, scr_l, ld, glop + p, (size_t)S.nwl, sL);
+    GridCol<LopT> cont{S.contT, lop, S.nwl, p, false};
+    rt_vjp_column(S, cont, gsfc[p], imu, C1, C2, scr_l, ld, gcont + p, (size_t)S.nwl, sC);
+    for (int k = 0; k < N; k++) gcont[(size_t)k * S.nwl + p] += glop[(size_t)k * S.nwl + p];
+  }
+  // dL/dT from the source function: (S-bar_L + S-bar_C) dS/dT, dS/dT = S (C1/T^2)(1 + S/C2)
+  for (int k = 0; k < N; k++) {
+    double v = 0.0;
+    if (valid) {
+      const double T = S.temp[k], Sk = planck_src(C1, C2, T);
+      v = (sL[(size_t)k * ld] + sC[(size_t)k * ld]) * Sk * (C1 / (T * T)) * (1.0 + Sk / C2);
+    }
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&gtemp[k], v);
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// reverse mode of the line opacity: per (line, depth) the sum over the line's window of
+// dL/dLineOpac x contribution, and the temperature derivative of the contributions
+// ------------------------------------------------------------------------------------
+// H(a, v) of the FP64 path with its partial derivatives: in the table box the bilinear
+// interpolant's slopes; outside it the continued fractions w(z) ~ i z / (sqrt(pi)(z^2 - 1/2))
+// (one pole) and i (z^2 - 1) / (sqrt(pi) z (z^2 - 3/2)) (two poles), H = Re w, z = |v| + i a,
+// dH/dv = Re w', dH/da = -Im w'.
+__device__ __forceinline__ double hvoigt64_d(const DevSetup &S, double A, int ia, double ua, double V, double &Hv,
+                                            double &Ha) {
+  const double av = fabs(V), sg = (V < 0.0) ? -1.0 : 1.0;
+  if (ia > 0 && av < 8.0) {
+    const double fv = av * S.idv;
+    const int iv = (int)fv;
+    const double tv = fv - (double)iv;
+    const float *t0 = S.vtab + (size_t)(ia - 1) * TAB_LD;
+    const float *t1 = t0 + TAB_LD;
+    const double a0 = t0[iv], a1 = t0[iv + 1], b0 = t1[iv], b1 = t1[iv + 1];
+    const double c0 = a0 + tv * (a1 - a0), c1 = b0 + tv * (b1 - b0);
+    Hv = sg * S.idv * ((1.0 - ua) * (a1 - a0) + ua * (b1 - b0));
+    Ha = (c1 - c0) * S.aida[ia - 1];
+    return c0 + ua * (c1 - c0);
+  }
+  const double a2 = A * A, v2 = av * av, s = v2 + a2;
+  // complex z = av + i A
+  const double zr = av, zi = A;
+  const double z2r = v2 - a2, z2i = 2.0 * av * A;
+  double H, wr, wi;  // w' = wr + i wi
+  if (s >= 196.0) {
+    H = kSqrtPiInv * A * (s + 0.5) / (s * s - v2 + a2 + 0.25);
+    // w' = -i K (z^2 + 1/2) / (z^2 - 1/2)^2
+    const double nr = z2r + 0.5, ni = z2i;
+    const double dr = z2r - 0.5, di = z2i;
+    const double d2r = dr * dr - di * di, d2i = 2.0 * dr * di, dd = d2r * d2r + d2i * d2i;
+    const double qr = (nr * d2r + ni * d2i) / dd, qi = (ni * d2r - nr * d2i) / dd;
+    wr = kSqrtPiInv * qi;
+    wi = -kSqrtPiInv * qr;
+  } else {
+    const double p = v2 - a2 - 1.5, q = 2.0 * av * A;
+    const double dr = av * p - A * q, di = av * q + A * p;
+    H = kSqrtPiInv * ((p + 0.5) * di - q * dr) / (dr * dr + di * di);
+    // f = (z^2 - 1) / (z^3 - 3z/2); f' = (2z (z^3 - 3z/2) - (z^2 - 1)(3z^2 - 3/2)) / (z^3 - 3z/2)^2;
+    // w' = i K f'
+    const double z3r = z2r * zr - z2i * zi, z3i = z2r * zi + z2i * zr;
+    const double gr = z3r - 1.5 * zr, gi = z3i - 1.5 * zi;     // g = z^3 - 3z/2
+    const double t1r = 2.0 * (zr * gr - zi * gi), t1i = 2.0 * (zr * gi + zi * gr);
+    const double ar = z2r - 1.0, ai = z2i, br = 3.0 * z2r - 1.5, bi = 3.0 * z2i;
+    const double t2r = ar * br - ai * bi, t2i = ar * bi + ai * br;
+    const double nr = t1r - t2r, ni = t1i - t2i;
+    const double g2r = gr * gr - gi * gi, g2i = 2.0 * gr * gi, dd = g2r * g2r + g2i * g2i;
+    const double fr = (nr * g2r + ni * g2i) / dd, fi = (ni * g2r - nr * g2i) / dd;
+    wr = -kSqrtPiInv * fi;
+    wi = kSqrtPiInv * fr;
+  }
+  Hv = sg * wr;
+  Ha = -wi;
+  return H;
+}
+
+// per-(line, depth) records for the reverse mode: the FP64 path's coefficients with
+// d ln LinCop / dT, d ln LinVvgt / dT and d LinAvgt / dT in the record's spare words
+__global__ void k_fill64_vjp(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+                             const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s, int4 *meta_s,
+                             int *slot_s) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nret) return;
+  XC xc;
+  line_meta(S, L, list_type, j, order, flag, il, ir, ic, xc, wlc_s, meta_s);
+  slot_s[j] = xc.IxPF;
+  const int N = S.ntau;
+  Coef64 *c = coef + (size_t)j * S.ntp;
+  for (int k = N; k < S.ntp; k++) {
+    Coef64 r{};
+    r.ia = 1;
+    c[k] = r;
+  }
+  for (int k = 0; k < N; k++) {
+    double cop, vv, av, ua;
+    int ia;
+    xline_depth(S, k, xc, cop, vv, av);
+    voigt_astate(S, av, ia, ua);
+    Coef64 r{};
+    r.vv = vv;
+    r.a = av;
+    r.cop = cop;
+    r.ua = ua;
+    r.ia = ia;
+    // temperature derivatives of XlineDepth (src/opacity.f90:247-291) at fixed electron
+    // and perturber densities
+    const double T = S.temp[k];
+    const double eh = exp(-xc.Hnux / S.xln[4 * N + k]);
+    const double dlnXstim = -eh * (xc.Hnux / (S.xln[4 * N + k] * T)) / (1.0 - eh);
+    const double dlnEfract = xc.Elo / (S.xln[5 * N + k] * T);
+    const double Dopl2 = T * xc.Xmass + S.vturb2;
+    const double dlnDopl = xc.Xmass / (2.0 * Dopl2);
+    double dGam = 0.0;
+    if (xc.MdQst == 0) dGam += xc.Cqst * S.xln[2 * N + k] / (6.0 * T);
+    if (xc.MdVdw == 0) dGam += 0.3 * xc.Cvdw * S.xln[3 * N + k] / T;
+    else if (xc.MdVdw == 1) dGam += 0.3 * S.vwfudge * S.xln[1 * N + k] * xc.Cvdw / T;
+    else if (xc.MdVdw == 2) {
+      const double Vbar = sqrt(8.0 * kXlnKM0 * T / kPi * xc.AboVbc);
+      dGam += 0.5 * xc.AboExp * xc.AboPre * pow(Vbar / 1e4, xc.AboExp) * S.xln[0 * N + k] / T;
+    }
+    const double Dndopl = sqrt(Dopl2) / xc.WaveL;
+    r.pad1 = dlnEfract + dlnXstim - dlnDopl;           // d ln LinCop / dT
+    r.pad2 = -dlnDopl;                                 // d ln LinVvgt / dT
+    r.pad3 = dGam / (Dndopl * kXlnC4Pi) - av * dlnDopl;  // d LinAvgt / dT
+    c[k] = r;
+  }
+}
+
+// One block per TILE-point wavelength tile, one wavefront per DPW depths (as the FP64
+// accumulation kernel).  For each candidate line the wave reduces sum_p gLop(d, p) c(d, p)
+// per depth into sl[line][d]; sum gLop dc/dT stays in registers per (lane, depth).
+__global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int nbins, int rstride,
+                                                     const int2 *__restrict__ rng, const Coef64 *__restrict__ coef,
+                                                     const double *__restrict__ wlc_s,
+                                                     const int4 *__restrict__ meta_s, const double *__restrict__ glop,
+                                                     double *__restrict__ sl, double *__restrict__ gtemp) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  if (t >= ntiles) return;
+  const int N = S.ntau;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
+  int pp[PPL];
+  bool valid[PPL];
+  double Wp[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    pp[q] = p0 + lane + 64 * q;
+    valid[q] = pp[q] <= S.nwl;
+    Wp[q] = valid[q] ? S.wave[pp[q] - 1] : 0.0;
+  }
+  const int d0 = wave * DPW;
+  double g[PPL][DPW], tT[DPW];
+#pragma unroll
+  for (int k = 0; k < DPW; k++) {
+    const int d = d0 + k;
+    const double inv = (d < N && S.motype == 1) ? 1.0 / S.cstd[d] : 1.0;
+    tT[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < PPL; q++) g[q][k] = (valid[q] && d < N) ? glop[(size_t)d * S.nwl + pp[q] - 1] * inv : 0.0;
+  }
+  for (int b = 0; b < nbins; b++) {
+    const int2 r = rng[(size_t)t * rstride + b];
+    for (int j = r.x; j < r.y; j++) {
+      const int4 m = meta_s[j];
+      if (m.y < p0 || m.x > p1) continue;
+      const double wc = wlc_s[j];
+      const Coef64 *c = coef + (size_t)j * S.ntp + d0;
+      const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
+#pragma unroll
+      for (int k = 0; k < DPW; k++) {
+        const Coef64 ck = c[k];
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+          if (!(valid[q] && pp[q] >= m.x && pp[q] <= m.y)) continue;
+          const double dld = hs ? (Wp[q] - wc) - hs[k] : (Wp[q] - wc);
+          const double v = dld * ck.vv;
+          double Hv, Ha;
+          const double H = hvoigt64_d(S, ck.a, ck.ia, ck.ua, v, Hv, Ha);
+          const double cval = H * ck.cop;
+          s += g[q][k] * cval;
+          // d c / dT = c d ln LinCop/dT + LinCop (Hv v d ln LinVvgt/dT + Ha d LinAvgt/dT)
+          tT[k] += g[q][k] * (cval * ck.pad1 + ck.cop * (Hv * v * ck.pad2 + Ha * ck.pad3));
+        }
+        s = wave_sum(s);
+        if (lane == 0 && s != 0.0) atomicAdd(&sl[(size_t)j * S.ntp + d0 + k], s);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < DPW; k++) {
+    const double v = wave_sum(tT[k]);
+    if (lane == 0 && d0 + k < N && v != 0.0) atomicAdd(&gtemp[d0 + k], v);
+  }
+}
+
+// per line: dL/d log10 gf = ln 10 sum_d sl[line][d] (to the line's input index), and
+// dL/d ln FractionPF[d][slot] += sl[line][d]
+__global__ void k_lineop_vjp_scatter(DevSetup S, int64_t nret, const int *order, const int *slot_s, const double *sl,
+                                     double *g_loggf, double *g_fpf) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nret) return;
+  const double *r = sl + (size_t)j * S.ntp;
+  const int slot = slot_s[j];
+  double tot = 0.0;
+  for (int d = 0; d < S.ntau; d++) {
+    tot += r[d];
+    if (r[d] != 0.0) atomicAdd(&g_fpf[(size_t)d * S.pf_ld + (slot - 1)], r[d]);
+  }
+  g_loggf[order[j]] = 2.302585092994045684 * tot;
+}
+
+// ------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------
+static inline unsigned nb(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+size_t rt_vjp_scratch_per_point(const DevSetup &S) {
+  return sizeof(double) * ((size_t)(S.ntau - 1) * S.nmu + 5 * (size_t)S.ntau);
+}
+
+void launch_rt_vjp(const DevSetup &S, const void *lop, int fp32, const double *gsfl, const double *gsfc, int p_lo,
+                   int np, double *scr, double *glop, double *gcont, double *gtemp, hipStream_t st) {
+  if (np <= 0) return;
+  if (fp32)
+    k_rt_vjp<float><<<nb(np, 64), 64, 0, st>>>(S, (const float *)lop, gsfl, gsfc, p_lo, np, scr, glop, gcont, gtemp);
+  else
+    k_rt_vjp<double><<<nb(np, 64), 64, 0, st>>>(S, (const double *)lop, gsfl, gsfc, p_lo, np, scr, glop, gcont,
+                                               gtemp);
+}
+
+void launch_fill64_vjp(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
+                       const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
+                       int4 *meta_s, int *slot_s, hipStream_t st) {
+  if (nret)
+    k_fill64_vjp<<<nb(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, wlc_s, meta_s,
+                                                slot_s);
+}
+
+void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
+                       const double *wlc_s, const int4 *meta_s, const double *glop, double *sl, double *gtemp,
+                       hipStream_t st) {
+  if (ntiles <= 0 || nbins <= 0) return;
+  const int nw = S.ntp / DPW;
+  k_lineop_vjp<<<8 * ((ntiles + 7) / 8), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, glop,
+                                                           sl, gtemp);
+}
+
+void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *slot_s, const double *sl,
+                               double *g_loggf, double *g_fpf, hipStream_t st) {
+  if (nret) k_lineop_vjp_scatter<<<nb(nret, 128), 128, 0, st>>>(S, nret, order, slot_s, sl, g_loggf, g_fpf);
+}
+
+}  // namespace fsg

@@ -85,6 +85,10 @@ struct fsg_ctx {
   // spherical rays on the host (field geometry of the polarised solution)
   std::vector<int> ntau_mu;
   std::vector<double> muloc;  // [nmu][2 ntau]
+  // reverse mode: dL/dLineOpac [ntau][nwl] of the last fsg_flux_vjp, and work buffers
+  double *glop = nullptr;
+  int have_glop = 0;
+  DBuf vscr, vgcont, vsl, vslot, vout;
 };
 
 struct fsg_dlines {
@@ -398,7 +402,8 @@ int fsg_finalize(fsg_ctx *c) {
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (void *p : c->owned) (void)hipFree(p);
   for (DBuf *b : {&c->flag, &c->il, &c->ir, &c->ic, &c->key, &c->key2, &c->idx, &c->idx2, &c->coef, &c->wlc,
-                  &c->meta, &c->rng, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp})
+                  &c->meta, &c->rng, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp, &c->vscr, &c->vgcont,
+                  &c->vsl, &c->vslot, &c->vout})
     b->release();
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -492,19 +497,22 @@ int fsg_lines_free(fsg_dlines *d) {
 // one pass: the coefficient arrays live in HBM (Ntau x 32 B per retained line in FP32).
 // When nlines_lim is smaller than the list, MarkWideLines and PACK follow the reference's
 // multi-block rules; the accumulated opacity does not depend on the block partition.
-int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_stats *stats) {
-  g_err.clear();
-  if (!c || !dl) return fail("fsg_lineop_dev: NULL argument");
-  if (nlines_lim < 1) return fail("fsg_lineop_dev: NlinesLim must be positive");
-  CK(hipSetDevice(c->dev));
-  fsg_lineop_stats st{};
+}  // extern "C"
+
+namespace {
+
+// pass 1 of LineopCalc on a device list, the evaluation count and the stable bin-major
+// order of the lines (retained lines keep their index order per bin); returns the bin
+// populations.  Retained lines: width bins 0..nbins-1, then the He I Stark lines (HE_KEY).
+struct LinePrep {
+  int nbins = 0;
+  int64_t nmain = 0, nhe = 0, nret = 0;
+  int hcnt[64], hhmax[64];
+  unsigned long long hu[4];
+};
+
+int scalar_pass1(fsg_ctx *c, fsg_dlines *dl, LinePrep &P) {
   const int64_t n = dl->L.n;
-  st.nlines_in = n;
-  st.nsegments = 1;
-  if (n == 0) {
-    if (stats) *stats = st;
-    return 0;
-  }
   const DevSetup &S = c->S;
   const int N = S.ntau;
   hipStream_t s = c->st;
@@ -524,30 +532,70 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
                c->key.as<unsigned>(), c->bincnt, c->binhmax, c->batched, s);
   launch_count_evals(n, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), N, c->u64, s);
   launch_iota(c->idx.as<int>(), n, s);
-  // stable bin-major order of the lines (retained lines keep their index order per bin)
   size_t tbytes = 0;
   CK(hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, c->key.as<unsigned>(), c->key2.as<unsigned>(),
                                         c->idx.as<int>(), c->idx2.as<int>(), (int)n, 0, 6, s));
   CK(c->tmp.ensure(tbytes));
   CK(hipcub::DeviceRadixSort::SortPairs(c->tmp.p, tbytes, c->key.as<unsigned>(), c->key2.as<unsigned>(),
                                         c->idx.as<int>(), c->idx2.as<int>(), (int)n, 0, 6, s));
-  int hcnt[64], hhmax[64];
-  unsigned long long hu[4];
-  CK(hipMemcpyAsync(hcnt, c->bincnt, sizeof(hcnt), hipMemcpyDeviceToHost, s));
-  CK(hipMemcpyAsync(hhmax, c->binhmax, sizeof(hhmax), hipMemcpyDeviceToHost, s));
-  CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(P.hcnt, c->bincnt, sizeof(P.hcnt), hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(P.hhmax, c->binhmax, sizeof(P.hhmax), hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(P.hu, c->u64, sizeof(P.hu), hipMemcpyDeviceToHost, s));
   CK(hipEventRecord(c->ev[1], s));
   CK(hipStreamSynchronize(s));
-  // retained lines: width bins 0..nbins-1, then the He I Stark lines (HE_KEY)
-  int64_t nmain = 0;
-  int nbins = 0;
+  P.nmain = 0;
+  P.nbins = 0;
   for (int b = 0; b < HE_KEY; b++) {
-    nmain += hcnt[b];
-    if (hcnt[b] > 0) nbins = b + 1;
+    P.nmain += P.hcnt[b];
+    if (P.hcnt[b] > 0) P.nbins = b + 1;
   }
-  const int64_t nhe = hcnt[HE_KEY];
-  const int64_t nret = nmain + nhe;
-  if (nbins > MAX_BINS) return fail("fsg_lineop_dev: too many window-width bins");
+  P.nhe = P.hcnt[HE_KEY];
+  P.nret = P.nmain + P.nhe;
+  if (P.nbins > MAX_BINS) return fail("LineopCalc: too many window-width bins");
+  return 0;
+}
+
+// groups of the bin-sorted retained lines: offsets, sizes and maximal half-widths of the
+// nbins width bins and, last, the He I group
+void group_tables(const LinePrep &P, std::vector<int64_t> &goff, std::vector<int64_t> &gcnt, std::vector<int> &ghmax) {
+  const int ng = P.nbins + 1;
+  goff.resize(ng);
+  gcnt.resize(ng);
+  ghmax.resize(ng);
+  int64_t o = 0;
+  for (int b = 0; b < ng; b++) {
+    int key = b < P.nbins ? b : HE_KEY;
+    goff[b] = o;
+    gcnt[b] = P.hcnt[key];
+    ghmax[b] = P.hhmax[key];
+    o += P.hcnt[key];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_stats *stats) {
+  g_err.clear();
+  if (!c || !dl) return fail("fsg_lineop_dev: NULL argument");
+  if (nlines_lim < 1) return fail("fsg_lineop_dev: NlinesLim must be positive");
+  CK(hipSetDevice(c->dev));
+  fsg_lineop_stats st{};
+  const int64_t n = dl->L.n;
+  st.nlines_in = n;
+  st.nsegments = 1;
+  if (n == 0) {
+    if (stats) *stats = st;
+    return 0;
+  }
+  const DevSetup &S = c->S;
+  hipStream_t s = c->st;
+  LinePrep P;
+  if (scalar_pass1(c, dl, P)) return 1;
+  const int nbins = P.nbins;
+  const int64_t nret = P.nret;
+  unsigned long long *hu = P.hu;
   st.nlines_use = nret;
   st.evals = (int64_t)hu[0];
   // MarkWideLines over the packed (single block, atomic/mixed) or full list
@@ -573,16 +621,9 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   bool timed = false;  // ms_opacity runs from the first accumulation launch to the last
   if (nret > 0) {
     const int ng = nbins + 1;
-    std::vector<int64_t> goff(ng), gcnt(ng);
-    std::vector<int> ghmax(ng);
-    int64_t o = 0;
-    for (int b = 0; b < ng; b++) {
-      int key = b < nbins ? b : HE_KEY;
-      goff[b] = o;
-      gcnt[b] = hcnt[key];
-      ghmax[b] = hhmax[key];
-      o += hcnt[key];
-    }
+    std::vector<int64_t> goff, gcnt;
+    std::vector<int> ghmax;
+    group_tables(P, goff, gcnt, ghmax);
     const size_t csz = c->precision == FSG_FP32 ? sizeof(CoefA) + sizeof(CoefB) : sizeof(Coef64);
     const int64_t chunk = chunk_lines(c, nret, nlines_lim, S.ntp * csz + sizeof(double) + sizeof(int4), 0);
     CK(c->coef.ensure((size_t)chunk * S.ntp * csz));
@@ -643,6 +684,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   st.ms_pass1 = ms1;
   st.ms_opacity = ms2;
   c->have_rt = 0;
+  c->have_glop = 0;
   if (stats) *stats = st;
   return 0;
 }
@@ -843,6 +885,7 @@ int fsg_maglineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, const doub
   st.ms_pass1 = ms1;
   st.ms_opacity = ms2;
   c->have_rt = 0;
+  c->have_glop = 0;
   if (stats) *stats = st;
   return 0;
 }
@@ -862,6 +905,129 @@ int fsg_maglineop(fsg_ctx *c, const fsg_lines *lines, int64_t nlines_lim, const 
   return rc;
 }
 
+int fsg_lines_set_gf(fsg_dlines *dl, const float *gf, int32_t on_device) {
+  g_err.clear();
+  if (!dl || (!gf && dl->L.n > 0)) return fail("fsg_lines_set_gf: NULL argument");
+  if (dl->L.n == 0) return 0;
+  fsg_ctx *c = dl->ctx;
+  CK(hipSetDevice(c->dev));
+  CK(hipMemcpyAsync((void *)dl->L.gf, gf, sizeof(float) * (size_t)dl->L.n,
+                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+// Reverse mode of RadTranCalc + the flux sums, in wavelength chunks whose checkpoint fits
+// the scratch budget.
+int fsg_flux_vjp(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glop_out, double *gcont_out,
+                 double *gtemp_out) {
+  g_err.clear();
+  if (!c || !gsfl || !gsfc) return fail("fsg_flux_vjp: NULL argument");
+  if (!c->have_rt || c->magnetic) return fail("fsg_flux_vjp: needs the scalar formal solution of fsg_radtran");
+  if (c->S.sph) return fail("fsg_flux_vjp: the reverse mode covers plane-parallel models");
+  CK(hipSetDevice(c->dev));
+  const DevSetup &S = c->S;
+  const size_t N = (size_t)S.ntau, Nw = (size_t)S.nwl;
+  hipStream_t s = c->st;
+  if (!c->glop && alloc_zero(c, sizeof(double) * N * Nw, (void **)&c->glop)) return 1;
+  CK(c->vgcont.ensure(sizeof(double) * N * Nw));
+  CK(c->vout.ensure(sizeof(double) * (2 * Nw + N)));
+  double *dg = c->vout.as<double>(), *dgc = dg + Nw, *dgt = dgc + Nw;
+  CK(hipMemcpyAsync(dg, gsfl, sizeof(double) * Nw, hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(dgc, gsfc, sizeof(double) * Nw, hipMemcpyHostToDevice, s));
+  CK(hipMemsetAsync(dgt, 0, sizeof(double) * N, s));
+  const size_t per = rt_vjp_scratch_per_point(S);
+  const int64_t chunk = std::min<int64_t>((int64_t)Nw, std::max<int64_t>(64, (int64_t)((size_t)1 << 30) / (int64_t)per));
+  CK(c->vscr.ensure(per * (size_t)chunk));
+  for (int64_t p0 = 0; p0 < (int64_t)Nw; p0 += chunk) {
+    const int np = (int)std::min<int64_t>(chunk, (int64_t)Nw - p0);
+    launch_rt_vjp(S, c->lineop, c->precision == FSG_FP32, dg, dgc, (int)p0, np, c->vscr.as<double>(), c->glop,
+                  c->vgcont.as<double>(), dgt, s);
+  }
+  CK(hipGetLastError());
+  for (int which = 0; which < 2; which++) {  // [ntau][nwl] -> (Ntau, Nwl) layout
+    double *out = which ? gcont_out : glop_out;
+    if (!out) continue;
+    std::vector<double> h(N * Nw);
+    CK(hipMemcpyAsync(h.data(), which ? c->vgcont.p : (void *)c->glop, sizeof(double) * h.size(),
+                      hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    for (size_t k = 0; k < N; k++)
+      for (size_t iw = 0; iw < Nw; iw++) out[iw * N + k] = h[k * Nw + iw];
+  }
+  if (gtemp_out) CK(hipMemcpyAsync(gtemp_out, dgt, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  c->have_glop = 1;
+  return 0;
+}
+
+// Reverse mode of LineopCalc for one list from the resident dL/dLineOpac: pass 1 again
+// (same windows and flags), then per chunk of coefficient records the per-(line, depth)
+// sums of dL/dLineOpac x contribution and the temperature derivative of the contributions.
+int fsg_lineop_vjp(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_out, double *gfpf_inout,
+                   double *gtemp_inout) {
+  g_err.clear();
+  if (!c || !dl) return fail("fsg_lineop_vjp: NULL argument");
+  if (nlines_lim < 1) return fail("fsg_lineop_vjp: NlinesLim must be positive");
+  if (!c->have_glop) return fail("fsg_lineop_vjp: call fsg_flux_vjp first");
+  CK(hipSetDevice(c->dev));
+  const DevSetup &S = c->S;
+  const int64_t n = dl->L.n;
+  const size_t N = (size_t)S.ntau;
+  hipStream_t s = c->st;
+  const size_t nfpf = N * (size_t)S.pf_ld;
+  CK(c->vout.ensure(sizeof(double) * ((size_t)std::max<int64_t>(n, 1) + nfpf + N)));
+  double *dgf = c->vout.as<double>(), *dfpf = dgf + std::max<int64_t>(n, 1), *dgt = dfpf + nfpf;
+  CK(hipMemsetAsync(c->vout.p, 0, sizeof(double) * ((size_t)std::max<int64_t>(n, 1) + nfpf + N), s));
+  if (n > 0) {
+    LinePrep P;
+    if (scalar_pass1(c, dl, P)) return 1;
+    if (P.nret > 0) {
+      const int ng = P.nbins + 1;
+      std::vector<int64_t> goff, gcnt;
+      std::vector<int> ghmax;
+      group_tables(P, goff, gcnt, ghmax);
+      const size_t per = S.ntp * (sizeof(Coef64) + sizeof(double)) + sizeof(double) + sizeof(int4) + sizeof(int);
+      const int64_t chunk = chunk_lines(c, P.nret, nlines_lim, per, 0);
+      CK(c->coef.ensure((size_t)chunk * S.ntp * sizeof(Coef64)));
+      CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
+      CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
+      CK(c->vsl.ensure((size_t)chunk * S.ntp * sizeof(double)));
+      CK(c->vslot.ensure((size_t)chunk * sizeof(int)));
+      const int ntiles = (S.nwl + TILE - 1) / TILE;
+      CK(c->rng.ensure((size_t)ntiles * ng * sizeof(int2)));
+      std::vector<std::vector<int>> tables;
+      for (int64_t s0 = 0; s0 < P.nret; s0 += chunk) {
+        const int64_t s1 = std::min(P.nret, s0 + chunk), m = s1 - s0;
+        tables.emplace_back();
+        chunk_tables(goff, gcnt, ghmax, s0, s1, tables.back());
+        CK(hipMemcpyAsync(c->binoff, tables.back().data(), sizeof(int) * tables.back().size(),
+                          hipMemcpyHostToDevice, s));
+        const int *order = c->idx2.as<int>() + s0;
+        launch_fill64_vjp(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
+                          c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(),
+                          c->vslot.as<int>(), s);
+        launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
+                           c->meta.as<int4>(), c->rng.as<int2>(), s);
+        CK(hipMemsetAsync(c->vsl.p, 0, (size_t)m * S.ntp * sizeof(double), s));
+        launch_lineop_vjp(S, ntiles, ng, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), c->wlc.as<double>(),
+                          c->meta.as<int4>(), c->glop, c->vsl.as<double>(), dgt, s);
+        launch_lineop_vjp_scatter(S, m, order, c->vslot.as<int>(), c->vsl.as<double>(), dgf, dfpf, s);
+      }
+      CK(hipGetLastError());
+      CK(hipStreamSynchronize(s));
+    }
+  }
+  if (gloggf_out && n > 0) CK(hipMemcpy(gloggf_out, dgf, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+  std::vector<double> h(nfpf + N);
+  CK(hipMemcpy(h.data(), dfpf, sizeof(double) * (nfpf + N), hipMemcpyDeviceToHost));
+  if (gfpf_inout)
+    for (size_t i = 0; i < nfpf; i++) gfpf_inout[i] += h[i];
+  if (gtemp_inout)
+    for (size_t k = 0; k < N; k++) gtemp_inout[k] += h[nfpf + k];
+  return 0;
+}
+
 int fsg_reset_lineop(fsg_ctx *c) {
   if (!c) return fail("fsg_reset_lineop: NULL context");
   CK(hipSetDevice(c->dev));
@@ -869,6 +1035,7 @@ int fsg_reset_lineop(fsg_ctx *c) {
   if (c->maglineop) CK(hipMemsetAsync(c->maglineop, 0, lop_elem(c) * 6 * (size_t)c->S.ntau * c->S.nwl, c->st));
   CK(hipStreamSynchronize(c->st));
   c->have_rt = 0;
+  c->have_glop = 0;
   return 0;
 }
 
@@ -914,6 +1081,7 @@ int fsg_set_lineop(fsg_ctx *c, const double *in) {
     CK(hipStreamSynchronize(c->st));
   }
   c->have_rt = 0;
+  c->have_glop = 0;
   return 0;
 }
 

@@ -286,4 +286,33 @@ __device__ __forceinline__ double tau_step_mu(const double *rhox, double xmu, in
   return Dx1 * (kp + 2.0 * k0 + Dx1 / 2.0 * Yp) / 3.0;
 }
 
+// coefficient-record metadata of retained line j (bin order): centre offset, window, centre
+// pixel and He I index; XlineSetup of the line in xc
+__device__ __forceinline__ void line_meta(const DevSetup &S, const DevLines &L, int list_type, int64_t j,
+                                          const int *order, const int *flag, const int *il, const int *ir,
+                                          const int *ic, XC &xc, double *wlc_s, int4 *meta_s) {
+  int64_t i = order[j];
+  int f = flag[i];
+  double wlc = L.wl[i] - S.wfirst;
+  xc = xline_setup(S, L, i, wlc, f & 0x6, list_type);
+  wlc_s[j] = wlc;
+  meta_s[j] = make_int4(il[i], ir[i], ic[i], (f >> 8) & 0xff);
+}
+
+// FP32 batched path: far-wing expansion records (CoefA) and general-path records (CoefB)
+constexpr int MAXMU = 16;
+// opacity column of one grid point: continuum, or continuum + accumulated line opacity
+template <typename LopT>
+struct GridCol {
+  const double *cont;
+  const LopT *lop;
+  int nwl, p;
+  bool with_line;
+  __device__ __forceinline__ double k(int d) const {
+    double v = cont[(size_t)d * nwl + p];
+    if (with_line) v = v + (double)lop[(size_t)d * nwl + p];
+    return v;
+  }
+};
+
 }  // namespace fsg

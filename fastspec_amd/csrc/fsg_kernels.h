@@ -53,4 +53,17 @@ void launch_magopacity(const DevSetup &S, const float2 *ftab, int ntiles, int nb
 void launch_stokes(const DevSetup &S, const void *lop, const void *mlop, int fp32, const double *geo, double *spz,
                    double *sfl, hipStream_t st);
 
+// reverse mode (fsg_adjoint.hip)
+size_t rt_vjp_scratch_per_point(const DevSetup &S);
+void launch_rt_vjp(const DevSetup &S, const void *lop, int fp32, const double *gsfl, const double *gsfc, int p_lo,
+                   int np, double *scr, double *glop, double *gcont, double *gtemp, hipStream_t st);
+void launch_fill64_vjp(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
+                       const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
+                       int4 *meta_s, int *slot_s, hipStream_t st);
+void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
+                       const double *wlc_s, const int4 *meta_s, const double *glop, double *sl, double *gtemp,
+                       hipStream_t st);
+void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *slot_s, const double *sl,
+                               double *g_loggf, double *g_fpf, hipStream_t st);
+
 }  // namespace fsg

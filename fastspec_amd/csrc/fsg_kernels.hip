@@ -218,18 +218,6 @@ __global__ void k_retained_flag(int64_t n, const int *flag, int *out) {
 // ------------------------------------------------------------------------------------
 // coefficient fill: one lane per retained line in bin-sorted order
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ void line_meta(const DevSetup &S, const DevLines &L, int list_type, int64_t j,
-                                          const int *order, const int *flag, const int *il, const int *ir,
-                                          const int *ic, XC &xc, double *wlc_s, int4 *meta_s) {
-  int64_t i = order[j];
-  int f = flag[i];
-  double wlc = L.wl[i] - S.wfirst;
-  xc = xline_setup(S, L, i, wlc, f & 0x6, list_type);
-  wlc_s[j] = wlc;
-  meta_s[j] = make_int4(il[i], ir[i], ic[i], (f >> 8) & 0xff);
-}
-
-// FP32 batched path: far-wing expansion records (CoefA) and general-path records (CoefB)
 __global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                          const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, double *wlc_s,
                          int4 *meta_s) {
@@ -563,7 +551,6 @@ __global__ void __launch_bounds__(1024) k_opacity64(DevSetup S, int ntiles, int 
 // opacity of one wavelength per lane, streamed from the bottom of the atmosphere up with
 // a sliding window of depth points; FP64 throughout.  Fused flux sums.
 // ------------------------------------------------------------------------------------
-constexpr int MAXMU = 16;
 
 // TauStep (src/solver.f90:591-615) element i (0-based, i <= N-2), Xmu = 1
 template <typename Col>
@@ -620,20 +607,6 @@ __device__ __forceinline__ void locprf_pp(const DevSetup &S, const Col &col, dou
     }
   }
 }
-
-// opacity column of one grid point: continuum, or continuum + accumulated line opacity
-template <typename LopT>
-struct GridCol {
-  const double *cont;
-  const LopT *lop;
-  int nwl, p;
-  bool with_line;
-  __device__ __forceinline__ double k(int d) const {
-    double v = cont[(size_t)d * nwl + p];
-    if (with_line) v = v + (double)lop[(size_t)d * nwl + p];
-    return v;
-  }
-};
 
 // DeloScalar along ray im (LocPrfSph, src/solver.f90:182-261) for NC opacity columns that
 // share the source function: points gathered through the ray map, optical depth steps on

@@ -136,6 +136,43 @@ class FastSpecGPU:
         _lib.check(self._lib.fsg_get_maglineop(self.ctx, ptr(out, C.c_double)))
         return out
 
+    # --- reverse mode ------------------------------------------------------------------
+    def set_gf(self, dlines: DeviceLines, gf) -> None:
+        """Replaces the gf column of a device list: a float32 numpy array, or a CUDA tensor
+        (float32, contiguous) whose producing work has finished."""
+        if hasattr(gf, "data_ptr"):
+            assert gf.dtype.itemsize == 4 and gf.is_contiguous() and gf.numel() == dlines.n
+            _lib.check(self._lib.fsg_lines_set_gf(dlines.handle, C.c_void_p(gf.data_ptr()), 1))
+        else:
+            a = np.ascontiguousarray(gf, np.float32)
+            assert a.shape == (dlines.n,)
+            _lib.check(self._lib.fsg_lines_set_gf(dlines.handle, a.ctypes.data_as(C.c_void_p), 0))
+
+    def flux_vjp(self, gsfl, gsfc, want_opac: bool = False):
+        """Reverse mode of RadTranCalc + flux: from dL/dSfL, dL/dSfC returns
+        (dL/dLineOpac, dL/dContOpac [nwl][ntau] or None, None, source-function part of dL/dT
+        [ntau]); dL/dLineOpac stays on the device for lineop_vjp."""
+        s = self.setup
+        a = np.ascontiguousarray(gsfl, np.float64)
+        b = np.ascontiguousarray(gsfc, np.float64)
+        glop = np.zeros((s.nwl, s.ntau)) if want_opac else None
+        gcont = np.zeros((s.nwl, s.ntau)) if want_opac else None
+        gt = np.zeros(s.ntau)
+        _lib.check(self._lib.fsg_flux_vjp(self.ctx, ptr(a, C.c_double), ptr(b, C.c_double), ptr(glop, C.c_double),
+                                          ptr(gcont, C.c_double), ptr(gt, C.c_double)))
+        return glop, gcont, gt
+
+    def lineop_vjp(self, dlines: DeviceLines, NlinesLim: Optional[int] = None):
+        """(dL/d log10 gf [n], dL/d ln FractionPF [ntau][pf_ld], line part of dL/dT [ntau])."""
+        s = self.setup
+        nlim = dlines.n if NlinesLim is None else int(NlinesLim)
+        g = np.zeros(max(dlines.n, 1))
+        gf = np.zeros((s.ntau, s.fraction_pf.shape[1]))
+        gt = np.zeros(s.ntau)
+        _lib.check(self._lib.fsg_lineop_vjp(self.ctx, dlines.handle, max(nlim, 1), ptr(g, C.c_double),
+                                            ptr(gf, C.c_double), ptr(gt, C.c_double)))
+        return g[: dlines.n], gf, gt
+
     def reset_lineop(self):
         _lib.check(self._lib.fsg_reset_lineop(self.ctx))
 

@@ -184,6 +184,27 @@ int fsg_lineop_dev(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_
  * patterns (lines->pi_off/si_off set); the list type is taken as FSG_LIST_MIXED. */
 int fsg_maglineop_dev(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, const double *bfield,
                       int32_t bframe, fsg_lineop_stats *stats);
+/* Replaces the gf column (linear REAL(4) values, 10**log gf) of a device list, from host
+ * memory or (on_device != 0) from device memory; windows and flags follow on the next
+ * fsg_lineop_dev.  No counterpart in the reference, whose lists are read once: it serves
+ * gradient-based fitting, where gf changes between syntheses. */
+int fsg_lines_set_gf(fsg_dlines *dl, const float *gf, int32_t on_device);
+
+/* Reverse mode (no counterpart in the reference, which has no derivatives).  For a loss
+ * L(SfL, SfC) of the flux sums of the last plane-parallel fsg_radtran:
+ *   fsg_flux_vjp    from dL/dSfL, dL/dSfC [nwl] computes and keeps on the device
+ *                   dL/dLineOpac; returns, where non-NULL, dL/dLineOpac and dL/dContOpac
+ *                   [nwl][ntau] and the source-function part of dL/dT [ntau].
+ *   fsg_lineop_vjp  for one list of the synthesis (its windows and flags held at their
+ *                   forward values): dL/d(log10 gf) [n] of its lines, and adds the line
+ *                   opacity's parts of dL/d(ln FractionPF) [ntau][pf_ld] and dL/dT [ntau].
+ * Derivatives with respect to T are taken at fixed ContOpac, FractionPF, Xne and
+ * perturber densities (the host's EOS and continuum chain them further). */
+int fsg_flux_vjp(fsg_ctx *ctx, const double *gsfl, const double *gsfc, double *glop_out,
+                 double *gcont_out, double *gtemp_out);
+int fsg_lineop_vjp(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_out,
+                   double *gfpf_inout, double *gtemp_inout);
+
 /* Device pointers to the context's resident arrays (valid until fsg_finalize):
  *   which = 0 LineOpac [ntau][nwl] (lambda fastest; FP32 or FP64 by precision)
  *           1 SpL [nmu][nwl] FP64   2 SpC [nmu][nwl] FP64
