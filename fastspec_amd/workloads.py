@@ -368,3 +368,26 @@ def config_c4(nlines: int = 5000):
     lines = magnetic_lines(nlines, 6000.0, 6100.0, SEED_BASE + 4)
     bfield = np.array([2000.0, 45.0, 30.0])
     return setup, lines, bfield, 1
+
+
+def config_c5(nmodels: int = 256, nlines: int = 50_000, step: float = 0.01, margin: float = 2.5):
+    """Differentiable batch: nmodels grey atmospheres (Teff ~ U(4500, 6500) K, log g ~
+    U(3.5, 4.8)), each with its own window of width ~ U(5, 50) A inside 5000-5500 A on a
+    0.01 A grid, and one shared seeded list of 50,000 atomic lines lambda ~ U(5000, 5500) A
+    (distributions as the plain configuration).  Returns the shared list and per model
+    (setup, indices of the shared lines within margin of its window)."""
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + 5))
+    lines = atomic_lines(nlines, 5000.0, 5500.0, SEED_BASE + 50)
+    teff = rng.uniform(4500.0, 6500.0, nmodels)
+    logg = rng.uniform(3.5, 4.8, nmodels)
+    width = rng.uniform(5.0, 50.0, nmodels)
+    start = 5000.0 + rng.uniform(0.0, 1.0, nmodels) * (500.0 - width)
+    models = []
+    for m in range(nmodels):
+        lo = float(np.round(start[m], 2))
+        hi = float(np.round(lo + width[m], 2))
+        atm = grey_atmosphere(float(teff[m]), float(logg[m]))
+        setup = build_setup(atm, lo, hi, step, 0)
+        idx = np.nonzero((lines.wl >= lo - margin) & (lines.wl <= hi + margin))[0]
+        models.append((setup, idx))
+    return lines, models
