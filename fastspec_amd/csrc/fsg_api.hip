@@ -595,7 +595,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   if (scalar_pass1(c, dl, P)) return 1;
   const int nbins = P.nbins;
   const int64_t nret = P.nret;
-  unsigned long long *hu = P.hu;
+  unsigned long long *hu = P.hu;  // [4]: evaluations, wide lines
   st.nlines_use = nret;
   st.evals = (int64_t)hu[0];
   // MarkWideLines over the packed (single block, atomic/mixed) or full list
@@ -626,7 +626,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
     group_tables(P, goff, gcnt, ghmax);
     const size_t csz = c->precision == FSG_FP32 ? sizeof(CoefA) + sizeof(CoefB) : sizeof(Coef64);
     const int64_t chunk = chunk_lines(c, nret, nlines_lim, S.ntp * csz + sizeof(double) + sizeof(int4), 0);
-    CK(c->coef.ensure((size_t)chunk * S.ntp * csz));
+    CK(c->coef.ensure((size_t)chunk * S.ntp * csz + sizeof(CoefB)));
     CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
     CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
     const int ntiles = (S.nwl + TILE - 1) / TILE;
@@ -638,8 +638,9 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       std::vector<int> &packed = tables.back();
       chunk_tables(goff, gcnt, ghmax, s0, s1, packed);
       CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * packed.size(), hipMemcpyHostToDevice, s));
+      // CoefA records (16 B) first, then the CoefB records from the next 32-byte boundary
       CoefA *ca = c->coef.as<CoefA>();
-      CoefB *cb = reinterpret_cast<CoefB *>(ca + (size_t)m * S.ntp);
+      CoefB *cb = reinterpret_cast<CoefB *>(ca + (((size_t)m * S.ntp + 1) & ~(size_t)1));
       const int *order = c->idx2.as<int>() + s0;
       if (c->precision == FSG_FP32)
         launch_fill32(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
@@ -675,7 +676,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   if (!timed) CK(hipEventRecord(c->ev[2], s));
   CK(hipEventRecord(c->ev[3], s));
   CK(hipGetLastError());
-  CK(hipMemcpyAsync(hu, c->u64, sizeof(hu), hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(hu, c->u64, sizeof(P.hu), hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
   st.nwide = (int64_t)hu[1];
   float ms1 = 0.f, ms2 = 0.f;

@@ -45,28 +45,24 @@ constexpr int MAX_BINS = 40;
 constexpr int WEAK_KEY = 63;    // sort key of rejected lines
 constexpr int HE_KEY = 62;      // sort key of retained lines with a tabulated He I Stark shift
 
-// Per-(line, depth) coefficients of the batched FP32 path, two 32-byte records kept in
-// separate arrays (CoefA for every evaluation, CoefB only on the general path).
+// Per-(line, depth) coefficients of the batched FP32 path, two records kept in separate
+// arrays (CoefA for every evaluation, CoefB only on the general path).
 //   Far-wing expansion (used when every lane of the wavefront has v^2 >= 400 (a^2 + 1/2)):
 //     with u = 1/v^2 the reference's one-pole fraction a (s + 1/2)/(s^2 - v^2 + a^2 + 1/4)/sqrt(pi)
 //     equals K u (1 + b u)/(1 + al u + b^2 u^2), K = LinCop a/sqrt(pi), al = 2a^2 - 1,
-//     b = a^2 + 1/2; it is expanded to K u (f0 + f1 u + f2 u^2 + f3 u^3), f0 = 1.
-//     w = 1/vv^2 (u = w / dl^2), k0..k3 = K f0..f3, dthr = the |dl| above which the
-//     expansion applies.
+//     b = a^2 + 1/2, expanded to K u (f0 + f1 u + f2 u^2), f0 = 1 (the next term is below
+//     1.1e-7 relative there).  With t = 1/dl^2 and w = 1/vv^2 (u = w t) the profile is
+//     t (k0 + t (k1 + t k2)), k_n = K f_n w^(n+1); dthr = the |dl| above which it applies.
 //   General path: vv = 1/Doppler width, a, a^2, a^2 + 1/4, K, LinCop, ua (a-interpolation
 //   weight), ia (a-cell of the table, 0 = outside the table box).
-struct alignas(32) CoefA {
-  float w, k0, k1, k2, k3, dthr, vv, a;
+struct alignas(16) CoefA {
+  float k0, k1, k2, dthr;
 };
 struct alignas(32) CoefB {
-  float a2, c1, ka, cop, ua;
-  int ia;
-  float pad0, pad1;
-};
-struct Coef32 {  // the general path's view (assembled from both records)
   float vv, a, a2, c1, ka, cop, ua;
   int ia;
 };
+using Coef32 = CoefB;  // the general path's view
 // FP64 plain path: the reference's quantities LinVvgt, LinAvgt, LinCop and the a-state
 struct alignas(64) Coef64 {
   double vv, a, cop, ua;

@@ -246,30 +246,26 @@ __global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, co
     // series of (1 + be u)/(1 + al u + be^2 u^2): f_n = -al f_{n-1} - be^2 f_{n-2}
     const double f1 = be - al;
     const double f2 = -al * f1 - be * be;
-    const double f3 = -al * f2 - be * be * f1;
+    const double w = 1.0 / (vv * vv);
     CoefA a;
-    a.w = (float)(1.0 / (vv * vv));
-    a.k0 = (float)K;
-    a.k1 = (float)(K * f1);
-    a.k2 = (float)(K * f2);
-    a.k3 = (float)(K * f3);
+    a.k0 = (float)(K * w);
+    a.k1 = (float)(K * f1 * w * w);
+    a.k2 = (float)(K * f2 * w * w * w);
     a.dthr = (float)(sqrt(fmax(196.0, 400.0 * be)) / vv * (1.0 + 1e-4));
-    a.vv = (float)vv;
-    a.a = (float)av;
     CoefB b;
+    b.vv = (float)vv;
+    b.a = (float)av;
     b.a2 = (float)a2;
     b.c1 = (float)(a2 + 0.25);
     b.ka = (float)K;
     b.cop = (float)cop;
     b.ua = (float)ua;
     b.ia = ia;
-    b.pad0 = b.pad1 = 0.f;
     A[k] = a;
     B[k] = b;
   }
 }
 
-// FP64 plain path: the reference's LinVvgt, LinAvgt, LinCop and the table a-state
 __global__ void k_fill64(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                          const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s, int4 *meta_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -363,11 +359,17 @@ __device__ __forceinline__ double contrib64(const DevSetup &S, const Coef64 &c, 
   return hvoigt64(S, c.a, c.ia, c.ua, v) * c.cop;
 }
 
-typedef float f8v __attribute__((ext_vector_type(8)));
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// Lines are added into plain FP32 partial sums in groups of FOLD lines; each group's partial
+// is then added to the running sum with Kahan compensation (blocked compensated summation:
+// the uncompensated part is at most FOLD terms long).
+constexpr int FOLD = 8;
 
 // FP32 batched path.  A block owns a tile of TILE = 64 x PPL wavelength points (PPL per
 // lane, 64 apart) and all depths, one wavefront per DPW depths.  Per (line, depth) the
-// 32-byte CoefA record is read with one scalar load (wave-uniform address), issued while
+// 16-byte CoefA record is read with one scalar load (wave-uniform address), issued while
 // the previous depth is computed.  Lines of the tabulated He I Stark table come in a
 // separate launch (HE = true) so that the common path carries no per-depth shift.
 template <bool HE>
@@ -391,19 +393,32 @@ __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int 
   }
   const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];  // tile edges
   const int d0 = wave * DPW;
-  float acc[PPL][DPW], cmp[PPL][DPW];
+  float acc[PPL][DPW], cmp[PPL][DPW], part[PPL][DPW];
 #pragma unroll
   for (int q = 0; q < PPL; q++)
 #pragma unroll
     for (int k = 0; k < DPW; k++) {
       acc[q][k] = 0.f;
       cmp[q][k] = 0.f;
+      part[q][k] = 0.f;
     }
+  int grp = 0;  // lines in the current partial sums (wave-uniform)
   for (int b = 0; b < nbins; b++) {
     const int2 r = rng[(size_t)t * rstride + b];
     for (int j = r.x; j < r.y; j++) {
       const int4 m = meta_s[j];
       if (m.y < p0 || m.x > p1) continue;  // window misses the tile (wave-uniform)
+      if (grp == FOLD) {
+#pragma unroll
+        for (int q = 0; q < PPL; q++)
+#pragma unroll
+          for (int k = 0; k < DPW; k++) {
+            kahan_add(acc[q][k], cmp[q][k], part[q][k]);
+            part[q][k] = 0.f;
+          }
+        grp = 0;
+      }
+      grp++;
       bool act[PPL];
       bool any = false;
 #pragma unroll
@@ -413,54 +428,44 @@ __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int 
       }
       if (!any) continue;
       const double wc = wlc_s[j];
-      float dlf[PPL], U[PPL];
+      float dlf[PPL], T2[PPL];
       double dl[PPL];
 #pragma unroll
       for (int q = 0; q < PPL; q++) {
         dl[q] = Wp[q] - wc;
         dlf[q] = (float)dl[q];
-        // U = 1/dl^2 on points inside the window, 0 outside (the expansion then adds 0)
-        U[q] = act[q] ? 1.0f / (dlf[q] * dlf[q]) : 0.0f;
+        // t = 1/dl^2 on points inside the window, 0 outside (the expansion then adds 0)
+        T2[q] = act[q] ? __builtin_amdgcn_rcpf(dlf[q] * dlf[q]) : 0.0f;
       }
-      const f8v *cp = reinterpret_cast<const f8v *>(ca + (size_t)j * S.ntp + d0);
+      const f4v *cp = reinterpret_cast<const f4v *>(ca + (size_t)j * S.ntp + d0);
       const CoefB *cbp = cb + (size_t)j * S.ntp + d0;
       const double *hs = HE ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
       // smallest |lambda - lambda_c| over the tile (0 if the centre lies inside it); the
       // far-wing expansion applies at depth k for the whole wavefront when it exceeds dthr
       const double dmin = (wc < Wt0) ? (Wt0 - wc) : (wc > Wt1 ? (wc - Wt1) : 0.0);
       const unsigned dmb = HE ? 0u : __builtin_amdgcn_readfirstlane(__float_as_uint((float)dmin));
-      f8v cu = cp[0];
+      f4v cu = cp[0];
 #pragma unroll
       for (int k = 0; k < DPW; k++) {
         // Scalar loads complete out of order, so a wait covers every load in flight: take
         // depth k's record (one wait, only it outstanding) before issuing depth k+1's load,
         // whose latency then overlaps depth k's arithmetic.
-        __asm__ volatile("" ::"s"(cu[5]) : "memory");
-        f8v nx = cu;
+        __asm__ volatile("" ::"s"(cu[3]) : "memory");
+        f4v nx = cu;
         if (k + 1 < DPW) nx = cp[k + 1];
-        if (!HE && dmb >= __float_as_uint(cu[5])) {
+        if (!HE && dmb >= __float_as_uint(cu[3])) {
 #pragma unroll
           for (int q = 0; q < PPL; q++) {
-            const float u = U[q] * cu[0];
-            const float val = u * fmaf(fmaf(fmaf(cu[4], u, cu[3]), u, cu[2]), u, cu[1]);
-            kahan_add(acc[q][k], cmp[q][k], val);
+            const float x = T2[q];
+            part[q][k] += x * fmaf(fmaf(cu[2], x, cu[1]), x, cu[0]);
           }
         } else {
-          const CoefB B = cbp[k];
-          Coef32 c;
-          c.vv = cu[6];
-          c.a = cu[7];
-          c.a2 = B.a2;
-          c.c1 = B.c1;
-          c.ka = B.ka;
-          c.cop = B.cop;
-          c.ua = B.ua;
-          c.ia = B.ia;
+          const CoefB c = cbp[k];
 #pragma unroll
           for (int q = 0; q < PPL; q++) {
             if (act[q]) {
               const float x = HE ? (float)(dl[q] - hs[k]) : dlf[q];
-              kahan_add(acc[q][k], cmp[q][k], contrib32(S, c, x * c.vv));
+              part[q][k] += contrib32(S, c, x * c.vv);
             }
           }
         }
@@ -475,6 +480,7 @@ __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int 
     for (int k = 0; k < DPW; k++) {
       const int d = d0 + k;
       if (d < N) {
+        kahan_add(acc[q][k], cmp[q][k], part[q][k]);
         float v = acc[q][k] - cmp[q][k];
         if (S.motype == 1) v = v / (float)S.cstd[d];
         const size_t o = (size_t)d * S.nwl + (pp[q] - 1);
