@@ -368,10 +368,11 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 constexpr int FOLD = 8;
 
 // FP32 batched path.  A block owns a tile of TILE = 64 x PPL wavelength points (PPL per
-// lane, 64 apart) and all depths, one wavefront per DPW depths.  Per (line, depth) the
-// 16-byte CoefA record is read with one scalar load (wave-uniform address), issued while
-// the previous depth is computed.  Lines of the tabulated He I Stark table come in a
-// separate launch (HE = true) so that the common path carries no per-depth shift.
+// lane, 64 apart) and all depths, one wavefront per DPW depths.  Per line the wave's DPW
+// 16-byte CoefA records are wave-uniform and arrive in one batch of scalar loads; when
+// every depth is in the far wing for the whole tile (the common case) the depths run as
+// straight-line FMA code.  Lines of the tabulated He I Stark table come in a separate
+// launch (HE = true) so that the common path carries no per-depth shift.
 template <bool HE>
 __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int nbins, int rstride,
                                                     const int2 *__restrict__ rng, const CoefA *__restrict__ ca,
@@ -440,36 +441,46 @@ __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int 
       const f4v *cp = reinterpret_cast<const f4v *>(ca + (size_t)j * S.ntp + d0);
       const CoefB *cbp = cb + (size_t)j * S.ntp + d0;
       const double *hs = HE ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
+      // the DPW far-wing records of this line: wave-uniform, one batch of scalar loads
+      f4v r[DPW];
+#pragma unroll
+      for (int k = 0; k < DPW; k++) r[k] = cp[k];
       // smallest |lambda - lambda_c| over the tile (0 if the centre lies inside it); the
       // far-wing expansion applies at depth k for the whole wavefront when it exceeds dthr
       const double dmin = (wc < Wt0) ? (Wt0 - wc) : (wc > Wt1 ? (wc - Wt1) : 0.0);
       const unsigned dmb = HE ? 0u : __builtin_amdgcn_readfirstlane(__float_as_uint((float)dmin));
-      f4v cu = cp[0];
+      unsigned fast = 0;
 #pragma unroll
-      for (int k = 0; k < DPW; k++) {
-        // Scalar loads complete out of order, so a wait covers every load in flight: take
-        // depth k's record (one wait, only it outstanding) before issuing depth k+1's load,
-        // whose latency then overlaps depth k's arithmetic.
-        __asm__ volatile("" ::"s"(cu[3]) : "memory");
-        f4v nx = cu;
-        if (k + 1 < DPW) nx = cp[k + 1];
-        if (!HE && dmb >= __float_as_uint(cu[3])) {
+      for (int k = 0; k < DPW; k++) fast |= (dmb >= __float_as_uint(r[k][3]) ? 1u : 0u) << k;
+      if (!HE && fast == (1u << DPW) - 1u) {
+        // the common case: every depth in the far wing, straight-line code
+#pragma unroll
+        for (int k = 0; k < DPW; k++)
 #pragma unroll
           for (int q = 0; q < PPL; q++) {
             const float x = T2[q];
-            part[q][k] += x * fmaf(fmaf(cu[2], x, cu[1]), x, cu[0]);
+            part[q][k] += x * fmaf(fmaf(r[k][2], x, r[k][1]), x, r[k][0]);
           }
-        } else {
-          const CoefB c = cbp[k];
+      } else {
 #pragma unroll
-          for (int q = 0; q < PPL; q++) {
-            if (act[q]) {
-              const float x = HE ? (float)(dl[q] - hs[k]) : dlf[q];
-              part[q][k] += contrib32(S, c, x * c.vv);
+        for (int k = 0; k < DPW; k++) {
+          if (!HE && ((fast >> k) & 1u)) {
+#pragma unroll
+            for (int q = 0; q < PPL; q++) {
+              const float x = T2[q];
+              part[q][k] += x * fmaf(fmaf(r[k][2], x, r[k][1]), x, r[k][0]);
+            }
+          } else {
+            const CoefB c = cbp[k];
+#pragma unroll
+            for (int q = 0; q < PPL; q++) {
+              if (act[q]) {
+                const float x = HE ? (float)(dl[q] - hs[k]) : dlf[q];
+                part[q][k] += contrib32(S, c, x * c.vv);
+              }
             }
           }
         }
-        cu = nx;
       }
     }
   }
