@@ -625,8 +625,10 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
     std::vector<int> ghmax;
     group_tables(P, goff, gcnt, ghmax);
     const size_t csz = c->precision == FSG_FP32 ? sizeof(CoefA) + sizeof(CoefB) : sizeof(Coef64);
-    const int64_t chunk = chunk_lines(c, nret, nlines_lim, S.ntp * csz + sizeof(double) + sizeof(int4), 0);
-    CK(c->coef.ensure((size_t)chunk * S.ntp * csz + sizeof(CoefB)));
+    // FP32 adds one float per (line, wavefront depth group): the group's largest far-wing threshold
+    const size_t wsz = c->precision == FSG_FP32 ? (size_t)(S.ntp / DPW) * sizeof(float) : 0;
+    const int64_t chunk = chunk_lines(c, nret, nlines_lim, S.ntp * csz + wsz + sizeof(double) + sizeof(int4), 0);
+    CK(c->coef.ensure((size_t)chunk * (S.ntp * csz + wsz) + sizeof(CoefB)));
     CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
     CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
     const int ntiles = (S.nwl + TILE - 1) / TILE;
@@ -641,10 +643,11 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       // CoefA records (16 B) first, then the CoefB records from the next 32-byte boundary
       CoefA *ca = c->coef.as<CoefA>();
       CoefB *cb = reinterpret_cast<CoefB *>(ca + (((size_t)m * S.ntp + 1) & ~(size_t)1));
+      float *dmaxw = reinterpret_cast<float *>(cb + (size_t)m * S.ntp);
       const int *order = c->idx2.as<int>() + s0;
       if (c->precision == FSG_FP32)
         launch_fill32(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
-                      c->ic.as<int>(), ca, cb, c->wlc.as<double>(), c->meta.as<int4>(), s);
+                      c->ic.as<int>(), ca, cb, dmaxw, c->wlc.as<double>(), c->meta.as<int4>(), s);
       else
         launch_fill64(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
                       c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
@@ -657,10 +660,10 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       timed = true;
       if (c->precision == FSG_FP32) {
         if (has_main && nbins > 0)
-          launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, c->wlc.as<double>(),
+          launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, c->wlc.as<double>(),
                            c->meta.as<int4>(), (float *)c->lineop, 0, s);
         if (has_he)
-          launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, c->wlc.as<double>(),
+          launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, dmaxw, c->wlc.as<double>(),
                            c->meta.as<int4>(), (float *)c->lineop, 1, s);
       } else {
         if (has_main && nbins > 0)

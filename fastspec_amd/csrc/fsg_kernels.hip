@@ -219,8 +219,8 @@ __global__ void k_retained_flag(int64_t n, const int *flag, int *out) {
 // coefficient fill: one lane per retained line in bin-sorted order
 // ------------------------------------------------------------------------------------
 __global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
-                         const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, double *wlc_s,
-                         int4 *meta_s) {
+                         const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
+                         double *wlc_s, int4 *meta_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
   XC xc;
@@ -263,6 +263,12 @@ __global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, co
     b.ia = ia;
     A[k] = a;
     B[k] = b;
+  }
+  // per wavefront depth group: the largest far-wing threshold (padding depths: 0)
+  for (int w = 0; w < S.ntp / DPW; w++) {
+    float mx = 0.0f;
+    for (int k = w * DPW; k < (w + 1) * DPW; k++) mx = fmaxf(mx, A[k].dthr);
+    dmaxw[(size_t)j * (S.ntp / DPW) + w] = mx;
   }
 }
 
@@ -376,23 +382,26 @@ constexpr int FOLD = 8;
 template <bool HE>
 __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int nbins, int rstride,
                                                     const int2 *__restrict__ rng, const CoefA *__restrict__ ca,
-                                                    const CoefB *__restrict__ cb, const double *__restrict__ wlc_s,
+                                                    const CoefB *__restrict__ cb, const float *__restrict__ dmaxw,
+                                                    const double *__restrict__ wlc_s,
                                                     const int4 *__restrict__ meta_s, float *__restrict__ lop) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
-  const int N = S.ntau;
+  const int N = S.ntau, nwv = S.ntp / DPW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
+  const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];  // tile edges
   int pp[PPL];
   bool valid[PPL];
   double Wp[PPL];
+  float ofs[PPL];  // offset of the point from the tile's first point (sub-angstrom, exact in FP32)
 #pragma unroll
   for (int q = 0; q < PPL; q++) {
     pp[q] = p0 + lane + 64 * q;
     valid[q] = pp[q] <= S.nwl;
     Wp[q] = valid[q] ? S.wave[pp[q] - 1] : 0.0;
+    ofs[q] = valid[q] ? (float)(Wp[q] - Wt0) : 0.0f;
   }
-  const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];  // tile edges
   const int d0 = wave * DPW;
   float acc[PPL][DPW], cmp[PPL][DPW], part[PPL][DPW];
 #pragma unroll
@@ -421,63 +430,63 @@ __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int 
       }
       grp++;
       bool act[PPL];
-      bool any = false;
 #pragma unroll
-      for (int q = 0; q < PPL; q++) {
-        act[q] = valid[q] && pp[q] >= m.x && pp[q] <= m.y;
-        any = any || act[q];
-      }
-      if (!any) continue;
+      for (int q = 0; q < PPL; q++) act[q] = valid[q] && pp[q] >= m.x && pp[q] <= m.y;
       const double wc = wlc_s[j];
-      float dlf[PPL], T2[PPL];
-      double dl[PPL];
-#pragma unroll
-      for (int q = 0; q < PPL; q++) {
-        dl[q] = Wp[q] - wc;
-        dlf[q] = (float)dl[q];
-        // t = 1/dl^2 on points inside the window, 0 outside (the expansion then adds 0)
-        T2[q] = act[q] ? __builtin_amdgcn_rcpf(dlf[q] * dlf[q]) : 0.0f;
-      }
       const f4v *cp = reinterpret_cast<const f4v *>(ca + (size_t)j * S.ntp + d0);
-      const CoefB *cbp = cb + (size_t)j * S.ntp + d0;
-      const double *hs = HE ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
-      // the DPW far-wing records of this line: wave-uniform, one batch of scalar loads
-      f4v r[DPW];
+      f4v rr[DPW];
 #pragma unroll
-      for (int k = 0; k < DPW; k++) r[k] = cp[k];
-      // smallest |lambda - lambda_c| over the tile (0 if the centre lies inside it); the
-      // far-wing expansion applies at depth k for the whole wavefront when it exceeds dthr
-      const double dmin = (wc < Wt0) ? (Wt0 - wc) : (wc > Wt1 ? (wc - Wt1) : 0.0);
-      const unsigned dmb = HE ? 0u : __builtin_amdgcn_readfirstlane(__float_as_uint((float)dmin));
-      unsigned fast = 0;
+      for (int k = 0; k < DPW; k++) rr[k] = cp[k];
+      // depths of this wave that leave the far wing somewhere in the tile (bit k): the
+      // far-wing expansion applies at depth k over the whole tile when the smallest
+      // |lambda - lambda_c| (0 if the centre lies inside the tile) is at least dthr
+      unsigned gen = (1u << DPW) - 1;
+      if (!HE) {
+        const double dmin = (wc < Wt0) ? (Wt0 - wc) : (wc > Wt1 ? (wc - Wt1) : 0.0);
+        const unsigned dmb = __builtin_amdgcn_readfirstlane(__float_as_uint((float)dmin));
+        gen = 0;
+        if (dmb < __float_as_uint(dmaxw[(size_t)j * nwv + wave])) {
 #pragma unroll
-      for (int k = 0; k < DPW; k++) fast |= (dmb >= __float_as_uint(r[k][3]) ? 1u : 0u) << k;
-      if (!HE && fast == (1u << DPW) - 1u) {
-        // the common case: every depth in the far wing, straight-line code
+          for (int k = 0; k < DPW; k++)
+            if (dmb < __float_as_uint(rr[k][3])) {
+              gen |= 1u << k;
+              rr[k][0] = rr[k][1] = rr[k][2] = 0.0f;
+            }
+        }
+        // far wing, straight-line FMA code over all depths (zero coefficients at the general
+        // depths).  dl = (point - first point) + (first point - centre), both rounded to FP32:
+        // the relative error of dl stays below 2^-23 (1 + tile width / |dl|) where it is used
+        const float bs = (float)(Wt0 - wc);
+        float T2[PPL];
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+          const float x = ofs[q] + bs;
+          T2[q] = act[q] ? __builtin_amdgcn_rcpf(fmaxf(x * x, 1e-30f)) : 0.0f;
+        }
 #pragma unroll
         for (int k = 0; k < DPW; k++)
 #pragma unroll
           for (int q = 0; q < PPL; q++) {
             const float x = T2[q];
-            part[q][k] += x * fmaf(fmaf(r[k][2], x, r[k][1]), x, r[k][0]);
+            part[q][k] += x * fmaf(fmaf(rr[k][2], x, rr[k][1]), x, rr[k][0]);
           }
-      } else {
+      }
+      if (gen) {
+        // line core / near wing: the Voigt table or the continued fractions
+        double dl[PPL];
+#pragma unroll
+        for (int q = 0; q < PPL; q++) dl[q] = Wp[q] - wc;
+        const CoefB *cbp = cb + (size_t)j * S.ntp + d0;
+        const double *hs = HE ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
 #pragma unroll
         for (int k = 0; k < DPW; k++) {
-          if (!HE && ((fast >> k) & 1u)) {
+          if (!(gen & (1u << k))) continue;
+          const CoefB c = cbp[k];
 #pragma unroll
-            for (int q = 0; q < PPL; q++) {
-              const float x = T2[q];
-              part[q][k] += x * fmaf(fmaf(r[k][2], x, r[k][1]), x, r[k][0]);
-            }
-          } else {
-            const CoefB c = cbp[k];
-#pragma unroll
-            for (int q = 0; q < PPL; q++) {
-              if (act[q]) {
-                const float x = HE ? (float)(dl[q] - hs[k]) : dlf[q];
-                part[q][k] += contrib32(S, c, x * c.vv);
-              }
+          for (int q = 0; q < PPL; q++) {
+            if (act[q]) {
+              const float x = HE ? (float)(dl[q] - hs[k]) : (float)dl[q];
+              part[q][k] += contrib32(S, c, x * c.vv);
             }
           }
         }
@@ -827,9 +836,11 @@ void launch_mark_wide(const DevSetup &S, int64_t n, const double *wl, const int 
 }
 
 void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
-                   const int *flag, const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, double *wlc_s,
-                   int4 *meta_s, hipStream_t st) {
-  if (nret) k_fill32<<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, ca, cb, wlc_s, meta_s);
+                   const int *flag, const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
+                   double *wlc_s, int4 *meta_s, hipStream_t st) {
+  if (nret)
+    k_fill32<<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, ca, cb, dmaxw, wlc_s,
+                                              meta_s);
 }
 
 void launch_fill64(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
@@ -845,13 +856,14 @@ void launch_tile_ranges(int ntiles, int tile, int nbins, int nwl, const int *bin
 }
 
 void launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
-                      const CoefB *cb, const double *wlc_s, const int4 *meta_s, float *lineop, int he, hipStream_t st) {
+                      const CoefB *cb, const float *dmaxw, const double *wlc_s, const int4 *meta_s, float *lineop,
+                      int he, hipStream_t st) {
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
   if (he)
-    k_opacity32<true><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, cb, wlc_s, meta_s, lineop);
+    k_opacity32<true><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, cb, dmaxw, wlc_s, meta_s, lineop);
   else
-    k_opacity32<false><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, cb, wlc_s, meta_s, lineop);
+    k_opacity32<false><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, cb, dmaxw, wlc_s, meta_s, lineop);
 }
 
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
