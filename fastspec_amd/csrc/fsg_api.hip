@@ -71,7 +71,8 @@ struct fsg_ctx {
   void *lineop = nullptr;   // [ntau][nwl] float or double
   double *spl = nullptr, *spc = nullptr, *sfl = nullptr, *sfc = nullptr;
   int have_rt = 0;
-  DBuf flag, il, ir, ic, key, key2, idx, idx2, coef, wlc, meta, rng, tmp, pos, wlcl, rflag;
+  DBuf flag, il, ir, ic, key, key2, idx, idx2, coef, wlc, meta, rng, rng2, tmp, pos, wlcl, rflag;
+  DBuf zkey, zkey2, zval, zperm;  // FP32 path: lines ordered by core-zone width
   int *bincnt = nullptr, *binhmax = nullptr, *binoff = nullptr;
   unsigned long long *u64 = nullptr;
   hipEvent_t ev[4] = {};
@@ -385,7 +386,7 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
   if (alloc_zero(c, lop_elem(c) * (size_t)N * Nw, &c->lineop) ||
       alloc_zero(c, sizeof(int) * 64, (void **)&c->bincnt) ||
       alloc_zero(c, sizeof(int) * 64, (void **)&c->binhmax) ||
-      alloc_zero(c, sizeof(int) * 3 * (MAX_BINS + 1), (void **)&c->binoff) ||
+      alloc_zero(c, sizeof(int) * (3 * (MAX_BINS + 1) + 2 * NZB), (void **)&c->binoff) ||
       alloc_zero(c, sizeof(unsigned long long) * 4, (void **)&c->u64)) {
     fsg_finalize(c);
     return 1;
@@ -402,8 +403,8 @@ int fsg_finalize(fsg_ctx *c) {
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (void *p : c->owned) (void)hipFree(p);
   for (DBuf *b : {&c->flag, &c->il, &c->ir, &c->ic, &c->key, &c->key2, &c->idx, &c->idx2, &c->coef, &c->wlc,
-                  &c->meta, &c->rng, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp, &c->vscr, &c->vgcont,
-                  &c->vsl, &c->vslot, &c->vout})
+                  &c->meta, &c->rng, &c->rng2, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp, &c->vscr,
+                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm})
     b->release();
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -625,14 +626,26 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
     std::vector<int> ghmax;
     group_tables(P, goff, gcnt, ghmax);
     const size_t csz = c->precision == FSG_FP32 ? sizeof(CoefA) + sizeof(CoefB) : sizeof(Coef64);
-    // FP32 adds one float per (line, wavefront depth group): the group's largest far-wing threshold
-    const size_t wsz = c->precision == FSG_FP32 ? (size_t)(S.ntp / DPW) * sizeof(float) : 0;
+    // FP32 adds one float per (line, wavefront depth group), the group's largest far-wing
+    // threshold, and the line's core zone (int2)
+    const size_t wsz = c->precision == FSG_FP32 ? (size_t)(S.ntp / DPW) * sizeof(float) + sizeof(int2) : 0;
     const int64_t chunk = chunk_lines(c, nret, nlines_lim, S.ntp * csz + wsz + sizeof(double) + sizeof(int4), 0);
     CK(c->coef.ensure((size_t)chunk * (S.ntp * csz + wsz) + sizeof(CoefB)));
     CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
     CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
     const int ntiles = (S.nwl + TILE - 1) / TILE;
     CK(c->rng.ensure((size_t)ntiles * ng * sizeof(int2)));
+    size_t ztmp = 0;
+    if (c->precision == FSG_FP32) {
+      CK(c->rng2.ensure((size_t)ntiles * NZB * sizeof(int2)));
+      CK(c->zkey.ensure((size_t)chunk * sizeof(unsigned long long)));
+      CK(c->zkey2.ensure((size_t)chunk * sizeof(unsigned long long)));
+      CK(c->zval.ensure((size_t)chunk * sizeof(int)));
+      CK(c->zperm.ensure((size_t)chunk * sizeof(int)));
+      ztmp = zone_sort_tmp_bytes(chunk);
+      CK(c->tmp.ensure(ztmp));
+    }
+    int *zoff = c->binoff + 3 * (MAX_BINS + 1), *zend = zoff + NZB;
     std::vector<std::vector<int>> tables;  // staged for upload; alive until the final sync
     for (int64_t s0 = 0; s0 < nret; s0 += chunk) {
       const int64_t s1 = std::min(nret, s0 + chunk), m = s1 - s0;
@@ -644,15 +657,20 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       CoefA *ca = c->coef.as<CoefA>();
       CoefB *cb = reinterpret_cast<CoefB *>(ca + (((size_t)m * S.ntp + 1) & ~(size_t)1));
       float *dmaxw = reinterpret_cast<float *>(cb + (size_t)m * S.ntp);
+      int2 *zone = reinterpret_cast<int2 *>(dmaxw + (((size_t)m * (S.ntp / DPW) + 1) & ~(size_t)1));
       const int *order = c->idx2.as<int>() + s0;
       if (c->precision == FSG_FP32)
         launch_fill32(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
-                      c->ic.as<int>(), ca, cb, dmaxw, c->wlc.as<double>(), c->meta.as<int4>(), s);
+                      c->ic.as<int>(), ca, cb, dmaxw, zone, c->wlc.as<double>(), c->meta.as<int4>(), s);
       else
         launch_fill64(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
                       c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
       launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
                          c->meta.as<int4>(), c->rng.as<int2>(), s);
+      if (c->precision == FSG_FP32)  // the chunk's lines by core-zone width, and per-tile candidate ranges
+        launch_zone_order(m, std::min<int64_t>(std::max<int64_t>(goff[nbins] - s0, 0), m), ntiles, S.nwl, c->meta.as<int4>(), zone, c->zkey.as<unsigned long long>(),
+                          c->zval.as<int>(), c->zkey2.as<unsigned long long>(), c->zperm.as<int>(), c->tmp.p, ztmp,
+                          zoff, zend, c->rng2.as<int2>(), s);
       // the range table has ng groups per tile: the first nbins are read by the main launch
       // (stride ng), the last one by the He launch (offset nbins, stride ng)
       const bool has_main = s0 < goff[nbins], has_he = s1 > goff[nbins];
@@ -660,10 +678,12 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       timed = true;
       if (c->precision == FSG_FP32) {
         if (has_main && nbins > 0)
-          launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, c->wlc.as<double>(),
-                           c->meta.as<int4>(), (float *)c->lineop, 0, s);
+          if (launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, zone, c->wlc.as<double>(),
+                               c->meta.as<int4>(), (float *)c->lineop, 0, s))
+            launch_opacity32_core(S, ntiles, c->rng2.as<int2>(), c->zperm.as<int>(), ca, cb, dmaxw, zone,
+                                  c->wlc.as<double>(), c->meta.as<int4>(), (float *)c->lineop, s);
         if (has_he)
-          launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, dmaxw, c->wlc.as<double>(),
+          launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, dmaxw, zone, c->wlc.as<double>(),
                            c->meta.as<int4>(), (float *)c->lineop, 1, s);
       } else {
         if (has_main && nbins > 0)

@@ -119,5 +119,6 @@ struct MagLine {
 
 // the polarised opacity kernel: one wavelength point per lane, DPW depths per wavefront
 constexpr int MTILE = 64;
+constexpr int NZB = 32;  // core-zone width bins of the FP32 path (the last one holds lines without a core zone)
 
 }  // namespace fsg

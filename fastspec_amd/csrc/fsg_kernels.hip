@@ -13,6 +13,8 @@
 //   LocPrfSph, DeloScalar      src/solver.f90:182-261
 //   TauStep, CentDer           src/solver.f90:591-647
 //   flux sums of DiskInteg     src/convol.f90:233-234
+#include <hipcub/hipcub.hpp>
+
 #include "fsg_kernels.h"
 #include "fsg_device.h"
 
@@ -220,7 +222,7 @@ __global__ void k_retained_flag(int64_t n, const int *flag, int *out) {
 // ------------------------------------------------------------------------------------
 __global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                          const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
-                         double *wlc_s, int4 *meta_s) {
+                         int2 *zone_s, double *wlc_s, int4 *meta_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
   XC xc;
@@ -265,12 +267,33 @@ __global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, co
     B[k] = b;
   }
   // per wavefront depth group: the largest far-wing threshold (padding depths: 0)
+  float zmax = 0.0f;
   for (int w = 0; w < S.ntp / DPW; w++) {
     float mx = 0.0f;
     for (int k = w * DPW; k < (w + 1) * DPW; k++) mx = fmaxf(mx, A[k].dthr);
     dmaxw[(size_t)j * (S.ntp / DPW) + w] = mx;
+    zmax = fmaxf(zmax, mx);
   }
+  // core zone of the line: the grid points with |lambda - lambda_c| < max_k dthr (1-based,
+  // empty when zl > zr); outside it the far-wing expansion holds at every depth
+  const double wc = wlc_s[j], zlo = wc - (double)zmax, zhi = wc + (double)zmax;
+  int a = 0, e = S.nwl;
+  while (a < e) {  // first point above zlo
+    const int mid = (a + e) >> 1;
+    if (S.wave[mid] > zlo) e = mid;
+    else a = mid + 1;
+  }
+  const int zl = a + 1;
+  a = 0;
+  e = S.nwl;
+  while (a < e) {  // first point at or above zhi
+    const int mid = (a + e) >> 1;
+    if (S.wave[mid] >= zhi) e = mid;
+    else a = mid + 1;
+  }
+  zone_s[j] = make_int2(zl, a);
 }
+
 
 __global__ void k_fill64(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                          const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s, int4 *meta_s) {
@@ -508,6 +531,351 @@ __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int 
       }
     }
   }
+}
+
+// FP32 batched path on the matrix cores.  Outside its core zone (|lambda - lambda_c| <
+// max_d dthr) the profile of line l at depth d and point p is the series
+// T (k0 + T (k1 + T k2)) with T = 1 / (lambda_p - lambda_l)^2, so the far-wing opacity of a
+// block of points is a sum of three matrix products,
+//     Lop[d, p] += sum_n sum_l K_n[d, l] X_n[l, p],   X_n = T^(n+1)  (0 inside the core zone),
+// and the core zones are added by k_opacity32_core.  v_mfma_f32_16x16x4_f32 takes 16 depths x
+// 4 lines (A, the CoefA records as loaded) times 4 lines x 16 points (B, computed per lane)
+// into a 16 x 16 FP32 block.  A workgroup owns a tile of TILE = 128 points and all depths (NQ
+// blocks of 16); wavefront w owns points 32w .. 32w+31 (two 16-point blocks).  Candidate lines
+// are compacted per wavefront with a ballot over 64 of them at a time and taken 4 per group;
+// the loads of the next group are in flight while a group runs.  Partial sums hold FOLDG
+// groups of lines and are then added to the running sums with Kahan compensation.
+// Lane layout of v_mfma_f32_16x16x4_f32: A[r][k] in lane 16k + r, B[k][c] in lane 16k + c,
+// D[4 (i / 16) + e][i % 16] in element e of lane i.
+typedef float mf4 __attribute__((ext_vector_type(4)));
+constexpr int FOLDG = 8;  // groups of 4 lines per partial sum
+
+__device__ __forceinline__ void kahan_add4(mf4 &acc, mf4 &cmp, mf4 val) {
+  const mf4 y = val - cmp;
+  const mf4 t = acc + y;
+  cmp = (t - acc) - y;
+  acc = t;
+}
+
+template <int NQ>
+__global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, int nbins, int rstride,
+                                                        const int2 *__restrict__ rng, const CoefA *__restrict__ ca,
+                                                        const double *__restrict__ wlc_s,
+                                                        const int4 *__restrict__ meta_s,
+                                                        const int2 *__restrict__ zone_s, float *__restrict__ lop) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  if (t >= ntiles) return;
+  const int N = S.ntau, ntp = S.ntp, nwl = S.nwl;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int col = lane & 15, kq = lane >> 4;  // point / depth within a 16-block; line of the group
+  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, nwl);
+  const double Wt0 = S.wave[p0 - 1];
+  int pp[2];     // this lane's points (B columns)
+  float ofs[2];  // their offsets from the tile's first point (sub-angstrom, exact in FP32)
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    pp[q] = p0 + 32 * wave + 16 * q + col;
+    ofs[q] = pp[q] <= nwl ? (float)(S.wave[pp[q] - 1] - Wt0) : 0.0f;
+  }
+  mf4 C[2][NQ], acc[2][NQ], cmp[2][NQ];
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int nb = 0; nb < NQ; nb++) {
+      C[q][nb] = mf4{0.f, 0.f, 0.f, 0.f};
+      acc[q][nb] = mf4{0.f, 0.f, 0.f, 0.f};
+      cmp[q][nb] = mf4{0.f, 0.f, 0.f, 0.f};
+    }
+  // cursor over the tile's candidate lines, bin after bin (wave-uniform): the lines with a
+  // point of the tile inside their window [il, ir] and outside their core zone [zl, zr],
+  // found with a ballot over 64 candidates at a time
+  int cbin = -1, cbase = 0, cend = 0;
+  uint64_t cmask = 0;
+  auto next_line = [&]() -> int {
+    while (!cmask) {
+      cbase += 64;
+      while (cbase >= cend) {
+        if (++cbin >= nbins) return -1;
+        const int2 r = rng[(size_t)t * rstride + cbin];
+        cbase = r.x;
+        cend = r.y;
+      }
+      bool ov = false;
+      if (cbase + lane < cend) {
+        const int2 mm = *reinterpret_cast<const int2 *>(meta_s + cbase + lane);
+        const int2 z = zone_s[cbase + lane];
+        const int lo = max(mm.x, p0), hi = min(mm.y, p1);
+        ov = (lo <= hi) & !((z.x <= lo) & (z.y >= hi));
+      }
+      cmask = __builtin_amdgcn_ballot_w64(ov);
+    }
+    const int jn = cbase + (int)__builtin_ctzll(cmask);
+    cmask &= cmask - 1;
+    return jn;
+  };
+  // software pipeline: the loads of group g + 1 are in flight while group g runs
+  int jn[4];
+  double wcn = 0.0;
+  int2 mn = make_int2(0, -1), zn = make_int2(1, 0);
+  mf4 An[NQ];
+  auto fetch = [&]() {
+#pragma unroll
+    for (int u = 0; u < 4; u++) jn[u] = next_line();
+    if (jn[0] < 0) return;
+    const int j = kq == 0 ? jn[0] : (kq == 1 ? jn[1] : (kq == 2 ? jn[2] : jn[3]));
+    const int jj = j >= 0 ? j : jn[0];
+    wcn = wlc_s[jj];
+    mn = *reinterpret_cast<const int2 *>(meta_s + jj);
+    zn = zone_s[jj];
+    if (j < 0) mn = make_int2(0, -1);  // no line: empty window
+#pragma unroll
+    for (int nb = 0; nb < NQ; nb++)
+      An[nb] = *reinterpret_cast<const mf4 *>(ca + (size_t)jj * ntp + min(16 * nb + col, ntp - 1));
+  };
+  fetch();
+  int ng = 0;  // groups in the partial sums (wave-uniform)
+  while (jn[0] >= 0) {
+    const double wc = wcn;
+    const int2 m = mn, z = zn;
+    mf4 A[NQ];
+#pragma unroll
+    for (int nb = 0; nb < NQ; nb++) A[nb] = An[nb];
+    fetch();
+    if (ng == FOLDG) {
+#pragma unroll
+      for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int nb = 0; nb < NQ; nb++) {
+          kahan_add4(acc[q][nb], cmp[q][nb], C[q][nb]);
+          C[q][nb] = mf4{0.f, 0.f, 0.f, 0.f};
+        }
+      ng = 0;
+    }
+    ng++;
+    // B: T^(n+1) at this lane's (line, point); dl = (point - first point) + (first point -
+    // centre) in FP32, relative error below 2^-23 (1 + tile width / |dl|) outside the core zone
+    const float bs = (float)(Wt0 - wc);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const float x = ofs[q] + bs;
+      const float T = __builtin_amdgcn_rcpf(fmaxf(x * x, 1e-12f));
+      const bool act = (pp[q] >= m.x) & (pp[q] <= m.y) & (pp[q] <= nwl) & ((pp[q] < z.x) | (pp[q] > z.y));
+      float X[3];
+      X[0] = act ? T : 0.0f;
+      X[1] = X[0] * T;
+      X[2] = X[1] * T;
+#pragma unroll
+      for (int n = 0; n < 3; n++)
+#pragma unroll
+        for (int nb = 0; nb < NQ; nb++)
+          C[q][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[nb][n], X[n], C[q][nb], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    if (pp[q] > nwl) continue;
+#pragma unroll
+    for (int nb = 0; nb < NQ; nb++) {
+      kahan_add4(acc[q][nb], cmp[q][nb], C[q][nb]);
+      const mf4 sum = acc[q][nb] - cmp[q][nb];
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int d = 16 * nb + 4 * kq + e;
+        if (d < N) {
+          float v = sum[e];
+          if (S.motype == 1) v = v / (float)S.cstd[d];
+          const size_t o = (size_t)d * nwl + (pp[q] - 1);
+          lop[o] = lop[o] + v;
+        }
+      }
+    }
+  }
+}
+
+// Core zones of the FP32 batched path: every line whose core zone [zl, zr] meets the tile
+// inside its window, at the points of zone and window, every depth.  Per (line, depth) the
+// far-wing expansion still applies where the whole tile lies beyond dthr (straight-line FMA
+// code); the other depths run the Voigt table / continued fractions (contrib32).  Same layout
+// as k_opacity32 (a tile of TILE points, one wavefront per DPW depths, PPL points per lane,
+// per-(line, depth) records through scalar loads).  The candidates come bin by bin from the
+// lines sorted by core-zone half width (perm), NZB - 1 bins per tile in rng.
+__global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles, const int2 *__restrict__ rng,
+                                                         const int *__restrict__ perm, const CoefA *__restrict__ ca,
+                                                         const CoefB *__restrict__ cb, const float *__restrict__ dmaxw,
+                                                         const double *__restrict__ wlc_s,
+                                                         const int4 *__restrict__ meta_s,
+                                                         const int2 *__restrict__ zone_s, float *__restrict__ lop) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  if (t >= ntiles) return;
+  const int N = S.ntau, nwv = S.ntp / DPW;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
+  const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];
+  int pp[PPL];
+  bool valid[PPL];
+  double Wp[PPL];
+  float ofs[PPL];
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    pp[q] = p0 + lane + 64 * q;
+    valid[q] = pp[q] <= S.nwl;
+    Wp[q] = valid[q] ? S.wave[pp[q] - 1] : 0.0;
+    ofs[q] = valid[q] ? (float)(Wp[q] - Wt0) : 0.0f;
+  }
+  const int d0 = wave * DPW;
+  float acc[PPL][DPW], cmp[PPL][DPW], part[PPL][DPW];
+#pragma unroll
+  for (int q = 0; q < PPL; q++)
+#pragma unroll
+    for (int k = 0; k < DPW; k++) {
+      acc[q][k] = 0.f;
+      cmp[q][k] = 0.f;
+      part[q][k] = 0.f;
+    }
+  int grp = 0;  // lines in the current partial sums (wave-uniform)
+  for (int b = 0; b < NZB - 1; b++) {
+    const int2 r = rng[(size_t)t * NZB + b];
+    for (int x = r.x; x < r.y; x++) {
+      const int j = perm[x];
+      const int4 m = meta_s[j];
+      const int2 z = zone_s[j];
+      const int lo = max(max(m.x, z.x), p0), hi = min(min(m.y, z.y), p1);
+      if (lo > hi) continue;  // zone and window miss the tile (wave-uniform)
+      const double wc = wlc_s[j];
+      const double dmin = (wc < Wt0) ? (Wt0 - wc) : (wc > Wt1 ? (wc - Wt1) : 0.0);
+      const unsigned dmb = __builtin_amdgcn_readfirstlane(__float_as_uint((float)dmin));
+      if (grp == FOLD) {
+#pragma unroll
+        for (int q = 0; q < PPL; q++)
+#pragma unroll
+          for (int k = 0; k < DPW; k++) {
+            kahan_add(acc[q][k], cmp[q][k], part[q][k]);
+            part[q][k] = 0.f;
+          }
+        grp = 0;
+      }
+      grp++;
+      bool act[PPL];
+#pragma unroll
+      for (int q = 0; q < PPL; q++) act[q] = valid[q] && pp[q] >= lo && pp[q] <= hi;
+      const f4v *cp = reinterpret_cast<const f4v *>(ca + (size_t)j * S.ntp + d0);
+      f4v rr[DPW];
+#pragma unroll
+      for (int k = 0; k < DPW; k++) rr[k] = cp[k];
+      // depths of this wave that leave the far wing somewhere in the tile (bit k)
+      unsigned gen = 0;
+      if (dmb < __float_as_uint(dmaxw[(size_t)j * nwv + wave])) {
+#pragma unroll
+        for (int k = 0; k < DPW; k++)
+          if (dmb < __float_as_uint(rr[k][3])) {
+            gen |= 1u << k;
+            rr[k][0] = rr[k][1] = rr[k][2] = 0.0f;
+          }
+      }
+      // far wing: straight-line FMA code over all depths (zero coefficients at the general
+      // depths); dl = (point - first point) + (first point - centre), both rounded to FP32
+      const float bs = (float)(Wt0 - wc);
+      float T2[PPL];
+#pragma unroll
+      for (int q = 0; q < PPL; q++) {
+        const float xx = ofs[q] + bs;
+        T2[q] = act[q] ? __builtin_amdgcn_rcpf(fmaxf(xx * xx, 1e-30f)) : 0.0f;
+      }
+#pragma unroll
+      for (int k = 0; k < DPW; k++)
+#pragma unroll
+        for (int q = 0; q < PPL; q++) {
+          const float xx = T2[q];
+          part[q][k] += xx * fmaf(fmaf(rr[k][2], xx, rr[k][1]), xx, rr[k][0]);
+        }
+      if (gen) {
+        const CoefB *cbp = cb + (size_t)j * S.ntp + d0;
+#pragma unroll
+        for (int k = 0; k < DPW; k++) {
+          if (!(gen & (1u << k))) continue;
+          const CoefB c = cbp[k];
+#pragma unroll
+          for (int q = 0; q < PPL; q++)
+            if (act[q]) part[q][k] += contrib32(S, c, (float)(Wp[q] - wc) * c.vv);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    if (!valid[q]) continue;
+#pragma unroll
+    for (int k = 0; k < DPW; k++) {
+      const int d = d0 + k;
+      if (d < N) {
+        kahan_add(acc[q][k], cmp[q][k], part[q][k]);
+        float v = acc[q][k] - cmp[q][k];
+        if (S.motype == 1) v = v / (float)S.cstd[d];
+        const size_t o = (size_t)d * S.nwl + (pp[q] - 1);
+        lop[o] = lop[o] + v;
+      }
+    }
+  }
+}
+
+// core-zone sort keys: (zone bin << 32) | centre point, the bin being the bit length of the
+// largest distance from the centre point to the ends of zone ∩ window; lines with no point in
+// zone ∩ window go to the last bin (NZB - 1), which no tile reads
+__global__ void k_zone_keys(int64_t m, int64_t mmain, const int4 *meta_s, const int2 *zone_s,
+                            unsigned long long *key, int *val) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int4 mm = meta_s[j];
+  const int2 z = zone_s[j];
+  const int lo = max(mm.x, z.x), hi = min(mm.y, z.y);
+  unsigned b = NZB - 1;
+  if (j < mmain && lo <= hi) {  // positions from mmain on hold the He I lines (their own kernel)
+    const int zh = max(max(mm.z - lo, hi - mm.z), 0);
+    b = zh == 0 ? 0u : (unsigned)(32 - __clz(zh));
+  }
+  key[j] = ((unsigned long long)b << 32) | (unsigned)mm.z;
+  val[j] = (int)j;
+}
+
+// first and one-past-last sorted position of each zone bin (zero for empty bins)
+__global__ void k_zone_bounds(int64_t m, const unsigned long long *key, int *zoff, int *zend) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const unsigned b = (unsigned)(key[i] >> 32);
+  if (i == 0 || (unsigned)(key[i - 1] >> 32) != b) zoff[b] = (int)i;
+  if (i == m - 1 || (unsigned)(key[i + 1] >> 32) != b) zend[b] = (int)(i + 1);
+}
+
+__global__ void k_fill_int(int *p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// per tile and zone bin b (NZB - 1 of them), the sorted positions of the lines whose centre
+// point lies within 2^b - 1 points of the tile (zh < 2^b in bin b)
+__global__ void k_tile_ranges_zone(int ntiles, int tile, int nwl, const int *zoff, const int *zend,
+                                   const unsigned long long *key, int2 *rng) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (t >= ntiles || b >= NZB - 1) return;
+  const int p0 = t * tile + 1, p1 = min(p0 + tile - 1, nwl);
+  const int off = zoff[b], cnt = zend[b] - zoff[b];
+  const long h = (1L << b) - 1;
+  const long lo = (long)p0 - h, hi = (long)p1 + h;
+  int a = 0, e = cnt;
+  while (a < e) {  // first with centre >= lo
+    const int mid = (a + e) >> 1;
+    if ((long)(unsigned)key[off + mid] < lo) a = mid + 1;
+    else e = mid;
+  }
+  const int s = a;
+  e = cnt;
+  while (a < e) {  // first with centre > hi
+    const int mid = (a + e) >> 1;
+    if ((long)(unsigned)key[off + mid] <= hi) a = mid + 1;
+    else e = mid;
+  }
+  rng[(size_t)t * NZB + b] = make_int2(off + s, off + a);
 }
 
 // FP64 plain path: the reference's arithmetic (table lookup in the box, continued
@@ -837,11 +1205,12 @@ void launch_mark_wide(const DevSetup &S, int64_t n, const double *wl, const int 
 
 void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                    const int *flag, const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
-                   double *wlc_s, int4 *meta_s, hipStream_t st) {
+                   int2 *zone_s, double *wlc_s, int4 *meta_s, hipStream_t st) {
   if (nret)
-    k_fill32<<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, ca, cb, dmaxw, wlc_s,
-                                              meta_s);
+    k_fill32<<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, ca, cb, dmaxw, zone_s,
+                                              wlc_s, meta_s);
 }
+
 
 void launch_fill64(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                    const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
@@ -855,15 +1224,64 @@ void launch_tile_ranges(int ntiles, int tile, int nbins, int nwl, const int *bin
   k_tile_ranges<<<grid, 256, 0, st>>>(ntiles, tile, nbins, nwl, bin_off, bin_cnt, bin_hmax, meta_s, rng);
 }
 
-void launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
-                      const CoefB *cb, const float *dmaxw, const double *wlc_s, const int4 *meta_s, float *lineop,
-                      int he, hipStream_t st) {
+// returns 1 when the matrix-core kernel ran (the core zones then need launch_opacity32_core)
+int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
+                     const CoefB *cb, const float *dmaxw, const int2 *zone_s, const double *wlc_s,
+                     const int4 *meta_s, float *lineop, int he, hipStream_t st) {
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
-  if (he)
+  const int nq = (S.ntau + 15) / 16;
+  if (he) {
     k_opacity32<true><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, cb, dmaxw, wlc_s, meta_s, lineop);
-  else
-    k_opacity32<false><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, cb, dmaxw, wlc_s, meta_s, lineop);
+    return 0;
+  }
+  static_assert(TILE == 128, "the matrix-core kernel covers 128 points with 4 wavefronts");
+  switch (nq) {  // depth blocks of 16: up to 128 depths on the matrix cores
+#define FSG_MFMA_CASE(Q)                                                                                  \
+  case Q:                                                                                                \
+    k_opacity32_mfma<Q><<<nb, 256, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, wlc_s, meta_s, zone_s, lineop); \
+    return 1;
+    FSG_MFMA_CASE(1)
+    FSG_MFMA_CASE(2)
+    FSG_MFMA_CASE(3)
+    FSG_MFMA_CASE(4)
+    FSG_MFMA_CASE(5)
+    FSG_MFMA_CASE(6)
+    FSG_MFMA_CASE(7)
+    FSG_MFMA_CASE(8)
+#undef FSG_MFMA_CASE
+    default:
+      k_opacity32<false><<<nb, 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, cb, dmaxw, wlc_s, meta_s, lineop);
+      return 0;
+  }
+}
+
+void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, const int4 *meta_s, const int2 *zone_s, unsigned long long *key,
+                       int *val, unsigned long long *key_sorted, int *perm, void *tmp, size_t tmp_bytes, int *zoff,
+                       int *zend, int2 *rng_core, hipStream_t st) {
+  k_fill_int<<<1, 2 * NZB, 0, st>>>(zoff, 2 * NZB, 0);  // zoff and zend are adjacent
+  if (m > 0) {
+    k_zone_keys<<<nblk(m, 256), 256, 0, st>>>(m, mmain, meta_s, zone_s, key, val);
+    (void)hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key_sorted, val, perm, (int)m, 0, 64, st);
+    k_zone_bounds<<<nblk(m, 256), 256, 0, st>>>(m, key_sorted, zoff, zend);
+  }
+  dim3 grid(nblk(ntiles, 256), NZB - 1);
+  k_tile_ranges_zone<<<grid, 256, 0, st>>>(ntiles, TILE, nwl, zoff, zend, key_sorted, rng_core);
+}
+
+size_t zone_sort_tmp_bytes(int64_t m) {
+  size_t b = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b, (unsigned long long *)nullptr, (unsigned long long *)nullptr,
+                                           (int *)nullptr, (int *)nullptr, (int)m, 0, 64);
+  return b;
+}
+
+void launch_opacity32_core(const DevSetup &S, int ntiles, const int2 *rng_core, const int *perm, const CoefA *ca,
+                           const CoefB *cb, const float *dmaxw, const int2 *zone_s, const double *wlc_s,
+                           const int4 *meta_s, float *lineop, hipStream_t st) {
+  int nw = (S.ntau + DPW - 1) / DPW;
+  int nb = 8 * ((ntiles + 7) / 8);
+  k_opacity32_core<<<nb, 64 * nw, 0, st>>>(S, ntiles, rng_core, perm, ca, cb, dmaxw, wlc_s, meta_s, zone_s, lineop);
 }
 
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
