@@ -16,9 +16,11 @@ LIB = PKG / "libfsgpu.so"
 SOURCES = ["fsg_kernels.hip", "fsg_magnetic.hip", "fsg_adjoint.hip", "fsg_api.hip"]
 ARCH = os.environ.get("FSG_OFFLOAD_ARCH", "gfx950")
 
-# Packed FP32 instructions are disabled for all device code.
+# Packed FP32 instructions are disabled for all device code.  MFMA accumulators are kept in
+# ordinary VGPRs (no AGPR copies around the compensated sums of the FP32 opacity kernel).
 HIPFLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-mllvm", "-amdgpu-mfma-vgpr-form",
             "-Wno-unused-command-line-argument"]
 
 

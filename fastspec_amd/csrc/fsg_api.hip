@@ -628,9 +628,10 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
     const size_t csz = c->precision == FSG_FP32 ? sizeof(CoefA) + sizeof(CoefB) : sizeof(Coef64);
     // FP32 adds one float per (line, wavefront depth group), the group's largest far-wing
     // threshold, and the line's core zone (int2)
-    const size_t wsz = c->precision == FSG_FP32 ? (size_t)(S.ntp / DPW) * sizeof(float) + sizeof(int2) : 0;
+    const size_t wsz =
+        c->precision == FSG_FP32 ? (size_t)(S.ntp / DPW) * sizeof(float) + sizeof(int2) + sizeof(LineRec) : 0;
     const int64_t chunk = chunk_lines(c, nret, nlines_lim, S.ntp * csz + wsz + sizeof(double) + sizeof(int4), 0);
-    CK(c->coef.ensure((size_t)chunk * (S.ntp * csz + wsz) + sizeof(CoefB)));
+    CK(c->coef.ensure((size_t)chunk * (S.ntp * csz + wsz) + 6 * 32));  // + alignment of five arrays
     CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
     CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
     const int ntiles = (S.nwl + TILE - 1) / TILE;
@@ -653,15 +654,22 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       std::vector<int> &packed = tables.back();
       chunk_tables(goff, gcnt, ghmax, s0, s1, packed);
       CK(hipMemcpyAsync(c->binoff, packed.data(), sizeof(int) * packed.size(), hipMemcpyHostToDevice, s));
-      // CoefA records (16 B) first, then the CoefB records from the next 32-byte boundary
-      CoefA *ca = c->coef.as<CoefA>();
-      CoefB *cb = reinterpret_cast<CoefB *>(ca + (((size_t)m * S.ntp + 1) & ~(size_t)1));
-      float *dmaxw = reinterpret_cast<float *>(cb + (size_t)m * S.ntp);
-      int2 *zone = reinterpret_cast<int2 *>(dmaxw + (((size_t)m * (S.ntp / DPW) + 1) & ~(size_t)1));
+      // carve the chunk's records out of the coefficient buffer, each array 32-byte aligned
+      char *cur = static_cast<char *>(c->coef.p);
+      auto carve = [&cur](size_t bytes) {
+        char *p = cur;
+        cur += (bytes + 31) & ~(size_t)31;
+        return p;
+      };
+      CoefA *ca = reinterpret_cast<CoefA *>(carve((size_t)m * S.ntp * sizeof(CoefA)));
+      CoefB *cb = reinterpret_cast<CoefB *>(carve((size_t)m * S.ntp * sizeof(CoefB)));
+      float *dmaxw = reinterpret_cast<float *>(carve((size_t)m * (S.ntp / DPW) * sizeof(float)));
+      int2 *zone = reinterpret_cast<int2 *>(carve((size_t)m * sizeof(int2)));
+      LineRec *lrec = reinterpret_cast<LineRec *>(carve((size_t)m * sizeof(LineRec)));
       const int *order = c->idx2.as<int>() + s0;
       if (c->precision == FSG_FP32)
         launch_fill32(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
-                      c->ic.as<int>(), ca, cb, dmaxw, zone, c->wlc.as<double>(), c->meta.as<int4>(), s);
+                      c->ic.as<int>(), ca, cb, dmaxw, zone, lrec, c->wlc.as<double>(), c->meta.as<int4>(), s);
       else
         launch_fill64(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
                       c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
@@ -678,12 +686,12 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       timed = true;
       if (c->precision == FSG_FP32) {
         if (has_main && nbins > 0)
-          if (launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, zone, c->wlc.as<double>(),
+          if (launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, lrec, c->wlc.as<double>(),
                                c->meta.as<int4>(), (float *)c->lineop, 0, s))
             launch_opacity32_core(S, ntiles, c->rng2.as<int2>(), c->zperm.as<int>(), ca, cb, dmaxw, zone,
                                   c->wlc.as<double>(), c->meta.as<int4>(), (float *)c->lineop, s);
         if (has_he)
-          launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, dmaxw, zone, c->wlc.as<double>(),
+          launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, dmaxw, lrec, c->wlc.as<double>(),
                            c->meta.as<int4>(), (float *)c->lineop, 1, s);
       } else {
         if (has_main && nbins > 0)

@@ -55,6 +55,13 @@ constexpr int HE_KEY = 62;      // sort key of retained lines with a tabulated H
 //     t (k0 + t (k1 + t k2)), k_n = K f_n w^(n+1); dthr = the |dl| above which it applies.
 //   General path: vv = 1/Doppler width, a, a^2, a^2 + 1/4, K, LinCop, ua (a-interpolation
 //   weight), ia (a-cell of the table, 0 = outside the table box).
+// per-line record of the FP32 matrix-core path: window, core zone and centre (32 bytes)
+struct alignas(16) LineRec {
+  int il, ir, zl, zr;
+  double wc;
+  int pad0, pad1;
+};
+
 struct alignas(16) CoefA {
   float k0, k1, k2, dthr;
 };

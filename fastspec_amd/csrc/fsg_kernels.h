@@ -22,14 +22,14 @@ void launch_mark_wide(const DevSetup &S, int64_t n, const double *wl, const int 
                       const double *wlc_list, int64_t m, int64_t nstep, unsigned long long *nwide, hipStream_t st);
 void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                    const int *flag, const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
-                   int2 *zone_s, double *wlc_s, int4 *meta_s, hipStream_t st);
+                   int2 *zone_s, LineRec *lrec, double *wlc_s, int4 *meta_s, hipStream_t st);
 void launch_fill64(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                    const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
                    int4 *meta_s, hipStream_t st);
 void launch_tile_ranges(int ntiles, int tile, int nbins, int nwl, const int *bin_off, const int *bin_cnt,
                         const int *bin_hmax, const int4 *meta_s, int2 *rng, hipStream_t st);
 int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
-                     const CoefB *cb, const float *dmaxw, const int2 *zone_s, const double *wlc_s,
+                     const CoefB *cb, const float *dmaxw, const LineRec *lrec, const double *wlc_s,
                      const int4 *meta_s, float *lineop, int he, hipStream_t st);
 void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, const int4 *meta_s, const int2 *zone_s, unsigned long long *key,
                        int *val, unsigned long long *key_sorted, int *perm, void *tmp, size_t tmp_bytes, int *zoff,

@@ -222,7 +222,7 @@ __global__ void k_retained_flag(int64_t n, const int *flag, int *out) {
 // ------------------------------------------------------------------------------------
 __global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                          const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
-                         int2 *zone_s, double *wlc_s, int4 *meta_s) {
+                         int2 *zone_s, LineRec *lrec, double *wlc_s, int4 *meta_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
   XC xc;
@@ -292,6 +292,15 @@ __global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, co
     else a = mid + 1;
   }
   zone_s[j] = make_int2(zl, a);
+  const int4 mm = meta_s[j];
+  LineRec r;
+  r.il = mm.x;
+  r.ir = mm.y;
+  r.zl = zl;
+  r.zr = a;
+  r.wc = wc;
+  r.pad0 = r.pad1 = 0;
+  lrec[j] = r;
 }
 
 
@@ -548,7 +557,10 @@ __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int 
 // Lane layout of v_mfma_f32_16x16x4_f32: A[r][k] in lane 16k + r, B[k][c] in lane 16k + c,
 // D[4 (i / 16) + e][i % 16] in element e of lane i.
 typedef float mf4 __attribute__((ext_vector_type(4)));
-constexpr int FOLDG = 8;  // groups of 4 lines per partial sum
+#ifndef FSG_FOLDG
+#define FSG_FOLDG 16
+#endif
+constexpr int FOLDG = FSG_FOLDG;  // groups of 4 lines per partial sum
 
 __device__ __forceinline__ void kahan_add4(mf4 &acc, mf4 &cmp, mf4 val) {
   const mf4 y = val - cmp;
@@ -557,12 +569,152 @@ __device__ __forceinline__ void kahan_add4(mf4 &acc, mf4 &cmp, mf4 val) {
   acc = t;
 }
 
+// Reads one ring slot of the matrix-core kernel from LDS: NQ A fragments (lane order, 1 KB
+// apart from byte address a) and the two halves of the lane's line record (at byte address
+// r and r + 64).  One asm block with its own wait: the compiler does not see these reads, so
+// it adds no wait for the direct global-to-LDS loads that fill other slots.
+template <int NQ>
+struct SlotRead;
+
+template <>
+struct SlotRead<1> {
+  __device__ __forceinline__ static void read(unsigned a, unsigned r, mf4 *A, mf4 &h0, mf4 &h1) {
+    asm volatile("ds_read_b128 %0, %3\n"
+                 "ds_read_b128 %1, %4\n"
+                 "ds_read_b128 %2, %4 offset:64\n"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(A[0]), "=&v"(h0), "=&v"(h1)
+                 : "v"(a), "v"(r));
+  }
+};
+
+template <>
+struct SlotRead<2> {
+  __device__ __forceinline__ static void read(unsigned a, unsigned r, mf4 *A, mf4 &h0, mf4 &h1) {
+    asm volatile("ds_read_b128 %0, %4\n"
+                 "ds_read_b128 %1, %4 offset:1024\n"
+                 "ds_read_b128 %2, %5\n"
+                 "ds_read_b128 %3, %5 offset:64\n"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(A[0]), "=&v"(A[1]), "=&v"(h0), "=&v"(h1)
+                 : "v"(a), "v"(r));
+  }
+};
+
+template <>
+struct SlotRead<3> {
+  __device__ __forceinline__ static void read(unsigned a, unsigned r, mf4 *A, mf4 &h0, mf4 &h1) {
+    asm volatile("ds_read_b128 %0, %5\n"
+                 "ds_read_b128 %1, %5 offset:1024\n"
+                 "ds_read_b128 %2, %5 offset:2048\n"
+                 "ds_read_b128 %3, %6\n"
+                 "ds_read_b128 %4, %6 offset:64\n"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(h0), "=&v"(h1)
+                 : "v"(a), "v"(r));
+  }
+};
+
+template <>
+struct SlotRead<4> {
+  __device__ __forceinline__ static void read(unsigned a, unsigned r, mf4 *A, mf4 &h0, mf4 &h1) {
+    asm volatile("ds_read_b128 %0, %6\n"
+                 "ds_read_b128 %1, %6 offset:1024\n"
+                 "ds_read_b128 %2, %6 offset:2048\n"
+                 "ds_read_b128 %3, %6 offset:3072\n"
+                 "ds_read_b128 %4, %7\n"
+                 "ds_read_b128 %5, %7 offset:64\n"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(h0), "=&v"(h1)
+                 : "v"(a), "v"(r));
+  }
+};
+
+template <>
+struct SlotRead<5> {
+  __device__ __forceinline__ static void read(unsigned a, unsigned r, mf4 *A, mf4 &h0, mf4 &h1) {
+    asm volatile("ds_read_b128 %0, %7\n"
+                 "ds_read_b128 %1, %7 offset:1024\n"
+                 "ds_read_b128 %2, %7 offset:2048\n"
+                 "ds_read_b128 %3, %7 offset:3072\n"
+                 "ds_read_b128 %4, %7 offset:4096\n"
+                 "ds_read_b128 %5, %8\n"
+                 "ds_read_b128 %6, %8 offset:64\n"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(A[4]), "=&v"(h0), "=&v"(h1)
+                 : "v"(a), "v"(r));
+  }
+};
+
+template <>
+struct SlotRead<6> {
+  __device__ __forceinline__ static void read(unsigned a, unsigned r, mf4 *A, mf4 &h0, mf4 &h1) {
+    asm volatile("ds_read_b128 %0, %8\n"
+                 "ds_read_b128 %1, %8 offset:1024\n"
+                 "ds_read_b128 %2, %8 offset:2048\n"
+                 "ds_read_b128 %3, %8 offset:3072\n"
+                 "ds_read_b128 %4, %8 offset:4096\n"
+                 "ds_read_b128 %5, %8 offset:5120\n"
+                 "ds_read_b128 %6, %9\n"
+                 "ds_read_b128 %7, %9 offset:64\n"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(A[4]), "=&v"(A[5]), "=&v"(h0), "=&v"(h1)
+                 : "v"(a), "v"(r));
+  }
+};
+
+template <>
+struct SlotRead<7> {
+  __device__ __forceinline__ static void read(unsigned a, unsigned r, mf4 *A, mf4 &h0, mf4 &h1) {
+    asm volatile("ds_read_b128 %0, %9\n"
+                 "ds_read_b128 %1, %9 offset:1024\n"
+                 "ds_read_b128 %2, %9 offset:2048\n"
+                 "ds_read_b128 %3, %9 offset:3072\n"
+                 "ds_read_b128 %4, %9 offset:4096\n"
+                 "ds_read_b128 %5, %9 offset:5120\n"
+                 "ds_read_b128 %6, %9 offset:6144\n"
+                 "ds_read_b128 %7, %10\n"
+                 "ds_read_b128 %8, %10 offset:64\n"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(A[4]), "=&v"(A[5]), "=&v"(A[6]), "=&v"(h0), "=&v"(h1)
+                 : "v"(a), "v"(r));
+  }
+};
+
+template <>
+struct SlotRead<8> {
+  __device__ __forceinline__ static void read(unsigned a, unsigned r, mf4 *A, mf4 &h0, mf4 &h1) {
+    asm volatile("ds_read_b128 %0, %10\n"
+                 "ds_read_b128 %1, %10 offset:1024\n"
+                 "ds_read_b128 %2, %10 offset:2048\n"
+                 "ds_read_b128 %3, %10 offset:3072\n"
+                 "ds_read_b128 %4, %10 offset:4096\n"
+                 "ds_read_b128 %5, %10 offset:5120\n"
+                 "ds_read_b128 %6, %10 offset:6144\n"
+                 "ds_read_b128 %7, %10 offset:7168\n"
+                 "ds_read_b128 %8, %11\n"
+                 "ds_read_b128 %9, %11 offset:64\n"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(A[4]), "=&v"(A[5]), "=&v"(A[6]), "=&v"(A[7]), "=&v"(h0), "=&v"(h1)
+                 : "v"(a), "v"(r));
+  }
+};
+
 template <int NQ>
 __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, int nbins, int rstride,
                                                         const int2 *__restrict__ rng, const CoefA *__restrict__ ca,
-                                                        const double *__restrict__ wlc_s,
-                                                        const int4 *__restrict__ meta_s,
-                                                        const int2 *__restrict__ zone_s, float *__restrict__ lop) {
+                                                        const LineRec *__restrict__ lrec, float *__restrict__ lop) {
+  // per wavefront a ring of R group slots in LDS, filled by direct global-to-LDS loads: the
+  // A fragments of the group (NQ x 64 lanes x 16 B, in lane order) and the records of its 4
+  // lines (two 16-byte halves each)
+#ifndef FSG_RING
+#define FSG_RING 2
+#endif
+  constexpr int R = FSG_RING;  // 2: 42 KB of LDS per workgroup for NQ = 5, three workgroups per CU
+  static_assert(R == 2 || R == 3, "ring of 2 or 3 slots");
+  constexpr int SLOT = NQ * 64 + 8;
+  __shared__ mf4 ring[4][R][SLOT];
+  typedef __attribute__((address_space(3))) void lds_void;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
   const int N = S.ntau, ntp = S.ntp, nwl = S.nwl;
@@ -602,10 +754,9 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
       }
       bool ov = false;
       if (cbase + lane < cend) {
-        const int2 mm = *reinterpret_cast<const int2 *>(meta_s + cbase + lane);
-        const int2 z = zone_s[cbase + lane];
-        const int lo = max(mm.x, p0), hi = min(mm.y, p1);
-        ov = (lo <= hi) & !((z.x <= lo) & (z.y >= hi));
+        const int4 w = *reinterpret_cast<const int4 *>(lrec + cbase + lane);  // il, ir, zl, zr
+        const int lo = max(w.x, p0), hi = min(w.y, p1);
+        ov = (lo <= hi) & !((w.z <= lo) & (w.w >= hi));
       }
       cmask = __builtin_amdgcn_ballot_w64(ov);
     }
@@ -613,34 +764,34 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
     cmask &= cmask - 1;
     return jn;
   };
-  // software pipeline: the loads of group g + 1 are in flight while group g runs
-  int jn[4];
-  double wcn = 0.0;
-  int2 mn = make_int2(0, -1), zn = make_int2(1, 0);
-  mf4 An[NQ];
-  auto fetch = [&]() {
+  // fetch: the next group's lines from the cursor and its loads into ring slot s; returns
+  // the number of lines (0 when the tile is done).  A[nb] reads depths 16 nb .. 16 nb + 15 of
+  // the line's record; depths past ntp land in the following records or in the CoefB array
+  // after the last one (finite values), and only feed accumulator rows d >= N, never stored.
+  auto fetch = [&](int s) -> int {
+    int jn[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) jn[u] = next_line();
-    if (jn[0] < 0) return;
+    if (jn[0] < 0) return 0;
+    const int nv = 1 + (jn[1] >= 0) + (jn[2] >= 0) + (jn[3] >= 0);
     const int j = kq == 0 ? jn[0] : (kq == 1 ? jn[1] : (kq == 2 ? jn[2] : jn[3]));
     const int jj = j >= 0 ? j : jn[0];
-    wcn = wlc_s[jj];
-    mn = *reinterpret_cast<const int2 *>(meta_s + jj);
-    zn = zone_s[jj];
-    if (j < 0) mn = make_int2(0, -1);  // no line: empty window
+    mf4 *slot = ring[wave][s];
+    const mf4 *src = reinterpret_cast<const mf4 *>(ca + (size_t)jj * ntp + col);
 #pragma unroll
     for (int nb = 0; nb < NQ; nb++)
-      An[nb] = *reinterpret_cast<const mf4 *>(ca + (size_t)jj * ntp + min(16 * nb + col, ntp - 1));
+      __builtin_amdgcn_global_load_lds((const void *)(src + 16 * nb), (lds_void *)(slot + 64 * nb), 16, 0, 0);
+    if (lane < 4) {
+      const int jr = lane == 0 ? jn[0] : (lane == 1 ? jn[1] : (lane == 2 ? jn[2] : jn[3]));
+      const mf4 *rs = reinterpret_cast<const mf4 *>(lrec + (jr >= 0 ? jr : jn[0]));
+      __builtin_amdgcn_global_load_lds((const void *)rs, (lds_void *)(slot + 64 * NQ), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void *)(rs + 1), (lds_void *)(slot + 64 * NQ + 4), 16, 0, 0);
+    }
+    return nv;
   };
-  fetch();
+  // the group in slot s, its loads complete
   int ng = 0;  // groups in the partial sums (wave-uniform)
-  while (jn[0] >= 0) {
-    const double wc = wcn;
-    const int2 m = mn, z = zn;
-    mf4 A[NQ];
-#pragma unroll
-    for (int nb = 0; nb < NQ; nb++) A[nb] = An[nb];
-    fetch();
+  auto run = [&](int s, int nv) {
     if (ng == FOLDG) {
 #pragma unroll
       for (int q = 0; q < 2; q++)
@@ -652,6 +803,13 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
       ng = 0;
     }
     ng++;
+    const unsigned sa = static_cast<unsigned>(reinterpret_cast<uintptr_t>((lds_void *)ring[wave][s]));
+    mf4 A[NQ], h0, h1;
+    SlotRead<NQ>::read(sa + 16 * lane, sa + 1024 * NQ + 16 * kq, A, h0, h1);
+    const int il = __float_as_int(h0[0]), ir = __float_as_int(h0[1]);
+    const int zl = __float_as_int(h0[2]), zr = __float_as_int(h0[3]);
+    const double wc = __hiloint2double(__float_as_int(h1[1]), __float_as_int(h1[0]));
+    const bool lv = kq < nv;
     // B: T^(n+1) at this lane's (line, point); dl = (point - first point) + (first point -
     // centre) in FP32, relative error below 2^-23 (1 + tile width / |dl|) outside the core zone
     const float bs = (float)(Wt0 - wc);
@@ -659,7 +817,7 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
     for (int q = 0; q < 2; q++) {
       const float x = ofs[q] + bs;
       const float T = __builtin_amdgcn_rcpf(fmaxf(x * x, 1e-12f));
-      const bool act = (pp[q] >= m.x) & (pp[q] <= m.y) & (pp[q] <= nwl) & ((pp[q] < z.x) | (pp[q] > z.y));
+      const bool act = lv & (pp[q] >= il) & (pp[q] <= ir) & (pp[q] <= nwl) & ((pp[q] < zl) | (pp[q] > zr));
       float X[3];
       X[0] = act ? T : 0.0f;
       X[1] = X[0] * T;
@@ -670,7 +828,37 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
         for (int nb = 0; nb < NQ; nb++)
           C[q][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[nb][n], X[n], C[q][nb], 0, 0, 0);
     }
+  };
+  // wait until at most n vector memory operations are outstanding (asm, so that the compiler
+  // neither moves nor merges it)
+#define FSG_VMCNT(n) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(n))
+  constexpr int LPG = NQ + 2;  // loads per group
+  if constexpr (R == 3) {
+    int nv0 = fetch(0), nv1 = fetch(1), s0 = 0;
+    while (nv0) {
+      const int s2 = s0 == 0 ? 2 : s0 - 1;  // slot of the group R - 1 ahead
+      const int nv2 = fetch(s2);
+      // slot s0 was filled R - 1 fetches ago; the loads issued since then are newer
+      if (nv2) FSG_VMCNT(2 * LPG);
+      else if (nv1) FSG_VMCNT(LPG);
+      else FSG_VMCNT(0);
+      run(s0, nv0);
+      nv0 = nv1;
+      nv1 = nv2;
+      s0 = s0 == 2 ? 0 : s0 + 1;
+    }
+  } else {
+    int nv0 = fetch(0), s0 = 0;
+    while (nv0) {
+      const int nv1 = fetch(s0 ^ 1);
+      if (nv1) FSG_VMCNT(LPG);
+      else FSG_VMCNT(0);
+      run(s0, nv0);
+      nv0 = nv1;
+      s0 ^= 1;
+    }
   }
+#undef FSG_VMCNT
 #pragma unroll
   for (int q = 0; q < 2; q++) {
     if (pp[q] > nwl) continue;
@@ -1205,10 +1393,10 @@ void launch_mark_wide(const DevSetup &S, int64_t n, const double *wl, const int 
 
 void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                    const int *flag, const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
-                   int2 *zone_s, double *wlc_s, int4 *meta_s, hipStream_t st) {
+                   int2 *zone_s, LineRec *lrec, double *wlc_s, int4 *meta_s, hipStream_t st) {
   if (nret)
     k_fill32<<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, ca, cb, dmaxw, zone_s,
-                                              wlc_s, meta_s);
+                                              lrec, wlc_s, meta_s);
 }
 
 
@@ -1226,7 +1414,7 @@ void launch_tile_ranges(int ntiles, int tile, int nbins, int nwl, const int *bin
 
 // returns 1 when the matrix-core kernel ran (the core zones then need launch_opacity32_core)
 int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
-                     const CoefB *cb, const float *dmaxw, const int2 *zone_s, const double *wlc_s,
+                     const CoefB *cb, const float *dmaxw, const LineRec *lrec, const double *wlc_s,
                      const int4 *meta_s, float *lineop, int he, hipStream_t st) {
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
@@ -1239,7 +1427,7 @@ int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, cons
   switch (nq) {  // depth blocks of 16: up to 128 depths on the matrix cores
 #define FSG_MFMA_CASE(Q)                                                                                  \
   case Q:                                                                                                \
-    k_opacity32_mfma<Q><<<nb, 256, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, wlc_s, meta_s, zone_s, lineop); \
+    k_opacity32_mfma<Q><<<nb, 256, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, lrec, lineop);            \
     return 1;
     FSG_MFMA_CASE(1)
     FSG_MFMA_CASE(2)
