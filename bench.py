@@ -6,8 +6,9 @@ Workload: the large atomic configuration -- 2,000,000 seeded lines, lambda log-u
 fastspec_amd/workloads.py); lines and host tables are uploaded to HBM before timing.
 
 One step = one spectrum: LineOpac reset, pass 1 (rejection + windows), width binning,
-coefficient fill, tiled opacity accumulation (FP32 Voigt, Kahan sums), FP64 DELO-Bezier for
-continuum and line+continuum at 7 mu, flux sums.  The FP64 plain GPU path (table Voigt,
+coefficient fill, tiled opacity accumulation (FP32: far-wing series on the matrix cores,
+core zones with the Voigt table on the vector units, blocked Kahan sums), FP64 DELO-Bezier
+for continuum and line+continuum at 7 mu, flux sums.  The FP64 plain GPU path (table Voigt,
 FP64 sums, one bin) is timed on the same input and the FP32 flux is compared with it.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
@@ -29,6 +30,20 @@ sys.path.insert(0, str(ROOT))
 METRIC = "line·depth·λ evals/s + spectra/s; FP32-batched vs FP64 GPU at ≤1e-4 flux err"
 PEAK_FP32_TFLOPS = 157.3
 FLOP_PER_EVAL = 14          # U_op cost of the table-lerp evaluation (fixed definition)
+MFMA_FLOP_PER_EVAL = 6      # far-wing series T (k0 + T (k1 + T k2)): three FMA per evaluation
+TRAFFIC_MD = ROOT / "profiles" / "opacity_c2_hbm_traffic.md"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the opacity stage from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel), or None."""
+    try:
+        for line in TRAFFIC_MD.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg: str) -> None:
@@ -181,6 +196,19 @@ def main():
     evals = float(st["evals"])
     value = evals / (r32["ms_per_step"] * 1e-3)
     achieved = evals * FLOP_PER_EVAL / (r32["opacity_ms"] * 1e-3) / 1e12
+    # the two kernels of the stage: far wings on the matrix cores (depths padded to blocks
+    # of 16), core zones on the vector units
+    far = evals - float(st["evals_core"])
+    pad = 16 * ((setup.ntau + 15) // 16) / setup.ntau
+    far_tf = far * MFMA_FLOP_PER_EVAL / (st["ms_far"] * 1e-3) / 1e12 if st["ms_far"] > 0 else None
+    kernels = {
+        "k_opacity32_mfma": {"ms": st["ms_far"], "evals": far, "mfma_flop_per_eval": MFMA_FLOP_PER_EVAL,
+                             "depth_padding": pad, "tflops": far_tf,
+                             "frac_of_fp32_mfma_peak": far_tf / PEAK_FP32_TFLOPS if far_tf else None,
+                             "issued_frac": far_tf * pad / PEAK_FP32_TFLOPS if far_tf else None},
+        "k_opacity32_core": {"ms": st["ms_core"], "evals": float(st["evals_core"]),
+                             "evals_per_s": st["evals_core"] / (st["ms_core"] * 1e-3) if st["ms_core"] > 0 else None},
+    }
     out = {
         "metric": METRIC, "value": value, "unit": "line·depth·λ evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -193,12 +221,14 @@ def main():
         "spectra_per_s": 1e3 / r32["ms_per_step"],
         "evals_per_step": evals,
         "retained_lines": st["nlines_use"],
-        # compute-bound on the FP32 vector ALUs (no dense contraction, so no MFMA): MI355X's
-        # FP32 peak, 157.3 TFLOP/s, is the same for the vector and the matrix units
+        # compute-bound: the far-wing series is a dense contraction over lines (FP32 MFMA,
+        # 157.3 TFLOP/s dense); achieved counts the fixed 14 flop per evaluation over the
+        # whole opacity stage (both kernels and the final add), kernels[] the per-kernel view
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "compute": "fp32 VALU", "kernel": "k_opacity32<false>", "kernel_ms": r32["opacity_ms"],
-                     "flop_per_eval": FLOP_PER_EVAL},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
+                     "compute": "fp32 MFMA (far wings) + fp32 VALU (core zones)",
+                     "kernel": "opacity stage: k_opacity32_mfma + k_opacity32_core (concurrent streams)",
+                     "kernel_ms": r32["opacity_ms"], "flop_per_eval": FLOP_PER_EVAL, "kernels": kernels},
     }
     if not args.no_fp64:
         r64 = run_path(setup, lines, FSG_FP64, 0, args.fp64_steps, 0)

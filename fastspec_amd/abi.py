@@ -69,6 +69,7 @@ class CStats(C.Structure):
         ("nlines_in", C.c_int64), ("nlines_use", C.c_int64), ("nwide", C.c_int64),
         ("nblocks", C.c_int64), ("nsegments", C.c_int64), ("evals", C.c_int64),
         ("ms_pass1", C.c_double), ("ms_opacity", C.c_double),
+        ("evals_core", C.c_int64), ("ms_far", C.c_double), ("ms_core", C.c_double),
     ]
 
     def as_dict(self) -> dict:

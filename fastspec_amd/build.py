@@ -13,6 +13,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libfsgpu.so"
+CALIB = PKG / "hbm_copy_cal"
 SOURCES = ["fsg_kernels.hip", "fsg_magnetic.hip", "fsg_adjoint.hip", "fsg_api.hip"]
 ARCH = os.environ.get("FSG_OFFLOAD_ARCH", "gfx950")
 
@@ -35,6 +36,10 @@ def build(verbose: bool = False) -> Path:
     objdir = PKG / "_build"
     objdir.mkdir(exist_ok=True)
     objs, procs = [], []
+    # the HBM counter calibration program (a 1 GiB copy) builds beside the library
+    cal = [hipcc(), *HIPFLAGS, str(CSRC / "hbm_copy_cal.hip"), "-o", str(CALIB)]
+    procs.append(("hbm_copy_cal.hip", subprocess.Popen(cal, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                                       text=True)))
     for src in SOURCES:  # the translation units compile concurrently
         obj = objdir / (Path(src).stem + ".o")
         cmd = [hipcc(), *HIPFLAGS, "-I", str(CSRC), "-c", str(CSRC / src), "-o", str(obj)]

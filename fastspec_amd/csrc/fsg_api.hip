@@ -76,6 +76,13 @@ struct fsg_ctx {
   int *bincnt = nullptr, *binhmax = nullptr, *binoff = nullptr;
   unsigned long long *u64 = nullptr;
   hipEvent_t ev[4] = {};
+  // FP32 path: the core-zone kernel runs on a second stream beside the far-wing kernel and
+  // accumulates into lop2 (zero between calls), added to lineop after the last chunk
+  hipStream_t st2 = nullptr;
+  hipEvent_t evf = nullptr, evj = nullptr;
+  hipEvent_t evt[4] = {};  // first and last far-wing launch (on st), core-zone launch (on st2)
+  float *lop2 = nullptr;
+  bool lop2_dirty = false;  // a core launch has not been followed by its add (an error path)
   // polarised path, allocated on first use
   void *maglineop = nullptr;  // [6][ntau][nwl] MagLineOpac, wavelength fastest
   float2 *ftab = nullptr;     // Faraday-Voigt table [NVA+1][TAB_LD]
@@ -261,6 +268,12 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
   CK(hipSetDevice(c->dev));
   CK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   for (auto &e : c->ev) CK(hipEventCreate(&e));
+  if (c->precision == FSG_FP32) {
+    CK(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+    CK(hipEventCreateWithFlags(&c->evf, hipEventDisableTiming));
+    CK(hipEventCreateWithFlags(&c->evj, hipEventDisableTiming));
+    for (auto &e : c->evt) CK(hipEventCreate(&e));
+  }
 
   DevSetup &S = c->S;
   const int N = s->ntau, Nw = s->nwl;
@@ -383,6 +396,10 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
     S.vtab = (const float *)tab;
     CK(hipStreamSynchronize(c->st));
   }
+  if (c->precision == FSG_FP32 && alloc_zero(c, sizeof(float) * (size_t)N * Nw, (void **)&c->lop2)) {
+    fsg_finalize(c);
+    return 1;
+  }
   if (alloc_zero(c, lop_elem(c) * (size_t)N * Nw, &c->lineop) ||
       alloc_zero(c, sizeof(int) * 64, (void **)&c->bincnt) ||
       alloc_zero(c, sizeof(int) * 64, (void **)&c->binhmax) ||
@@ -406,8 +423,14 @@ int fsg_finalize(fsg_ctx *c) {
                   &c->meta, &c->rng, &c->rng2, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp, &c->vscr,
                   &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm})
     b->release();
+  if (c->st2) (void)hipStreamSynchronize(c->st2);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->evt)
+    if (e) (void)hipEventDestroy(e);
+  if (c->evf) (void)hipEventDestroy(c->evf);
+  if (c->evj) (void)hipEventDestroy(c->evj);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return 0;
@@ -620,6 +643,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   // NlinesStep blocks, src/synth.f90:1011-1088).  Groups 0..nbins-1 are the width bins,
   // group nbins the He I lines.
   bool timed = false;  // ms_opacity runs from the first accumulation launch to the last
+  int nfar = 0, ncore = 0;  // FP32 far-wing and core-zone launches, for their timings
   if (nret > 0) {
     const int ng = nbins + 1;
     std::vector<int64_t> goff, gcnt;
@@ -647,8 +671,16 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       CK(c->tmp.ensure(ztmp));
     }
     int *zoff = c->binoff + 3 * (MAX_BINS + 1), *zend = zoff + NZB;
+    const int64_t nlop = (int64_t)S.ntau * S.nwl;
+    bool core_pending = false;  // a core-zone launch on st2 may still read the chunk buffers
+    if (c->precision == FSG_FP32 && c->lop2_dirty) {
+      CK(hipStreamSynchronize(c->st2));
+      launch_zero_f32(c->lop2, nlop, s);
+      c->lop2_dirty = false;
+    }
     std::vector<std::vector<int>> tables;  // staged for upload; alive until the final sync
     for (int64_t s0 = 0; s0 < nret; s0 += chunk) {
+      if (core_pending) CK(hipStreamWaitEvent(s, c->evj, 0));  // before the chunk buffers are reused
       const int64_t s1 = std::min(nret, s0 + chunk), m = s1 - s0;
       tables.emplace_back();
       std::vector<int> &packed = tables.back();
@@ -676,20 +708,36 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
                          c->meta.as<int4>(), c->rng.as<int2>(), s);
       if (c->precision == FSG_FP32)  // the chunk's lines by core-zone width, and per-tile candidate ranges
-        launch_zone_order(m, std::min<int64_t>(std::max<int64_t>(goff[nbins] - s0, 0), m), ntiles, S.nwl, c->meta.as<int4>(), zone, c->zkey.as<unsigned long long>(),
-                          c->zval.as<int>(), c->zkey2.as<unsigned long long>(), c->zperm.as<int>(), c->tmp.p, ztmp,
-                          zoff, zend, c->rng2.as<int2>(), s);
+        launch_zone_order(m, std::min<int64_t>(std::max<int64_t>(goff[nbins] - s0, 0), m), ntiles, S.nwl, S.ntau,
+                          c->meta.as<int4>(), zone, c->zkey.as<unsigned long long>(), c->zval.as<int>(),
+                          c->zkey2.as<unsigned long long>(), c->zperm.as<int>(), c->tmp.p, ztmp, zoff, zend,
+                          c->rng2.as<int2>(), c->u64 + 2, s);
       // the range table has ng groups per tile: the first nbins are read by the main launch
       // (stride ng), the last one by the He launch (offset nbins, stride ng)
       const bool has_main = s0 < goff[nbins], has_he = s1 > goff[nbins];
       if (!timed) CK(hipEventRecord(c->ev[2], s));
       timed = true;
       if (c->precision == FSG_FP32) {
-        if (has_main && nbins > 0)
-          if (launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, lrec, c->wlc.as<double>(),
-                               c->meta.as<int4>(), (float *)c->lineop, 0, s))
+        if (has_main && nbins > 0) {
+          // the core zones on st2 (into lop2) while the far wings run on st
+          CK(hipEventRecord(c->evf, s));
+          CK(hipStreamWaitEvent(c->st2, c->evf, 0));
+          if (!nfar) CK(hipEventRecord(c->evt[0], s));
+          const int mf = launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, lrec,
+                                          c->wlc.as<double>(), c->meta.as<int4>(), (float *)c->lineop, 0, s);
+          CK(hipEventRecord(c->evt[1], s));
+          nfar++;
+          if (mf) {
+            if (!ncore) CK(hipEventRecord(c->evt[2], c->st2));
             launch_opacity32_core(S, ntiles, c->rng2.as<int2>(), c->zperm.as<int>(), ca, cb, dmaxw, zone,
-                                  c->wlc.as<double>(), c->meta.as<int4>(), (float *)c->lineop, s);
+                                  c->wlc.as<double>(), c->meta.as<int4>(), c->lop2, c->st2);
+            CK(hipEventRecord(c->evt[3], c->st2));
+            CK(hipEventRecord(c->evj, c->st2));
+            ncore++;
+            core_pending = true;
+            c->lop2_dirty = true;
+          }
+        }
         if (has_he)
           launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, dmaxw, lrec, c->wlc.as<double>(),
                            c->meta.as<int4>(), (float *)c->lineop, 1, s);
@@ -703,6 +751,11 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       }
       st.nblocks++;
     }
+    if (core_pending) {  // lop2 is zero again afterwards (allocated zeroed)
+      CK(hipStreamWaitEvent(s, c->evj, 0));
+      launch_add_clear_f32((float *)c->lineop, c->lop2, nlop, s);
+      c->lop2_dirty = false;
+    }
   }
   if (!timed) CK(hipEventRecord(c->ev[2], s));
   CK(hipEventRecord(c->ev[3], s));
@@ -715,6 +768,15 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   CK(hipEventElapsedTime(&ms2, c->ev[2], c->ev[3]));
   st.ms_pass1 = ms1;
   st.ms_opacity = ms2;
+  st.evals_core = (int64_t)hu[2];
+  if (nfar) {
+    CK(hipEventElapsedTime(&ms1, c->evt[0], c->evt[1]));
+    st.ms_far = ms1;
+  }
+  if (ncore) {
+    CK(hipEventElapsedTime(&ms1, c->evt[2], c->evt[3]));
+    st.ms_core = ms1;
+  }
   c->have_rt = 0;
   c->have_glop = 0;
   if (stats) *stats = st;

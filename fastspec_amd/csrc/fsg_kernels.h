@@ -31,10 +31,13 @@ void launch_tile_ranges(int ntiles, int tile, int nbins, int nwl, const int *bin
 int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
                      const CoefB *cb, const float *dmaxw, const LineRec *lrec, const double *wlc_s,
                      const int4 *meta_s, float *lineop, int he, hipStream_t st);
-void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, const int4 *meta_s, const int2 *zone_s, unsigned long long *key,
-                       int *val, unsigned long long *key_sorted, int *perm, void *tmp, size_t tmp_bytes, int *zoff,
-                       int *zend, int2 *rng_core, hipStream_t st);
+void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
+                       const int2 *zone_s, unsigned long long *key, int *val, unsigned long long *key_sorted, int *perm,
+                       void *tmp, size_t tmp_bytes, int *zoff, int *zend, int2 *rng_core,
+                       unsigned long long *evals_core, hipStream_t st);
 size_t zone_sort_tmp_bytes(int64_t m);
+void launch_zero_f32(float *p, int64_t n, hipStream_t st);
+void launch_add_clear_f32(float *dst, float *src, int64_t n, hipStream_t st);
 void launch_opacity32_core(const DevSetup &S, int ntiles, const int2 *rng_core, const int *perm, const CoefA *ca,
                            const CoefB *cb, const float *dmaxw, const int2 *zone_s, const double *wlc_s,
                            const int4 *meta_s, float *lineop, hipStream_t st);

@@ -1009,8 +1009,8 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
 // core-zone sort keys: (zone bin << 32) | centre point, the bin being the bit length of the
 // largest distance from the centre point to the ends of zone ∩ window; lines with no point in
 // zone ∩ window go to the last bin (NZB - 1), which no tile reads
-__global__ void k_zone_keys(int64_t m, int64_t mmain, const int4 *meta_s, const int2 *zone_s,
-                            unsigned long long *key, int *val) {
+__global__ void k_zone_keys(int64_t m, int64_t mmain, int ntau, const int4 *meta_s, const int2 *zone_s,
+                            unsigned long long *key, int *val, unsigned long long *evals_core) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= m) return;
   const int4 mm = meta_s[j];
@@ -1020,6 +1020,7 @@ __global__ void k_zone_keys(int64_t m, int64_t mmain, const int4 *meta_s, const 
   if (j < mmain && lo <= hi) {  // positions from mmain on hold the He I lines (their own kernel)
     const int zh = max(max(mm.z - lo, hi - mm.z), 0);
     b = zh == 0 ? 0u : (unsigned)(32 - __clz(zh));
+    atomicAdd(evals_core, (unsigned long long)(hi - lo + 1) * (unsigned long long)ntau);
   }
   key[j] = ((unsigned long long)b << 32) | (unsigned)mm.z;
   val[j] = (int)j;
@@ -1032,6 +1033,20 @@ __global__ void k_zone_bounds(int64_t m, const unsigned long long *key, int *zof
   const unsigned b = (unsigned)(key[i] >> 32);
   if (i == 0 || (unsigned)(key[i - 1] >> 32) != b) zoff[b] = (int)i;
   if (i == m - 1 || (unsigned)(key[i + 1] >> 32) != b) zend[b] = (int)(i + 1);
+}
+
+__global__ void k_zero_f32(float *p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+// dst += src, and src back to zero for the next use
+__global__ void k_add_clear_f32(float *dst, float *src, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    dst[i] = dst[i] + src[i];
+    src[i] = 0.0f;
+  }
 }
 
 __global__ void k_fill_int(int *p, int n, int v) {
@@ -1444,17 +1459,26 @@ int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, cons
   }
 }
 
-void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, const int4 *meta_s, const int2 *zone_s, unsigned long long *key,
-                       int *val, unsigned long long *key_sorted, int *perm, void *tmp, size_t tmp_bytes, int *zoff,
-                       int *zend, int2 *rng_core, hipStream_t st) {
+void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
+                       const int2 *zone_s, unsigned long long *key, int *val, unsigned long long *key_sorted, int *perm,
+                       void *tmp, size_t tmp_bytes, int *zoff, int *zend, int2 *rng_core,
+                       unsigned long long *evals_core, hipStream_t st) {
   k_fill_int<<<1, 2 * NZB, 0, st>>>(zoff, 2 * NZB, 0);  // zoff and zend are adjacent
   if (m > 0) {
-    k_zone_keys<<<nblk(m, 256), 256, 0, st>>>(m, mmain, meta_s, zone_s, key, val);
+    k_zone_keys<<<nblk(m, 256), 256, 0, st>>>(m, mmain, ntau, meta_s, zone_s, key, val, evals_core);
     (void)hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key_sorted, val, perm, (int)m, 0, 64, st);
     k_zone_bounds<<<nblk(m, 256), 256, 0, st>>>(m, key_sorted, zoff, zend);
   }
   dim3 grid(nblk(ntiles, 256), NZB - 1);
   k_tile_ranges_zone<<<grid, 256, 0, st>>>(ntiles, TILE, nwl, zoff, zend, key_sorted, rng_core);
+}
+
+void launch_zero_f32(float *p, int64_t n, hipStream_t st) {
+  if (n) k_zero_f32<<<nblk(n, 256), 256, 0, st>>>(p, n);
+}
+
+void launch_add_clear_f32(float *dst, float *src, int64_t n, hipStream_t st) {
+  if (n) k_add_clear_f32<<<nblk(n, 256), 256, 0, st>>>(dst, src, n);
 }
 
 size_t zone_sort_tmp_bytes(int64_t m) {

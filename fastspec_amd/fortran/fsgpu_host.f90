@@ -65,6 +65,8 @@ MODULE FsgpuHost
   TYPE, BIND(C) :: FsgStats
     INTEGER(C_INT64_T) :: nlines_in, nlines_use, nwide, nblocks, nsegments, evals
     REAL(C_DOUBLE) :: ms_pass1, ms_opacity
+    INTEGER(C_INT64_T) :: evals_core
+    REAL(C_DOUBLE) :: ms_far, ms_core
   END TYPE FsgStats
 
   INTERFACE

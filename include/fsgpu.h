@@ -116,6 +116,12 @@ typedef struct fsg_lineop_stats {
   int64_t nsegments;             /* wavelength segments (1 on the device)            */
   int64_t evals;                 /* sum over retained lines of window pixels x Ntau  */
   double ms_pass1, ms_opacity;   /* device time of the two stages                    */
+  /* FP32 batched path only (zero otherwise): the evaluations inside the lines' core
+   * zones (|lambda - lambda_c| < max_d dthr, Voigt table / continued fractions; the rest
+   * are far-wing series on the matrix cores) and the device time of the far-wing and
+   * core-zone kernels, from their first launch to their last (they run concurrently) */
+  int64_t evals_core;
+  double ms_far, ms_core;
 } fsg_lineop_stats;
 
 typedef struct fsg_ctx fsg_ctx;
