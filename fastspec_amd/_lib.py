@@ -16,7 +16,7 @@ EXPORTS = (
     "fsg_get_lineop", "fsg_set_lineop", "fsg_get_maglineop", "fsg_radtran",
     "fsg_magradtran", "fsg_flux", "fsg_lines_upload", "fsg_lines_free", "fsg_lineop_dev",
     "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop", "fsg_maglineop_dev", "fsg_lines_set_gf",
-    "fsg_flux_vjp", "fsg_lineop_vjp",
+    "fsg_flux_vjp", "fsg_lineop_vjp", "fsg_integ_convol", "fsg_get_spectra",
 )
 
 _LIB = None
@@ -50,6 +50,8 @@ def load() -> C.CDLL:
     lib.fsg_radtran.argtypes = [vp, dp, dp]
     lib.fsg_magradtran.argtypes = [vp, dp, C.c_int32, C.c_int32, dp, dp]
     lib.fsg_flux.argtypes = [vp, dp, dp]
+    lib.fsg_integ_convol.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_double]
+    lib.fsg_get_spectra.argtypes = [vp, dp, dp, dp]
     lib.fsg_lines_upload.argtypes = [vp, C.c_int32, C.POINTER(CLines), C.POINTER(vp)]
     lib.fsg_lines_free.argtypes = [vp]
     lib.fsg_lineop_dev.argtypes = [vp, vp, C.c_int64, C.POINTER(CStats)]

@@ -73,6 +73,11 @@ struct fsg_ctx {
   int have_rt = 0;
   DBuf flag, il, ir, ic, key, key2, idx, idx2, coef, wlc, meta, rng, rng2, tmp, pos, wlcl, rflag;
   DBuf zkey, zkey2, zval, zperm;  // FP32 path: lines ordered by core-zone width
+  // convolution and disk integration: host copies of mu, weights, the grid's last point;
+  // device scratch for the convolved rows, kernels and row tables
+  std::vector<double> hxmu, hwmu;
+  double wlast = 0.0;
+  DBuf cv_a, cv_b, cv_kern, cv_rows, cv_w;
   int *bincnt = nullptr, *binhmax = nullptr, *binoff = nullptr;
   unsigned long long *u64 = nullptr;
   hipEvent_t ev[4] = {};
@@ -312,6 +317,9 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
   rc |= upload(c, s->spc_slot, s->nspc, &S.spc_slot, c->owned);
   rc |= upload(c, s->xmu, s->nmu, &S.xmu, c->owned);
   rc |= upload(c, s->wmu, s->nmu, &S.wmu, c->owned);
+  c->hxmu.assign(s->xmu, s->xmu + s->nmu);
+  c->hwmu.assign(s->wmu, s->wmu + s->nmu);
+  c->wlast = s->wfirst + s->wave[Nw - 1];
   S.ntp = DPW * ((N + DPW - 1) / DPW);
   std::vector<double> shf((size_t)std::max(s->nhe, 1) * S.ntp, 0.0);  // zero-padded to ntp depths
   if (s->nhe > 0) {
@@ -421,7 +429,8 @@ int fsg_finalize(fsg_ctx *c) {
   for (void *p : c->owned) (void)hipFree(p);
   for (DBuf *b : {&c->flag, &c->il, &c->ir, &c->ic, &c->key, &c->key2, &c->idx, &c->idx2, &c->coef, &c->wlc,
                   &c->meta, &c->rng, &c->rng2, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp, &c->vscr,
-                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm})
+                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm, &c->cv_a,
+                  &c->cv_b, &c->cv_kern, &c->cv_rows, &c->cv_w})
     b->release();
   if (c->st2) (void)hipStreamSynchronize(c->st2);
   for (auto &e : c->ev)
@@ -1338,6 +1347,27 @@ int fsg_magradtran(fsg_ctx *c, const double *bfield, int32_t nbfield, int32_t bf
   return 0;
 }
 
+int fsg_get_spectra(fsg_ctx *c, double *spl_out, double *spz_out, double *spc_out) {
+  g_err.clear();
+  if (!c) return fail("fsg_get_spectra: NULL context");
+  if (!c->have_rt) return fail("fsg_get_spectra: no formal solution computed since the last opacity change");
+  if (spl_out && c->magnetic) return fail("fsg_get_spectra: the last formal solution was polarised (read SpZ)");
+  if (spz_out && !c->magnetic) return fail("fsg_get_spectra: the last formal solution was not polarised");
+  CK(hipSetDevice(c->dev));
+  if (spl_out && copy_spec_out(c, c->spl, spl_out)) return 1;
+  if (spz_out) {
+    const size_t Nw = (size_t)c->S.nwl, M = (size_t)c->S.nmu;
+    std::vector<double> h(4 * M * Nw);
+    CK(hipMemcpyAsync(h.data(), c->spz, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    CK(hipStreamSynchronize(c->st));
+    for (size_t q = 0; q < 4; q++)
+      for (size_t m = 0; m < M; m++)
+        for (size_t iw = 0; iw < Nw; iw++) spz_out[(iw * 4 + q) * M + m] = h[(q * M + m) * Nw + iw];
+  }
+  if (spc_out && copy_spec_out(c, c->spc, spc_out)) return 1;
+  return 0;
+}
+
 int fsg_flux(fsg_ctx *c, double *sfl_out, double *sfc_out) {
   g_err.clear();
   if (!c) return fail("fsg_flux: NULL context");
@@ -1347,6 +1377,128 @@ int fsg_flux(fsg_ctx *c, double *sfl_out, double *sfc_out) {
   if (sfl_out) CK(hipMemcpyAsync(sfl_out, c->sfl, Nw * sizeof(double), hipMemcpyDeviceToHost, c->st));
   if (sfc_out) CK(hipMemcpyAsync(sfc_out, c->sfc, Nw * sizeof(double), hipMemcpyDeviceToHost, c->st));
   CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int fsg_integ_convol(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, double vgau) {
+  g_err.clear();
+  if (!c) return fail("fsg_integ_convol: NULL context");
+  if (!c->have_rt) return fail("fsg_integ_convol: no formal solution computed since the last opacity change");
+  if (vsini < 0.0 || vmacro < 0.0 || vgau < 0.0) return fail("Fastspec: invalid value(s) of convolution parameters");
+  if (cimode != 1 && cimode != 2) return fail("fsg_integ_convol: CImode must be 1 or 2");
+  CK(hipSetDevice(c->dev));
+  const DevSetup &S = c->S;
+  const int M = S.nmu, Nw = S.nwl;
+  hipStream_t st = c->st;
+  std::vector<double> kern{1.0};  // tap 0: the identity (a copy)
+  std::vector<ConvRow> rows;
+  struct Launch {
+    size_t first;
+    int n, maxk;
+  };
+  std::vector<Launch> launches;
+  auto add_launch = [&](size_t first) {
+    int maxk = 1;
+    for (size_t i = first; i < rows.size(); i++) maxk = std::max(maxk, rows[i].nk);
+    launches.push_back(Launch{first, (int)(rows.size() - first), maxk});
+  };
+  std::vector<double> gk;
+  int gh = 0, goff = 0;
+  if (vgau > 0.0) {
+    convol_gauss_kernel(Nw, vgau, S.wmode, S.wfirst, c->wlast, S.dstep, gk, gh);
+    goff = (int)kern.size();
+    kern.insert(kern.end(), gk.begin(), gk.end());
+  }
+  const int ng = (int)gk.size();
+  // rows of the spectra: Stokes I..V blocks after a polarised solution, else SpL; then SpC
+  const int nlrow = c->magnetic ? 4 * M : M;
+  double *lbase = c->magnetic ? c->spz : c->spl;
+  if (cimode == 1) {
+    // IntegConvol(1): every intensity spectrum broadened into cv_a, then copied back
+    if (vgau > 0.0) {
+      CK(c->cv_a.ensure(sizeof(double) * (size_t)(nlrow + M) * Nw));
+      double *a = c->cv_a.as<double>();
+      for (int r = 0; r < nlrow; r++)
+        rows.push_back(ConvRow{lbase + (size_t)r * Nw, a + (size_t)r * Nw, goff, ng, gh});
+      for (int m = 0; m < M; m++)
+        rows.push_back(ConvRow{c->spc + (size_t)m * Nw, a + (size_t)(nlrow + m) * Nw, goff, ng, gh});
+      add_launch(0);
+    }
+  } else {
+    // DiskInteg: rotation, macroturbulence, the weighted sum over mu, Gaussian broadening
+    ConvolSetup cs{Nw, M, S.wmode, S.wfirst, c->wlast, S.dstep, vsini, vmacro, c->hxmu.data(), c->hwmu.data()};
+    std::vector<double> dk;
+    std::vector<int> roff, rn, moff;
+    int mn = 0;
+    diskinteg_kernels(cs, dk, roff, rn, moff, mn);
+    const int base = (int)kern.size();
+    kern.insert(kern.end(), dk.begin(), dk.end());
+    CK(c->cv_a.ensure(sizeof(double) * (size_t)2 * M * Nw));
+    CK(c->cv_b.ensure(sizeof(double) * (size_t)2 * M * Nw));
+    double *a = c->cv_a.as<double>(), *b = c->cv_b.as<double>();
+    // annulus spectra in a: [2][M][Nw] (line, continuum), rotated or copied
+    for (int t = 0; t < 2; t++)
+      for (int m = 0; m < M; m++) {
+        const double *in = t == 0 ? lbase + (size_t)m * Nw : c->spc + (size_t)m * Nw;
+        double *out = a + ((size_t)t * M + m) * Nw;
+        if (roff[m] >= 0) rows.push_back(ConvRow{in, out, base + roff[m], rn[m], (int)((rn[m] - 1.0) / 2.0)});
+        else rows.push_back(ConvRow{in, out, 0, 1, 0});
+      }
+    add_launch(0);
+    double *fin = a;
+    if (vmacro > 0.0) {
+      const size_t first = rows.size();
+      for (int t = 0; t < 2; t++)
+        for (int m = 0; m < M; m++)
+          rows.push_back(ConvRow{a + ((size_t)t * M + m) * Nw, b + ((size_t)t * M + m) * Nw, base + moff[m], mn,
+                                 (int)((mn - 1.0) / 2.0)});
+      add_launch(first);
+      fin = b;
+    }
+    if (vgau > 0.0) {  // the flux sums land in the free buffer, broadened into SfL, SfC
+      double *fr = fin == a ? b : a;
+      const size_t first = rows.size();
+      rows.push_back(ConvRow{fr, c->sfl, goff, ng, gh});
+      rows.push_back(ConvRow{fr + Nw, c->sfc, goff, ng, gh});
+      add_launch(first);
+    }
+  }
+  CK(c->cv_kern.ensure(sizeof(double) * kern.size()));
+  CK(c->cv_rows.ensure(sizeof(ConvRow) * std::max<size_t>(rows.size(), 1)));
+  CK(c->cv_w.ensure(sizeof(double) * M));
+  CK(hipMemcpyAsync(c->cv_kern.p, kern.data(), sizeof(double) * kern.size(), hipMemcpyHostToDevice, st));
+  if (!rows.empty())
+    CK(hipMemcpyAsync(c->cv_rows.p, rows.data(), sizeof(ConvRow) * rows.size(), hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(c->cv_w.p, c->hwmu.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
+  const ConvRow *drows = c->cv_rows.as<ConvRow>();
+  const double *dkern = c->cv_kern.as<double>();
+  if (cimode == 1) {
+    if (!launches.empty()) {
+      launch_conv_rows(Nw, launches[0].n, launches[0].maxk, drows, dkern, st);
+      CK(hipMemcpyAsync(lbase, c->cv_a.p, sizeof(double) * (size_t)nlrow * Nw, hipMemcpyDeviceToDevice, st));
+      CK(hipMemcpyAsync(c->spc, c->cv_a.as<double>() + (size_t)nlrow * Nw, sizeof(double) * (size_t)M * Nw,
+                        hipMemcpyDeviceToDevice, st));
+    }
+  } else {
+    double *a = c->cv_a.as<double>(), *b = c->cv_b.as<double>();
+    launch_conv_rows(Nw, launches[0].n, launches[0].maxk, drows, dkern, st);
+    double *fin = a;
+    size_t li = 1;
+    if (vmacro > 0.0) {
+      launch_conv_rows(Nw, launches[li].n, launches[li].maxk, drows + launches[li].first, dkern, st);
+      li++;
+      fin = b;
+    }
+    if (vgau > 0.0) {
+      double *fr = fin == a ? b : a;
+      launch_mu_sum(Nw, M, fin, fin + (size_t)M * Nw, c->cv_w.as<double>(), fr, fr + Nw, st);
+      launch_conv_rows(Nw, launches[li].n, launches[li].maxk, drows + launches[li].first, dkern, st);
+    } else {
+      launch_mu_sum(Nw, M, fin, fin + (size_t)M * Nw, c->cv_w.as<double>(), c->sfl, c->sfc, st);
+    }
+  }
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(st));  // the host tables above stay alive until here
   return 0;
 }
 

@@ -3,6 +3,8 @@
 
 #include <type_traits>
 
+#include <vector>
+
 #include "fsg_common.h"
 
 namespace fsg {
@@ -76,5 +78,24 @@ void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, co
                        hipStream_t st);
 void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *slot_s, const double *sl,
                                double *g_loggf, double *g_fpf, hipStream_t st);
+
+// convolution and disk integration (fsg_convol.hip)
+struct ConvolSetup {
+  int nwl, nmu, wmode;
+  double wfirst, wlast, dstep, vsini, vmacro;
+  const double *xmu, *wmu;  // host arrays [nmu]
+};
+struct ConvRow {  // one convolved spectrum: out[i] = sum_j in[clamp(i + j - h)] kern[koff + j]
+  const double *in;
+  double *out;
+  int koff, nk, h;
+};
+void convol_gauss_kernel(int nw, double fwhm, int wmode, double wfirst, double wlast, double dstep,
+                         std::vector<double> &kern, int &h);
+void diskinteg_kernels(const ConvolSetup &cs, std::vector<double> &kern, std::vector<int> &rot_off,
+                       std::vector<int> &rot_n, std::vector<int> &mac_off, int &mac_n);
+void launch_conv_rows(int nwl, int nrows, int maxk, const ConvRow *d_rows, const double *d_kern, hipStream_t st);
+void launch_mu_sum(int nwl, int nmu, const double *rl, const double *rc, const double *w, double *sfl, double *sfc,
+                   hipStream_t st);
 
 }  // namespace fsg

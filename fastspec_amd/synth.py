@@ -192,6 +192,7 @@ class FastSpecGPU:
         spl = np.zeros((s.nwl, s.nmu)) if intensities else None
         spc = np.zeros((s.nwl, s.nmu)) if intensities else None
         _lib.check(self._lib.fsg_radtran(self.ctx, ptr(spl, C.c_double), ptr(spc, C.c_double)))
+        self._polarised = False
         return spl, spc
 
     radtran_calc = RadTranCalc
@@ -204,6 +205,7 @@ class FastSpecGPU:
         spc = np.zeros((s.nwl, s.nmu)) if intensities else None
         _lib.check(self._lib.fsg_magradtran(self.ctx, ptr(b, C.c_double), b.shape[0], int(bframe),
                                             ptr(spz, C.c_double), ptr(spc, C.c_double)))
+        self._polarised = True
         return spz, spc
 
     mag_radtran_calc = MagRadTranCalc
@@ -215,11 +217,32 @@ class FastSpecGPU:
         _lib.check(self._lib.fsg_flux(self.ctx, ptr(sfl, C.c_double), ptr(sfc, C.c_double)))
         return sfl, sfc
 
-    def IntegConvol(self, CImode: int = 2):
-        """CImode 2 with no rotation, macroturbulence or instrumental profile: the flux sums."""
-        if CImode != 2:
-            raise _lib.FsgError("IntegConvol: only the disk integration (CImode 2) is on the device")
-        return self.flux()
+    def IntegConvol(self, CImode: int = 2, Vsini: float = 0.0, Vmacro: float = 0.0, Vgau: float = 0.0):
+        """IntegConvol (src/convol.f90:27-64) on the device.  CImode 2: disk integration with
+        rotation, radial-tangential macroturbulence and Gaussian broadening, returns (SfL,
+        SfC); CImode 1: Gaussian broadening of the intensity spectra, returns them as
+        get_spectra() does.  Velocities in km/s; Vgau above 1000 is a resolving power."""
+        if Vgau > 1000.0:
+            Vgau = 299792.458 / Vgau
+        _lib.check(self._lib.fsg_integ_convol(self.ctx, CImode, Vsini, Vmacro, Vgau))
+        if CImode == 2:
+            return self.flux()
+        return self.get_spectra()
+
+    def get_spectra(self, magnetic: bool | None = None):
+        """Intensities on the device: (SpL, SpC) [nwl][nmu], or (SpZ [nwl][4][nmu], SpC) after
+        MagRadTranCalc (or when magnetic is given as True)."""
+        if magnetic is None:
+            magnetic = getattr(self, "_polarised", False)
+        s = self.setup
+        spc = np.zeros((s.nwl, s.nmu))
+        if magnetic:
+            spz = np.zeros((s.nwl, 4, s.nmu))
+            _lib.check(self._lib.fsg_get_spectra(self.ctx, None, ptr(spz, C.c_double), ptr(spc, C.c_double)))
+            return spz, spc
+        spl = np.zeros((s.nwl, s.nmu))
+        _lib.check(self._lib.fsg_get_spectra(self.ctx, ptr(spl, C.c_double), None, ptr(spc, C.c_double)))
+        return spl, spc
 
     def device_ptr(self, which: int) -> int:
         p = C.c_void_p()

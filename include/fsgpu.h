@@ -179,6 +179,25 @@ int fsg_magradtran(fsg_ctx *ctx, const double *bfield, int32_t nbfield, int32_t 
  * after fsg_magradtran (src/fastspec.f90:437-440).  sfl_out/sfc_out [nwl]. */
 int fsg_flux(fsg_ctx *ctx, double *sfl_out, double *sfc_out);
 
+/* Convolution and disk integration of the last formal solution: replaces IntegConvol
+ * (src/convol.f90:27-64) with DiskInteg (67-246) and ConvolGau (249-300).
+ *   cimode 1: Gaussian broadening (FWHM vgau, km/s) of every intensity spectrum on the
+ *             device: SpL and SpC, or the four Stokes spectra and SpC after fsg_magradtran;
+ *   cimode 2: disk integration of SpL (Stokes I after fsg_magradtran) and SpC with the
+ *             rotational kernel of each annulus (vsini), radial-tangential macroturbulence
+ *             (vmacro) and Gaussian broadening of the flux (vgau); SfL and SfC on the device
+ *             are replaced (read them with fsg_flux).
+ * The annuli come from the context's mu and flux weights (the host's Xmu and Wmu of
+ * MuQuadrature for its DiskQuad); vgau is the host's Vgau after its resolving-power
+ * conversion (src/fastspec.f90:103). */
+int fsg_integ_convol(fsg_ctx *ctx, int32_t cimode, double vsini, double vmacro, double vgau);
+
+/* The intensity spectra of the last formal solution as they stand on the device (after
+ * fsg_integ_convol with cimode 1, the broadened ones), in the host's layouts for WriteOut
+ * (src/ioutils.f90:1751-1875): spl_out [nwl][nmu] SpL, or spz_out [nwl][4][nmu] SpZ after
+ * fsg_magradtran; spc_out [nwl][nmu] SpC.  NULL pointers are skipped. */
+int fsg_get_spectra(fsg_ctx *ctx, double *spl_out, double *spz_out, double *spc_out);
+
 /* Device-resident variants for callers that keep data in HBM (the PyTorch wrapper and
  * the benchmark).  A device line list is uploaded once and reused. */
 typedef struct fsg_dlines fsg_dlines;

@@ -406,11 +406,16 @@ EXPORT void fso_gauss_mu(int quad, int N, double *X, double *W) {
   }
 }
 
-/* MuQuadrature flux weights (src/utils.f90:1273-1333), nodes optionally set */
-EXPORT void fso_mu_quadrature(int quad, int N, int set_nodes, double *X, double *W) {
+/* MuQuadrature (src/utils.f90:1273-1333): flux weights W, nodes X optionally set, and the
+ * projected radii R[N+1] of the annulus boundaries (R may be NULL) */
+EXPORT void fso_mu_quadrature_r(int quad, int N, int set_nodes, double *X, double *W, double *Rout) {
   if (N == 1) {
     if (set_nodes) X[0] = 1.0;
     W[0] = 1.0;
+    if (Rout) {
+      Rout[0] = 0.0;
+      Rout[1] = 1.0;
+    }
     return;
   }
   if (quad == 0) {
@@ -422,6 +427,7 @@ EXPORT void fso_mu_quadrature(int quad, int N, int set_nodes, double *X, double 
     for (int i = 2; i <= N; i++) R[i - 1] = sqrt(0.5 * (Rmu[i - 2] * Rmu[i - 2] + Rmu[i - 1] * Rmu[i - 1]));
     R[N] = 1.0;
     for (int i = 1; i <= N; i++) W[i - 1] = R[i] * R[i] - R[i - 1] * R[i - 1];
+    if (Rout) memcpy(Rout, R, sizeof(double) * (N + 1));
     free(Rmu);
     free(R);
     return;
@@ -432,8 +438,17 @@ EXPORT void fso_mu_quadrature(int quad, int N, int set_nodes, double *X, double 
     if (set_nodes) X[i] = Xg[i];
     W[i] = 2.0 * Wg[i] * Xg[i];
   }
+  if (Rout) {  /* each annulus encloses the projected area of its weight */
+    Rout[0] = 0.0;
+    for (int i = 1; i <= N - 1; i++) Rout[i] = sqrt(Rout[i - 1] * Rout[i - 1] + W[i - 1]);
+    Rout[N] = 1.0;
+  }
   free(Xg);
   free(Wg);
+}
+
+EXPORT void fso_mu_quadrature(int quad, int N, int set_nodes, double *X, double *W) {
+  fso_mu_quadrature_r(quad, N, set_nodes, X, W, NULL);
 }
 
 /* MergeShifts (src/utils.f90:976-1004) */
@@ -2037,5 +2052,210 @@ EXPORT int fso_flux(fso_ctx *c, double *sfl, double *sfc) {
       sfl[i] = sfl[i] + IL * c->wmu[im];
       sfc[i] = sfc[i] + c->spc[(size_t)i * M + im] * c->wmu[im];
     }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------
+ * Convolution and disk integration: IntegConvol (src/convol.f90:27-64), DiskInteg (67-246),
+ * ConvolGau (249-300), SemiArea (src/utils.f90:1435-1445)
+ * ------------------------------------------------------------------------------------ */
+EXPORT double fso_semi_area(double R, double V) {
+  if (R <= 0.0) return 0.0;
+  double X = fmax(-R, fmin(R, V));
+  return 0.5 * (X * sqrt(fmax(0.0, (R - X) * (R + X))) + R * R * asin(X / R));
+}
+
+/* out[i] = sum_{j=1..nk} in[clamp(i + j - 1 - h)] kern[j], the padded sums of DiskInteg and
+ * ConvolGau (edge values repeated), j ascending; 0-based i, kern[0..nk-1] */
+static void conv_clamped(int nw, const double *in, int nk, const double *kern, int h, double *out) {
+  for (int i = 0; i < nw; i++) {
+    double sm = 0.0;
+    for (int j = 0; j < nk; j++) {
+      int k = i + j - h;
+      k = k < 0 ? 0 : (k > nw - 1 ? nw - 1 : k);
+      sm = sm + in[k] * kern[j];
+    }
+    out[i] = sm;
+  }
+}
+
+/* strided spectrum <-> contiguous copy */
+static void gather_s(int nw, const double *base, size_t stride, double *out) {
+  for (int i = 0; i < nw; i++) out[i] = base[(size_t)i * stride];
+}
+static void scatter_s(int nw, const double *in, double *base, size_t stride) {
+  for (int i = 0; i < nw; i++) base[(size_t)i * stride] = in[i];
+}
+
+/* ConvolGau on a contiguous spectrum S[nw] (FWHM in km/s) */
+EXPORT void fso_convolgau(int nw, double *S, double fwhm, int wmode, double wfirst, double wlast, double dstep) {
+  double sigma = wmode == 0 ? fwhm * (wfirst + wlast) / kVc / 2.0 / dstep : fwhm / dstep;
+  sigma = sigma / (2.0 * sqrt(2.0 * log(2.0)));
+  int j = (int)(10.0 * sigma), jm = (int)((nw - 3.0) / 2.0);
+  j = j < jm ? j : jm;
+  j = j > 3 ? j : 3;
+  const int ng = 2 * j + 1;
+  double *kern = malloc(sizeof(double) * ng), norm = 0.0;
+  for (int i = 1; i <= ng; i++) {
+    double arg = fmax(-20.0, -0.5 * pow((double)(i - j - 1) / sigma, 2));
+    kern[i - 1] = exp(arg);
+    norm = norm + kern[i - 1];
+  }
+  for (int i = 0; i < ng; i++) kern[i] = kern[i] / norm;
+  const int k = (int)((ng - 1.0) / 2.0);
+  double *T = malloc(sizeof(double) * nw);
+  conv_clamped(nw, S, ng, kern, k, T);
+  memcpy(S, T, sizeof(double) * nw);
+  free(T);
+  free(kern);
+}
+
+static double wlast_of(const fso_ctx *c) { return c->wfirst + c->wave[c->nwl]; }
+
+/* DiskInteg: SfL, SfC [nwl] from SpL (Stokes I after a polarised solution) and SpC, with the
+ * quadrature quad's weights and annulus radii for the context's mu */
+static int disk_integ(fso_ctx *c, double vsini, double vmacro, double vgau, int quad, double *sfl, double *sfc) {
+  const int M = c->nmu, Nwl = c->nwl;
+  double *Wq = malloc(sizeof(double) * M), *MuRadii = malloc(sizeof(double) * (M + 1));
+  fso_mu_quadrature_r(quad, M, 0, c->xmu, Wq, MuRadii);
+  const double wlast = wlast_of(c);
+  const double Dv = c->wmode == 0 ? 2.0 * c->dstep / (c->wfirst + wlast) * kVc : c->dstep;
+  int Nrk = 2 * (int)(vsini * 1.0 / Dv) + 3;
+  const double Sigma = vmacro / Dv / sqrt(2.0);
+  int Nmk = (int)(10.0 * Sigma), nmm = (int)((Nwl - 3) / 2);
+  Nmk = Nmk < nmm ? Nmk : nmm;
+  Nmk = Nmk > 3 ? Nmk : 3;
+  const int Nm = 2 * Nmk + 1;
+  double *Rkern = malloc(sizeof(double) * (Nrk > 0 ? Nrk : 1));
+  double *Mkern = malloc(sizeof(double) * Nm), *Mr = malloc(sizeof(double) * Nm), *Mt = malloc(sizeof(double) * Nm);
+  double *SiL = malloc(sizeof(double) * Nwl), *SiC = malloc(sizeof(double) * Nwl);
+  double *TL = malloc(sizeof(double) * Nwl), *TC = malloc(sizeof(double) * Nwl);
+  for (int i = 0; i < Nwl; i++) sfl[i] = sfc[i] = 0.0;
+  for (int imu = 1; imu <= M; imu++) {
+    if (c->magnetic) gather_s(Nwl, c->spz + (imu - 1), (size_t)4 * M, SiL);
+    else gather_s(Nwl, c->spl + (imu - 1), (size_t)M, SiL);
+    gather_s(Nwl, c->spc + (imu - 1), (size_t)M, SiC);
+    if (vsini > 0.0) {
+      const double R1 = vsini * MuRadii[imu - 1], R2 = vsini * MuRadii[imu];
+      const double Maxv = R2;
+      Nrk = 2 * (int)(Maxv / Dv) + 3;
+      double norm = 0.0;
+      for (int i = 1; i <= Nrk; i++) {
+        const double V = fabs(Dv * (double)(i - 1 - (Nrk - 1) / 2));
+        Rkern[i - 1] = fso_semi_area(R2, V + 0.5 * Dv) - fso_semi_area(R2, V - 0.5 * Dv) -
+                       fso_semi_area(R1, V + 0.5 * Dv) + fso_semi_area(R1, V - 0.5 * Dv);
+        norm = norm + Rkern[i - 1];
+      }
+      for (int i = 0; i < Nrk; i++) Rkern[i] = Rkern[i] / norm;
+      if (R2 > 0.5 * Dv) {
+        const int h = (int)((Nrk - 1.0) / 2.0);
+        conv_clamped(Nwl, SiL, Nrk, Rkern, h, TL);
+        conv_clamped(Nwl, SiC, Nrk, Rkern, h, TC);
+        memcpy(SiL, TL, sizeof(double) * Nwl);
+        memcpy(SiC, TC, sizeof(double) * Nwl);
+      }
+    }
+    if (vmacro > 0.0) {
+      const double Sigr = Sigma * c->xmu[imu - 1], Sigt = Sigma * sqrt(1.0 - c->xmu[imu - 1] * c->xmu[imu - 1]);
+      double *kk[2] = {Mr, Mt};
+      const double sg[2] = {Sigr, Sigt};
+      for (int t = 0; t < 2; t++) {
+        if (sg[t] > 0.0) {
+          double norm = 0.0;
+          for (int i = 1; i <= Nm; i++) {
+            const double arg = fmax(-20.0, -0.5 * pow((double)(i - Nmk - 1) / sg[t], 2));
+            kk[t][i - 1] = exp(arg);
+            norm = norm + kk[t][i - 1];
+          }
+          for (int i = 0; i < Nm; i++) kk[t][i] = kk[t][i] / norm;
+        } else {
+          for (int i = 0; i < Nm; i++) kk[t][i] = 0.0;
+          kk[t][Nmk] = 1.0;
+        }
+      }
+      for (int i = 0; i < Nm; i++) Mkern[i] = 0.5 * Mr[i] + 0.5 * Mt[i];
+      const int h = (int)((Nm - 1.0) / 2.0);
+      conv_clamped(Nwl, SiL, Nm, Mkern, h, TL);
+      conv_clamped(Nwl, SiC, Nm, Mkern, h, TC);
+      memcpy(SiL, TL, sizeof(double) * Nwl);
+      memcpy(SiC, TC, sizeof(double) * Nwl);
+    }
+    for (int i = 0; i < Nwl; i++) {
+      sfl[i] = sfl[i] + SiL[i] * Wq[imu - 1];
+      sfc[i] = sfc[i] + SiC[i] * Wq[imu - 1];
+    }
+  }
+  if (vgau > 0.0) {
+    fso_convolgau(Nwl, sfl, vgau, c->wmode, c->wfirst, wlast, c->dstep);
+    fso_convolgau(Nwl, sfc, vgau, c->wmode, c->wfirst, wlast, c->dstep);
+  }
+  free(Wq);
+  free(MuRadii);
+  free(Rkern);
+  free(Mkern);
+  free(Mr);
+  free(Mt);
+  free(SiL);
+  free(SiC);
+  free(TL);
+  free(TC);
+  return 0;
+}
+
+/* IntegConvol: cimode 1 Gaussian broadening of the intensity spectra in place (SpL and SpC,
+ * or the four Stokes spectra and SpC); cimode 2 disk integration into sfl, sfc */
+EXPORT int fso_integ_convol(fso_ctx *c, int cimode, double vsini, double vmacro, double vgau, int quad,
+                            double *sfl, double *sfc) {
+  const int M = c->nmu, Nwl = c->nwl;
+  if (cimode == 2) return disk_integ(c, vsini, vmacro, vgau, quad, sfl, sfc);
+  if (cimode != 1) {
+    snprintf(g_err, sizeof g_err, "IntegConvol: unknown mode %d", cimode);
+    return 1;
+  }
+  const double wlast = wlast_of(c);
+  double *T = malloc(sizeof(double) * Nwl);
+  for (int imu = 0; imu < M; imu++) {
+    if (c->magnetic) {
+      for (int st = 0; st < 4; st++) {
+        gather_s(Nwl, c->spz + (size_t)st * M + imu, (size_t)4 * M, T);
+        fso_convolgau(Nwl, T, vgau, c->wmode, c->wfirst, wlast, c->dstep);
+        scatter_s(Nwl, T, c->spz + (size_t)st * M + imu, (size_t)4 * M);
+      }
+    } else {
+      gather_s(Nwl, c->spl + imu, (size_t)M, T);
+      fso_convolgau(Nwl, T, vgau, c->wmode, c->wfirst, wlast, c->dstep);
+      scatter_s(Nwl, T, c->spl + imu, (size_t)M);
+    }
+    gather_s(Nwl, c->spc + imu, (size_t)M, T);
+    fso_convolgau(Nwl, T, vgau, c->wmode, c->wfirst, wlast, c->dstep);
+    scatter_s(Nwl, T, c->spc + imu, (size_t)M);
+  }
+  free(T);
+  return 0;
+}
+
+/* replaces the held intensity spectra (spz non-NULL: a polarised solution) */
+EXPORT int fso_set_spectra(fso_ctx *c, const double *spl, const double *spc, const double *spz) {
+  const size_t M = c->nmu, Nwl = c->nwl;
+  if (spz) {
+    if (!c->spz) c->spz = malloc(sizeof(double) * Nwl * 4 * M);
+    memcpy(c->spz, spz, sizeof(double) * Nwl * 4 * M);
+    c->magnetic = 1;
+  } else {
+    if (!c->spl) c->spl = malloc(sizeof(double) * Nwl * M);
+    memcpy(c->spl, spl, sizeof(double) * Nwl * M);
+    c->magnetic = 0;
+  }
+  if (!c->spc) c->spc = malloc(sizeof(double) * Nwl * M);
+  memcpy(c->spc, spc, sizeof(double) * Nwl * M);
+  return 0;
+}
+
+/* the intensity spectra as held by the oracle: spl/spc [nwl][nmu], spz [nwl][4][nmu] */
+EXPORT int fso_get_spectra(fso_ctx *c, double *spl, double *spc, double *spz) {
+  const size_t M = c->nmu, Nwl = c->nwl;
+  if (spl && c->spl) memcpy(spl, c->spl, sizeof(double) * Nwl * M);
+  if (spc && c->spc) memcpy(spc, c->spc, sizeof(double) * Nwl * M);
+  if (spz && c->spz) memcpy(spz, c->spz, sizeof(double) * Nwl * 4 * M);
   return 0;
 }

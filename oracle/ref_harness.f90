@@ -24,6 +24,7 @@ MODULE RefHarness
   USE HeStark
   USE Opacity
   USE Solver
+  USE Convol
 
   IMPLICIT NONE
 
@@ -508,6 +509,64 @@ CONTAINS
     REAL(8) :: r(n + 1)
     CALL MuQuadrature(quad, n, x, w, r, .TRUE.)
   END SUBROUTINE ref_mu_quadrature
+
+  ! DiskInteg (src/convol.f90:67-246) on intensity spectra spl, spc (Nmu, Nwl): sets the
+  ! module variables it reads and returns SfL, SfC
+  SUBROUTINE ref_diskinteg(nmu_, nwl_, wmode_, wfirst_, wlast_, dstep_, quad, xmu_, vsini_, vmacro_, vgau_, &
+                           spl_, spc_, sfl_, sfc_) BIND(C, NAME='ref_diskinteg')
+    INTEGER(C_INT), VALUE :: nmu_, nwl_, wmode_, quad
+    REAL(C_DOUBLE), VALUE :: wfirst_, wlast_, dstep_, vsini_, vmacro_, vgau_
+    REAL(C_DOUBLE), INTENT(IN) :: xmu_(nmu_), spl_(nmu_, nwl_), spc_(nmu_, nwl_)
+    REAL(C_DOUBLE), INTENT(OUT) :: sfl_(nwl_), sfc_(nwl_)
+    Nmu = nmu_
+    Nwl = nwl_
+    Wmode = wmode_
+    Wfirst = wfirst_
+    Wlast = wlast_
+    Dstep = dstep_
+    DiskQuad = quad
+    Vsini = vsini_
+    Vmacro = vmacro_
+    Vgau = vgau_
+    IF (ALLOCATED(Xmu)) DEALLOCATE(Xmu)
+    IF (ALLOCATED(Wmu)) DEALLOCATE(Wmu)
+    ALLOCATE(Xmu(Nmu), Wmu(Nmu))
+    Xmu = xmu_
+    IF (ALLOCATED(SpL)) DEALLOCATE(SpL)
+    IF (ALLOCATED(SpC)) DEALLOCATE(SpC)
+    IF (ALLOCATED(SfL)) DEALLOCATE(SfL)
+    IF (ALLOCATED(SfC)) DEALLOCATE(SfC)
+    ALLOCATE(SpL(Nmu, Nwl), SpC(Nmu, Nwl))
+    SpL = spl_
+    SpC = spc_
+    CALL DiskInteg
+    sfl_ = SfL
+    sfc_ = SfC
+  END SUBROUTINE ref_diskinteg
+
+  ! ConvolGau (src/convol.f90:249-300) on one spectrum
+  SUBROUTINE ref_convolgau(nw, s, fwhm, wmode_, wfirst_, wlast_, dstep_) BIND(C, NAME='ref_convolgau')
+    INTEGER(C_INT), VALUE :: nw, wmode_
+    REAL(C_DOUBLE), VALUE :: fwhm, wfirst_, wlast_, dstep_
+    REAL(C_DOUBLE), INTENT(INOUT) :: s(nw)
+    Wmode = wmode_
+    Wfirst = wfirst_
+    Wlast = wlast_
+    Dstep = dstep_
+    CALL ConvolGau(nw, s, fwhm)
+  END SUBROUTINE ref_convolgau
+
+  REAL(C_DOUBLE) FUNCTION ref_semi_area(r, v) BIND(C, NAME='ref_semi_area')
+    REAL(C_DOUBLE), VALUE :: r, v
+    ref_semi_area = SemiArea(r, v)
+  END FUNCTION ref_semi_area
+
+  SUBROUTINE ref_mu_quadrature_r(quad, n, x, w, r) BIND(C, NAME='ref_mu_quadrature_r')
+    INTEGER(C_INT), VALUE :: quad, n
+    REAL(C_DOUBLE), INTENT(INOUT) :: x(n)
+    REAL(C_DOUBLE), INTENT(OUT) :: w(n), r(n + 1)
+    CALL MuQuadrature(quad, n, x, w, r, .FALSE.)
+  END SUBROUTINE ref_mu_quadrature_r
 
   INTEGER(C_INT) FUNCTION ref_nthreads() BIND(C, NAME='ref_nthreads')
     INTEGER :: n

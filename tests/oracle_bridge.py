@@ -70,6 +70,13 @@ def _load_oracle() -> C.CDLL:
     lib.fso_locprfsph.argtypes = [vp, _dp, _dp, _dp]
     lib.fso_sph_field_geometry.argtypes = [vp, _dp, C.c_int, C.c_int, _dp]
     lib.fso_stokes_prf_sph.argtypes = [vp, C.c_int, _dp, _dp, _dp]
+    lib.fso_mu_quadrature_r.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+    lib.fso_semi_area.argtypes = [C.c_double, C.c_double]
+    lib.fso_semi_area.restype = C.c_double
+    lib.fso_convolgau.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double]
+    lib.fso_integ_convol.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]
+    lib.fso_get_spectra.argtypes = [vp, _dp, _dp, _dp]
+    lib.fso_set_spectra.argtypes = [vp, _dp, _dp, _dp]
     lib.fso_last_error.restype = C.c_char_p
     return lib
 
@@ -193,6 +200,38 @@ class Oracle:
         self.lib.fso_flux(self.ctx, ptr(sfl, C.c_double), ptr(sfc, C.c_double))
         return sfl, sfc
 
+    def integ_convol(self, cimode: int, vsini: float, vmacro: float, vgau: float, quad: int):
+        """IntegConvol: mode 2 returns (SfL, SfC) of DiskInteg; mode 1 broadens the held
+        intensity spectra in place (read them with spectra())."""
+        n = self.setup.nwl
+        sfl, sfc = np.zeros(n), np.zeros(n)
+        rc = self.lib.fso_integ_convol(self.ctx, cimode, vsini, vmacro, vgau, quad, ptr(sfl, C.c_double),
+                                       ptr(sfc, C.c_double))
+        if rc != 0:
+            raise RuntimeError(self.lib.fso_last_error().decode())
+        return sfl, sfc
+
+    def set_spectra(self, spl, spc, spz=None):
+        """Replaces the held intensities: spl or spz (a polarised solution), and spc."""
+        c = np.ascontiguousarray(spc, np.float64)
+        if spz is not None:
+            z = np.ascontiguousarray(spz, np.float64)
+            self.lib.fso_set_spectra(self.ctx, None, ptr(c, C.c_double), ptr(z, C.c_double))
+        else:
+            l = np.ascontiguousarray(spl, np.float64)
+            self.lib.fso_set_spectra(self.ctx, ptr(l, C.c_double), ptr(c, C.c_double), None)
+
+    def spectra(self, magnetic: bool = False):
+        s = self.setup
+        spc = np.zeros((s.nwl, s.nmu))
+        if magnetic:
+            spz = np.zeros((s.nwl, 4, s.nmu))
+            self.lib.fso_get_spectra(self.ctx, None, ptr(spc, C.c_double), ptr(spz, C.c_double))
+            return spz, spc
+        spl = np.zeros((s.nwl, s.nmu))
+        self.lib.fso_get_spectra(self.ctx, ptr(spl, C.c_double), ptr(spc, C.c_double), None)
+        return spl, spc
+
     def sphere_geometry(self) -> dict:
         """SphereInit's rays: ntau_mu, graz [nmu]; map (1-based layers), rhox_mu, muloc
         [nmu][2 ntau]."""
@@ -261,6 +300,12 @@ def ref_lib() -> C.CDLL:
         lib.ref_air2vac.restype = C.c_double
         lib.ref_air2vac.argtypes = [C.c_double]
         lib.ref_nthreads.restype = C.c_int32
+        lib.ref_semi_area.restype = C.c_double
+        lib.ref_semi_area.argtypes = [C.c_double, C.c_double]
+        lib.ref_convolgau.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double]
+        lib.ref_diskinteg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                                      C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
+        lib.ref_mu_quadrature_r.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
         _REF = lib
     return _REF
 
@@ -425,3 +470,27 @@ class Ref:
         arrs = [np.ascontiguousarray(x, np.float64) for x in (mlop, lop, src, *geo)]
         self.lib.ref_stokesprfpp(C.c_int(iwl), *(ptr(x, C.c_double) for x in arrs), ptr(out, C.c_double))
         return out
+
+
+def ref_diskinteg(setup: Setup, quad: int, vsini: float, vmacro: float, vgau: float, spl, spc):
+    """The reference's DiskInteg on intensities spl, spc [nwl][nmu]: (SfL, SfC)."""
+    lib = ref_lib()
+    wave = np.asarray(setup.wave, np.float64)
+    wlast = setup.wfirst + wave[-1]
+    a = np.ascontiguousarray(spl, np.float64)
+    b = np.ascontiguousarray(spc, np.float64)
+    xmu = np.ascontiguousarray(setup.xmu, np.float64)
+    sfl, sfc = np.zeros(setup.nwl), np.zeros(setup.nwl)
+    lib.ref_diskinteg(setup.nmu, setup.nwl, setup.wmode, setup.wfirst, wlast, setup.dstep, quad,
+                      ptr(xmu, C.c_double), vsini, vmacro, vgau, ptr(a, C.c_double), ptr(b, C.c_double),
+                      ptr(sfl, C.c_double), ptr(sfc, C.c_double))
+    return sfl, sfc
+
+
+def ref_convolgau(setup: Setup, s, fwhm: float):
+    lib = ref_lib()
+    wave = np.asarray(setup.wave, np.float64)
+    out = np.array(s, np.float64, copy=True)
+    lib.ref_convolgau(out.shape[0], ptr(out, C.c_double), fwhm, setup.wmode, setup.wfirst, setup.wfirst + wave[-1],
+                      setup.dstep)
+    return out
