@@ -46,6 +46,10 @@ def measured_traffic():
     return None
 
 
+# Vsini, Vmacro (km/s) and the instrumental profile (R = 100,000) of the convolution leg
+CONVOL = (10.0, 3.0, 1e5)
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -70,10 +74,18 @@ def run_path(setup, lines, precision, batched, steps, warmup):
         p1ms.append(st["ms_pass1"])
     t1 = time.perf_counter()
     sfl, sfc = g.flux()
+    # IntegConvol(2) on the same intensities: rotation, macroturbulence and the instrumental
+    # profile; each call synchronises and copies SfL, SfC back, so wall-clock covers it whole
+    g.IntegConvol(2, *CONVOL)
+    reps = 5
+    t2 = time.perf_counter()
+    for _ in range(reps):
+        g.IntegConvol(2, *CONVOL)
+    conv_ms = (time.perf_counter() - t2) * 1e3 / reps
     dl.free()
     g.close()
     return dict(ms_per_step=(t1 - t0) * 1e3 / steps, stats=st, opacity_ms=float(np.mean(kms)),
-                pass1_ms=float(np.mean(p1ms)), fnorm=sfl / sfc)
+                pass1_ms=float(np.mean(p1ms)), fnorm=sfl / sfc, conv_ms=conv_ms)
 
 
 def run_c5(steps: int, warmup: int):
@@ -219,6 +231,8 @@ def main():
                    "nlines": lines.n, "nwl": setup.nwl, "ntau": setup.ntau, "nmu": setup.nmu,
                    "parallelism": "single GPU"},
         "spectra_per_s": 1e3 / r32["ms_per_step"],
+        "integ_convol": {"ms": r32["conv_ms"], "cimode": 2, "vsini": CONVOL[0], "vmacro": CONVOL[1],
+                         "resolution": CONVOL[2], "note": "fsg_integ_convol incl. host copy of SfL, SfC"},
         "evals_per_step": evals,
         "retained_lines": st["nlines_use"],
         # compute-bound: the far-wing series is a dense contraction over lines (FP32 MFMA,

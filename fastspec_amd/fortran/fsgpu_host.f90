@@ -11,6 +11,7 @@
 !   GpuRadTranCalc     replaces RadTranCalc                       src/synth.f90:1825
 !   GpuMagRadTranCalc  replaces MagRadTranCalc                    src/synth.f90:1904
 !   GpuFlux            the flux sums of DiskInteg (CImode 2 without convolution)
+!   GpuIntegConvol     replaces IntegConvol(CImode)               src/convol.f90:27
 !   GpuGetLineop/GpuSetLineop   LineOpac to/from the host (SAVE flag 2, OPACITY keyword)
 !
 ! The line-opacity drivers leave LinFlag, Wlcent, LinSpc and LinStr allocated when the
@@ -131,6 +132,16 @@ MODULE FsgpuHost
       TYPE(C_PTR), VALUE :: Ctx
       REAL(C_DOUBLE), INTENT(OUT) :: SfLOut(*), SfCOut(*)
     END FUNCTION fsg_flux
+    INTEGER(C_INT) FUNCTION fsg_integ_convol(Ctx, Cimode, Vsini_, Vmacro_, Vgau_) BIND(C, NAME='fsg_integ_convol')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE, C_INT32_T
+      TYPE(C_PTR), VALUE :: Ctx
+      INTEGER(C_INT32_T), VALUE :: Cimode
+      REAL(C_DOUBLE), VALUE :: Vsini_, Vmacro_, Vgau_
+    END FUNCTION fsg_integ_convol
+    INTEGER(C_INT) FUNCTION fsg_get_spectra(Ctx, SpLOut, SpZOut, SpCOut) BIND(C, NAME='fsg_get_spectra')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: Ctx, SpLOut, SpZOut, SpCOut
+    END FUNCTION fsg_get_spectra
     INTEGER(C_SIZE_T) FUNCTION c_strlen(S) BIND(C, NAME='strlen')
       IMPORT :: C_SIZE_T, C_PTR
       TYPE(C_PTR), VALUE :: S
@@ -470,6 +481,32 @@ CONTAINS
     ALLOCATE(SfL(Nwl), SfC(Nwl))
     CALL GpuCheck(fsg_flux(GpuCtx, SfL, SfC))
   END SUBROUTINE GpuFlux
+
+  ! Replaces IntegConvol(CImode) with the module's Vsini, Vmacro, Vgau: CImode 1 broadens the
+  ! intensity spectra (SpL, SpC or SpZ, SpC) with the Gaussian instrumental profile, CImode 2
+  ! disk-integrates them into SfL, SfC with rotation, macroturbulence and the instrumental
+  ! profile (Stokes I after the polarised solution, src/fastspec.f90:437-440)
+  SUBROUTINE GpuIntegConvol(CImode)
+    INTEGER, INTENT(IN) :: CImode
+    CALL GpuCheck(fsg_integ_convol(GpuCtx, INT(CImode, C_INT32_T), Vsini, Vmacro, Vgau))
+    IF (CImode == 2) THEN
+      CALL GpuFlux
+    ELSE IF (Magnetic) THEN
+      CALL GetSpectra(SpZ, SpC, .TRUE.)
+    ELSE
+      CALL GetSpectra(SpL, SpC, .FALSE.)
+    END IF
+  CONTAINS
+    SUBROUTINE GetSpectra(SpOut, SpcOut, Polarised)
+      REAL(8), INTENT(OUT), TARGET :: SpOut(*), SpcOut(*)
+      LOGICAL, INTENT(IN) :: Polarised
+      IF (Polarised) THEN
+        CALL GpuCheck(fsg_get_spectra(GpuCtx, C_NULL_PTR, C_LOC(SpOut), C_LOC(SpcOut)))
+      ELSE
+        CALL GpuCheck(fsg_get_spectra(GpuCtx, C_LOC(SpOut), C_NULL_PTR, C_LOC(SpcOut)))
+      END IF
+    END SUBROUTINE GetSpectra
+  END SUBROUTINE GpuIntegConvol
 
   ! LineOpac(Ntau, Nwl) from the device (SAVE flag 2) and to it (OPACITY keyword)
   SUBROUTINE GpuGetLineop

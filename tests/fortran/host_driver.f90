@@ -72,10 +72,24 @@ PROGRAM HostDriver
   OutFlags = [1, 0, 0, 1, 1]
   CALL HeStarkInit
 
+  ! optional arguments 3-5: Vsini, Vmacro, Vgau for a disk integration with convolution
+  IF (COMMAND_ARGUMENT_COUNT() >= 5) THEN
+    CALL GET_COMMAND_ARGUMENT(3, arg)
+    READ(arg, *) Vsini
+    CALL GET_COMMAND_ARGUMENT(4, arg)
+    READ(arg, *) Vmacro
+    CALL GET_COMMAND_ARGUMENT(5, arg)
+    READ(arg, *) Vgau
+  END IF
+
   CALL GpuInit(prec)
   CALL GpuLineopCalc(1, nl, DoLinStr)
   CALL GpuRadTranCalc
-  CALL GpuFlux
+  IF (Vsini > 0d0 .OR. Vmacro > 0d0 .OR. Vgau > 0d0) THEN
+    CALL GpuIntegConvol(2)
+  ELSE
+    CALL GpuFlux
+  END IF
   OPEN(11, FILE='out.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
   WRITE(11) Nwl, NlinesUse, MERGE(1, 0, DoLinStr)
   WRITE(11) SfL, SfC, SpL

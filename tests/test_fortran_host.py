@@ -56,3 +56,25 @@ def test_fortran_host_matches_oracle(tmp_path, prec):
     tol = 1e-8 if prec == 0 else 1e-4
     assert np.max(np.abs(out["sfl"] / out["sfc"] - sfl / sfc)) <= tol
     assert np.max(np.abs(out["spl"] - spl) / spl) <= (1e-9 if prec == 0 else 1e-6)
+
+
+@pytest.mark.gpu
+def test_fortran_host_integ_convol(tmp_path):
+    """GpuIntegConvol(2) with rotation, macroturbulence and the instrumental profile against
+    the oracle's IntegConvol(2) after its own FP64 synthesis."""
+    drv = _driver()
+    setup, lines = W.config_c1(300)
+    write_case(tmp_path / "case.bin", setup, lines)
+    r = subprocess.run([str(drv), str(tmp_path / "case.bin"), "0", "12.0", "3.0", "4.0"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "done" in r.stdout, r.stdout + r.stderr
+    out = read_out(tmp_path / "out.bin", setup.nmu)
+    orc = Oracle(setup)
+    orc.lineop(lines)
+    spl, _ = orc.radtran()
+    sfl, sfc = orc.integ_convol(2, 12.0, 3.0, 4.0, 1)
+    f0l, f0c = orc.flux()
+    assert np.max(np.abs(out["sfl"] / out["sfc"] - sfl / sfc)) <= 1e-8
+    assert np.max(np.abs(out["sfc"] - sfc) / sfc) <= 1e-9
+    assert np.max(np.abs(out["spl"] - spl) / spl) <= 1e-9  # CImode 2 leaves the intensities alone
+    assert np.max(np.abs(sfl / sfc - f0l / f0c)) > 1e-3  # and the lines did broaden
