@@ -72,7 +72,8 @@ struct fsg_ctx {
   double *spl = nullptr, *spc = nullptr, *sfl = nullptr, *sfc = nullptr;
   int have_rt = 0;
   DBuf flag, il, ir, ic, key, key2, idx, idx2, coef, wlc, meta, rng, rng2, tmp, pos, wlcl, rflag;
-  DBuf zkey, zkey2, zval, zperm;  // FP32 path: lines ordered by core-zone width
+  DBuf zkey, zkey2, zval, zperm, zrec;  // FP32 path: lines ordered by core-zone width
+  DBuf fcap, foff, ftmp, flist;          // FP32 path: per-tile line lists of the matrix-core kernel
   // convolution and disk integration: host copies of mu, weights, the grid's last point;
   // device scratch for the convolved rows, kernels and row tables
   std::vector<double> hxmu, hwmu;
@@ -429,7 +430,7 @@ int fsg_finalize(fsg_ctx *c) {
   for (void *p : c->owned) (void)hipFree(p);
   for (DBuf *b : {&c->flag, &c->il, &c->ir, &c->ic, &c->key, &c->key2, &c->idx, &c->idx2, &c->coef, &c->wlc,
                   &c->meta, &c->rng, &c->rng2, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp, &c->vscr,
-                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm, &c->cv_a,
+                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm, &c->zrec, &c->fcap, &c->foff, &c->ftmp, &c->flist, &c->cv_a,
                   &c->cv_b, &c->cv_kern, &c->cv_rows, &c->cv_w})
     b->release();
   if (c->st2) (void)hipStreamSynchronize(c->st2);
@@ -664,7 +665,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
     const size_t wsz =
         c->precision == FSG_FP32 ? (size_t)(S.ntp / DPW) * sizeof(float) + sizeof(int2) + sizeof(LineRec) : 0;
     const int64_t chunk = chunk_lines(c, nret, nlines_lim, S.ntp * csz + wsz + sizeof(double) + sizeof(int4), 0);
-    CK(c->coef.ensure((size_t)chunk * (S.ntp * csz + wsz) + 6 * 32));  // + alignment of five arrays
+    CK(c->coef.ensure((size_t)chunk * (S.ntp * csz + wsz) + 6 * 32 + sizeof(LineRec)));  // + alignment, empty record
     CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
     CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
     const int ntiles = (S.nwl + TILE - 1) / TILE;
@@ -676,6 +677,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       CK(c->zkey2.ensure((size_t)chunk * sizeof(unsigned long long)));
       CK(c->zval.ensure((size_t)chunk * sizeof(int)));
       CK(c->zperm.ensure((size_t)chunk * sizeof(int)));
+      CK(c->zrec.ensure((size_t)(chunk + 1) * sizeof(ZoneRec)));  // + the spare record the kernel reads ahead
       ztmp = zone_sort_tmp_bytes(chunk);
       CK(c->tmp.ensure(ztmp));
     }
@@ -706,7 +708,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       CoefB *cb = reinterpret_cast<CoefB *>(carve((size_t)m * S.ntp * sizeof(CoefB)));
       float *dmaxw = reinterpret_cast<float *>(carve((size_t)m * (S.ntp / DPW) * sizeof(float)));
       int2 *zone = reinterpret_cast<int2 *>(carve((size_t)m * sizeof(int2)));
-      LineRec *lrec = reinterpret_cast<LineRec *>(carve((size_t)m * sizeof(LineRec)));
+      LineRec *lrec = reinterpret_cast<LineRec *>(carve((size_t)(m + 1) * sizeof(LineRec)));  // + an empty one
       const int *order = c->idx2.as<int>() + s0;
       if (c->precision == FSG_FP32)
         launch_fill32(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
@@ -718,12 +720,29 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
                          c->meta.as<int4>(), c->rng.as<int2>(), s);
       if (c->precision == FSG_FP32)  // the chunk's lines by core-zone width, and per-tile candidate ranges
         launch_zone_order(m, std::min<int64_t>(std::max<int64_t>(goff[nbins] - s0, 0), m), ntiles, S.nwl, S.ntau,
-                          c->meta.as<int4>(), zone, c->zkey.as<unsigned long long>(), c->zval.as<int>(),
-                          c->zkey2.as<unsigned long long>(), c->zperm.as<int>(), c->tmp.p, ztmp, zoff, zend,
-                          c->rng2.as<int2>(), c->u64 + 2, s);
+                          c->meta.as<int4>(), zone, c->wlc.as<double>(), c->zkey.as<unsigned long long>(),
+                          c->zval.as<int>(), c->zkey2.as<unsigned long long>(), c->zperm.as<int>(),
+                          c->zrec.as<ZoneRec>(), c->tmp.p, ztmp, zoff, zend, c->rng2.as<int2>(), c->u64 + 2, s);
       // the range table has ng groups per tile: the first nbins are read by the main launch
       // (stride ng), the last one by the He launch (offset nbins, stride ng)
       const bool has_main = s0 < goff[nbins], has_he = s1 > goff[nbins];
+      if (c->precision == FSG_FP32 && has_main && nbins > 0) {
+        if ((S.ntau + 15) / 16 <= MFMA_MAX_NQ) {
+          // the matrix-core kernel's line lists; their total size comes back to the host
+          const size_t ftmp = far_scan_tmp_bytes(ntiles);
+          CK(c->fcap.ensure((size_t)(ntiles + 1) * sizeof(long long)));
+          CK(c->foff.ensure((size_t)(ntiles + 1) * sizeof(long long)));
+          CK(c->ftmp.ensure(ftmp));
+          launch_far_cap(ntiles, nbins, ng, c->rng.as<int2>(), c->fcap.as<long long>(), c->foff.as<long long>(),
+                         c->ftmp.p, ftmp, lrec, m, s);
+          long long ftotal = 0;
+          CK(hipMemcpyAsync(&ftotal, c->foff.as<long long>() + ntiles, sizeof(long long), hipMemcpyDeviceToHost, s));
+          CK(hipStreamSynchronize(s));
+          CK(c->flist.ensure((size_t)ftotal * sizeof(int)));
+          launch_far_lists(ntiles, S.nwl, nbins, ng, c->rng.as<int2>(), lrec, c->foff.as<long long>(),
+                           c->flist.as<int>(), s);
+        }
+      }
       if (!timed) CK(hipEventRecord(c->ev[2], s));
       timed = true;
       if (c->precision == FSG_FP32) {
@@ -733,13 +752,14 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
           CK(hipStreamWaitEvent(c->st2, c->evf, 0));
           if (!nfar) CK(hipEventRecord(c->evt[0], s));
           const int mf = launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, lrec,
-                                          c->wlc.as<double>(), c->meta.as<int4>(), (float *)c->lineop, 0, s);
+                                          c->wlc.as<double>(), c->meta.as<int4>(), c->foff.as<long long>(),
+                                          c->flist.as<int>(), m, (float *)c->lineop, 0, s);
           CK(hipEventRecord(c->evt[1], s));
           nfar++;
           if (mf) {
             if (!ncore) CK(hipEventRecord(c->evt[2], c->st2));
-            launch_opacity32_core(S, ntiles, c->rng2.as<int2>(), c->zperm.as<int>(), ca, cb, dmaxw, zone,
-                                  c->wlc.as<double>(), c->meta.as<int4>(), c->lop2, c->st2);
+            launch_opacity32_core(S, ntiles, c->rng2.as<int2>(), c->zrec.as<ZoneRec>(), ca, cb, dmaxw, c->lop2,
+                                  c->st2);
             CK(hipEventRecord(c->evt[3], c->st2));
             CK(hipEventRecord(c->evj, c->st2));
             ncore++;
@@ -749,7 +769,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
         }
         if (has_he)
           launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, dmaxw, lrec, c->wlc.as<double>(),
-                           c->meta.as<int4>(), (float *)c->lineop, 1, s);
+                           c->meta.as<int4>(), nullptr, nullptr, 0, (float *)c->lineop, 1, s);
       } else {
         if (has_main && nbins > 0)
           launch_opacity64(S, ntiles, nbins, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), c->wlc.as<double>(),

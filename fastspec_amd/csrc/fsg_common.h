@@ -62,6 +62,13 @@ struct alignas(16) LineRec {
   int pad0, pad1;
 };
 
+// per-line record of the core-zone kernel, in core-zone order: zone ∩ window [lo, hi], the
+// line's position j in the chunk and its centre wavelength
+struct alignas(32) ZoneRec {
+  int lo, hi, j, pad0;
+  double wc, pad1;
+};
+
 struct alignas(16) CoefA {
   float k0, k1, k2, dthr;
 };

@@ -32,17 +32,26 @@ void launch_tile_ranges(int ntiles, int tile, int nbins, int nwl, const int *bin
                         const int *bin_hmax, const int4 *meta_s, int2 *rng, hipStream_t st);
 int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
                      const CoefB *cb, const float *dmaxw, const LineRec *lrec, const double *wlc_s,
-                     const int4 *meta_s, float *lineop, int he, hipStream_t st);
+                     const int4 *meta_s, const long long *loff, const int *list, int64_t mnull, float *lineop,
+                     int he, hipStream_t st);
+// the matrix-core kernel's per-tile line lists: capacities and their offsets (ntiles + 1
+// entries, the last one the total), then the lists
+constexpr int MFMA_MAX_NQ = 8;     // depth blocks of 16 on the matrix cores (up to 128 depths)
+constexpr int FAR_END_GROUPS = 8;  // groups of -1 after each list: the deepest look-ahead
+size_t far_scan_tmp_bytes(int ntiles);
+void launch_far_cap(int ntiles, int nbins, int rstride, const int2 *rng, long long *cap, long long *loff, void *tmp,
+                    size_t tmp_bytes, LineRec *lrec, int64_t mnull, hipStream_t st);
+void launch_far_lists(int ntiles, int nwl, int nbins, int rstride, const int2 *rng, const LineRec *lrec,
+                      const long long *loff, int *list, hipStream_t st);
 void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
-                       const int2 *zone_s, unsigned long long *key, int *val, unsigned long long *key_sorted, int *perm,
-                       void *tmp, size_t tmp_bytes, int *zoff, int *zend, int2 *rng_core,
-                       unsigned long long *evals_core, hipStream_t st);
+                       const int2 *zone_s, const double *wlc_s, unsigned long long *key, int *val,
+                       unsigned long long *key_sorted, int *perm, ZoneRec *zrec, void *tmp, size_t tmp_bytes,
+                       int *zoff, int *zend, int2 *rng_core, unsigned long long *evals_core, hipStream_t st);
 size_t zone_sort_tmp_bytes(int64_t m);
 void launch_zero_f32(float *p, int64_t n, hipStream_t st);
 void launch_add_clear_f32(float *dst, float *src, int64_t n, hipStream_t st);
-void launch_opacity32_core(const DevSetup &S, int ntiles, const int2 *rng_core, const int *perm, const CoefA *ca,
-                           const CoefB *cb, const float *dmaxw, const int2 *zone_s, const double *wlc_s,
-                           const int4 *meta_s, float *lineop, hipStream_t st);
+void launch_opacity32_core(const DevSetup &S, int ntiles, const int2 *rng_core, const ZoneRec *zrec, const CoefA *ca,
+                           const CoefB *cb, const float *dmaxw, float *lineop, hipStream_t st);
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
                       const double *wlc_s, const int4 *meta_s, double *lineop, hipStream_t st);
 void launch_linstr(const DevSetup &S, const DevLines &L, int list_type, const int *flag, float *linstr,

@@ -393,6 +393,48 @@ __device__ __forceinline__ float contrib32(const DevSetup &S, const Coef32 &c, f
   return c.ka * num * __builtin_amdgcn_rcpf(den);
 }
 
+// contrib32 for the PPL points of a lane at once, without branches: both table gathers are
+// issued (at a valid cell for the points outside the table box) before the continued
+// fractions are evaluated, and each point takes the branch contrib32 would take.  Points
+// with act false add nothing.
+__device__ __forceinline__ void contrib32_lane(const DevSetup &S, const Coef32 &c, const float (&v)[PPL],
+                                               const bool (&act)[PPL], float (&out)[PPL]) {
+  float4 e[PPL];
+  float tv[PPL];
+  bool tab[PPL];
+  const size_t row = (size_t)max(c.ia - 1, 0) * TAB_LD;
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    const float av = fabsf(v[q]);
+    tab[q] = c.ia > 0 && av < 8.0f;
+    const float fv = av * (float)S.idv;
+    const int iv = tab[q] ? (int)fv : 0;
+    tv[q] = fv - (float)iv;
+    const float *t0 = S.vtab + row + iv, *t1 = t0 + TAB_LD;
+    e[q] = make_float4(t0[0], t0[1], t1[0], t1[1]);
+  }
+#pragma unroll
+  for (int q = 0; q < PPL; q++) {
+    const float av = fabsf(v[q]);
+    const float v2 = av * av;
+    const float s = v2 + c.a2;
+    // one-pole continued fraction (s >= 196)
+    const float den1 = fmaf(s, s, c.c1 - v2);
+    const float r1 = c.ka * (s + 0.5f) * __builtin_amdgcn_rcpf(den1);
+    // two-pole continued fraction
+    const float p = v2 - c.a2 - 1.5f;
+    const float qq = 2.0f * av * c.a;
+    const float num = fmaf(p, p + 0.5f, fmaf(qq, qq, v2));
+    const float den2 = s * fmaf(p, p, qq * qq);
+    const float r2 = c.ka * num * __builtin_amdgcn_rcpf(den2);
+    const float c0 = fmaf(tv[q], e[q].y - e[q].x, e[q].x);
+    const float c1 = fmaf(tv[q], e[q].w - e[q].z, e[q].z);
+    const float rt = fmaf(c.ua, c1 - c0, c0) * c.cop;
+    const float r = tab[q] ? rt : (s >= 196.0f ? r1 : r2);
+    out[q] = act[q] ? r : 0.0f;
+  }
+}
+
 __device__ __forceinline__ double contrib64(const DevSetup &S, const Coef64 &c, double v) {
   return hvoigt64(S, c.a, c.ia, c.ua, v) * c.cop;
 }
@@ -701,26 +743,28 @@ struct SlotRead<8> {
 };
 
 template <int NQ>
-__global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, int nbins, int rstride,
-                                                        const int2 *__restrict__ rng, const CoefA *__restrict__ ca,
-                                                        const LineRec *__restrict__ lrec, float *__restrict__ lop) {
-  // per wavefront a ring of R group slots in LDS, filled by direct global-to-LDS loads: the
-  // A fragments of the group (NQ x 64 lanes x 16 B, in lane order) and the records of its 4
-  // lines (two 16-byte halves each)
+__global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, const long long *__restrict__ loff,
+                                                        const int *__restrict__ list, const CoefA *__restrict__ ca,
+                                                        const LineRec *__restrict__ lrec, int64_t mnull,
+                                                        float *__restrict__ lop) {
+  // A ring of R group slots in LDS shared by the workgroup's 4 wavefronts (they all run the
+  // tile's list), filled by direct global-to-LDS loads R - 1 groups ahead: the A fragments of
+  // the group (NQ x 64 lanes x 16 B, in lane order; depth block nb loaded by wavefront nb % 4)
+  // and the records of its 4 lines (two 16-byte halves each, by wavefront NQ % 4).
 #ifndef FSG_RING
-#define FSG_RING 2
+#define FSG_RING 4
 #endif
-  constexpr int R = FSG_RING;  // 2: 42 KB of LDS per workgroup for NQ = 5, three workgroups per CU
-  static_assert(R == 2 || R == 3, "ring of 2 or 3 slots");
+  constexpr int R = FSG_RING;
+  static_assert(R >= 2 && R <= 8, "ring of 2 to 8 slots");
   constexpr int SLOT = NQ * 64 + 8;
-  __shared__ mf4 ring[4][R][SLOT];
+  __shared__ mf4 ring[R][SLOT];
   typedef __attribute__((address_space(3))) void lds_void;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  if (t >= ntiles) return;
+  if (t >= ntiles) return;  // whole workgroups leave together (the barriers below)
   const int N = S.ntau, ntp = S.ntp, nwl = S.nwl;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int col = lane & 15, kq = lane >> 4;  // point / depth within a 16-block; line of the group
-  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, nwl);
+  const int p0 = t * TILE + 1;
   const double Wt0 = S.wave[p0 - 1];
   int pp[2];     // this lane's points (B columns)
   float ofs[2];  // their offsets from the tile's first point (sub-angstrom, exact in FP32)
@@ -738,60 +782,39 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
       acc[q][nb] = mf4{0.f, 0.f, 0.f, 0.f};
       cmp[q][nb] = mf4{0.f, 0.f, 0.f, 0.f};
     }
-  // cursor over the tile's candidate lines, bin after bin (wave-uniform): the lines with a
-  // point of the tile inside their window [il, ir] and outside their core zone [zl, zr],
-  // found with a ballot over 64 candidates at a time
-  int cbin = -1, cbase = 0, cend = 0;
-  uint64_t cmask = 0;
-  auto next_line = [&]() -> int {
-    while (!cmask) {
-      cbase += 64;
-      while (cbase >= cend) {
-        if (++cbin >= nbins) return -1;
-        const int2 r = rng[(size_t)t * rstride + cbin];
-        cbase = r.x;
-        cend = r.y;
-      }
-      bool ov = false;
-      if (cbase + lane < cend) {
-        const int4 w = *reinterpret_cast<const int4 *>(lrec + cbase + lane);  // il, ir, zl, zr
-        const int lo = max(w.x, p0), hi = min(w.y, p1);
-        ov = (lo <= hi) & !((w.z <= lo) & (w.w >= hi));
-      }
-      cmask = __builtin_amdgcn_ballot_w64(ov);
+  // the tile's lines in groups of 4 (k_far_lists): -1 past the end, R groups of -1 after
+  // the last line, so the look-ahead of R - 1 groups stays inside the list
+  const int4 *lg = reinterpret_cast<const int4 *>(list + loff[t]);
+  // this wavefront's share of a group's loads (the same count every group: a past-the-end
+  // line loads line 0's coefficients and the empty record at position mnull)
+  constexpr int LW = NQ % 4;  // the wavefront that also loads the line records
+  const int nla = (NQ - wave + 3) / 4;  // depth blocks wave, wave + 4, ... below NQ
+  auto fetch = [&](int s, const int4 jg) {
+    const int jn[4] = {jg.x, jg.y, jg.z, jg.w};
+    const int j = jn[kq];
+    const mf4 *src = reinterpret_cast<const mf4 *>(ca + (size_t)(j >= 0 ? j : 0) * ntp + col);
+    mf4 *slot = ring[s];
+#pragma unroll
+    for (int u = 0; u < (NQ + 3) / 4; u++) {
+      const int nb = wave + 4 * u;
+      if (nb < NQ)
+        __builtin_amdgcn_global_load_lds((const void *)(src + 16 * nb), (lds_void *)(slot + 64 * nb), 16, 0, 0);
     }
-    const int jn = cbase + (int)__builtin_ctzll(cmask);
-    cmask &= cmask - 1;
-    return jn;
-  };
-  // fetch: the next group's lines from the cursor and its loads into ring slot s; returns
-  // the number of lines (0 when the tile is done).  A[nb] reads depths 16 nb .. 16 nb + 15 of
-  // the line's record; depths past ntp land in the following records or in the CoefB array
-  // after the last one (finite values), and only feed accumulator rows d >= N, never stored.
-  auto fetch = [&](int s) -> int {
-    int jn[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) jn[u] = next_line();
-    if (jn[0] < 0) return 0;
-    const int nv = 1 + (jn[1] >= 0) + (jn[2] >= 0) + (jn[3] >= 0);
-    const int j = kq == 0 ? jn[0] : (kq == 1 ? jn[1] : (kq == 2 ? jn[2] : jn[3]));
-    const int jj = j >= 0 ? j : jn[0];
-    mf4 *slot = ring[wave][s];
-    const mf4 *src = reinterpret_cast<const mf4 *>(ca + (size_t)jj * ntp + col);
-#pragma unroll
-    for (int nb = 0; nb < NQ; nb++)
-      __builtin_amdgcn_global_load_lds((const void *)(src + 16 * nb), (lds_void *)(slot + 64 * nb), 16, 0, 0);
-    if (lane < 4) {
-      const int jr = lane == 0 ? jn[0] : (lane == 1 ? jn[1] : (lane == 2 ? jn[2] : jn[3]));
-      const mf4 *rs = reinterpret_cast<const mf4 *>(lrec + (jr >= 0 ? jr : jn[0]));
+    if (wave == LW && lane < 8) {  // lane i: half i / 4 of line i % 4
+      const int jr = jn[lane & 3];
+      const mf4 *rs = reinterpret_cast<const mf4 *>(lrec + (jr >= 0 ? (int64_t)jr : mnull)) + (lane >> 2);
       __builtin_amdgcn_global_load_lds((const void *)rs, (lds_void *)(slot + 64 * NQ), 16, 0, 0);
-      __builtin_amdgcn_global_load_lds((const void *)(rs + 1), (lds_void *)(slot + 64 * NQ + 4), 16, 0, 0);
     }
-    return nv;
+    return jg.x >= 0;
   };
-  // the group in slot s, its loads complete
-  int ng = 0;  // groups in the partial sums (wave-uniform)
-  auto run = [&](int s, int nv) {
+  int ng = 0;  // groups in the partial sums
+  mf4 A[NQ], h0, h1;
+  // the group in slot s: its A fragments and line records into registers
+  auto read = [&](int s) {
+    const unsigned sa = static_cast<unsigned>(reinterpret_cast<uintptr_t>((lds_void *)ring[s]));
+    SlotRead<NQ>::read(sa + 16 * lane, sa + 1024 * NQ + 16 * kq, A, h0, h1);
+  };
+  auto run = [&]() {
     if (ng == FOLDG) {
 #pragma unroll
       for (int q = 0; q < 2; q++)
@@ -803,13 +826,9 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
       ng = 0;
     }
     ng++;
-    const unsigned sa = static_cast<unsigned>(reinterpret_cast<uintptr_t>((lds_void *)ring[wave][s]));
-    mf4 A[NQ], h0, h1;
-    SlotRead<NQ>::read(sa + 16 * lane, sa + 1024 * NQ + 16 * kq, A, h0, h1);
     const int il = __float_as_int(h0[0]), ir = __float_as_int(h0[1]);
     const int zl = __float_as_int(h0[2]), zr = __float_as_int(h0[3]);
     const double wc = __hiloint2double(__float_as_int(h1[1]), __float_as_int(h1[0]));
-    const bool lv = kq < nv;
     // B: T^(n+1) at this lane's (line, point); dl = (point - first point) + (first point -
     // centre) in FP32, relative error below 2^-23 (1 + tile width / |dl|) outside the core zone
     const float bs = (float)(Wt0 - wc);
@@ -817,7 +836,7 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
     for (int q = 0; q < 2; q++) {
       const float x = ofs[q] + bs;
       const float T = __builtin_amdgcn_rcpf(fmaxf(x * x, 1e-12f));
-      const bool act = lv & (pp[q] >= il) & (pp[q] <= ir) & (pp[q] <= nwl) & ((pp[q] < zl) | (pp[q] > zr));
+      const bool act = (pp[q] >= il) & (pp[q] <= ir) & (pp[q] <= nwl) & ((pp[q] < zl) | (pp[q] > zr));
       float X[3];
       X[0] = act ? T : 0.0f;
       X[1] = X[0] * T;
@@ -829,34 +848,28 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
           C[q][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[nb][n], X[n], C[q][nb], 0, 0, 0);
     }
   };
-  // wait until at most n vector memory operations are outstanding (asm, so that the compiler
-  // neither moves nor merges it)
+  // wait until at most n of this wavefront's vector memory operations are outstanding (asm,
+  // so that the compiler neither moves nor merges it); n counts the loads of the R - 2 groups
+  // issued after the one about to run
 #define FSG_VMCNT(n) asm volatile("s_waitcnt vmcnt(%0)" ::"n"(n))
-  constexpr int LPG = NQ + 2;  // loads per group
-  if constexpr (R == 3) {
-    int nv0 = fetch(0), nv1 = fetch(1), s0 = 0;
-    while (nv0) {
-      const int s2 = s0 == 0 ? 2 : s0 - 1;  // slot of the group R - 1 ahead
-      const int nv2 = fetch(s2);
-      // slot s0 was filled R - 1 fetches ago; the loads issued since then are newer
-      if (nv2) FSG_VMCNT(2 * LPG);
-      else if (nv1) FSG_VMCNT(LPG);
-      else FSG_VMCNT(0);
-      run(s0, nv0);
-      nv0 = nv1;
-      nv1 = nv2;
-      s0 = s0 == 2 ? 0 : s0 + 1;
-    }
-  } else {
-    int nv0 = fetch(0), s0 = 0;
-    while (nv0) {
-      const int nv1 = fetch(s0 ^ 1);
-      if (nv1) FSG_VMCNT(LPG);
-      else FSG_VMCNT(0);
-      run(s0, nv0);
-      nv0 = nv1;
-      s0 ^= 1;
-    }
+  int nvalid = -1;  // groups in the list, once the look-ahead has met its end
+#pragma unroll
+  for (int h = 0; h < R - 1; h++)
+    if (!fetch(h, lg[h]) && nvalid < 0) nvalid = h;
+  int4 jnext = lg[R - 1];  // the indices of the next group to fetch
+  for (int g = 0; nvalid < 0 || g < nvalid; g++) {
+    // this wavefront's loads of group g are complete; after the barrier every wavefront's
+    // are, and every wavefront has finished reading group g - 1, whose slot is refilled next
+    const int nl = nla + (wave == LW);
+    if (nl == 1) FSG_VMCNT(1 * (R - 2));
+    else if (nl == 2) FSG_VMCNT(2 * (R - 2));
+    else FSG_VMCNT(3 * (R - 2));
+    __builtin_amdgcn_s_barrier();
+    const int h = g + R - 1;
+    if (!fetch(h % R, jnext) && nvalid < 0) nvalid = h;
+    read(g % R);
+    jnext = lg[h + 1];  // in flight during the matrix products
+    run();
   }
 #undef FSG_VMCNT
 #pragma unroll
@@ -888,11 +901,9 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
 // per-(line, depth) records through scalar loads).  The candidates come bin by bin from the
 // lines sorted by core-zone half width (perm), NZB - 1 bins per tile in rng.
 __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles, const int2 *__restrict__ rng,
-                                                         const int *__restrict__ perm, const CoefA *__restrict__ ca,
+                                                         const ZoneRec *__restrict__ zrec, const CoefA *__restrict__ ca,
                                                          const CoefB *__restrict__ cb, const float *__restrict__ dmaxw,
-                                                         const double *__restrict__ wlc_s,
-                                                         const int4 *__restrict__ meta_s,
-                                                         const int2 *__restrict__ zone_s, float *__restrict__ lop) {
+                                                         float *__restrict__ lop) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
   const int N = S.ntau, nwv = S.ntp / DPW;
@@ -923,13 +934,14 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
   int grp = 0;  // lines in the current partial sums (wave-uniform)
   for (int b = 0; b < NZB - 1; b++) {
     const int2 r = rng[(size_t)t * NZB + b];
+    ZoneRec nx = zrec[r.x];  // one record ahead (the array holds one spare record at the end)
     for (int x = r.x; x < r.y; x++) {
-      const int j = perm[x];
-      const int4 m = meta_s[j];
-      const int2 z = zone_s[j];
-      const int lo = max(max(m.x, z.x), p0), hi = min(min(m.y, z.y), p1);
+      const ZoneRec zr = nx;
+      nx = zrec[x + 1];
+      const int j = zr.j;
+      const int lo = max(zr.lo, p0), hi = min(zr.hi, p1);
       if (lo > hi) continue;  // zone and window miss the tile (wave-uniform)
-      const double wc = wlc_s[j];
+      const double wc = zr.wc;
       const double dmin = (wc < Wt0) ? (Wt0 - wc) : (wc > Wt1 ? (wc - Wt1) : 0.0);
       const unsigned dmb = __builtin_amdgcn_readfirstlane(__float_as_uint((float)dmin));
       if (grp == FOLD) {
@@ -977,14 +989,23 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
           part[q][k] += xx * fmaf(fmaf(rr[k][2], xx, rr[k][1]), xx, rr[k][0]);
         }
       if (gen) {
+        // the wave's general-path records all in one batch of scalar loads
         const CoefB *cbp = cb + (size_t)j * S.ntp + d0;
+        CoefB cc[DPW];
+#pragma unroll
+        for (int k = 0; k < DPW; k++) cc[k] = cbp[k];
+        float dl[PPL];
+#pragma unroll
+        for (int q = 0; q < PPL; q++) dl[q] = (float)(Wp[q] - wc);
 #pragma unroll
         for (int k = 0; k < DPW; k++) {
           if (!(gen & (1u << k))) continue;
-          const CoefB c = cbp[k];
+          float v[PPL], r[PPL];
 #pragma unroll
-          for (int q = 0; q < PPL; q++)
-            if (act[q]) part[q][k] += contrib32(S, c, (float)(Wp[q] - wc) * c.vv);
+          for (int q = 0; q < PPL; q++) v[q] = dl[q] * cc[k].vv;
+          contrib32_lane(S, cc[k], v, act, r);
+#pragma unroll
+          for (int q = 0; q < PPL; q++) part[q][k] += r[q];
         }
       }
     }
@@ -1004,6 +1025,61 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
       }
     }
   }
+}
+
+// The matrix-core kernel's line lists: per tile, the positions of the candidate lines (the
+// per-bin ranges of k_tile_ranges) that have a point of the tile inside their window and
+// outside their core zone, in candidate order, padded with -1 to a multiple of 4 and followed
+// by FAR_END_GROUPS groups of four -1.  cap[t] bounds the list: candidates rounded up to 4,
+// plus the end groups.
+__global__ void k_far_cap(int ntiles, int nbins, int rstride, const int2 *rng, long long *cap, LineRec *lrec,
+                          int64_t mnull) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  if (t == ntiles) {  // the empty record that past-the-end group entries load: no window
+    LineRec r;
+    r.il = 1;
+    r.ir = 0;
+    r.zl = 1;
+    r.zr = 0;
+    r.wc = 0.0;
+    r.pad0 = r.pad1 = 0;
+    lrec[mnull] = r;
+  }
+  long long n = 0;
+  if (t < ntiles)
+    for (int b = 0; b < nbins; b++) {
+      const int2 r = rng[(size_t)t * rstride + b];
+      n += r.y - r.x;
+    }
+  cap[t] = t < ntiles ? ((n + 3) & ~3LL) + 4 * FAR_END_GROUPS : 0;  // the entry past the last tile: the total
+}
+
+__global__ void __launch_bounds__(64) k_far_lists(int ntiles, int nwl, int nbins, int rstride, const int2 *rng,
+                                                  const LineRec *lrec, const long long *loff, int *list) {
+  const int t = blockIdx.x;
+  if (t >= ntiles) return;
+  const int lane = threadIdx.x;
+  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, nwl);
+  int *out = list + loff[t];
+  int n = 0;
+  for (int b = 0; b < nbins; b++) {
+    const int2 r = rng[(size_t)t * rstride + b];
+    for (int base = r.x; base < r.y; base += 64) {
+      const int x = base + lane;
+      bool ov = false;
+      if (x < r.y) {
+        const int4 w = *reinterpret_cast<const int4 *>(lrec + x);  // il, ir, zl, zr
+        const int lo = max(w.x, p0), hi = min(w.y, p1);
+        ov = (lo <= hi) & !((w.z <= lo) & (w.w >= hi));
+      }
+      const uint64_t m = __builtin_amdgcn_ballot_w64(ov);
+      if (ov) out[n + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] = x;
+      n += __builtin_popcountll(m);
+    }
+  }
+  const int end = ((n + 3) & ~3) + 4 * FAR_END_GROUPS;
+  for (int i = n + lane; i < end; i += 64) out[i] = -1;
 }
 
 // core-zone sort keys: (zone bin << 32) | centre point, the bin being the bit length of the
@@ -1033,6 +1109,24 @@ __global__ void k_zone_bounds(int64_t m, const unsigned long long *key, int *zof
   const unsigned b = (unsigned)(key[i] >> 32);
   if (i == 0 || (unsigned)(key[i - 1] >> 32) != b) zoff[b] = (int)i;
   if (i == m - 1 || (unsigned)(key[i + 1] >> 32) != b) zend[b] = (int)(i + 1);
+}
+
+// the core-zone kernel's records in core-zone order (one scalar load per candidate)
+__global__ void k_zone_records(int64_t m, const int *perm, const int4 *meta_s, const int2 *zone_s,
+                               const double *wlc_s, ZoneRec *zrec) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= m) return;
+  const int j = perm[x];
+  const int4 mm = meta_s[j];
+  const int2 z = zone_s[j];
+  ZoneRec r;
+  r.lo = max(mm.x, z.x);
+  r.hi = min(mm.y, z.y);
+  r.j = j;
+  r.pad0 = 0;
+  r.wc = wlc_s[j];
+  r.pad1 = 0.0;
+  zrec[x] = r;
 }
 
 __global__ void k_zero_f32(float *p, int64_t n) {
@@ -1428,9 +1522,27 @@ void launch_tile_ranges(int ntiles, int tile, int nbins, int nwl, const int *bin
 }
 
 // returns 1 when the matrix-core kernel ran (the core zones then need launch_opacity32_core)
+size_t far_scan_tmp_bytes(int ntiles) {
+  size_t b = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (long long *)nullptr, (long long *)nullptr, ntiles + 1);
+  return b;
+}
+
+void launch_far_cap(int ntiles, int nbins, int rstride, const int2 *rng, long long *cap, long long *loff, void *tmp,
+                    size_t tmp_bytes, LineRec *lrec, int64_t mnull, hipStream_t st) {
+  k_far_cap<<<nblk(ntiles + 1, 256), 256, 0, st>>>(ntiles, nbins, rstride, rng, cap, lrec, mnull);
+  (void)hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, cap, loff, ntiles + 1, st);
+}
+
+void launch_far_lists(int ntiles, int nwl, int nbins, int rstride, const int2 *rng, const LineRec *lrec,
+                      const long long *loff, int *list, hipStream_t st) {
+  if (ntiles > 0) k_far_lists<<<ntiles, 64, 0, st>>>(ntiles, nwl, nbins, rstride, rng, lrec, loff, list);
+}
+
 int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
                      const CoefB *cb, const float *dmaxw, const LineRec *lrec, const double *wlc_s,
-                     const int4 *meta_s, float *lineop, int he, hipStream_t st) {
+                     const int4 *meta_s, const long long *loff, const int *list, int64_t mnull, float *lineop,
+                     int he, hipStream_t st) {
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
   const int nq = (S.ntau + 15) / 16;
@@ -1442,7 +1554,7 @@ int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, cons
   switch (nq) {  // depth blocks of 16: up to 128 depths on the matrix cores
 #define FSG_MFMA_CASE(Q)                                                                                  \
   case Q:                                                                                                \
-    k_opacity32_mfma<Q><<<nb, 256, 0, st>>>(S, ntiles, nbins, rstride, rng, ca, lrec, lineop);            \
+    k_opacity32_mfma<Q><<<nb, 256, 0, st>>>(S, ntiles, loff, list, ca, lrec, mnull, lineop);              \
     return 1;
     FSG_MFMA_CASE(1)
     FSG_MFMA_CASE(2)
@@ -1460,14 +1572,15 @@ int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, cons
 }
 
 void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
-                       const int2 *zone_s, unsigned long long *key, int *val, unsigned long long *key_sorted, int *perm,
-                       void *tmp, size_t tmp_bytes, int *zoff, int *zend, int2 *rng_core,
-                       unsigned long long *evals_core, hipStream_t st) {
+                       const int2 *zone_s, const double *wlc_s, unsigned long long *key, int *val,
+                       unsigned long long *key_sorted, int *perm, ZoneRec *zrec, void *tmp, size_t tmp_bytes,
+                       int *zoff, int *zend, int2 *rng_core, unsigned long long *evals_core, hipStream_t st) {
   k_fill_int<<<1, 2 * NZB, 0, st>>>(zoff, 2 * NZB, 0);  // zoff and zend are adjacent
   if (m > 0) {
     k_zone_keys<<<nblk(m, 256), 256, 0, st>>>(m, mmain, ntau, meta_s, zone_s, key, val, evals_core);
     (void)hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key_sorted, val, perm, (int)m, 0, 64, st);
     k_zone_bounds<<<nblk(m, 256), 256, 0, st>>>(m, key_sorted, zoff, zend);
+    k_zone_records<<<nblk(m, 256), 256, 0, st>>>(m, perm, meta_s, zone_s, wlc_s, zrec);
   }
   dim3 grid(nblk(ntiles, 256), NZB - 1);
   k_tile_ranges_zone<<<grid, 256, 0, st>>>(ntiles, TILE, nwl, zoff, zend, key_sorted, rng_core);
@@ -1488,12 +1601,11 @@ size_t zone_sort_tmp_bytes(int64_t m) {
   return b;
 }
 
-void launch_opacity32_core(const DevSetup &S, int ntiles, const int2 *rng_core, const int *perm, const CoefA *ca,
-                           const CoefB *cb, const float *dmaxw, const int2 *zone_s, const double *wlc_s,
-                           const int4 *meta_s, float *lineop, hipStream_t st) {
+void launch_opacity32_core(const DevSetup &S, int ntiles, const int2 *rng_core, const ZoneRec *zrec, const CoefA *ca,
+                           const CoefB *cb, const float *dmaxw, float *lineop, hipStream_t st) {
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
-  k_opacity32_core<<<nb, 64 * nw, 0, st>>>(S, ntiles, rng_core, perm, ca, cb, dmaxw, wlc_s, meta_s, zone_s, lineop);
+  k_opacity32_core<<<nb, 64 * nw, 0, st>>>(S, ntiles, rng_core, zrec, ca, cb, dmaxw, lineop);
 }
 
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
