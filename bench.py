@@ -215,10 +215,11 @@ def main():
     far_tf = far * MFMA_FLOP_PER_EVAL / (st["ms_far"] * 1e-3) / 1e12 if st["ms_far"] > 0 else None
     kernels = {
         "k_opacity32_mfma": {"ms": st["ms_far"], "evals": far, "mfma_flop_per_eval": MFMA_FLOP_PER_EVAL,
+                             "lds_ring_slots": st["far_ring"],
                              "depth_padding": pad, "tflops": far_tf,
                              "frac_of_fp32_mfma_peak": far_tf / PEAK_FP32_TFLOPS if far_tf else None,
                              "issued_frac": far_tf * pad / PEAK_FP32_TFLOPS if far_tf else None},
-        "k_opacity32_core": {"ms": st["ms_core"], "evals": float(st["evals_core"]),
+        "k_opacity32_core": {"ms": st["ms_core"], "evals": float(st["evals_core"]), "tile_points": st["core_tile"],
                              "evals_per_s": st["evals_core"] / (st["ms_core"] * 1e-3) if st["ms_core"] > 0 else None},
     }
     out = {

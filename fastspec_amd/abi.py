@@ -70,6 +70,7 @@ class CStats(C.Structure):
         ("nblocks", C.c_int64), ("nsegments", C.c_int64), ("evals", C.c_int64),
         ("ms_pass1", C.c_double), ("ms_opacity", C.c_double),
         ("evals_core", C.c_int64), ("ms_far", C.c_double), ("ms_core", C.c_double),
+        ("core_tile", C.c_int32), ("far_ring", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -10,8 +10,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/fsgpu.h"
@@ -89,6 +95,10 @@ struct fsg_ctx {
   hipEvent_t evt[4] = {};  // first and last far-wing launch (on st), core-zone launch (on st2)
   float *lop2 = nullptr;
   bool lop2_dirty = false;  // a core launch has not been followed by its add (an error path)
+  // launch geometry of the FP32 opacity kernels, chosen by the autotuner on the first chunk
+  // of the first FP32 line-opacity call: core-zone tile of 64 core_pp points, far_ring slots
+  int core_pp = 2, far_ring = 4;
+  bool tuned = false;
   // polarised path, allocated on first use
   void *maglineop = nullptr;  // [6][ntau][nwl] MagLineOpac, wavelength fastest
   float2 *ftab = nullptr;     // Faraday-Voigt table [NVA+1][TAB_LD]
@@ -113,6 +123,126 @@ struct fsg_dlines {
   int zeeman = 0;
   std::vector<void *> owned;
 };
+
+namespace {
+
+// ------------------------------------------------------------------------------------
+// Autotuner of the FP32 opacity launch geometry.  Candidates: the core-zone kernel's tile
+// (64 or 128 points) and the matrix-core kernel's LDS ring depth (2 or 4 group slots); the
+// far-wing kernel's results do not depend on its ring depth, the core-zone kernel's only
+// through the grouping of its blocked compensated sums.  Each candidate runs twice on the
+// chunk at hand into the scratch accumulator lop2 (zeroed afterwards), and the faster of its
+// two device times counts.  Choices are kept per process for a workload signature (device,
+// depth count, grid size and chunk size in powers of two) and, with FSG_AUTOTUNE_CACHE set
+// to a file name, in that file ("signature core_pp far_ring" lines).  FSG_AUTOTUNE=0 keeps
+// the defaults (128-point core tiles, 4 ring slots) without timing.
+// ------------------------------------------------------------------------------------
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<int, int>> g_tune;
+bool g_tune_file_read = false;
+
+int ilog2_64(int64_t v) {
+  int b = 0;
+  while (v > 1) {
+    v >>= 1;
+    b++;
+  }
+  return b;
+}
+
+std::string tune_signature(const fsg_ctx *c, int64_t m) {
+  hipDeviceProp_t prop{};
+  (void)hipGetDeviceProperties(&prop, c->dev);
+  std::ostringstream k;
+  k << prop.gcnArchName << "/cu" << prop.multiProcessorCount << "/ntau" << c->S.ntau << "/nwl2^" << ilog2_64(c->S.nwl)
+    << "/lines2^" << ilog2_64(m);
+  std::string s = k.str();
+  for (char &ch : s)
+    if (ch == ' ' || ch == ':') ch = '_';
+  return s;
+}
+
+void tune_read_file() {  // under g_tune_mu
+  if (g_tune_file_read) return;
+  g_tune_file_read = true;
+  const char *fn = std::getenv("FSG_AUTOTUNE_CACHE");
+  if (!fn || !*fn) return;
+  std::ifstream in(fn);
+  std::string key;
+  int pp = 0, ring = 0;
+  while (in >> key >> pp >> ring)
+    if ((pp == 1 || pp == 2) && (ring == 2 || ring == 4)) g_tune[key] = {pp, ring};
+}
+
+void tune_write_file(const std::string &key, int pp, int ring) {  // under g_tune_mu
+  const char *fn = std::getenv("FSG_AUTOTUNE_CACHE");
+  if (!fn || !*fn) return;
+  std::ofstream out(fn, std::ios::app);
+  out << key << " " << pp << " " << ring << "\n";
+}
+
+}  // namespace
+
+// the chunk's coefficient records, line lists and zone order are complete on c->st
+static int autotune(fsg_ctx *c, int ntiles, int64_t m, const CoefA *ca, const CoefB *cb, const float *dmaxw,
+                    const LineRec *lrec, const int *zoff, const int *zend) {
+  const std::string key = tune_signature(c, m);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_read_file();
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      c->core_pp = it->second.first;
+      c->far_ring = it->second.second;
+      return 0;
+    }
+  }
+  const char *en = std::getenv("FSG_AUTOTUNE");
+  if (en && en[0] == '0') return 0;
+  const DevSetup &S = c->S;
+  hipStream_t s = c->st;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  auto timed = [&](auto &&launch, float &best) -> int {
+    for (int rep = 0; rep < 2; rep++) {
+      CK(hipEventRecord(e0, s));
+      launch();
+      CK(hipEventRecord(e1, s));
+      CK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      CK(hipEventElapsedTime(&ms, e0, e1));
+      best = std::min(best, ms);
+    }
+    return 0;
+  };
+  const int rings[2] = {2, 4}, pps[2] = {1, 2};
+  float tf[2] = {1e30f, 1e30f}, tc[2] = {1e30f, 1e30f};
+  int rc = 0;
+  for (int i = 0; i < 2 && !rc; i++)
+    rc = timed(
+        [&] {
+          launch_opacity32(S, ntiles, 0, 0, nullptr, ca, cb, dmaxw, lrec, nullptr, nullptr, c->foff.as<long long>(),
+                           c->flist.as<int>(), m, c->lop2, 0, rings[i], s);
+        },
+        tf[i]);
+  for (int i = 0; i < 2 && !rc; i++) {
+    launch_core_ranges(pps[i], S.nwl, zoff, zend, c->zkey2.as<unsigned long long>(), c->rng2.as<int2>(), s);
+    rc = timed([&] { launch_opacity32_core(S, pps[i], c->rng2.as<int2>(), c->zrec.as<ZoneRec>(), ca, cb, dmaxw, c->lop2, s); },
+               tc[i]);
+  }
+  launch_zero_f32(c->lop2, (int64_t)S.ntau * S.nwl, s);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  CK(hipGetLastError());
+  c->far_ring = tf[0] < tf[1] ? 2 : 4;
+  c->core_pp = tc[0] < tc[1] ? 1 : 2;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = {c->core_pp, c->far_ring};
+  tune_write_file(key, c->core_pp, c->far_ring);
+  return 0;
+}
 
 namespace {
 
@@ -672,7 +802,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
     CK(c->rng.ensure((size_t)ntiles * ng * sizeof(int2)));
     size_t ztmp = 0;
     if (c->precision == FSG_FP32) {
-      CK(c->rng2.ensure((size_t)ntiles * NZB * sizeof(int2)));
+      CK(c->rng2.ensure((size_t)((S.nwl + 63) / 64) * NZB * sizeof(int2)));  // core tiles of 64 points at least
       CK(c->zkey.ensure((size_t)chunk * sizeof(unsigned long long)));
       CK(c->zkey2.ensure((size_t)chunk * sizeof(unsigned long long)));
       CK(c->zval.ensure((size_t)chunk * sizeof(int)));
@@ -722,7 +852,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
         launch_zone_order(m, std::min<int64_t>(std::max<int64_t>(goff[nbins] - s0, 0), m), ntiles, S.nwl, S.ntau,
                           c->meta.as<int4>(), zone, c->wlc.as<double>(), c->zkey.as<unsigned long long>(),
                           c->zval.as<int>(), c->zkey2.as<unsigned long long>(), c->zperm.as<int>(),
-                          c->zrec.as<ZoneRec>(), c->tmp.p, ztmp, zoff, zend, c->rng2.as<int2>(), c->u64 + 2, s);
+                          c->zrec.as<ZoneRec>(), c->tmp.p, ztmp, zoff, zend, c->u64 + 2, s);
       // the range table has ng groups per tile: the first nbins are read by the main launch
       // (stride ng), the last one by the He launch (offset nbins, stride ng)
       const bool has_main = s0 < goff[nbins], has_he = s1 > goff[nbins];
@@ -741,6 +871,11 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
           CK(c->flist.ensure((size_t)ftotal * sizeof(int)));
           launch_far_lists(ntiles, S.nwl, nbins, ng, c->rng.as<int2>(), lrec, c->foff.as<long long>(),
                            c->flist.as<int>(), s);
+          if (!c->tuned) {
+            if (autotune(c, ntiles, m, ca, cb, dmaxw, lrec, zoff, zend)) return 1;
+            c->tuned = true;
+          }
+          launch_core_ranges(c->core_pp, S.nwl, zoff, zend, c->zkey2.as<unsigned long long>(), c->rng2.as<int2>(), s);
         }
       }
       if (!timed) CK(hipEventRecord(c->ev[2], s));
@@ -753,12 +888,12 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
           if (!nfar) CK(hipEventRecord(c->evt[0], s));
           const int mf = launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, lrec,
                                           c->wlc.as<double>(), c->meta.as<int4>(), c->foff.as<long long>(),
-                                          c->flist.as<int>(), m, (float *)c->lineop, 0, s);
+                                          c->flist.as<int>(), m, (float *)c->lineop, 0, c->far_ring, s);
           CK(hipEventRecord(c->evt[1], s));
           nfar++;
           if (mf) {
             if (!ncore) CK(hipEventRecord(c->evt[2], c->st2));
-            launch_opacity32_core(S, ntiles, c->rng2.as<int2>(), c->zrec.as<ZoneRec>(), ca, cb, dmaxw, c->lop2,
+            launch_opacity32_core(S, c->core_pp, c->rng2.as<int2>(), c->zrec.as<ZoneRec>(), ca, cb, dmaxw, c->lop2,
                                   c->st2);
             CK(hipEventRecord(c->evt[3], c->st2));
             CK(hipEventRecord(c->evj, c->st2));
@@ -769,7 +904,7 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
         }
         if (has_he)
           launch_opacity32(S, ntiles, 1, ng, c->rng.as<int2>() + nbins, ca, cb, dmaxw, lrec, c->wlc.as<double>(),
-                           c->meta.as<int4>(), nullptr, nullptr, 0, (float *)c->lineop, 1, s);
+                           c->meta.as<int4>(), nullptr, nullptr, 0, (float *)c->lineop, 1, c->far_ring, s);
       } else {
         if (has_main && nbins > 0)
           launch_opacity64(S, ntiles, nbins, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), c->wlc.as<double>(),
@@ -805,6 +940,10 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   if (ncore) {
     CK(hipEventElapsedTime(&ms1, c->evt[2], c->evt[3]));
     st.ms_core = ms1;
+  }
+  if (c->precision == FSG_FP32) {
+    st.core_tile = 64 * c->core_pp;
+    st.far_ring = c->far_ring;
   }
   c->have_rt = 0;
   c->have_glop = 0;

@@ -33,7 +33,7 @@ void launch_tile_ranges(int ntiles, int tile, int nbins, int nwl, const int *bin
 int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
                      const CoefB *cb, const float *dmaxw, const LineRec *lrec, const double *wlc_s,
                      const int4 *meta_s, const long long *loff, const int *list, int64_t mnull, float *lineop,
-                     int he, hipStream_t st);
+                     int he, int ring, hipStream_t st);
 // the matrix-core kernel's per-tile line lists: capacities and their offsets (ntiles + 1
 // entries, the last one the total), then the lists
 constexpr int MFMA_MAX_NQ = 8;     // depth blocks of 16 on the matrix cores (up to 128 depths)
@@ -46,11 +46,14 @@ void launch_far_lists(int ntiles, int nwl, int nbins, int rstride, const int2 *r
 void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
                        const int2 *zone_s, const double *wlc_s, unsigned long long *key, int *val,
                        unsigned long long *key_sorted, int *perm, ZoneRec *zrec, void *tmp, size_t tmp_bytes,
-                       int *zoff, int *zend, int2 *rng_core, unsigned long long *evals_core, hipStream_t st);
+                       int *zoff, int *zend, unsigned long long *evals_core, hipStream_t st);
+// per tile of 64 core_pp points and zone bin, the candidate range of the core-zone kernel
+void launch_core_ranges(int core_pp, int nwl, const int *zoff, const int *zend, const unsigned long long *key_sorted,
+                        int2 *rng_core, hipStream_t st);
 size_t zone_sort_tmp_bytes(int64_t m);
 void launch_zero_f32(float *p, int64_t n, hipStream_t st);
 void launch_add_clear_f32(float *dst, float *src, int64_t n, hipStream_t st);
-void launch_opacity32_core(const DevSetup &S, int ntiles, const int2 *rng_core, const ZoneRec *zrec, const CoefA *ca,
+void launch_opacity32_core(const DevSetup &S, int core_pp, const int2 *rng_core, const ZoneRec *zrec, const CoefA *ca,
                            const CoefB *cb, const float *dmaxw, float *lineop, hipStream_t st);
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
                       const double *wlc_s, const int4 *meta_s, double *lineop, hipStream_t st);

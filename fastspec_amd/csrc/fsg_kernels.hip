@@ -393,18 +393,19 @@ __device__ __forceinline__ float contrib32(const DevSetup &S, const Coef32 &c, f
   return c.ka * num * __builtin_amdgcn_rcpf(den);
 }
 
-// contrib32 for the PPL points of a lane at once, without branches: both table gathers are
+// contrib32 for the P points of a lane at once, without branches: both table gathers are
 // issued (at a valid cell for the points outside the table box) before the continued
 // fractions are evaluated, and each point takes the branch contrib32 would take.  Points
 // with act false add nothing.
-__device__ __forceinline__ void contrib32_lane(const DevSetup &S, const Coef32 &c, const float (&v)[PPL],
-                                               const bool (&act)[PPL], float (&out)[PPL]) {
-  float4 e[PPL];
-  float tv[PPL];
-  bool tab[PPL];
+template <int P>
+__device__ __forceinline__ void contrib32_lane(const DevSetup &S, const Coef32 &c, const float (&v)[P],
+                                               const bool (&act)[P], float (&out)[P]) {
+  float4 e[P];
+  float tv[P];
+  bool tab[P];
   const size_t row = (size_t)max(c.ia - 1, 0) * TAB_LD;
 #pragma unroll
-  for (int q = 0; q < PPL; q++) {
+  for (int q = 0; q < P; q++) {
     const float av = fabsf(v[q]);
     tab[q] = c.ia > 0 && av < 8.0f;
     const float fv = av * (float)S.idv;
@@ -414,7 +415,7 @@ __device__ __forceinline__ void contrib32_lane(const DevSetup &S, const Coef32 &
     e[q] = make_float4(t0[0], t0[1], t1[0], t1[1]);
   }
 #pragma unroll
-  for (int q = 0; q < PPL; q++) {
+  for (int q = 0; q < P; q++) {
     const float av = fabsf(v[q]);
     const float v2 = av * av;
     const float s = v2 + c.a2;
@@ -742,7 +743,7 @@ struct SlotRead<8> {
   }
 };
 
-template <int NQ>
+template <int NQ, int R>
 __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, const long long *__restrict__ loff,
                                                         const int *__restrict__ list, const CoefA *__restrict__ ca,
                                                         const LineRec *__restrict__ lrec, int64_t mnull,
@@ -751,10 +752,6 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
   // tile's list), filled by direct global-to-LDS loads R - 1 groups ahead: the A fragments of
   // the group (NQ x 64 lanes x 16 B, in lane order; depth block nb loaded by wavefront nb % 4)
   // and the records of its 4 lines (two 16-byte halves each, by wavefront NQ % 4).
-#ifndef FSG_RING
-#define FSG_RING 4
-#endif
-  constexpr int R = FSG_RING;
   static_assert(R >= 2 && R <= 8, "ring of 2 to 8 slots");
   constexpr int SLOT = NQ * 64 + 8;
   __shared__ mf4 ring[R][SLOT];
@@ -897,9 +894,10 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
 // inside its window, at the points of zone and window, every depth.  Per (line, depth) the
 // far-wing expansion still applies where the whole tile lies beyond dthr (straight-line FMA
 // code); the other depths run the Voigt table / continued fractions (contrib32).  Same layout
-// as k_opacity32 (a tile of TILE points, one wavefront per DPW depths, PPL points per lane,
+// as k_opacity32 (a tile of 64 CP points, one wavefront per DPW depths, CP points per lane,
 // per-(line, depth) records through scalar loads).  The candidates come bin by bin from the
-// lines sorted by core-zone half width (perm), NZB - 1 bins per tile in rng.
+// lines sorted by core-zone half width (zrec), NZB - 1 bins per tile in rng.
+template <int CP>
 __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles, const int2 *__restrict__ rng,
                                                          const ZoneRec *__restrict__ zrec, const CoefA *__restrict__ ca,
                                                          const CoefB *__restrict__ cb, const float *__restrict__ dmaxw,
@@ -908,23 +906,24 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
   if (t >= ntiles) return;
   const int N = S.ntau, nwv = S.ntp / DPW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
+  constexpr int TC = 64 * CP;  // points per tile
+  const int p0 = t * TC + 1, p1 = min(p0 + TC - 1, S.nwl);
   const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];
-  int pp[PPL];
-  bool valid[PPL];
-  double Wp[PPL];
-  float ofs[PPL];
+  int pp[CP];
+  bool valid[CP];
+  double Wp[CP];
+  float ofs[CP];
 #pragma unroll
-  for (int q = 0; q < PPL; q++) {
+  for (int q = 0; q < CP; q++) {
     pp[q] = p0 + lane + 64 * q;
     valid[q] = pp[q] <= S.nwl;
     Wp[q] = valid[q] ? S.wave[pp[q] - 1] : 0.0;
     ofs[q] = valid[q] ? (float)(Wp[q] - Wt0) : 0.0f;
   }
   const int d0 = wave * DPW;
-  float acc[PPL][DPW], cmp[PPL][DPW], part[PPL][DPW];
+  float acc[CP][DPW], cmp[CP][DPW], part[CP][DPW];
 #pragma unroll
-  for (int q = 0; q < PPL; q++)
+  for (int q = 0; q < CP; q++)
 #pragma unroll
     for (int k = 0; k < DPW; k++) {
       acc[q][k] = 0.f;
@@ -946,7 +945,7 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
       const unsigned dmb = __builtin_amdgcn_readfirstlane(__float_as_uint((float)dmin));
       if (grp == FOLD) {
 #pragma unroll
-        for (int q = 0; q < PPL; q++)
+        for (int q = 0; q < CP; q++)
 #pragma unroll
           for (int k = 0; k < DPW; k++) {
             kahan_add(acc[q][k], cmp[q][k], part[q][k]);
@@ -955,9 +954,9 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
         grp = 0;
       }
       grp++;
-      bool act[PPL];
+      bool act[CP];
 #pragma unroll
-      for (int q = 0; q < PPL; q++) act[q] = valid[q] && pp[q] >= lo && pp[q] <= hi;
+      for (int q = 0; q < CP; q++) act[q] = valid[q] && pp[q] >= lo && pp[q] <= hi;
       const f4v *cp = reinterpret_cast<const f4v *>(ca + (size_t)j * S.ntp + d0);
       f4v rr[DPW];
 #pragma unroll
@@ -975,16 +974,16 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
       // far wing: straight-line FMA code over all depths (zero coefficients at the general
       // depths); dl = (point - first point) + (first point - centre), both rounded to FP32
       const float bs = (float)(Wt0 - wc);
-      float T2[PPL];
+      float T2[CP];
 #pragma unroll
-      for (int q = 0; q < PPL; q++) {
+      for (int q = 0; q < CP; q++) {
         const float xx = ofs[q] + bs;
         T2[q] = act[q] ? __builtin_amdgcn_rcpf(fmaxf(xx * xx, 1e-30f)) : 0.0f;
       }
 #pragma unroll
       for (int k = 0; k < DPW; k++)
 #pragma unroll
-        for (int q = 0; q < PPL; q++) {
+        for (int q = 0; q < CP; q++) {
           const float xx = T2[q];
           part[q][k] += xx * fmaf(fmaf(rr[k][2], xx, rr[k][1]), xx, rr[k][0]);
         }
@@ -994,24 +993,24 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
         CoefB cc[DPW];
 #pragma unroll
         for (int k = 0; k < DPW; k++) cc[k] = cbp[k];
-        float dl[PPL];
+        float dl[CP];
 #pragma unroll
-        for (int q = 0; q < PPL; q++) dl[q] = (float)(Wp[q] - wc);
+        for (int q = 0; q < CP; q++) dl[q] = (float)(Wp[q] - wc);
 #pragma unroll
         for (int k = 0; k < DPW; k++) {
           if (!(gen & (1u << k))) continue;
-          float v[PPL], r[PPL];
+          float v[CP], r[CP];
 #pragma unroll
-          for (int q = 0; q < PPL; q++) v[q] = dl[q] * cc[k].vv;
+          for (int q = 0; q < CP; q++) v[q] = dl[q] * cc[k].vv;
           contrib32_lane(S, cc[k], v, act, r);
 #pragma unroll
-          for (int q = 0; q < PPL; q++) part[q][k] += r[q];
+          for (int q = 0; q < CP; q++) part[q][k] += r[q];
         }
       }
     }
   }
 #pragma unroll
-  for (int q = 0; q < PPL; q++) {
+  for (int q = 0; q < CP; q++) {
     if (!valid[q]) continue;
 #pragma unroll
     for (int k = 0; k < DPW; k++) {
@@ -1542,7 +1541,7 @@ void launch_far_lists(int ntiles, int nwl, int nbins, int rstride, const int2 *r
 int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const CoefA *ca,
                      const CoefB *cb, const float *dmaxw, const LineRec *lrec, const double *wlc_s,
                      const int4 *meta_s, const long long *loff, const int *list, int64_t mnull, float *lineop,
-                     int he, hipStream_t st) {
+                     int he, int ring, hipStream_t st) {
   int nw = (S.ntau + DPW - 1) / DPW;
   int nb = 8 * ((ntiles + 7) / 8);
   const int nq = (S.ntau + 15) / 16;
@@ -1554,7 +1553,8 @@ int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, cons
   switch (nq) {  // depth blocks of 16: up to 128 depths on the matrix cores
 #define FSG_MFMA_CASE(Q)                                                                                  \
   case Q:                                                                                                \
-    k_opacity32_mfma<Q><<<nb, 256, 0, st>>>(S, ntiles, loff, list, ca, lrec, mnull, lineop);              \
+    if (ring == 2) k_opacity32_mfma<Q, 2><<<nb, 256, 0, st>>>(S, ntiles, loff, list, ca, lrec, mnull, lineop); \
+    else k_opacity32_mfma<Q, 4><<<nb, 256, 0, st>>>(S, ntiles, loff, list, ca, lrec, mnull, lineop);          \
     return 1;
     FSG_MFMA_CASE(1)
     FSG_MFMA_CASE(2)
@@ -1574,7 +1574,7 @@ int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, cons
 void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
                        const int2 *zone_s, const double *wlc_s, unsigned long long *key, int *val,
                        unsigned long long *key_sorted, int *perm, ZoneRec *zrec, void *tmp, size_t tmp_bytes,
-                       int *zoff, int *zend, int2 *rng_core, unsigned long long *evals_core, hipStream_t st) {
+                       int *zoff, int *zend, unsigned long long *evals_core, hipStream_t st) {
   k_fill_int<<<1, 2 * NZB, 0, st>>>(zoff, 2 * NZB, 0);  // zoff and zend are adjacent
   if (m > 0) {
     k_zone_keys<<<nblk(m, 256), 256, 0, st>>>(m, mmain, ntau, meta_s, zone_s, key, val, evals_core);
@@ -1582,8 +1582,13 @@ void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, int ntau, 
     k_zone_bounds<<<nblk(m, 256), 256, 0, st>>>(m, key_sorted, zoff, zend);
     k_zone_records<<<nblk(m, 256), 256, 0, st>>>(m, perm, meta_s, zone_s, wlc_s, zrec);
   }
+}
+
+void launch_core_ranges(int core_pp, int nwl, const int *zoff, const int *zend, const unsigned long long *key_sorted,
+                        int2 *rng_core, hipStream_t st) {
+  const int tile = 64 * core_pp, ntiles = (nwl + tile - 1) / tile;
   dim3 grid(nblk(ntiles, 256), NZB - 1);
-  k_tile_ranges_zone<<<grid, 256, 0, st>>>(ntiles, TILE, nwl, zoff, zend, key_sorted, rng_core);
+  k_tile_ranges_zone<<<grid, 256, 0, st>>>(ntiles, tile, nwl, zoff, zend, key_sorted, rng_core);
 }
 
 void launch_zero_f32(float *p, int64_t n, hipStream_t st) {
@@ -1601,11 +1606,13 @@ size_t zone_sort_tmp_bytes(int64_t m) {
   return b;
 }
 
-void launch_opacity32_core(const DevSetup &S, int ntiles, const int2 *rng_core, const ZoneRec *zrec, const CoefA *ca,
+void launch_opacity32_core(const DevSetup &S, int core_pp, const int2 *rng_core, const ZoneRec *zrec, const CoefA *ca,
                            const CoefB *cb, const float *dmaxw, float *lineop, hipStream_t st) {
-  int nw = (S.ntau + DPW - 1) / DPW;
-  int nb = 8 * ((ntiles + 7) / 8);
-  k_opacity32_core<<<nb, 64 * nw, 0, st>>>(S, ntiles, rng_core, zrec, ca, cb, dmaxw, lineop);
+  const int nw = (S.ntau + DPW - 1) / DPW;
+  const int tile = 64 * core_pp, ntiles = (S.nwl + tile - 1) / tile;
+  const int nb = 8 * ((ntiles + 7) / 8);
+  if (core_pp == 1) k_opacity32_core<1><<<nb, 64 * nw, 0, st>>>(S, ntiles, rng_core, zrec, ca, cb, dmaxw, lineop);
+  else k_opacity32_core<2><<<nb, 64 * nw, 0, st>>>(S, ntiles, rng_core, zrec, ca, cb, dmaxw, lineop);
 }
 
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,

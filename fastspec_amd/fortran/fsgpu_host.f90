@@ -68,6 +68,7 @@ MODULE FsgpuHost
     REAL(C_DOUBLE) :: ms_pass1, ms_opacity
     INTEGER(C_INT64_T) :: evals_core
     REAL(C_DOUBLE) :: ms_far, ms_core
+    INTEGER(C_INT32_T) :: core_tile, far_ring
   END TYPE FsgStats
 
   INTERFACE

@@ -122,6 +122,9 @@ typedef struct fsg_lineop_stats {
    * core-zone kernels, from their first launch to their last (they run concurrently) */
   int64_t evals_core;
   double ms_far, ms_core;
+  /* FP32 batched path: the launch geometry the autotuner chose (points per core-zone
+   * tile, LDS ring slots of the far-wing kernel) */
+  int32_t core_tile, far_ring;
 } fsg_lineop_stats;
 
 typedef struct fsg_ctx fsg_ctx;
