@@ -90,6 +90,29 @@ def run_path(setup, lines, precision, batched, steps, warmup):
                 pass1_ms=float(np.mean(p1ms)), fnorm=sfl / sfc, conv_ms=conv_ms)
 
 
+def run_linstr(setup, lines):
+    """LinStrCalc for every retained line (SAVE flag 1: two DELO solves per mu at the line
+    centre) as the difference of two FP32 LineopCalc calls with and without it; both
+    upload the list and copy flags and windows back."""
+    from fastspec_amd.abi import FSG_FP32
+    from fastspec_amd.synth import FastSpecGPU
+
+    g = FastSpecGPU(setup, precision=FSG_FP32, batched=1)
+    g.LineopCalc(lines, linstr=True)
+    t = {}
+    for flag in (False, True, False, True):
+        g.reset_lineop()
+        t0 = time.perf_counter()
+        r = g.LineopCalc(lines, linstr=flag)
+        t.setdefault(flag, []).append(time.perf_counter() - t0)
+    g.close()
+    nret = int(r["stats"]["nlines_use"])
+    dt = (min(t[True]) - min(t[False])) * 1e3
+    return {"ms": dt, "lines": nret, "delo_solves": 2 * setup.nmu * nret,
+            "lines_per_s": nret / (dt * 1e-3) if dt > 0 else None,
+            "note": "difference of LineopCalc with and without LinStr (min of two calls each)"}
+
+
 def run_c5(steps: int, warmup: int):
     """The differentiable batch: 256 grey atmospheres with their own windows over one shared
     list of 50,000 lines (FP32 opacity, FP64 solver).  A step synthesises all 256 spectra
@@ -205,6 +228,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-linstr", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -271,6 +295,10 @@ def main():
                              "opacity_kernel_ms": r64["opacity_ms"]}
         out["speedup_vs_fp64_plain"] = r64["ms_per_step"] / r32["ms_per_step"]
         out["max_flux_err_vs_fp64"] = err
+    if not args.no_linstr:
+        log("LinStrCalc for the retained C2 lines")
+        out["linstr"] = run_linstr(setup, lines)
+        log(json.dumps(out["linstr"]))
     if not args.no_c5:
         log("differentiable batch (256 atmospheres)")
         out["c5_differentiable_batch"] = run_c5(1, 1)

@@ -593,15 +593,15 @@ __global__ void __launch_bounds__(1024) k_opacity32(DevSetup S, int ntiles, int 
 // and the core zones are added by k_opacity32_core.  v_mfma_f32_16x16x4_f32 takes 16 depths x
 // 4 lines (A, the CoefA records as loaded) times 4 lines x 16 points (B, computed per lane)
 // into a 16 x 16 FP32 block.  A workgroup owns a tile of TILE = 128 points and all depths (NQ
-// blocks of 16); wavefront w owns points 32w .. 32w+31 (two 16-point blocks).  Candidate lines
-// are compacted per wavefront with a ballot over 64 of them at a time and taken 4 per group;
-// the loads of the next group are in flight while a group runs.  Partial sums hold FOLDG
-// groups of lines and are then added to the running sums with Kahan compensation.
+// blocks of 16); wavefront w owns points 32w .. 32w+31 (two 16-point blocks).  The tile's
+// lines come from its list 4 per group; the four wavefronts share a ring of group slots in
+// LDS whose loads run ahead of the products.  Partial sums hold FOLDG groups of lines and
+// are then added to the running sums with Kahan compensation.
 // Lane layout of v_mfma_f32_16x16x4_f32: A[r][k] in lane 16k + r, B[k][c] in lane 16k + c,
 // D[4 (i / 16) + e][i % 16] in element e of lane i.
 typedef float mf4 __attribute__((ext_vector_type(4)));
 #ifndef FSG_FOLDG
-#define FSG_FOLDG 16
+#define FSG_FOLDG 32
 #endif
 constexpr int FOLDG = FSG_FOLDG;  // groups of 4 lines per partial sum
 
@@ -786,9 +786,10 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
   // line loads line 0's coefficients and the empty record at position mnull)
   constexpr int LW = NQ % 4;  // the wavefront that also loads the line records
   const int nla = (NQ - wave + 3) / 4;  // depth blocks wave, wave + 4, ... below NQ
+  // per-lane choice among a group's 4 indices as two selects on fixed lane bits
+  const bool kq1 = kq & 1, kq2 = kq & 2, ln1 = lane & 1, ln2 = lane & 2;
   auto fetch = [&](int s, const int4 jg) {
-    const int jn[4] = {jg.x, jg.y, jg.z, jg.w};
-    const int j = jn[kq];
+    const int j = kq2 ? (kq1 ? jg.w : jg.z) : (kq1 ? jg.y : jg.x);
     const mf4 *src = reinterpret_cast<const mf4 *>(ca + (size_t)(j >= 0 ? j : 0) * ntp + col);
     mf4 *slot = ring[s];
 #pragma unroll
@@ -798,7 +799,7 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
         __builtin_amdgcn_global_load_lds((const void *)(src + 16 * nb), (lds_void *)(slot + 64 * nb), 16, 0, 0);
     }
     if (wave == LW && lane < 8) {  // lane i: half i / 4 of line i % 4
-      const int jr = jn[lane & 3];
+      const int jr = ln2 ? (ln1 ? jg.w : jg.z) : (ln1 ? jg.y : jg.x);
       const mf4 *rs = reinterpret_cast<const mf4 *>(lrec + (jr >= 0 ? (int64_t)jr : mnull)) + (lane >> 2);
       __builtin_amdgcn_global_load_lds((const void *)rs, (lds_void *)(slot + 64 * NQ), 16, 0, 0);
     }
@@ -1242,18 +1243,16 @@ __global__ void __launch_bounds__(1024) k_opacity64(DevSetup S, int ntiles, int 
 // a sliding window of depth points; FP64 throughout.  Fused flux sums.
 // ------------------------------------------------------------------------------------
 
-// TauStep (src/solver.f90:591-615) element i (0-based, i <= N-2), Xmu = 1
-template <typename Col>
-__device__ __forceinline__ double tau_step_i(const double *rhox, const Col &col, int i) {
+// TauStep (src/solver.f90:591-615) element i (0-based, i <= N-2), Xmu = 1, from the
+// opacities xm, x0, xp at depths i - 1, i, i + 1 (i = 0 uses x0 and xp only)
+__device__ __forceinline__ double tau_step_v(const double *rhox, int i, double xm, double x0, double xp) {
   if (i == 0) {
     double Dx1 = (rhox[1] - rhox[0]) / 1.0;
-    double x0 = col.k(0), x1 = col.k(1);
-    double Yp = (x1 - x0) / Dx1;
-    return Dx1 * (x1 + 2.0 * x0 + Dx1 / 2.0 * Yp) / 3.0;
+    double Yp = (xp - x0) / Dx1;
+    return Dx1 * (xp + 2.0 * x0 + Dx1 / 2.0 * Yp) / 3.0;
   }
   double Dx = (rhox[i] - rhox[i - 1]) / 1.0;
   double Dx1 = (rhox[i + 1] - rhox[i]) / 1.0;
-  double xm = col.k(i - 1), x0 = col.k(i), xp = col.k(i + 1);
   double Der = (x0 - xm) / Dx, Der1 = (xp - x0) / Dx1, Yp;
   if (Der * Der1 > 0.0) {
     double Lambda = (1.0 + Dx1 / (Dx + Dx1)) / 3.0;
@@ -1265,7 +1264,8 @@ __device__ __forceinline__ double tau_step_i(const double *rhox, const Col &col,
 }
 
 // Emergent intensities at all mu of one opacity column (LocPrfPP, src/solver.f90:123-179),
-// streamed from the bottom up; col.k(d) is the opacity at depth d.
+// streamed from the bottom up; col.k(d) is the opacity at depth d, read once per depth (a
+// window of the opacities at the two depths above the last optical-depth step).
 template <typename Col>
 __device__ __forceinline__ void locprf_pp(const DevSetup &S, const Col &col, double C1, double C2, const double *imu,
                                           double *Y) {
@@ -1274,8 +1274,11 @@ __device__ __forceinline__ void locprf_pp(const DevSetup &S, const Col &col, dou
   double S_ip1 = planck_src(C1, C2, S.temp[N - 1]);
   double S_i = planck_src(C1, C2, S.temp[N - 2]);
   double S_im1 = (N >= 3) ? planck_src(C1, C2, S.temp[N - 3]) : 0.0;
-  double dt_i = tau_step_i(S.rhox, col, i);
-  double dt_im1 = (i >= 1) ? tau_step_i(S.rhox, col, i - 1) : 0.0;
+  const double kp = col.k(N - 1), k0 = col.k(N - 2);
+  double kb = (N >= 3) ? col.k(N - 3) : 0.0;  // k(i - 1)
+  double ka = (N >= 4) ? col.k(N - 4) : 0.0;  // k(i - 2)
+  double dt_i = tau_step_v(S.rhox, i, kb, k0, kp);
+  double dt_im1 = (i >= 1) ? tau_step_v(S.rhox, i - 1, ka, kb, k0) : 0.0;
   double Sd_ip1 = (S_ip1 - S_i) / dt_i;  // Srd(N)
   for (int m = 0; m < M; m++) Y[m] = S_ip1;
   for (; i >= 0; i--) {
@@ -1293,7 +1296,10 @@ __device__ __forceinline__ void locprf_pp(const DevSetup &S, const Col &col, dou
     dt_i = dt_im1;
     if (i >= 2) {
       S_im1 = planck_src(C1, C2, S.temp[i - 2]);
-      dt_im1 = tau_step_i(S.rhox, col, i - 2);
+      const double kn = (i >= 3) ? col.k(i - 3) : 0.0;
+      dt_im1 = tau_step_v(S.rhox, i - 2, kn, ka, kb);
+      kb = ka;
+      ka = kn;
     }
   }
 }
