@@ -427,17 +427,50 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
       for (int k = 0; k < DPW; k++) {
         const Coef64 ck = c[k];
         double s = 0.0;
+        bool act[PPL];
+        double v[PPL];
+        bool near = false;
+        // far-wing series of the one-pole fraction (the FP32 path's expansion): where every
+        // point of the wavefront has v^2 >= 400 (a^2 + 1/2), H = a/sqrt(pi) u (1 + f1 u + f2 u^2),
+        // u = 1/v^2, to 1e-7 relative, and its derivatives in closed form
+        const double a = ck.a, a2 = a * a, be = a2 + 0.5, al = 2.0 * a2 - 1.0;
+        const double thr = fmax(196.0, 400.0 * be);
 #pragma unroll
         for (int q = 0; q < PPL; q++) {
-          if (!(valid[q] && pp[q] >= m.x && pp[q] <= m.y)) continue;
+          act[q] = valid[q] && pp[q] >= m.x && pp[q] <= m.y;
           const double dld = hs ? (Wp[q] - wc) - hs[k] : (Wp[q] - wc);
-          const double v = dld * ck.vv;
-          double Hv, Ha;
-          const double H = hvoigt64_d(S, ck.a, ck.ia, ck.ua, v, Hv, Ha);
-          const double cval = H * ck.cop;
-          s += g[q][k] * cval;
-          // d c / dT = c d ln LinCop/dT + LinCop (Hv v d ln LinVvgt/dT + Ha d LinAvgt/dT)
-          tT[k] += g[q][k] * (cval * ck.pad1 + ck.cop * (Hv * v * ck.pad2 + Ha * ck.pad3));
+          v[q] = dld * ck.vv;
+          near |= act[q] && v[q] * v[q] < thr;
+        }
+        if (__builtin_amdgcn_ballot_w64(near) == 0) {
+          const double f1 = be - al, f2 = -al * f1 - be * be;
+          const double f1a = -2.0 * a, f2a = -4.0 * a * f1 + 2.0 * a * al - 4.0 * a * be;  // d f/da
+          const double A0 = kSqrtPiInv * a;
+#pragma unroll
+          for (int q = 0; q < PPL; q++) {
+            if (!act[q]) continue;
+            const double v2 = v[q] * v[q];
+            double u = __builtin_amdgcn_rcp(v2);
+            u = u * (2.0 - v2 * u);  // one Newton step
+            const double P = 1.0 + u * (f1 + f2 * u);
+            const double H = A0 * u * P;
+            const double Hv = -2.0 * A0 * u * (v[q] * u) * (1.0 + u * (2.0 * f1 + 3.0 * f2 * u));
+            const double Ha = kSqrtPiInv * u * P + A0 * u * u * (f1a + f2a * u);
+            const double cval = H * ck.cop;
+            s += g[q][k] * cval;
+            tT[k] += g[q][k] * (cval * ck.pad1 + ck.cop * (Hv * v[q] * ck.pad2 + Ha * ck.pad3));
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < PPL; q++) {
+            if (!act[q]) continue;
+            double Hv, Ha;
+            const double H = hvoigt64_d(S, ck.a, ck.ia, ck.ua, v[q], Hv, Ha);
+            const double cval = H * ck.cop;
+            s += g[q][k] * cval;
+            // d c / dT = c d ln LinCop/dT + LinCop (Hv v d ln LinVvgt/dT + Ha d LinAvgt/dT)
+            tT[k] += g[q][k] * (cval * ck.pad1 + ck.cop * (Hv * v[q] * ck.pad2 + Ha * ck.pad3));
+          }
         }
         s = wave_sum(s);
         if (lane == 0 && s != 0.0) atomicAdd(&sl[(size_t)j * S.ntp + d0 + k], s);
