@@ -232,6 +232,30 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Eight sums across the wavefront at once: each exchange level hands half of a lane's values
+// to its partner and keeps the other half, so the eight values need 4 + 2 + 1 exchanges and
+// then three more within groups of eight lanes.  On return lanes 8k .. 8k + 7 hold the sum of
+// value k over the wavefront.
+__device__ __forceinline__ double wave_sum8(const double (&s)[8], int lane) {
+  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+  double t4[4], t2[2];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double keep = h5 ? s[4 + i] : s[i], give = h5 ? s[i] : s[4 + i];
+    t4[i] = keep + __shfl_xor(give, 32, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const double keep = h4 ? t4[2 + i] : t4[i], give = h4 ? t4[i] : t4[2 + i];
+    t2[i] = keep + __shfl_xor(give, 16, 64);
+  }
+  double t = (h3 ? t2[1] : t2[0]) + __shfl_xor(h3 ? t2[0] : t2[1], 8, 64);
+  t += __shfl_xor(t, 4, 64);
+  t += __shfl_xor(t, 2, 64);
+  t += __shfl_xor(t, 1, 64);
+  return t;
+}
+
 // One lane per wavelength point of [p_lo, p_lo + np): the line column (cont + LineOpac,
 // adjoint gsfl) and the continuum column (adjoint gsfc).  Outputs dL/dLineOpac and
 // dL/dContOpac [ntau][nwl] and the source-function part of dL/dT [ntau] (accumulated).
@@ -423,6 +447,7 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
       const double wc = wlc_s[j];
       const Coef64 *c = coef + (size_t)j * S.ntp + d0;
       const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
+      double sk[DPW];  // per depth: this lane's share of the sum over the tile's points
 #pragma unroll
       for (int k = 0; k < DPW; k++) {
         const Coef64 ck = c[k];
@@ -472,9 +497,17 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
             tT[k] += g[q][k] * (cval * ck.pad1 + ck.cop * (Hv * v[q] * ck.pad2 + Ha * ck.pad3));
           }
         }
-        s = wave_sum(s);
-        if (lane == 0 && s != 0.0) atomicAdd(&sl[(size_t)j * S.ntp + d0 + k], s);
+        sk[k] = s;
       }
+      // the wavefront's 9 depths: 8 sums by one transposed reduction, the ninth alone
+      static_assert(DPW == 9, "eight depths in one reduction and one more");
+      double s8[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) s8[k] = sk[k];
+      const double t = wave_sum8(s8, lane);
+      if ((lane & 7) == 0 && t != 0.0) atomicAdd(&sl[(size_t)j * S.ntp + d0 + (lane >> 3)], t);
+      const double t8 = wave_sum(sk[8]);
+      if (lane == 0 && t8 != 0.0) atomicAdd(&sl[(size_t)j * S.ntp + d0 + 8], t8);
     }
   }
 #pragma unroll
