@@ -6,7 +6,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-from .abi import CLines, CSetup, CStats
+from .abi import CLines, CSetup, CSpecies, CStats
 
 LIB_PATH = Path(__file__).resolve().parent / "libfsgpu.so"
 
@@ -17,6 +17,7 @@ EXPORTS = (
     "fsg_magradtran", "fsg_flux", "fsg_lines_upload", "fsg_lines_free", "fsg_lineop_dev",
     "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop", "fsg_maglineop_dev", "fsg_lines_set_gf",
     "fsg_flux_vjp", "fsg_lineop_vjp", "fsg_integ_convol", "fsg_get_spectra",
+    "fsg_lines_scan", "fsg_lines_read", "fsg_lines_stream",
 )
 
 _LIB = None
@@ -62,6 +63,13 @@ def load() -> C.CDLL:
     lib.fsg_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp)]
     lib.fsg_stream.argtypes = [vp, C.POINTER(vp)]
     lib.fsg_reset_lineop.argtypes = [vp]
+    i64p = C.POINTER(C.c_int64)
+    sp = C.POINTER(CSpecies)
+    lib.fsg_lines_scan.argtypes = [C.c_char_p, C.c_double, C.c_double, ip, i64p, i64p]
+    lib.fsg_lines_read.argtypes = [C.c_char_p, C.c_int64, C.c_int64, sp, C.c_int32, C.c_int32, C.c_int32,
+                                   dp, fp, fp, fp, fp, fp, ip, C.POINTER(C.c_uint8), i64p]
+    lib.fsg_lines_stream.argtypes = [vp, C.c_char_p, C.c_double, C.c_double, sp, C.c_int32, C.c_int32,
+                                     C.c_int32, C.POINTER(vp), i64p]
     _LIB = lib
     return lib
 

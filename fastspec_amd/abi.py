@@ -51,6 +51,11 @@ class CSetup(C.Structure):
     ]
 
 
+class CSpecies(C.Structure):
+    """fsg_species_entry: (symbol, ion) -> species slot of the host's table."""
+    _fields_ = [("el", C.c_char * 8), ("ion", C.c_int32), ("slot", C.c_int32), ("is_mol", C.c_int32)]
+
+
 class CLines(C.Structure):
     _fields_ = [
         ("n", C.c_int64),

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/fsgpu.h"
+#include "fsg_ingest.h"
 #include "fsg_kernels.h"
 
 using namespace fsg;
@@ -645,6 +646,159 @@ int fsg_lines_upload(fsg_ctx *c, int32_t list_type, const fsg_lines *h, fsg_dlin
     return 1;
   }
   *out = d;
+  return 0;
+}
+
+static int species_table(const fsg_species_entry *species, int32_t nspecies, int nspc, SpeciesMap &m, bool &any_mol) {
+  any_mol = false;
+  if (nspecies < 0 || (nspecies > 0 && !species)) return fail("fsg_lines: species table is NULL");
+  for (int32_t i = 0; i < nspecies; i++) {
+    const fsg_species_entry &e = species[i];
+    std::string el(e.el, strnlen(e.el, sizeof(e.el)));
+    while (!el.empty() && el.back() == ' ') el.pop_back();
+    while (!el.empty() && el.front() == ' ') el.erase(el.begin());
+    if (e.slot < 1 || (nspc > 0 && e.slot > nspc)) return fail("LineopCalc: species index out of range");
+    m[SpeciesKey(el, e.ion)] = {e.slot, (uint8_t)(e.is_mol != 0)};
+    any_mol |= e.is_mol != 0;
+  }
+  return 0;
+}
+
+int fsg_lines_scan(const char *dir, double wstart, double wend, int32_t *list_type, int64_t *first, int64_t *count) {
+  g_err.clear();
+  if (!dir || !first || !count) return fail("fsg_lines_scan: NULL argument");
+  Sidecar sc;
+  std::string err;
+  if (sidecar_open(dir, sc, err) || sidecar_select(sc, wstart, wend, *first, *count, err)) return fail(err);
+  if (list_type) *list_type = sc.molecular ? FSG_LIST_MOLECULAR : FSG_LIST_ATOMIC;
+  return 0;
+}
+
+int fsg_lines_read(const char *dir, int64_t first, int64_t count, const fsg_species_entry *species, int32_t nspecies,
+                   int32_t mol_slot, int32_t nthreads, double *wl, float *elo, float *gf, float *grad, float *gqst,
+                   float *gvdw, int32_t *spc, uint8_t *is_mol, int64_t *kept) {
+  g_err.clear();
+  if (!dir || !kept || (count > 0 && (!wl || !elo || !gf || !grad || !spc))) return fail("fsg_lines_read: NULL argument");
+  Sidecar sc;
+  std::string err;
+  if (sidecar_open(dir, sc, err)) return fail(err);
+  if (first < 0 || count < 0 || first + count > sc.nl) return fail("fsg_lines_read: lines outside the file");
+  if (!sc.molecular && count > 0 && (!gqst || !gvdw)) return fail("fsg_lines_read: atomic lists need Gqst and Gvdw");
+  SpeciesMap m;
+  bool any_mol = false;
+  if (species_table(species, nspecies, 0, m, any_mol)) return 1;
+  IngestOut o{wl, elo, gf, grad, gqst, gvdw, spc, is_mol};
+  if (sidecar_convert(sc, first, count, m, mol_slot, nthreads, o, *kept, err)) return fail(err);
+  return 0;
+}
+
+int fsg_lines_stream(fsg_ctx *c, const char *dir, double wstart, double wend, const fsg_species_entry *species,
+                     int32_t nspecies, int32_t mol_slot, int32_t nthreads, fsg_dlines **out, int64_t *kept) {
+  g_err.clear();
+  if (!c || !dir || !out || !kept) return fail("fsg_lines_stream: NULL argument");
+  *out = nullptr;
+  *kept = 0;
+  Sidecar sc;
+  std::string err;
+  int64_t first = 0, count = 0;
+  if (sidecar_open(dir, sc, err) || sidecar_select(sc, wstart, wend, first, count, err)) return fail(err);
+  SpeciesMap m;
+  bool any_mol = false;
+  if (species_table(species, nspecies, c->S.nspc, m, any_mol)) return 1;
+  if (sc.molecular && (mol_slot < 1 || mol_slot > c->S.nspc)) return fail("LineopCalc: species index out of range");
+  CK(hipSetDevice(c->dev));
+  fsg_dlines *d = new fsg_dlines();
+  d->ctx = c;
+  d->list_type = sc.molecular ? FSG_LIST_MOLECULAR : (any_mol ? FSG_LIST_MIXED : FSG_LIST_ATOMIC);
+  const size_t nmax = (size_t)std::max<int64_t>(count, 1);
+  double *dwl = nullptr;
+  float *delo = nullptr, *dgf = nullptr, *dgrad = nullptr, *dgq = nullptr, *dgv = nullptr;
+  int32_t *dspc = nullptr;
+  uint8_t *dmol = nullptr;
+  auto dalloc = [&](void **p, size_t bytes) -> bool {
+    if (hipMalloc(p, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
+    d->owned.push_back(*p);
+    return true;
+  };
+  bool ok = dalloc((void **)&dwl, nmax * 8) && dalloc((void **)&delo, nmax * 4) && dalloc((void **)&dgf, nmax * 4) &&
+            dalloc((void **)&dgrad, nmax * 4) && dalloc((void **)&dspc, nmax * 4);
+  if (ok && !sc.molecular) ok = dalloc((void **)&dgq, nmax * 4) && dalloc((void **)&dgv, nmax * 4);
+  if (ok && d->list_type == FSG_LIST_MIXED) ok = dalloc((void **)&dmol, nmax);
+  if (!ok) {
+    fsg_lines_free(d);
+    return fail("fsg_lines_stream: out of device memory for the line list");
+  }
+  // two pinned buffer sets; chunk k converts into set k % 2 while set (k - 1) % 2 copies
+  int64_t cl = 1LL << 22;  // lines per pinned chunk; FSG_INGEST_CHUNK overrides it
+  if (const char *e = std::getenv("FSG_INGEST_CHUNK")) cl = std::max<int64_t>(1, std::atoll(e));
+  const int64_t chunk = std::min<int64_t>(std::max<int64_t>(count, 1), cl);
+  const size_t per = (size_t)chunk * (8 + 4 * 6 + 1);
+  char *pin[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  int rc = 0;
+  for (int b = 0; b < 2 && !rc; b++) {
+    if (hipHostMalloc((void **)&pin[b], per, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&done[b], hipEventDisableTiming) != hipSuccess)
+      rc = fail("fsg_lines_stream: pinned staging buffers unavailable");
+  }
+  int64_t total = 0;
+  for (int64_t k = 0, off = first; !rc && off < first + count; k++, off += chunk) {
+    const int b = (int)(k & 1);
+    if (k >= 2 && hipEventSynchronize(done[b]) != hipSuccess) {
+      rc = fail("fsg_lines_stream: copy to the device failed");
+      break;
+    }
+    char *p = pin[b];
+    IngestOut o;
+    o.wl = reinterpret_cast<double *>(p);
+    o.elo = reinterpret_cast<float *>(p + chunk * 8);
+    o.gf = o.elo + chunk;
+    o.grad = o.gf + chunk;
+    o.gqst = o.grad + chunk;
+    o.gvdw = o.gqst + chunk;
+    o.spc = reinterpret_cast<int32_t *>(o.gvdw + chunk);
+    o.is_mol = reinterpret_cast<uint8_t *>(o.spc + chunk);
+    const int64_t n = std::min<int64_t>(chunk, first + count - off);
+    int64_t kn = 0;
+    if (sidecar_convert(sc, off, n, m, mol_slot, nthreads, o, kn, err)) {
+      rc = fail(err);
+      break;
+    }
+    hipStream_t s = c->st;
+    bool cp = hipMemcpyAsync(dwl + total, o.wl, kn * 8, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(delo + total, o.elo, kn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(dgf + total, o.gf, kn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(dgrad + total, o.grad, kn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(dspc + total, o.spc, kn * 4, hipMemcpyHostToDevice, s) == hipSuccess;
+    if (cp && dgq) cp = hipMemcpyAsync(dgq + total, o.gqst, kn * 4, hipMemcpyHostToDevice, s) == hipSuccess &&
+                        hipMemcpyAsync(dgv + total, o.gvdw, kn * 4, hipMemcpyHostToDevice, s) == hipSuccess;
+    if (cp && dmol) cp = hipMemcpyAsync(dmol + total, o.is_mol, kn, hipMemcpyHostToDevice, s) == hipSuccess;
+    if (!cp || hipEventRecord(done[b], s) != hipSuccess) {
+      rc = fail("fsg_lines_stream: copy to the device failed");
+      break;
+    }
+    total += kn;
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess && !rc) rc = fail("fsg_lines_stream: copy to the device failed");
+  for (int b = 0; b < 2; b++) {
+    if (pin[b]) (void)hipHostFree(pin[b]);
+    if (done[b]) (void)hipEventDestroy(done[b]);
+  }
+  if (rc) {
+    fsg_lines_free(d);
+    return 1;
+  }
+  d->L.n = total;
+  d->L.wl = dwl;
+  d->L.elo = delo;
+  d->L.gf = dgf;
+  d->L.grad = dgrad;
+  d->L.spc = dspc;
+  d->L.gqst = dgq;
+  d->L.gvdw = dgv;
+  d->L.is_mol = dmol;
+  *out = d;
+  *kept = total;
   return 0;
 }
 

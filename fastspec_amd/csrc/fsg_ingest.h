@@ -1,0 +1,56 @@
+// fsg_ingest.h -- host side of the line-list sidecar reader (no device code).
+//
+// A sidecar is a directory holding the /Lines group of a FastSpec HDF5 line list as raw
+// little-endian column files (<COLUMN>.bin, one value per line, in file order) and a
+// header.txt of "key=value" lines:
+//   format=fastspec-lines-1
+//   name=atom | <molecule name>     (the NAME attribute; 'atom...' marks an atomic list)
+//   wl_vac=0|1  wmin=<A>  wmax=<A>  nl=<lines>  hflag=<int>
+//   columns=W:f8,EL:S3,ION:i2,E:f4,LGGF:f4,GRAD:f4,GQST:f4,GVDW:f4
+// Selection and conversions follow ReadHDF (src/ioutils.f90:545-1100).
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace fsg {
+
+struct SidecarCol {
+  char kind = 0;  // 'f' float, 'i' integer, 'S' fixed-length string
+  int size = 0;   // bytes per value
+};
+
+struct Sidecar {
+  std::string dir, name;
+  int wl_vac = 0, hflag = 0;
+  bool molecular = false;
+  double wmin = 0.0, wmax = 0.0;
+  int64_t nl = 0;
+  std::map<std::string, SidecarCol> cols;
+};
+
+// (element symbol, ion) -> (species slot, molecule flag) of the host's species table
+using SpeciesKey = std::pair<std::string, int>;
+using SpeciesMap = std::map<SpeciesKey, std::pair<int32_t, uint8_t>>;
+
+// converted columns of the selected lines (the caller owns the arrays, count entries each)
+struct IngestOut {
+  double *wl = nullptr;
+  float *elo = nullptr, *gf = nullptr, *grad = nullptr, *gqst = nullptr, *gvdw = nullptr;
+  int32_t *spc = nullptr;
+  uint8_t *is_mol = nullptr;
+};
+
+int sidecar_open(const char *dir, Sidecar &sc, std::string &err);
+// lines [first, first + count) of the file are inside [wstart, wend] (0-based first)
+int sidecar_select(const Sidecar &sc, double wstart, double wend, int64_t &first, int64_t &count, std::string &err);
+// reads and converts lines [first, first + count) with nthreads threads into out (packed,
+// lines above the EOS ion limit dropped); kept receives the number written
+int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const SpeciesMap &species, int32_t mol_slot,
+                    int nthreads, const IngestOut &out, int64_t &kept, std::string &err);
+double vac2air(double w);
+
+}  // namespace fsg

@@ -1,0 +1,287 @@
+// fsg_ingest.hip -- the line-list sidecar reader (host code only).
+//
+// Reference routines restated here (file:line in the reference tree):
+//   ReadHDF: range cull, W read, sort check, bounds, column slices, ion filter and the
+//            conversions of gf and the damping constants   src/ioutils.f90:696-1076
+//   LowerBoundReal8, UpperBoundReal8                       src/utils.f90:701-737
+//   Vac2Air                                                src/utils.f90:1007-1017
+// The columns are read with pread from raw files by several threads (the reference reads
+// HDF5 hyperslabs); every other step keeps the reference's order and REAL(4) arithmetic.
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <thread>
+
+#include "fsg_ingest.h"
+
+namespace fsg {
+
+namespace {
+
+constexpr int kMaxIon = 5;  // Maxion, src/shared.f90:434
+
+// reads count values of size bytes from column file <dir>/<name>.bin starting at value first
+bool read_col(const Sidecar &sc, const std::string &name, int64_t first, int64_t count, void *dst, std::string &err) {
+  auto it = sc.cols.find(name);
+  if (it == sc.cols.end()) {
+    err = "fsg_lines: column " + name + " missing in " + sc.dir;
+    return false;
+  }
+  const int sz = it->second.size;
+  const std::string fn = sc.dir + "/" + name + ".bin";
+  const int fd = ::open(fn.c_str(), O_RDONLY);
+  if (fd < 0) {
+    err = "fsg_lines: cannot open " + fn;
+    return false;
+  }
+  char *p = static_cast<char *>(dst);
+  int64_t off = first * sz, left = count * sz;
+  while (left > 0) {
+    const ssize_t r = ::pread(fd, p, (size_t)std::min<int64_t>(left, 1LL << 30), (off_t)off);
+    if (r <= 0) {
+      ::close(fd);
+      err = "fsg_lines: short read of " + fn;
+      return false;
+    }
+    p += r;
+    off += r;
+    left -= r;
+  }
+  ::close(fd);
+  return true;
+}
+
+std::string trim_el(const char *s, int n) {  // ADJUSTL(TRIM(El)): blanks and NULs removed
+  int a = 0, b = n;
+  while (b > a && (s[b - 1] == ' ' || s[b - 1] == '\0')) b--;
+  while (a < b && s[a] == ' ') a++;
+  return std::string(s + a, s + b);
+}
+
+// REAL(4) 10.0**x as the reference's compiler evaluates it: the power of ten becomes the C
+// library's exp10f (the same bits as flang's scalar code; powf differs in about one value
+// in five by one ulp)
+inline float pow10f(float x) { return ::exp10f(x); }
+
+}  // namespace
+
+double vac2air(double w) {
+  const double S = 1e4 / w, S2 = S * S;
+  const double N = 1.0 + 8.34254e-5 + 2.406147e-2 / (130.0 - S2) + 1.5998e-4 / (38.9 - S2);
+  return w > 2000.0 ? w / N : w;
+}
+
+int sidecar_open(const char *dir, Sidecar &sc, std::string &err) {
+  sc = Sidecar();
+  sc.dir = dir ? dir : "";
+  std::ifstream in(sc.dir + "/header.txt");
+  if (!in) {
+    err = "fsg_lines: " + sc.dir + " has no header.txt";
+    return 1;
+  }
+  std::map<std::string, std::string> kv;
+  std::string line;
+  while (std::getline(in, line)) {
+    const size_t e = line.find('=');
+    if (e == std::string::npos) continue;
+    kv[line.substr(0, e)] = line.substr(e + 1);
+  }
+  if (kv["format"] != "fastspec-lines-1") {
+    err = "fsg_lines: " + sc.dir + " is not a FastSpec line-list sidecar";
+    return 1;
+  }
+  try {
+    sc.name = kv["name"];
+    sc.wl_vac = std::stoi(kv.count("wl_vac") ? kv["wl_vac"] : "0");
+    sc.hflag = std::stoi(kv.count("hflag") ? kv["hflag"] : "0");
+    sc.wmin = std::stod(kv.at("wmin"));
+    sc.wmax = std::stod(kv.at("wmax"));
+    sc.nl = std::stoll(kv.at("nl"));
+    std::stringstream cs(kv.at("columns"));
+    std::string item;
+    while (std::getline(cs, item, ',')) {
+      const size_t c = item.find(':');
+      if (c == std::string::npos || c + 2 > item.size()) throw std::invalid_argument("column");
+      SidecarCol col;
+      col.kind = item[c + 1];
+      col.size = std::stoi(item.substr(c + 2));
+      if ((col.kind != 'f' && col.kind != 'i' && col.kind != 'S') || col.size <= 0) throw std::invalid_argument("type");
+      sc.cols[item.substr(0, c)] = col;
+    }
+  } catch (...) {
+    err = "fsg_lines: malformed header.txt in " + sc.dir;
+    return 1;
+  }
+  sc.molecular = sc.name.compare(0, 4, "atom") != 0;
+  const std::vector<std::string> need =
+      sc.molecular ? std::vector<std::string>{"W", "E", "LGGF", "GRAD"}
+                   : std::vector<std::string>{"W", "EL", "ION", "E", "LGGF", "GRAD", "GQST", "GVDW"};
+  for (const auto &n : need)
+    if (!sc.cols.count(n)) {
+      err = "fsg_lines: column " + n + " missing in " + sc.dir;
+      return 1;
+    }
+  const auto ck = [&](const char *n, char k, int s) {
+    const SidecarCol &c = sc.cols.at(n);
+    return c.kind == k && c.size == s;
+  };
+  bool ok = ck("W", 'f', 8) && ck("E", 'f', 4) && ck("LGGF", 'f', 4) && ck("GRAD", 'f', 4);
+  if (!sc.molecular)
+    ok = ok && ck("GQST", 'f', 4) && ck("GVDW", 'f', 4) && sc.cols.at("ION").kind == 'i' &&
+         (sc.cols.at("ION").size == 2 || sc.cols.at("ION").size == 4) && sc.cols.at("EL").kind == 'S';
+  if (!ok) {
+    err = "fsg_lines: unsupported column types in " + sc.dir;
+    return 1;
+  }
+  if (sc.cols.count("ISOT") || sc.cols.count("JLO")) {
+    err = "fsg_lines: isotope and Zeeman columns are not read from sidecars (" + sc.dir + ")";
+    return 1;
+  }
+  return 0;
+}
+
+int sidecar_select(const Sidecar &sc, double wstart, double wend, int64_t &first, int64_t &count, std::string &err) {
+  first = 0;
+  count = 0;
+  // whole-file cull on the WMIN/WMAX attributes, converted to air first (ioutils.f90:696-727)
+  double wmin = sc.wmin, wmax = sc.wmax;
+  if (sc.wl_vac == 1) {
+    wmin = vac2air(wmin);
+    wmax = vac2air(wmax);
+  }
+  if (wmax < wstart || wmin > wend || sc.nl <= 0) return 0;
+  // the full W column, converted, checked for order (ioutils.f90:729-749)
+  std::vector<double> W((size_t)sc.nl);
+  if (!read_col(sc, "W", 0, sc.nl, W.data(), err)) return 1;
+  if (sc.wl_vac == 1)
+    for (double &w : W) w = vac2air(w);
+  for (int64_t i = 1; i < sc.nl; i++)
+    if (!(W[i - 1] <= W[i])) {
+      err = "ReadHDF: wavelength array is not sorted in " + sc.dir;
+      return 1;
+    }
+  // LowerBoundReal8 / UpperBoundReal8 (utils.f90:701-737), 1-based I1, I2
+  const int64_t i1 = (int64_t)(std::lower_bound(W.begin(), W.end(), wstart) - W.begin()) + 1;
+  const int64_t i2 = (int64_t)(std::upper_bound(W.begin(), W.end(), wend) - W.begin());
+  if (i1 > i2 || i1 > sc.nl || i2 < 1) return 0;
+  first = i1 - 1;
+  count = i2 - i1 + 1;
+  return 0;
+}
+
+int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const SpeciesMap &species, int32_t mol_slot,
+                    int nthreads, const IngestOut &out, int64_t &kept, std::string &err) {
+  kept = 0;
+  if (count <= 0) return 0;
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(nthreads, 1), (count + 65535) / 65536));
+  struct Part {
+    int64_t a = 0, n = 0, kept = 0;
+    std::vector<double> wl;
+    std::vector<float> elo, gf, grad, gqst, gvdw;
+    std::vector<int32_t> spc;
+    std::vector<uint8_t> mol;
+    std::string err;
+  };
+  std::vector<Part> parts((size_t)nt);
+  for (int t = 0; t < nt; t++) {
+    parts[t].a = first + count * t / nt;
+    parts[t].n = first + count * (t + 1) / nt - parts[t].a;
+  }
+  const int elsz = sc.molecular ? 0 : sc.cols.at("EL").size;
+  const int ionsz = sc.molecular ? 0 : sc.cols.at("ION").size;
+  auto work = [&](Part &P) {
+    const int64_t n = P.n;
+    std::vector<double> W((size_t)n);
+    std::vector<float> E((size_t)n), G((size_t)n), R((size_t)n), Q, V;
+    std::vector<char> EL;
+    std::vector<char> ION;
+    bool ok = read_col(sc, "W", P.a, n, W.data(), P.err) && read_col(sc, "E", P.a, n, E.data(), P.err) &&
+              read_col(sc, "LGGF", P.a, n, G.data(), P.err) && read_col(sc, "GRAD", P.a, n, R.data(), P.err);
+    if (ok && !sc.molecular) {
+      Q.resize((size_t)n);
+      V.resize((size_t)n);
+      EL.resize((size_t)n * elsz);
+      ION.resize((size_t)n * ionsz);
+      ok = read_col(sc, "GQST", P.a, n, Q.data(), P.err) && read_col(sc, "GVDW", P.a, n, V.data(), P.err) &&
+           read_col(sc, "EL", P.a, n, EL.data(), P.err) && read_col(sc, "ION", P.a, n, ION.data(), P.err);
+    }
+    if (!ok) return;
+    P.wl.reserve((size_t)n);
+    P.elo.reserve((size_t)n);
+    P.gf.reserve((size_t)n);
+    P.grad.reserve((size_t)n);
+    P.spc.reserve((size_t)n);
+    if (!sc.molecular) {
+      P.gqst.reserve((size_t)n);
+      P.gvdw.reserve((size_t)n);
+      P.mol.reserve((size_t)n);
+    }
+    for (int64_t i = 0; i < n; i++) {
+      double w = W[(size_t)i];
+      if (sc.wl_vac == 1) w = vac2air(w);
+      if (sc.molecular) {  // LinesM: Grad as stored, gf = 10**LGGF (ioutils.f90:806-813)
+        P.wl.push_back(w);
+        P.elo.push_back(E[(size_t)i]);
+        P.gf.push_back(pow10f(G[(size_t)i]));
+        P.grad.push_back(R[(size_t)i]);
+        P.spc.push_back(mol_slot);
+        continue;
+      }
+      const int ion = ionsz == 2 ? (int)reinterpret_cast<const int16_t *>(ION.data())[i]
+                                 : (int)reinterpret_cast<const int32_t *>(ION.data())[i];
+      if (ion > kMaxIon) continue;  // ions above the EOS limit (ioutils.f90:921-923)
+      const std::string el = trim_el(EL.data() + (size_t)i * elsz, elsz);
+      auto it = species.find(SpeciesKey(el, ion));
+      if (it == species.end()) {
+        P.err = "fsg_lines: species " + el + " " + std::to_string(ion) + " of " + sc.dir +
+                " is not in the species table";
+        return;
+      }
+      // LinesA (ioutils.f90:1056-1076): gf = 10**LGGF; GRAD 10** when 0 < x < 20; GQST and
+      // GVDW 10** when negative
+      const float g = R[(size_t)i], q = Q[(size_t)i], v = V[(size_t)i];
+      P.wl.push_back(w);
+      P.elo.push_back(E[(size_t)i]);
+      P.gf.push_back(pow10f(G[(size_t)i]));
+      P.grad.push_back((g < 20.0f && g > 0.0f) ? pow10f(g) : g);
+      P.gqst.push_back(q < 0.0f ? pow10f(q) : q);
+      P.gvdw.push_back(v < 0.0f ? pow10f(v) : v);
+      P.spc.push_back(it->second.first);
+      P.mol.push_back(it->second.second);
+    }
+    P.kept = (int64_t)P.wl.size();
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) th.emplace_back(work, std::ref(parts[t]));
+  work(parts[0]);
+  for (auto &x : th) x.join();
+  for (const Part &P : parts)
+    if (!P.err.empty()) {
+      err = P.err;
+      return 1;
+    }
+  int64_t o = 0;
+  for (const Part &P : parts) {
+    const size_t k = (size_t)P.kept;
+    std::memcpy(out.wl + o, P.wl.data(), k * sizeof(double));
+    std::memcpy(out.elo + o, P.elo.data(), k * sizeof(float));
+    std::memcpy(out.gf + o, P.gf.data(), k * sizeof(float));
+    std::memcpy(out.grad + o, P.grad.data(), k * sizeof(float));
+    std::memcpy(out.spc + o, P.spc.data(), k * sizeof(int32_t));
+    if (!sc.molecular) {
+      if (out.gqst) std::memcpy(out.gqst + o, P.gqst.data(), k * sizeof(float));
+      if (out.gvdw) std::memcpy(out.gvdw + o, P.gvdw.data(), k * sizeof(float));
+      if (out.is_mol) std::memcpy(out.is_mol + o, P.mol.data(), k);
+    }
+    o += P.kept;
+  }
+  kept = o;
+  return 0;
+}
+
+}  // namespace fsg

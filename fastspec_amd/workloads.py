@@ -243,6 +243,33 @@ def atomic_lines(n: int, wl_lo: float, wl_hi: float, seed: int, log_uniform: boo
     )
 
 
+# (symbol, ion, species slot, molecule) of ATOMIC_SPECIES, for sidecar line lists
+ATOMIC_SPECIES_TABLE = [(name[:-1], ion, k + 1, False) for k, (name, _, ion, *_r) in enumerate(ATOMIC_SPECIES)]
+
+
+def atomic_columns(n: int, wl_lo: float, wl_hi: float, seed: int, log_uniform: bool = False,
+                   nspecies: int = len(ATOMIC_SPECIES)) -> dict:
+    """The /Lines columns (EL, ION, W, E, LGGF, GRAD, GQST, GVDW; log10 values as stored in an
+    HDF5 list) of atomic_lines(n, wl_lo, wl_hi, seed, ...): the same draws, before the
+    reader's conversions."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    if log_uniform:
+        wl = np.exp(rng.uniform(math.log(wl_lo), math.log(wl_hi), n))
+    else:
+        wl = rng.uniform(wl_lo, wl_hi, n)
+    wl.sort()
+    lggf = rng.uniform(-5.0, 0.0, n)
+    elo = rng.uniform(0.0, 6.0, n)
+    spc = rng.integers(1, nspecies + 1, n)
+    grad = rng.normal(8.0, 0.3, n)
+    gqst = rng.normal(-6.0, 0.5, n)
+    gvdw = rng.normal(-7.5, 0.3, n)
+    el = np.array([ATOMIC_SPECIES_TABLE[k - 1][0].encode() for k in range(1, nspecies + 1)], "S3")[spc - 1]
+    ion = np.array([ATOMIC_SPECIES_TABLE[k - 1][1] for k in range(1, nspecies + 1)], np.int16)[spc - 1]
+    return {"W": wl, "EL": el, "ION": ion, "E": elo.astype(np.float32), "LGGF": lggf.astype(np.float32),
+            "GRAD": grad.astype(np.float32), "GQST": gqst.astype(np.float32), "GVDW": gvdw.astype(np.float32)}
+
+
 def molecular_lines(n: int, wl_lo: float, wl_hi: float, seed: int, spc: int = 1) -> LineList:
     """Seeded single-species diatomic list: log gf ~ N(-4, 1.5), E_low ~ Exp(1 eV)."""
     rng = np.random.Generator(np.random.PCG64(seed))

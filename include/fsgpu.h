@@ -218,6 +218,40 @@ int fsg_maglineop_dev(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, const do
  * gradient-based fitting, where gf changes between syntheses. */
 int fsg_lines_set_gf(fsg_dlines *dl, const float *gf, int32_t on_device);
 
+/* Line-list ingest from a sidecar directory: the /Lines group of a FastSpec HDF5 list as
+ * raw little-endian column files <COLUMN>.bin (W f8, EL S<k>, ION i2, E, LGGF, GRAD, GQST,
+ * GVDW f4; W, E, LGGF, GRAD for molecular lists) and a header.txt of key=value lines
+ * (format=fastspec-lines-1, name, wl_vac, wmin, wmax, nl, hflag, columns=NAME:type,...).
+ * Replaces ReadHDF (src/ioutils.f90:545-1100) for atomic and molecular lists: whole-file
+ * cull on WMIN/WMAX, Vac2Air when WL_VAC = 1, the sort check on the full W column, the
+ * LowerBoundReal8/UpperBoundReal8 selection of [wstart, wend], the ion filter (ION <= 5)
+ * and the REAL(4) conversions gf = 10**LGGF, GRAD 10** for 0 < x < 20, GQST and GVDW 10**
+ * when negative.  Species (EL, ION) map to the host's species slots through a table; a
+ * molecular list takes mol_slot.  Isotope and Zeeman columns are not read. */
+typedef struct fsg_species_entry {
+  char el[8];       /* element or molecule symbol, blank- or NUL-padded        */
+  int32_t ion;      /* ionisation stage, 1 = neutral                           */
+  int32_t slot;     /* 1-based species slot (LinSpc)                           */
+  int32_t is_mol;   /* IsMolecule(El): the list becomes a mixed list           */
+} fsg_species_entry;
+/* the selection only: 0-based first line and line count inside [wstart, wend];
+ * list_type receives FSG_LIST_MOLECULAR or FSG_LIST_ATOMIC.  Host only. */
+int fsg_lines_scan(const char *dir, double wstart, double wend, int32_t *list_type, int64_t *first,
+                   int64_t *count);
+/* lines [first, first + count) converted into caller arrays of count entries (gqst, gvdw,
+ * is_mol may be NULL for molecular lists), read by nthreads host threads; kept receives
+ * the lines written after the ion filter.  Host only. */
+int fsg_lines_read(const char *dir, int64_t first, int64_t count, const fsg_species_entry *species,
+                   int32_t nspecies, int32_t mol_slot, int32_t nthreads, double *wl, float *elo, float *gf,
+                   float *grad, float *gqst, float *gvdw, int32_t *spc, uint8_t *is_mol, int64_t *kept);
+/* scan + read straight into a new device list: chunks of up to 2^22 lines (environment
+ * FSG_INGEST_CHUNK overrides the count) are converted by
+ * nthreads host threads into one of two pinned buffers while the other one's columns copy
+ * to HBM on the library stream.  No counterpart in the reference (whose list stays on the
+ * host); the device list is the one fsg_lines_upload would make from the same lines. */
+int fsg_lines_stream(fsg_ctx *ctx, const char *dir, double wstart, double wend, const fsg_species_entry *species,
+                     int32_t nspecies, int32_t mol_slot, int32_t nthreads, fsg_dlines **out, int64_t *kept);
+
 /* Reverse mode (no counterpart in the reference, which has no derivatives).  For a loss
  * L(SfL, SfC) of the flux sums of the last plane-parallel fsg_radtran:
  *   fsg_flux_vjp    from dL/dSfL, dL/dSfC [nwl] computes and keeps on the device

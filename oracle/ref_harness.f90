@@ -561,6 +561,29 @@ CONTAINS
     ref_semi_area = SemiArea(r, v)
   END FUNCTION ref_semi_area
 
+  ! Vac2Air and the bound searches of ReadHDF (src/utils.f90:701-737, 1007-1017)
+  REAL(C_DOUBLE) FUNCTION ref_vac2air(w) BIND(C, NAME='ref_vac2air')
+    REAL(C_DOUBLE), VALUE :: w
+    ref_vac2air = Vac2Air(w)
+  END FUNCTION ref_vac2air
+
+  ! REAL(4) 10.0**x as the reference's compiler evaluates it in scalar code (the line loop
+  ! of ReadHDF, src/ioutils.f90:1056-1076); one call per value keeps it scalar
+  SUBROUTINE ref_pow10(x, y) BIND(C, NAME='ref_pow10')
+    REAL(C_FLOAT), VALUE :: x
+    REAL(C_FLOAT), INTENT(OUT) :: y
+    y = 10.0**x
+  END SUBROUTINE ref_pow10
+
+  SUBROUTINE ref_bounds(n, a, x1, x2, i1, i2) BIND(C, NAME='ref_bounds')
+    INTEGER(C_INT), VALUE :: n
+    REAL(C_DOUBLE), INTENT(IN) :: a(n)
+    REAL(C_DOUBLE), VALUE :: x1, x2
+    INTEGER(C_INT), INTENT(OUT) :: i1, i2
+    i1 = LowerBoundReal8(a, x1)
+    i2 = UpperBoundReal8(a, x2)
+  END SUBROUTINE ref_bounds
+
   SUBROUTINE ref_mu_quadrature_r(quad, n, x, w, r) BIND(C, NAME='ref_mu_quadrature_r')
     INTEGER(C_INT), VALUE :: quad, n
     REAL(C_DOUBLE), INTENT(INOUT) :: x(n)

@@ -302,6 +302,10 @@ def ref_lib() -> C.CDLL:
         lib.ref_nthreads.restype = C.c_int32
         lib.ref_semi_area.restype = C.c_double
         lib.ref_semi_area.argtypes = [C.c_double, C.c_double]
+        lib.ref_pow10.argtypes = [C.c_float, C.POINTER(C.c_float)]
+        lib.ref_vac2air.restype = C.c_double
+        lib.ref_vac2air.argtypes = [C.c_double]
+        lib.ref_bounds.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         lib.ref_convolgau.argtypes = [C.c_int, _dp, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double]
         lib.ref_diskinteg.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
                                       C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp]
