@@ -706,6 +706,38 @@ int fsg_lines_stream(fsg_ctx *c, const char *dir, double wstart, double wend, co
   bool any_mol = false;
   if (species_table(species, nspecies, c->S.nspc, m, any_mol)) return 1;
   if (sc.molecular && (mol_slot < 1 || mol_slot > c->S.nspc)) return fail("LineopCalc: species index out of range");
+  if (sc.zeeman) {
+    // magnetic lists (a few thousand lines) are converted on the host with their Zeeman
+    // patterns and uploaded as fsg_lines_upload does
+    const size_t n = (size_t)std::max<int64_t>(count, 1);
+    std::vector<double> wl(n);
+    std::vector<float> elo(n), gf(n), grad(n), gq(n), gv(n);
+    std::vector<int32_t> spc(n);
+    std::vector<uint8_t> mol(n);
+    ZeemanOut z;
+    IngestOut o{wl.data(), elo.data(), gf.data(), grad.data(), gq.data(), gv.data(), spc.data(), mol.data()};
+    int64_t kn = 0;
+    if (sidecar_convert(sc, first, count, m, mol_slot, nthreads, o, kn, err, &z)) return fail(err);
+    fsg_lines h{};
+    h.n = kn;
+    h.wl = wl.data();
+    h.elo = elo.data();
+    h.gf = gf.data();
+    h.grad = grad.data();
+    h.gqst = gq.data();
+    h.gvdw = gv.data();
+    h.spc = spc.data();
+    h.is_mol = mol.data();
+    h.pi_off = z.pi_off.data();
+    h.spl_pi = z.spl_pi.data();
+    h.str_pi = z.str_pi.data();
+    h.si_off = z.si_off.data();
+    h.spl_si = z.spl_si.data();
+    h.str_si = z.str_si.data();
+    if (fsg_lines_upload(c, FSG_LIST_MIXED, &h, out)) return 1;
+    *kept = kn;
+    return 0;
+  }
   CK(hipSetDevice(c->dev));
   fsg_dlines *d = new fsg_dlines();
   d->ctx = c;

@@ -6,7 +6,7 @@
 //   format=fastspec-lines-1
 //   name=atom | <molecule name>     (the NAME attribute; 'atom...' marks an atomic list)
 //   wl_vac=0|1  wmin=<A>  wmax=<A>  nl=<lines>  hflag=<int>
-//   columns=W:f8,EL:S3,ION:i2,E:f4,LGGF:f4,GRAD:f4,GQST:f4,GVDW:f4
+//   columns=W:f8,EL:S3,ION:i2,E:f4,LGGF:f4,GRAD:f4,GQST:f4,GVDW:f4[,JLO:f4,JUP:f4,GLO:f4,GUP:f4]
 // Selection and conversions follow ReadHDF (src/ioutils.f90:545-1100).
 #pragma once
 
@@ -27,6 +27,7 @@ struct Sidecar {
   std::string dir, name;
   int wl_vac = 0, hflag = 0;
   bool molecular = false;
+  bool zeeman = false;  // JLO, JUP, GLO, GUP columns: a magnetic list
   double wmin = 0.0, wmax = 0.0;
   int64_t nl = 0;
   std::map<std::string, SidecarCol> cols;
@@ -44,13 +45,23 @@ struct IngestOut {
   uint8_t *is_mol = nullptr;
 };
 
+// Zeeman patterns of the kept lines in CSR form (sigma: blue half, then red half)
+struct ZeemanOut {
+  std::vector<int32_t> pi_off, si_off;
+  std::vector<float> spl_pi, str_pi, spl_si, str_si;
+};
+
 int sidecar_open(const char *dir, Sidecar &sc, std::string &err);
 // lines [first, first + count) of the file are inside [wstart, wend] (0-based first)
 int sidecar_select(const Sidecar &sc, double wstart, double wend, int64_t &first, int64_t &count, std::string &err);
 // reads and converts lines [first, first + count) with nthreads threads into out (packed,
 // lines above the EOS ion limit dropped); kept receives the number written
 int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const SpeciesMap &species, int32_t mol_slot,
-                    int nthreads, const IngestOut &out, int64_t &kept, std::string &err);
+                    int nthreads, const IngestOut &out, int64_t &kept, std::string &err, ZeemanOut *z = nullptr);
+// ZeemanPattern (src/utils.f90:901-972 with MergeShifts 976-1004): pi shifts and strengths,
+// one sigma branch
+void zeeman_pattern(float jlo, float jup, float glo, float gup, std::vector<float> &spi, std::vector<float> &rpi,
+                    std::vector<float> &ssig, std::vector<float> &rsig);
 double vac2air(double w);
 
 }  // namespace fsg

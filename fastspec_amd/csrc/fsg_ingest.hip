@@ -5,6 +5,8 @@
 //            conversions of gf and the damping constants   src/ioutils.f90:696-1076
 //   LowerBoundReal8, UpperBoundReal8                       src/utils.f90:701-737
 //   Vac2Air                                                src/utils.f90:1007-1017
+//   ZeemanPattern, MergeShifts; the reader's pi/sigma layout src/utils.f90:901-1004,
+//                                                          src/ioutils.f90:997-1031
 // The columns are read with pread from raw files by several threads (the reference reads
 // HDF5 hyperslabs); every other step keeps the reference's order and REAL(4) arithmetic.
 #include <fcntl.h>
@@ -68,7 +70,83 @@ std::string trim_el(const char *s, int n) {  // ADJUSTL(TRIM(El)): blanks and NU
 // in five by one ulp)
 inline float pow10f(float x) { return ::exp10f(x); }
 
+// components coinciding in shift within tol become one with the summed strength
+void merge_shifts(std::vector<double> &g, std::vector<double> &r, double tol) {
+  std::vector<double> gm, rm;
+  for (size_t i = 0; i < g.size(); i++) {
+    size_t k = 0;
+    for (; k < gm.size(); k++)
+      if (std::fabs(g[i] - gm[k]) < tol) {
+        rm[k] = rm[k] + r[i];
+        break;
+      }
+    if (k == gm.size()) {
+      gm.push_back(g[i]);
+      rm.push_back(r[i]);
+    }
+  }
+  g.swap(gm);
+  r.swap(rm);
+}
+
 }  // namespace
+
+void zeeman_pattern(float jlo, float jup, float glo, float gup, std::vector<float> &spi, std::vector<float> &rpi,
+                    std::vector<float> &ssig, std::vector<float> &rsig) {
+  const double tol = 1e-5;
+  const double Jl = jlo, Ju = jup, Gl = glo, Gu = gup;
+  const int nm = (int)std::lround(2.0 * Ju) + 1;
+  std::vector<double> gp, rp, gs, rs;
+  for (int i = 1; i <= nm; i++) {  // upper sublevels m = Jup .. -Jup
+    const double m = Ju - (double)(i - 1);
+    double p, q;
+    if (Jl == Ju) {
+      p = m * m;
+      q = 0.25 * (Ju - m) * (Ju + m + 1.0);
+    } else if (Jl > Ju) {
+      p = (Ju + 1.0) * (Ju + 1.0) - m * m;
+      q = 0.25 * (Ju + m + 1.0) * (Ju + m + 2.0);
+    } else {
+      p = Ju * Ju - m * m;
+      q = 0.25 * (Ju - m) * (Ju - m - 1.0);
+    }
+    double a = m * (Gu - Gl), b = m * Gu - (m + 1.0) * Gl;
+    if (std::fabs(a) < tol) a = 0.0;
+    if (std::fabs(b) < tol) b = 0.0;
+    gp.push_back(a);
+    rp.push_back(p);
+    gs.push_back(b);
+    rs.push_back(q);
+  }
+  std::vector<double> g1, r1, g2, r2;  // vanishing intensities dropped; sigma sign convention
+  for (int i = 0; i < nm; i++) {
+    if (std::fabs(rp[i]) > tol) {
+      g1.push_back(gp[i]);
+      r1.push_back(rp[i]);
+    }
+    if (std::fabs(rs[i]) > tol) {
+      g2.push_back(-gs[i]);
+      r2.push_back(rs[i]);
+    }
+  }
+  merge_shifts(g1, r1, tol);
+  merge_shifts(g2, r2, tol);
+  double s1 = 0.0, s2 = 0.0;
+  for (double v : r1) s1 = s1 + v;
+  for (double v : r2) s2 = s2 + v;
+  spi.clear();
+  rpi.clear();
+  ssig.clear();
+  rsig.clear();
+  for (size_t i = 0; i < g1.size(); i++) {
+    spi.push_back((float)g1[i]);
+    rpi.push_back((float)(s1 > 0.0 ? r1[i] / s1 : r1[i]));
+  }
+  for (size_t i = 0; i < g2.size(); i++) {
+    ssig.push_back((float)g2[i]);
+    rsig.push_back((float)(s2 > 0.0 ? r2[i] / s2 : r2[i]));
+  }
+}
 
 double vac2air(double w) {
   const double S = 1e4 / w, S2 = S * S;
@@ -138,8 +216,15 @@ int sidecar_open(const char *dir, Sidecar &sc, std::string &err) {
     err = "fsg_lines: unsupported column types in " + sc.dir;
     return 1;
   }
-  if (sc.cols.count("ISOT") || sc.cols.count("JLO")) {
-    err = "fsg_lines: isotope and Zeeman columns are not read from sidecars (" + sc.dir + ")";
+  if (sc.cols.count("ISOT")) {
+    err = "fsg_lines: isotope columns are not read from sidecars (" + sc.dir + ")";
+    return 1;
+  }
+  sc.zeeman = sc.cols.count("JLO") != 0;
+  if (sc.zeeman &&
+      (sc.molecular || !sc.cols.count("JUP") || !sc.cols.count("GLO") || !sc.cols.count("GUP") ||
+       !ck("JLO", 'f', 4) || !ck("JUP", 'f', 4) || !ck("GLO", 'f', 4) || !ck("GUP", 'f', 4))) {
+    err = "fsg_lines: incomplete Zeeman columns in " + sc.dir;
     return 1;
   }
   return 0;
@@ -175,9 +260,18 @@ int sidecar_select(const Sidecar &sc, double wstart, double wend, int64_t &first
 }
 
 int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const SpeciesMap &species, int32_t mol_slot,
-                    int nthreads, const IngestOut &out, int64_t &kept, std::string &err) {
+                    int nthreads, const IngestOut &out, int64_t &kept, std::string &err, ZeemanOut *z) {
   kept = 0;
+  if (z) {
+    *z = ZeemanOut();
+    z->pi_off.push_back(0);
+    z->si_off.push_back(0);
+  }
   if (count <= 0) return 0;
+  if (sc.zeeman && !z) {
+    err = "fsg_lines: " + sc.dir + " is a Zeeman list: read it with fsg_lines_stream";
+    return 1;
+  }
   const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(nthreads, 1), (count + 65535) / 65536));
   struct Part {
     int64_t a = 0, n = 0, kept = 0;
@@ -185,6 +279,7 @@ int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const Speci
     std::vector<float> elo, gf, grad, gqst, gvdw;
     std::vector<int32_t> spc;
     std::vector<uint8_t> mol;
+    ZeemanOut zp;  // this part's patterns, offsets from 0
     std::string err;
   };
   std::vector<Part> parts((size_t)nt);
@@ -210,7 +305,19 @@ int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const Speci
       ok = read_col(sc, "GQST", P.a, n, Q.data(), P.err) && read_col(sc, "GVDW", P.a, n, V.data(), P.err) &&
            read_col(sc, "EL", P.a, n, EL.data(), P.err) && read_col(sc, "ION", P.a, n, ION.data(), P.err);
     }
+    std::vector<float> JL, JU, GL, GU;
+    if (ok && sc.zeeman) {
+      JL.resize((size_t)n);
+      JU.resize((size_t)n);
+      GL.resize((size_t)n);
+      GU.resize((size_t)n);
+      ok = read_col(sc, "JLO", P.a, n, JL.data(), P.err) && read_col(sc, "JUP", P.a, n, JU.data(), P.err) &&
+           read_col(sc, "GLO", P.a, n, GL.data(), P.err) && read_col(sc, "GUP", P.a, n, GU.data(), P.err);
+      P.zp.pi_off.push_back(0);
+      P.zp.si_off.push_back(0);
+    }
     if (!ok) return;
+    std::vector<float> spi, rpi, ssig, rsig;
     P.wl.reserve((size_t)n);
     P.elo.reserve((size_t)n);
     P.gf.reserve((size_t)n);
@@ -253,6 +360,30 @@ int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const Speci
       P.gvdw.push_back(v < 0.0f ? pow10f(v) : v);
       P.spc.push_back(it->second.first);
       P.mol.push_back(it->second.second);
+      if (sc.zeeman) {
+        ZeemanOut &Z = P.zp;
+        if (GL[(size_t)i] > 90.0f || GU[(size_t)i] > 90.0f) {
+          // a missing Lande factor: one unshifted pi component and a null sigma pair
+          Z.spl_pi.push_back(0.0f);
+          Z.str_pi.push_back(1.0f);
+          Z.spl_si.insert(Z.spl_si.end(), {0.0f, 0.0f});
+          Z.str_si.insert(Z.str_si.end(), {1.0f, 1.0f});
+        } else {
+          zeeman_pattern(JL[(size_t)i], JU[(size_t)i], GL[(size_t)i], GU[(size_t)i], spi, rpi, ssig, rsig);
+          Z.spl_pi.insert(Z.spl_pi.end(), spi.begin(), spi.end());
+          Z.str_pi.insert(Z.str_pi.end(), rpi.begin(), rpi.end());
+          for (size_t k = 0; k < ssig.size(); k++) {  // blue (negative shift) half
+            Z.spl_si.push_back(-ssig[k]);
+            Z.str_si.push_back(rsig[k]);
+          }
+          for (size_t k = 0; k < ssig.size(); k++) {  // red half
+            Z.spl_si.push_back(ssig[k]);
+            Z.str_si.push_back(rsig[k]);
+          }
+        }
+        Z.pi_off.push_back((int32_t)Z.spl_pi.size());
+        Z.si_off.push_back((int32_t)Z.spl_si.size());
+      }
     }
     P.kept = (int64_t)P.wl.size();
   };
@@ -279,6 +410,17 @@ int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const Speci
       if (out.is_mol) std::memcpy(out.is_mol + o, P.mol.data(), k);
     }
     o += P.kept;
+    if (z && sc.zeeman) {  // concatenate the parts' patterns, offsets shifted
+      const int32_t bp = (int32_t)z->spl_pi.size(), bs = (int32_t)z->spl_si.size();
+      for (size_t k = 1; k < P.zp.pi_off.size(); k++) {
+        z->pi_off.push_back(bp + P.zp.pi_off[k]);
+        z->si_off.push_back(bs + P.zp.si_off[k]);
+      }
+      z->spl_pi.insert(z->spl_pi.end(), P.zp.spl_pi.begin(), P.zp.spl_pi.end());
+      z->str_pi.insert(z->str_pi.end(), P.zp.str_pi.begin(), P.zp.str_pi.end());
+      z->spl_si.insert(z->spl_si.end(), P.zp.spl_si.begin(), P.zp.spl_si.end());
+      z->str_si.insert(z->str_si.end(), P.zp.str_si.begin(), P.zp.str_si.end());
+    }
   }
   kept = o;
   return 0;

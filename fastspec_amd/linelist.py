@@ -20,7 +20,9 @@ from .abi import FSG_LIST_ATOMIC, FSG_LIST_MIXED, FSG_LIST_MOLECULAR, CSpecies, 
 
 ATOMIC_COLUMNS = ("W", "EL", "ION", "E", "LGGF", "GRAD", "GQST", "GVDW")
 MOLECULAR_COLUMNS = ("W", "E", "LGGF", "GRAD")
-_TYPES = {"W": "<f8", "ION": "<i2", "E": "<f4", "LGGF": "<f4", "GRAD": "<f4", "GQST": "<f4", "GVDW": "<f4"}
+ZEEMAN_COLUMNS = ("JLO", "JUP", "GLO", "GUP")
+_TYPES = {"W": "<f8", "ION": "<i2", "E": "<f4", "LGGF": "<f4", "GRAD": "<f4", "GQST": "<f4", "GVDW": "<f4",
+          "JLO": "<f4", "JUP": "<f4", "GLO": "<f4", "GUP": "<f4"}
 
 
 def _type_code(dt: np.dtype) -> str:
@@ -36,6 +38,8 @@ def write_sidecar(path, columns: Dict[str, np.ndarray], name: str = "atom", wl_v
     path = Path(path)
     path.mkdir(parents=True, exist_ok=True)
     need = MOLECULAR_COLUMNS if not name.startswith("atom") else ATOMIC_COLUMNS
+    if name.startswith("atom") and "JLO" in columns:  # a Zeeman list (name 'atom Zeeman')
+        need = need + ZEEMAN_COLUMNS
     n = len(columns["W"])
     types = []
     for col in need:

@@ -227,7 +227,10 @@ int fsg_lines_set_gf(fsg_dlines *dl, const float *gf, int32_t on_device);
  * LowerBoundReal8/UpperBoundReal8 selection of [wstart, wend], the ion filter (ION <= 5)
  * and the REAL(4) conversions gf = 10**LGGF, GRAD 10** for 0 < x < 20, GQST and GVDW 10**
  * when negative.  Species (EL, ION) map to the host's species slots through a table; a
- * molecular list takes mol_slot.  Isotope and Zeeman columns are not read. */
+ * molecular list takes mol_slot.  A list with JLO, JUP, GLO, GUP columns gets its Zeeman
+ * patterns from ZeemanPattern (src/utils.f90:901-1004; a Lande factor above 90 marks a
+ * line without splitting) and is read only by fsg_lines_stream.  Isotope columns are not
+ * read. */
 typedef struct fsg_species_entry {
   char el[8];       /* element or molecule symbol, blank- or NUL-padded        */
   int32_t ion;      /* ionisation stage, 1 = neutral                           */

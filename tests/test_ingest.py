@@ -163,3 +163,19 @@ def test_errors(tmp_path):
     (tmp_path / "x.fsl" / "header.txt").write_text("format=other\n")
     with pytest.raises(FsgError, match="not a FastSpec line-list sidecar"):
         LL.scan(tmp_path / "x.fsl", 0.0, 1e4)
+
+
+def test_zeeman_sidecar_needs_the_stream_reader(tmp_path):
+    cols = W.atomic_columns(50, 6000.0, 6010.0, 5)
+    cols.update(JLO=np.ones(50, np.float32), JUP=np.ones(50, np.float32), GLO=np.ones(50, np.float32),
+                GUP=np.full(50, 1.5, np.float32))
+    path = LL.write_sidecar(tmp_path / "z.fsl", cols, name="atom Zeeman")
+    assert LL.scan(path, 0.0, 1e4)[2] == 50
+    with pytest.raises(FsgError, match="Zeeman list"):
+        LL.read(path, 0.0, 1e4, species=W.ATOMIC_SPECIES_TABLE)
+    del cols["GUP"]
+    (path / "GUP.bin").unlink()
+    hdr = (path / "header.txt").read_text().replace(",GUP:f4", "")
+    (path / "header.txt").write_text(hdr)
+    with pytest.raises(FsgError, match="incomplete Zeeman columns"):
+        LL.scan(path, 0.0, 1e4)
