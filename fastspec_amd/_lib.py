@@ -6,7 +6,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-from .abi import CLines, CSetup, CSpecies, CStats
+from .abi import CIsotope, CLines, CSetup, CSpecies, CStats
 
 LIB_PATH = Path(__file__).resolve().parent / "libfsgpu.so"
 
@@ -66,10 +66,11 @@ def load() -> C.CDLL:
     i64p = C.POINTER(C.c_int64)
     sp = C.POINTER(CSpecies)
     lib.fsg_lines_scan.argtypes = [C.c_char_p, C.c_double, C.c_double, ip, i64p, i64p]
-    lib.fsg_lines_read.argtypes = [C.c_char_p, C.c_int64, C.c_int64, sp, C.c_int32, C.c_int32, C.c_int32,
-                                   dp, fp, fp, fp, fp, fp, ip, C.POINTER(C.c_uint8), i64p]
-    lib.fsg_lines_stream.argtypes = [vp, C.c_char_p, C.c_double, C.c_double, sp, C.c_int32, C.c_int32,
-                                     C.c_int32, C.POINTER(vp), i64p]
+    isp = C.POINTER(CIsotope)
+    lib.fsg_lines_read.argtypes = [C.c_char_p, C.c_int64, C.c_int64, sp, C.c_int32, isp, C.c_int32, C.c_int32,
+                                   C.c_int32, dp, fp, fp, fp, fp, fp, ip, C.POINTER(C.c_uint8), i64p]
+    lib.fsg_lines_stream.argtypes = [vp, C.c_char_p, C.c_double, C.c_double, sp, C.c_int32, isp, C.c_int32,
+                                     C.c_int32, C.c_int32, C.POINTER(vp), i64p]
     _LIB = lib
     return lib
 

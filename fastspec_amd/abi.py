@@ -56,6 +56,11 @@ class CSpecies(C.Structure):
     _fields_ = [("el", C.c_char * 8), ("ion", C.c_int32), ("slot", C.c_int32), ("is_mol", C.c_int32)]
 
 
+class CIsotope(C.Structure):
+    """fsg_isotope_entry: (element, mass number) -> isotopic fraction of the host's table."""
+    _fields_ = [("el", C.c_char * 8), ("a", C.c_int32), ("frac", C.c_double)]
+
+
 class CLines(C.Structure):
     _fields_ = [
         ("n", C.c_int64),

@@ -664,6 +664,17 @@ static int species_table(const fsg_species_entry *species, int32_t nspecies, int
   return 0;
 }
 
+static int isotope_table(const fsg_isotope_entry *isotopes, int32_t n, IsotopeMap &m) {
+  if (n < 0 || (n > 0 && !isotopes)) return fail("fsg_lines: isotope table is NULL");
+  for (int32_t i = 0; i < n; i++) {
+    std::string el(isotopes[i].el, strnlen(isotopes[i].el, sizeof(isotopes[i].el)));
+    while (!el.empty() && el.back() == ' ') el.pop_back();
+    while (!el.empty() && el.front() == ' ') el.erase(el.begin());
+    m[SpeciesKey(el, isotopes[i].a)] = isotopes[i].frac;
+  }
+  return 0;
+}
+
 int fsg_lines_scan(const char *dir, double wstart, double wend, int32_t *list_type, int64_t *first, int64_t *count) {
   g_err.clear();
   if (!dir || !first || !count) return fail("fsg_lines_scan: NULL argument");
@@ -675,8 +686,9 @@ int fsg_lines_scan(const char *dir, double wstart, double wend, int32_t *list_ty
 }
 
 int fsg_lines_read(const char *dir, int64_t first, int64_t count, const fsg_species_entry *species, int32_t nspecies,
-                   int32_t mol_slot, int32_t nthreads, double *wl, float *elo, float *gf, float *grad, float *gqst,
-                   float *gvdw, int32_t *spc, uint8_t *is_mol, int64_t *kept) {
+                   const fsg_isotope_entry *isotopes, int32_t nisotopes, int32_t mol_slot, int32_t nthreads,
+                   double *wl, float *elo, float *gf, float *grad, float *gqst, float *gvdw, int32_t *spc,
+                   uint8_t *is_mol, int64_t *kept) {
   g_err.clear();
   if (!dir || !kept || (count > 0 && (!wl || !elo || !gf || !grad || !spc))) return fail("fsg_lines_read: NULL argument");
   Sidecar sc;
@@ -687,13 +699,16 @@ int fsg_lines_read(const char *dir, int64_t first, int64_t count, const fsg_spec
   SpeciesMap m;
   bool any_mol = false;
   if (species_table(species, nspecies, 0, m, any_mol)) return 1;
+  IsotopeMap im;
+  if (isotope_table(isotopes, nisotopes, im)) return 1;
   IngestOut o{wl, elo, gf, grad, gqst, gvdw, spc, is_mol};
-  if (sidecar_convert(sc, first, count, m, mol_slot, nthreads, o, *kept, err)) return fail(err);
+  if (sidecar_convert(sc, first, count, m, im, mol_slot, nthreads, o, *kept, err)) return fail(err);
   return 0;
 }
 
 int fsg_lines_stream(fsg_ctx *c, const char *dir, double wstart, double wend, const fsg_species_entry *species,
-                     int32_t nspecies, int32_t mol_slot, int32_t nthreads, fsg_dlines **out, int64_t *kept) {
+                     int32_t nspecies, const fsg_isotope_entry *isotopes, int32_t nisotopes, int32_t mol_slot,
+                     int32_t nthreads, fsg_dlines **out, int64_t *kept) {
   g_err.clear();
   if (!c || !dir || !out || !kept) return fail("fsg_lines_stream: NULL argument");
   *out = nullptr;
@@ -705,6 +720,8 @@ int fsg_lines_stream(fsg_ctx *c, const char *dir, double wstart, double wend, co
   SpeciesMap m;
   bool any_mol = false;
   if (species_table(species, nspecies, c->S.nspc, m, any_mol)) return 1;
+  IsotopeMap im;
+  if (isotope_table(isotopes, nisotopes, im)) return 1;
   if (sc.molecular && (mol_slot < 1 || mol_slot > c->S.nspc)) return fail("LineopCalc: species index out of range");
   if (sc.zeeman) {
     // magnetic lists (a few thousand lines) are converted on the host with their Zeeman
@@ -717,7 +734,7 @@ int fsg_lines_stream(fsg_ctx *c, const char *dir, double wstart, double wend, co
     ZeemanOut z;
     IngestOut o{wl.data(), elo.data(), gf.data(), grad.data(), gq.data(), gv.data(), spc.data(), mol.data()};
     int64_t kn = 0;
-    if (sidecar_convert(sc, first, count, m, mol_slot, nthreads, o, kn, err, &z)) return fail(err);
+    if (sidecar_convert(sc, first, count, m, im, mol_slot, nthreads, o, kn, err, &z)) return fail(err);
     fsg_lines h{};
     h.n = kn;
     h.wl = wl.data();
@@ -792,7 +809,7 @@ int fsg_lines_stream(fsg_ctx *c, const char *dir, double wstart, double wend, co
     o.is_mol = reinterpret_cast<uint8_t *>(o.spc + chunk);
     const int64_t n = std::min<int64_t>(chunk, first + count - off);
     int64_t kn = 0;
-    if (sidecar_convert(sc, off, n, m, mol_slot, nthreads, o, kn, err)) {
+    if (sidecar_convert(sc, off, n, m, im, mol_slot, nthreads, o, kn, err)) {
       rc = fail(err);
       break;
     }

@@ -28,6 +28,7 @@ struct Sidecar {
   int wl_vac = 0, hflag = 0;
   bool molecular = false;
   bool zeeman = false;  // JLO, JUP, GLO, GUP columns: a magnetic list
+  bool isot = false;    // ISOT column: isotope mass numbers (0 = not given)
   double wmin = 0.0, wmax = 0.0;
   int64_t nl = 0;
   std::map<std::string, SidecarCol> cols;
@@ -36,6 +37,8 @@ struct Sidecar {
 // (element symbol, ion) -> (species slot, molecule flag) of the host's species table
 using SpeciesKey = std::pair<std::string, int>;
 using SpeciesMap = std::map<SpeciesKey, std::pair<int32_t, uint8_t>>;
+// (element symbol, mass number) -> isotopic fraction of the host's isotope table
+using IsotopeMap = std::map<SpeciesKey, double>;
 
 // converted columns of the selected lines (the caller owns the arrays, count entries each)
 struct IngestOut {
@@ -56,8 +59,9 @@ int sidecar_open(const char *dir, Sidecar &sc, std::string &err);
 int sidecar_select(const Sidecar &sc, double wstart, double wend, int64_t &first, int64_t &count, std::string &err);
 // reads and converts lines [first, first + count) with nthreads threads into out (packed,
 // lines above the EOS ion limit dropped); kept receives the number written
-int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const SpeciesMap &species, int32_t mol_slot,
-                    int nthreads, const IngestOut &out, int64_t &kept, std::string &err, ZeemanOut *z = nullptr);
+int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const SpeciesMap &species,
+                    const IsotopeMap &isotopes, int32_t mol_slot, int nthreads, const IngestOut &out, int64_t &kept,
+                    std::string &err, ZeemanOut *z = nullptr);
 // ZeemanPattern (src/utils.f90:901-972 with MergeShifts 976-1004): pi shifts and strengths,
 // one sigma branch
 void zeeman_pattern(float jlo, float jup, float glo, float gup, std::vector<float> &spi, std::vector<float> &rpi,

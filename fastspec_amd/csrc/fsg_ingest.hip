@@ -1,8 +1,9 @@
 // fsg_ingest.hip -- the line-list sidecar reader (host code only).
 //
 // Reference routines restated here (file:line in the reference tree):
-//   ReadHDF: range cull, W read, sort check, bounds, column slices, ion filter and the
-//            conversions of gf and the damping constants   src/ioutils.f90:696-1076
+//   ReadHDF: range cull, W read, sort check, bounds, column slices, ion filter, isotopic
+//            fractions and the conversions of gf and the damping constants
+//                                                          src/ioutils.f90:696-1076
 //   LowerBoundReal8, UpperBoundReal8                       src/utils.f90:701-737
 //   Vac2Air                                                src/utils.f90:1007-1017
 //   ZeemanPattern, MergeShifts; the reader's pi/sigma layout src/utils.f90:901-1004,
@@ -216,8 +217,10 @@ int sidecar_open(const char *dir, Sidecar &sc, std::string &err) {
     err = "fsg_lines: unsupported column types in " + sc.dir;
     return 1;
   }
-  if (sc.cols.count("ISOT")) {
-    err = "fsg_lines: isotope columns are not read from sidecars (" + sc.dir + ")";
+  sc.isot = sc.cols.count("ISOT") != 0;
+  if (sc.isot && (sc.molecular || sc.cols.at("ISOT").kind != 'i' ||
+                  (sc.cols.at("ISOT").size != 2 && sc.cols.at("ISOT").size != 4))) {
+    err = "fsg_lines: unsupported ISOT column in " + sc.dir;
     return 1;
   }
   sc.zeeman = sc.cols.count("JLO") != 0;
@@ -259,8 +262,9 @@ int sidecar_select(const Sidecar &sc, double wstart, double wend, int64_t &first
   return 0;
 }
 
-int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const SpeciesMap &species, int32_t mol_slot,
-                    int nthreads, const IngestOut &out, int64_t &kept, std::string &err, ZeemanOut *z) {
+int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const SpeciesMap &species,
+                    const IsotopeMap &isotopes, int32_t mol_slot, int nthreads, const IngestOut &out, int64_t &kept,
+                    std::string &err, ZeemanOut *z) {
   kept = 0;
   if (z) {
     *z = ZeemanOut();
@@ -304,6 +308,12 @@ int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const Speci
       ION.resize((size_t)n * ionsz);
       ok = read_col(sc, "GQST", P.a, n, Q.data(), P.err) && read_col(sc, "GVDW", P.a, n, V.data(), P.err) &&
            read_col(sc, "EL", P.a, n, EL.data(), P.err) && read_col(sc, "ION", P.a, n, ION.data(), P.err);
+    }
+    std::vector<char> ISO;
+    const int isosz = sc.isot ? sc.cols.at("ISOT").size : 0;
+    if (ok && sc.isot) {
+      ISO.resize((size_t)n * isosz);
+      ok = read_col(sc, "ISOT", P.a, n, ISO.data(), P.err);
     }
     std::vector<float> JL, JU, GL, GU;
     if (ok && sc.zeeman) {
@@ -349,12 +359,27 @@ int sidecar_convert(const Sidecar &sc, int64_t first, int64_t count, const Speci
                 " is not in the species table";
         return;
       }
-      // LinesA (ioutils.f90:1056-1076): gf = 10**LGGF; GRAD 10** when 0 < x < 20; GQST and
-      // GVDW 10** when negative
+      // isotopic fraction of a line with a mass number (ioutils.f90:940-962), 1 otherwise
+      float giso = 1.0f;
+      if (sc.isot) {
+        const int a = isosz == 2 ? (int)reinterpret_cast<const int16_t *>(ISO.data())[i]
+                                 : (int)reinterpret_cast<const int32_t *>(ISO.data())[i];
+        if (a > 0) {
+          auto f = isotopes.find(SpeciesKey(el, a));
+          if (f == isotopes.end()) {
+            P.err = "ReadHDF: isotope " + std::to_string(a) + el + " in the ISOT column of " + sc.dir +
+                    " is not in the isotopic table";
+            return;
+          }
+          giso = (float)f->second;
+        }
+      }
+      // LinesA (ioutils.f90:1056-1076): gf = 10**LGGF (times the isotopic fraction); GRAD
+      // 10** when 0 < x < 20; GQST and GVDW 10** when negative
       const float g = R[(size_t)i], q = Q[(size_t)i], v = V[(size_t)i];
       P.wl.push_back(w);
       P.elo.push_back(E[(size_t)i]);
-      P.gf.push_back(pow10f(G[(size_t)i]));
+      P.gf.push_back(sc.isot ? pow10f(G[(size_t)i]) * giso : pow10f(G[(size_t)i]));
       P.grad.push_back((g < 20.0f && g > 0.0f) ? pow10f(g) : g);
       P.gqst.push_back(q < 0.0f ? pow10f(q) : q);
       P.gvdw.push_back(v < 0.0f ? pow10f(v) : v);

@@ -16,13 +16,14 @@ from typing import Dict, Iterable, Optional, Tuple
 import numpy as np
 
 from . import _lib
-from .abi import FSG_LIST_ATOMIC, FSG_LIST_MIXED, FSG_LIST_MOLECULAR, CSpecies, LineList, ptr
+from .abi import FSG_LIST_ATOMIC, FSG_LIST_MIXED, FSG_LIST_MOLECULAR, CIsotope, CSpecies, LineList, ptr
 
 ATOMIC_COLUMNS = ("W", "EL", "ION", "E", "LGGF", "GRAD", "GQST", "GVDW")
 MOLECULAR_COLUMNS = ("W", "E", "LGGF", "GRAD")
 ZEEMAN_COLUMNS = ("JLO", "JUP", "GLO", "GUP")
+ISOTOPE_COLUMNS = ("ISOT",)
 _TYPES = {"W": "<f8", "ION": "<i2", "E": "<f4", "LGGF": "<f4", "GRAD": "<f4", "GQST": "<f4", "GVDW": "<f4",
-          "JLO": "<f4", "JUP": "<f4", "GLO": "<f4", "GUP": "<f4"}
+          "JLO": "<f4", "JUP": "<f4", "GLO": "<f4", "GUP": "<f4", "ISOT": "<i2"}
 
 
 def _type_code(dt: np.dtype) -> str:
@@ -40,6 +41,8 @@ def write_sidecar(path, columns: Dict[str, np.ndarray], name: str = "atom", wl_v
     need = MOLECULAR_COLUMNS if not name.startswith("atom") else ATOMIC_COLUMNS
     if name.startswith("atom") and "JLO" in columns:  # a Zeeman list (name 'atom Zeeman')
         need = need + ZEEMAN_COLUMNS
+    if name.startswith("atom") and "ISOT" in columns:  # isotope mass numbers (0 = not given)
+        need = need + ISOTOPE_COLUMNS
     n = len(columns["W"])
     types = []
     for col in need:
@@ -87,6 +90,17 @@ def species_table(entries: Iterable[Tuple[str, int, int, bool]]):
     return arr, len(ent)
 
 
+def isotope_table(entries: Iterable[Tuple[str, int, float]]):
+    """ctypes array of fsg_isotope_entry from (element, mass number, fraction) tuples."""
+    ent = list(entries)
+    arr = (CIsotope * max(len(ent), 1))()
+    for i, (el, a, frac) in enumerate(ent):
+        arr[i].el = el.encode()[:8]
+        arr[i].a = int(a)
+        arr[i].frac = float(frac)
+    return arr, len(ent)
+
+
 def scan(path, wstart: float, wend: float):
     """(list type, first, count) of the lines inside [wstart, wend] (fsg_lines_scan)."""
     lib = _lib.load()
@@ -95,11 +109,13 @@ def scan(path, wstart: float, wend: float):
     return lt.value, first.value, count.value
 
 
-def read(path, wstart: float, wend: float, species=(), mol_slot: int = 0, nthreads: int = 8) -> LineList:
+def read(path, wstart: float, wend: float, species=(), mol_slot: int = 0, nthreads: int = 8,
+         isotopes=()) -> LineList:
     """The selected, converted lines as a LineList ready for fsg_lines_upload (fsg_lines_read)."""
     lib = _lib.load()
     lt, first, count = scan(path, wstart, wend)
     tab, ns = species_table(species)
+    itab, ni = isotope_table(isotopes)
     wl = np.zeros(count)
     elo, gf, grad = (np.zeros(count, np.float32) for _ in range(3))
     mol = lt == FSG_LIST_MOLECULAR
@@ -108,7 +124,7 @@ def read(path, wstart: float, wend: float, species=(), mol_slot: int = 0, nthrea
     spc = np.zeros(count, np.int32)
     im = None if mol else np.zeros(count, np.uint8)
     kept = C.c_int64()
-    _lib.check(lib.fsg_lines_read(str(path).encode(), first, count, tab, ns, mol_slot, nthreads,
+    _lib.check(lib.fsg_lines_read(str(path).encode(), first, count, tab, ns, itab, ni, mol_slot, nthreads,
                                   ptr(wl, C.c_double), ptr(elo, C.c_float), ptr(gf, C.c_float), ptr(grad, C.c_float),
                                   ptr(gq, C.c_float), ptr(gv, C.c_float), ptr(spc, C.c_int32),
                                   ptr(im, C.c_uint8), C.byref(kept)))
@@ -125,14 +141,15 @@ class StreamedLines:
     a DeviceLines is (lineop_dev, lineop_vjp, set_gf)."""
 
     def __init__(self, synth, path, wstart: float, wend: float, species=(), mol_slot: int = 0,
-                 nthreads: int = 8):
+                 nthreads: int = 8, isotopes=()):
         self._lib = _lib.load()
         self.synth = synth
         tab, ns = species_table(species)
+        itab, ni = isotope_table(isotopes)
         self.handle = C.c_void_p()
         kept = C.c_int64()
-        _lib.check(self._lib.fsg_lines_stream(synth.ctx, str(path).encode(), wstart, wend, tab, ns, mol_slot,
-                                              nthreads, C.byref(self.handle), C.byref(kept)))
+        _lib.check(self._lib.fsg_lines_stream(synth.ctx, str(path).encode(), wstart, wend, tab, ns, itab, ni,
+                                              mol_slot, nthreads, C.byref(self.handle), C.byref(kept)))
         self.n = kept.value
         self.list_type = scan(path, wstart, wend)[0]
 

@@ -229,14 +229,20 @@ int fsg_lines_set_gf(fsg_dlines *dl, const float *gf, int32_t on_device);
  * when negative.  Species (EL, ION) map to the host's species slots through a table; a
  * molecular list takes mol_slot.  A list with JLO, JUP, GLO, GUP columns gets its Zeeman
  * patterns from ZeemanPattern (src/utils.f90:901-1004; a Lande factor above 90 marks a
- * line without splitting) and is read only by fsg_lines_stream.  Isotope columns are not
- * read. */
+ * line without splitting) and is read only by fsg_lines_stream.  A line with an ISOT mass
+ * number above 0 has its gf scaled by the REAL(4) isotopic fraction of the host's isotope
+ * table (src/ioutils.f90:940-962). */
 typedef struct fsg_species_entry {
   char el[8];       /* element or molecule symbol, blank- or NUL-padded        */
   int32_t ion;      /* ionisation stage, 1 = neutral                           */
   int32_t slot;     /* 1-based species slot (LinSpc)                           */
   int32_t is_mol;   /* IsMolecule(El): the list becomes a mixed list           */
 } fsg_species_entry;
+typedef struct fsg_isotope_entry {
+  char el[8];       /* element symbol                                          */
+  int32_t a;        /* mass number                                             */
+  double frac;      /* isotopic fraction (IsotFrac)                            */
+} fsg_isotope_entry;
 /* the selection only: 0-based first line and line count inside [wstart, wend];
  * list_type receives FSG_LIST_MOLECULAR or FSG_LIST_ATOMIC.  Host only. */
 int fsg_lines_scan(const char *dir, double wstart, double wend, int32_t *list_type, int64_t *first,
@@ -245,15 +251,17 @@ int fsg_lines_scan(const char *dir, double wstart, double wend, int32_t *list_ty
  * is_mol may be NULL for molecular lists), read by nthreads host threads; kept receives
  * the lines written after the ion filter.  Host only. */
 int fsg_lines_read(const char *dir, int64_t first, int64_t count, const fsg_species_entry *species,
-                   int32_t nspecies, int32_t mol_slot, int32_t nthreads, double *wl, float *elo, float *gf,
-                   float *grad, float *gqst, float *gvdw, int32_t *spc, uint8_t *is_mol, int64_t *kept);
+                   int32_t nspecies, const fsg_isotope_entry *isotopes, int32_t nisotopes, int32_t mol_slot,
+                   int32_t nthreads, double *wl, float *elo, float *gf, float *grad, float *gqst, float *gvdw,
+                   int32_t *spc, uint8_t *is_mol, int64_t *kept);
 /* scan + read straight into a new device list: chunks of up to 2^22 lines (environment
  * FSG_INGEST_CHUNK overrides the count) are converted by
  * nthreads host threads into one of two pinned buffers while the other one's columns copy
  * to HBM on the library stream.  No counterpart in the reference (whose list stays on the
  * host); the device list is the one fsg_lines_upload would make from the same lines. */
 int fsg_lines_stream(fsg_ctx *ctx, const char *dir, double wstart, double wend, const fsg_species_entry *species,
-                     int32_t nspecies, int32_t mol_slot, int32_t nthreads, fsg_dlines **out, int64_t *kept);
+                     int32_t nspecies, const fsg_isotope_entry *isotopes, int32_t nisotopes, int32_t mol_slot,
+                     int32_t nthreads, fsg_dlines **out, int64_t *kept);
 
 /* Reverse mode (no counterpart in the reference, which has no derivatives).  For a loss
  * L(SfL, SfC) of the flux sums of the last plane-parallel fsg_radtran:

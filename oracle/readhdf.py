@@ -10,6 +10,7 @@ Reference lines restated:
   molecular LinesM: gf = 10**LGGF, Grad as stored           src/ioutils.f90:806-813
   atomic: ion filter (ION <= Maxion = 5)                     src/ioutils.f90:921-934
   atomic LinesA: gf, GRAD (0 < x < 20), GQST, GVDW (< 0)     src/ioutils.f90:1056-1076
+  isotopic fractions of lines with an ISOT mass number       src/ioutils.f90:940-962,1078
   Vac2Air                                                    src/utils.f90:1007-1017
 Pinned by tests/test_ingest.py: Vac2Air and the bound searches against the reference's
 compiled routines (oracle/_ref); the bounds against brute force; 10.0**x against scalar
@@ -85,8 +86,9 @@ def _p10(x):
     return np.array([_LIBM.exp10f(float(v)) for v in x], np.float32)
 
 
-def convert_atomic(cols, first, count, species, wl_vac=0):
-    """LinesA of the selected lines; species maps (symbol, ion) -> (slot, is_mol)"""
+def convert_atomic(cols, first, count, species, wl_vac=0, isotopes=None):
+    """LinesA of the selected lines; species maps (symbol, ion) -> (slot, is_mol), isotopes
+    (symbol, mass number) -> fraction for the lines with an ISOT column entry above 0"""
     sl = slice(first, first + count)
     ion = np.asarray(cols["ION"][sl]).astype(np.int64)
     keep = ion <= MAXION
@@ -100,8 +102,14 @@ def convert_atomic(cols, first, count, species, wl_vac=0):
     v = np.asarray(cols["GVDW"][sl], np.float32)[keep]
     slot = np.array([species[(e, int(i))][0] for e, i in zip(el, ion)], np.int32)
     mol = np.array([species[(e, int(i))][1] for e, i in zip(el, ion)], np.uint8)
+    gf = _p10(np.asarray(cols["LGGF"][sl])[keep])
+    if "ISOT" in cols:  # gf times the REAL(4) isotopic fraction (src/ioutils.f90:940-962, 1078)
+        iso = np.asarray(cols["ISOT"][sl]).astype(np.int64)[keep]
+        fr = np.array([np.float32(isotopes[(e, int(a))]) if a > 0 else np.float32(1.0) for e, a in zip(el, iso)],
+                      np.float32)
+        gf = (gf * fr).astype(np.float32)
     return dict(
-        wl=w, elo=np.asarray(cols["E"][sl], np.float32)[keep], gf=_p10(np.asarray(cols["LGGF"][sl])[keep]),
+        wl=w, elo=np.asarray(cols["E"][sl], np.float32)[keep], gf=gf,
         grad=np.where((g < 20.0) & (g > 0.0), _p10(g), g).astype(np.float32),
         gqst=np.where(q < 0.0, _p10(q), q).astype(np.float32),
         gvdw=np.where(v < 0.0, _p10(v), v).astype(np.float32), spc=slot, is_mol=mol)

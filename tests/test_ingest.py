@@ -179,3 +179,22 @@ def test_zeeman_sidecar_needs_the_stream_reader(tmp_path):
     (path / "header.txt").write_text(hdr)
     with pytest.raises(FsgError, match="incomplete Zeeman columns"):
         LL.scan(path, 0.0, 1e4)
+
+
+def test_isotopic_fractions(tmp_path):
+    cols = W.atomic_columns(2000, 5000.0, 5020.0, 8)
+    iso = np.zeros(2000, np.int16)
+    iso[::3] = np.where(cols["EL"][::3] == b"Fe", 56, np.where(cols["EL"][::3] == b"Ti", 48, 0))
+    iso[1::7] = np.where(cols["EL"][1::7] == b"Fe", 54, 0)
+    cols["ISOT"] = iso
+    table = [("Fe", 54, 0.05845), ("Fe", 56, 0.91754), ("Ti", 48, 0.7372)]
+    path = LL.write_sidecar(tmp_path / "iso.fsl", cols)
+    got = LL.read(path, 5001.0, 5019.0, species=W.ATOMIC_SPECIES_TABLE, isotopes=table)
+    f, c = R.select(cols["W"], cols["W"].min(), cols["W"].max(), 0, 5001.0, 5019.0)
+    ref = R.convert_atomic(cols, f, c, SPECIES, isotopes={(e, a): v for e, a, v in table})
+    for k in ("wl", "elo", "gf", "grad", "gqst", "gvdw", "spc"):
+        assert np.array_equal(getattr(got, k), ref[k]), k
+    plain = R.convert_atomic({k: v for k, v in cols.items() if k != "ISOT"}, f, c, SPECIES)
+    assert np.count_nonzero(got.gf != plain["gf"]) > 100  # the scaled lines differ
+    with pytest.raises(FsgError, match="not in the isotopic table"):
+        LL.read(path, 5001.0, 5019.0, species=W.ATOMIC_SPECIES_TABLE, isotopes=table[1:])
