@@ -262,6 +262,7 @@ def main():
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-linstr", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--only-c5", action="store_true", help="run only the differentiable batch (profiling)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -271,6 +272,11 @@ def main():
 
     from fastspec_amd import workloads as W
     from fastspec_amd.abi import FSG_FP32, FSG_FP64
+
+    if args.only_c5:  # the differentiable batch alone (for profiling); not the benchmark line
+        res = run_c5(args.steps, args.warmup)
+        print(json.dumps({"c5_differentiable_batch": res}), flush=True)
+        return
 
     log("building the C2 workload")
     setup, lines = W.config_c2(nlines=args.nlines)
