@@ -7,6 +7,8 @@
 //   LocPrfPP, TauStep, CentDer, DELO coefficients   src/solver.f90:123-179, 591-647
 //   flux sums of DiskInteg                          src/convol.f90:233-234
 //   XlineDepth, HVoigt, LineOpacity1                src/opacity.f90:239-346, src/voigt.f90:76-110
+#include <algorithm>
+
 #include "fsg_device.h"
 #include "fsg_kernels.h"
 
@@ -439,9 +441,12 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
 #pragma unroll
     for (int q = 0; q < PPL; q++) g[q][k] = (valid[q] && d < N) ? glop[(size_t)d * S.nwl + pp[q] - 1] * inv : 0.0;
   }
+  // the tile's candidates are shared among gridDim.y workgroups (every output is an atomic
+  // sum), so that small grids still fill the device
+  const int slice = blockIdx.y, nslice = gridDim.y;
   for (int b = 0; b < nbins; b++) {
     const int2 r = rng[(size_t)t * rstride + b];
-    for (int j = r.x; j < r.y; j++) {
+    for (int j = r.x + slice; j < r.y; j += nslice) {
       const int4 m = meta_s[j];
       if (m.y < p0 || m.x > p1) continue;
       const double wc = wlc_s[j];
@@ -565,8 +570,12 @@ void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, co
                        hipStream_t st) {
   if (ntiles <= 0 || nbins <= 0) return;
   const int nw = S.ntp / DPW;
-  k_lineop_vjp<<<8 * ((ntiles + 7) / 8), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, glop,
-                                                           sl, gtemp);
+  const int nb = 8 * ((ntiles + 7) / 8);
+  // enough workgroups for about 8 wavefronts per SIMD of the 256 CUs (C5's small windows have
+  // a few dozen tiles; C2 has thousands and keeps one workgroup per tile)
+  const int nslice = std::max(1, std::min(64, (8 * 1024) / std::max(1, nb * nw)));
+  k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, glop, sl,
+                                                     gtemp);
 }
 
 void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *slot_s, const double *sl,
