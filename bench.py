@@ -145,52 +145,74 @@ def run_ingest(setup, nlines: int):
             "threads": 16, "note": "fsg_lines_stream from a sidecar in the page cache (min of 3)"}
 
 
-def run_c5(steps: int, warmup: int):
+def run_c5(steps: int, warmup: int, contexts: bool = False):
     """The differentiable batch: 256 grey atmospheres with their own windows over one shared
     list of 50,000 lines (FP32 opacity, FP64 solver).  A step synthesises all 256 spectra
     (gf of the shared list set on the device, LineopCalc, RadTranCalc, flux); the adjoint
     step adds the reverse mode of L = sum a F/Fc and gathers dL/d log gf of the shared
-    lines.  Contexts and uploads are made before timing."""
+    lines.  Default: one batch context, every stage launched once for all atmospheres over
+    their concatenated grids.  contexts=True: one context per atmosphere, driven from
+    C5_THREADS host threads (the earlier design, for comparison).  Contexts and uploads are
+    made before timing."""
     from fastspec_amd import workloads as W
     from fastspec_amd.abi import FSG_FP32
-    from fastspec_amd.synth import FastSpecGPU
+    from fastspec_amd.synth import FastSpecBatch, FastSpecGPU
 
     lines, models = W.config_c5()
     loggf = np.log10(lines.gf.astype(np.float64))
     rng = np.random.default_rng(11)
-    ctxs = []
-    for setup, idx in models:
-        g = FastSpecGPU(setup, precision=FSG_FP32)
-        ctxs.append((g, g.upload(lines.take(idx)), idx, rng.normal(size=setup.nwl)))
+    weights = [rng.normal(size=setup.nwl) for setup, _ in models]
+    if contexts:
+        ctxs = []
+        for (setup, idx), a in zip(models, weights):
+            g = FastSpecGPU(setup, precision=FSG_FP32)
+            ctxs.append((g, g.upload(lines.take(idx)), idx, a))
 
-    def one(item, gf, adjoint):
-        g, dl, idx, a = item
-        g.set_gf(dl, gf[idx])
-        g.reset_lineop()
-        ev = g.lineop_dev(dl)["evals"]
-        g.RadTranCalc(intensities=False)
-        sfl, sfc = g.flux()
-        gg = None
-        if adjoint:
-            g.flux_vjp(a / sfc, -a * sfl / sfc**2)
-            gg, _, _ = g.lineop_vjp(dl)
-        return ev, gg
-
-    def step(adjoint):
-        # the contexts (each with its own streams) are driven from C5_THREADS host threads,
-        # so that several small syntheses share the device
-        gf = np.power(np.float32(10.0), loggf.astype(np.float32))
-        grad = np.zeros(lines.n)
-        evals = 0
-        res = list(pool.map(lambda it: one(it, gf, adjoint), ctxs))
-        for (ev, gg), (_, _, idx, _) in zip(res, ctxs):
-            evals += ev
+        def one(item, gf, adjoint):
+            g, dl, idx, a = item
+            g.set_gf(dl, gf[idx])
+            g.reset_lineop()
+            ev = g.lineop_dev(dl)["evals"]
+            g.RadTranCalc(intensities=False)
+            sfl, sfc = g.flux()
+            gg = None
             if adjoint:
-                grad[idx] += gg
-        return evals, grad
+                g.flux_vjp(a / sfc, -a * sfl / sfc**2)
+                gg, _, _ = g.lineop_vjp(dl)
+            return ev, gg
 
-    from concurrent.futures import ThreadPoolExecutor
-    pool = ThreadPoolExecutor(max_workers=C5_THREADS)
+        from concurrent.futures import ThreadPoolExecutor
+        pool = ThreadPoolExecutor(max_workers=C5_THREADS)
+
+        def step(adjoint):
+            gf = np.power(np.float32(10.0), loggf.astype(np.float32))
+            grad = np.zeros(lines.n)
+            evals = 0
+            res = list(pool.map(lambda it: one(it, gf, adjoint), ctxs))
+            for (ev, gg), (_, _, idx, _) in zip(res, ctxs):
+                evals += ev
+                if adjoint:
+                    grad[idx] += gg
+            return evals, grad
+    else:
+        b = FastSpecBatch([setup for setup, _ in models], precision=FSG_FP32)
+        shared = b.upload(lines)
+        bl = b.batch_lines(shared, [idx for _, idx in models])
+
+        def step(adjoint):
+            gf = np.power(np.float32(10.0), loggf.astype(np.float32))
+            b.set_gf(bl, gf)
+            b.reset_lineop()
+            evals = b.lineop_dev(bl)["evals"]
+            b.RadTranCalc(intensities=False)
+            sfl, sfc = b.flux()
+            grad = None
+            if adjoint:
+                b.flux_vjp([a / c for a, c in zip(weights, sfc)],
+                           [-a * l / c**2 for a, l, c in zip(weights, sfl, sfc)])
+                gg, _, _ = b.lineop_vjp(bl)
+                grad = bl.to_shared(gg)
+            return evals, grad
 
     out = {}
     for name, adj in (("forward", False), ("forward_adjoint", True)):
@@ -200,17 +222,23 @@ def run_c5(steps: int, warmup: int):
         for _ in range(steps):
             evals, grad = step(adj)
         dt = (time.perf_counter() - t0) / steps
-        out[name] = {"s_per_batch": dt, "spectra_per_s": len(ctxs) / dt, "evals_per_s": evals / dt}
+        out[name] = {"s_per_batch": dt, "spectra_per_s": len(models) / dt, "evals_per_s": evals / dt}
         if adj:
             out[name]["grad_loggf_nonzero"] = int(np.count_nonzero(grad))
-    pool.shutdown()
-    for g, dl, _, _ in ctxs:
-        dl.free()
-        g.close()
+    if contexts:
+        pool.shutdown()
+        for g, dl, _, _ in ctxs:
+            dl.free()
+            g.close()
+        how = f"one context per atmosphere, driven from {C5_THREADS} host threads"
+    else:
+        bl.free()
+        shared.free()
+        b.close()
+        how = "one batch context: every stage launched once for all atmospheres over their concatenated grids"
     out["workload"] = (f"{len(models)} grey atmospheres (Teff 4500-6500 K, log g 3.5-4.8), windows 5-50 A "
                        f"in 5000-5500 A at 0.01 A ({sum(s.nwl for s, _ in models)} points), shared list "
-                       f"of {lines.n} lines; FP32 opacity, FP64 solver; adjoint of L = sum a F/Fc; "
-                       f"contexts driven from {C5_THREADS} host threads")
+                       f"of {lines.n} lines; FP32 opacity, FP64 solver; adjoint of L = sum a F/Fc; {how}")
     out["evals_per_batch"] = evals
     return out
 
@@ -263,6 +291,8 @@ def main():
     ap.add_argument("--no-linstr", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--only-c5", action="store_true", help="run only the differentiable batch (profiling)")
+    ap.add_argument("--c5-contexts", action="store_true",
+                    help="differentiable batch with one context per atmosphere (the earlier design)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -274,7 +304,7 @@ def main():
     from fastspec_amd.abi import FSG_FP32, FSG_FP64
 
     if args.only_c5:  # the differentiable batch alone (for profiling); not the benchmark line
-        res = run_c5(args.steps, args.warmup)
+        res = run_c5(args.steps, args.warmup, args.c5_contexts)
         print(json.dumps({"c5_differentiable_batch": res}), flush=True)
         return
 
@@ -344,7 +374,7 @@ def main():
         log(json.dumps(out["ingest"]))
     if not args.no_c5:
         log("differentiable batch (256 atmospheres)")
-        out["c5_differentiable_batch"] = run_c5(1, 1)
+        out["c5_differentiable_batch"] = run_c5(3, 1, args.c5_contexts)
         log(json.dumps(out["c5_differentiable_batch"]))
     if not args.no_cpu_baseline:
         log("timing the reference CPU kernels on a bounded sample")

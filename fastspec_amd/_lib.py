@@ -17,7 +17,8 @@ EXPORTS = (
     "fsg_magradtran", "fsg_flux", "fsg_lines_upload", "fsg_lines_free", "fsg_lineop_dev",
     "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop", "fsg_maglineop_dev", "fsg_lines_set_gf",
     "fsg_flux_vjp", "fsg_lineop_vjp", "fsg_integ_convol", "fsg_get_spectra",
-    "fsg_lines_scan", "fsg_lines_read", "fsg_lines_stream",
+    "fsg_lines_scan", "fsg_lines_read", "fsg_lines_stream", "fsg_init_batch", "fsg_batch_offsets",
+    "fsg_lines_batch",
 )
 
 _LIB = None
@@ -71,6 +72,9 @@ def load() -> C.CDLL:
                                    C.c_int32, dp, fp, fp, fp, fp, fp, ip, C.POINTER(C.c_uint8), i64p]
     lib.fsg_lines_stream.argtypes = [vp, C.c_char_p, C.c_double, C.c_double, sp, C.c_int32, isp, C.c_int32,
                                      C.c_int32, C.c_int32, C.POINTER(vp), i64p]
+    lib.fsg_init_batch.argtypes = [C.POINTER(CSetup), C.c_int32, C.POINTER(vp)]
+    lib.fsg_batch_offsets.argtypes = [vp, i64p]
+    lib.fsg_lines_batch.argtypes = [vp, vp, i64p, ip, C.c_int64, C.POINTER(vp)]
     _LIB = lib
     return lib
 

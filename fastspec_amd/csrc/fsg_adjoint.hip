@@ -269,6 +269,10 @@ __global__ void __launch_bounds__(64) k_rt_vjp(DevSetup S, const LopT *lop, cons
   const bool valid = lane_p < np;
   const int p = p_lo + (valid ? lane_p : 0);
   const int N = S.ntau, M = S.nmu;
+  // batch contexts: the wavefront's points (p_lo is a multiple of 64) belong to one
+  // atmosphere, whose dL/dT row receives the wave's sums
+  if (S.batch) gtemp += (size_t)S.pmap[p_lo + blockIdx.x * blockDim.x] * N;
+  point_setup(S, p);
   const double Wl = S.wave[p] + S.wfirst;
   const double C1 = kPlkC1 / Wl, C2 = exp(50.7649141 - 5.0 * log(Wl));
   double imu[MAXMU];
@@ -359,11 +363,12 @@ __device__ __forceinline__ double hvoigt64_d(const DevSetup &S, double A, int ia
 
 // per-(line, depth) records for the reverse mode: the FP64 path's coefficients with
 // d ln LinCop / dT, d ln LinVvgt / dT and d LinAvgt / dT in the record's spare words
-__global__ void k_fill64_vjp(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+__global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                              const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s, int4 *meta_s,
                              int *slot_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
+  const DevSetup S = line_setup(S0, L, order[j]);
   XC xc;
   line_meta(S, L, list_type, j, order, flag, il, ir, ic, xc, wlc_s, meta_s);
   slot_s[j] = xc.IxPF;
@@ -422,6 +427,7 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
   const int N = S.ntau;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
+  if (S.batch) gtemp += (size_t)S.pmap[p0 - 1] * N;  // the tile's atmosphere (batch contexts)
   int pp[PPL];
   bool valid[PPL];
   double Wp[PPL];
@@ -524,10 +530,11 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
 
 // per line: dL/d log10 gf = ln 10 sum_d sl[line][d] (to the line's input index), and
 // dL/d ln FractionPF[d][slot] += sl[line][d]
-__global__ void k_lineop_vjp_scatter(DevSetup S, int64_t nret, const int *order, const int *slot_s, const double *sl,
-                                     double *g_loggf, double *g_fpf) {
+__global__ void k_lineop_vjp_scatter(DevSetup S, int64_t nret, const int *order, const int *atm, const int *slot_s,
+                                     const double *sl, double *g_loggf, double *g_fpf) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
+  if (atm) g_fpf += (size_t)atm[order[j]] * S.ntau * S.pf_ld;  // the line's atmosphere (batch lists)
   const double *r = sl + (size_t)j * S.ntp;
   const int slot = slot_s[j];
   double tot = 0.0;
@@ -578,9 +585,9 @@ void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, co
                                                      gtemp);
 }
 
-void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *slot_s, const double *sl,
-                               double *g_loggf, double *g_fpf, hipStream_t st) {
-  if (nret) k_lineop_vjp_scatter<<<nb(nret, 128), 128, 0, st>>>(S, nret, order, slot_s, sl, g_loggf, g_fpf);
+void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *atm, const int *slot_s,
+                               const double *sl, double *g_loggf, double *g_fpf, hipStream_t st) {
+  if (nret) k_lineop_vjp_scatter<<<nb(nret, 128), 128, 0, st>>>(S, nret, order, atm, slot_s, sl, g_loggf, g_fpf);
 }
 
 }  // namespace fsg

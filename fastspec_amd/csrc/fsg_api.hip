@@ -114,6 +114,10 @@ struct fsg_ctx {
   double *glop = nullptr;
   int have_glop = 0;
   DBuf vscr, vgcont, vsl, vslot, vout;
+  // batch contexts: one context per atmosphere (its own set-up on the device) and the first
+  // point of each atmosphere in the concatenated grid (nb + 1 entries, the last the total)
+  std::vector<fsg_ctx *> kids;
+  std::vector<int64_t> boff;
 };
 
 struct fsg_dlines {
@@ -556,6 +560,7 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
 
 int fsg_finalize(fsg_ctx *c) {
   if (!c) return 0;
+  for (fsg_ctx *k : c->kids) fsg_finalize(k);
   (void)hipSetDevice(c->dev);
   if (c->st) (void)hipStreamSynchronize(c->st);
   for (void *p : c->owned) (void)hipFree(p);
@@ -574,6 +579,181 @@ int fsg_finalize(fsg_ctx *c) {
   if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
+  return 0;
+}
+
+// Batch contexts.  Each atmosphere gets an ordinary context (its set-up, XlineInit constants,
+// ContOpac and grid on the device); the batch context itself is an ordinary context over the
+// concatenated grid (atmosphere b at points [boff[b], boff[b] + nwl_b), each start a multiple
+// of SEG_ALIGN; padding points continue the grid's last step, repeat its last ContOpac
+// column and never lie inside a window), whose set-up carries the atmospheres' set-ups and
+// the atmosphere of every point.
+int fsg_init_batch(const fsg_setup *setups, int32_t nb, fsg_ctx **ctx_out) {
+  g_err.clear();
+  if (!setups || !ctx_out) return fail("fsg_init_batch: NULL argument");
+  *ctx_out = nullptr;
+  if (nb < 1) return fail("fsg_init_batch: no atmospheres");
+  const fsg_setup &s0 = setups[0];
+  if (s0.ntau < 3 || s0.nmu < 1 || s0.nmu > 16 || !s0.xmu || !s0.wmu || s0.nspc < 0 || s0.nslot < 1)
+    return fail("fsg_init_batch: bad first set-up");
+  const int N = s0.ntau;
+  auto same = [](const void *a, const void *b, size_t bytes) {
+    return bytes == 0 || (a && b && std::memcmp(a, b, bytes) == 0);
+  };
+  std::vector<int64_t> off(nb + 1, 0);
+  for (int b = 0; b < nb; b++) {
+    const fsg_setup &s = setups[b];
+    if (s.precision != s0.precision || s.ntau != N || s.nmu != s0.nmu || s.device != s0.device ||
+        s.batched != s0.batched || s.nspc != s0.nspc || s.nslot != s0.nslot || s.pf_ld != s0.pf_ld)
+      return fail("fsg_init_batch: the set-ups differ in precision, depth count, mu count, device or species tables");
+    if (!same(s.xmu, s0.xmu, sizeof(double) * s.nmu) || !same(s.wmu, s0.wmu, sizeof(double) * s.nmu))
+      return fail("fsg_init_batch: the set-ups differ in their mu quadrature");
+    if (!same(s.spc_ion, s0.spc_ion, sizeof(int32_t) * s.nspc) ||
+        !same(s.spc_slot, s0.spc_slot, sizeof(int32_t) * s.nspc) ||
+        !same(s.pot_ion, s0.pot_ion, sizeof(float) * s.nslot) || !same(s.xamass, s0.xamass, sizeof(float) * s.nslot))
+      return fail("fsg_init_batch: the set-ups differ in their species tables");
+    if (s.motype != 0) return fail("fsg_init_batch: batches need Motype 0 models");
+    if (s.radius > 0.0) return fail("fsg_init_batch: batches need plane-parallel models");
+    if (s.nhe != 0) return fail("fsg_init_batch: batches take no He I Stark tables");
+    if (s.nwl < 2 || !s.wave || !s.contop) return fail("fsg_init_batch: a set-up has no grid or ContOpac");
+    off[b + 1] = off[b] + ((int64_t)s.nwl + SEG_ALIGN - 1) / SEG_ALIGN * SEG_ALIGN;
+  }
+  const int64_t total = off[nb];
+  if (total > (int64_t)1 << 30) return fail("fsg_init_batch: the concatenated grid is too large");
+  std::vector<fsg_ctx *> kids;
+  auto drop = [&kids]() {
+    for (fsg_ctx *k : kids) fsg_finalize(k);
+  };
+  for (int b = 0; b < nb; b++) {
+    fsg_ctx *k = nullptr;
+    if (fsg_init(&setups[b], &k)) {
+      std::string e = g_err;
+      drop();
+      return fail("fsg_init_batch: atmosphere " + std::to_string(b) + ": " + e);
+    }
+    kids.push_back(k);
+  }
+  // the concatenated grid (each atmosphere's points relative to its own Wfirst) and ContOpac
+  std::vector<double> wave((size_t)total), cont((size_t)total * N);
+  std::vector<int> pmap((size_t)total);
+  for (int b = 0; b < nb; b++) {
+    const fsg_setup &s = setups[b];
+    const int64_t o = off[b], nw = s.nwl;
+    std::memcpy(wave.data() + o, s.wave, sizeof(double) * nw);
+    std::memcpy(cont.data() + o * N, s.contop, sizeof(double) * nw * N);
+    const double step = s.wave[nw - 1] - s.wave[nw - 2];
+    for (int64_t p = nw; p < off[b + 1] - o; p++) {
+      wave[o + p] = s.wave[nw - 1] + (double)(p - nw + 1) * step;
+      std::memcpy(cont.data() + (o + p) * N, s.contop + (nw - 1) * N, sizeof(double) * N);
+    }
+    std::fill(pmap.begin() + o, pmap.begin() + off[b + 1], b);
+  }
+  fsg_setup sup = s0;
+  sup.nwl = (int32_t)total;
+  sup.wave = wave.data();
+  sup.contop = cont.data();
+  fsg_ctx *c = nullptr;
+  if (fsg_init(&sup, &c)) {
+    std::string e = g_err;
+    drop();
+    return fail("fsg_init_batch: " + e);
+  }
+  c->kids = kids;
+  c->boff = off;
+  std::vector<DevSetup> tab(nb);
+  for (int b = 0; b < nb; b++) {
+    tab[b] = kids[b]->S;
+    tab[b].goff = (int)off[b];
+  }
+  int rc = upload(c, tab.data(), tab.size(), &c->S.batch, c->owned);
+  rc |= upload(c, pmap.data(), pmap.size(), &c->S.pmap, c->owned);
+  if (rc || hipStreamSynchronize(c->st) != hipSuccess) {
+    std::string e = g_err;
+    fsg_finalize(c);
+    return fail("fsg_init_batch: " + e);
+  }
+  c->S.nbatch = nb;
+  *ctx_out = c;
+  return 0;
+}
+
+int fsg_batch_offsets(const fsg_ctx *c, int64_t *off_out) {
+  g_err.clear();
+  if (!c || !off_out) return fail("fsg_batch_offsets: NULL argument");
+  if (!c->S.batch) return fail("fsg_batch_offsets: not a batch context");
+  std::copy(c->boff.begin(), c->boff.end(), off_out);
+  return 0;
+}
+
+int fsg_lines_batch(fsg_ctx *c, const fsg_dlines *src, const int64_t *index, const int32_t *atm, int64_t n,
+                    fsg_dlines **out) {
+  g_err.clear();
+  if (!c || !src || !out || (n > 0 && (!index || !atm))) return fail("fsg_lines_batch: NULL argument");
+  *out = nullptr;
+  if (!c->S.batch) return fail("fsg_lines_batch: not a batch context");
+  if (src->zeeman) return fail("fsg_lines_batch: batches take no Zeeman lists");
+  if (src->L.atm) return fail("fsg_lines_batch: the source is itself a batch list");
+  if (n < 0) return fail("fsg_lines_batch: negative line count");
+  const int64_t ns = src->L.n;
+  std::vector<double> w((size_t)std::max<int64_t>(ns, 1));
+  CK(hipSetDevice(c->dev));
+  if (ns > 0) CK(hipMemcpy(w.data(), src->L.wl, sizeof(double) * ns, hipMemcpyDeviceToHost));
+  for (int64_t k = 0; k < n; k++) {
+    if (index[k] < 0 || index[k] >= ns) return fail("fsg_lines_batch: line index out of range");
+    if (atm[k] < 0 || atm[k] >= c->S.nbatch) return fail("fsg_lines_batch: atmosphere index out of range");
+    if (k > 0 && atm[k] < atm[k - 1]) return fail("fsg_lines_batch: atmosphere indices must not decrease");
+    if (k > 0 && atm[k] == atm[k - 1] && w[index[k]] < w[index[k - 1]])
+      return fail("ReadHDF: line list is not sorted in wavelength");
+  }
+  fsg_dlines *d = new fsg_dlines();
+  d->ctx = c;
+  d->list_type = src->list_type;
+  d->L.n = n;
+  const size_t m = (size_t)std::max<int64_t>(n, 1);
+  auto dev = [&](size_t bytes) -> void * {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    d->owned.push_back(p);
+    return p;
+  };
+  double *wl = (double *)dev(m * 8);
+  float *elo = (float *)dev(m * 4), *gf = (float *)dev(m * 4), *grad = (float *)dev(m * 4);
+  int *spc = (int *)dev(m * 4);
+  float *gq = src->L.gqst ? (float *)dev(m * 4) : nullptr, *gv = src->L.gvdw ? (float *)dev(m * 4) : nullptr;
+  uint8_t *im = src->L.is_mol ? (uint8_t *)dev(m) : nullptr;
+  int64_t *idx = (int64_t *)dev(m * 8);
+  int *da = (int *)dev(m * 4);
+  if (!wl || !elo || !gf || !grad || !spc || (src->L.gqst && !gq) || (src->L.gvdw && !gv) || (src->L.is_mol && !im) ||
+      !idx || !da) {
+    fsg_lines_free(d);
+    return fail("fsg_lines_batch: out of device memory");
+  }
+  d->L.wl = wl;
+  d->L.elo = elo;
+  d->L.gf = gf;
+  d->L.grad = grad;
+  d->L.spc = spc;
+  d->L.gqst = gq;
+  d->L.gvdw = gv;
+  d->L.is_mol = im;
+  d->L.atm = da;
+  std::vector<int> ha(atm, atm + n);
+  int rc = 0;
+  if (n > 0) {
+    if (hipMemcpyAsync(idx, index, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+        hipMemcpyAsync(da, ha.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->st) != hipSuccess)
+      rc = fail("fsg_lines_batch: copy to the device failed");
+    if (!rc) {
+      launch_gather_lines(src->L, idx, n, wl, elo, gf, grad, gq, gv, spc, im, c->st);
+      if (hipGetLastError() != hipSuccess) rc = fail("fsg_lines_batch: gather kernel failed");
+    }
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess && !rc) rc = fail("fsg_lines_batch: gather failed");
+  if (rc) {
+    fsg_lines_free(d);
+    return 1;
+  }
+  *out = d;
   return 0;
 }
 
@@ -947,6 +1127,8 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   g_err.clear();
   if (!c || !dl) return fail("fsg_lineop_dev: NULL argument");
   if (nlines_lim < 1) return fail("fsg_lineop_dev: NlinesLim must be positive");
+  if ((c->S.batch != nullptr) != (dl->L.atm != nullptr))
+    return fail("fsg_lineop_dev: batch contexts take batch lists (fsg_lines_batch) and only they do");
   CK(hipSetDevice(c->dev));
   fsg_lineop_stats st{};
   const int64_t n = dl->L.n;
@@ -965,22 +1147,25 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
   unsigned long long *hu = P.hu;  // [4]: evaluations, wide lines
   st.nlines_use = nret;
   st.evals = (int64_t)hu[0];
-  // MarkWideLines over the packed (single block, atomic/mixed) or full list
+  // MarkWideLines over the packed (single block, atomic/mixed) or full list; a statistic
+  // here (every line is accumulated over its whole window), not counted for batch lists
   const int64_t nstep = std::min<int64_t>(n, nlines_lim);
   const int pack = (nstep == n) && (dl->list_type == FSG_LIST_ATOMIC || dl->list_type == FSG_LIST_MIXED);
-  CK(c->wlcl.ensure(n * sizeof(double)));
-  CK(c->rflag.ensure(n * sizeof(int)));
-  CK(c->pos.ensure(n * sizeof(int)));
-  if (pack) {
-    launch_retained_flag(n, c->flag.as<int>(), c->rflag.as<int>(), s);
-    size_t sb = 0;
-    CK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, c->rflag.as<int>(), c->pos.as<int>(), (int)n, s));
-    CK(c->tmp.ensure(sb));
-    CK(hipcub::DeviceScan::ExclusiveSum(c->tmp.p, sb, c->rflag.as<int>(), c->pos.as<int>(), (int)n, s));
+  if (!S.batch) {
+    CK(c->wlcl.ensure(n * sizeof(double)));
+    CK(c->rflag.ensure(n * sizeof(int)));
+    CK(c->pos.ensure(n * sizeof(int)));
+    if (pack) {
+      launch_retained_flag(n, c->flag.as<int>(), c->rflag.as<int>(), s);
+      size_t sb = 0;
+      CK(hipcub::DeviceScan::ExclusiveSum(nullptr, sb, c->rflag.as<int>(), c->pos.as<int>(), (int)n, s));
+      CK(c->tmp.ensure(sb));
+      CK(hipcub::DeviceScan::ExclusiveSum(c->tmp.p, sb, c->rflag.as<int>(), c->pos.as<int>(), (int)n, s));
+    }
+    launch_wlc_list(n, dl->L.wl, S.wfirst, c->flag.as<int>(), c->pos.as<int>(), c->wlcl.as<double>(), pack, s);
+    launch_mark_wide(S, n, dl->L.wl, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), c->wlcl.as<double>(),
+                     pack ? nret : n, nstep, c->u64 + 1, s);
   }
-  launch_wlc_list(n, dl->L.wl, S.wfirst, c->flag.as<int>(), c->pos.as<int>(), c->wlcl.as<double>(), pack, s);
-  launch_mark_wide(S, n, dl->L.wl, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(), c->wlcl.as<double>(),
-                   pack ? nret : n, nstep, c->u64 + 1, s);
   // Accumulation over chunks of the bin-sorted retained lines: the coefficient records of
   // at most `chunk` lines are resident at once (the device analogue of the reference's
   // NlinesStep blocks, src/synth.f90:1011-1088).  Groups 0..nbins-1 are the width bins,
@@ -1208,6 +1393,7 @@ int fsg_lineop(fsg_ctx *c, int32_t list_type, const fsg_lines *lines, int64_t nl
                int32_t *flags_out, int32_t *ileft_out, int32_t *iright_out, fsg_lineop_stats *stats) {
   g_err.clear();
   if (!c || !lines) return fail("fsg_lineop: NULL argument");
+  if (c->S.batch) return fail("fsg_lineop: batch contexts take batch lists (fsg_lines_batch, fsg_lineop_dev)");
   fsg_dlines *dl = nullptr;
   if (fsg_lines_upload(c, list_type, lines, &dl)) return 1;
   int rc = fsg_lineop_dev(c, dl, nlines_lim, stats);
@@ -1224,6 +1410,7 @@ int fsg_maglineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, const doub
                       fsg_lineop_stats *stats) {
   g_err.clear();
   if (!c || !dl || !bfield) return fail("fsg_maglineop_dev: NULL argument");
+  if (c->S.batch) return fail("fsg_maglineop_dev: batch contexts have no polarised path");
   if (nlines_lim < 1) return fail("fsg_maglineop_dev: NlinesLim must be positive");
   if (bframe < 0 || bframe > 2) return fail("fsg_maglineop_dev: BFRAME must be 0, 1 or 2");
   const int64_t n = dl->L.n;
@@ -1360,6 +1547,7 @@ int fsg_maglineop(fsg_ctx *c, const fsg_lines *lines, int64_t nlines_lim, const 
                   fsg_lineop_stats *stats) {
   g_err.clear();
   if (!c || !lines || !bfield) return fail("fsg_maglineop: NULL argument");
+  if (c->S.batch) return fail("fsg_maglineop: batch contexts have no polarised path");
   if (lines->n > 0 && (!lines->pi_off || !lines->si_off))
     return fail("MagLineopCalc: line list has no Zeeman patterns");
   fsg_dlines *dl = nullptr;
@@ -1396,13 +1584,15 @@ int fsg_flux_vjp(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glo
   hipStream_t s = c->st;
   if (!c->glop && alloc_zero(c, sizeof(double) * N * Nw, (void **)&c->glop)) return 1;
   CK(c->vgcont.ensure(sizeof(double) * N * Nw));
-  CK(c->vout.ensure(sizeof(double) * (2 * Nw + N)));
+  const size_t NB = (size_t)std::max(S.nbatch, 1);  // dL/dT rows: one per atmosphere of a batch
+  CK(c->vout.ensure(sizeof(double) * (2 * Nw + NB * N)));
   double *dg = c->vout.as<double>(), *dgc = dg + Nw, *dgt = dgc + Nw;
   CK(hipMemcpyAsync(dg, gsfl, sizeof(double) * Nw, hipMemcpyHostToDevice, s));
   CK(hipMemcpyAsync(dgc, gsfc, sizeof(double) * Nw, hipMemcpyHostToDevice, s));
-  CK(hipMemsetAsync(dgt, 0, sizeof(double) * N, s));
+  CK(hipMemsetAsync(dgt, 0, sizeof(double) * NB * N, s));
   const size_t per = rt_vjp_scratch_per_point(S);
-  const int64_t chunk = std::min<int64_t>((int64_t)Nw, std::max<int64_t>(64, (int64_t)((size_t)1 << 30) / (int64_t)per));
+  // chunks of whole wavefronts (a batch's atmospheres start at multiples of 64 points)
+  const int64_t chunk = std::min<int64_t>((int64_t)Nw, std::max<int64_t>(64, (int64_t)((size_t)1 << 30) / (int64_t)per) & ~(int64_t)63);
   CK(c->vscr.ensure(per * (size_t)chunk));
   for (int64_t p0 = 0; p0 < (int64_t)Nw; p0 += chunk) {
     const int np = (int)std::min<int64_t>(chunk, (int64_t)Nw - p0);
@@ -1420,7 +1610,7 @@ int fsg_flux_vjp(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glo
     for (size_t k = 0; k < N; k++)
       for (size_t iw = 0; iw < Nw; iw++) out[iw * N + k] = h[k * Nw + iw];
   }
-  if (gtemp_out) CK(hipMemcpyAsync(gtemp_out, dgt, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  if (gtemp_out) CK(hipMemcpyAsync(gtemp_out, dgt, sizeof(double) * NB * N, hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
   c->have_glop = 1;
   return 0;
@@ -1440,10 +1630,14 @@ int fsg_lineop_vjp(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
   const int64_t n = dl->L.n;
   const size_t N = (size_t)S.ntau;
   hipStream_t s = c->st;
-  const size_t nfpf = N * (size_t)S.pf_ld;
-  CK(c->vout.ensure(sizeof(double) * ((size_t)std::max<int64_t>(n, 1) + nfpf + N)));
+  if ((c->S.batch != nullptr) != (dl->L.atm != nullptr))
+    return fail("fsg_lineop_vjp: batch contexts take batch lists (fsg_lines_batch) and only they do");
+  // a batch has one dL/d ln FractionPF block and one dL/dT row per atmosphere
+  const size_t NB = (size_t)std::max(S.nbatch, 1);
+  const size_t nfpf = NB * N * (size_t)S.pf_ld;
+  CK(c->vout.ensure(sizeof(double) * ((size_t)std::max<int64_t>(n, 1) + nfpf + NB * N)));
   double *dgf = c->vout.as<double>(), *dfpf = dgf + std::max<int64_t>(n, 1), *dgt = dfpf + nfpf;
-  CK(hipMemsetAsync(c->vout.p, 0, sizeof(double) * ((size_t)std::max<int64_t>(n, 1) + nfpf + N), s));
+  CK(hipMemsetAsync(c->vout.p, 0, sizeof(double) * ((size_t)std::max<int64_t>(n, 1) + nfpf + NB * N), s));
   if (n > 0) {
     LinePrep P;
     if (scalar_pass1(c, dl, P)) return 1;
@@ -1477,19 +1671,19 @@ int fsg_lineop_vjp(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
         CK(hipMemsetAsync(c->vsl.p, 0, (size_t)m * S.ntp * sizeof(double), s));
         launch_lineop_vjp(S, ntiles, ng, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), c->wlc.as<double>(),
                           c->meta.as<int4>(), c->glop, c->vsl.as<double>(), dgt, s);
-        launch_lineop_vjp_scatter(S, m, order, c->vslot.as<int>(), c->vsl.as<double>(), dgf, dfpf, s);
+        launch_lineop_vjp_scatter(S, m, order, dl->L.atm, c->vslot.as<int>(), c->vsl.as<double>(), dgf, dfpf, s);
       }
       CK(hipGetLastError());
       CK(hipStreamSynchronize(s));
     }
   }
   if (gloggf_out && n > 0) CK(hipMemcpy(gloggf_out, dgf, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
-  std::vector<double> h(nfpf + N);
-  CK(hipMemcpy(h.data(), dfpf, sizeof(double) * (nfpf + N), hipMemcpyDeviceToHost));
+  std::vector<double> h(nfpf + NB * N);
+  CK(hipMemcpy(h.data(), dfpf, sizeof(double) * (nfpf + NB * N), hipMemcpyDeviceToHost));
   if (gfpf_inout)
     for (size_t i = 0; i < nfpf; i++) gfpf_inout[i] += h[i];
   if (gtemp_inout)
-    for (size_t k = 0; k < N; k++) gtemp_inout[k] += h[nfpf + k];
+    for (size_t k = 0; k < NB * N; k++) gtemp_inout[k] += h[nfpf + k];
   return 0;
 }
 
@@ -1618,6 +1812,7 @@ int fsg_magradtran(fsg_ctx *c, const double *bfield, int32_t nbfield, int32_t bf
                    double *spc_out) {
   g_err.clear();
   if (!c || !bfield) return fail("fsg_magradtran: NULL argument");
+  if (c->S.batch) return fail("fsg_magradtran: batch contexts have no polarised path");
   const int M = c->S.nmu;
   if (bframe < 0 || bframe > 2) return fail("fsg_magradtran: BFRAME must be 0, 1 or 2");
   if (bframe == 2 && nbfield < 1 + 2 * M) return fail("BFRAME 2 needs 1+2*Nmu BFIELD values");
@@ -1745,6 +1940,7 @@ int fsg_flux(fsg_ctx *c, double *sfl_out, double *sfc_out) {
 int fsg_integ_convol(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, double vgau) {
   g_err.clear();
   if (!c) return fail("fsg_integ_convol: NULL context");
+  if (c->S.batch) return fail("fsg_integ_convol: convolve the atmospheres of a batch one by one");
   if (!c->have_rt) return fail("fsg_integ_convol: no formal solution computed since the last opacity change");
   if (vsini < 0.0 || vmacro < 0.0 || vgau < 0.0) return fail("Fastspec: invalid value(s) of convolution parameters");
   if (cimode != 1 && cimode != 2) return fail("fsg_integ_convol: CImode must be 1 or 2");

@@ -111,7 +111,19 @@ struct DevSetup {
   const int *ntau_mu, *graz_mu;  // [nmu] points on the ray, ray misses the deepest layer
   const int *map_mu;             // [nmu][2 ntau] model layer (0-based) of each ray point
   const double *rhox_mu;         // [nmu][2 ntau] column mass along the ray
+  // batch contexts (several model atmospheres in one launch sequence): the set-up of the
+  // concatenated grid carries the atmospheres' own set-ups (batch[nbatch]) and the
+  // atmosphere of each point of the concatenated grid (pmap[nwl]); an atmosphere's own
+  // set-up has goff = its first point in the concatenated grid (0 outside batches)
+  int goff, nbatch;
+  const DevSetup *batch;
+  const int *pmap;
 };
+
+// Each atmosphere's points start at a multiple of SEG_ALIGN in the concatenated grid of a
+// batch, so that no tile of the opacity kernels (TILE, 64 or 128 points) and no wavefront
+// of the per-point kernels spans two atmospheres.
+constexpr int SEG_ALIGN = 256;
 
 struct DevLines {
   int64_t n;
@@ -119,6 +131,7 @@ struct DevLines {
   const float *elo, *gf, *grad, *gqst, *gvdw;
   const int *spc;
   const uint8_t *is_mol;
+  const int *atm;  // batch lists: the atmosphere (0-based) of each line, else nullptr
 };
 
 // Zeeman patterns of a device-resident list (src/ioutils.f90:1012-1031): per line i the

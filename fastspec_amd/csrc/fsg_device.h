@@ -34,6 +34,23 @@ __device__ __forceinline__ int initial_flag(const DevLines &L, int64_t i, int li
   return f;
 }
 
+// Batch contexts: per-line work (pass 1, coefficient fill, line strengths) runs on the
+// set-up of the line's own atmosphere, in that atmosphere's grid; per-point work takes the
+// temperature, column mass and wavelength origin of the point's atmosphere.
+__device__ __forceinline__ DevSetup line_setup(const DevSetup &S, const DevLines &L, int64_t i) {
+  if (S.batch) return S.batch[L.atm[i]];
+  return S;
+}
+
+__device__ __forceinline__ void point_setup(DevSetup &S, int p) {
+  if (S.batch) {
+    const DevSetup *B = S.batch + S.pmap[p];
+    S.temp = B->temp;
+    S.rhox = B->rhox;
+    S.wfirst = B->wfirst;
+  }
+}
+
 // XlineSetup (src/opacity.f90:96-235)
 __device__ inline XC xline_setup(const DevSetup &S, const DevLines &L, int64_t i, double wlc, int flag,
                           int list_type) {
@@ -296,7 +313,8 @@ __device__ __forceinline__ void line_meta(const DevSetup &S, const DevLines &L, 
   double wlc = L.wl[i] - S.wfirst;
   xc = xline_setup(S, L, i, wlc, f & 0x6, list_type);
   wlc_s[j] = wlc;
-  meta_s[j] = make_int4(il[i], ir[i], ic[i], (f >> 8) & 0xff);
+  // window and centre in the (concatenated) grid the accumulation kernels run on
+  meta_s[j] = make_int4(il[i] + S.goff, ir[i] + S.goff, ic[i] + S.goff, (f >> 8) & 0xff);
 }
 
 // FP32 batched path: far-wing expansion records (CoefA) and general-path records (CoefB)

@@ -15,6 +15,8 @@ void launch_xline_init(int ntau, const double *temp, const double *xne, const fl
 void launch_pass1(const DevSetup &S, const DevLines &L, int list_type, int *flag, int *il, int *ir, int *ic,
                   unsigned *key, int *bin_count, int *bin_hmax, int batched, hipStream_t st);
 void launch_iota(int *v, int64_t n, hipStream_t st);
+void launch_gather_lines(const DevLines &src, const int64_t *idx, int64_t n, double *wl, float *elo, float *gf,
+                         float *grad, float *gqst, float *gvdw, int *spc, uint8_t *is_mol, hipStream_t st);
 void launch_count_evals(int64_t n, const int *flag, const int *il, const int *ir, int ntau,
                         unsigned long long *acc, hipStream_t st);
 void launch_retained_flag(int64_t n, const int *flag, int *out, hipStream_t st);
@@ -88,8 +90,8 @@ void launch_fill64_vjp(const DevSetup &S, const DevLines &L, int list_type, int6
 void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
                        const double *wlc_s, const int4 *meta_s, const double *glop, double *sl, double *gtemp,
                        hipStream_t st);
-void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *slot_s, const double *sl,
-                               double *g_loggf, double *g_fpf, hipStream_t st);
+void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *atm, const int *slot_s,
+                               const double *sl, double *g_loggf, double *g_fpf, hipStream_t st);
 
 // convolution and disk integration (fsg_convol.hip)
 struct ConvolSetup {

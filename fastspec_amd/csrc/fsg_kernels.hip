@@ -92,10 +92,11 @@ __global__ void k_xline_init(int ntau, const double *temp, const double *xne, co
 // maxima (bound test, centre test, analytic window) are streamed in one depth loop; the
 // decisions are taken afterwards in the reference's order.
 // ------------------------------------------------------------------------------------
-__global__ void k_pass1(DevSetup S, DevLines L, int list_type, int *flag_out, int *il_out, int *ir_out,
+__global__ void k_pass1(DevSetup S0, DevLines L, int list_type, int *flag_out, int *il_out, int *ir_out,
                         int *ic_out, unsigned *key_out, int *bin_count, int *bin_hmax, int batched) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= L.n) return;
+  const DevSetup S = line_setup(S0, L, i);  // windows in the line's own grid
   const int N = S.ntau;
   int flag = initial_flag(L, i, list_type);
   double wlc = L.wl[i] - S.wfirst;
@@ -154,6 +155,22 @@ __global__ void k_pass1(DevSetup S, DevLines L, int list_type, int *flag_out, in
   }
   atomicAdd(&bin_count[key], 1);
   key_out[i] = key;
+}
+
+// batch lists: line k is line idx[k] of the source list
+__global__ void k_gather_lines(DevLines src, const int64_t *idx, int64_t n, double *wl, float *elo, float *gf,
+                               float *grad, float *gqst, float *gvdw, int *spc, uint8_t *is_mol) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = idx[k];
+  wl[k] = src.wl[i];
+  elo[k] = src.elo[i];
+  gf[k] = src.gf[i];
+  grad[k] = src.grad[i];
+  spc[k] = src.spc[i];
+  if (gqst) gqst[k] = src.gqst[i];
+  if (gvdw) gvdw[k] = src.gvdw[i];
+  if (is_mol) is_mol[k] = src.is_mol[i];
 }
 
 __global__ void k_iota(int *v, int64_t n) {
@@ -220,11 +237,12 @@ __global__ void k_retained_flag(int64_t n, const int *flag, int *out) {
 // ------------------------------------------------------------------------------------
 // coefficient fill: one lane per retained line in bin-sorted order
 // ------------------------------------------------------------------------------------
-__global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+__global__ void k_fill32(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                          const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
                          int2 *zone_s, LineRec *lrec, double *wlc_s, int4 *meta_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
+  const DevSetup S = line_setup(S0, L, order[j]);
   XC xc;
   line_meta(S, L, list_type, j, order, flag, il, ir, ic, xc, wlc_s, meta_s);
   const int N = S.ntau;
@@ -291,23 +309,24 @@ __global__ void k_fill32(DevSetup S, DevLines L, int list_type, int64_t nret, co
     if (S.wave[mid] >= zhi) e = mid;
     else a = mid + 1;
   }
-  zone_s[j] = make_int2(zl, a);
+  zone_s[j] = make_int2(zl + S.goff, a + S.goff);
   const int4 mm = meta_s[j];
   LineRec r;
   r.il = mm.x;
   r.ir = mm.y;
-  r.zl = zl;
-  r.zr = a;
+  r.zl = zl + S.goff;
+  r.zr = a + S.goff;
   r.wc = wc;
   r.pad0 = r.pad1 = 0;
   lrec[j] = r;
 }
 
 
-__global__ void k_fill64(DevSetup S, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+__global__ void k_fill64(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                          const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s, int4 *meta_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
+  const DevSetup S = line_setup(S0, L, order[j]);
   XC xc;
   line_meta(S, L, list_type, j, order, flag, il, ir, ic, xc, wlc_s, meta_s);
   const int N = S.ntau;
@@ -1375,6 +1394,7 @@ __global__ void __launch_bounds__(256) k_radtran(DevSetup S, const LopT *lop, do
                                                  double *sfl, double *sfc) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= S.nwl) return;
+  point_setup(S, p);
   const int M = S.nmu;
   double Wl = S.wave[p] + S.wfirst;
   double C1 = kPlkC1 / Wl;
@@ -1416,10 +1436,11 @@ struct LineCentreCol {
 };
 
 // LinStrCalc (src/synth.f90:1774-1822): central depth of each retained line
-__global__ void __launch_bounds__(128) k_linstr(DevSetup S, DevLines L, int list_type, const int *flag,
+__global__ void __launch_bounds__(128) k_linstr(DevSetup S0, DevLines L, int list_type, const int *flag,
                                                 float *linstr) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= L.n) return;
+  const DevSetup S = line_setup(S0, L, i);
   int f = flag[i];
   if (f & 1) {
     linstr[i] = 0.0f;
@@ -1480,6 +1501,11 @@ void launch_pass1(const DevSetup &S, const DevLines &L, int list_type, int *flag
                   unsigned *key, int *bin_count, int *bin_hmax, int batched, hipStream_t st) {
   if (L.n == 0) return;
   k_pass1<<<nblk(L.n, 128), 128, 0, st>>>(S, L, list_type, flag, il, ir, ic, key, bin_count, bin_hmax, batched);
+}
+
+void launch_gather_lines(const DevLines &src, const int64_t *idx, int64_t n, double *wl, float *elo, float *gf,
+                         float *grad, float *gqst, float *gvdw, int *spc, uint8_t *is_mol, hipStream_t st) {
+  if (n > 0) k_gather_lines<<<nblk(n, 256), 256, 0, st>>>(src, idx, n, wl, elo, gf, grad, gqst, gvdw, spc, is_mol);
 }
 
 void launch_iota(int *v, int64_t n, hipStream_t st) {

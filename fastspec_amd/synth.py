@@ -255,4 +255,130 @@ class FastSpecGPU:
         return p.value
 
 
-__all__ = ["FastSpecGPU", "DeviceLines", "FSG_FP32", "FSG_FP64"]
+class BatchLines:
+    """A batch list (fsg_lines_batch): atmosphere b takes the lines indices[b] (ascending) of a
+    shared device list."""
+
+    def __init__(self, batch: "FastSpecBatch", shared: DeviceLines, indices):
+        self._lib = _lib.load()
+        self.batch = batch
+        idx = [np.asarray(ix, np.int64) for ix in indices]
+        assert len(idx) == batch.nb, "one index array per atmosphere"
+        self.index = np.ascontiguousarray(np.concatenate(idx) if idx else np.zeros(0, np.int64))
+        self.atm = np.ascontiguousarray(np.concatenate([np.full(len(ix), b, np.int32) for b, ix in enumerate(idx)]))
+        self.bounds = np.concatenate([[0], np.cumsum([len(ix) for ix in idx])]).astype(np.int64)
+        self.n = int(self.index.shape[0])
+        self.nshared = shared.n
+        self.list_type = shared.list_type
+        self.handle = C.c_void_p()
+        _lib.check(self._lib.fsg_lines_batch(batch.ctx, shared.handle, ptr(self.index, C.c_int64),
+                                             ptr(self.atm, C.c_int32), self.n, C.byref(self.handle)))
+
+    def per_atmosphere(self, a):
+        """Splits an array over the batch list's lines into the atmospheres' parts."""
+        return [a[self.bounds[b]:self.bounds[b + 1]] for b in range(len(self.bounds) - 1)]
+
+    def to_shared(self, a):
+        """Sums an array over the batch list's lines onto the shared list's lines."""
+        return np.bincount(self.index, weights=a, minlength=self.nshared)
+
+    free = DeviceLines.free
+    __del__ = DeviceLines.__del__
+
+
+class FastSpecBatch(FastSpecGPU):
+    """Many model atmospheres synthesised in one launch sequence (fsg_init_batch): the
+    atmospheres' grids are concatenated, and every stage runs once for all of them.  Per
+    atmosphere the results are those of a FastSpecGPU context on its own set-up."""
+
+    def __init__(self, setups, precision: int = FSG_FP32, batched: Optional[int] = None, device: int = 0):
+        from .abi import CSetup
+        self._lib = _lib.load()
+        self.setups = list(setups)
+        self.nb = len(self.setups)
+        self.setup = self.setups[0]
+        self.precision = precision
+        self._cs = [su.cstruct(precision=precision, batched=batched, device=device) for su in self.setups]
+        arr = (CSetup * self.nb)(*self._cs)
+        self.ctx = C.c_void_p()
+        _lib.check(self._lib.fsg_init_batch(arr, self.nb, C.byref(self.ctx)))
+        self.off = np.zeros(self.nb + 1, np.int64)
+        _lib.check(self._lib.fsg_batch_offsets(self.ctx, ptr(self.off, C.c_int64)))
+        self.nwl_total = int(self.off[-1])
+
+    def split(self, a):
+        """The atmospheres' parts of an array over the concatenated grid (first axis)."""
+        return [a[self.off[b]:self.off[b] + su.nwl] for b, su in enumerate(self.setups)]
+
+    def join(self, parts):
+        """An array over the concatenated grid from the atmospheres' parts (zero between)."""
+        first = np.asarray(parts[0])
+        out = np.zeros((self.nwl_total,) + first.shape[1:], dtype=np.float64)
+        for b, su in enumerate(self.setups):
+            out[self.off[b]:self.off[b] + su.nwl] = parts[b]
+        return out
+
+    def batch_lines(self, shared: DeviceLines, indices) -> BatchLines:
+        return BatchLines(self, shared, indices)
+
+    def set_gf(self, dlines, gf) -> None:
+        """gf of a batch list from the shared list's gf (numpy, float32)."""
+        if isinstance(dlines, BatchLines):
+            gf = np.asarray(gf, np.float32)[dlines.index]
+        super().set_gf(dlines, gf)
+
+    def LineopCalc(self, *a, **k):
+        raise NotImplementedError("batch contexts take batch lists: batch_lines() and lineop_dev()")
+
+    def RadTranCalc(self, intensities: bool = True):
+        """Intensities per atmosphere ([nwl][nmu] each), or None."""
+        m = self.setup.nmu
+        spl = np.zeros((self.nwl_total, m)) if intensities else None
+        spc = np.zeros((self.nwl_total, m)) if intensities else None
+        _lib.check(self._lib.fsg_radtran(self.ctx, ptr(spl, C.c_double), ptr(spc, C.c_double)))
+        self._polarised = False
+        return (self.split(spl), self.split(spc)) if intensities else (None, None)
+
+    radtran_calc = RadTranCalc
+
+    def flux(self):
+        """(SfL, SfC) per atmosphere: two lists of [nwl] arrays."""
+        sfl = np.zeros(self.nwl_total)
+        sfc = np.zeros(self.nwl_total)
+        _lib.check(self._lib.fsg_flux(self.ctx, ptr(sfl, C.c_double), ptr(sfc, C.c_double)))
+        return self.split(sfl), self.split(sfc)
+
+    def get_lineop(self):
+        out = np.zeros((self.nwl_total, self.setup.ntau))
+        _lib.check(self._lib.fsg_get_lineop(self.ctx, ptr(out, C.c_double)))
+        return self.split(out)
+
+    def flux_vjp(self, gsfl, gsfc, want_opac: bool = False):
+        """From per-atmosphere dL/dSfL, dL/dSfC: per-atmosphere dL/dLineOpac and dL/dContOpac
+        (or None) and the source-function part of dL/dT [nb][ntau]."""
+        n = self.setup.ntau
+        a = np.ascontiguousarray(self.join(gsfl))
+        b = np.ascontiguousarray(self.join(gsfc))
+        glop = np.zeros((self.nwl_total, n)) if want_opac else None
+        gcont = np.zeros((self.nwl_total, n)) if want_opac else None
+        gt = np.zeros((self.nb, n))
+        _lib.check(self._lib.fsg_flux_vjp(self.ctx, ptr(a, C.c_double), ptr(b, C.c_double), ptr(glop, C.c_double),
+                                          ptr(gcont, C.c_double), ptr(gt, C.c_double)))
+        if want_opac:
+            return self.split(glop), self.split(gcont), gt
+        return None, None, gt
+
+    def lineop_vjp(self, dlines: BatchLines, NlinesLim: Optional[int] = None):
+        """(dL/d log10 gf [n] over the batch list, dL/d ln FractionPF [nb][ntau][pf_ld], line
+        part of dL/dT [nb][ntau])."""
+        su = self.setup
+        nlim = dlines.n if NlinesLim is None else int(NlinesLim)
+        g = np.zeros(max(dlines.n, 1))
+        gf = np.zeros((self.nb, su.ntau, su.fraction_pf.shape[1]))
+        gt = np.zeros((self.nb, su.ntau))
+        _lib.check(self._lib.fsg_lineop_vjp(self.ctx, dlines.handle, max(nlim, 1), ptr(g, C.c_double),
+                                            ptr(gf, C.c_double), ptr(gt, C.c_double)))
+        return g[: dlines.n], gf, gt
+
+
+__all__ = ["FastSpecGPU", "FastSpecBatch", "DeviceLines", "BatchLines", "FSG_FP32", "FSG_FP64"]

@@ -278,6 +278,30 @@ int fsg_flux_vjp(fsg_ctx *ctx, const double *gsfl, const double *gsfc, double *g
 int fsg_lineop_vjp(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_out,
                    double *gfpf_inout, double *gtemp_inout);
 
+/* Batch contexts: many model atmospheres synthesised in one launch sequence (the
+ * differentiable batch of many small syntheses).  No counterpart in the reference, which
+ * synthesises one atmosphere per run (src/fastspec.f90:241-461); per atmosphere the stages
+ * and results are those of an ordinary context.  The atmospheres' grids are concatenated,
+ * each starting at a multiple of 256 points (fsg_batch_offsets); on a batch context every
+ * array that the entry points above describe as [nwl] or [ntau][nwl] spans the concatenated
+ * grid (points between atmospheres lie in no window and are to be ignored).  All set-ups
+ * share precision, Ntau, the mu quadrature, the species and EOS slot tables and the device;
+ * each is a Motype 0, plane-parallel model without He I Stark tables.  A batch context
+ * takes fsg_lineop_dev on batch lists, fsg_radtran, fsg_flux, fsg_get_spectra,
+ * fsg_get_lineop, fsg_set_lineop, fsg_reset_lineop, fsg_flux_vjp (gtemp_out [nb][ntau])
+ * and fsg_lineop_vjp (gfpf_inout [nb][ntau][pf_ld], gtemp_inout [nb][ntau]); the other
+ * entry points refuse it. */
+int fsg_init_batch(const fsg_setup *setups, int32_t nb, fsg_ctx **ctx_out);
+/* off_out[nb + 1]: the first point (0-based) of each atmosphere in the concatenated grid,
+ * then the grid's point count. */
+int fsg_batch_offsets(const fsg_ctx *ctx, int64_t *off_out);
+/* A batch list for fsg_lineop_dev: line k is line index[k] of the device list src (a list
+ * of the same device; not a Zeeman list) for atmosphere atm[k].  Atmosphere indices must
+ * not decrease, and wavelengths must not decrease within an atmosphere (each atmosphere's
+ * lines are a list as ReadHDF returns it). */
+int fsg_lines_batch(fsg_ctx *batch, const fsg_dlines *src, const int64_t *index, const int32_t *atm, int64_t n,
+                    fsg_dlines **out);
+
 /* Device pointers to the context's resident arrays (valid until fsg_finalize):
  *   which = 0 LineOpac [ntau][nwl] (lambda fastest; FP32 or FP64 by precision)
  *           1 SpL [nmu][nwl] FP64   2 SpC [nmu][nwl] FP64

@@ -43,6 +43,22 @@ def test_init_rejects_bad_setup_without_gpu_work():
     assert b"Ntau" in lib.fsg_last_error()
 
 
+def test_init_batch_rejects_mismatched_setups_without_gpu_work():
+    from fastspec_amd import workloads as W
+    from fastspec_amd.abi import FSG_FP64, CSetup
+    lib = _lib.load()
+    a = W.build_setup(W.grey_atmosphere(5000.0, 4.0), 5000.0, 5001.0, 0.01, 0)
+    b = W.build_setup(W.grey_atmosphere(5000.0, 4.0, ntau=48), 5000.0, 5001.0, 0.01, 0)
+    c = W.build_setup(W.grey_atmosphere(5000.0, 4.0), 5000.0, 5001.0, 0.01, 0, motype=1)
+    ctx = ctypes.c_void_p()
+    for other, msg in ((b, b"depth count"), (c, b"Motype 0")):
+        keep = [a.cstruct(precision=FSG_FP64), other.cstruct(precision=FSG_FP64)]
+        arr = (CSetup * 2)(*keep)
+        assert lib.fsg_init_batch(arr, 2, ctypes.byref(ctx)) != 0
+        assert msg in lib.fsg_last_error()
+    assert lib.fsg_init_batch(arr, 0, ctypes.byref(ctx)) != 0
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_LIB", None)
     monkeypatch.setattr(_lib, "LIB_PATH", tmp_path / "libfsgpu.so")

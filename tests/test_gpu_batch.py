@@ -1,0 +1,136 @@
+"""Batch contexts (fsg_init_batch, fsg_lines_batch): many atmospheres synthesised in one
+launch sequence over their concatenated grids give, per atmosphere, the results of an
+ordinary context on that atmosphere alone -- opacity, flux, and the reverse mode (dL/dT,
+dL/d ln FractionPF, dL/d log gf) -- and the FP64 batch flux agrees with the FP64 oracle."""
+import numpy as np
+import pytest
+
+from fastspec_amd import workloads as W
+from fastspec_amd.abi import FSG_FP32, FSG_FP64
+from fastspec_amd.synth import FastSpecBatch, FastSpecGPU
+from fastspec_amd._lib import FsgError
+from oracle_bridge import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c5_small():
+    # differentiable-batch workload, scaled down: 6 atmospheres, windows of 5-50 A
+    lines, models = W.config_c5(nmodels=6, nlines=4000)
+    rng = np.random.default_rng(3)
+    weights = [rng.normal(size=s.nwl) for s, _ in models]
+    return lines, models, weights
+
+
+def _single(setup, lines, prec, a=None):
+    with FastSpecGPU(setup, precision=prec) as g:
+        dl = g.upload(lines)
+        st = g.lineop_dev(dl)
+        lop = g.get_lineop()
+        g.RadTranCalc(intensities=False)
+        sfl, sfc = g.flux()
+        out = dict(sfl=sfl, sfc=sfc, lop=lop, evals=st["evals"])
+        if a is not None:
+            _, _, gt = g.flux_vjp(a / sfc, -a * sfl / sfc**2)
+            ggf, gfpf, gtl = g.lineop_vjp(dl)
+            out.update(gt=gt + gtl, ggf=ggf, gfpf=gfpf)
+        dl.free()
+    return out
+
+
+def _batch(lines, models, prec, weights=None):
+    setups = [s for s, _ in models]
+    with FastSpecBatch(setups, precision=prec) as b:
+        shared = b.upload(lines)
+        bl = b.batch_lines(shared, [idx for _, idx in models])
+        st = b.lineop_dev(bl)
+        lop = b.get_lineop()
+        b.RadTranCalc(intensities=False)
+        sfl, sfc = b.flux()
+        out = dict(sfl=sfl, sfc=sfc, lop=lop, evals=st["evals"], off=b.off.copy())
+        if weights is not None:
+            gsfl = [a / c for a, c in zip(weights, sfc)]
+            gsfc = [-a * l / c**2 for a, l, c in zip(weights, sfl, sfc)]
+            _, _, gt = b.flux_vjp(gsfl, gsfc)
+            ggf, gfpf, gtl = b.lineop_vjp(bl)
+            out.update(gt=gt + gtl, ggf=bl.per_atmosphere(ggf), gfpf=gfpf, ggf_shared=bl.to_shared(ggf))
+        bl.free()
+        shared.free()
+    return out
+
+
+def test_layout(c5_small):
+    lines, models, _ = c5_small
+    out = _batch(lines, models[:3], FSG_FP64)
+    off = out["off"]
+    assert off[0] == 0 and np.all(off % 256 == 0)
+    for b, (s, _) in enumerate(models[:3]):
+        assert off[b + 1] - off[b] >= s.nwl
+
+
+@pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])
+def test_batch_matches_single_contexts(c5_small, prec):
+    lines, models, _ = c5_small
+    bo = _batch(lines, models, prec)
+    evals = 0
+    for b, (setup, idx) in enumerate(models):
+        so = _single(setup, lines.take(idx), prec)
+        evals += so["evals"]
+        if prec == FSG_FP64:
+            # the same sums in the same order: identical
+            assert np.array_equal(bo["lop"][b], so["lop"])
+            assert np.array_equal(bo["sfl"][b], so["sfl"])
+            assert np.array_equal(bo["sfc"][b], so["sfc"])
+        else:
+            # the core-zone tile width (autotuned per grid) may group the compensated sums
+            # differently
+            scale = so["lop"].max() + 1e-300
+            assert np.max(np.abs(bo["lop"][b] - so["lop"])) <= 1e-6 * scale
+            assert np.max(np.abs(bo["sfl"][b] / bo["sfc"][b] - so["sfl"] / so["sfc"])) <= 1e-7
+            assert np.array_equal(bo["sfc"][b], so["sfc"])
+    assert bo["evals"] == evals
+
+
+def test_batch_flux_matches_oracle(c5_small):
+    lines, models, _ = c5_small
+    bo = _batch(lines, models[:2], FSG_FP64)
+    for b, (setup, idx) in enumerate(models[:2]):
+        o = Oracle(setup)
+        o.lineop(lines.take(idx))
+        o.radtran()
+        sfl, sfc = o.flux()
+        assert np.max(np.abs(bo["sfl"][b] / bo["sfc"][b] - sfl / sfc)) <= 1e-9
+
+
+def test_batch_reverse_mode_matches_single_contexts(c5_small):
+    lines, models, weights = c5_small
+    bo = _batch(lines, models, FSG_FP64, weights)
+    shared = np.zeros(lines.n)
+    for b, (setup, idx) in enumerate(models):
+        so = _single(setup, lines.take(idx), FSG_FP64, weights[b])
+        # dL/dT sums use atomic adds (order free): equal to rounding
+        assert np.allclose(bo["gt"][b], so["gt"], rtol=1e-10, atol=1e-12 * np.abs(so["gt"]).max())
+        assert np.allclose(bo["gfpf"][b], so["gfpf"], rtol=1e-10, atol=1e-12 * np.abs(so["gfpf"]).max())
+        assert np.allclose(bo["ggf"][b], so["ggf"], rtol=1e-10, atol=1e-12 * np.abs(so["ggf"]).max())
+        np.add.at(shared, idx, so["ggf"])
+    assert np.allclose(bo["ggf_shared"], shared, rtol=1e-10, atol=1e-12 * np.abs(shared).max())
+
+
+def test_batch_refusals(c5_small):
+    lines, models, _ = c5_small
+    setups = [s for s, _ in models[:2]]
+    with FastSpecBatch(setups, precision=FSG_FP64) as b:
+        shared = b.upload(lines)
+        with pytest.raises(FsgError, match="batch lists"):
+            b.lineop_dev(shared)  # a plain list on a batch context
+        with pytest.raises(FsgError, match="not sorted"):
+            b.batch_lines(shared, [models[0][1][::-1], models[1][1]])
+        shared.free()
+    with FastSpecGPU(setups[0], precision=FSG_FP64) as g, FastSpecBatch(setups, precision=FSG_FP64) as b:
+        shared = b.upload(lines)
+        bl = b.batch_lines(shared, [models[0][1], models[1][1]])
+        with pytest.raises(FsgError, match="batch lists"):
+            g.lineop_dev(bl)  # a batch list on an ordinary context
+        bl.free()
+        shared.free()
