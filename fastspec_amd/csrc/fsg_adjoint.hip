@@ -364,8 +364,8 @@ __device__ __forceinline__ double hvoigt64_d(const DevSetup &S, double A, int ia
 // per-(line, depth) records for the reverse mode: the FP64 path's coefficients with
 // d ln LinCop / dT, d ln LinVvgt / dT and d LinAvgt / dT in the record's spare words
 __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
-                             const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s, int4 *meta_s,
-                             int *slot_s) {
+                             const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv, double *wlc_s,
+                             int4 *meta_s, int *slot_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
   const DevSetup S = line_setup(S0, L, order[j]);
@@ -374,10 +374,12 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
   slot_s[j] = xc.IxPF;
   const int N = S.ntau;
   Coef64 *c = coef + (size_t)j * S.ntp;
+  CoefV *cv = coefv + (size_t)j * S.ntp;
   for (int k = N; k < S.ntp; k++) {
     Coef64 r{};
     r.ia = 1;
     c[k] = r;
+    cv[k] = CoefV{};  // padding depths: the series with zero coefficients everywhere
   }
   for (int k = 0; k < N; k++) {
     double cop, vv, av, ua;
@@ -411,6 +413,29 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
     r.pad2 = -dlnDopl;                                 // d ln LinVvgt / dT
     r.pad3 = dGam / (Dndopl * kXlnC4Pi) - av * dlnDopl;  // d LinAvgt / dT
     c[k] = r;
+    // far-wing series H = a/sqrt(pi) (u + f1 u^2 + f2 u^3), u = 1/v^2, of the one-pole
+    // fraction (the FP32 path's expansion), valid to 1e-7 relative where v^2 >= 400 (a^2 + 1/2):
+    //   c     = LinCop H
+    //   dc/dT = c dlnLinCop/dT + LinCop (v dH/dv dlnLinVvgt/dT + dH/da dLinAvgt/dT),
+    //   v dH/dv = -2 a/sqrt(pi) (u + 2 f1 u^2 + 3 f2 u^3)
+    //   dH/da   = 1/sqrt(pi) (u + f1 u^2 + f2 u^3) + a/sqrt(pi) (f1a u^2 + f2a u^3)
+    {
+      const double a2 = av * av, be = a2 + 0.5, al = 2.0 * a2 - 1.0;
+      const double f1 = be - al, f2 = -al * f1 - be * be;
+      const double f1a = -2.0 * av, f2a = -4.0 * av * f1 + 2.0 * av * al - 4.0 * av * be;  // d f/da
+      const double A0 = kSqrtPiInv * av, w = 1.0 / (vv * vv);
+      const double cA = cop * A0, c1 = cA * r.pad1, c2 = -2.0 * cA * r.pad2, c3 = cop * r.pad3;
+      CoefV q;
+      q.k1 = cA * w;
+      q.k2 = cA * f1 * w * w;
+      q.k3 = cA * f2 * w * w * w;
+      q.m1 = (c1 + c2 + c3 * kSqrtPiInv) * w;
+      q.m2 = (c1 * f1 + 2.0 * c2 * f1 + c3 * (kSqrtPiInv * f1 + A0 * f1a)) * w * w;
+      q.m3 = (c1 * f2 + 3.0 * c2 * f2 + c3 * (kSqrtPiInv * f2 + A0 * f2a)) * w * w * w;
+      q.dthr = sqrt(fmax(196.0, 400.0 * be)) / vv * (1.0 + 1e-12);
+      q.pad = 0.0;
+      cv[k] = q;
+    }
   }
 }
 
@@ -419,6 +444,7 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
 // per depth into sl[line][d]; sum gLop dc/dT stays in registers per (lane, depth).
 __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int nbins, int rstride,
                                                      const int2 *__restrict__ rng, const Coef64 *__restrict__ coef,
+                                                     const CoefV *__restrict__ coefv,
                                                      const double *__restrict__ wlc_s,
                                                      const int4 *__restrict__ meta_s, const double *__restrict__ glop,
                                                      double *__restrict__ sl, double *__restrict__ gtemp) {
@@ -457,55 +483,53 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
       if (m.y < p0 || m.x > p1) continue;
       const double wc = wlc_s[j];
       const Coef64 *c = coef + (size_t)j * S.ntp + d0;
+      const CoefV *cv = coefv + (size_t)j * S.ntp + d0;
       const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
+      // distance from the line centre to the points the line adds to here (window and tile;
+      // 0 if the centre lies among them): at a depth whose series threshold it reaches, every
+      // such point is in the far wing (wave-uniform)
+      const double Wlo = S.wave[max(m.x, p0) - 1], Whi = S.wave[min(m.y, p1) - 1];
+      const double dmin = hs ? 0.0 : ((wc < Wlo) ? (Wlo - wc) : (wc > Whi ? (wc - Whi) : 0.0));
+      bool act[PPL];
+      double dl[PPL], T1[PPL], T2[PPL], T3[PPL];
+#pragma unroll
+      for (int q = 0; q < PPL; q++) {
+        act[q] = valid[q] && pp[q] >= m.x && pp[q] <= m.y;
+        dl[q] = Wp[q] - wc;
+        // T = 1/dlambda^2, zero outside the window: the series adds nothing there
+        const double d2 = act[q] ? dl[q] * dl[q] : 1.0;
+        double r = __builtin_amdgcn_rcp(d2);
+        r = r * (2.0 - d2 * r);  // Newton steps to full FP64 precision
+        r = r * (2.0 - d2 * r);
+        T1[q] = act[q] ? r : 0.0;
+        T2[q] = T1[q] * T1[q];
+        T3[q] = T2[q] * T1[q];
+      }
       double sk[DPW];  // per depth: this lane's share of the sum over the tile's points
 #pragma unroll
       for (int k = 0; k < DPW; k++) {
-        const Coef64 ck = c[k];
+        const CoefV q3 = cv[k];
         double s = 0.0;
-        bool act[PPL];
-        double v[PPL];
-        bool near = false;
-        // far-wing series of the one-pole fraction (the FP32 path's expansion): where every
-        // point of the wavefront has v^2 >= 400 (a^2 + 1/2), H = a/sqrt(pi) u (1 + f1 u + f2 u^2),
-        // u = 1/v^2, to 1e-7 relative, and its derivatives in closed form
-        const double a = ck.a, a2 = a * a, be = a2 + 0.5, al = 2.0 * a2 - 1.0;
-        const double thr = fmax(196.0, 400.0 * be);
-#pragma unroll
-        for (int q = 0; q < PPL; q++) {
-          act[q] = valid[q] && pp[q] >= m.x && pp[q] <= m.y;
-          const double dld = hs ? (Wp[q] - wc) - hs[k] : (Wp[q] - wc);
-          v[q] = dld * ck.vv;
-          near |= act[q] && v[q] * v[q] < thr;
-        }
-        if (__builtin_amdgcn_ballot_w64(near) == 0) {
-          const double f1 = be - al, f2 = -al * f1 - be * be;
-          const double f1a = -2.0 * a, f2a = -4.0 * a * f1 + 2.0 * a * al - 4.0 * a * be;  // d f/da
-          const double A0 = kSqrtPiInv * a;
+        if (dmin >= q3.dthr) {
+          // far wing at every point: the series in T with the precomputed coefficients
 #pragma unroll
           for (int q = 0; q < PPL; q++) {
-            if (!act[q]) continue;
-            const double v2 = v[q] * v[q];
-            double u = __builtin_amdgcn_rcp(v2);
-            u = u * (2.0 - v2 * u);  // one Newton step
-            const double P = 1.0 + u * (f1 + f2 * u);
-            const double H = A0 * u * P;
-            const double Hv = -2.0 * A0 * u * (v[q] * u) * (1.0 + u * (2.0 * f1 + 3.0 * f2 * u));
-            const double Ha = kSqrtPiInv * u * P + A0 * u * u * (f1a + f2a * u);
-            const double cval = H * ck.cop;
-            s += g[q][k] * cval;
-            tT[k] += g[q][k] * (cval * ck.pad1 + ck.cop * (Hv * v[q] * ck.pad2 + Ha * ck.pad3));
+            const double x1 = g[q][k] * T1[q], x2 = g[q][k] * T2[q], x3 = g[q][k] * T3[q];
+            s += q3.k1 * x1 + q3.k2 * x2 + q3.k3 * x3;
+            tT[k] += q3.m1 * x1 + q3.m2 * x2 + q3.m3 * x3;
           }
         } else {
+          const Coef64 ck = c[k];
 #pragma unroll
           for (int q = 0; q < PPL; q++) {
             if (!act[q]) continue;
+            const double v = (hs ? dl[q] - hs[k] : dl[q]) * ck.vv;
             double Hv, Ha;
-            const double H = hvoigt64_d(S, ck.a, ck.ia, ck.ua, v[q], Hv, Ha);
+            const double H = hvoigt64_d(S, ck.a, ck.ia, ck.ua, v, Hv, Ha);
             const double cval = H * ck.cop;
             s += g[q][k] * cval;
             // d c / dT = c d ln LinCop/dT + LinCop (Hv v d ln LinVvgt/dT + Ha d LinAvgt/dT)
-            tT[k] += g[q][k] * (cval * ck.pad1 + ck.cop * (Hv * v[q] * ck.pad2 + Ha * ck.pad3));
+            tT[k] += g[q][k] * (cval * ck.pad1 + ck.cop * (Hv * v * ck.pad2 + Ha * ck.pad3));
           }
         }
         sk[k] = s;
@@ -565,24 +589,24 @@ void launch_rt_vjp(const DevSetup &S, const void *lop, int fp32, const double *g
 }
 
 void launch_fill64_vjp(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
-                       const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
-                       int4 *meta_s, int *slot_s, hipStream_t st) {
+                       const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv,
+                       double *wlc_s, int4 *meta_s, int *slot_s, hipStream_t st) {
   if (nret)
-    k_fill64_vjp<<<nb(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, wlc_s, meta_s,
-                                                slot_s);
+    k_fill64_vjp<<<nb(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, coefv, wlc_s,
+                                                meta_s, slot_s);
 }
 
 void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
-                       const double *wlc_s, const int4 *meta_s, const double *glop, double *sl, double *gtemp,
-                       hipStream_t st) {
+                       const CoefV *coefv, const double *wlc_s, const int4 *meta_s, const double *glop, double *sl,
+                       double *gtemp, hipStream_t st) {
   if (ntiles <= 0 || nbins <= 0) return;
   const int nw = S.ntp / DPW;
   const int nb = 8 * ((ntiles + 7) / 8);
   // enough workgroups for about 8 wavefronts per SIMD of the 256 CUs (C5's small windows have
   // a few dozen tiles; C2 has thousands and keeps one workgroup per tile)
   const int nslice = std::max(1, std::min(64, (8 * 1024) / std::max(1, nb * nw)));
-  k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, wlc_s, meta_s, glop, sl,
-                                                     gtemp);
+  k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, coefv, wlc_s, meta_s, glop,
+                                                     sl, gtemp);
 }
 
 void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *atm, const int *slot_s,

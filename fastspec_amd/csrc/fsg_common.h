@@ -83,6 +83,13 @@ struct alignas(64) Coef64 {
   int ia, pad0;
   double pad1, pad2, pad3;
 };
+// Reverse mode, far wings: where the one-pole fraction's series holds (|dlambda| >= dthr),
+// a (line, depth) contributes c = sum_n k_n T^n and dc/dT = sum_n m_n T^n (n = 1..3) with
+// T = 1/dlambda^2 shared by all depths of a point (u = 1/v^2 = T / LinVvgt^2 folded into the
+// coefficients).
+struct alignas(64) CoefV {
+  double k1, k2, k3, m1, m2, m3, dthr, pad;
+};
 
 // Everything the kernels read from the set-up (device pointers + scalars).
 struct DevSetup {

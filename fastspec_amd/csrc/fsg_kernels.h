@@ -85,11 +85,11 @@ size_t rt_vjp_scratch_per_point(const DevSetup &S);
 void launch_rt_vjp(const DevSetup &S, const void *lop, int fp32, const double *gsfl, const double *gsfc, int p_lo,
                    int np, double *scr, double *glop, double *gcont, double *gtemp, hipStream_t st);
 void launch_fill64_vjp(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
-                       const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s,
-                       int4 *meta_s, int *slot_s, hipStream_t st);
+                       const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv,
+                       double *wlc_s, int4 *meta_s, int *slot_s, hipStream_t st);
 void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
-                       const double *wlc_s, const int4 *meta_s, const double *glop, double *sl, double *gtemp,
-                       hipStream_t st);
+                       const CoefV *coefv, const double *wlc_s, const int4 *meta_s, const double *glop, double *sl,
+                       double *gtemp, hipStream_t st);
 void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *atm, const int *slot_s,
                                const double *sl, double *g_loggf, double *g_fpf, hipStream_t st);
 

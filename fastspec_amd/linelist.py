@@ -150,6 +150,8 @@ class StreamedLines:
         kept = C.c_int64()
         _lib.check(self._lib.fsg_lines_stream(synth.ctx, str(path).encode(), wstart, wend, tab, ns, itab, ni,
                                               mol_slot, nthreads, C.byref(self.handle), C.byref(kept)))
+        from .synth import _LIVE
+        _LIVE.add(self)  # released at exit before the contexts
         self.n = kept.value
         self.list_type = scan(path, wstart, wend)[0]
 

@@ -13,13 +13,31 @@ Errors raised by the library carry the reference's ErrStop messages where it has
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from typing import Optional
 
 import numpy as np
 
 from . import _lib
 from .abi import FSG_FP32, FSG_FP64, CStats, LineList, Setup, ptr
+
+# Contexts and device lists not yet released.  Interpreter teardown runs finalizers in no
+# set order and may run them after parts of the runtime are gone, so whatever the caller
+# has not closed is released here at exit, device lists before the contexts they live in.
+_LIVE: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    live = list(_LIVE)
+    for o in live:
+        if not isinstance(o, FastSpecGPU):
+            o.free()
+    for o in live:
+        if isinstance(o, FastSpecGPU):
+            o.close()
 
 
 class DeviceLines:
@@ -33,6 +51,7 @@ class DeviceLines:
         cl = lines.cstruct()
         self.handle = C.c_void_p()
         _lib.check(self._lib.fsg_lines_upload(synth.ctx, lines.list_type, C.byref(cl), C.byref(self.handle)))
+        _LIVE.add(self)
 
     def free(self):
         if self.handle:
@@ -57,6 +76,7 @@ class FastSpecGPU:
         self._cs = setup.cstruct(precision=precision, batched=batched, device=device)
         self.ctx = C.c_void_p()
         _lib.check(self._lib.fsg_init(C.byref(self._cs), C.byref(self.ctx)))
+        _LIVE.add(self)
 
     def close(self):
         if self.ctx:
@@ -273,6 +293,7 @@ class BatchLines:
         self.handle = C.c_void_p()
         _lib.check(self._lib.fsg_lines_batch(batch.ctx, shared.handle, ptr(self.index, C.c_int64),
                                              ptr(self.atm, C.c_int32), self.n, C.byref(self.handle)))
+        _LIVE.add(self)
 
     def per_atmosphere(self, a):
         """Splits an array over the batch list's lines into the atmospheres' parts."""
@@ -302,6 +323,7 @@ class FastSpecBatch(FastSpecGPU):
         arr = (CSetup * self.nb)(*self._cs)
         self.ctx = C.c_void_p()
         _lib.check(self._lib.fsg_init_batch(arr, self.nb, C.byref(self.ctx)))
+        _LIVE.add(self)
         self.off = np.zeros(self.nb + 1, np.int64)
         _lib.check(self._lib.fsg_batch_offsets(self.ctx, ptr(self.off, C.c_int64)))
         self.nwl_total = int(self.off[-1])
