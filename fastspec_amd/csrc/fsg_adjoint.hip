@@ -330,29 +330,38 @@ __device__ __forceinline__ double hvoigt64_d(const DevSetup &S, double A, int ia
   const double zr = av, zi = A;
   const double z2r = v2 - a2, z2i = 2.0 * av * A;
   double H, wr, wi;  // w' = wr + i wi
+  // one reciprocal per point: 1/|d|^2 with two Newton steps (full FP64 precision), the
+  // derivative's denominator being |d^2|^2 = |d|^4
+  auto inv = [](double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = r * (2.0 - x * r);
+    return r * (2.0 - x * r);
+  };
   if (s >= 196.0) {
-    H = kSqrtPiInv * A * (s + 0.5) / (s * s - v2 + a2 + 0.25);
+    // d = z^2 - 1/2, |d|^2 = s^2 - v^2 + a^2 + 1/4
+    const double dr = z2r - 0.5, di = z2i;
+    const double rd = inv(dr * dr + di * di), rdd = rd * rd;
+    H = kSqrtPiInv * A * (s + 0.5) * rd;
     // w' = -i K (z^2 + 1/2) / (z^2 - 1/2)^2
     const double nr = z2r + 0.5, ni = z2i;
-    const double dr = z2r - 0.5, di = z2i;
-    const double d2r = dr * dr - di * di, d2i = 2.0 * dr * di, dd = d2r * d2r + d2i * d2i;
-    const double qr = (nr * d2r + ni * d2i) / dd, qi = (ni * d2r - nr * d2i) / dd;
+    const double d2r = dr * dr - di * di, d2i = 2.0 * dr * di;
+    const double qr = (nr * d2r + ni * d2i) * rdd, qi = (ni * d2r - nr * d2i) * rdd;
     wr = kSqrtPiInv * qi;
     wi = -kSqrtPiInv * qr;
   } else {
+    // g = z^3 - 3z/2 = z (z^2 - 3/2) = dr + i di
     const double p = v2 - a2 - 1.5, q = 2.0 * av * A;
     const double dr = av * p - A * q, di = av * q + A * p;
-    H = kSqrtPiInv * ((p + 0.5) * di - q * dr) / (dr * dr + di * di);
-    // f = (z^2 - 1) / (z^3 - 3z/2); f' = (2z (z^3 - 3z/2) - (z^2 - 1)(3z^2 - 3/2)) / (z^3 - 3z/2)^2;
-    // w' = i K f'
-    const double z3r = z2r * zr - z2i * zi, z3i = z2r * zi + z2i * zr;
-    const double gr = z3r - 1.5 * zr, gi = z3i - 1.5 * zi;     // g = z^3 - 3z/2
+    const double rg = inv(dr * dr + di * di), rgg = rg * rg;
+    H = kSqrtPiInv * ((p + 0.5) * di - q * dr) * rg;
+    // f = (z^2 - 1) / g; f' = (2z g - (z^2 - 1)(3z^2 - 3/2)) / g^2; w' = i K f'
+    const double gr = dr, gi = di;
     const double t1r = 2.0 * (zr * gr - zi * gi), t1i = 2.0 * (zr * gi + zi * gr);
     const double ar = z2r - 1.0, ai = z2i, br = 3.0 * z2r - 1.5, bi = 3.0 * z2i;
     const double t2r = ar * br - ai * bi, t2i = ar * bi + ai * br;
     const double nr = t1r - t2r, ni = t1i - t2i;
-    const double g2r = gr * gr - gi * gi, g2i = 2.0 * gr * gi, dd = g2r * g2r + g2i * g2i;
-    const double fr = (nr * g2r + ni * g2i) / dd, fi = (ni * g2r - nr * g2i) / dd;
+    const double g2r = gr * gr - gi * gi, g2i = 2.0 * gr * gi;
+    const double fr = (nr * g2r + ni * g2i) * rgg, fi = (ni * g2r - nr * g2i) * rgg;
     wr = -kSqrtPiInv * fi;
     wi = kSqrtPiInv * fr;
   }

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -69,6 +70,17 @@ struct DBuf {
 
 }  // namespace
 
+// pass 1 of a line list: bin populations and maximal half-widths, evaluation count
+struct LinePrep {
+  int nbins = 0;
+  int64_t nmain = 0, nhe = 0, nret = 0;
+  int hcnt[64], hhmax[64];
+  unsigned long long hu[4];
+};
+
+// identities of device line lists (never reused within a process)
+static std::atomic<uint64_t> g_list_id{1};
+
 struct fsg_ctx {
   int dev = 0;
   hipStream_t st = nullptr;
@@ -118,9 +130,15 @@ struct fsg_ctx {
   // point of each atmosphere in the concatenated grid (nb + 1 entries, the last the total)
   std::vector<fsg_ctx *> kids;
   std::vector<int64_t> boff;
+  // the list (identity, gf generation) whose pass-1 results (flags, windows, bin order) the
+  // pass-1 buffers hold, for the reverse mode of the same synthesis; id 0: none
+  uint64_t p1_id = 0, p1_gen = 0;
+  LinePrep p1;
 };
 
 struct fsg_dlines {
+  uint64_t id = g_list_id++;
+  uint64_t gen = 0;  // bumped when the gf column changes
   fsg_ctx *ctx = nullptr;
   int list_type = 0;
   DevLines L{};
@@ -1051,14 +1069,9 @@ namespace {
 // pass 1 of LineopCalc on a device list, the evaluation count and the stable bin-major
 // order of the lines (retained lines keep their index order per bin); returns the bin
 // populations.  Retained lines: width bins 0..nbins-1, then the He I Stark lines (HE_KEY).
-struct LinePrep {
-  int nbins = 0;
-  int64_t nmain = 0, nhe = 0, nret = 0;
-  int hcnt[64], hhmax[64];
-  unsigned long long hu[4];
-};
 
 int scalar_pass1(fsg_ctx *c, fsg_dlines *dl, LinePrep &P) {
+  c->p1_id = 0;  // the buffers are overwritten from here on
   const int64_t n = dl->L.n;
   const DevSetup &S = c->S;
   const int N = S.ntau;
@@ -1099,7 +1112,20 @@ int scalar_pass1(fsg_ctx *c, fsg_dlines *dl, LinePrep &P) {
   P.nhe = P.hcnt[HE_KEY];
   P.nret = P.nmain + P.nhe;
   if (P.nbins > MAX_BINS) return fail("LineopCalc: too many window-width bins");
+  c->p1_id = dl->id;
+  c->p1_gen = dl->gen;
+  c->p1 = P;
   return 0;
+}
+
+// pass 1 of the list, or the results still held from the last pass 1 of the same list with
+// the same gf (the forward synthesis before its reverse mode)
+int scalar_pass1_cached(fsg_ctx *c, fsg_dlines *dl, LinePrep &P) {
+  if (c->p1_id == dl->id && c->p1_gen == dl->gen) {
+    P = c->p1;
+    return 0;
+  }
+  return scalar_pass1(c, dl, P);
 }
 
 // groups of the bin-sorted retained lines: offsets, sizes and maximal half-widths of the
@@ -1432,6 +1458,7 @@ int fsg_maglineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, const doub
   const double bsplit = (bframe == 0)
                             ? kKz * std::sqrt(bfield[0] * bfield[0] + bfield[1] * bfield[1] + bfield[2] * bfield[2])
                             : kKz * std::fabs(bfield[0]);
+  c->p1_id = 0;  // the pass-1 buffers now hold the Zeeman list's results
   CK(c->flag.ensure(n * sizeof(int)));
   CK(c->il.ensure(n * sizeof(int)));
   CK(c->ir.ensure(n * sizeof(int)));
@@ -1564,6 +1591,7 @@ int fsg_lines_set_gf(fsg_dlines *dl, const float *gf, int32_t on_device) {
   if (dl->L.n == 0) return 0;
   fsg_ctx *c = dl->ctx;
   CK(hipSetDevice(c->dev));
+  dl->gen++;  // windows and flags of the list change with gf
   CK(hipMemcpyAsync((void *)dl->L.gf, gf, sizeof(float) * (size_t)dl->L.n,
                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->st));
   CK(hipStreamSynchronize(c->st));
@@ -1640,7 +1668,7 @@ int fsg_lineop_vjp(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
   CK(hipMemsetAsync(c->vout.p, 0, sizeof(double) * ((size_t)std::max<int64_t>(n, 1) + nfpf + NB * N), s));
   if (n > 0) {
     LinePrep P;
-    if (scalar_pass1(c, dl, P)) return 1;
+    if (scalar_pass1_cached(c, dl, P)) return 1;
     if (P.nret > 0) {
       const int ng = P.nbins + 1;
       std::vector<int64_t> goff, gcnt;
