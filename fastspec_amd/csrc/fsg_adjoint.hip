@@ -372,7 +372,7 @@ __device__ __forceinline__ double hvoigt64_d(const DevSetup &S, double A, int ia
 
 // per-(line, depth) records for the reverse mode: the FP64 path's coefficients with
 // d ln LinCop / dT, d ln LinVvgt / dT and d LinAvgt / dT in the record's spare words
-__global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+__global__ void __launch_bounds__(128) k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                              const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv, double *wlc_s,
                              int4 *meta_s, int *slot_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -614,6 +614,8 @@ void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, co
   // enough workgroups for about 8 wavefronts per SIMD of the 256 CUs (C5's small windows have
   // a few dozen tiles; C2 has thousands and keeps one workgroup per tile)
   const int nslice = std::max(1, std::min(64, (8 * 1024) / std::max(1, nb * nw)));
+  // the 1024-thread bound keeps 128 registers and 4 wavefronts per SIMD, which outruns the
+  // spill-free 138-register build at 3 (C5: 86 vs 117 ms)
   k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, coefv, wlc_s, meta_s, glop,
                                                      sl, gtemp);
 }

@@ -92,7 +92,7 @@ __global__ void k_xline_init(int ntau, const double *temp, const double *xne, co
 // maxima (bound test, centre test, analytic window) are streamed in one depth loop; the
 // decisions are taken afterwards in the reference's order.
 // ------------------------------------------------------------------------------------
-__global__ void k_pass1(DevSetup S0, DevLines L, int list_type, int *flag_out, int *il_out, int *ir_out,
+__global__ void __launch_bounds__(128) k_pass1(DevSetup S0, DevLines L, int list_type, int *flag_out, int *il_out, int *ir_out,
                         int *ic_out, unsigned *key_out, int *bin_count, int *bin_hmax, int batched) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= L.n) return;
@@ -237,7 +237,7 @@ __global__ void k_retained_flag(int64_t n, const int *flag, int *out) {
 // ------------------------------------------------------------------------------------
 // coefficient fill: one lane per retained line in bin-sorted order
 // ------------------------------------------------------------------------------------
-__global__ void k_fill32(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+__global__ void __launch_bounds__(128) k_fill32(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                          const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
                          int2 *zone_s, LineRec *lrec, double *wlc_s, int4 *meta_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -322,7 +322,7 @@ __global__ void k_fill32(DevSetup S0, DevLines L, int list_type, int64_t nret, c
 }
 
 
-__global__ void k_fill64(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+__global__ void __launch_bounds__(128) k_fill64(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                          const int *il, const int *ir, const int *ic, Coef64 *coef, double *wlc_s, int4 *meta_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;

@@ -196,7 +196,7 @@ __device__ double mag_bound_max(const DevSetup &S, const XC &xc, const MagLine &
   return mx;
 }
 
-__global__ void k_magpass1(DevSetup S, DevLines L, MagLine Z, double bsplit, int *flag_out, int *il_out,
+__global__ void __launch_bounds__(64) k_magpass1(DevSetup S, DevLines L, MagLine Z, double bsplit, int *flag_out, int *il_out,
                            int *ir_out, int *ic_out, unsigned *key_out, int *bin_count, int *bin_hmax, int batched) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= L.n) return;
