@@ -372,7 +372,7 @@ __device__ __forceinline__ double hvoigt64_d(const DevSetup &S, double A, int ia
 
 // per-(line, depth) records for the reverse mode: the FP64 path's coefficients with
 // d ln LinCop / dT, d ln LinVvgt / dT and d LinAvgt / dT in the record's spare words
-__global__ void __launch_bounds__(128) k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
+__global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                              const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv, double *wlc_s,
                              int4 *meta_s, int *slot_s) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -153,9 +153,11 @@ namespace {
 // Autotuner of the FP32 opacity launch geometry.  Candidates: the core-zone kernel's tile
 // (64 or 128 points) and the matrix-core kernel's LDS ring depth (2 or 4 group slots); the
 // far-wing kernel's results do not depend on its ring depth, the core-zone kernel's only
-// through the grouping of its blocked compensated sums.  Each candidate runs twice on the
-// chunk at hand into the scratch accumulator lop2 (zeroed afterwards), and the faster of its
-// two device times counts.  Choices are kept per process for a workload signature (device,
+// through the grouping of its blocked compensated sums.  Each of the four pairs runs twice on
+// the chunk at hand as the accumulation stage runs it -- both kernels at once on the two
+// streams, into scratch accumulators -- and the pair with the shortest wall time wins (alone,
+// the wider core tile can be faster while it slows the pair down by taking more registers
+// beside the matrix-core kernel).  Choices are kept per process for a workload signature (device,
 // depth count, grid size and chunk size in powers of two) and, with FSG_AUTOTUNE_CACHE set
 // to a file name, in that file ("signature core_pp far_ring" lines).  FSG_AUTOTUNE=0 keeps
 // the defaults (128-point core tiles, 4 ring slots) without timing.
@@ -224,43 +226,61 @@ static int autotune(fsg_ctx *c, int ntiles, int64_t m, const CoefA *ca, const Co
   if (en && en[0] == '0') return 0;
   const DevSetup &S = c->S;
   hipStream_t s = c->st;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  // every candidate pair runs as in the accumulation stage: the far-wing kernel on the
+  // library stream and the core-zone kernel on the second stream at once, each into its own
+  // scratch accumulator (lop2, and tune_scr released afterwards); the pair's wall time counts
+  DBuf tune_scr;
+  const size_t scr_bytes = sizeof(float) * (size_t)S.ntau * S.nwl;
+  if (tune_scr.ensure(scr_bytes) != hipSuccess) {  // no room to time the pair: keep the defaults
+    (void)hipGetLastError();
+    return 0;
+  }
+  CK(hipMemsetAsync(tune_scr.p, 0, scr_bytes, s));
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
-  auto timed = [&](auto &&launch, float &best) -> int {
-    for (int rep = 0; rep < 2; rep++) {
-      CK(hipEventRecord(e0, s));
-      launch();
-      CK(hipEventRecord(e1, s));
-      CK(hipEventSynchronize(e1));
-      float ms = 0.f;
-      CK(hipEventElapsedTime(&ms, e0, e1));
-      best = std::min(best, ms);
-    }
-    return 0;
-  };
+  CK(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
   const int rings[2] = {2, 4}, pps[2] = {1, 2};
-  float tf[2] = {1e30f, 1e30f}, tc[2] = {1e30f, 1e30f};
+  float best = 1e30f;
+  int bring = c->far_ring, bpp = c->core_pp;
   int rc = 0;
-  for (int i = 0; i < 2 && !rc; i++)
-    rc = timed(
-        [&] {
-          launch_opacity32(S, ntiles, 0, 0, nullptr, ca, cb, dmaxw, lrec, nullptr, nullptr, c->foff.as<long long>(),
-                           c->flist.as<int>(), m, c->lop2, 0, rings[i], s);
-        },
-        tf[i]);
-  for (int i = 0; i < 2 && !rc; i++) {
-    launch_core_ranges(pps[i], S.nwl, zoff, zend, c->zkey2.as<unsigned long long>(), c->rng2.as<int2>(), s);
-    rc = timed([&] { launch_opacity32_core(S, pps[i], c->rng2.as<int2>(), c->zrec.as<ZoneRec>(), ca, cb, dmaxw, c->lop2, s); },
-               tc[i]);
+  for (int ip = 0; ip < 2 && !rc; ip++) {
+    launch_core_ranges(pps[ip], S.nwl, zoff, zend, c->zkey2.as<unsigned long long>(), c->rng2.as<int2>(), s);
+    for (int ir = 0; ir < 2 && !rc; ir++) {
+      for (int rep = 0; rep < 2 && !rc; rep++) {
+        float ms = 0.f;
+        if (hipEventRecord(e0, s) != hipSuccess || hipStreamWaitEvent(c->st2, e0, 0) != hipSuccess) {
+          rc = fail("autotune: event record failed");
+          break;
+        }
+        launch_opacity32(S, ntiles, 0, 0, nullptr, ca, cb, dmaxw, lrec, nullptr, nullptr, c->foff.as<long long>(),
+                         c->flist.as<int>(), m, c->lop2, 0, rings[ir], s);
+        launch_opacity32_core(S, pps[ip], c->rng2.as<int2>(), c->zrec.as<ZoneRec>(), ca, cb, dmaxw,
+                              tune_scr.as<float>(), c->st2);
+        if (hipEventRecord(e2, c->st2) != hipSuccess || hipStreamWaitEvent(s, e2, 0) != hipSuccess ||
+            hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = fail("autotune: timing failed");
+          break;
+        }
+        if (ms < best) {
+          best = ms;
+          bring = rings[ir];
+          bpp = pps[ip];
+        }
+      }
+    }
   }
   launch_zero_f32(c->lop2, (int64_t)S.ntau * S.nwl, s);
+  (void)hipStreamSynchronize(s);
+  tune_scr.release();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
   if (rc) return rc;
   CK(hipGetLastError());
-  c->far_ring = tf[0] < tf[1] ? 2 : 4;
-  c->core_pp = tc[0] < tc[1] ? 1 : 2;
+  c->far_ring = bring;
+  c->core_pp = bpp;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune[key] = {c->core_pp, c->far_ring};
   tune_write_file(key, c->core_pp, c->far_ring);

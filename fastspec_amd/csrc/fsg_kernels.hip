@@ -992,21 +992,24 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
           }
       }
       // far wing: straight-line FMA code over all depths (zero coefficients at the general
-      // depths); dl = (point - first point) + (first point - centre), both rounded to FP32
-      const float bs = (float)(Wt0 - wc);
-      float T2[CP];
-#pragma unroll
-      for (int q = 0; q < CP; q++) {
-        const float xx = ofs[q] + bs;
-        T2[q] = act[q] ? __builtin_amdgcn_rcpf(fmaxf(xx * xx, 1e-30f)) : 0.0f;
-      }
-#pragma unroll
-      for (int k = 0; k < DPW; k++)
+      // depths), skipped when every depth of the wave takes the general path (wave-uniform);
+      // dl = (point - first point) + (first point - centre), both rounded to FP32
+      if (gen != (1u << DPW) - 1u) {
+        const float bs = (float)(Wt0 - wc);
+        float T2[CP];
 #pragma unroll
         for (int q = 0; q < CP; q++) {
-          const float xx = T2[q];
-          part[q][k] += xx * fmaf(fmaf(rr[k][2], xx, rr[k][1]), xx, rr[k][0]);
+          const float xx = ofs[q] + bs;
+          T2[q] = act[q] ? __builtin_amdgcn_rcpf(fmaxf(xx * xx, 1e-30f)) : 0.0f;
         }
+#pragma unroll
+        for (int k = 0; k < DPW; k++)
+#pragma unroll
+          for (int q = 0; q < CP; q++) {
+            const float xx = T2[q];
+            part[q][k] += xx * fmaf(fmaf(rr[k][2], xx, rr[k][1]), xx, rr[k][0]);
+          }
+      }
       if (gen) {
         // the wave's general-path records all in one batch of scalar loads
         const CoefB *cbp = cb + (size_t)j * S.ntp + d0;
