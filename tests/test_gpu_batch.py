@@ -134,3 +134,50 @@ def test_batch_refusals(c5_small):
             g.lineop_dev(bl)  # a batch list on an ordinary context
         bl.free()
         shared.free()
+
+
+@pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])
+def test_batch_in_chunks(c5_small, prec):
+    """NlinesLim below the batch list's size: chunks of coefficient records mix the
+    atmospheres' lines; the opacity is the same sum in another order."""
+    lines, models, _ = c5_small
+    setups = [s for s, _ in models]
+    with FastSpecBatch(setups, precision=prec) as b:
+        shared = b.upload(lines)
+        bl = b.batch_lines(shared, [idx for _, idx in models])
+        st = b.lineop_dev(bl, NlinesLim=max(bl.n // 5, 1))
+        lop = b.get_lineop()
+        bl.free()
+        shared.free()
+    assert st["nblocks"] >= 2
+    tol = 1e-12 if prec == FSG_FP64 else 1e-6
+    for k, (setup, idx) in enumerate(models):
+        so = _single(setup, lines.take(idx), prec)
+        assert np.max(np.abs(lop[k] - so["lop"])) <= tol * so["lop"].max()
+
+
+def test_open_objects_released_at_exit(tmp_path):
+    """Contexts and lists left open are released by the wrappers at interpreter exit (device
+    lists first) and the process ends cleanly."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    child = (
+        "import sys; sys.path.insert(0, sys.argv[1])\n"
+        "from fastspec_amd import workloads as W\n"
+        "from fastspec_amd.abi import FSG_FP32\n"
+        "from fastspec_amd.synth import FastSpecBatch, FastSpecGPU\n"
+        "lines, models = W.config_c5(nmodels=3, nlines=2000)\n"
+        "b = FastSpecBatch([s for s, _ in models], precision=FSG_FP32)\n"
+        "shared = b.upload(lines)\n"
+        "bl = b.batch_lines(shared, [i for _, i in models])\n"
+        "b.lineop_dev(bl)\n"
+        "g = FastSpecGPU(models[0][0], precision=FSG_FP32)\n"
+        "dl = g.upload(lines.take(models[0][1]))\n"
+        "g.lineop_dev(dl)\n"
+        "print('done', flush=True)\n"
+    )
+    r = subprocess.run([sys.executable, "-c", child, str(root)], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "done" in r.stdout
