@@ -243,7 +243,7 @@ def run_c5(steps: int, warmup: int, contexts: bool = False):
     return out
 
 
-def cpu_baseline(budget_s: float = 20.0):
+def cpu_baseline():
     """The reference's own OpenMP kernels (oracle/_ref, built from the reference sources)
     on a bounded sample of the same workload: the C2 grid and line density restricted to
     a wavelength range around 5000 A.  Times LineOpacity1 over the sample grid and the
@@ -255,7 +255,7 @@ def cpu_baseline(budget_s: float = 20.0):
 
     if not ref_available():
         return None
-    width = 1000.0  # about 15 s of the reference kernels on 16 host cores
+    width = 500.0  # about 25 s of the reference kernels on 16 host cores (1000 A took 47 s)
     setup, lines = W.config_c2(wl_lo=5000.0, wl_hi=5000.0 + width)
     orc = Oracle(setup)
     ro = orc.pass1(lines)
