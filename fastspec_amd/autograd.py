@@ -11,13 +11,19 @@ which dL/dT and dL/d ln FractionPF are taken: the host's EOS and continuum, whic
 abundances onto the library's inputs, stay outside, so a new atmosphere needs a new
 context.  The element-abundance gradient of a trace species is the sum of its slots' rows:
 dL/d log10 A = ln 10 sum_depth dL/d ln FractionPF.
+
+    F/Fc (all atmospheres) = NormalisedFluxBatch.apply(loggf, temp, lnfpf, batch, blines, nlim)
+
+is the same over a batch context (many atmospheres in one launch sequence, one shared
+list): temp [nb][ntau], lnfpf [nb][ntau][pf_ld], and the gradient in loggf summed over the
+atmospheres that use each line.
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 
-from .synth import DeviceLines, FastSpecGPU
+from .synth import BatchLines, DeviceLines, FastSpecBatch, FastSpecGPU
 
 
 class NormalisedFlux(torch.autograd.Function):
@@ -69,4 +75,55 @@ def normalised_flux(synth: FastSpecGPU, dlines: DeviceLines, loggf: torch.Tensor
     return NormalisedFlux.apply(loggf, temp, lnfpf, synth, dlines, nlines_lim)
 
 
-__all__ = ["NormalisedFlux", "normalised_flux"]
+class NormalisedFluxBatch(torch.autograd.Function):
+    """F/Fc of every atmosphere of a batch context (fsg_init_batch), concatenated over the
+    atmospheres' own grids, as a function of the shared list's log gf, the temperatures
+    [nb][ntau] and ln FractionPF [nb][ntau][pf_ld] (zeros: the contexts' values)."""
+
+    @staticmethod
+    def forward(ctx, loggf, temp, lnfpf, batch: FastSpecBatch, blines: BatchLines, nlines_lim=None):
+        t_now = torch.as_tensor(np.stack([su.temp for su in batch.setups]), dtype=torch.float64)
+        if not torch.equal(temp.detach().to("cpu", torch.float64), t_now):
+            raise ValueError("NormalisedFluxBatch: temp differs from the batch's atmospheres; build a new batch")
+        if bool(torch.any(lnfpf.detach() != 0)):
+            raise ValueError("NormalisedFluxBatch: lnfpf must be zero (FractionPF is fixed by the batch)")
+        gf = torch.pow(torch.tensor(10.0, dtype=torch.float32), loggf.detach().to("cpu").float())
+        batch.set_gf(blines, gf.numpy())
+        batch.reset_lineop()
+        batch.lineop_dev(blines, nlines_lim)
+        batch.RadTranCalc(intensities=False)
+        sfl, sfc = batch.flux()
+        ctx.batch, ctx.blines, ctx.nlim = batch, blines, nlines_lim
+        ctx.sfl, ctx.sfc = sfl, sfc
+        ctx.meta = [(x.dtype, x.device) for x in (loggf, temp, lnfpf)]
+        f = np.concatenate([l / c for l, c in zip(sfl, sfc)])
+        return torch.as_tensor(f, dtype=torch.float64, device=loggf.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        g = g.detach().to("cpu", torch.float64).numpy()
+        sfl, sfc = ctx.sfl, ctx.sfc
+        cut = np.cumsum([len(x) for x in sfl])[:-1]
+        gb = np.split(g, cut)
+        _, _, gt_src = ctx.batch.flux_vjp([a / c for a, c in zip(gb, sfc)],
+                                          [-a * l / (c * c) for a, l, c in zip(gb, sfl, sfc)])
+        g_loggf, g_fpf, gt_line = ctx.batch.lineop_vjp(ctx.blines, ctx.nlim)
+        g_shared = ctx.blines.to_shared(g_loggf)
+        out = [torch.as_tensor(a).to(dtype=dt, device=dev)
+               for a, (dt, dev) in zip((g_shared, gt_src + gt_line, g_fpf), ctx.meta)]
+        return (*out, None, None, None)
+
+
+def normalised_flux_batch(batch: FastSpecBatch, blines: BatchLines, loggf: torch.Tensor, temp=None, lnfpf=None,
+                          nlines_lim=None) -> torch.Tensor:
+    """F/Fc of all atmospheres, concatenated (atmosphere b: batch.setups[b].nwl points);
+    differentiable in the shared list's loggf (and temp [nb][ntau] / lnfpf when given with
+    requires_grad)."""
+    if temp is None:
+        temp = torch.as_tensor(np.stack([su.temp for su in batch.setups]), dtype=torch.float64)
+    if lnfpf is None:
+        lnfpf = torch.zeros((batch.nb,) + batch.setup.fraction_pf.shape, dtype=torch.float64)
+    return NormalisedFluxBatch.apply(loggf, temp, lnfpf, batch, blines, nlines_lim)
+
+
+__all__ = ["NormalisedFlux", "normalised_flux", "NormalisedFluxBatch", "normalised_flux_batch"]

@@ -181,3 +181,39 @@ def test_open_objects_released_at_exit(tmp_path):
     r = subprocess.run([sys.executable, "-c", child, str(root)], capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "done" in r.stdout
+
+
+def test_torch_autograd_batch(c5_small):
+    """The torch.autograd.Function over a batch context: loss and gradients (shared log gf,
+    temperature per atmosphere) equal the sums of the per-atmosphere wrapper's."""
+    torch = pytest.importorskip("torch")
+    from fastspec_amd.autograd import normalised_flux, normalised_flux_batch
+    lines, models, weights = c5_small
+    models, weights = models[:3], weights[:3]
+    loggf0 = np.log10(lines.gf.astype(np.float64))
+    with FastSpecBatch([s for s, _ in models], precision=FSG_FP64) as b:
+        shared = b.upload(lines)
+        bl = b.batch_lines(shared, [idx for _, idx in models])
+        loggf = torch.tensor(loggf0, dtype=torch.float64, requires_grad=True)
+        temp = torch.tensor(np.stack([s.temp for s, _ in models]), dtype=torch.float64, requires_grad=True)
+        f = normalised_flux_batch(b, bl, loggf, temp=temp)
+        a = torch.as_tensor(np.concatenate(weights))
+        loss = (a * f).sum()
+        loss.backward()
+        bl.free()
+        shared.free()
+    total, g_shared = 0.0, np.zeros(lines.n)
+    for k, (setup, idx) in enumerate(models):
+        lg = torch.tensor(loggf0[idx], dtype=torch.float64, requires_grad=True)
+        tk = torch.tensor(setup.temp, dtype=torch.float64, requires_grad=True)
+        with FastSpecGPU(setup, precision=FSG_FP64) as g:
+            dl = g.upload(lines.take(idx))
+            fk = normalised_flux(g, dl, lg, temp=tk)
+            lk = (torch.as_tensor(weights[k]) * fk).sum()
+            lk.backward()
+            dl.free()
+        total += lk.item()
+        np.add.at(g_shared, idx, lg.grad.numpy())
+        assert np.allclose(temp.grad[k].numpy(), tk.grad.numpy(), rtol=1e-9, atol=1e-12 * np.abs(tk.grad.numpy()).max())
+    assert abs(loss.item() - total) <= 1e-12 * sum(np.abs(w).sum() for w in weights)
+    assert np.allclose(loggf.grad.numpy(), g_shared, rtol=1e-9, atol=1e-12 * np.abs(g_shared).max())
