@@ -255,7 +255,7 @@ def cpu_baseline():
 
     if not ref_available():
         return None
-    width = 500.0  # about 25 s of the reference kernels on 16 host cores (1000 A took 47 s)
+    width = 500.0  # about 20 s of the reference kernels on 16 host cores (1000 A took 47 s)
     setup, lines = W.config_c2(wl_lo=5000.0, wl_hi=5000.0 + width)
     orc = Oracle(setup)
     ro = orc.pass1(lines)

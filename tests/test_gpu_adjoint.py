@@ -248,3 +248,53 @@ def test_torch_autograd(case):
     assert abs(loss.item() - b["L"]) < 1e-5 * np.sum(np.abs(a))
     assert _rel(loggf.grad.cpu().numpy(), b["ggf"]) < 1e-4
     assert _rel(temp.grad.numpy(), b["gt"]) < 1e-4
+
+
+def _loss_reference(setup, lines, a, windows):
+    """L from the reference's own compiled kernels (oracle/_ref: XlineSetup/XlineDepth,
+    LineOpacity1, the RadTranCalc loop of LocPrfPP) and the flux sums, with the windows and
+    flags given (held fixed, as the reverse mode holds them)."""
+    from oracle_bridge import Ref
+    flags, il, ir = windows
+    ref = Ref(setup)
+    ref.set_lines(lines)
+    cop, vv, av, _ = ref.xline()
+    ref.set_windows(flags & ~8, il, ir)
+    ref.set_block(cop, vv, av)
+    lop = ref.lineop1(1, setup.nwl, 1, lines.n)
+    spl, spc = ref.radtran(1, setup.nwl, lop)
+    sfl, sfc = spl @ setup.wmu, spc @ setup.wmu
+    return float(np.sum(a * sfl / sfc))
+
+
+@pytest.mark.ref
+def test_gradients_vs_reference_code_finite_differences(smooth_case):
+    """dL/d log gf and dL/dT of the library's reverse mode against centred finite
+    differences of the reference's own kernels, compiled from its sources."""
+    setup, lines, a = smooth_case
+    b = _backward(setup, lines, a)
+    ro = Oracle(setup).pass1(lines)
+    win = (ro["flags"], ro["ileft"], ro["iright"])
+    l0 = _loss_reference(setup, lines, a, win)
+    assert abs(l0 - b["L"]) <= 1e-9 * np.sum(np.abs(a))  # same forward
+    ggf = b["ggf"]
+    for i in np.argsort(-np.abs(ggf))[:4]:
+        out, lg = [], []
+        for sgn in (1, -1):
+            l2 = lines.take(np.arange(lines.n))
+            gf = l2.gf.copy()
+            gf[i] = np.float32(gf[i] * 10.0 ** (sgn * 1e-3))
+            lg.append(np.log10(np.float64(gf[i])))
+            l2.gf = gf
+            out.append(_loss_reference(setup, l2, a, win))
+        fd = (out[0] - out[1]) / (lg[0] - lg[1])
+        assert abs(ggf[i] - fd) <= REL * abs(fd), (i, ggf[i], fd)
+    gt, h = b["gt"], 1e-4
+    for k in np.argsort(-np.abs(gt))[:3]:
+        out = []
+        for sgn in (1, -1):
+            t = setup.temp.copy()
+            t[k] += sgn * h
+            out.append(_loss_reference(dataclasses.replace(setup, temp=t), lines, a, win))
+        fd = (out[0] - out[1]) / (2 * h)
+        assert abs(gt[k] - fd) <= REL * abs(fd), (k, gt[k], fd)
