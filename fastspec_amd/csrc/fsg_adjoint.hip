@@ -611,11 +611,13 @@ void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, co
   if (ntiles <= 0 || nbins <= 0) return;
   const int nw = S.ntp / DPW;
   const int nb = 8 * ((ntiles + 7) / 8);
-  // enough workgroups for about 8 wavefronts per SIMD of the 256 CUs (C5's small windows have
-  // a few dozen tiles; C2 has thousands and keeps one workgroup per tile)
+  // enough workgroups for about 8 wavefronts per SIMD of the 256 CUs (a 5-50 A window at
+  // 0.01 A has a few dozen tiles; a 1.2-million-point grid has thousands and keeps one
+  // workgroup per tile)
   const int nslice = std::max(1, std::min(64, (8 * 1024) / std::max(1, nb * nw)));
   // the 1024-thread bound keeps 128 registers and 4 wavefronts per SIMD, which outruns the
-  // spill-free 138-register build at 3 (C5: 86 vs 117 ms)
+  // spill-free 138-register build at 3 (256 atmospheres of 2,760 points on average: 86 vs
+  // 117 ms)
   k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, coefv, wlc_s, meta_s, glop,
                                                      sl, gtemp);
 }
