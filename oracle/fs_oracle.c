@@ -1534,25 +1534,33 @@ static int lineop_common(fso_ctx *c, int list_type, const fsg_lines *L, int64_t 
         if (!(ls.LinFlag[iline] & 8)) continue;
         xline_setup(c, &ls, iline, &XC);
         for (int k = 1; k <= N; k++) xline_depth(c, k, &XC, &Cop[k - 1], &Vv[k - 1], &Av[k - 1]);
-        for (int iwl = ls.Ileft[iline]; iwl <= ls.Iright[iline]; iwl++) {
-          if (!magnetic) {
-            line_opacity2(c, &ls, WAVE(iwl), iline, iwl, Cop, Vv, Av, LineOpacL);
-            for (int k = 1; k <= N; k++) LOP(k, iwl) = LOP(k, iwl) + LineOpacL[k - 1];
-          } else {
-            double *M6w = calloc((size_t)6 * N, sizeof(double));
-            mag_line_sum(c, &ls, WAVE(iwl), iline, Cop, Vv, Av, M6w);
-            mag_motype_div(c, M6w);
-            double *dst = c->maglineop + (size_t)(iwl - 1) * N * 6;
-            for (int q = 0; q < 6 * N; q++) dst[q] = dst[q] + M6w[q];
-            free(M6w);
+        /* one line's window points are independent (the reference's OpenMP loop over
+         * the wide line's window, src/synth.f90:990-999) */
+#pragma omp parallel
+        {
+          double *LopW = malloc(sizeof(double) * N);
+          double *M6w = malloc(sizeof(double) * 6 * N);
+#pragma omp for schedule(static)
+          for (int iwl = ls.Ileft[iline]; iwl <= ls.Iright[iline]; iwl++) {
+            if (!magnetic) {
+              line_opacity2(c, &ls, WAVE(iwl), iline, iwl, Cop, Vv, Av, LopW);
+              for (int k = 1; k <= N; k++) LOP(k, iwl) = LOP(k, iwl) + LopW[k - 1];
+            } else {
+              memset(M6w, 0, sizeof(double) * 6 * N);
+              mag_line_sum(c, &ls, WAVE(iwl), iline, Cop, Vv, Av, M6w);
+              mag_motype_div(c, M6w);
+              double *dst = c->maglineop + (size_t)(iwl - 1) * N * 6;
+              for (int q = 0; q < 6 * N; q++) dst[q] = dst[q] + M6w[q];
+            }
           }
+          free(LopW);
+          free(M6w);
         }
       }
     }
     /* segment loop (src/synth.f90:1003-1088, 1366-1447) */
     int64_t iline1 = 1, ilineb = 1, iline2;
     int iwl1 = 1, iwl2;
-    double *M6 = malloc(sizeof(double) * 6 * N);
     for (;;) {
       if (NlinesStep >= Nlines) {
         iwl2 = Nwl;
@@ -1591,27 +1599,37 @@ static int lineop_common(fso_ctx *c, int list_type, const fsg_lines *L, int64_t 
         }
       }
       st.nsegments++;
-      for (int iwl = iwl1; iwl <= iwl2; iwl++) {
-        if (!magnetic) {
-          line_opacity1(c, &ls, WAVE(iwl), Lstart[iwl], Lstop[iwl], iwl, Loff, bCop, bVv, bAv, LineOpacL);
-          for (int k = 1; k <= N; k++) LOP(k, iwl) = LOP(k, iwl) + LineOpacL[k - 1];
-        } else {
-          memset(M6, 0, sizeof(double) * 6 * N);
-          for (int64_t iline = Lstart[iwl]; iline <= Lstop[iwl]; iline++) {
-            int flag = ls.LinFlag[iline];
-            if ((flag & 1) || (flag & 8) || iwl < ls.Ileft[iline] || iwl > ls.Iright[iline]) continue;
-            size_t b = (size_t)(iline - Loff - 1) * N;
-            mag_line_sum(c, &ls, WAVE(iwl), iline, bCop + b, bVv + b, bAv + b, M6);
+      /* wavelength points are independent, as in the reference's OpenMP loop over iwl
+       * (src/synth.f90:1075-1080); each point's sum keeps its line order, so the result
+       * does not depend on the thread count */
+#pragma omp parallel
+      {
+        double *LopW = malloc(sizeof(double) * N);
+        double *M6w = malloc(sizeof(double) * 6 * N);
+#pragma omp for schedule(dynamic, 16)
+        for (int iwl = iwl1; iwl <= iwl2; iwl++) {
+          if (!magnetic) {
+            line_opacity1(c, &ls, WAVE(iwl), Lstart[iwl], Lstop[iwl], iwl, Loff, bCop, bVv, bAv, LopW);
+            for (int k = 1; k <= N; k++) LOP(k, iwl) = LOP(k, iwl) + LopW[k - 1];
+          } else {
+            memset(M6w, 0, sizeof(double) * 6 * N);
+            for (int64_t iline = Lstart[iwl]; iline <= Lstop[iwl]; iline++) {
+              int flag = ls.LinFlag[iline];
+              if ((flag & 1) || (flag & 8) || iwl < ls.Ileft[iline] || iwl > ls.Iright[iline]) continue;
+              size_t b = (size_t)(iline - Loff - 1) * N;
+              mag_line_sum(c, &ls, WAVE(iwl), iline, bCop + b, bVv + b, bAv + b, M6w);
+            }
+            mag_motype_div(c, M6w);
+            double *dst = c->maglineop + (size_t)(iwl - 1) * N * 6;
+            for (int q = 0; q < 6 * N; q++) dst[q] = dst[q] + M6w[q];
           }
-          mag_motype_div(c, M6);
-          double *dst = c->maglineop + (size_t)(iwl - 1) * N * 6;
-          for (int q = 0; q < 6 * N; q++) dst[q] = dst[q] + M6[q];
         }
+        free(LopW);
+        free(M6w);
       }
       if (iwl2 == Nwl) break;
       iwl1 = iwl2 + 1;
     }
-    free(M6);
     free(Lstart);
     free(Lstop);
   }
@@ -1691,16 +1709,21 @@ EXPORT int fso_radtran(fso_ctx *c, double *spl_out, double *spc_out) {
   c->spl = malloc(sizeof(double) * (size_t)Nwl * M);
   c->spc = malloc(sizeof(double) * (size_t)Nwl * M);
   c->magnetic = 0;
-  double *Src = malloc(sizeof(double) * N), *Xki = malloc(sizeof(double) * N);
-  for (int iwl = 1; iwl <= Nwl; iwl++) {
-    planck(c, WAVE(iwl) + c->wfirst, Src);
-    for (int k = 1; k <= N; k++) Xki[k - 1] = CONT(k, iwl);
-    locprf(c, c->spc + (size_t)(iwl - 1) * M, Xki, Src);
-    for (int k = 1; k <= N; k++) Xki[k - 1] = CONT(k, iwl) + LOP(k, iwl);
-    locprf(c, c->spl + (size_t)(iwl - 1) * M, Xki, Src);
+  /* independent wavelength points, as the reference's OpenMP loop (src/synth.f90:1850) */
+#pragma omp parallel
+  {
+    double *Src = malloc(sizeof(double) * N), *Xki = malloc(sizeof(double) * N);
+#pragma omp for schedule(dynamic, 64)
+    for (int iwl = 1; iwl <= Nwl; iwl++) {
+      planck(c, WAVE(iwl) + c->wfirst, Src);
+      for (int k = 1; k <= N; k++) Xki[k - 1] = CONT(k, iwl);
+      locprf(c, c->spc + (size_t)(iwl - 1) * M, Xki, Src);
+      for (int k = 1; k <= N; k++) Xki[k - 1] = CONT(k, iwl) + LOP(k, iwl);
+      locprf(c, c->spl + (size_t)(iwl - 1) * M, Xki, Src);
+    }
+    free(Src);
+    free(Xki);
   }
-  free(Src);
-  free(Xki);
   if (spl_out) memcpy(spl_out, c->spl, sizeof(double) * (size_t)Nwl * M);
   if (spc_out) memcpy(spc_out, c->spc, sizeof(double) * (size_t)Nwl * M);
   return 0;
@@ -2018,17 +2041,21 @@ EXPORT int fso_magradtran(fso_ctx *c, const double *bfield, int32_t nbfield, int
   } else if (fso_field_geometry(M, c->xmu, bfield, nbfield, bframe, SPhi2, CPhi2, CPhi, S2Psi, C2Psi)) {
     return 1;
   }
-  double *Src = malloc(sizeof(double) * N), *Xki = malloc(sizeof(double) * N);
-  for (int iwl = 1; iwl <= Nwl; iwl++) {
-    planck(c, WAVE(iwl) + c->wfirst, Src);
-    for (int k = 1; k <= N; k++) Xki[k - 1] = CONT(k, iwl);
-    locprf(c, c->spc + (size_t)(iwl - 1) * M, Xki, Src);
-    if (geo) fso_stokes_prf_sph(c, iwl, Src, geo, c->spz + (size_t)(iwl - 1) * 4 * M);
-    else fso_stokes_prf_pp(c, iwl, Src, SPhi2, CPhi2, CPhi, S2Psi, C2Psi, c->spz + (size_t)(iwl - 1) * 4 * M);
+#pragma omp parallel
+  {
+    double *Src = malloc(sizeof(double) * N), *Xki = malloc(sizeof(double) * N);
+#pragma omp for schedule(dynamic, 64)
+    for (int iwl = 1; iwl <= Nwl; iwl++) {
+      planck(c, WAVE(iwl) + c->wfirst, Src);
+      for (int k = 1; k <= N; k++) Xki[k - 1] = CONT(k, iwl);
+      locprf(c, c->spc + (size_t)(iwl - 1) * M, Xki, Src);
+      if (geo) fso_stokes_prf_sph(c, iwl, Src, geo, c->spz + (size_t)(iwl - 1) * 4 * M);
+      else fso_stokes_prf_pp(c, iwl, Src, SPhi2, CPhi2, CPhi, S2Psi, C2Psi, c->spz + (size_t)(iwl - 1) * 4 * M);
+    }
+    free(Src);
+    free(Xki);
   }
   free(geo);
-  free(Src);
-  free(Xki);
   free(SPhi2);
   free(CPhi2);
   free(CPhi);

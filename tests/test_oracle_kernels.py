@@ -83,9 +83,14 @@ def test_xline_scalings(c1):
 
 # ---- pass 1: rejection and windows --------------------------------------------------------
 
-@pytest.mark.ref
-def test_windows_equal_reference_kernels(c1):
-    setup, lines = c1
+@pytest.fixture(scope="module")
+def c2_log():
+    """A 10 A slice of the large atomic configuration: logarithmic grid (Wmode 1) at
+    R = 10^6 with that configuration's line density."""
+    return W.config_c2(2_000_000, 5995.0, 6005.0)
+
+
+def _windows_equal_reference(setup, lines, step=3):
     orc = Oracle(setup)
     ro = orc.lineop(lines)
     ref = Ref(setup)
@@ -93,7 +98,8 @@ def test_windows_equal_reference_kernels(c1):
     cop, vv, av, _ = ref.xline()
     wave = setup.wave
     wlc = lines.wl - setup.wfirst
-    for i in range(0, lines.n, 3):
+    checked = 0
+    for i in range(0, lines.n, step):
         iwl = ref.lib.ref_waveidx(wlc[i])
         assert iwl == oracle_lib().fso_wave_idx(orc.ctx, wlc[i])
         dW = (wlc[i] - wave[iwl - 1]) / (wave[iwl] - wave[iwl - 1])
@@ -114,6 +120,56 @@ def test_windows_equal_reference_kernels(c1):
         il = ref.lib.ref_waveidx(wlc[i] - dw)
         ir = min(ref.lib.ref_waveidx(wlc[i] + dw) + 1, setup.nwl)
         assert (ro["ileft"][i], ro["iright"][i]) == (il, ir), i
+        checked += 1
+    return checked
+
+
+@pytest.mark.ref
+def test_windows_equal_reference_kernels(c1):
+    setup, lines = c1
+    assert setup.wmode == 0
+    assert _windows_equal_reference(setup, lines) > 100
+
+
+@pytest.mark.ref
+def test_log_grid_windows_equal_reference_kernels(c2_log):
+    """Wmode 1: WaveIdx's logarithmic branch (src/opacity.f90:604-605) and the pass-1
+    windows built on it (src/synth.f90:904-906), line by line against the compiled
+    reference routines."""
+    setup, lines = c2_log
+    assert setup.wmode == 1 and setup.rstep > 0
+    assert _windows_equal_reference(setup, lines, step=2) > 500
+
+
+@pytest.mark.ref
+def test_log_grid_wave_idx_equals_reference(c2_log):
+    """The oracle's WaveIdx on a logarithmic grid equals the compiled reference's at every
+    grid point (the knife edges), one ulp either side of them, the midpoints, off-grid
+    values below and above (the clamps) and random offsets; and it brackets: the point
+    returned is the last grid point at or below the wavelength, within one point at the
+    knife edges only."""
+    setup, lines = c2_log
+    ref = Ref(setup)
+    orc = Oracle(setup)
+    lib = oracle_lib()
+    wave = setup.wave
+    rng = np.random.default_rng(17)
+    pts = np.concatenate([wave, np.nextafter(wave, -np.inf), np.nextafter(wave, np.inf),
+                          0.5 * (wave[1:] + wave[:-1]), [-3.0, -1e-9, wave[-1] + 1e-9, wave[-1] + 4.0],
+                          rng.uniform(-1.0, wave[-1] + 1.0, 4000)])
+    got = np.array([lib.fso_wave_idx(orc.ctx, float(x)) for x in pts])
+    exp = np.array([ref.lib.ref_waveidx(float(x)) for x in pts])
+    assert np.array_equal(got, exp)
+    # bracketing by brute force (1-based index of the last grid point <= w, clamped to
+    # 1..Nwl-1)
+    bf = np.clip(np.searchsorted(wave, pts, side="right"), 1, setup.nwl - 1)
+    mid = 0.5 * (wave[1:] + wave[:-1])
+    gm = np.array([lib.fso_wave_idx(orc.ctx, float(x)) for x in mid])
+    assert np.array_equal(gm, np.arange(1, setup.nwl))
+    assert np.max(np.abs(got - bf)) <= 1
+    # a plausible slip -- the linear branch on a logarithmic grid -- fails the comparison
+    lin = np.clip(np.floor(np.clip(mid / setup.dstep, 0, setup.nwl - 2)).astype(int) + 1, 1, setup.nwl - 1)
+    assert not np.array_equal(lin, gm)
 
 
 def test_window_bounds_threshold(c1):
@@ -140,10 +196,12 @@ def test_window_bounds_threshold(c1):
 # ---- LineOpac ----------------------------------------------------------------------------
 
 @pytest.mark.ref
-def test_lineop_equals_reference_lineopacity1(c1):
+@pytest.mark.parametrize("grid", ["c1", "c2_log"])
+def test_lineop_equals_reference_lineopacity1(grid, request):
     """All retained lines (wide ones included) summed by the reference's LineOpacity1 over
-    the whole grid equal the oracle's LineOpac (wide lines reorder the sum only)."""
-    setup, lines = c1
+    the whole grid equal the oracle's LineOpac (wide lines reorder the sum only); on the
+    linear grid of the plain configuration and a logarithmic R = 10^6 slice."""
+    setup, lines = request.getfixturevalue(grid)
     orc = Oracle(setup)
     ro = orc.lineop(lines)
     lop = orc.get_lineop()
@@ -237,8 +295,9 @@ def test_motype1_scales_by_standard_opacity():
 # ---- formal solution -------------------------------------------------------------------
 
 @pytest.mark.ref
-def test_radtran_equals_reference(c1):
-    setup, lines = c1
+@pytest.mark.parametrize("grid", ["c1", "c2_log"])
+def test_radtran_equals_reference(grid, request):
+    setup, lines = request.getfixturevalue(grid)
     orc = Oracle(setup)
     orc.lineop(lines)
     lop = orc.get_lineop()
