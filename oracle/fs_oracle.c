@@ -1178,6 +1178,11 @@ static float lin_str_calc(fso_ctx *c, double Wl, const double *LineOpacL, const 
   return res;
 }
 
+/* LinStrCalc of one line-centre column (the checker's closed-form tests call it alone) */
+EXPORT float fso_lin_str(fso_ctx *c, double Wl, const double *LineOpacL, const double *ContOpacL) {
+  return lin_str_calc(c, Wl, LineOpacL, ContOpacL);
+}
+
 /* WlcIdx (src/synth.f90:677-706) over the current (packed) Wlcent */
 static int64_t wlc_idx(const list_state *ls, double W) {
   if (ls->nlines == 0 || W < ls->Wlcent[1]) return 0;

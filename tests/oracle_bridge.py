@@ -77,6 +77,8 @@ def _load_oracle() -> C.CDLL:
     lib.fso_integ_convol.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]
     lib.fso_get_spectra.argtypes = [vp, _dp, _dp, _dp]
     lib.fso_set_spectra.argtypes = [vp, _dp, _dp, _dp]
+    lib.fso_lin_str.argtypes = [vp, C.c_double, _dp, _dp]
+    lib.fso_lin_str.restype = C.c_float
     lib.fso_last_error.restype = C.c_char_p
     return lib
 
