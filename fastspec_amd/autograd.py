@@ -25,6 +25,40 @@ import torch
 
 from .synth import BatchLines, DeviceLines, FastSpecBatch, FastSpecGPU
 
+# Each forward leaves its state in the context (LineOpac, the formal solution, the list's gf
+# and the pass-1 buffers) and backward reads that state.  A serial number per context names
+# the forward whose state the context holds; backward of an earlier forward (two forwards
+# before one backward, as in a loss over two parameter vectors or gradcheck's perturbed
+# evaluations) first runs that forward again from its saved gf.
+_SERIAL = "_autograd_serial"
+
+
+def _next_serial(synth) -> int:
+    n = getattr(synth, _SERIAL, 0) + 1
+    setattr(synth, _SERIAL, n)
+    return n
+
+
+def _gf32(loggf: torch.Tensor) -> np.ndarray:
+    """gf = 10**log gf in REAL(4), as the host's reader converts it."""
+    return torch.pow(torch.tensor(10.0, dtype=torch.float32), loggf.detach().to("cpu").float()).numpy()
+
+
+def _forward_scalar(synth: FastSpecGPU, dlines: DeviceLines, gf: np.ndarray, nlines_lim):
+    synth.set_gf(dlines, gf)
+    synth.reset_lineop()
+    synth.lineop_dev(dlines, nlines_lim)
+    synth.RadTranCalc(intensities=False)
+    return synth.flux()
+
+
+def _forward_batch(batch: FastSpecBatch, blines: BatchLines, gf: np.ndarray, nlines_lim):
+    batch.set_gf(blines, gf)
+    batch.reset_lineop()
+    batch.lineop_dev(blines, nlines_lim)
+    batch.RadTranCalc(intensities=False)
+    return batch.flux()
+
 
 class NormalisedFlux(torch.autograd.Function):
     @staticmethod
@@ -35,19 +69,11 @@ class NormalisedFlux(torch.autograd.Function):
             raise ValueError("NormalisedFlux: temp differs from the context's atmosphere; build a new context")
         if bool(torch.any(lnfpf.detach() != 0)):
             raise ValueError("NormalisedFlux: lnfpf must be zero (FractionPF is fixed by the context)")
-        # gf = 10**log gf in REAL(4), as the host's reader converts it
-        gf = torch.pow(torch.tensor(10.0, dtype=torch.float32, device=loggf.device), loggf.detach().float())
-        if gf.is_cuda:
-            torch.cuda.current_stream(gf.device).synchronize()
-            synth.set_gf(dlines, gf.contiguous())
-        else:
-            synth.set_gf(dlines, gf.numpy())
-        synth.reset_lineop()
-        synth.lineop_dev(dlines, nlines_lim)
-        synth.RadTranCalc(intensities=False)
-        sfl, sfc = synth.flux()
+        gf = _gf32(loggf)
+        sfl, sfc = _forward_scalar(synth, dlines, gf, nlines_lim)
         ctx.synth, ctx.dlines, ctx.nlim = synth, dlines, nlines_lim
-        ctx.sfl, ctx.sfc = sfl, sfc
+        ctx.sfl, ctx.sfc, ctx.gf = sfl, sfc, gf
+        ctx.serial = _next_serial(synth)
         ctx.meta = [(x.dtype, x.device) for x in (loggf, temp, lnfpf)]
         return torch.as_tensor(sfl / sfc, dtype=torch.float64, device=loggf.device)
 
@@ -55,6 +81,9 @@ class NormalisedFlux(torch.autograd.Function):
     def backward(ctx, g):
         g = g.detach().to("cpu", torch.float64).numpy()
         sfl, sfc = ctx.sfl, ctx.sfc
+        if getattr(ctx.synth, _SERIAL, 0) != ctx.serial:  # another forward ran since
+            _forward_scalar(ctx.synth, ctx.dlines, ctx.gf, ctx.nlim)
+            setattr(ctx.synth, _SERIAL, ctx.serial)
         # d(F/Fc): dL/dSfL = g / Fc, dL/dSfC = -g F / Fc^2
         _, _, gt_src = ctx.synth.flux_vjp(g / sfc, -g * sfl / (sfc * sfc))
         g_loggf, g_fpf, gt_line = ctx.synth.lineop_vjp(ctx.dlines, ctx.nlim)
@@ -87,14 +116,11 @@ class NormalisedFluxBatch(torch.autograd.Function):
             raise ValueError("NormalisedFluxBatch: temp differs from the batch's atmospheres; build a new batch")
         if bool(torch.any(lnfpf.detach() != 0)):
             raise ValueError("NormalisedFluxBatch: lnfpf must be zero (FractionPF is fixed by the batch)")
-        gf = torch.pow(torch.tensor(10.0, dtype=torch.float32), loggf.detach().to("cpu").float())
-        batch.set_gf(blines, gf.numpy())
-        batch.reset_lineop()
-        batch.lineop_dev(blines, nlines_lim)
-        batch.RadTranCalc(intensities=False)
-        sfl, sfc = batch.flux()
+        gf = _gf32(loggf)
+        sfl, sfc = _forward_batch(batch, blines, gf, nlines_lim)
         ctx.batch, ctx.blines, ctx.nlim = batch, blines, nlines_lim
-        ctx.sfl, ctx.sfc = sfl, sfc
+        ctx.sfl, ctx.sfc, ctx.gf = sfl, sfc, gf
+        ctx.serial = _next_serial(batch)
         ctx.meta = [(x.dtype, x.device) for x in (loggf, temp, lnfpf)]
         f = np.concatenate([l / c for l, c in zip(sfl, sfc)])
         return torch.as_tensor(f, dtype=torch.float64, device=loggf.device)
@@ -103,6 +129,9 @@ class NormalisedFluxBatch(torch.autograd.Function):
     def backward(ctx, g):
         g = g.detach().to("cpu", torch.float64).numpy()
         sfl, sfc = ctx.sfl, ctx.sfc
+        if getattr(ctx.batch, _SERIAL, 0) != ctx.serial:  # another forward ran since
+            _forward_batch(ctx.batch, ctx.blines, ctx.gf, ctx.nlim)
+            setattr(ctx.batch, _SERIAL, ctx.serial)
         cut = np.cumsum([len(x) for x in sfl])[:-1]
         gb = np.split(g, cut)
         _, _, gt_src = ctx.batch.flux_vjp([a / c for a, c in zip(gb, sfc)],

@@ -43,6 +43,17 @@ int fail(const std::string &msg) {
       return fail(std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_));       \
   } while (0)
 
+// CK for set-up code that owns a partly built context: releases it before returning
+#define CKF(ctx, call)                                                                        \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) {                                                                   \
+      fail(std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_));              \
+      fsg_finalize(ctx);                                                                      \
+      return 1;                                                                               \
+    }                                                                                         \
+  } while (0)
+
 // grow-only device buffer
 struct DBuf {
   void *p = nullptr;
@@ -108,9 +119,12 @@ struct fsg_ctx {
   hipEvent_t evt[4] = {};  // first and last far-wing launch (on st), core-zone launch (on st2)
   float *lop2 = nullptr;
   bool lop2_dirty = false;  // a core launch has not been followed by its add (an error path)
-  // launch geometry of the FP32 opacity kernels, chosen by the autotuner on the first chunk
-  // of the first FP32 line-opacity call: core-zone tile of 64 core_pp points, far_ring slots
-  int core_pp = 2, far_ring = 4;
+  // launch geometry of the FP32 opacity kernels: core-zone tile of 64 core_pp points (fixed:
+  // it sets how lines group into the compensated sums, so it never depends on timing;
+  // FSG_CORE_TILE=128 selects 128-point tiles), far_ring LDS slots of the far-wing kernel
+  // (chosen by the autotuner on the first chunk of the first FP32 line-opacity call; it does
+  // not change any result)
+  int core_pp = 1, far_ring = 4;
   bool tuned = false;
   // polarised path, allocated on first use
   void *maglineop = nullptr;  // [6][ntau][nwl] MagLineOpac, wavelength fastest
@@ -150,17 +164,15 @@ struct fsg_dlines {
 namespace {
 
 // ------------------------------------------------------------------------------------
-// Autotuner of the FP32 opacity launch geometry.  Candidates: the core-zone kernel's tile
-// (64 or 128 points) and the matrix-core kernel's LDS ring depth (2 or 4 group slots); the
-// far-wing kernel's results do not depend on its ring depth, the core-zone kernel's only
-// through the grouping of its blocked compensated sums.  Each of the four pairs runs twice on
-// the chunk at hand as the accumulation stage runs it -- both kernels at once on the two
-// streams, into scratch accumulators -- and the pair with the shortest wall time wins (alone,
-// the wider core tile can be faster while it slows the pair down by taking more registers
-// beside the matrix-core kernel).  Choices are kept per process for a workload signature (device,
-// depth count, grid size and chunk size in powers of two) and, with FSG_AUTOTUNE_CACHE set
-// to a file name, in that file ("signature core_pp far_ring" lines).  FSG_AUTOTUNE=0 keeps
-// the defaults (128-point core tiles, 4 ring slots) without timing.
+// Autotuner of the FP32 opacity launch geometry: the matrix-core kernel's LDS ring depth (2 or
+// 4 group slots), which changes no result.  The core-zone tile is not tuned: it sets the
+// grouping of the compensated sums, and a choice made from wall-clock times would make the
+// FP32 bits depend on the run.  Each ring depth runs twice on the chunk at hand as the
+// accumulation stage runs it -- both kernels at once on the two streams, into scratch
+// accumulators -- and the shortest wall time wins.  Choices are kept per process for a
+// workload signature (device, depth count, grid size and chunk size in powers of two, core
+// tile) and, with FSG_AUTOTUNE_CACHE set to a file name, in that file ("signature core_pp
+// far_ring" lines).  FSG_AUTOTUNE=0 keeps the default (4 ring slots) without timing.
 // ------------------------------------------------------------------------------------
 std::mutex g_tune_mu;
 std::map<std::string, std::pair<int, int>> g_tune;
@@ -180,7 +192,7 @@ std::string tune_signature(const fsg_ctx *c, int64_t m) {
   (void)hipGetDeviceProperties(&prop, c->dev);
   std::ostringstream k;
   k << prop.gcnArchName << "/cu" << prop.multiProcessorCount << "/ntau" << c->S.ntau << "/nwl2^" << ilog2_64(c->S.nwl)
-    << "/lines2^" << ilog2_64(m);
+    << "/lines2^" << ilog2_64(m) << "/cp" << c->core_pp;
   std::string s = k.str();
   for (char &ch : s)
     if (ch == ' ' || ch == ':') ch = '_';
@@ -216,8 +228,7 @@ static int autotune(fsg_ctx *c, int ntiles, int64_t m, const CoefA *ca, const Co
     std::lock_guard<std::mutex> lk(g_tune_mu);
     tune_read_file();
     auto it = g_tune.find(key);
-    if (it != g_tune.end()) {
-      c->core_pp = it->second.first;
+    if (it != g_tune.end() && it->second.first == c->core_pp) {
       c->far_ring = it->second.second;
       return 0;
     }
@@ -240,11 +251,11 @@ static int autotune(fsg_ctx *c, int ntiles, int64_t m, const CoefA *ca, const Co
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
   CK(hipEventCreateWithFlags(&e2, hipEventDisableTiming));
-  const int rings[2] = {2, 4}, pps[2] = {1, 2};
+  const int rings[2] = {2, 4}, pps[1] = {c->core_pp};
   float best = 1e30f;
   int bring = c->far_ring, bpp = c->core_pp;
   int rc = 0;
-  for (int ip = 0; ip < 2 && !rc; ip++) {
+  for (int ip = 0; ip < 1 && !rc; ip++) {
     launch_core_ranges(pps[ip], S.nwl, zoff, zend, c->zkey2.as<unsigned long long>(), c->rng2.as<int2>(), s);
     for (int ir = 0; ir < 2 && !rc; ir++) {
       for (int rep = 0; rep < 2 && !rc; rep++) {
@@ -444,14 +455,18 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
   c->dev = s->device;
   c->precision = s->precision;
   c->batched = s->batched ? 1 : 0;
-  CK(hipSetDevice(c->dev));
-  CK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
-  for (auto &e : c->ev) CK(hipEventCreate(&e));
+  {
+    const char *ct = std::getenv("FSG_CORE_TILE");
+    if (ct && std::atoi(ct) == 128) c->core_pp = 2;
+  }
+  CKF(c, hipSetDevice(c->dev));
+  CKF(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  for (auto &e : c->ev) CKF(c, hipEventCreate(&e));
   if (c->precision == FSG_FP32) {
-    CK(hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
-    CK(hipEventCreateWithFlags(&c->evf, hipEventDisableTiming));
-    CK(hipEventCreateWithFlags(&c->evj, hipEventDisableTiming));
-    for (auto &e : c->evt) CK(hipEventCreate(&e));
+    CKF(c, hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking));
+    CKF(c, hipEventCreateWithFlags(&c->evf, hipEventDisableTiming));
+    CKF(c, hipEventCreateWithFlags(&c->evj, hipEventDisableTiming));
+    for (auto &e : c->evt) CKF(c, hipEventCreate(&e));
   }
 
   DevSetup &S = c->S;
@@ -513,7 +528,7 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
     for (size_t iw = 0; iw < (size_t)Nw; iw++)
       for (int k = 0; k < N; k++) ct[(size_t)k * Nw + iw] = s->contop[iw * N + k];
     rc |= upload(c, ct.data(), ct.size(), &S.contT, c->owned);
-    CK(hipStreamSynchronize(c->st));
+    CKF(c, hipStreamSynchronize(c->st));
   }
   // spherical models: the rays of SphereInit
   if (s->radius > 0.0 && s->nmu > 0) {
@@ -576,7 +591,7 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
     }
     launch_voigt_table((float *)tab, S.anode, dv, c->st);
     S.vtab = (const float *)tab;
-    CK(hipStreamSynchronize(c->st));
+    CKF(c, hipStreamSynchronize(c->st));
   }
   if (c->precision == FSG_FP32 && alloc_zero(c, sizeof(float) * (size_t)N * Nw, (void **)&c->lop2)) {
     fsg_finalize(c);
@@ -590,8 +605,8 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
     fsg_finalize(c);
     return 1;
   }
-  CK(hipGetLastError());
-  CK(hipStreamSynchronize(c->st));
+  CKF(c, hipGetLastError());
+  CKF(c, hipStreamSynchronize(c->st));
   *ctx_out = c;
   return 0;
 }
@@ -1725,12 +1740,15 @@ int fsg_lineop_vjp(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
         launch_lineop_vjp_scatter(S, m, order, dl->L.atm, c->vslot.as<int>(), c->vsl.as<double>(), dgf, dfpf, s);
       }
       CK(hipGetLastError());
-      CK(hipStreamSynchronize(s));
     }
   }
-  if (gloggf_out && n > 0) CK(hipMemcpy(gloggf_out, dgf, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+  // the read-backs are ordered on the library stream after the clear and the kernels (the
+  // stream is non-blocking, so a null-stream copy would not wait for them)
   std::vector<double> h(nfpf + NB * N);
-  CK(hipMemcpy(h.data(), dfpf, sizeof(double) * (nfpf + NB * N), hipMemcpyDeviceToHost));
+  if (gloggf_out && n > 0)
+    CK(hipMemcpyAsync(gloggf_out, dgf, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(h.data(), dfpf, sizeof(double) * (nfpf + NB * N), hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
   if (gfpf_inout)
     for (size_t i = 0; i < nfpf; i++) gfpf_inout[i] += h[i];
   if (gtemp_inout)

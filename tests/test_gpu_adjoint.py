@@ -298,3 +298,132 @@ def test_gradients_vs_reference_code_finite_differences(smooth_case):
             out.append(_loss_reference(dataclasses.replace(setup, temp=t), lines, a, win))
         fd = (out[0] - out[1]) / (2 * h)
         assert abs(gt[k] - fd) <= REL * abs(fd), (k, gt[k], fd)
+
+
+def test_torch_two_forwards_then_backward(case):
+    """Two forwards on one context before one backward: each output's gradient is taken at
+    its own log gf (the context is brought back to the first forward's state)."""
+    torch = pytest.importorskip("torch")
+    from fastspec_amd.autograd import normalised_flux
+    setup, lines, a = case
+    x0 = np.log10(lines.gf.astype(np.float64))
+    x1 = x0.copy()
+    strong = np.argsort(-x0)[:30]
+    x1[strong] += 0.4
+    t0 = torch.tensor(x0, dtype=torch.float64, requires_grad=True)
+    t1 = torch.tensor(x1, dtype=torch.float64, requires_grad=True)
+    w = torch.as_tensor(a)
+    with _gpu(setup) as g:
+        dl = g.upload(lines)
+        f0 = normalised_flux(g, dl, t0)
+        f1 = normalised_flux(g, dl, t1)
+        ((w * f0).sum() + 2.0 * (w * f1).sum()).backward()
+        dl.free()
+    for x, t, scale in ((x0, t0, 1.0), (x1, t1, 2.0)):
+        l2 = lines.take(np.arange(lines.n))
+        l2.gf = torch.pow(torch.tensor(10.0, dtype=torch.float32), torch.tensor(x).float()).numpy()
+        b = _backward(setup, l2, scale * a)
+        assert _rel(t.grad.numpy(), b["ggf"]) < 1e-10
+
+
+def test_reverse_mode_multi_chunk_equals_one_chunk(case):
+    """The reverse pass rebuilds each chunk's records from the list (per-block
+    checkpointing): chunks of 37 lines give the one-chunk gradients (FP64 sums in another
+    order only)."""
+    setup, lines, a = case
+    one = _backward(setup, lines, a)
+    with _gpu(setup) as g:
+        dl = g.upload(lines)
+        st = g.lineop_dev(dl, 37)
+        assert st["nblocks"] == -(-st["nlines_use"] // 37) and st["nblocks"] >= 3
+        g.RadTranCalc(intensities=False)
+        sfl, sfc = g.flux()
+        _, _, gt = g.flux_vjp(a / sfc, -a * sfl / sfc**2)
+        ggf, gfpf, gtl = g.lineop_vjp(dl, 37)
+        dl.free()
+    assert _rel(ggf, one["ggf"]) < 1e-10
+    assert _rel(gt + gtl, one["gt"]) < 1e-10
+    # a species' dL/d ln FractionPF sums the signed contributions of all its lines, added per
+    # chunk: the order changes the last digits (measured 1.8e-10)
+    assert _rel(gfpf, one["gfpf"]) < 1e-9
+
+
+@pytest.fixture(scope="module")
+def c5_atmospheres():
+    """Three atmospheres of the differentiable batch (coolest, middle, hottest of the 256)
+    with their own windows and the shared list's lines that reach them."""
+    lines, models = W.config_c5()
+    teff = np.array([s.meta["atmosphere"].teff for s, _ in models])
+    pick = [int(np.argmin(teff)), int(np.argsort(teff)[128]), int(np.argmax(teff))]
+    out = []
+    for m in pick:
+        setup, idx = models[m]
+        sub = lines.take(idx)
+        a = np.random.default_rng(100 + m).normal(size=setup.nwl)
+        out.append((setup, sub, a))
+    return out
+
+
+def _backward_chunks(setup, lines, a, nlim):
+    with _gpu(setup) as g:
+        dl = g.upload(lines)
+        g.lineop_dev(dl, nlim)
+        g.RadTranCalc(intensities=False)
+        sfl, sfc = g.flux()
+        _, _, gt = g.flux_vjp(a / sfc, -a * sfl / sfc**2)
+        ggf, gfpf, gtl = g.lineop_vjp(dl, nlim)
+        dl.free()
+    return dict(gt=gt + gtl, ggf=ggf, gfpf=gfpf, L=float(np.sum(a * sfl / sfc)))
+
+
+def _loss_chunks(setup, lines, a, nlim):
+    with _gpu(setup) as g:
+        g.LineopCalc(lines, NlinesLim=nlim)
+        g.RadTranCalc(intensities=False)
+        sfl, sfc = g.flux()
+    return float(np.sum(a * sfl / sfc))
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_c5_atmosphere_gradients_vs_finite_differences(c5_atmospheres, k):
+    """One atmosphere of the differentiable batch, its lines in chunks of a third of the list
+    (forward and reverse): dL/d log gf, dL/dT and dL/d ln FractionPF against centred finite
+    differences of the forward."""
+    setup, lines, a = c5_atmospheres[k]
+    nlim = max(lines.n // 3, 1)
+    b = _backward_chunks(setup, lines, a, nlim)
+    ggf = b["ggf"]
+    for i in np.argsort(-np.abs(ggf))[:6]:
+        out, lg = [], []
+        for sgn in (1, -1):
+            l2 = lines.take(np.arange(lines.n))
+            gf = l2.gf.copy()
+            gf[i] = np.float32(gf[i] * 10.0 ** (sgn * 1e-3))
+            lg.append(np.log10(np.float64(gf[i])))
+            l2.gf = gf
+            out.append(_loss_chunks(setup, l2, a, nlim))
+        fd = (out[0] - out[1]) / (lg[0] - lg[1])
+        assert abs(ggf[i] - fd) <= REL * abs(fd) + 1e-9 * np.abs(ggf).max(), (i, ggf[i], fd)
+    # T and FractionPF on the smooth version (every line retained, windows spanning the grid)
+    smooth = dataclasses.replace(setup, eps_opac=1e-12)
+    bs = _backward_chunks(smooth, lines, a, nlim)
+    gt, h = bs["gt"], 1e-4
+    for kk in np.argsort(-np.abs(gt))[:4]:
+        out = []
+        for sgn in (1, -1):
+            t = smooth.temp.copy()
+            t[kk] += sgn * h
+            out.append(_loss_chunks(dataclasses.replace(smooth, temp=t), lines, a, nlim))
+        fd = (out[0] - out[1]) / (2 * h)
+        assert abs(gt[kk] - fd) <= REL * abs(fd) + 1e-6 * np.abs(gt).max(), (kk, gt[kk], fd)
+    g = bs["gfpf"]
+    for slot in np.unique(lines.spc):
+        col = slot - 1
+        fd, dlog = [], []
+        for sgn in (1, -1):
+            f = smooth.fraction_pf.copy()
+            f[:, col] = (f[:, col] * np.exp(sgn * 1e-3)).astype(np.float32)
+            dlog.append(np.log(f[:, col].astype(np.float64) / smooth.fraction_pf[:, col]))
+            fd.append(_loss_chunks(dataclasses.replace(smooth, fraction_pf=f), lines, a, nlim))
+        an = float(np.sum(g[:, col] * dlog[0])) - float(np.sum(g[:, col] * dlog[1]))
+        assert abs(an - (fd[0] - fd[1])) <= REL * abs(fd[0] - fd[1]), (slot, an, fd[0] - fd[1])
