@@ -18,7 +18,7 @@ EXPORTS = (
     "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop", "fsg_maglineop_dev", "fsg_lines_set_gf",
     "fsg_flux_vjp", "fsg_lineop_vjp", "fsg_integ_convol", "fsg_get_spectra",
     "fsg_lines_scan", "fsg_lines_read", "fsg_lines_stream", "fsg_init_batch", "fsg_batch_offsets",
-    "fsg_lines_batch",
+    "fsg_lines_batch", "fsg_integ_convol_noalloc",
 )
 
 _LIB = None
@@ -53,6 +53,7 @@ def load() -> C.CDLL:
     lib.fsg_magradtran.argtypes = [vp, dp, C.c_int32, C.c_int32, dp, dp]
     lib.fsg_flux.argtypes = [vp, dp, dp]
     lib.fsg_integ_convol.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_double]
+    lib.fsg_integ_convol_noalloc.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_double]
     lib.fsg_get_spectra.argtypes = [vp, dp, dp, dp]
     lib.fsg_lines_upload.argtypes = [vp, C.c_int32, C.POINTER(CLines), C.POINTER(vp)]
     lib.fsg_lines_free.argtypes = [vp]

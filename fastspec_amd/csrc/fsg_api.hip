@@ -81,6 +81,42 @@ struct DBuf {
 
 }  // namespace
 
+// Work space of fsg_integ_convol_noalloc, the entry point that allocates nothing: one
+// array per GPU, held from the library's load until the process ends.  In host code the
+// array's name is only a stand-in; its device address on each GPU comes from
+// hipGetSymbolAddress (fsg_init, once per device, with that device current) and is the only
+// address passed to kernels and copies.  Calls on one GPU share it and race if they overlap.
+constexpr size_t kStaticScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// records the device address of `scratch` on the current device dev (once per device)
+int static_scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail("fsg_init: device ordinal beyond the static work-space table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return 0;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(std::string("hipGetSymbolAddress of the static work space: ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = p;
+  return 0;
+}
+
+int static_scratch(int dev, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch_ptr[dev]) return fail("static work space not set up on this device");
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
 // pass 1 of a line list: bin populations and maximal half-widths, evaluation count
 struct LinePrep {
   int nbins = 0;
@@ -460,6 +496,10 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
     if (ct && std::atoi(ct) == 128) c->core_pp = 2;
   }
   CKF(c, hipSetDevice(c->dev));
+  if (static_scratch_init(c->dev)) {
+    fsg_finalize(c);
+    return 1;
+  }
   CKF(c, hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   for (auto &e : c->ev) CKF(c, hipEventCreate(&e));
   if (c->precision == FSG_FP32) {
@@ -2006,18 +2046,89 @@ int fsg_flux(fsg_ctx *c, double *sfl_out, double *sfc_out) {
   return 0;
 }
 
-int fsg_integ_convol(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, double vgau) {
+}  // extern "C"
+
+namespace {
+
+// IntegConvol on the device.  The work areas -- the rotated / macroturbulent annulus spectra
+// (a, b: [2][nmu][nwl]; IntegConvol(1) broadens into a), the kernel taps, the row table and
+// the flux weights -- come either from the context's grow-only buffers (fsg_integ_convol)
+// or, for fsg_integ_convol_noalloc, from the static array `scratch` of this device.  Every
+// part of a work area that a kernel reads was written earlier in the same call, so nothing
+// needs clearing between calls.
+int integ_convol_run(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, double vgau, bool noalloc,
+                     const char *name) {
   g_err.clear();
-  if (!c) return fail("fsg_integ_convol: NULL context");
-  if (c->S.batch) return fail("fsg_integ_convol: convolve the atmospheres of a batch one by one");
-  if (!c->have_rt) return fail("fsg_integ_convol: no formal solution computed since the last opacity change");
+  const std::string nm(name);
+  if (!c) return fail(nm + ": NULL context");
+  if (c->S.batch) return fail(nm + ": convolve the atmospheres of a batch one by one");
+  if (!c->have_rt) return fail(nm + ": no formal solution computed since the last opacity change");
   if (vsini < 0.0 || vmacro < 0.0 || vgau < 0.0) return fail("Fastspec: invalid value(s) of convolution parameters");
-  if (cimode != 1 && cimode != 2) return fail("fsg_integ_convol: CImode must be 1 or 2");
+  if (cimode != 1 && cimode != 2) return fail(nm + ": CImode must be 1 or 2");
   CK(hipSetDevice(c->dev));
   const DevSetup &S = c->S;
   const int M = S.nmu, Nw = S.nwl;
   hipStream_t st = c->st;
-  std::vector<double> kern{1.0};  // tap 0: the identity (a copy)
+  // kernel taps: tap 0 is the identity (a copy), then the Gaussian, then DiskInteg's
+  std::vector<double> kern{1.0};
+  std::vector<double> gk;
+  int gh = 0, goff = 0;
+  if (vgau > 0.0) {
+    convol_gauss_kernel(Nw, vgau, S.wmode, S.wfirst, c->wlast, S.dstep, gk, gh);
+    goff = (int)kern.size();
+    kern.insert(kern.end(), gk.begin(), gk.end());
+  }
+  const int ng = (int)gk.size();
+  std::vector<double> dk;
+  std::vector<int> roff, rn, moff;
+  int mn = 0, base = 0;
+  if (cimode == 2) {
+    ConvolSetup cs{Nw, M, S.wmode, S.wfirst, c->wlast, S.dstep, vsini, vmacro, c->hxmu.data(), c->hwmu.data()};
+    diskinteg_kernels(cs, dk, roff, rn, moff, mn);
+    base = (int)kern.size();
+    kern.insert(kern.end(), dk.begin(), dk.end());
+  }
+  // rows of the spectra: Stokes I..V blocks after a polarised solution, else SpL; then SpC
+  const int nlrow = c->magnetic ? 4 * M : M;
+  double *lbase = c->magnetic ? c->spz : c->spl;
+  const size_t na = cimode == 1 ? (vgau > 0.0 ? (size_t)(nlrow + M) * Nw : 0) : (size_t)2 * M * Nw;
+  const size_t nbw = cimode == 2 ? (size_t)2 * M * Nw : 0;
+  const size_t nrow = cimode == 1 ? (vgau > 0.0 ? (size_t)(nlrow + M) : 0)
+                                  : (size_t)2 * M + (vmacro > 0.0 ? 2 * M : 0) + (vgau > 0.0 ? 2 : 0);
+  double *a = nullptr, *b = nullptr, *dkern = nullptr, *dw = nullptr;
+  ConvRow *drows = nullptr;
+  if (!noalloc) {
+    if (na) CK(c->cv_a.ensure(sizeof(double) * na));
+    if (nbw) CK(c->cv_b.ensure(sizeof(double) * nbw));
+    CK(c->cv_kern.ensure(sizeof(double) * kern.size()));
+    CK(c->cv_rows.ensure(sizeof(ConvRow) * std::max<size_t>(nrow, 1)));
+    CK(c->cv_w.ensure(sizeof(double) * M));
+    a = c->cv_a.as<double>();
+    b = c->cv_b.as<double>();
+    dkern = c->cv_kern.as<double>();
+    drows = c->cv_rows.as<ConvRow>();
+    dw = c->cv_w.as<double>();
+  } else {
+    char *sp = nullptr;
+    if (static_scratch(c->dev, (void **)&sp)) return 1;
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t oa = 0, ob = oa + up(sizeof(double) * na), ok = ob + up(sizeof(double) * nbw),
+                 orw = ok + up(sizeof(double) * kern.size()), ow = orw + up(sizeof(ConvRow) * nrow),
+                 total = ow + up(sizeof(double) * M);
+    if (total > kStaticScratchBytes) {
+      char msg[256];
+      std::snprintf(msg, sizeof msg,
+                    "%s: this convolution needs %zu bytes of work space (%d points, %d rows of spectra), more "
+                    "than the %zu of the static buffer; use fsg_integ_convol",
+                    name, total, Nw, nlrow + M, (size_t)kStaticScratchBytes);
+      return fail(msg);
+    }
+    a = (double *)(sp + oa);
+    b = (double *)(sp + ob);
+    dkern = (double *)(sp + ok);
+    drows = (ConvRow *)(sp + orw);
+    dw = (double *)(sp + ow);
+  }
   std::vector<ConvRow> rows;
   struct Launch {
     size_t first;
@@ -2029,22 +2140,9 @@ int fsg_integ_convol(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, do
     for (size_t i = first; i < rows.size(); i++) maxk = std::max(maxk, rows[i].nk);
     launches.push_back(Launch{first, (int)(rows.size() - first), maxk});
   };
-  std::vector<double> gk;
-  int gh = 0, goff = 0;
-  if (vgau > 0.0) {
-    convol_gauss_kernel(Nw, vgau, S.wmode, S.wfirst, c->wlast, S.dstep, gk, gh);
-    goff = (int)kern.size();
-    kern.insert(kern.end(), gk.begin(), gk.end());
-  }
-  const int ng = (int)gk.size();
-  // rows of the spectra: Stokes I..V blocks after a polarised solution, else SpL; then SpC
-  const int nlrow = c->magnetic ? 4 * M : M;
-  double *lbase = c->magnetic ? c->spz : c->spl;
   if (cimode == 1) {
-    // IntegConvol(1): every intensity spectrum broadened into cv_a, then copied back
+    // IntegConvol(1): every intensity spectrum broadened into a, then copied back
     if (vgau > 0.0) {
-      CK(c->cv_a.ensure(sizeof(double) * (size_t)(nlrow + M) * Nw));
-      double *a = c->cv_a.as<double>();
       for (int r = 0; r < nlrow; r++)
         rows.push_back(ConvRow{lbase + (size_t)r * Nw, a + (size_t)r * Nw, goff, ng, gh});
       for (int m = 0; m < M; m++)
@@ -2052,17 +2150,7 @@ int fsg_integ_convol(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, do
       add_launch(0);
     }
   } else {
-    // DiskInteg: rotation, macroturbulence, the weighted sum over mu, Gaussian broadening
-    ConvolSetup cs{Nw, M, S.wmode, S.wfirst, c->wlast, S.dstep, vsini, vmacro, c->hxmu.data(), c->hwmu.data()};
-    std::vector<double> dk;
-    std::vector<int> roff, rn, moff;
-    int mn = 0;
-    diskinteg_kernels(cs, dk, roff, rn, moff, mn);
-    const int base = (int)kern.size();
-    kern.insert(kern.end(), dk.begin(), dk.end());
-    CK(c->cv_a.ensure(sizeof(double) * (size_t)2 * M * Nw));
-    CK(c->cv_b.ensure(sizeof(double) * (size_t)2 * M * Nw));
-    double *a = c->cv_a.as<double>(), *b = c->cv_b.as<double>();
+    // DiskInteg: rotation, macroturbulence, the weighted sum over mu, Gaussian broadening;
     // annulus spectra in a: [2][M][Nw] (line, continuum), rotated or copied
     for (int t = 0; t < 2; t++)
       for (int m = 0; m < M; m++) {
@@ -2090,24 +2178,18 @@ int fsg_integ_convol(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, do
       add_launch(first);
     }
   }
-  CK(c->cv_kern.ensure(sizeof(double) * kern.size()));
-  CK(c->cv_rows.ensure(sizeof(ConvRow) * std::max<size_t>(rows.size(), 1)));
-  CK(c->cv_w.ensure(sizeof(double) * M));
-  CK(hipMemcpyAsync(c->cv_kern.p, kern.data(), sizeof(double) * kern.size(), hipMemcpyHostToDevice, st));
-  if (!rows.empty())
-    CK(hipMemcpyAsync(c->cv_rows.p, rows.data(), sizeof(ConvRow) * rows.size(), hipMemcpyHostToDevice, st));
-  CK(hipMemcpyAsync(c->cv_w.p, c->hwmu.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
-  const ConvRow *drows = c->cv_rows.as<ConvRow>();
-  const double *dkern = c->cv_kern.as<double>();
+  if (rows.size() != nrow) return fail(nm + ": internal error (row count)");
+  CK(hipMemcpyAsync(dkern, kern.data(), sizeof(double) * kern.size(), hipMemcpyHostToDevice, st));
+  if (!rows.empty()) CK(hipMemcpyAsync(drows, rows.data(), sizeof(ConvRow) * rows.size(), hipMemcpyHostToDevice, st));
+  CK(hipMemcpyAsync(dw, c->hwmu.data(), sizeof(double) * M, hipMemcpyHostToDevice, st));
   if (cimode == 1) {
     if (!launches.empty()) {
       launch_conv_rows(Nw, launches[0].n, launches[0].maxk, drows, dkern, st);
-      CK(hipMemcpyAsync(lbase, c->cv_a.p, sizeof(double) * (size_t)nlrow * Nw, hipMemcpyDeviceToDevice, st));
-      CK(hipMemcpyAsync(c->spc, c->cv_a.as<double>() + (size_t)nlrow * Nw, sizeof(double) * (size_t)M * Nw,
-                        hipMemcpyDeviceToDevice, st));
+      CK(hipMemcpyAsync(lbase, a, sizeof(double) * (size_t)nlrow * Nw, hipMemcpyDeviceToDevice, st));
+      CK(hipMemcpyAsync(c->spc, a + (size_t)nlrow * Nw, sizeof(double) * (size_t)M * Nw, hipMemcpyDeviceToDevice,
+                        st));
     }
   } else {
-    double *a = c->cv_a.as<double>(), *b = c->cv_b.as<double>();
     launch_conv_rows(Nw, launches[0].n, launches[0].maxk, drows, dkern, st);
     double *fin = a;
     size_t li = 1;
@@ -2118,15 +2200,27 @@ int fsg_integ_convol(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, do
     }
     if (vgau > 0.0) {
       double *fr = fin == a ? b : a;
-      launch_mu_sum(Nw, M, fin, fin + (size_t)M * Nw, c->cv_w.as<double>(), fr, fr + Nw, st);
+      launch_mu_sum(Nw, M, fin, fin + (size_t)M * Nw, dw, fr, fr + Nw, st);
       launch_conv_rows(Nw, launches[li].n, launches[li].maxk, drows + launches[li].first, dkern, st);
     } else {
-      launch_mu_sum(Nw, M, fin, fin + (size_t)M * Nw, c->cv_w.as<double>(), c->sfl, c->sfc, st);
+      launch_mu_sum(Nw, M, fin, fin + (size_t)M * Nw, dw, c->sfl, c->sfc, st);
     }
   }
   CK(hipGetLastError());
   CK(hipStreamSynchronize(st));  // the host tables above stay alive until here
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsg_integ_convol(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, double vgau) {
+  return integ_convol_run(c, cimode, vsini, vmacro, vgau, false, "fsg_integ_convol");
+}
+
+int fsg_integ_convol_noalloc(fsg_ctx *c, int32_t cimode, double vsini, double vmacro, double vgau) {
+  return integ_convol_run(c, cimode, vsini, vmacro, vgau, true, "fsg_integ_convol_noalloc");
 }
 
 int fsg_device_ptr(fsg_ctx *c, int32_t which, void **ptr_out) {

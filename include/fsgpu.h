@@ -195,6 +195,16 @@ int fsg_flux(fsg_ctx *ctx, double *sfl_out, double *sfc_out);
  * conversion (src/fastspec.f90:103). */
 int fsg_integ_convol(fsg_ctx *ctx, int32_t cimode, double vsini, double vmacro, double vgau);
 
+/* fsg_integ_convol without allocation: the same stages, kernels and results, with every work
+ * area (annulus spectra, kernel taps, row table, weights) carved from one static device
+ * array of 64 MiB per GPU instead of buffers the context grows with hipMalloc.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid allocation).  A call
+ * that needs more than 64 MiB (cimode 2: 32 * nmu * nwl bytes plus the taps, about 299,000
+ * points at 7 mu) fails with a message and changes nothing.  All calls on one GPU share the
+ * array: calls that overlap in time on one GPU race. */
+int fsg_integ_convol_noalloc(fsg_ctx *ctx, int32_t cimode, double vsini, double vmacro, double vgau);
+
 /* The intensity spectra of the last formal solution as they stand on the device (after
  * fsg_integ_convol with cimode 1, the broadened ones), in the host's layouts for WriteOut
  * (src/ioutils.f90:1751-1875): spl_out [nwl][nmu] SpL, or spz_out [nwl][4][nmu] SpZ after
