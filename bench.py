@@ -11,6 +11,12 @@ core zones with the Voigt table on the vector units, blocked Kahan sums), FP64 D
 for continuum and line+continuum at 7 mu, flux sums.  The FP64 plain GPU path (table Voigt,
 FP64 sums, one bin) is timed on the same input and the FP32 flux is compared with it.
 
+Further legs beside the headline line's workload (each reported in its own key): the plain
+configuration (C1, 20,000 lines), the Stokes configuration (C4, 5,000 Zeeman lines), the
+cool-dwarf molecular configuration (C3, 200,000,000 diatomic lines streamed from a sidecar
+in blocks of 10^7), the differentiable batch (C5), LinStrCalc, ingest, convolution, the C2
+synthesis from host buffers to host buffers, and the reference's CPU kernels.
+
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
 from __future__ import annotations
@@ -28,7 +34,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "line·depth·λ evals/s + spectra/s; FP32-batched vs FP64 GPU at ≤1e-4 flux err"
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3     # published FP32 vector / dense matrix peak (vector: packed FP32)
+PEAK_FP32_VALU_TFLOPS = 78.65  # FP32 vector without packed instructions (the kernels avoid them)
+PEAK_FP64_TFLOPS = 78.6
+FLOP_PER_MAG_EVAL = 24       # Zeeman component evaluation (two table lerps, strength, sums)
+FLOP_PER_STOKES_POINT = 740  # Stokes DELO-Bezier per (lambda, mu, depth)
+FLOP_PER_RT_LAMBDA = 27052   # scalar DELO for line and continuum, 72 depths, 7 mu
 FLOP_PER_EVAL = 14          # U_op cost of the table-lerp evaluation (fixed definition)
 MFMA_FLOP_PER_EVAL = 6      # far-wing series T (k0 + T (k1 + T k2)): three FMA per evaluation
 TRAFFIC_MD = ROOT / "profiles" / "opacity_c2_hbm_traffic.md"
@@ -243,39 +254,239 @@ def run_c5(steps: int, warmup: int, contexts: bool = False):
     return out
 
 
+def run_host_to_host(setup, lines, steps: int):
+    """The C2 synthesis as the reference's seams see it: lines and intensities in host
+    memory -- fsg_lineop uploads the list and returns flags and windows, fsg_radtran and
+    fsg_flux copy SpL, SpC, SfL, SfC back -- with the context (grid, atmosphere, ContOpac)
+    made beforehand; its set-up time, which uploads ContOpac, is reported beside it."""
+    from fastspec_amd.abi import FSG_FP32
+    from fastspec_amd.synth import FastSpecGPU
+
+    t0 = time.perf_counter()
+    g = FastSpecGPU(setup, precision=FSG_FP32, batched=1)
+    init_s = time.perf_counter() - t0
+    g.LineopCalc(lines)
+    g.RadTranCalc()
+    g.flux()
+    t = []
+    for _ in range(steps):
+        g.reset_lineop()
+        t0 = time.perf_counter()
+        g.LineopCalc(lines)
+        g.RadTranCalc()
+        g.flux()
+        t.append(time.perf_counter() - t0)
+    g.close()
+    ms = float(np.median(t)) * 1e3
+    return {"ms_per_spectrum": ms, "spectra_per_s": 1e3 / ms, "init_ms": init_s * 1e3,
+            "note": "fsg_lineop from host lines + fsg_radtran with SpL/SpC to host + fsg_flux to host "
+                    "(median of the steps); init_ms: fsg_init (grid, atmosphere, 693 MB ContOpac, tables)"}
+
+
+def _device_steps(g, dl, steps, warmup, nlim=None):
+    for _ in range(warmup):
+        g.reset_lineop()
+        g.lineop_dev(dl, nlim)
+        g.RadTranCalc(intensities=False)
+    t0 = time.perf_counter()
+    st = None
+    for _ in range(steps):
+        g.reset_lineop()
+        st = g.lineop_dev(dl, nlim)
+        g.RadTranCalc(intensities=False)
+        sfl, sfc = g.flux()
+    dt = (time.perf_counter() - t0) / steps
+    return dt, st, sfl / sfc
+
+
+def run_c1(steps: int, warmup: int):
+    """The plain configuration in full (20,000 lines, 5000-5100 A, 0.01 A, 72 depths, 7 mu):
+    FP32 batched and FP64 plain, lines resident in HBM."""
+    from fastspec_amd import workloads as W
+    from fastspec_amd.abi import FSG_FP32, FSG_FP64
+    from fastspec_amd.synth import FastSpecGPU
+
+    setup, lines = W.config_c1()
+    out = {"workload": f"C1 plain: {lines.n} lines 5000-5100 A, {setup.nwl} points, 72 depths, 7 mu"}
+    fn = {}
+    for name, prec, batched in (("fp32_batched", FSG_FP32, 1), ("fp64_plain", FSG_FP64, 0)):
+        g = FastSpecGPU(setup, precision=prec, batched=batched)
+        dl = g.upload(lines)
+        dt, st, fn[name] = _device_steps(g, dl, steps, warmup)
+        dl.free()
+        g.close()
+        out[name] = {"ms_per_spectrum": dt * 1e3, "spectra_per_s": 1.0 / dt, "evals_per_s": st["evals"] / dt,
+                     "opacity_ms": st["ms_opacity"], "pass1_ms": st["ms_pass1"]}
+    out["evals_per_spectrum"] = st["evals"]
+    out["max_flux_err_fp32_vs_fp64"] = float(np.max(np.abs(fn["fp32_batched"] - fn["fp64_plain"])))
+    return out
+
+
+def run_c4(steps: int, warmup: int):
+    """The Stokes configuration in full (5,000 Zeeman lines, 6000-6100 A, 0.005 A, B = 2 kG at
+    45/30 deg): MagLineopCalc (six Faraday-Voigt profile sums per Zeeman component) and the
+    Stokes DELO-Bezier solution, FP32 profiles and FP64 profiles; U_mag = (component,
+    depth, lambda) evaluations inside the windows."""
+    from fastspec_amd import workloads as W
+    from fastspec_amd.abi import FSG_FP32, FSG_FP64
+    from fastspec_amd.synth import FastSpecGPU
+
+    setup, lines, bfield, bframe = W.config_c4()
+    ncomp = np.diff(lines.pi_off) + np.diff(lines.si_off)
+    out = {"workload": f"C4 Stokes: {lines.n} Zeeman lines 6000-6100 A, {setup.nwl} points, 72 depths, "
+                       f"7 mu, B = 2 kG at 45/30 deg"}
+    for name, prec in (("fp32_profiles", FSG_FP32), ("fp64_profiles", FSG_FP64)):
+        g = FastSpecGPU(setup, precision=prec)
+        r = g.MagLineopCalc(lines, bfield, bframe)
+        keep = (r["flags"] & 1) == 0
+        w = (r["iright"] - r["ileft"] + 1).astype(np.int64)
+        umag = int(np.sum(w[keep] * ncomp[keep])) * setup.ntau
+        dl = g.upload(lines)
+        for _ in range(warmup):
+            g.reset_lineop()
+            g.maglineop_dev(dl, bfield, bframe)
+            g.MagRadTranCalc(bfield, bframe, intensities=False)
+        t_op, t_rt, t_all, st = [], [], [], None
+        for _ in range(steps):
+            g.reset_lineop()
+            t0 = time.perf_counter()
+            st = g.maglineop_dev(dl, bfield, bframe)
+            t1 = time.perf_counter()
+            g.MagRadTranCalc(bfield, bframe, intensities=False)
+            g.flux()
+            t2 = time.perf_counter()
+            t_op.append(st["ms_opacity"])
+            t_rt.append((t2 - t1) * 1e3)
+            t_all.append((t2 - t0) * 1e3)
+        dl.free()
+        g.close()
+        ms_op, ms_rt, ms = float(np.median(t_op)), float(np.median(t_rt)), float(np.median(t_all))
+        stokes_flop = FLOP_PER_STOKES_POINT * setup.nwl * setup.nmu * setup.ntau
+        out[name] = {"ms_per_spectrum": ms, "spectra_per_s": 1e3 / ms, "u_mag": umag,
+                     "u_mag_per_s": umag / (ms_op * 1e-3), "opacity_ms": ms_op,
+                     "opacity_frac_of_fp32_peak": umag * FLOP_PER_MAG_EVAL / (ms_op * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                     "line_depth_lambda_evals": int(st["evals"]),
+                     "stokes_rt_ms": ms_rt,
+                     "stokes_rt_frac_of_fp64_peak": stokes_flop / (ms_rt * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                     "note": "stokes_rt_ms: fsg_magradtran + fsg_flux wall time (Stokes DELO, "
+                             f"{FLOP_PER_STOKES_POINT} flop per (lambda, mu, depth))"}
+    return out
+
+
+def run_c3(steps: int, fp64: bool):
+    """The cool-dwarf molecular configuration: 200,000,000 seeded diatomic (FeH) lines
+    log-uniform in 4000-25000 A on an R = 3x10^5 log grid (549,775 points), Teff 3200 K, log g
+    5.0, 72 depths, 7 mu.  The list is written as a sidecar of the HDF5 columns (W, E, LGGF,
+    GRAD; 4 GB) and streamed to HBM by fsg_lines_stream (ReadHDF's selection and conversions
+    on 16 host threads into pinned chunks); the synthesis then accumulates it in blocks of
+    NlinesLim = 10^7 lines."""
+    import shutil
+    import tempfile
+
+    from fastspec_amd import linelist as LL
+    from fastspec_amd import workloads as W
+    from fastspec_amd.abi import FSG_FP32, FSG_FP64
+    from fastspec_amd.synth import FastSpecGPU
+
+    need = 4.2e9
+    tmp_root = None
+    for cand in (tempfile.gettempdir(), "/dev/shm", str(ROOT / "gpurun_out")):
+        try:
+            if shutil.disk_usage(cand).free > 2 * need:
+                tmp_root = cand
+                break
+        except OSError:
+            pass
+    if tmp_root is None:
+        return {"skipped": "no file system with 8.4 GB free for the 4 GB sidecar"}
+    setup, spc = W.config_c3_setup()
+    out = {"workload": f"C3 molecular: {W.C3_NLINES} FeH lines 4000-25000 A, R=3e5 log grid ({setup.nwl} "
+                       f"points), 72 depths, 7 mu, grey Teff 3200 logg 5.0, NlinesLim {W.C3_NLINES_LIM}"}
+    with tempfile.TemporaryDirectory(dir=tmp_root) as tmp:
+        t0 = time.perf_counter()
+        nbytes = W.write_diatomic_sidecar(Path(tmp) / "feh")
+        out["sidecar"] = {"bytes": nbytes, "write_s": time.perf_counter() - t0}
+        log(f"C3 sidecar written: {nbytes / 1e9:.2f} GB in {out['sidecar']['write_s']:.1f} s")
+        for name, prec, batched in (("fp32_batched", FSG_FP32, 1), ("fp64_plain", FSG_FP64, 0)):
+            if name == "fp64_plain" and not fp64:
+                continue
+            g = FastSpecGPU(setup, precision=prec, batched=batched)
+            t0 = time.perf_counter()
+            sl = LL.StreamedLines(g, Path(tmp) / "feh", *W.C3_RANGE, mol_slot=spc, nthreads=16)
+            ingest = time.perf_counter() - t0
+            nst = steps if name == "fp32_batched" else 1
+            dt, st, fn = _device_steps(g, sl, nst, 1 if name == "fp32_batched" else 0, W.C3_NLINES_LIM)
+            sl.free()
+            g.close()
+            out[name] = {"ms_per_spectrum": dt * 1e3, "spectra_per_s": 1.0 / dt, "evals_per_s": st["evals"] / dt,
+                         "evals": st["evals"], "lines": sl.n, "retained": st["nlines_use"],
+                         "blocks": st["nblocks"], "pass1_ms": st["ms_pass1"], "opacity_ms": st["ms_opacity"],
+                         "ingest_s": ingest, "ingest_lines_per_s": sl.n / ingest,
+                         "roofline_frac_14flop": st["evals"] * FLOP_PER_EVAL / (st["ms_opacity"] * 1e-3) / 1e12 /
+                         PEAK_FP32_TFLOPS}
+            out.setdefault("_fn", {})[name] = fn
+            log(f"C3 {name}: {json.dumps(out[name])}")
+    fns = out.pop("_fn", {})
+    if len(fns) == 2:
+        out["max_flux_err_fp32_vs_fp64"] = float(np.max(np.abs(fns["fp32_batched"] - fns["fp64_plain"])))
+    return out
+
+
+def host_cpu():
+    """Core count and model of the host CPU."""
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
 def cpu_baseline():
     """The reference's own OpenMP kernels (oracle/_ref, built from the reference sources)
     on a bounded sample of the same workload: the C2 grid and line density restricted to
     a wavelength range around 5000 A.  Times LineOpacity1 over the sample grid and the
     RadTranCalc loop (LocPrfPP); counts evaluations from the windows."""
     sys.path.insert(0, str(ROOT / "tests"))
-    from oracle_bridge import Oracle, Ref, ref_available
+    import oracle_bridge
+    from oracle_bridge import Oracle, Ref
 
     from fastspec_amd import workloads as W
 
-    if not ref_available():
+    o3 = ROOT / "oracle" / "_ref" / "o3" / "libfsref.so"
+    if not o3.exists():
         return None
+    oracle_bridge.REF_SO = o3  # the -O3 build (the reference Makefile's level) for timing
     width = 500.0  # about 20 s of the reference kernels on 16 host cores (1000 A took 47 s)
     setup, lines = W.config_c2(wl_lo=5000.0, wl_hi=5000.0 + width)
     orc = Oracle(setup)
     ro = orc.pass1(lines)
     ref = Ref(setup)
     ref.set_lines(lines)
-    cop, vv, av, _ = ref.xline()
-    ref.set_windows(ro["flags"] & ~8, ro["ileft"], ro["iright"])
-    ref.set_block(cop, vv, av)
     nthreads = int(ref.lib.ref_nthreads())
     evals = int(ro["stats"]["evals"])
     t0 = time.perf_counter()
+    cop, vv, av, _ = ref.xline()  # XlineSetup + XlineDepth of every line (pass 1's coefficients)
+    t1 = time.perf_counter()
+    ref.set_windows(ro["flags"] & ~8, ro["ileft"], ro["iright"])
+    ref.set_block(cop, vv, av)
     lop = ref.lineop1(1, setup.nwl, 1, lines.n)
+    t2 = time.perf_counter()
     ref.radtran(1, setup.nwl, lop)
-    dt = time.perf_counter() - t0
+    t3 = time.perf_counter()
+    dt = t3 - t0
+    cpu = host_cpu()
     return {"value": evals / dt, "unit": "line·depth·λ evals/s", "cores": nthreads,
             "kind": "reference",
             "sample": f"C2 restricted to {5000:.0f}-{5000 + width:.0f} A ({setup.nwl} grid points, "
-                      f"{lines.n} lines, {evals:.3e} evals); reference LineOpacity1 + LocPrfPP "
-                      f"(amdflang -O2 -fopenmp, {nthreads} threads) in {dt:.2f} s",
-            "seconds": dt}
+                      f"{lines.n} lines, {evals:.3e} evals); reference XlineSetup/XlineDepth + LineOpacity1 + "
+                      f"LocPrfPP (amdflang -O3 -fopenmp, {nthreads} threads; windows from the oracle's pass 1) "
+                      f"in {dt:.2f} s",
+            "seconds": dt, "stages_s": {"xline": t1 - t0, "lineopacity1": t2 - t1, "locprfpp": t3 - t2},
+            "host": cpu}
 
 
 def main():
@@ -290,6 +501,11 @@ def main():
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-linstr", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--c3-fp64", action="store_true", help="also time the FP64 plain path on C3")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-to-host C2 timing")
     ap.add_argument("--only-c5", action="store_true", help="run only the differentiable batch (profiling)")
     ap.add_argument("--c5-contexts", action="store_true",
                     help="differentiable batch with one context per atmosphere (the earlier design)")
@@ -332,6 +548,12 @@ def main():
         "k_opacity32_core": {"ms": st["ms_core"], "evals": float(st["evals_core"]), "tile_points": st["core_tile"],
                              "evals_per_s": st["evals_core"] / (st["ms_core"] * 1e-3) if st["ms_core"] > 0 else None},
     }
+    core = kernels["k_opacity32_core"]
+    if core["evals_per_s"]:
+        # the fixed 14 flop per evaluation against the published FP32 peak and against the
+        # vector units' rate without packed FP32 instructions (which the kernels do not use)
+        core["frac_of_fp32_peak_14flop"] = core["evals_per_s"] * FLOP_PER_EVAL / 1e12 / PEAK_FP32_TFLOPS
+        core["frac_of_fp32_valu_unpacked_14flop"] = core["evals_per_s"] * FLOP_PER_EVAL / 1e12 / PEAK_FP32_VALU_TFLOPS
     out = {
         "metric": METRIC, "value": value, "unit": "line·depth·λ evals/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -372,6 +594,22 @@ def main():
         log("line-list ingest from a sidecar")
         out["ingest"] = run_ingest(setup, args.nlines)
         log(json.dumps(out["ingest"]))
+    if not args.no_host:
+        log("C2 from host buffers to host buffers")
+        out["c2_host_to_host"] = run_host_to_host(setup, lines, args.steps)
+        log(json.dumps(out["c2_host_to_host"]))
+    if not args.no_c1:
+        log("C1 plain configuration")
+        out["c1_plain"] = run_c1(args.steps, args.warmup)
+        log(json.dumps(out["c1_plain"]))
+    if not args.no_c4:
+        log("C4 Stokes configuration")
+        out["c4_stokes"] = run_c4(args.steps, args.warmup)
+        log(json.dumps(out["c4_stokes"]))
+    if not args.no_c3:
+        log("C3 molecular configuration (200M lines)")
+        del lines
+        out["c3_molecular"] = run_c3(max(1, min(args.steps, 2)), args.c3_fp64)
     if not args.no_c5:
         log("differentiable batch (256 atmospheres)")
         out["c5_differentiable_batch"] = run_c5(3, 1, args.c5_contexts)

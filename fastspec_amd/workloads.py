@@ -144,13 +144,20 @@ def wave_grid(wstart: float, wend: float, step: float, wmode: int):
     return wave, wstart, step, rstep
 
 
+# diatomic species of the molecular configuration: (name, mass [amu], log10 N/N_H at and
+# below the reference temperature t0, dissociation energy [eV], partition function, t0 [K]).
+# The number density falls with temperature as exp(D0/kT - D0/kT0) above t0.
+MOLECULAR_SPECIES = [("FeH", 56.853, -8.0, 1.63, 3000.0, 3200.0)]
+
+
 def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode: int,
                 species=ATOMIC_SPECIES, nmu: int = 7, vmicro: float = 1.0,
                 motype: int = 0, eps_opac: float = 1e-4, extra_slots: int = 4,
-                radius: float = -1.0) -> Setup:
+                radius: float = -1.0, molecules=()) -> Setup:
     """Host-side inputs (EOS tables, continuum) on the given grid.
 
-    EOS slots: 1..len(species) are the line species, then H I, He I, H2, H II, He II.
+    EOS slots: 1..len(species) are the line species, then H I, He I, H2, H II, He II, then the
+    molecules (MOLECULAR_SPECIES entries), species slots after the atomic ones.
     radius > 0 makes a spherical model: layer heights from dh = -dm / rho, zero at
     tau_5000 = 1."""
     ntau = atm.temp.shape[0]
@@ -158,7 +165,8 @@ def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode:
     lam = wave + wfirst
     t, ne, nh = atm.temp, atm.xne, atm.n_h
     nsp = len(species)
-    nslot = nsp + 5
+    nmol = len(molecules)
+    nslot = nsp + 5 + nmol
     pf_ld = nslot + extra_slots
     fraction_pf = np.zeros((ntau, pf_ld), np.float32)
     pot_ion = np.zeros(nslot, np.float32)
@@ -188,6 +196,11 @@ def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode:
         fraction_pf[:, nsp + j] = (n_s / u).astype(np.float32)
         pot_ion[nsp + j] = chi
         xamass[nsp + j] = mass
+    for j, (_, mass, logab, d0, u, t0) in enumerate(molecules):
+        frac = 10.0**logab * np.minimum(1.0, np.exp(d0 * EV / KB * (1.0 / t - 1.0 / t0)))
+        fraction_pf[:, nsp + 5 + j] = (frac * nh / u).astype(np.float32)
+        pot_ion[nsp + 5 + j] = d0
+        xamass[nsp + 5 + j] = mass
     contop = (atm.kappa_std[None, :] * continuum_shape(lam)[:, None]).astype(np.float64)
     contop_std = None
     rhox = atm.rhox
@@ -209,11 +222,11 @@ def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode:
         fraction_pf=fraction_pf, pot_ion=pot_ion, xamass=xamass,
         frac_h1=n_h1.astype(np.float32), frac_he1=n_he1.astype(np.float32),
         frac_hmol=n_h2.astype(np.float32),
-        spc_ion=np.array([s[2] for s in species], np.int32),
-        spc_slot=np.arange(1, nsp + 1, dtype=np.int32),
+        spc_ion=np.array([s[2] for s in species] + [1] * nmol, np.int32),
+        spc_slot=np.concatenate([np.arange(1, nsp + 1), nsp + 5 + np.arange(1, nmol + 1)]).astype(np.int32),
         he1_slot=nsp + 2, xmu=xmu, wmu=wmu, motype=motype, eps_opac=eps_opac, vmicro=vmicro,
         radius=radius, height=height,
-        meta=dict(atmosphere=atm, species=[s[0] for s in species]),
+        meta=dict(atmosphere=atm, species=[s[0] for s in species] + [m[0] for m in molecules]),
     )
 
 
@@ -279,6 +292,100 @@ def molecular_lines(n: int, wl_lo: float, wl_hi: float, seed: int, spc: int = 1)
     grad = rng.normal(8.0, 0.3, n)
     return LineList(wl=wl, elo=elo.astype(np.float32), gf=_f32pow10(lggf), grad=_f32pow10(grad),
                     spc=np.full(n, spc, np.int32), list_type=FSG_LIST_MOLECULAR)
+
+
+def _diatomic_plan(n: int, wl_lo: float, wl_hi: float, seed: int, nbins: int):
+    """Bin counts (multinomial over nbins equal bins of ln lambda) and bin edges."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cnt = rng.multinomial(n, np.full(nbins, 1.0 / nbins))
+    a, b = math.log(wl_lo), math.log(wl_hi)
+    return cnt, a + (b - a) * np.arange(nbins + 1) / nbins
+
+
+def _diatomic_bin(seed: int, i: int, lo: float, hi: float, m: int) -> dict:
+    # every bin has its own stream, so a slice of the list needs only the bins it touches
+    r = np.random.Generator(np.random.PCG64([seed, i]))
+    w = np.exp(np.sort(r.uniform(lo, hi, m)))
+    lggf = r.normal(-4.0, 1.5, m).astype(np.float32)
+    elo = r.exponential(1.0, m).astype(np.float32)
+    grad = r.normal(8.0, 0.3, m).astype(np.float32)
+    return {"W": w, "E": elo, "LGGF": lggf, "GRAD": grad}
+
+
+def iter_diatomic_bins(n: int, wl_lo: float, wl_hi: float, seed: int, nbins: int = 0):
+    """The seeded diatomic list of the molecular configuration bin by bin, in wavelength
+    order (columns as diatomic_columns)."""
+    if nbins <= 0:
+        nbins = max(1, n // 1_000_000)
+    cnt, edges = _diatomic_plan(n, wl_lo, wl_hi, seed, nbins)
+    for i in range(nbins):
+        yield _diatomic_bin(seed, i, edges[i], edges[i + 1], int(cnt[i]))
+
+
+def diatomic_columns(n: int, wl_lo: float, wl_hi: float, seed: int, first: int = 0, count: int = -1,
+                     nbins: int = 0) -> dict:
+    """Lines [first, first + count) of the seeded diatomic list of the molecular
+    configuration as the /Lines columns of its HDF5 form (W, E, LGGF, GRAD; log values):
+    lambda log-uniform in [wl_lo, wl_hi], log gf ~ N(-4, 1.5), E_low ~ Exp(1 eV), log Grad ~
+    N(8, 0.3).  The list is generated bin by bin (equal bins of ln lambda with multinomial
+    counts, each bin's draws sorted: the order statistics of n independent draws), so any
+    slice of it comes out without the whole list in memory."""
+    if count < 0:
+        count = n - first
+    if nbins <= 0:
+        nbins = max(1, n // 1_000_000)
+    cnt, edges = _diatomic_plan(n, wl_lo, wl_hi, seed, nbins)
+    starts = np.concatenate([[0], np.cumsum(cnt)])
+    cols = {k: [] for k in ("W", "E", "LGGF", "GRAD")}
+    for i in range(nbins):
+        s0, s1 = starts[i], starts[i + 1]
+        if s1 <= first or s0 >= first + count:
+            continue
+        c = _diatomic_bin(seed, i, edges[i], edges[i + 1], int(cnt[i]))
+        lo, hi = max(first, s0) - s0, min(first + count, s1) - s0
+        for k in cols:
+            cols[k].append(c[k][lo:hi])
+    return {k: (np.concatenate(v) if v else np.zeros(0, np.float64 if k == "W" else np.float32))
+            for k, v in cols.items()}
+
+
+def write_diatomic_sidecar(path, n: int = 0, wl_lo: float = 0.0, wl_hi: float = 0.0, seed: int = 0) -> int:
+    """The molecular configuration's list (default: all 200,000,000 lines) as a sidecar
+    directory (W f8, E, LGGF, GRAD f4, header), written bin by bin; returns the bytes
+    written."""
+    from pathlib import Path
+    n = n or C3_NLINES
+    wl_lo, wl_hi = (wl_lo, wl_hi) if wl_lo > 0 else C3_RANGE
+    seed = seed or SEED_BASE + 3
+    path = Path(path)
+    path.mkdir(parents=True, exist_ok=True)
+    files = {k: open(path / f"{k}.bin", "wb") for k in ("W", "E", "LGGF", "GRAD")}
+    nb, wmin, wmax, k = 0, None, None, 0
+    try:
+        for c in iter_diatomic_bins(n, wl_lo, wl_hi, seed):
+            for key, f in files.items():
+                a = np.ascontiguousarray(c[key], "<f8" if key == "W" else "<f4")
+                f.write(a.tobytes())
+                nb += a.nbytes
+            if len(c["W"]):
+                wmin = c["W"][0] if wmin is None else wmin
+                wmax = c["W"][-1]
+            k += len(c["W"])
+    finally:
+        for f in files.values():
+            f.close()
+    hdr = ["format=fastspec-lines-1", "name=FeH", "wl_vac=0", "hflag=0", f"wmin={float(wmin)!r}",
+           f"wmax={float(wmax)!r}", f"nl={k}", "columns=W:f8,E:f4,LGGF:f4,GRAD:f4"]
+    (path / "header.txt").write_text("\n".join(hdr) + "\n")
+    return nb
+
+
+def diatomic_lines(cols: dict, spc: int) -> LineList:
+    """The reader's conversions of diatomic columns (gf = 10**log gf, Grad = 10**log Grad in
+    REAL(4), src/ioutils.f90:1060-1066): a molecular list of species spc."""
+    return LineList(wl=np.asarray(cols["W"], np.float64), elo=np.asarray(cols["E"], np.float32),
+                    gf=_f32pow10(cols["LGGF"]), grad=_f32pow10(cols["GRAD"]),
+                    spc=np.full(len(cols["W"]), spc, np.int32), list_type=FSG_LIST_MOLECULAR)
 
 
 def zeeman_pattern(jlo: float, jup: float, glo: float, gup: float):
@@ -386,6 +493,50 @@ def config_c2(nlines: int = 2_000_000, wl_lo: float = 3000.0, wl_hi: float = 100
     n = int(round(nlines * frac)) if (wl_lo, wl_hi) != (3000.0, 10000.0) else nlines
     lines = atomic_lines(n, wl_lo, wl_hi, SEED_BASE + 2, log_uniform=True)
     return setup, lines
+
+
+C3_NLINES = 200_000_000
+C3_RANGE = (4000.0, 25000.0)
+
+
+def config_c3_setup(wl_lo: float = C3_RANGE[0], wl_hi: float = C3_RANGE[1]):
+    """Cool-dwarf molecular run: grey atmosphere Teff 3200 K, log g 5.0, 72 depths, R = 3x10^5
+    log grid (step 0.999308193 km/s; 549,775 points over 4000-25000 A), the diatomic FeH as
+    the line species.  Returns (setup, the FeH species slot)."""
+    atm = grey_atmosphere(3200.0, 5.0)
+    setup = build_setup(atm, wl_lo, wl_hi, 0.999308193, 1, molecules=MOLECULAR_SPECIES)
+    return setup, len(ATOMIC_SPECIES) + 1
+
+
+def config_c3(nlines: int = C3_NLINES, wl_lo: float = C3_RANGE[0], wl_hi: float = C3_RANGE[1]):
+    """The molecular configuration (or a wavelength slice of it with the full list's line
+    density): 200,000,000 seeded diatomic lines log-uniform in 4000-25000 A, log gf ~
+    N(-4, 1.5), E_low ~ Exp(1 eV); blocks of NlinesLim = 10^7 lines (C3_NLINES_LIM).  The
+    slice's lines are those of the full list that fall into it."""
+    setup, spc = config_c3_setup(wl_lo, wl_hi)
+    if (wl_lo, wl_hi) == C3_RANGE:
+        cols = diatomic_columns(nlines, *C3_RANGE, SEED_BASE + 3)
+    else:
+        cols = diatomic_slice(nlines, wl_lo, wl_hi)
+    return setup, diatomic_lines(cols, spc)
+
+
+C3_NLINES_LIM = 10_000_000
+
+
+def diatomic_slice(nlines: int, wl_lo: float, wl_hi: float) -> dict:
+    """Columns of the full molecular list's lines inside [wl_lo, wl_hi] (only the bins that
+    reach the slice are generated)."""
+    nbins = max(1, nlines // 1_000_000)
+    a, b = math.log(C3_RANGE[0]), math.log(C3_RANGE[1])
+    i0 = max(0, int(math.floor((math.log(wl_lo) - a) / (b - a) * nbins)))
+    i1 = min(nbins - 1, int(math.floor((math.log(wl_hi) - a) / (b - a) * nbins)))
+    cnt, _ = _diatomic_plan(nlines, *C3_RANGE, SEED_BASE + 3, nbins)
+    starts = np.concatenate([[0], np.cumsum(cnt)])
+    cols = diatomic_columns(nlines, *C3_RANGE, SEED_BASE + 3, first=int(starts[i0]),
+                            count=int(starts[i1 + 1] - starts[i0]), nbins=nbins)
+    keep = (cols["W"] >= wl_lo) & (cols["W"] <= wl_hi)
+    return {k: v[keep] for k, v in cols.items()}
 
 
 def config_c4(nlines: int = 5000):

@@ -267,6 +267,8 @@ CONTAINS
     INTEGER(C_INT), INTENT(OUT) :: flag(Nlines)
     INTEGER :: i, k
     TYPE(XlineCoef) :: XC
+    ! lines are independent, as in pass 1's OpenMP loop (src/synth.f90:833)
+    !$OMP PARALLEL DO SCHEDULE(DYNAMIC, 256) DEFAULT(SHARED) PRIVATE(i, k, XC)
     DO i = 1, Nlines
       CALL XlineSetup(i, XC)
       IF (XC%IxHe > 0) LinFlag(i) = IOR(LinFlag(i), ISHFT(XC%IxHe, 8))
@@ -275,6 +277,7 @@ CONTAINS
       END DO
       flag(i) = LinFlag(i)
     END DO
+    !$OMP END PARALLEL DO
   END SUBROUTINE ref_xline
 
   ! Overrides the per-line flags and windows (after pass 1 of the oracle)
