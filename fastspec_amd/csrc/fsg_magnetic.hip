@@ -166,15 +166,25 @@ __device__ __forceinline__ void fvoigt32(const float2 *ftab, float idv, float A,
 // ------------------------------------------------------------------------------------
 // MagLineopCalc pass 1 (src/synth.f90:1182-1273): one lane per line
 // ------------------------------------------------------------------------------------
-// MagLineOpacity2 (src/opacity.f90:468-520): max over depth of the Stokes I bound
-// max(Ps/2, Pp/2 + Ps/4) LinCop / ContOpac at Wl
-__device__ double mag_bound_max(const DevSetup &S, const XC &xc, const MagLine &Z, int64_t i, double wlc, int ihe,
-                                int iwl, double dW, double Wl) {
+// one wavefront per line: the lanes share the line's depths (and its Zeeman components for
+// the shift scaling), maxima over depths by wavefront reduction (exact, so the decisions are
+// those of a serial depth loop); the search itself is wave-uniform
+__device__ __forceinline__ double wave_max64(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// MagLineOpacity2 (src/opacity.f90:468-520): the Stokes I bound max(Ps/2, Pp/2 + Ps/4)
+// LinCop / ContOpac at Wl, maximum over the lane's depths.  The component shifts are the
+// scaled REAL(4) shifts of this field (shift * Bsplit * lambda^2), formed here as pass 1 stores
+// them.
+__device__ double mag_bound_lane(const DevSetup &S, const XC &xc, const MagLine &Z, int64_t i, double wlc, int ihe,
+                                 int iwl, double dW, double Wl, double bsplit, double wl2, int lane) {
   const int N = S.ntau;
   const int q0 = Z.pi_off[i], q1 = Z.pi_off[i + 1], r0 = Z.si_off[i], r1 = Z.si_off[i + 1];
   double mx = -INFINITY;
   const double dWl0 = Wl - wlc;
-  for (int k = 0; k < N; k++) {
+  for (int k = lane; k < N; k += 64) {
     double cop, vv, av;
     xline_depth(S, k, xc, cop, vv, av);
     double dWl = dWl0;
@@ -184,9 +194,11 @@ __device__ double mag_bound_max(const DevSetup &S, const XC &xc, const MagLine &
     line_astate(S, av, ia, ua);
     double Pp = 0.0, Ps = 0.0;
     for (int q = q0; q < q1; q++)
-      Pp = Pp + (double)Z.str_pi[q] * hvoigt64(S, av, ia, ua, (dWl - (double)Z.sp_pi[q]) * vv);
+      Pp = Pp + (double)Z.str_pi[q] *
+                    hvoigt64(S, av, ia, ua, (dWl - (double)(float)((double)Z.spl_pi[q] * bsplit * wl2)) * vv);
     for (int q = r0; q < r1; q++)
-      Ps = Ps + (double)Z.str_si[q] * hvoigt64(S, av, ia, ua, (dWl - (double)Z.sp_si[q]) * vv);
+      Ps = Ps + (double)Z.str_si[q] *
+                    hvoigt64(S, av, ia, ua, (dWl - (double)(float)((double)Z.spl_si[q] * bsplit * wl2)) * vv);
     double lo = fmax(0.5 * Ps, 0.5 * Pp + 0.25 * Ps) * cop;
     if (S.motype == 1) lo = lo / S.cstd[k];
     const double *ck = S.contT + (size_t)k * S.nwl;
@@ -196,26 +208,33 @@ __device__ double mag_bound_max(const DevSetup &S, const XC &xc, const MagLine &
   return mx;
 }
 
-__global__ void __launch_bounds__(64) k_magpass1(DevSetup S, DevLines L, MagLine Z, double bsplit, int *flag_out, int *il_out,
-                           int *ir_out, int *ic_out, unsigned *key_out, int *bin_count, int *bin_hmax, int batched) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= L.n) return;
+// MagLineopCalc pass 1: blocks of MAGP1_WAVES wavefronts, one line per wavefront
+constexpr int MAGP1_WAVES = 4;
+
+__global__ void __launch_bounds__(64 * MAGP1_WAVES) k_magpass1(DevSetup S, DevLines L, MagLine Z, double bsplit,
+                                                               int *flag_out, int *il_out, int *ir_out, int *ic_out,
+                                                               unsigned *key_out, int *bin_count, int *bin_hmax,
+                                                               int batched) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * MAGP1_WAVES + (threadIdx.x >> 6);
+  if (i >= L.n) return;  // wave-uniform
   const int N = S.ntau;
   int flag = initial_flag(L, i, 2);
   const double wlc = L.wl[i] - S.wfirst;
   // the Zeeman pattern in Angstrom for this field: shift * Bsplit * lambda^2, stored REAL(4)
   const double wl2 = (wlc + S.wfirst) * (wlc + S.wfirst);
   double SplMax = 0.0;
-  for (int q = Z.pi_off[i]; q < Z.pi_off[i + 1]; q++) {
+  for (int q = Z.pi_off[i] + lane; q < Z.pi_off[i + 1]; q += 64) {
     float v = (float)((double)Z.spl_pi[q] * bsplit * wl2);
     Z.sp_pi[q] = v;
     SplMax = fmax(SplMax, fabs((double)v));
   }
-  for (int q = Z.si_off[i]; q < Z.si_off[i + 1]; q++) {
+  for (int q = Z.si_off[i] + lane; q < Z.si_off[i + 1]; q += 64) {
     float v = (float)((double)Z.spl_si[q] * bsplit * wl2);
     Z.sp_si[q] = v;
     SplMax = fmax(SplMax, fabs((double)v));
   }
+  SplMax = wave_max64(SplMax);
   XC xc = xline_setup(S, L, i, wlc, flag, 2);
   if (xc.IxHe > 0) flag |= (xc.IxHe << 8);
   const int ihe = xc.IxHe;
@@ -223,7 +242,7 @@ __global__ void __launch_bounds__(64) k_magpass1(DevSetup S, DevLines L, MagLine
   const double dW = (wlc - S.wave[iwl - 1]) / (S.wave[iwl] - S.wave[iwl - 1]);
   // field-free centre test (LineOpacity2 at the centre); there is no bound test here
   double maxc = -INFINITY;
-  for (int k = 0; k < N; k++) {
+  for (int k = lane; k < N; k += 64) {
     double cop, vv, av;
     xline_depth(S, k, xc, cop, vv, av);
     double dWl = (ihe > 0) ? 0.0 - S.he_shf[(size_t)(ihe - 1) * S.ntp + k] : 0.0;
@@ -233,6 +252,10 @@ __global__ void __launch_bounds__(64) k_magpass1(DevSetup S, DevLines L, MagLine
     double kc = ck[iwl - 1] * (1.0 - dW) + ck[iwl] * dW;
     maxc = fmax(maxc, lo / kc);
   }
+  maxc = wave_max64(maxc);
+  auto bound = [&](double Wl) {
+    return wave_max64(mag_bound_lane(S, xc, Z, i, wlc, ihe, iwl, dW, Wl, bsplit, wl2, lane));
+  };
   int il = 1, ir = S.nwl;
   const bool weak = maxc < S.eps_opac;
   if (weak) {
@@ -243,7 +266,7 @@ __global__ void __launch_bounds__(64) k_magpass1(DevSetup S, DevLines L, MagLine
     const double wabs = wlc + S.wfirst;
     for (int idw = 0;; idw++) {
       const double dWo = (kDWstart * pow(kDWstep, (double)idw)) * wabs / 5e3;
-      if (mag_bound_max(S, xc, Z, i, wlc, ihe, iwl, dW, wlc + SplMax + dWo) < S.eps_opac) {
+      if (bound(wlc + SplMax + dWo) < S.eps_opac) {
         int isw, ikw;
         if (wlc > 0.5 * (S.wave[0] + S.wave[S.nwl - 1])) {
           isw = -1;
@@ -262,7 +285,7 @@ __global__ void __launch_bounds__(64) k_magpass1(DevSetup S, DevLines L, MagLine
         ib = wave_idx(S, wlc + isw * (SplMax + dWo)) + ikw;
         while (abs(ia - ib) > 1) {
           int im = (ia + ib) / 2;
-          if (mag_bound_max(S, xc, Z, i, wlc, ihe, iwl, dW, S.wave[im - 1]) < S.eps_opac) ib = im;
+          if (bound(S.wave[im - 1]) < S.eps_opac) ib = im;
           else ia = im;
         }
         if (isw == 1) {
@@ -278,6 +301,7 @@ __global__ void __launch_bounds__(64) k_magpass1(DevSetup S, DevLines L, MagLine
       }
     }
   }
+  if (lane != 0) return;
   flag_out[i] = flag;
   il_out[i] = il;
   ir_out[i] = ir;
@@ -701,7 +725,8 @@ void launch_fvoigt_table(float2 *tab, const double *anode, double dv, hipStream_
 void launch_magpass1(const DevSetup &S, const DevLines &L, const MagLine &Z, double bsplit, int *flag, int *il,
                      int *ir, int *ic, unsigned *key, int *bin_count, int *bin_hmax, int batched, hipStream_t st) {
   if (L.n == 0) return;
-  k_magpass1<<<nb(L.n, 64), 64, 0, st>>>(S, L, Z, bsplit, flag, il, ir, ic, key, bin_count, bin_hmax, batched);
+  k_magpass1<<<nb(L.n, MAGP1_WAVES), 64 * MAGP1_WAVES, 0, st>>>(S, L, Z, bsplit, flag, il, ir, ic, key, bin_count,
+                                                              bin_hmax, batched);
 }
 
 size_t magcoef_size(int fp32) { return fp32 ? sizeof(MagCoef<float>) : sizeof(MagCoef<double>); }
