@@ -15,7 +15,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libfsgpu.so"
 CALIB = PKG / "hbm_copy_cal"
 SOURCES = ["fsg_kernels.hip", "fsg_magnetic.hip", "fsg_adjoint.hip", "fsg_convol.hip", "fsg_ingest.hip",
-           "fsg_api.hip"]
+           "fsg_hdf5.hip", "fsg_api.hip"]
 ARCH = os.environ.get("FSG_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for all device code.  MFMA accumulators are kept in
@@ -57,7 +57,8 @@ def build(verbose: bool = False) -> Path:
     if failed:
         raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
+    # zlib inflates the HDF5 lists' chunks; libhdf5 itself is loaded at run time (dlopen)
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-lz", "-ldl"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -23,8 +24,12 @@ struct SidecarCol {
   int size = 0;   // bytes per value
 };
 
+struct H5List;  // an open HDF5 list (fsg_hdf5.hip)
+
+// a line list on disk: a sidecar directory, or an HDF5 file (h5 set)
 struct Sidecar {
   std::string dir, name;
+  std::shared_ptr<H5List> h5;
   int wl_vac = 0, hflag = 0;
   bool molecular = false;
   bool zeeman = false;  // JLO, JUP, GLO, GUP columns: a magnetic list
@@ -54,7 +59,13 @@ struct ZeemanOut {
   std::vector<float> spl_pi, str_pi, spl_si, str_si;
 };
 
+// opens a sidecar directory or (by its signature) an HDF5 list file
 int sidecar_open(const char *dir, Sidecar &sc, std::string &err);
+// HDF5 lists through libhdf5 loaded at run time (fsg_hdf5.hip)
+bool hdf5_file(const char *path);
+int hdf5_open(const char *path, Sidecar &sc, std::string &err);
+// rows [first, first + count) of a column, as stored (little-endian values, fixed strings)
+int hdf5_read(const Sidecar &sc, const std::string &name, int64_t first, int64_t count, void *dst, std::string &err);
 // lines [first, first + count) of the file are inside [wstart, wend] (0-based first)
 int sidecar_select(const Sidecar &sc, double wstart, double wend, int64_t &first, int64_t &count, std::string &err);
 // reads and converts lines [first, first + count) with nthreads threads into out (packed,

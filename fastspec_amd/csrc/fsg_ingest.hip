@@ -29,7 +29,9 @@ namespace {
 constexpr int kMaxIon = 5;  // Maxion, src/shared.f90:434
 
 // reads count values of size bytes from column file <dir>/<name>.bin starting at value first
+// (or rows of the HDF5 list's column dataset)
 bool read_col(const Sidecar &sc, const std::string &name, int64_t first, int64_t count, void *dst, std::string &err) {
+  if (sc.h5) return hdf5_read(sc, name, first, count, dst, err) == 0;
   auto it = sc.cols.find(name);
   if (it == sc.cols.end()) {
     err = "fsg_lines: column " + name + " missing in " + sc.dir;
@@ -155,9 +157,20 @@ double vac2air(double w) {
   return w > 2000.0 ? w / N : w;
 }
 
+namespace {
+
+// the column set and types a list must have (both sources)
+int check_columns(Sidecar &sc, std::string &err);
+
+}  // namespace
+
 int sidecar_open(const char *dir, Sidecar &sc, std::string &err) {
   sc = Sidecar();
   sc.dir = dir ? dir : "";
+  if (dir && hdf5_file(dir)) {
+    if (hdf5_open(dir, sc, err)) return 1;
+    return check_columns(sc, err);
+  }
   std::ifstream in(sc.dir + "/header.txt");
   if (!in) {
     err = "fsg_lines: " + sc.dir + " has no header.txt";
@@ -197,6 +210,12 @@ int sidecar_open(const char *dir, Sidecar &sc, std::string &err) {
     return 1;
   }
   sc.molecular = sc.name.compare(0, 4, "atom") != 0;
+  return check_columns(sc, err);
+}
+
+namespace {
+
+int check_columns(Sidecar &sc, std::string &err) {
   const std::vector<std::string> need =
       sc.molecular ? std::vector<std::string>{"W", "E", "LGGF", "GRAD"}
                    : std::vector<std::string>{"W", "EL", "ION", "E", "LGGF", "GRAD", "GQST", "GVDW"};
@@ -232,6 +251,8 @@ int sidecar_open(const char *dir, Sidecar &sc, std::string &err) {
   }
   return 0;
 }
+
+}  // namespace
 
 int sidecar_select(const Sidecar &sc, double wstart, double wend, int64_t &first, int64_t &count, std::string &err) {
   first = 0;

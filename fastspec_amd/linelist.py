@@ -1,5 +1,6 @@
-"""Line-list ingest: the raw-column sidecar of a FastSpec HDF5 line list and the library's
-reader of it (fsg_lines_scan, fsg_lines_read, fsg_lines_stream).
+"""Line-list ingest: FastSpec HDF5 line lists and their raw-column sidecars, and the library's
+reader of both (fsg_lines_scan, fsg_lines_read, fsg_lines_stream take a sidecar directory or
+an .h5 file).
 
 A sidecar directory holds the /Lines group of an HDF5 list as it is laid out by the
 reference's converters (python/vald2hdf.py:763-790, python/mol2hdf.py): one raw
@@ -63,6 +64,60 @@ def write_sidecar(path, columns: Dict[str, np.ndarray], name: str = "atom", wl_v
     ]
     (path / "header.txt").write_text("\n".join(hdr) + "\n")
     return path
+
+
+def _h5_string_attr(obj, name: str, value: str):
+    """A scalar string attribute as the reference's converter writes it: fixed length,
+    NUL-terminated ASCII (python/vald2hdf.py:738-758)."""
+    import h5py  # noqa: PLC0415 - optional dependency
+    raw = value.encode("latin-1")
+    tid = h5py.h5t.C_S1.copy()
+    tid.set_size(len(raw) + 1)
+    tid.set_strpad(h5py.h5t.STR_NULLTERM)
+    tid.set_cset(h5py.h5t.CSET_ASCII)
+    space = h5py.h5s.create(h5py.h5s.SCALAR)
+    aid = h5py.h5a.create(obj.id, name.encode("ascii"), tid, space)
+    try:
+        aid.write(np.array(raw, dtype=f"S{len(raw) + 1}"), mtype=tid)
+    finally:
+        aid.close()
+
+
+def write_hdf5(path, columns: Dict[str, np.ndarray], name: str = "atom", wl_vac: int = 0, hflag: int = 0,
+               chunk: int = 1 << 18, mol_id: str = "", mol_ion: int = 1):
+    """Writes a line list in the reference's HDF5 layout (python/vald2hdf.py:763-842,
+    python/mol2hdf.py): /Lines with W f8 contiguous, the other columns chunked (chunk rows;
+    the converters use 262,144) with the shuffle and gzip-4 filters, and the group
+    attributes NL, NAME, WL_VAC, HFLAG (atomic) or ID, ION (molecular), WMIN, WMAX.  Needs
+    h5py (in this image: the conda Python 3.9)."""
+    import h5py  # noqa: PLC0415 - optional dependency
+    atomic = name.startswith("atom")
+    order = [c for c in (ATOMIC_COLUMNS + ZEEMAN_COLUMNS + ISOTOPE_COLUMNS if atomic else MOLECULAR_COLUMNS)
+             if c in columns]
+    n = len(columns["W"])
+    with h5py.File(path, "w") as f:
+        g = f.create_group("Lines")
+        for col in order:
+            a = np.asarray(columns[col])
+            a = a.astype(a.dtype if col == "EL" else _TYPES[col])
+            if col == "W" or n == 0:
+                d = g.create_dataset(col, data=a)
+            else:
+                d = g.create_dataset(col, data=a, chunks=(min(n, chunk),), shuffle=True, compression="gzip",
+                                     compression_opts=4)
+            _h5_string_attr(d, "UNIT", "-")
+        g.attrs.create("NL", np.int32(n))
+        _h5_string_attr(g, "NAME", name)
+        if atomic:
+            g.attrs.create("HFLAG", np.int16(hflag))
+        else:
+            _h5_string_attr(g, "ID", mol_id or name)
+            g.attrs.create("ION", np.int16(mol_ion))
+        g.attrs.create("WL_VAC", np.int16(wl_vac))
+        w = np.asarray(columns["W"], np.float64)
+        g.attrs.create("WMIN", np.float64(w[0] if n else 0.0))
+        g.attrs.create("WMAX", np.float64(w[-1] if n else 0.0))
+    return Path(path)
 
 
 def hdf5_to_sidecar(h5file, path):

@@ -431,6 +431,26 @@ def zeeman_pattern(jlo: float, jup: float, glo: float, gup: float):
     return P, S
 
 
+def zeeman_level_columns(n: int, seed: int) -> dict:
+    """Seeded JLO, JUP, GLO, GUP columns of a Zeeman list (J <= 6 with dJ in {0, +-1}, not
+    0 -> 0; Lande g ~ U(0.5, 2.5)); every 41st line has no lower Lande factor (99, a line
+    without splitting, as the reference's converter marks it)."""
+    rng = np.random.default_rng(seed)
+    jlo = np.empty(n, np.float32)
+    jup = np.empty(n, np.float32)
+    for i in range(n):
+        while True:
+            a = 0.5 * rng.integers(0, 13)
+            b = a + rng.integers(-1, 2)
+            if 0 <= b <= 6 and not (a == 0 and b == 0):
+                break
+        jlo[i], jup[i] = a, b
+    glo = rng.uniform(0.5, 2.5, n).astype(np.float32)
+    gup = rng.uniform(0.5, 2.5, n).astype(np.float32)
+    glo[::41] = 99.0
+    return {"JLO": jlo, "JUP": jup, "GLO": glo, "GUP": gup}
+
+
 def zeeman_patterns(n: int, seed: int):
     """Seeded Zeeman patterns for n lines in CSR form: J <= 6, Lande g ~ U(0.5, 2.5); sigma
     components stored blue (negated shifts) then red, as the reference's reader does
