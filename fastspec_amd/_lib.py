@@ -18,7 +18,7 @@ EXPORTS = (
     "fsg_device_ptr", "fsg_stream", "fsg_reset_lineop", "fsg_maglineop_dev", "fsg_lines_set_gf",
     "fsg_flux_vjp", "fsg_lineop_vjp", "fsg_integ_convol", "fsg_get_spectra",
     "fsg_lines_scan", "fsg_lines_read", "fsg_lines_stream", "fsg_init_batch", "fsg_batch_offsets",
-    "fsg_lines_batch", "fsg_integ_convol_noalloc",
+    "fsg_lines_batch", "fsg_integ_convol_noalloc", "fsg_flux_dev", "fsg_flux_vjp_dev", "fsg_lineop_vjp_dev",
 )
 
 _LIB = None
@@ -28,10 +28,24 @@ class FsgError(RuntimeError):
     pass
 
 
+def _share_torch_runtime() -> None:
+    """torch's ROCm wheel carries its own HIP runtime under the soname the library links
+    (libamdhip64.so.7).  Loaded after torch, the library binds to that runtime, so library
+    and torch tensors share the device; two runtimes in one process do not (torch then sees
+    no GPU).  FSG_NO_TORCH=1 leaves torch out (the library then loads the system runtime)."""
+    if os.environ.get("FSG_NO_TORCH") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except Exception:  # torch absent: the library's own runtime
+        pass
+
+
 def load() -> C.CDLL:
     global _LIB
     if _LIB is not None:
         return _LIB
+    _share_torch_runtime()
     if not LIB_PATH.exists():
         raise FsgError(f"{LIB_PATH} is not built: run `python -m fastspec_amd.build` "
                        "(the HIP library is required; there is no CPU path)")
@@ -62,6 +76,9 @@ def load() -> C.CDLL:
     lib.fsg_lines_set_gf.argtypes = [vp, vp, C.c_int32]
     lib.fsg_flux_vjp.argtypes = [vp, dp, dp, dp, dp, dp]
     lib.fsg_lineop_vjp.argtypes = [vp, vp, C.c_int64, dp, dp, dp]
+    lib.fsg_flux_dev.argtypes = [vp, vp, vp]
+    lib.fsg_flux_vjp_dev.argtypes = [vp, vp, vp, dp]
+    lib.fsg_lineop_vjp_dev.argtypes = [vp, vp, C.c_int64, vp, dp, dp]
     lib.fsg_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp)]
     lib.fsg_stream.argtypes = [vp, C.POINTER(vp)]
     lib.fsg_reset_lineop.argtypes = [vp]

@@ -1,5 +1,8 @@
 """PyTorch autograd over libfsgpu: the normalised flux of one synthesis as a differentiable
 function of the lines' log gf, the temperature of each depth and the EOS number densities.
+Parameters on a CUDA device stay there: gf goes to the library as a device pointer, F/Fc
+comes back in device tensors (fsg_flux_dev) and the gradients through the device-pointer
+reverse-mode entry points (the library shares torch's HIP runtime; see _lib).
 
     F/Fc = NormalisedFlux.apply(loggf, temp, lnfpf, synth, dlines, nlines_lim)
 
@@ -39,25 +42,64 @@ def _next_serial(synth) -> int:
     return n
 
 
-def _gf32(loggf: torch.Tensor) -> np.ndarray:
-    """gf = 10**log gf in REAL(4), as the host's reader converts it."""
-    return torch.pow(torch.tensor(10.0, dtype=torch.float32), loggf.detach().to("cpu").float()).numpy()
+def _gf32(loggf: torch.Tensor):
+    """gf = 10**log gf in REAL(4), as the host's reader converts it: a numpy array for host
+    tensors, a finished CUDA tensor on the parameters' device for device tensors."""
+    ten = torch.tensor(10.0, dtype=torch.float32, device=loggf.device)
+    gf = torch.pow(ten, loggf.detach().float()).contiguous()
+    if gf.is_cuda:
+        torch.cuda.current_stream(gf.device).synchronize()
+        return gf
+    return gf.numpy()
 
 
-def _forward_scalar(synth: FastSpecGPU, dlines: DeviceLines, gf: np.ndarray, nlines_lim):
+def _forward_scalar(synth: FastSpecGPU, dlines: DeviceLines, gf, nlines_lim):
     synth.set_gf(dlines, gf)
     synth.reset_lineop()
     synth.lineop_dev(dlines, nlines_lim)
     synth.RadTranCalc(intensities=False)
-    return synth.flux()
 
 
-def _forward_batch(batch: FastSpecBatch, blines: BatchLines, gf: np.ndarray, nlines_lim):
+def _forward_batch(batch: FastSpecBatch, blines: BatchLines, gf, nlines_lim):
     batch.set_gf(blines, gf)
     batch.reset_lineop()
     batch.lineop_dev(blines, nlines_lim)
     batch.RadTranCalc(intensities=False)
-    return batch.flux()
+
+
+def _flux(synth, device):
+    """SfL, SfC over the context's grid: CUDA tensors filled on the device (fsg_flux_dev), or
+    host arrays."""
+    if device.type == "cuda":
+        sfl = torch.empty(synth.nwl_total, dtype=torch.float64, device=device)
+        sfc = torch.empty_like(sfl)
+        synth.flux_device(sfl, sfc)
+        return sfl, sfc
+    sfl, sfc = np.zeros(synth.nwl_total), np.zeros(synth.nwl_total)
+    from . import _lib
+    from .abi import ptr
+    import ctypes as C
+    _lib.check(synth._lib.fsg_flux(synth.ctx, ptr(sfl, C.c_double), ptr(sfc, C.c_double)))
+    return torch.from_numpy(sfl), torch.from_numpy(sfc)
+
+
+def _vjp(synth, dlines, nlim, g, sfl, sfc):
+    """dL/d log gf (on g's device), dL/dT and dL/d ln FractionPF (host) from dL/d(F/Fc) = g
+    over the context's grid: d(F/Fc) gives dL/dSfL = g / Fc, dL/dSfC = -g F / Fc^2."""
+    gsfl = (g / sfc).contiguous()
+    gsfc = (-g * sfl / (sfc * sfc)).contiguous()
+    if g.is_cuda:
+        torch.cuda.current_stream(g.device).synchronize()
+        gt_src = synth.flux_vjp_device(gsfl, gsfc)
+        gl = torch.empty(dlines.n, dtype=torch.float64, device=g.device)
+        gfpf, gt_line = synth.lineop_vjp_device(dlines, gl, nlim)
+        return gl, gt_src + gt_line, gfpf
+    if isinstance(synth, FastSpecBatch):
+        _, _, gt_src = synth.flux_vjp(synth.split(gsfl.numpy()), synth.split(gsfc.numpy()))
+    else:
+        _, _, gt_src = synth.flux_vjp(gsfl.numpy(), gsfc.numpy())
+    gl, gfpf, gt_line = synth.lineop_vjp(dlines, nlim)
+    return torch.from_numpy(np.ascontiguousarray(gl)), gt_src + gt_line, gfpf
 
 
 class NormalisedFlux(torch.autograd.Function):
@@ -70,25 +112,23 @@ class NormalisedFlux(torch.autograd.Function):
         if bool(torch.any(lnfpf.detach() != 0)):
             raise ValueError("NormalisedFlux: lnfpf must be zero (FractionPF is fixed by the context)")
         gf = _gf32(loggf)
-        sfl, sfc = _forward_scalar(synth, dlines, gf, nlines_lim)
+        _forward_scalar(synth, dlines, gf, nlines_lim)
+        sfl, sfc = _flux(synth, loggf.device)
         ctx.synth, ctx.dlines, ctx.nlim = synth, dlines, nlines_lim
         ctx.sfl, ctx.sfc, ctx.gf = sfl, sfc, gf
         ctx.serial = _next_serial(synth)
         ctx.meta = [(x.dtype, x.device) for x in (loggf, temp, lnfpf)]
-        return torch.as_tensor(sfl / sfc, dtype=torch.float64, device=loggf.device)
+        return (sfl / sfc).to(loggf.device)
 
     @staticmethod
     def backward(ctx, g):
-        g = g.detach().to("cpu", torch.float64).numpy()
-        sfl, sfc = ctx.sfl, ctx.sfc
         if getattr(ctx.synth, _SERIAL, 0) != ctx.serial:  # another forward ran since
             _forward_scalar(ctx.synth, ctx.dlines, ctx.gf, ctx.nlim)
             setattr(ctx.synth, _SERIAL, ctx.serial)
-        # d(F/Fc): dL/dSfL = g / Fc, dL/dSfC = -g F / Fc^2
-        _, _, gt_src = ctx.synth.flux_vjp(g / sfc, -g * sfl / (sfc * sfc))
-        g_loggf, g_fpf, gt_line = ctx.synth.lineop_vjp(ctx.dlines, ctx.nlim)
+        g = g.detach().to(ctx.sfl.device, torch.float64)
+        g_loggf, gt, g_fpf = _vjp(ctx.synth, ctx.dlines, ctx.nlim, g, ctx.sfl, ctx.sfc)
         out = [torch.as_tensor(a).to(dtype=dt, device=dev)
-               for a, (dt, dev) in zip((g_loggf, gt_src + gt_line, g_fpf), ctx.meta)]
+               for a, (dt, dev) in zip((g_loggf, gt, g_fpf), ctx.meta)]
         return (*out, None, None, None)
 
 
@@ -117,29 +157,31 @@ class NormalisedFluxBatch(torch.autograd.Function):
         if bool(torch.any(lnfpf.detach() != 0)):
             raise ValueError("NormalisedFluxBatch: lnfpf must be zero (FractionPF is fixed by the batch)")
         gf = _gf32(loggf)
-        sfl, sfc = _forward_batch(batch, blines, gf, nlines_lim)
+        _forward_batch(batch, blines, gf, nlines_lim)
+        sfl, sfc = _flux(batch, loggf.device)
         ctx.batch, ctx.blines, ctx.nlim = batch, blines, nlines_lim
         ctx.sfl, ctx.sfc, ctx.gf = sfl, sfc, gf
         ctx.serial = _next_serial(batch)
         ctx.meta = [(x.dtype, x.device) for x in (loggf, temp, lnfpf)]
-        f = np.concatenate([l / c for l, c in zip(sfl, sfc)])
-        return torch.as_tensor(f, dtype=torch.float64, device=loggf.device)
+        # the atmospheres' points of the concatenated grid, in order
+        sel = np.concatenate([np.arange(batch.off[b], batch.off[b] + su.nwl) for b, su in enumerate(batch.setups)])
+        ctx.sel = torch.as_tensor(sel, device=sfl.device)
+        return (sfl / sfc)[ctx.sel].to(loggf.device)
 
     @staticmethod
     def backward(ctx, g):
-        g = g.detach().to("cpu", torch.float64).numpy()
-        sfl, sfc = ctx.sfl, ctx.sfc
         if getattr(ctx.batch, _SERIAL, 0) != ctx.serial:  # another forward ran since
             _forward_batch(ctx.batch, ctx.blines, ctx.gf, ctx.nlim)
             setattr(ctx.batch, _SERIAL, ctx.serial)
-        cut = np.cumsum([len(x) for x in sfl])[:-1]
-        gb = np.split(g, cut)
-        _, _, gt_src = ctx.batch.flux_vjp([a / c for a, c in zip(gb, sfc)],
-                                          [-a * l / (c * c) for a, l, c in zip(gb, sfl, sfc)])
-        g_loggf, g_fpf, gt_line = ctx.batch.lineop_vjp(ctx.blines, ctx.nlim)
-        g_shared = ctx.blines.to_shared(g_loggf)
+        gfull = torch.zeros(ctx.batch.nwl_total, dtype=torch.float64, device=ctx.sfl.device)
+        gfull[ctx.sel] = g.detach().to(ctx.sfl.device, torch.float64)
+        g_loggf, gt, g_fpf = _vjp(ctx.batch, ctx.blines, ctx.nlim, gfull, ctx.sfl, ctx.sfc)
+        # per-line gradients of the batch list summed onto the shared list
+        idx = torch.as_tensor(ctx.blines.index, device=g_loggf.device)
+        g_shared = torch.zeros(ctx.blines.nshared, dtype=torch.float64, device=g_loggf.device)
+        g_shared.index_add_(0, idx, g_loggf)
         out = [torch.as_tensor(a).to(dtype=dt, device=dev)
-               for a, (dt, dev) in zip((g_shared, gt_src + gt_line, g_fpf), ctx.meta)]
+               for a, (dt, dev) in zip((g_shared, gt, g_fpf), ctx.meta)]
         return (*out, None, None, None)
 
 

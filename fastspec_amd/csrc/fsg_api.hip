@@ -1675,8 +1675,13 @@ int fsg_lines_set_gf(fsg_dlines *dl, const float *gf, int32_t on_device) {
 
 // Reverse mode of RadTranCalc + the flux sums, in wavelength chunks whose checkpoint fits
 // the scratch budget.
-int fsg_flux_vjp(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glop_out, double *gcont_out,
-                 double *gtemp_out) {
+}  // extern "C"
+
+namespace {
+
+// fsg_flux_vjp with dL/dSfL, dL/dSfC in host memory or (dev_in) in device memory
+int flux_vjp_run(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glop_out, double *gcont_out,
+                 double *gtemp_out, bool dev_in) {
   g_err.clear();
   if (!c || !gsfl || !gsfc) return fail("fsg_flux_vjp: NULL argument");
   if (!c->have_rt || c->magnetic) return fail("fsg_flux_vjp: needs the scalar formal solution of fsg_radtran");
@@ -1690,8 +1695,9 @@ int fsg_flux_vjp(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glo
   const size_t NB = (size_t)std::max(S.nbatch, 1);  // dL/dT rows: one per atmosphere of a batch
   CK(c->vout.ensure(sizeof(double) * (2 * Nw + NB * N)));
   double *dg = c->vout.as<double>(), *dgc = dg + Nw, *dgt = dgc + Nw;
-  CK(hipMemcpyAsync(dg, gsfl, sizeof(double) * Nw, hipMemcpyHostToDevice, s));
-  CK(hipMemcpyAsync(dgc, gsfc, sizeof(double) * Nw, hipMemcpyHostToDevice, s));
+  const hipMemcpyKind kin = dev_in ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  CK(hipMemcpyAsync(dg, gsfl, sizeof(double) * Nw, kin, s));
+  CK(hipMemcpyAsync(dgc, gsfc, sizeof(double) * Nw, kin, s));
   CK(hipMemsetAsync(dgt, 0, sizeof(double) * NB * N, s));
   const size_t per = rt_vjp_scratch_per_point(S);
   // chunks of whole wavefronts (a batch's atmospheres start at multiples of 64 points)
@@ -1719,11 +1725,29 @@ int fsg_flux_vjp(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glo
   return 0;
 }
 
+}  // namespace
+
+extern "C" {
+
+int fsg_flux_vjp(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glop_out, double *gcont_out,
+                 double *gtemp_out) {
+  return flux_vjp_run(c, gsfl, gsfc, glop_out, gcont_out, gtemp_out, false);
+}
+
+int fsg_flux_vjp_dev(fsg_ctx *c, const double *gsfl_dev, const double *gsfc_dev, double *gtemp_out) {
+  return flux_vjp_run(c, gsfl_dev, gsfc_dev, nullptr, nullptr, gtemp_out, true);
+}
+
 // Reverse mode of LineopCalc for one list from the resident dL/dLineOpac: pass 1 again
 // (same windows and flags), then per chunk of coefficient records the per-(line, depth)
 // sums of dL/dLineOpac x contribution and the temperature derivative of the contributions.
-int fsg_lineop_vjp(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_out, double *gfpf_inout,
-                   double *gtemp_inout) {
+}  // extern "C"
+
+namespace {
+
+// fsg_lineop_vjp with dL/d log gf written to host or (dev_out) device memory
+int lineop_vjp_run(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_out, double *gfpf_inout,
+                   double *gtemp_inout, bool dev_out) {
   g_err.clear();
   if (!c || !dl) return fail("fsg_lineop_vjp: NULL argument");
   if (nlines_lim < 1) return fail("fsg_lineop_vjp: NlinesLim must be positive");
@@ -1786,7 +1810,8 @@ int fsg_lineop_vjp(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
   // stream is non-blocking, so a null-stream copy would not wait for them)
   std::vector<double> h(nfpf + NB * N);
   if (gloggf_out && n > 0)
-    CK(hipMemcpyAsync(gloggf_out, dgf, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s));
+    CK(hipMemcpyAsync(gloggf_out, dgf, sizeof(double) * (size_t)n,
+                      dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   CK(hipMemcpyAsync(h.data(), dfpf, sizeof(double) * (nfpf + NB * N), hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
   if (gfpf_inout)
@@ -1794,6 +1819,20 @@ int fsg_lineop_vjp(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
   if (gtemp_inout)
     for (size_t k = 0; k < NB * N; k++) gtemp_inout[k] += h[nfpf + k];
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsg_lineop_vjp(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_out, double *gfpf_inout,
+                   double *gtemp_inout) {
+  return lineop_vjp_run(c, dl, nlines_lim, gloggf_out, gfpf_inout, gtemp_inout, false);
+}
+
+int fsg_lineop_vjp_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_dev, double *gfpf_inout,
+                       double *gtemp_inout) {
+  return lineop_vjp_run(c, dl, nlines_lim, gloggf_dev, gfpf_inout, gtemp_inout, true);
 }
 
 int fsg_reset_lineop(fsg_ctx *c) {
@@ -2031,6 +2070,18 @@ int fsg_get_spectra(fsg_ctx *c, double *spl_out, double *spz_out, double *spc_ou
         for (size_t iw = 0; iw < Nw; iw++) spz_out[(iw * 4 + q) * M + m] = h[(q * M + m) * Nw + iw];
   }
   if (spc_out && copy_spec_out(c, c->spc, spc_out)) return 1;
+  return 0;
+}
+
+int fsg_flux_dev(fsg_ctx *c, double *sfl_dev, double *sfc_dev) {
+  g_err.clear();
+  if (!c) return fail("fsg_flux_dev: NULL context");
+  if (!c->have_rt) return fail("fsg_flux: no formal solution computed since the last opacity change");
+  CK(hipSetDevice(c->dev));
+  const size_t Nw = (size_t)c->S.nwl;
+  if (sfl_dev) CK(hipMemcpyAsync(sfl_dev, c->sfl, Nw * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  if (sfc_dev) CK(hipMemcpyAsync(sfc_dev, c->sfc, Nw * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  CK(hipStreamSynchronize(c->st));
   return 0;
 }
 

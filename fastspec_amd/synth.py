@@ -77,6 +77,7 @@ class FastSpecGPU:
         self.ctx = C.c_void_p()
         _lib.check(self._lib.fsg_init(C.byref(self._cs), C.byref(self.ctx)))
         _LIVE.add(self)
+        self.nwl_total = setup.nwl  # points of the arrays the library returns
 
     def close(self):
         if self.ctx:
@@ -181,6 +182,41 @@ class FastSpecGPU:
         _lib.check(self._lib.fsg_flux_vjp(self.ctx, ptr(a, C.c_double), ptr(b, C.c_double), ptr(glop, C.c_double),
                                           ptr(gcont, C.c_double), ptr(gt, C.c_double)))
         return glop, gcont, gt
+
+    # --- device-pointer variants (torch CUDA tensors of this context's device) ---------------
+    def flux_device(self, sfl, sfc) -> None:
+        """SfL, SfC into float64 CUDA tensors of nwl values (fsg_flux_dev)."""
+        for t in (sfl, sfc):
+            assert t.is_cuda and t.dtype.itemsize == 8 and t.is_contiguous() and t.numel() == self.nwl_total
+        _lib.check(self._lib.fsg_flux_dev(self.ctx, C.c_void_p(sfl.data_ptr()), C.c_void_p(sfc.data_ptr())))
+
+    def flux_vjp_device(self, gsfl, gsfc) -> np.ndarray:
+        """Reverse mode of RadTranCalc + flux from dL/dSfL, dL/dSfC in finished float64 CUDA
+        tensors (fsg_flux_vjp_dev); returns the source-function part of dL/dT (host)."""
+        for t in (gsfl, gsfc):
+            assert t.is_cuda and t.dtype.itemsize == 8 and t.is_contiguous() and t.numel() == self.nwl_total
+        gt = np.zeros(self._gtemp_shape())
+        _lib.check(self._lib.fsg_flux_vjp_dev(self.ctx, C.c_void_p(gsfl.data_ptr()), C.c_void_p(gsfc.data_ptr()),
+                                              ptr(gt, C.c_double)))
+        return gt
+
+    def lineop_vjp_device(self, dlines, gloggf, NlinesLim: Optional[int] = None):
+        """dL/d log10 gf into a float64 CUDA tensor of n values (fsg_lineop_vjp_dev); returns
+        (dL/d ln FractionPF, line part of dL/dT) on the host."""
+        assert gloggf.is_cuda and gloggf.dtype.itemsize == 8 and gloggf.is_contiguous()
+        assert gloggf.numel() == dlines.n
+        nlim = dlines.n if NlinesLim is None else int(NlinesLim)
+        gf = np.zeros(self._gfpf_shape())
+        gt = np.zeros(self._gtemp_shape())
+        _lib.check(self._lib.fsg_lineop_vjp_dev(self.ctx, dlines.handle, max(nlim, 1), C.c_void_p(gloggf.data_ptr()),
+                                                ptr(gf, C.c_double), ptr(gt, C.c_double)))
+        return gf, gt
+
+    def _gtemp_shape(self):
+        return (self.setup.ntau,)
+
+    def _gfpf_shape(self):
+        return (self.setup.ntau, self.setup.fraction_pf.shape[1])
 
     def lineop_vjp(self, dlines: DeviceLines, NlinesLim: Optional[int] = None):
         """(dL/d log10 gf [n], dL/d ln FractionPF [ntau][pf_ld], line part of dL/dT [ntau])."""
@@ -328,6 +364,12 @@ class FastSpecBatch(FastSpecGPU):
         _lib.check(self._lib.fsg_batch_offsets(self.ctx, ptr(self.off, C.c_int64)))
         self.nwl_total = int(self.off[-1])
 
+    def _gtemp_shape(self):
+        return (self.nb, self.setup.ntau)
+
+    def _gfpf_shape(self):
+        return (self.nb, self.setup.ntau, self.setup.fraction_pf.shape[1])
+
     def split(self, a):
         """The atmospheres' parts of an array over the concatenated grid (first axis)."""
         return [a[self.off[b]:self.off[b] + su.nwl] for b, su in enumerate(self.setups)]
@@ -344,9 +386,15 @@ class FastSpecBatch(FastSpecGPU):
         return BatchLines(self, shared, indices)
 
     def set_gf(self, dlines, gf) -> None:
-        """gf of a batch list from the shared list's gf (numpy, float32)."""
+        """gf of a batch list from the shared list's gf (numpy float32, or a finished float32
+        CUDA tensor)."""
         if isinstance(dlines, BatchLines):
-            gf = np.asarray(gf, np.float32)[dlines.index]
+            if hasattr(gf, "data_ptr"):
+                import torch
+                gf = gf[torch.as_tensor(dlines.index, device=gf.device)].contiguous()
+                torch.cuda.current_stream(gf.device).synchronize()
+            else:
+                gf = np.asarray(gf, np.float32)[dlines.index]
         super().set_gf(dlines, gf)
 
     def LineopCalc(self, *a, **k):

@@ -288,6 +288,21 @@ int fsg_flux_vjp(fsg_ctx *ctx, const double *gsfl, const double *gsfc, double *g
 int fsg_lineop_vjp(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_out,
                    double *gfpf_inout, double *gtemp_inout);
 
+/* Device-pointer variants for callers whose tensors live in HBM (the PyTorch wrapper: with
+ * torch loaded first the library shares its HIP runtime and device).  They replace the same
+ * seams as the host variants (the flux sums of DiskInteg, src/convol.f90:233-234, and the
+ * reverse mode above); the pointer arguments named *_dev are device memory of the
+ * context's device, written or read on the library stream, which is synchronised before
+ * return (producers of *_dev inputs must have finished).
+ *   fsg_flux_dev        SfL, SfC [nwl] to device memory (either may be NULL)
+ *   fsg_flux_vjp_dev    dL/dSfL, dL/dSfC [nwl] from device memory; gtemp_out host [ntau]
+ *                       (batch: [nb][ntau]) or NULL
+ *   fsg_lineop_vjp_dev  dL/d(log10 gf) [n] to device memory; gfpf_inout, gtemp_inout host */
+int fsg_flux_dev(fsg_ctx *ctx, double *sfl_dev, double *sfc_dev);
+int fsg_flux_vjp_dev(fsg_ctx *ctx, const double *gsfl_dev, const double *gsfc_dev, double *gtemp_out);
+int fsg_lineop_vjp_dev(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_dev,
+                       double *gfpf_inout, double *gtemp_inout);
+
 /* Batch contexts: many model atmospheres synthesised in one launch sequence (the
  * differentiable batch of many small syntheses).  No counterpart in the reference, which
  * synthesises one atmosphere per run (src/fastspec.f90:241-461); per atmosphere the stages
