@@ -333,4 +333,26 @@ struct GridCol {
   }
 };
 
+// Width-bin histogram of pass 1, aggregated over the wavefront: one atomic per distinct bin
+// of the wavefront's lines (instead of one per line on a handful of hot addresses), the
+// bin's largest half-width h from a wavefront maximum (h < 0: no half-width).  Every lane of
+// the wavefront calls it; lanes with valid false take no part.
+__device__ __forceinline__ void wave_bin_update(int *bin_count, int *bin_hmax, unsigned key, int h, bool valid) {
+  const int lane = (int)(threadIdx.x & 63);
+  uint64_t todo = __builtin_amdgcn_ballot_w64(valid);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const unsigned k = (unsigned)__builtin_amdgcn_readlane((int)key, leader);
+    const bool mine = valid && key == k;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(mine);
+    int hv = mine ? h : -1;
+    for (int off = 32; off > 0; off >>= 1) hv = max(hv, __shfl_xor(hv, off, 64));
+    if (lane == leader) {
+      atomicAdd(&bin_count[k], (int)__builtin_popcountll(m));
+      if (hv >= 0) atomicMax(&bin_hmax[k], hv);
+    }
+    todo &= ~m;
+  }
+}
+
 }  // namespace fsg

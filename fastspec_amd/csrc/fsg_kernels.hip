@@ -95,7 +95,10 @@ __global__ void k_xline_init(int ntau, const double *temp, const double *xne, co
 __global__ void __launch_bounds__(128) k_pass1(DevSetup S0, DevLines L, int list_type, int *flag_out, int *il_out, int *ir_out,
                         int *ic_out, unsigned *key_out, int *bin_count, int *bin_hmax, int batched) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= L.n) return;
+  const bool valid = i < L.n;  // every lane reaches the wavefront's bin update
+  unsigned key = 0;
+  int h = -1;
+  if (valid) {
   const DevSetup S = line_setup(S0, L, i);  // windows in the line's own grid
   const int N = S.ntau;
   int flag = initial_flag(L, i, list_type);
@@ -144,17 +147,16 @@ __global__ void __launch_bounds__(128) k_pass1(DevSetup S0, DevLines L, int list
   il_out[i] = il;
   ir_out[i] = ir;
   ic_out[i] = iwl;
-  unsigned key;
   if (weak) {
     key = WEAK_KEY;
   } else {
-    int h = max(iwl - il, ir - iwl);
+    h = max(iwl - il, ir - iwl);
     if (ihe > 0) key = HE_KEY;
     else key = batched ? (h == 0 ? 0u : (unsigned)(32 - __clz(h))) : 0u;
-    atomicMax(&bin_hmax[key], h);
   }
-  atomicAdd(&bin_count[key], 1);
   key_out[i] = key;
+  }  // valid
+  wave_bin_update(bin_count, bin_hmax, key, h, valid);
 }
 
 // batch lists: line k is line idx[k] of the source list
@@ -1110,18 +1112,22 @@ __global__ void __launch_bounds__(64) k_far_lists(int ntiles, int nwl, int nbins
 __global__ void k_zone_keys(int64_t m, int64_t mmain, int ntau, const int4 *meta_s, const int2 *zone_s,
                             unsigned long long *key, int *val, unsigned long long *evals_core) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= m) return;
-  const int4 mm = meta_s[j];
-  const int2 z = zone_s[j];
-  const int lo = max(mm.x, z.x), hi = min(mm.y, z.y);
-  unsigned b = NZB - 1;
-  if (j < mmain && lo <= hi) {  // positions from mmain on hold the He I lines (their own kernel)
-    const int zh = max(max(mm.z - lo, hi - mm.z), 0);
-    b = zh == 0 ? 0u : (unsigned)(32 - __clz(zh));
-    atomicAdd(evals_core, (unsigned long long)(hi - lo + 1) * (unsigned long long)ntau);
+  unsigned long long ev = 0;  // the wavefront's core-zone evaluations, added once
+  if (j < m) {
+    const int4 mm = meta_s[j];
+    const int2 z = zone_s[j];
+    const int lo = max(mm.x, z.x), hi = min(mm.y, z.y);
+    unsigned b = NZB - 1;
+    if (j < mmain && lo <= hi) {  // positions from mmain on hold the He I lines (their own kernel)
+      const int zh = max(max(mm.z - lo, hi - mm.z), 0);
+      b = zh == 0 ? 0u : (unsigned)(32 - __clz(zh));
+      ev = (unsigned long long)(hi - lo + 1) * (unsigned long long)ntau;
+    }
+    key[j] = ((unsigned long long)b << 32) | (unsigned)mm.z;
+    val[j] = (int)j;
   }
-  key[j] = ((unsigned long long)b << 32) | (unsigned)mm.z;
-  val[j] = (int)j;
+  for (int off = 32; off > 0; off >>= 1) ev += __shfl_xor(ev, off, 64);
+  if ((threadIdx.x & 63) == 0 && ev) atomicAdd(evals_core, ev);
 }
 
 // first and one-past-last sorted position of each zone bin (zero for empty bins)
