@@ -139,6 +139,7 @@ struct fsg_ctx {
   int have_rt = 0;
   DBuf flag, il, ir, ic, key, key2, idx, idx2, coef, wlc, meta, rng, rng2, tmp, pos, wlcl, rflag;
   DBuf zkey, zkey2, zval, zperm, zrec;  // FP32 path: lines ordered by core-zone width
+  DBuf rngn;                             // FP32 path: candidate ranges of the narrow-line tiles
   DBuf fcap, foff, ftmp, flist;          // FP32 path: per-tile line lists of the matrix-core kernel
   // convolution and disk integration: host copies of mu, weights, the grid's last point;
   // device scratch for the convolved rows, kernels and row tables
@@ -659,7 +660,7 @@ int fsg_finalize(fsg_ctx *c) {
   for (void *p : c->owned) (void)hipFree(p);
   for (DBuf *b : {&c->flag, &c->il, &c->ir, &c->ic, &c->key, &c->key2, &c->idx, &c->idx2, &c->coef, &c->wlc,
                   &c->meta, &c->rng, &c->rng2, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp, &c->vscr,
-                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm, &c->zrec, &c->fcap, &c->foff, &c->ftmp, &c->flist, &c->cv_a,
+                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm, &c->zrec, &c->rngn, &c->fcap, &c->foff, &c->ftmp, &c->flist, &c->cv_a,
                   &c->cv_b, &c->cv_kern, &c->cv_rows, &c->cv_w})
     b->release();
   if (c->st2) (void)hipStreamSynchronize(c->st2);
@@ -1289,6 +1290,13 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
     CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
     const int ntiles = (S.nwl + TILE - 1) / TILE;
     CK(c->rng.ensure((size_t)ntiles * ng * sizeof(int2)));
+    // FP32 batched path: the leading width bins whose half-widths stay below NARROW_HMAX
+    // points go to the narrow-line kernel (up to 128 depths)
+    int nbn = 0;
+    if (c->precision == FSG_FP32 && c->batched && S.ntau <= 128)
+      while (nbn < nbins && ghmax[nbn] <= NARROW_HMAX) nbn++;
+    const int ntiles_n = (S.nwl + NARROW_TILE - 1) / NARROW_TILE;
+    if (nbn > 0) CK(c->rngn.ensure((size_t)ntiles_n * nbn * sizeof(int2)));
     size_t ztmp = 0;
     if (c->precision == FSG_FP32) {
       CK(c->rng2.ensure((size_t)((S.nwl + 63) / 64) * NZB * sizeof(int2)));  // core tiles of 64 points at least
@@ -1337,29 +1345,38 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
                       c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
       launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
                          c->meta.as<int4>(), c->rng.as<int2>(), s);
+      // chunk positions below mnarrow hold the lines of the narrow bins
+      const int64_t mnarrow = nbn > 0 ? std::min<int64_t>(std::max<int64_t>(goff[nbn] - s0, 0), m) : 0;
+      if (mnarrow > 0)
+        launch_tile_ranges(ntiles_n, NARROW_TILE, nbn, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
+                           c->meta.as<int4>(), c->rngn.as<int2>(), s);
       if (c->precision == FSG_FP32)  // the chunk's lines by core-zone width, and per-tile candidate ranges
-        launch_zone_order(m, std::min<int64_t>(std::max<int64_t>(goff[nbins] - s0, 0), m), ntiles, S.nwl, S.ntau,
+        launch_zone_order(m, mnarrow, std::min<int64_t>(std::max<int64_t>(goff[nbins] - s0, 0), m), ntiles, S.nwl, S.ntau,
                           c->meta.as<int4>(), zone, c->wlc.as<double>(), c->zkey.as<unsigned long long>(),
                           c->zval.as<int>(), c->zkey2.as<unsigned long long>(), c->zperm.as<int>(),
                           c->zrec.as<ZoneRec>(), c->tmp.p, ztmp, zoff, zend, c->u64 + 2, s);
       // the range table has ng groups per tile: the first nbins are read by the main launch
       // (stride ng), the last one by the He launch (offset nbins, stride ng)
       const bool has_main = s0 < goff[nbins], has_he = s1 > goff[nbins];
-      if (c->precision == FSG_FP32 && has_main && nbins > 0) {
+      // the far-wing and core-zone kernels take the bins from nbn on (rng offset by nbn
+      // groups, stride ng); the narrow bins have their own kernel
+      const bool has_wide = s1 > goff[nbn] && s0 < goff[nbins];
+      const int nbw = nbins - nbn;
+      const int2 *rngw = c->rng.as<int2>() + nbn;
+      if (c->precision == FSG_FP32 && has_wide && nbw > 0) {
         if ((S.ntau + 15) / 16 <= MFMA_MAX_NQ) {
           // the matrix-core kernel's line lists; their total size comes back to the host
           const size_t ftmp = far_scan_tmp_bytes(ntiles);
           CK(c->fcap.ensure((size_t)(ntiles + 1) * sizeof(long long)));
           CK(c->foff.ensure((size_t)(ntiles + 1) * sizeof(long long)));
           CK(c->ftmp.ensure(ftmp));
-          launch_far_cap(ntiles, nbins, ng, c->rng.as<int2>(), c->fcap.as<long long>(), c->foff.as<long long>(),
-                         c->ftmp.p, ftmp, lrec, m, s);
+          launch_far_cap(ntiles, nbw, ng, rngw, c->fcap.as<long long>(), c->foff.as<long long>(), c->ftmp.p, ftmp,
+                         lrec, m, s);
           long long ftotal = 0;
           CK(hipMemcpyAsync(&ftotal, c->foff.as<long long>() + ntiles, sizeof(long long), hipMemcpyDeviceToHost, s));
           CK(hipStreamSynchronize(s));
           CK(c->flist.ensure((size_t)ftotal * sizeof(int)));
-          launch_far_lists(ntiles, S.nwl, nbins, ng, c->rng.as<int2>(), lrec, c->foff.as<long long>(),
-                           c->flist.as<int>(), s);
+          launch_far_lists(ntiles, S.nwl, nbw, ng, rngw, lrec, c->foff.as<long long>(), c->flist.as<int>(), s);
           if (!c->tuned) {
             if (autotune(c, ntiles, m, ca, cb, dmaxw, lrec, zoff, zend)) return 1;
             c->tuned = true;
@@ -1370,12 +1387,15 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       if (!timed) CK(hipEventRecord(c->ev[2], s));
       timed = true;
       if (c->precision == FSG_FP32) {
-        if (has_main && nbins > 0) {
+        if (mnarrow > 0)
+          launch_opacity32_narrow(S, nbn, c->rngn.as<int2>(), cb, c->wlc.as<double>(), c->meta.as<int4>(),
+                                  (float *)c->lineop, s);
+        if (has_wide && nbw > 0) {
           // the core zones on st2 (into lop2) while the far wings run on st
           CK(hipEventRecord(c->evf, s));
           CK(hipStreamWaitEvent(c->st2, c->evf, 0));
           if (!nfar) CK(hipEventRecord(c->evt[0], s));
-          const int mf = launch_opacity32(S, ntiles, nbins, ng, c->rng.as<int2>(), ca, cb, dmaxw, lrec,
+          const int mf = launch_opacity32(S, ntiles, nbw, ng, rngw, ca, cb, dmaxw, lrec,
                                           c->wlc.as<double>(), c->meta.as<int4>(), c->foff.as<long long>(),
                                           c->flist.as<int>(), m, (float *)c->lineop, 0, c->far_ring, s);
           CK(hipEventRecord(c->evt[1], s));

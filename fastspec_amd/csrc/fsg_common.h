@@ -153,6 +153,10 @@ struct MagLine {
 
 // the polarised opacity kernel: one wavelength point per lane, DPW depths per wavefront
 constexpr int MTILE = 64;
+// narrow windows: width bins whose largest half-width is at most NARROW_HMAX points go to the
+// narrow-line kernel (tiles of NARROW_TILE points x all depths, up to 128 depths)
+constexpr int NARROW_HMAX = 16;
+constexpr int NARROW_TILE = 8;
 constexpr int NZB = 32;  // core-zone width bins of the FP32 path (the last one holds lines without a core zone)
 
 }  // namespace fsg

@@ -45,7 +45,7 @@ void launch_far_cap(int ntiles, int nbins, int rstride, const int2 *rng, long lo
                     size_t tmp_bytes, LineRec *lrec, int64_t mnull, hipStream_t st);
 void launch_far_lists(int ntiles, int nwl, int nbins, int rstride, const int2 *rng, const LineRec *lrec,
                       const long long *loff, int *list, hipStream_t st);
-void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
+void launch_zone_order(int64_t m, int64_t mnarrow, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
                        const int2 *zone_s, const double *wlc_s, unsigned long long *key, int *val,
                        unsigned long long *key_sorted, int *perm, ZoneRec *zrec, void *tmp, size_t tmp_bytes,
                        int *zoff, int *zend, unsigned long long *evals_core, hipStream_t st);
@@ -55,6 +55,10 @@ void launch_core_ranges(int core_pp, int nwl, const int *zoff, const int *zend, 
 size_t zone_sort_tmp_bytes(int64_t m);
 void launch_zero_f32(float *p, int64_t n, hipStream_t st);
 void launch_add_clear_f32(float *dst, float *src, int64_t n, hipStream_t st);
+// the lines of the narrow width bins (half-widths up to NARROW_HMAX points), tiles of
+// NARROW_TILE points; rng [tile][nbins] candidate ranges from launch_tile_ranges
+void launch_opacity32_narrow(const DevSetup &S, int nbins, const int2 *rng, const CoefB *cb, const double *wlc_s,
+                             const int4 *meta_s, float *lineop, hipStream_t st);
 void launch_opacity32_core(const DevSetup &S, int core_pp, const int2 *rng_core, const ZoneRec *zrec, const CoefA *ca,
                            const CoefB *cb, const float *dmaxw, float *lineop, hipStream_t st);
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,

@@ -1051,6 +1051,57 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
   }
 }
 
+// Narrow windows (the lines of the width bins whose half-widths stay below NARROW_HMAX points:
+// molecular forests at moderate resolving power, a few points per line).  With one point per
+// lane and 64 points per wavefront such a line would keep a few lanes busy; here a workgroup
+// owns a tile of NARROW_TILE points and all depths, and each wavefront's lanes are
+// NARROW_TILE points x 8 depths (lane = 8 depth + point offset), so a window of a few points
+// fills most lanes.  Per-(line, depth) records come through vector loads that the lanes of
+// one depth share.  Every point of a window takes the general path of the core-zone kernel
+// (table in the box, one- or two-pole fraction outside: the reference's HVoigt in FP32), and
+// the lines of the tile are summed in the order of their bins with the same blocked Kahan
+// compensation (groups of FOLD lines).
+__global__ void __launch_bounds__(1024) k_opacity32_narrow(DevSetup S, int ntiles, int nbins, const int2 *__restrict__ rng,
+                                                           const CoefB *__restrict__ cb,
+                                                           const double *__restrict__ wlc_s,
+                                                           const int4 *__restrict__ meta_s, float *__restrict__ lop) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  if (t >= ntiles) return;  // workgroup-uniform
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int d = wave * 8 + (lane >> 3);
+  const int p0 = t * NARROW_TILE + 1, p1 = min(p0 + NARROW_TILE - 1, S.nwl);
+  const int p = p0 + (lane & 7);
+  const bool valid = p <= p1 && d < S.ntau;
+  const double Wp = p <= S.nwl ? S.wave[p - 1] : 0.0;
+  float acc = 0.f, cmp = 0.f, part = 0.f;
+  int grp = 0;  // lines in the partial sum (wave-uniform)
+  for (int b = 0; b < nbins; b++) {
+    const int2 r = rng[(size_t)t * nbins + b];
+    for (int j = r.x; j < r.y; j++) {
+      const int4 mm = meta_s[j];
+      if (mm.y < p0 || mm.x > p1) continue;  // the window misses the tile (wave-uniform)
+      if (grp == FOLD) {
+        kahan_add(acc, cmp, part);
+        part = 0.f;
+        grp = 0;
+      }
+      grp++;
+      const bool act = valid && p >= mm.x && p <= mm.y;
+      const CoefB c = cb[(size_t)j * S.ntp + min(d, S.ntp - 1)];
+      float v[1] = {(float)(Wp - wlc_s[j]) * c.vv}, out[1];
+      const bool a1[1] = {act};
+      contrib32_lane(S, c, v, a1, out);
+      part += out[0];
+    }
+  }
+  if (!valid) return;
+  kahan_add(acc, cmp, part);
+  float v = acc - cmp;
+  if (S.motype == 1) v = v / (float)S.cstd[d];
+  const size_t o = (size_t)d * S.nwl + (p - 1);
+  lop[o] = lop[o] + v;
+}
+
 // The matrix-core kernel's line lists: per tile, the positions of the candidate lines (the
 // per-bin ranges of k_tile_ranges) that have a point of the tile inside their window and
 // outside their core zone, in candidate order, padded with -1 to a multiple of 4 and followed
@@ -1109,8 +1160,8 @@ __global__ void __launch_bounds__(64) k_far_lists(int ntiles, int nwl, int nbins
 // core-zone sort keys: (zone bin << 32) | centre point, the bin being the bit length of the
 // largest distance from the centre point to the ends of zone ∩ window; lines with no point in
 // zone ∩ window go to the last bin (NZB - 1), which no tile reads
-__global__ void k_zone_keys(int64_t m, int64_t mmain, int ntau, const int4 *meta_s, const int2 *zone_s,
-                            unsigned long long *key, int *val, unsigned long long *evals_core) {
+__global__ void k_zone_keys(int64_t m, int64_t mnarrow, int64_t mmain, int ntau, const int4 *meta_s,
+                            const int2 *zone_s, unsigned long long *key, int *val, unsigned long long *evals_core) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long ev = 0;  // the wavefront's core-zone evaluations, added once
   if (j < m) {
@@ -1118,7 +1169,9 @@ __global__ void k_zone_keys(int64_t m, int64_t mmain, int ntau, const int4 *meta
     const int2 z = zone_s[j];
     const int lo = max(mm.x, z.x), hi = min(mm.y, z.y);
     unsigned b = NZB - 1;
-    if (j < mmain && lo <= hi) {  // positions from mmain on hold the He I lines (their own kernel)
+    // positions below mnarrow hold the narrow lines and from mmain on the He I lines (both
+    // have kernels of their own)
+    if (j >= mnarrow && j < mmain && lo <= hi) {
       const int zh = max(max(mm.z - lo, hi - mm.z), 0);
       b = zh == 0 ? 0u : (unsigned)(32 - __clz(zh));
       ev = (unsigned long long)(hi - lo + 1) * (unsigned long long)ntau;
@@ -1612,13 +1665,13 @@ int launch_opacity32(const DevSetup &S, int ntiles, int nbins, int rstride, cons
   }
 }
 
-void launch_zone_order(int64_t m, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
+void launch_zone_order(int64_t m, int64_t mnarrow, int64_t mmain, int ntiles, int nwl, int ntau, const int4 *meta_s,
                        const int2 *zone_s, const double *wlc_s, unsigned long long *key, int *val,
                        unsigned long long *key_sorted, int *perm, ZoneRec *zrec, void *tmp, size_t tmp_bytes,
                        int *zoff, int *zend, unsigned long long *evals_core, hipStream_t st) {
   k_fill_int<<<1, 2 * NZB, 0, st>>>(zoff, 2 * NZB, 0);  // zoff and zend are adjacent
   if (m > 0) {
-    k_zone_keys<<<nblk(m, 256), 256, 0, st>>>(m, mmain, ntau, meta_s, zone_s, key, val, evals_core);
+    k_zone_keys<<<nblk(m, 256), 256, 0, st>>>(m, mnarrow, mmain, ntau, meta_s, zone_s, key, val, evals_core);
     (void)hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, key, key_sorted, val, perm, (int)m, 0, 64, st);
     k_zone_bounds<<<nblk(m, 256), 256, 0, st>>>(m, key_sorted, zoff, zend);
     k_zone_records<<<nblk(m, 256), 256, 0, st>>>(m, perm, meta_s, zone_s, wlc_s, zrec);
@@ -1654,6 +1707,14 @@ void launch_opacity32_core(const DevSetup &S, int core_pp, const int2 *rng_core,
   const int nb = 8 * ((ntiles + 7) / 8);
   if (core_pp == 1) k_opacity32_core<1><<<nb, 64 * nw, 0, st>>>(S, ntiles, rng_core, zrec, ca, cb, dmaxw, lineop);
   else k_opacity32_core<2><<<nb, 64 * nw, 0, st>>>(S, ntiles, rng_core, zrec, ca, cb, dmaxw, lineop);
+}
+
+void launch_opacity32_narrow(const DevSetup &S, int nbins, const int2 *rng, const CoefB *cb, const double *wlc_s,
+                             const int4 *meta_s, float *lineop, hipStream_t st) {
+  const int ntiles = (S.nwl + NARROW_TILE - 1) / NARROW_TILE;
+  const int nw = (S.ntau + 7) / 8;
+  if (nbins > 0 && ntiles > 0)
+    k_opacity32_narrow<<<8 * ((ntiles + 7) / 8), 64 * nw, 0, st>>>(S, ntiles, nbins, rng, cb, wlc_s, meta_s, lineop);
 }
 
 void launch_opacity64(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
