@@ -1069,6 +1069,7 @@ __global__ void __launch_bounds__(1024) k_opacity32_narrow(DevSetup S, int ntile
   if (t >= ntiles) return;  // workgroup-uniform
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int d = wave * 8 + (lane >> 3);
+  const int dc = min(d, S.ntp - 1);
   const int p0 = t * NARROW_TILE + 1, p1 = min(p0 + NARROW_TILE - 1, S.nwl);
   const int p = p0 + (lane & 7);
   const bool valid = p <= p1 && d < S.ntau;
@@ -1077,21 +1078,44 @@ __global__ void __launch_bounds__(1024) k_opacity32_narrow(DevSetup S, int ntile
   int grp = 0;  // lines in the partial sum (wave-uniform)
   for (int b = 0; b < nbins; b++) {
     const int2 r = rng[(size_t)t * nbins + b];
-    for (int j = r.x; j < r.y; j++) {
-      const int4 mm = meta_s[j];
-      if (mm.y < p0 || mm.x > p1) continue;  // the window misses the tile (wave-uniform)
-      if (grp == FOLD) {
-        kahan_add(acc, cmp, part);
-        part = 0.f;
-        grp = 0;
+    // candidates 64 at a time: each lane reads one candidate's window and centre, a ballot
+    // keeps those whose window meets the tile, and they are taken in candidate order with
+    // the next one's record loaded while the current one is evaluated
+    for (int base = r.x; base < r.y; base += 64) {
+      const int jl = base + lane;
+      int4 mm = make_int4(1, 0, 0, 0);
+      double wl = 0.0;
+      if (jl < r.y) {
+        mm = meta_s[jl];
+        wl = wlc_s[jl];
       }
-      grp++;
-      const bool act = valid && p >= mm.x && p <= mm.y;
-      const CoefB c = cb[(size_t)j * S.ntp + min(d, S.ntp - 1)];
-      float v[1] = {(float)(Wp - wlc_s[j]) * c.vv}, out[1];
-      const bool a1[1] = {act};
-      contrib32_lane(S, c, v, a1, out);
-      part += out[0];
+      uint64_t todo = __builtin_amdgcn_ballot_w64(jl < r.y && mm.y >= p0 && mm.x <= p1);
+      if (!todo) continue;
+      int k = __builtin_ctzll(todo);
+      CoefB cn = cb[(size_t)(base + k) * S.ntp + dc];
+      while (true) {
+        const CoefB c = cn;
+        const int il = __builtin_amdgcn_readlane(mm.x, k), ir = __builtin_amdgcn_readlane(mm.y, k);
+        const double wck = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(wl), k),
+                                            __builtin_amdgcn_readlane(__double2loint(wl), k));
+        todo &= todo - 1;
+        const bool more = todo != 0;
+        const int kn = more ? __builtin_ctzll(todo) : k;
+        if (more) cn = cb[(size_t)(base + kn) * S.ntp + dc];
+        if (grp == FOLD) {
+          kahan_add(acc, cmp, part);
+          part = 0.f;
+          grp = 0;
+        }
+        grp++;
+        const bool act = valid && p >= il && p <= ir;
+        float v[1] = {(float)(Wp - wck) * c.vv}, out[1];
+        const bool a1[1] = {act};
+        contrib32_lane(S, c, v, a1, out);
+        part += out[0];
+        if (!more) break;
+        k = kn;
+      }
     }
   }
   if (!valid) return;
