@@ -6,7 +6,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-from .abi import CIsotope, CLines, CSetup, CSpecies, CStats
+from .abi import CAtmosphere, CIsotope, CLines, CSetup, CSpecies, CStats
 
 LIB_PATH = Path(__file__).resolve().parent / "libfsgpu.so"
 
@@ -19,6 +19,7 @@ EXPORTS = (
     "fsg_flux_vjp", "fsg_lineop_vjp", "fsg_integ_convol", "fsg_get_spectra",
     "fsg_lines_scan", "fsg_lines_read", "fsg_lines_stream", "fsg_init_batch", "fsg_batch_offsets",
     "fsg_lines_batch", "fsg_integ_convol_noalloc", "fsg_flux_dev", "fsg_flux_vjp_dev", "fsg_lineop_vjp_dev",
+    "fsg_set_atmosphere",
 )
 
 _LIB = None
@@ -77,7 +78,7 @@ def load() -> C.CDLL:
     lib.fsg_flux_vjp.argtypes = [vp, dp, dp, dp, dp, dp]
     lib.fsg_lineop_vjp.argtypes = [vp, vp, C.c_int64, dp, dp, dp]
     lib.fsg_flux_dev.argtypes = [vp, vp, vp]
-    lib.fsg_flux_vjp_dev.argtypes = [vp, vp, vp, dp]
+    lib.fsg_flux_vjp_dev.argtypes = [vp, vp, vp, vp, dp]
     lib.fsg_lineop_vjp_dev.argtypes = [vp, vp, C.c_int64, vp, dp, dp]
     lib.fsg_device_ptr.argtypes = [vp, C.c_int32, C.POINTER(vp)]
     lib.fsg_stream.argtypes = [vp, C.POINTER(vp)]
@@ -93,6 +94,7 @@ def load() -> C.CDLL:
     lib.fsg_init_batch.argtypes = [C.POINTER(CSetup), C.c_int32, C.POINTER(vp)]
     lib.fsg_batch_offsets.argtypes = [vp, i64p]
     lib.fsg_lines_batch.argtypes = [vp, vp, i64p, ip, C.c_int64, C.POINTER(vp)]
+    lib.fsg_set_atmosphere.argtypes = [vp, C.c_int32, C.POINTER(CAtmosphere)]
     _LIB = lib
     return lib
 

@@ -74,6 +74,15 @@ class CLines(C.Structure):
     ]
 
 
+class CAtmosphere(C.Structure):
+    """fsg_atmosphere: a new EOS / continuum state for fsg_set_atmosphere (NULL: keep)."""
+    _fields_ = [
+        ("temp", _dp), ("xne", _dp), ("rho", _dp), ("rhox", _dp), ("contop", _dp), ("contop_std", _dp),
+        ("fraction_pf", _fp), ("frac_h1", _fp), ("frac_he1", _fp), ("frac_hmol", _fp),
+        ("he_gam", _dp),
+    ]
+
+
 class CStats(C.Structure):
     _fields_ = [
         ("nlines_in", C.c_int64), ("nlines_use", C.c_int64), ("nwide", C.c_int64),

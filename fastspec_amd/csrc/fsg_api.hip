@@ -185,6 +185,9 @@ struct fsg_ctx {
   // pass-1 buffers hold, for the reverse mode of the same synthesis; id 0: none
   uint64_t p1_id = 0, p1_gen = 0;
   LinePrep p1;
+  // per-depth H I, He I and H2 number fractions behind XlineInit's constants (kept for
+  // fsg_set_atmosphere)
+  const float *fh1 = nullptr, *fhe1 = nullptr, *fhmol = nullptr;
 };
 
 struct fsg_dlines {
@@ -592,10 +595,9 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
     c->muloc = R.muloc;
   }
   // per-depth perturber densities and XlineInit
-  const float *fh1, *fhe1, *fhmol;
-  rc |= upload(c, s->frac_h1, N, &fh1, c->owned);
-  rc |= upload(c, s->frac_he1, N, &fhe1, c->owned);
-  rc |= upload(c, s->frac_hmol, N, &fhmol, c->owned);
+  rc |= upload(c, s->frac_h1, N, &c->fh1, c->owned);
+  rc |= upload(c, s->frac_he1, N, &c->fhe1, c->owned);
+  rc |= upload(c, s->frac_hmol, N, &c->fhmol, c->owned);
   if (rc) {
     fsg_finalize(c);
     return 1;
@@ -605,7 +607,7 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
     fsg_finalize(c);
     return 1;
   }
-  launch_xline_init(N, S.temp, S.xne, fh1, fhe1, fhmol, s->h2brd, (double *)xln, c->st);
+  launch_xline_init(N, S.temp, S.xne, c->fh1, c->fhe1, c->fhmol, s->h2brd, (double *)xln, c->st);
   S.xln = (const double *)xln;
   S.vturb2 = 1e26 / kXlnC / kXlnC * (s->vmicro * s->vmicro);
   S.xlnz = std::sqrt(kPi) * (kCe * kCe) / (kMe * (kVc * kVc)) * 1e-18;
@@ -768,6 +770,91 @@ int fsg_init_batch(const fsg_setup *setups, int32_t nb, fsg_ctx **ctx_out) {
   }
   c->S.nbatch = nb;
   *ctx_out = c;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The depth-dependent inputs of an ordinary context replaced on its stream from host arrays
+// in the layouts of fsg_setup (NULL members keep their values).
+int set_atmosphere_one(fsg_ctx *c, const fsg_atmosphere *a) {
+  const DevSetup &S = c->S;
+  const size_t N = (size_t)S.ntau, Nw = (size_t)S.nwl;
+  hipStream_t s = c->st;
+  auto put = [s](const void *src, const void *dst, size_t bytes) {
+    return src ? hipMemcpyAsync(const_cast<void *>(dst), src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+  };
+  CK(put(a->temp, S.temp, sizeof(double) * N));
+  CK(put(a->xne, S.xne, sizeof(double) * N));
+  CK(put(a->rho, S.rho, sizeof(double) * N));
+  CK(put(a->rhox, S.rhox, sizeof(double) * N));
+  CK(put(a->fraction_pf, S.fpf, sizeof(float) * N * (size_t)S.pf_ld));
+  CK(put(a->frac_h1, c->fh1, sizeof(float) * N));
+  CK(put(a->frac_he1, c->fhe1, sizeof(float) * N));
+  CK(put(a->frac_hmol, c->fhmol, sizeof(float) * N));
+  CK(put(a->contop_std, S.cstd, sizeof(double) * N));
+  CK(put(a->he_gam, S.he_gam, sizeof(double) * (size_t)S.nhe * N));
+  std::vector<double> ct;
+  if (a->contop) {  // ContOpac(Ntau, Nwl) -> [ntau][nwl]
+    ct.resize(N * Nw);
+    for (size_t iw = 0; iw < Nw; iw++)
+      for (size_t k = 0; k < N; k++) ct[k * Nw + iw] = a->contop[iw * N + k];
+    CK(put(ct.data(), S.contT, sizeof(double) * ct.size()));
+  }
+  // XlineInit's per-depth constants follow T, Xne and the perturber fractions
+  if (a->temp || a->xne || a->frac_h1 || a->frac_he1 || a->frac_hmol)
+    launch_xline_init(S.ntau, S.temp, S.xne, c->fh1, c->fhe1, c->fhmol, S.h2brd, const_cast<double *>(S.xln), s);
+  CK(hipGetLastError());
+  CK(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsg_set_atmosphere(fsg_ctx *c, int32_t atm, const fsg_atmosphere *a) {
+  g_err.clear();
+  if (!c || !a) return fail("fsg_set_atmosphere: NULL argument");
+  if (atm < 0 || atm >= std::max(c->S.nbatch, 1)) return fail("fsg_set_atmosphere: atmosphere index out of range");
+  fsg_ctx *k = c->S.batch ? c->kids[atm] : c;
+  if (a->contop_std && k->S.motype != 1) return fail("fsg_set_atmosphere: ContOpacStd belongs to Motype 1 models");
+  if (a->rhox && k->S.sph) return fail("fsg_set_atmosphere: the rays of a spherical model are fixed at fsg_init");
+  if (a->he_gam && k->S.nhe == 0) return fail("fsg_set_atmosphere: the context has no He I Stark tables");
+  if (a->temp && k->S.nhe > 0 && !a->he_gam)
+    return fail("fsg_set_atmosphere: a new temperature needs the He I Stark widths at it (he_gam)");
+  CK(hipSetDevice(c->dev));
+  if (set_atmosphere_one(k, a)) return 1;
+  if (c->S.batch) {
+    // the batch's own ContOpac over the concatenated grid: this atmosphere's segment, its
+    // padding points repeating the last column
+    const size_t N = (size_t)c->S.ntau, tot = (size_t)c->S.nwl;
+    const size_t o = (size_t)c->boff[atm], seg = (size_t)(c->boff[atm + 1] - c->boff[atm]);
+    const size_t nw = (size_t)k->S.nwl;
+    if (a->contop) {
+      std::vector<double> h(N * seg);
+      for (size_t kk = 0; kk < N; kk++)
+        for (size_t p = 0; p < seg; p++) h[kk * seg + p] = a->contop[std::min(p, nw - 1) * N + kk];
+      CK(hipMemcpy2DAsync(const_cast<double *>(c->S.contT) + o, sizeof(double) * tot, h.data(), sizeof(double) * seg,
+                          sizeof(double) * seg, N, hipMemcpyHostToDevice, c->st));
+      CK(hipStreamSynchronize(c->st));
+    }
+    // the batch context's own set-up is the first atmosphere's
+    if (atm == 0) {
+      fsg_atmosphere b = *a;
+      b.contop = nullptr;
+      if (set_atmosphere_one(c, &b)) return 1;
+    }
+  }
+  // windows, flags, the formal solution and the reverse-mode state of earlier calls belong
+  // to the old atmosphere
+  for (fsg_ctx *x : {c, k}) {
+    x->p1_id = 0;
+    x->have_rt = 0;
+    x->have_glop = 0;
+  }
   return 0;
 }
 
@@ -1701,7 +1788,7 @@ namespace {
 
 // fsg_flux_vjp with dL/dSfL, dL/dSfC in host memory or (dev_in) in device memory
 int flux_vjp_run(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glop_out, double *gcont_out,
-                 double *gtemp_out, bool dev_in) {
+                 double *gtemp_out, bool dev_in, double *gcont_dev = nullptr) {
   g_err.clear();
   if (!c || !gsfl || !gsfc) return fail("fsg_flux_vjp: NULL argument");
   if (!c->have_rt || c->magnetic) return fail("fsg_flux_vjp: needs the scalar formal solution of fsg_radtran");
@@ -1739,6 +1826,7 @@ int flux_vjp_run(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glo
     for (size_t k = 0; k < N; k++)
       for (size_t iw = 0; iw < Nw; iw++) out[iw * N + k] = h[k * Nw + iw];
   }
+  if (gcont_dev) CK(hipMemcpyAsync(gcont_dev, c->vgcont.p, sizeof(double) * N * Nw, hipMemcpyDeviceToDevice, s));
   if (gtemp_out) CK(hipMemcpyAsync(gtemp_out, dgt, sizeof(double) * NB * N, hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
   c->have_glop = 1;
@@ -1754,8 +1842,9 @@ int fsg_flux_vjp(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glo
   return flux_vjp_run(c, gsfl, gsfc, glop_out, gcont_out, gtemp_out, false);
 }
 
-int fsg_flux_vjp_dev(fsg_ctx *c, const double *gsfl_dev, const double *gsfc_dev, double *gtemp_out) {
-  return flux_vjp_run(c, gsfl_dev, gsfc_dev, nullptr, nullptr, gtemp_out, true);
+int fsg_flux_vjp_dev(fsg_ctx *c, const double *gsfl_dev, const double *gsfc_dev, double *gcont_dev,
+                     double *gtemp_out) {
+  return flux_vjp_run(c, gsfl_dev, gsfc_dev, nullptr, nullptr, gtemp_out, true, gcont_dev);
 }
 
 // Reverse mode of LineopCalc for one list from the resident dL/dLineOpac: pass 1 again

@@ -11,6 +11,7 @@ reference's byte layout so that existing readers keep working:
       i4 Nrec, S8 species[Nrec], i4 ion[Nrec], f8 lambda[Nrec], f4 depth[Nrec]
   fs_lineop.bin  WriteOut   src/ioutils.f90:1759-1766 (also read back by the OPACITY keyword)
       i4 Nwl, i4 Ntau, f8 lambda[Nwl], f8 LineOpac(Ntau, Nwl), f8 Rhox, T, Xne, Xna, Rho [Ntau]
+  fs_contop.bin  WriteOut   src/ioutils.f90:1773-1786, the same layout with ContOpac
 
 Arrays passed in use this package's host layouts: intensities [nwl][nmu], Stokes
 [nwl][4][nmu], LineOpac [nwl][ntau] (the memory order of the Fortran arrays).
@@ -131,6 +132,12 @@ def write_lineop(path, wave, lineop, rhox, temp, xne, xna, rho):
         _w(f, lineop, F8)
         for a in (rhox, temp, xne, xna, rho):
             _w(f, a, F8)
+
+
+def write_contop(path, wave, contop, rhox, temp, xne, xna, rho):
+    """The continuum-opacity file (src/ioutils.f90:1773-1786): the line-opacity layout with
+    ContOpac in place of LineOpac."""
+    write_lineop(path, wave, contop, rhox, temp, xne, xna, rho)
 
 
 def read_lineop(path, expect_wave: Optional[np.ndarray] = None) -> dict:

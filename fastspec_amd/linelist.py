@@ -205,8 +205,9 @@ class StreamedLines:
         kept = C.c_int64()
         _lib.check(self._lib.fsg_lines_stream(synth.ctx, str(path).encode(), wstart, wend, tab, ns, itab, ni,
                                               mol_slot, nthreads, C.byref(self.handle), C.byref(kept)))
-        from .synth import _LIVE
+        from .synth import _LIVE, _adopt
         _LIVE.add(self)  # released at exit before the contexts
+        _adopt(synth, self)
         self.n = kept.value
         self.list_type = scan(path, wstart, wend)[0]
 

@@ -40,6 +40,12 @@ def _release_all():
             o.close()
 
 
+def _adopt(synth, dlist) -> None:
+    """Records a device list on its context, which frees it before fsg_finalize (a list
+    released after its context would touch freed state)."""
+    synth.__dict__.setdefault("_lists", weakref.WeakSet()).add(dlist)
+
+
 class DeviceLines:
     """A line list resident in HBM (fsg_lines_upload)."""
 
@@ -52,6 +58,7 @@ class DeviceLines:
         self.handle = C.c_void_p()
         _lib.check(self._lib.fsg_lines_upload(synth.ctx, lines.list_type, C.byref(cl), C.byref(self.handle)))
         _LIVE.add(self)
+        _adopt(synth, self)
 
     def free(self):
         if self.handle:
@@ -78,9 +85,12 @@ class FastSpecGPU:
         _lib.check(self._lib.fsg_init(C.byref(self._cs), C.byref(self.ctx)))
         _LIVE.add(self)
         self.nwl_total = setup.nwl  # points of the arrays the library returns
+        self.fraction_pf0 = [setup.fraction_pf.copy()]  # FractionPF as built (per atmosphere)
 
     def close(self):
         if self.ctx:
+            for dl in list(getattr(self, "_lists", ())):
+                dl.free()
             self._lib.fsg_finalize(self.ctx)
             self.ctx = C.c_void_p()
 
@@ -95,6 +105,38 @@ class FastSpecGPU:
 
     def __exit__(self, *exc):
         self.close()
+
+    # --- atmosphere state -------------------------------------------------------------
+    _ATM_F64 = ("temp", "xne", "rho", "rhox", "contop", "contop_std", "he_gam")
+    _ATM_F32 = ("fraction_pf", "frac_h1", "frac_he1", "frac_hmol")
+
+    def set_atmosphere(self, atm: int = 0, **arrays) -> None:
+        """fsg_set_atmosphere: a new EOS / continuum state of the context (of atmosphere atm
+        of a batch) -- any of temp, xne, rho, rhox [ntau], contop [nwl][ntau], contop_std [ntau],
+        fraction_pf [ntau][pf_ld], frac_h1, frac_he1, frac_hmol [ntau], he_gam [nhe][ntau];
+        the host copy of the set-up follows.  Line opacity must be computed again after it."""
+        import dataclasses
+        from .abi import CAtmosphere
+        unknown = set(arrays) - set(self._ATM_F64 + self._ATM_F32)
+        if unknown:
+            raise TypeError(f"set_atmosphere: unknown arrays {sorted(unknown)}")
+        setups = getattr(self, "setups", None)
+        if not 0 <= atm < (len(setups) if setups is not None else 1):
+            raise ValueError("set_atmosphere: atmosphere index out of range")
+        old = setups[atm] if setups is not None else self.setup
+        given = {k: v for k, v in arrays.items() if v is not None}
+        new = dataclasses.replace(old, **given)  # converts to the set-up's dtypes
+        for k in given:
+            ref = getattr(old, k)
+            if ref is not None and getattr(new, k).shape != ref.shape:
+                raise ValueError(f"set_atmosphere: {k} has shape {getattr(new, k).shape}, expected {ref.shape}")
+        ca = CAtmosphere(**{k: ptr(getattr(new, k), C.c_double if k in self._ATM_F64 else C.c_float)
+                            for k in given})
+        _lib.check(self._lib.fsg_set_atmosphere(self.ctx, int(atm), C.byref(ca)))
+        if setups is not None:
+            setups[atm] = new
+        if atm == 0:
+            self.setup = new
 
     # --- line opacity -------------------------------------------------------------------
     def LineopCalc(self, lines: LineList, NlinesLim: Optional[int] = None, linstr: bool = False) -> dict:
@@ -190,13 +232,19 @@ class FastSpecGPU:
             assert t.is_cuda and t.dtype.itemsize == 8 and t.is_contiguous() and t.numel() == self.nwl_total
         _lib.check(self._lib.fsg_flux_dev(self.ctx, C.c_void_p(sfl.data_ptr()), C.c_void_p(sfc.data_ptr())))
 
-    def flux_vjp_device(self, gsfl, gsfc) -> np.ndarray:
+    def flux_vjp_device(self, gsfl, gsfc, gcont=None) -> np.ndarray:
         """Reverse mode of RadTranCalc + flux from dL/dSfL, dL/dSfC in finished float64 CUDA
-        tensors (fsg_flux_vjp_dev); returns the source-function part of dL/dT (host)."""
+        tensors (fsg_flux_vjp_dev); dL/dContOpac into gcont (a float64 CUDA tensor of ntau x
+        nwl values, wavelength fastest) when given; returns the source-function part of dL/dT
+        (host)."""
         for t in (gsfl, gsfc):
             assert t.is_cuda and t.dtype.itemsize == 8 and t.is_contiguous() and t.numel() == self.nwl_total
+        if gcont is not None:
+            assert gcont.is_cuda and gcont.dtype.itemsize == 8 and gcont.is_contiguous()
+            assert gcont.numel() == self.nwl_total * self.setup.ntau
         gt = np.zeros(self._gtemp_shape())
         _lib.check(self._lib.fsg_flux_vjp_dev(self.ctx, C.c_void_p(gsfl.data_ptr()), C.c_void_p(gsfc.data_ptr()),
+                                              C.c_void_p(gcont.data_ptr() if gcont is not None else None),
                                               ptr(gt, C.c_double)))
         return gt
 
@@ -300,6 +348,61 @@ class FastSpecGPU:
         _lib.check(self._lib.fsg_get_spectra(self.ctx, ptr(spl, C.c_double), None, ptr(spc, C.c_double)))
         return spl, spc
 
+    # --- output files -------------------------------------------------------------------
+    def WriteOut(self, prefix: str = "fs", out_flags=(0, 1, 1, 1, 1), xna=None) -> list:
+        """WriteOut (src/ioutils.f90:1751-1823): the context's results in the reference's
+        output files, record for record -- <prefix>_lineop.bin (flag 2: LineOpac; the
+        reference writes no file of MagLineOpac), _contop.bin (flag 3: the ContOpac the context was built with), _spec.bin (flag 4, Nmu
+        > 0: the last formal solution, polarised after MagRadTranCalc) and _flux.bin (flag 5,
+        Nmu > 1: SfL, SfC as the last flux or IntegConvol left them).  Xna, which the library
+        does not hold, is written as given (zeros by default).  Returns the paths written."""
+        from . import formats as F
+        s = self.setup
+        lam = s.wavelengths
+        xna = np.zeros(s.ntau) if xna is None else np.asarray(xna, np.float64)
+        atm = (s.rhox, s.temp, s.xne, xna, s.rho)
+        fl = tuple(out_flags) + (0,) * (5 - len(out_flags))
+        out = []
+        if fl[1] > 0:
+            out.append(f"{prefix}_lineop.bin")
+            F.write_lineop(out[-1], lam, self.get_lineop(), *atm)
+        if fl[2] > 0:
+            out.append(f"{prefix}_contop.bin")
+            F.write_contop(out[-1], lam, s.contop, *atm)
+        if fl[3] > 0 and s.nmu > 0:
+            out.append(f"{prefix}_spec.bin")
+            if getattr(self, "_polarised", False):
+                spz, spc = self.get_spectra(magnetic=True)
+                F.write_spec(out[-1], lam, s.xmu, spc=spc, spz=spz)
+            else:
+                spl, spc = self.get_spectra(magnetic=False)
+                F.write_spec(out[-1], lam, s.xmu, spl, spc)
+        if fl[4] > 0 and s.nmu > 1:
+            out.append(f"{prefix}_flux.bin")
+            F.write_flux(out[-1], lam, *self.flux())
+        return out
+
+    write_out = WriteOut
+
+    def WriteLinStr(self, path: str, lines: LineList, result: dict, species, append: bool = False) -> int:
+        """WriteLinStr (src/ioutils.f90:1826-1875): one block of the line-strength file for a
+        list that LineopCalc/MagLineopCalc ran with linstr=True -- the retained lines (flag
+        bit 0 clear) in list order, their species name (species[spc - 1], as Elemc), ion,
+        wavelength (Wlcent + Wfirst) and line depth.  The first list creates the file, later
+        ones append (append=True).  Returns the record count."""
+        from . import formats as F
+        if result.get("linstr") is None:
+            raise ValueError("WriteLinStr: the list ran without linstr=True")
+        keep = np.flatnonzero((result["flags"] & 1) == 0)
+        sidx = lines.spc[keep] - 1
+        names = [species[k] for k in sidx]
+        wfirst = self.setup.wfirst
+        F.write_linst(path, names, self.setup.spc_ion[sidx], (lines.wl[keep] - wfirst) + wfirst,
+                      result["linstr"][keep], append=append)
+        return int(keep.size)
+
+    write_linstr = WriteLinStr
+
     def device_ptr(self, which: int) -> int:
         p = C.c_void_p()
         _lib.check(self._lib.fsg_device_ptr(self.ctx, which, C.byref(p)))
@@ -330,6 +433,7 @@ class BatchLines:
         _lib.check(self._lib.fsg_lines_batch(batch.ctx, shared.handle, ptr(self.index, C.c_int64),
                                              ptr(self.atm, C.c_int32), self.n, C.byref(self.handle)))
         _LIVE.add(self)
+        _adopt(batch, self)
 
     def per_atmosphere(self, a):
         """Splits an array over the batch list's lines into the atmospheres' parts."""
@@ -363,6 +467,7 @@ class FastSpecBatch(FastSpecGPU):
         self.off = np.zeros(self.nb + 1, np.int64)
         _lib.check(self._lib.fsg_batch_offsets(self.ctx, ptr(self.off, C.c_int64)))
         self.nwl_total = int(self.off[-1])
+        self.fraction_pf0 = [su.fraction_pf.copy() for su in self.setups]
 
     def _gtemp_shape(self):
         return (self.nb, self.setup.ntau)

@@ -295,13 +295,37 @@ int fsg_lineop_vjp(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, double *glo
  * context's device, written or read on the library stream, which is synchronised before
  * return (producers of *_dev inputs must have finished).
  *   fsg_flux_dev        SfL, SfC [nwl] to device memory (either may be NULL)
- *   fsg_flux_vjp_dev    dL/dSfL, dL/dSfC [nwl] from device memory; gtemp_out host [ntau]
- *                       (batch: [nb][ntau]) or NULL
+ *   fsg_flux_vjp_dev    dL/dSfL, dL/dSfC [nwl] from device memory; gcont_dev: dL/dContOpac
+ *                       [ntau][nwl] (lambda fastest) to device memory, or NULL; gtemp_out host
+ *                       [ntau] (batch: [nb][ntau]) or NULL
  *   fsg_lineop_vjp_dev  dL/d(log10 gf) [n] to device memory; gfpf_inout, gtemp_inout host */
 int fsg_flux_dev(fsg_ctx *ctx, double *sfl_dev, double *sfc_dev);
-int fsg_flux_vjp_dev(fsg_ctx *ctx, const double *gsfl_dev, const double *gsfc_dev, double *gtemp_out);
+int fsg_flux_vjp_dev(fsg_ctx *ctx, const double *gsfl_dev, const double *gsfc_dev, double *gcont_dev,
+                     double *gtemp_out);
 int fsg_lineop_vjp_dev(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, double *gloggf_dev,
                        double *gfpf_inout, double *gtemp_inout);
+
+/* A new atmosphere state on an existing context: the results of the host's EOS and
+ * continuum (EosSolve, src/synth.f90:334-406; ContopCalc, src/synth.f90:642-674) for a new
+ * temperature or abundance vector, replacing in place what fsg_init uploaded.  No
+ * counterpart in the reference, which synthesises one atmosphere per run; it serves fits
+ * that move T and abundances (the host maps them through its EOS and continuum and chains
+ * their Jacobians onto the reverse mode's dL/dT, dL/d ln FractionPF and dL/dContOpac).
+ * Arrays as in fsg_setup; NULL members keep the context's values.  rhox only for
+ * plane-parallel models (the rays of a spherical one are built from it); contop_std only for
+ * Motype 1 models; a context with He I Stark tables needs he_gam with a new temperature.
+ * atm: the atmosphere of a batch context (0 for an ordinary context).  Windows, flags and
+ * formal solutions of earlier calls are dropped: line opacity must be computed again. */
+typedef struct fsg_atmosphere {
+  const double *temp, *xne, *rho;        /* [ntau] */
+  const double *rhox;                    /* [ntau] column mass (plane-parallel models) */
+  const double *contop;                  /* [nwl][ntau] ContOpac(Ntau, Nwl) */
+  const double *contop_std;              /* [ntau] */
+  const float *fraction_pf;              /* [ntau][pf_ld] */
+  const float *frac_h1, *frac_he1, *frac_hmol;  /* [ntau] */
+  const double *he_gam;                  /* [nhe][ntau] */
+} fsg_atmosphere;
+int fsg_set_atmosphere(fsg_ctx *ctx, int32_t atm, const fsg_atmosphere *a);
 
 /* Batch contexts: many model atmospheres synthesised in one launch sequence (the
  * differentiable batch of many small syntheses).  No counterpart in the reference, which

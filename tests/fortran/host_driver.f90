@@ -3,7 +3,11 @@
 ! TEST INFRASTRUCTURE.  Reads a case file written by tests/fortran_case.py into the
 ! reference's module globals (as Init/EosSolve/ContopCalc would leave them), runs
 ! GpuInit, GpuLineopCalc, GpuRadTranCalc and GpuFlux, and writes the flux, the line flags
-! and the line strengths to out.bin.
+! and the line strengths to out.bin.  It then leaves the results, fetched into the module
+! arrays by the binding, in the reference's output files fs_lineop.bin, fs_spec.bin,
+! fs_flux.bin and fs_linst.bin, with the unformatted stream records of WriteOut and
+! WriteLinStr (src/ioutils.f90:1751-1875; that file needs the HDF5 Fortran modules, which
+! this image lacks, so its records are repeated here).
 PROGRAM HostDriver
   USE Constants
   USE Config
@@ -21,6 +25,8 @@ PROGRAM HostDriver
   INTEGER :: nslot, nspc_, nl, ltype
   LOGICAL :: DoLinStr
   CHARACTER(256) :: arg
+  INTEGER, ALLOCATABLE :: keep(:)
+  INTEGER :: i
 
   CALL GET_COMMAND_ARGUMENT(1, arg)
   prec = FSG_FP32
@@ -95,6 +101,33 @@ PROGRAM HostDriver
   WRITE(11) SfL, SfC, SpL
   IF (DoLinStr) WRITE(11) LinFlag, LinStr
   CLOSE(11)
+  CALL GpuGetLineop
+  OPEN(12, FILE='fs_lineop.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+  WRITE(12) Nwl, Ntau
+  WRITE(12) Wave + Wfirst, LineOpac
+  WRITE(12) Atmos%Rhox, Atmos%T, Atmos%Xne, Atmos%Xna, Atmos%Rho
+  CLOSE(12)
+  OPEN(12, FILE='fs_spec.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+  WRITE(12) Nwl, Nmu
+  WRITE(12) Xmu
+  WRITE(12) Wave + Wfirst
+  WRITE(12) SpL, SpC
+  CLOSE(12)
+  OPEN(12, FILE='fs_flux.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+  WRITE(12) Nwl
+  WRITE(12) Wave + Wfirst
+  WRITE(12) SfL, SfC
+  CLOSE(12)
+  IF (DoLinStr) THEN
+    keep = PACK([(i, i=1, Nlines)], .NOT. BTEST(LinFlag, 0))
+    OPEN(12, FILE='fs_linst.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+    WRITE(12) SIZE(keep)
+    WRITE(12) Elemc(LinSpc(keep))
+    WRITE(12) Ion(LinSpc(keep))
+    WRITE(12) Wlcent(keep) + Wfirst
+    WRITE(12) LinStr(keep)
+    CLOSE(12)
+  END IF
   CALL GpuLineopRelease
   CALL GpuFinalize
   WRITE(*, '(A, I0)') 'host_driver: done, lines used ', NlinesUse

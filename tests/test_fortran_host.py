@@ -56,6 +56,37 @@ def test_fortran_host_matches_oracle(tmp_path, prec):
     tol = 1e-8 if prec == 0 else 1e-4
     assert np.max(np.abs(out["sfl"] / out["sfc"] - sfl / sfc)) <= tol
     assert np.max(np.abs(out["spl"] - spl) / spl) <= (1e-9 if prec == 0 else 1e-6)
+    _check_output_files(tmp_path, setup, lines, orc, ro, prec)
+
+
+def _check_output_files(tmp_path, setup, lines, orc, ro, prec):
+    """The GPU synthesis in the reference's output files (WriteOut, WriteLinStr records),
+    read back with the byte-layout readers that parse the reference's shipped examples."""
+    from fastspec_amd import formats as F
+    lam = setup.wavelengths
+    lop_ref = orc.get_lineop()
+    spl, spc = orc.radtran()
+    sfl, sfc = orc.flux()
+    s = F.read_spec(tmp_path / "fs_spec.bin")
+    assert not s["magnetic"] and np.array_equal(s["wave"], lam) and np.array_equal(s["xmu"], setup.xmu)
+    assert np.max(np.abs(s["spl"] - spl) / spl) <= (1e-9 if prec == 0 else 1e-6)
+    assert np.max(np.abs(s["spc"] - spc) / spc) <= 1e-9
+    f = F.read_flux(tmp_path / "fs_flux.bin")
+    assert np.array_equal(f["wave"], lam)
+    assert np.max(np.abs(f["sfl"] / f["sfc"] - sfl / sfc)) <= (1e-8 if prec == 0 else 1e-4)
+    lo = F.read_lineop(tmp_path / "fs_lineop.bin", expect_wave=lam)
+    assert np.array_equal(lo["temp"], setup.temp) and np.array_equal(lo["rho"], setup.rho)
+    assert np.array_equal(lo["rhox"], setup.rhox) and np.array_equal(lo["xne"], setup.xne)
+    rel = np.max(np.abs(lo["lineop"] - lop_ref) / (lop_ref + setup.contop))
+    assert rel <= (1e-7 if prec == 0 else 5e-4)
+    ls = F.read_linst(tmp_path / "fs_linst.bin")
+    keep = (ro["flags"] & 1) == 0
+    assert ls["blocks"] == 1 and len(ls["wave"]) == int(keep.sum()) == ro["stats"]["nlines_use"]
+    # Wlcent + Wfirst, Wlcent = Wl - Wfirst as the host keeps it
+    assert np.array_equal(ls["wave"], (lines.wl[keep] - setup.wfirst) + setup.wfirst)
+    assert set(ls["species"]) == {"Fe"}
+    assert np.array_equal(ls["ion"], setup.spc_ion[lines.spc[keep] - 1])
+    assert np.max(np.abs(ls["depth"] - ro["linstr"][keep])) <= 2e-6
 
 
 @pytest.mark.gpu
