@@ -6,7 +6,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-from .abi import CAtmosphere, CIsotope, CLines, CSetup, CSpecies, CStats
+from .abi import CAtmosphere, CContopEos, CIsotope, CLines, CSetup, CSpecies, CStats
 
 LIB_PATH = Path(__file__).resolve().parent / "libfsgpu.so"
 
@@ -19,7 +19,7 @@ EXPORTS = (
     "fsg_flux_vjp", "fsg_lineop_vjp", "fsg_integ_convol", "fsg_get_spectra",
     "fsg_lines_scan", "fsg_lines_read", "fsg_lines_stream", "fsg_init_batch", "fsg_batch_offsets",
     "fsg_lines_batch", "fsg_integ_convol_noalloc", "fsg_flux_dev", "fsg_flux_vjp_dev", "fsg_lineop_vjp_dev",
-    "fsg_set_atmosphere",
+    "fsg_set_atmosphere", "fsg_contop",
 )
 
 _LIB = None
@@ -95,6 +95,7 @@ def load() -> C.CDLL:
     lib.fsg_batch_offsets.argtypes = [vp, i64p]
     lib.fsg_lines_batch.argtypes = [vp, vp, i64p, ip, C.c_int64, C.POINTER(vp)]
     lib.fsg_set_atmosphere.argtypes = [vp, C.c_int32, C.POINTER(CAtmosphere)]
+    lib.fsg_contop.argtypes = [vp, C.POINTER(CContopEos), C.c_int32, C.c_int32, dp]
     _LIB = lib
     return lib
 

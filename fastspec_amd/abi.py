@@ -83,6 +83,11 @@ class CAtmosphere(C.Structure):
     ]
 
 
+class CContopEos(C.Structure):
+    """fsg_contop_eos: the EOS rows the device's continuum sources read ([ntau] REAL(4))."""
+    _fields_ = [("fpf_h1", _fp), ("fpf_h2", _fp), ("frac_hmin", _fp), ("fpf_he1", _fp), ("frac_hmol", _fp)]
+
+
 class CStats(C.Structure):
     _fields_ = [
         ("nlines_in", C.c_int64), ("nlines_use", C.c_int64), ("nwide", C.c_int64),

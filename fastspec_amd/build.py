@@ -14,8 +14,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libfsgpu.so"
 CALIB = PKG / "hbm_copy_cal"
-SOURCES = ["fsg_kernels.hip", "fsg_magnetic.hip", "fsg_adjoint.hip", "fsg_convol.hip", "fsg_ingest.hip",
-           "fsg_hdf5.hip", "fsg_api.hip"]
+SOURCES = ["fsg_kernels.hip", "fsg_magnetic.hip", "fsg_adjoint.hip", "fsg_convol.hip", "fsg_contop.hip",
+           "fsg_ingest.hip", "fsg_hdf5.hip", "fsg_api.hip"]
 ARCH = os.environ.get("FSG_OFFLOAD_ARCH", "gfx950")
 
 # Packed FP32 instructions are disabled for all device code.  MFMA accumulators are kept in

@@ -188,6 +188,7 @@ struct fsg_ctx {
   // per-depth H I, He I and H2 number fractions behind XlineInit's constants (kept for
   // fsg_set_atmosphere)
   const float *fh1 = nullptr, *fhe1 = nullptr, *fhmol = nullptr;
+  DBuf ctab;  // fsg_contop: per-depth state and the H- tables
 };
 
 struct fsg_dlines {
@@ -662,7 +663,7 @@ int fsg_finalize(fsg_ctx *c) {
   for (void *p : c->owned) (void)hipFree(p);
   for (DBuf *b : {&c->flag, &c->il, &c->ir, &c->ic, &c->key, &c->key2, &c->idx, &c->idx2, &c->coef, &c->wlc,
                   &c->meta, &c->rng, &c->rng2, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp, &c->vscr,
-                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm, &c->zrec, &c->rngn, &c->fcap, &c->foff, &c->ftmp, &c->flist, &c->cv_a,
+                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm, &c->zrec, &c->rngn, &c->fcap, &c->foff, &c->ftmp, &c->flist, &c->ctab, &c->cv_a,
                   &c->cv_b, &c->cv_kern, &c->cv_rows, &c->cv_w})
     b->release();
   if (c->st2) (void)hipStreamSynchronize(c->st2);
@@ -855,6 +856,71 @@ int fsg_set_atmosphere(fsg_ctx *c, int32_t atm, const fsg_atmosphere *a) {
     x->have_rt = 0;
     x->have_glop = 0;
   }
+  return 0;
+}
+
+int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t add, double *contop_out) {
+  g_err.clear();
+  if (!c || !e) return fail("fsg_contop: NULL argument");
+  if (c->S.batch) return fail("fsg_contop: a batch context takes each atmosphere's ContOpac through fsg_set_atmosphere");
+  const unsigned mask = (unsigned)ifop_mask;
+  // H2PLOP, HMINOP, HRAYOP, HEMIOP, HERAOP, ELECOP, H2RAOP (Ifop 2, 3, 4, 7, 8, 12, 13)
+  const unsigned on_device = (1u << 1) | (1u << 2) | (1u << 3) | (1u << 6) | (1u << 7) | (1u << 11) | (1u << 12);
+  if (mask & ~on_device) {
+    std::string m = "fsg_contop: continuum source(s)";
+    for (int b = 0; b < 32; b++)
+      if ((mask & ~on_device) >> b & 1u) m += " " + std::to_string(b + 1);
+    return fail(m + " are not evaluated on the device (the host's Xcont keeps them)");
+  }
+  const bool h1 = mask & ((1u << 1) | (1u << 2) | (1u << 3)), he1 = mask & ((1u << 6) | (1u << 7));
+  if ((h1 && !e->fpf_h1) || ((mask >> 1 & 1u) && !e->fpf_h2) || ((mask >> 2 & 1u) && !e->frac_hmin) ||
+      (he1 && !e->fpf_he1) || ((mask >> 12 & 1u) && !e->frac_hmol))
+    return fail("fsg_contop: a population the selected sources need is NULL");
+  CK(hipSetDevice(c->dev));
+  const DevSetup &S = c->S;
+  const size_t N = (size_t)S.ntau, Nw = (size_t)S.nwl;
+  hipStream_t s = c->st;
+  // per-depth state as ContopSetup forms it (src/synth.f90:589-596): T, Xne and Rho from the
+  // context's atmosphere
+  std::vector<double> T(N), xne(N), rho(N);
+  CK(hipMemcpyAsync(T.data(), S.temp, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(xne.data(), S.xne, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  CK(hipMemcpyAsync(rho.data(), S.rho, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  std::vector<double> tab;
+  contop_tables(tab);
+  std::vector<double> blk(11 * N + tab.size(), 0.0);
+  auto col = [&](int r) { return blk.data() + (size_t)r * N; };
+  const float *pops[5] = {e->fpf_h1, e->fpf_h2, e->frac_hmin, e->fpf_he1, e->frac_hmol};
+  for (size_t k = 0; k < N; k++) {
+    const double tk = kKbolt * T[k];
+    col(0)[k] = T[k];
+    col(1)[k] = kHc / tk;     // Hkt
+    col(2)[k] = tk * kErg2ev;  // Tkev
+    col(3)[k] = 5040.0 / T[k];
+    col(4)[k] = xne[k];
+    col(5)[k] = rho[k];
+    for (int q = 0; q < 5; q++) col(6 + q)[k] = pops[q] ? (double)pops[q][k] : 0.0;
+  }
+  std::copy(tab.begin(), tab.end(), blk.begin() + 11 * N);
+  CK(c->ctab.ensure(sizeof(double) * blk.size()));
+  double *dblk = c->ctab.as<double>();
+  CK(hipMemcpyAsync(dblk, blk.data(), sizeof(double) * blk.size(), hipMemcpyHostToDevice, s));
+  launch_contop(S.nwl, S.ntau, S.wave, S.wfirst, dblk, dblk + 11 * N, S.cstd, S.motype, mask, add ? 1 : 0,
+                const_cast<double *>(S.contT), s);
+  CK(hipGetLastError());
+  if (contop_out) {  // [ntau][nwl] -> ContOpac(Ntau, Nwl)
+    std::vector<double> h(N * Nw);
+    CK(hipMemcpyAsync(h.data(), S.contT, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    for (size_t k = 0; k < N; k++)
+      for (size_t iw = 0; iw < Nw; iw++) contop_out[iw * N + k] = h[k * Nw + iw];
+  }
+  CK(hipStreamSynchronize(s));
+  // windows and formal solutions of earlier calls belong to the old continuum
+  c->p1_id = 0;
+  c->have_rt = 0;
+  c->have_glop = 0;
   return 0;
 }
 

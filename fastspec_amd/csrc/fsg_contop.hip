@@ -1,0 +1,262 @@
+// fsg_contop.hip -- continuum opacity on the device: ContopCalc's per-wavelength loop
+// (src/synth.f90:663-669) for the sources of Xcont (src/contop.f90:267-314) that depend on
+// the hydrogen, helium and electron populations alone:
+//
+//   H2PLOP  H2+ bound-free and free-free        src/contop.f90:1102-1113
+//   HMINOP  H- bound-free and free-free         src/contop.f90:1128-1181
+//   HRAYOP  Rayleigh scattering by H I          src/contop.f90:1115-1124
+//   HEMIOP  He- free-free                       src/contop.f90:1518-1527
+//   HERAOP  Rayleigh scattering by He I         src/contop.f90:1529-1539
+//   ELECOP  Thomson scattering                  src/contop.f90:2764-2768
+//   H2RAOP  Rayleigh scattering by H2           src/contop.f90:2770-2780
+//   Linter, Map1 (HMINOP's interpolations)      src/contop.f90:500-601
+//
+// One lane per wavelength of the grid: the wavelength's own quantities (frequency, the H-
+// free-free table interpolated in wavelength for each of its 11 temperatures, the H-
+// bound-free cross-section, the Rayleigh cross-sections) are formed once, then the lane runs
+// down the depths, writing ContOpac [ntau][nwl] (wavelength fastest: a wavefront's stores are
+// contiguous).  Per-depth state (T, hc/kT, kT in eV, 5040/T, Xne, Rho and the five
+// populations) comes from the host, formed there as ContopSetup forms it.  All arithmetic is
+// FP64 in the reference's order without contraction (the pragma below also covers the
+// air-to-vacuum conversion of the shared header), so that each source matches the compiled
+// reference to rounding.
+#pragma clang fp contract(off)
+
+#include <cmath>
+#include <vector>
+
+#include "fsg_device.h"
+#include "fsg_kernels.h"
+
+namespace fsg {
+
+// Linter (src/contop.f90:500-516) at one abscissa: linear interpolation between the first
+// tabulated point above x and the one before it (extrapolation past either end); the
+// ordinates yo[0], yo[ys], yo[2 ys], ...
+__device__ __forceinline__ double linter1(const double *xo, const double *yo, int ys, int n, double x) {
+  int i = 1;  // 0-based index of the upper point
+  while (x >= xo[i] && i < n - 1) i++;
+  const double y0 = yo[(i - 1) * ys], y1 = yo[i * ys];
+  return y0 + (y1 - y0) / (xo[i] - xo[i - 1]) * (x - xo[i - 1]);
+}
+
+// Map1 (src/contop.f90:518-601) at one abscissa: the parabolas through the neighbouring
+// points on either side of x, blended by the size of their curvature; straight lines in the
+// first two intervals and past the last point.
+__device__ double map1_one(const double *xo, const double *fo, int n, double x) {
+  int l = 2;  // 1-based as in the reference
+  while (true) {
+    if (x < xo[l - 1]) break;
+    l++;
+    if (l > n) break;
+  }
+  auto X = [xo](int k) { return xo[k - 1]; };
+  auto F = [fo](int k) { return fo[k - 1]; };
+  double a, b, c;
+  if (l > n || l == 2 || l == 3) {
+    l = l > n ? n : l;
+    c = 0.0;
+    b = (F(l) - F(l - 1)) / (X(l) - X(l - 1));
+    a = F(l) - X(l) * b;
+  } else {
+    const int l1 = l - 1, l2 = l - 2;
+    double d = (F(l1) - F(l2)) / (X(l1) - X(l2));
+    const double cbac = F(l) / ((X(l) - X(l1)) * (X(l) - X(l2))) +
+                        (F(l2) / (X(l) - X(l2)) - F(l1) / (X(l) - X(l1))) / (X(l1) - X(l2));
+    const double bbac = d - (X(l1) + X(l2)) * cbac;
+    const double abac = F(l2) - X(l2) * d + X(l1) * X(l2) * cbac;
+    if (l == n) {
+      a = abac;
+      b = bbac;
+      c = cbac;
+    } else {
+      d = (F(l) - F(l1)) / (X(l) - X(l1));
+      const double cfor = F(l + 1) / ((X(l + 1) - X(l)) * (X(l + 1) - X(l1))) +
+                          (F(l1) / (X(l + 1) - X(l1)) - F(l) / (X(l + 1) - X(l))) / (X(l) - X(l1));
+      const double bfor = d - (X(l) + X(l1)) * cfor;
+      const double afor = F(l1) - X(l1) * d + X(l) * X(l1) * cfor;
+      double wt = 0.0;
+      if (fabs(cfor) != 0.0) wt = fabs(cfor) / (fabs(cfor) + fabs(cbac));
+      a = afor + wt * (abac - afor);
+      b = bfor + wt * (bbac - bfor);
+      c = cfor + wt * (cbac - cfor);
+    }
+  }
+  return a + (b + c * x) * x;
+}
+
+__global__ void __launch_bounds__(256) k_contop(int nwl, int ntau, const double *__restrict__ wave, double wfirst,
+                                                const double *__restrict__ dep, const double *__restrict__ tab,
+                                                const double *__restrict__ cstd, int motype, unsigned mask, int add,
+                                                double *__restrict__ contT) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nwl) return;
+  const double *T = dep, *Hkt = dep + ntau, *Tkev = dep + 2 * ntau, *Theta = dep + 3 * ntau;
+  const double *Xne = dep + 4 * ntau, *Rho = dep + 5 * ntau;
+  const double *Fh1 = dep + 6 * ntau, *Fh2 = dep + 7 * ntau, *Fhmin = dep + 8 * ntau;
+  const double *Fhe1 = dep + 9 * ntau, *Fhmol = dep + 10 * ntau;
+  const double *hmwlog = tab, *hmfflog = tab + CONT_HM_NW;  // [22], [22][11]
+  const double *thetaff = tab + CONT_HM_NW * (1 + CONT_HM_NT);  // [11]
+  const double *wbf = thetaff + CONT_HM_NT, *bf = wbf + CONT_HMBF_N;  // [85], [85]
+
+  const double clight = kVc * 1e13;
+  const double Freq = clight / air2vac(wave[p] + wfirst);
+  const double Freqlg = log(Freq);
+  const bool on_h2p = (mask >> 1) & 1, on_hmin = (mask >> 2) & 1, on_hray = (mask >> 3) & 1;
+  const bool on_hemi = (mask >> 6) & 1, on_hera = (mask >> 7) & 1, on_el = (mask >> 11) & 1;
+  const bool on_h2ra = (mask >> 12) & 1;
+
+  // H2PLOP
+  const bool h2p = on_h2p && !(Freq > 3.28805e15);
+  double Fr = 0.0, Es = 0.0;
+  if (h2p) {
+    Fr = -3.0233e3 + (3.7797e2 + (-1.82496e1 + (3.9207e-1 - 3.1672e-3 * Freqlg) * Freqlg) * Freqlg) * Freqlg;
+    const double F15 = Freq * 1.e-15;
+    Es = -7.342e-3 + (-2.409 + (1.028 + (-0.4230 + (0.1224 - 0.01351 * F15) * F15) * F15) * F15) * F15;
+  }
+  // HMINOP: the free-free table at this wavelength for each tabulated temperature, and the
+  // bound-free cross-section
+  double fftt[CONT_HM_NT];
+  double hminbf = 0.0;
+  if (on_hmin) {
+    const double wl = clight / Freq * 0.1;
+    const double wlog = log(wl);
+    for (int it = 0; it < CONT_HM_NT; it++)
+      fftt[it] = exp(linter1(hmwlog, hmfflog + it, CONT_HM_NT, CONT_HM_NW, wlog)) / thetaff[it] * 5040.0 * 1.380658e-16;
+    if (Freq > 1.82365e14) hminbf = map1_one(wbf, bf, CONT_HMBF_N, wl);
+    hminbf = fmax(hminbf, 0.0);
+  }
+  // HRAYOP, HEMIOP, HERAOP, H2RAOP: their wavelength factors
+  double sig_h = 0.0, sig_he = 0.0, sig_h2 = 0.0, A = 0.0, B = 0.0, C = 0.0;
+  if (on_hray) {
+    const double w = clight / fmin(Freq, 2.463e15), w2 = w * w;
+    sig_h = (5.799e-13 + 1.422e-6 / w2 + 2.784 / (w2 * w2)) / (w2 * w2);
+  }
+  if (on_hemi) {
+    A = 3.397e-26 + (-5.216e-11 + 7.039e05 / Freq) / Freq;
+    B = -4.116e-22 + (1.067e-06 + 8.135e09 / Freq) / Freq;
+    C = 5.081e-17 + (-8.724e-03 - 5.659e12 / Freq) / Freq;
+  }
+  if (on_hera) {
+    const double w = kVc / 1e2 / fmin(Freq * 1.0e-15, 5.15), w2 = w * w;
+    const double s1 = 1.0 + (2.44e5 + 5.94e10 / (w2 - 2.90e5)) / w2;
+    sig_he = 5.484e-14 / w2 / w2 * s1 * s1;
+  }
+  if (on_h2ra) {
+    const double w = clight / fmin(Freq, 2.922e15), w2 = w * w;
+    sig_h2 = (8.14e-13 + 1.28e-6 / w2 + 1.61 / (w2 * w2)) / (w2 * w2);
+  }
+
+  for (int k = 0; k < ntau; k++) {
+    const double ehvkt = exp(-Freq * Hkt[k]);
+    const double stim = 1.0 - ehvkt;
+    double ah2p = 0.0, ahmin = 0.0, ahemin = 0.0, sh = 0.0, she = 0.0, sel = 0.0, sh2 = 0.0;
+    if (h2p) ah2p = exp(-Es / Tkev[k] + Fr) * Fh1[k] * 2.0 * Fh2[k] / Rho[k] * stim;
+    if (on_hmin) {
+      const double fft = linter1(thetaff, fftt, 1, CONT_HM_NT, Theta[k]);
+      const double hminff = fft * Fh1[k] * 2.0 * Xne[k] / Rho[k];
+      const double h = hminbf * 1.e-18 * (1.0 - ehvkt) * Fhmin[k] / Rho[k];
+      ahmin = h + hminff;
+    }
+    if (on_hemi) ahemin = (A * T[k] + B + C / T[k]) * Xne[k] * Fhe1[k] / Rho[k] * 1.0e-20;
+    if (on_hray) sh = sig_h * Fh1[k] * 2.0 / Rho[k];
+    if (on_hera) she = sig_he * Fhe1[k] / Rho[k];
+    if (on_el) sel = 0.6653e-24 * Xne[k] / Rho[k];
+    if (on_h2ra) sh2 = sig_h2 * Fhmol[k] / Rho[k];
+    // Xcont's sum (the absorption sources, then the scattering ones)
+    double copac = ah2p + ahmin + ahemin;
+    copac = copac + sh + she + sel + sh2;
+    if (motype == 1) copac = copac / cstd[k];
+    const size_t o = (size_t)k * nwl + p;
+    contT[o] = add ? contT[o] + copac : copac;
+  }
+}
+
+void launch_contop(int nwl, int ntau, const double *wave, double wfirst, const double *dep, const double *tab,
+                   const double *cstd, int motype, unsigned mask, int add, double *contT, hipStream_t st) {
+  k_contop<<<(nwl + 255) / 256, 256, 0, st>>>(nwl, ntau, wave, wfirst, dep, tab, cstd, motype, mask, add, contT);
+}
+
+// ------------------------------------------------------------------------------------
+// host side: HMINOP's tables (src/contop.f90:183-208 and 1133-1162; Bell & Berrington 1987
+// free-free absorption, 1e-26 cm^4/dyn per H atom and unit electron pressure, at 22
+// wavelengths [um] x 11 temperatures theta = 5040/T; Wishart 1979 bound-free cross-section,
+// 1e-18 cm^2, at 85 wavelengths [nm]) and the logarithms ContopInit takes of them (host
+// libm, as the reference's host computes them)
+// ------------------------------------------------------------------------------------
+namespace {
+const double kHmWavek[22] = {
+    0.5, 0.4, 0.35, 0.3, 0.25, 0.2, 0.18, 0.16,
+    0.14, 0.12, 0.1, 0.09, 0.08, 0.07, 0.06, 0.05,
+    0.04, 0.03, 0.02, 0.01, 0.008, 0.006,
+};
+const double kHmFf[242] = {
+    0.0178, 0.0222, 0.0308, 0.0402, 0.0498, 0.0596, 0.0695, 0.0795, 0.0896, 0.131, 0.172,
+    0.0228, 0.028, 0.0388, 0.0499, 0.0614, 0.0732, 0.0851, 0.0972, 0.11, 0.16, 0.211,
+    0.0277, 0.0342, 0.0476, 0.0615, 0.076, 0.0908, 0.105, 0.121, 0.136, 0.199, 0.262,
+    0.0364, 0.0447, 0.0616, 0.0789, 0.0966, 0.114, 0.132, 0.15, 0.169, 0.243, 0.318,
+    0.052, 0.0633, 0.0859, 0.108, 0.131, 0.154, 0.178, 0.201, 0.225, 0.321, 0.418,
+    0.0791, 0.0959, 0.129, 0.161, 0.194, 0.227, 0.26, 0.293, 0.327, 0.463, 0.602,
+    0.0965, 0.117, 0.157, 0.195, 0.234, 0.272, 0.311, 0.351, 0.39, 0.549, 0.711,
+    0.121, 0.146, 0.195, 0.241, 0.288, 0.334, 0.381, 0.428, 0.475, 0.667, 0.861,
+    0.154, 0.188, 0.249, 0.309, 0.367, 0.424, 0.482, 0.539, 0.597, 0.83, 1.07,
+    0.208, 0.25, 0.332, 0.409, 0.484, 0.557, 0.63, 0.702, 0.774, 1.06, 1.36,
+    0.293, 0.354, 0.468, 0.576, 0.677, 0.777, 0.874, 0.969, 1.06, 1.45, 1.83,
+    0.358, 0.432, 0.572, 0.702, 0.825, 0.943, 1.06, 1.17, 1.28, 1.73, 2.17,
+    0.448, 0.539, 0.711, 0.871, 1.02, 1.16, 1.29, 1.43, 1.57, 2.09, 2.6,
+    0.579, 0.699, 0.924, 1.13, 1.33, 1.51, 1.69, 1.86, 2.02, 2.67, 3.31,
+    0.781, 0.94, 1.24, 1.52, 1.78, 2.02, 2.26, 2.48, 2.69, 3.52, 4.31,
+    1.11, 1.34, 1.77, 2.17, 2.53, 2.87, 3.2, 3.51, 3.8, 4.92, 5.97,
+    1.73, 2.08, 2.74, 3.37, 3.9, 4.5, 5.01, 5.5, 5.95, 7.59, 9.06,
+    3.04, 3.65, 4.8, 5.86, 6.86, 7.79, 8.67, 9.5, 10.3, 13.2, 15.6,
+    6.79, 8.16, 10.7, 13.1, 15.3, 17.4, 19.4, 21.2, 23.0, 29.5, 35.0,
+    27.0, 32.4, 42.6, 51.9, 60.7, 68.9, 76.8, 84.2, 91.4, 117.0, 140.0,
+    42.3, 50.6, 66.4, 80.8, 94.5, 107.0, 120.0, 131.0, 142.0, 183.0, 219.0,
+    75.1, 90.0, 118.0, 144.0, 168.0, 191.0, 212.0, 234.0, 253.0, 325.0, 388.0,
+};
+const double kHmTheta[11] = {
+    0.5, 0.6, 0.8, 1.0, 1.2, 1.4, 1.6, 1.8,
+    2.0, 2.8, 3.6,
+};
+const double kHmBfWave[85] = {
+    18.0, 19.6, 21.4, 23.6, 26.4, 29.8, 34.3, 40.4,
+    49.1, 62.6, 111.3, 112.1, 112.67, 112.95, 113.05, 113.1,
+    113.2, 113.23, 113.5, 114.4, 121.0, 139.0, 164.0, 175.0,
+    200.0, 225.0, 250.0, 275.0, 300.0, 325.0, 350.0, 375.0,
+    400.0, 425.0, 450.0, 475.0, 500.0, 525.0, 550.0, 575.0,
+    600.0, 625.0, 650.0, 675.0, 700.0, 725.0, 750.0, 775.0,
+    800.0, 825.0, 850.0, 875.0, 900.0, 925.0, 950.0, 975.0,
+    1000.0, 1025.0, 1050.0, 1075.0, 1100.0, 1125.0, 1150.0, 1175.0,
+    1200.0, 1225.0, 1250.0, 1275.0, 1300.0, 1325.0, 1350.0, 1375.0,
+    1400.0, 1425.0, 1450.0, 1475.0, 1500.0, 1525.0, 1550.0, 1575.0,
+    1600.0, 1610.0, 1620.0, 1630.0, 1643.91,
+};
+const double kHmBfSigma[85] = {
+    0.067, 0.088, 0.117, 0.155, 0.206, 0.283, 0.414, 0.703,
+    1.24, 2.33, 11.6, 13.9, 24.3, 66.7, 95.0, 56.6,
+    20.0, 14.6, 8.5, 7.1, 5.43, 5.91, 7.29, 7.918,
+    9.453, 11.08, 12.75, 14.46, 16.19, 17.92, 19.65, 21.35,
+    23.02, 24.65, 26.24, 27.77, 29.23, 30.62, 31.94, 33.17,
+    34.32, 35.37, 36.32, 37.17, 37.91, 38.54, 39.07, 39.48,
+    39.77, 39.95, 40.01, 39.95, 39.77, 39.48, 39.06, 38.53,
+    37.89, 37.13, 36.25, 35.28, 34.19, 33.01, 31.72, 30.34,
+    28.87, 27.33, 25.71, 24.02, 22.26, 20.46, 18.62, 16.74,
+    14.85, 12.95, 11.07, 9.211, 7.407, 5.677, 4.052, 2.575,
+    1.302, 0.8697, 0.4974, 0.1989, 0.0,
+};
+}  // namespace
+
+void contop_tables(std::vector<double> &tab) {
+  tab.assign((size_t)CONT_HM_NW * (1 + CONT_HM_NT) + CONT_HM_NT + 2 * CONT_HMBF_N, 0.0);
+  double *w = tab.data(), *ff = w + CONT_HM_NW, *th = ff + CONT_HM_NW * CONT_HM_NT;
+  double *wb = th + CONT_HM_NT, *sb = wb + CONT_HMBF_N;
+  for (int i = 0; i < CONT_HM_NW; i++) w[i] = std::log(91.134 / kHmWavek[i]);
+  for (int i = 0; i < CONT_HM_NW * CONT_HM_NT; i++) ff[i] = std::log(kHmFf[i] * 1.e-26);
+  for (int i = 0; i < CONT_HM_NT; i++) th[i] = kHmTheta[i];
+  for (int i = 0; i < CONT_HMBF_N; i++) {
+    wb[i] = kHmBfWave[i];
+    sb[i] = kHmBfSigma[i];
+  }
+}
+
+}  // namespace fsg

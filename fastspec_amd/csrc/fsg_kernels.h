@@ -113,6 +113,16 @@ void convol_gauss_kernel(int nw, double fwhm, int wmode, double wfirst, double w
 void diskinteg_kernels(const ConvolSetup &cs, std::vector<double> &kern, std::vector<int> &rot_off,
                        std::vector<int> &rot_n, std::vector<int> &mac_off, int &mac_n);
 void launch_conv_rows(int nwl, int nrows, int maxk, const ConvRow *d_rows, const double *d_kern, hipStream_t st);
+
+// continuum opacity (fsg_contop.hip): the H- free-free table is CONT_HM_NW wavelengths x
+// CONT_HM_NT temperatures, the H- bound-free cross-section CONT_HMBF_N points; tab holds
+// ln(91.134 / lambda_k) [NW], ln(K_ff) [NW][NT], theta [NT], lambda_bf [N], sigma_bf [N];
+// dep holds T, hc/kT, kT [eV], 5040/T, Xne, Rho, FractionPF(H I), FractionPF(H II),
+// Fraction(H-), FractionPF(He I), Fraction(H2) [11][ntau]; mask bit s-1 switches source s
+constexpr int CONT_HM_NW = 22, CONT_HM_NT = 11, CONT_HMBF_N = 85;
+void launch_contop(int nwl, int ntau, const double *wave, double wfirst, const double *dep, const double *tab,
+                   const double *cstd, int motype, unsigned mask, int add, double *contT, hipStream_t st);
+void contop_tables(std::vector<double> &tab);  // host: the table block of launch_contop
 void launch_mu_sum(int nwl, int nmu, const double *rl, const double *rc, const double *w, double *sfl, double *sfc,
                    hipStream_t st);
 

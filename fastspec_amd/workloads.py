@@ -226,8 +226,23 @@ def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode:
         spc_slot=np.concatenate([np.arange(1, nsp + 1), nsp + 5 + np.arange(1, nmol + 1)]).astype(np.int32),
         he1_slot=nsp + 2, xmu=xmu, wmu=wmu, motype=motype, eps_opac=eps_opac, vmicro=vmicro,
         radius=radius, height=height,
-        meta=dict(atmosphere=atm, species=[s[0] for s in species] + [m[0] for m in molecules]),
+        meta=dict(atmosphere=atm, species=[s[0] for s in species] + [m[0] for m in molecules], n_atomic=nsp),
     )
+
+
+def continuum_populations(setup: Setup) -> dict:
+    """The EOS rows the device's continuum sources read, for a set-up made by build_setup:
+    FractionPF of H I, H II and He I (their slots), Fraction of H2, and Fraction of H- from
+    the Saha equation against H I (binding 0.754 eV, statistical weights 1 and 2)."""
+    nsp = int(setup.meta["n_atomic"])  # H I, He I, H2, H II, He II follow the atomic slots
+    ev_k = EV / KB
+    t = setup.temp
+    n_h1 = setup.frac_h1.astype(np.float64)
+    c_saha = 4.1412e-16  # (h^2 / (2 pi m_e k))^(3/2) [cm^3 K^(3/2)]
+    n_hmin = n_h1 * setup.xne * c_saha * t**-1.5 / 4.0 * np.exp(0.754 * ev_k / t)
+    return dict(fpf_h1=setup.fraction_pf[:, nsp].copy(), fpf_h2=setup.fraction_pf[:, nsp + 3].copy(),
+                frac_hmin=n_hmin.astype(np.float32), fpf_he1=setup.fraction_pf[:, nsp + 1].copy(),
+                frac_hmol=setup.frac_hmol.copy())
 
 
 def _f32pow10(x):

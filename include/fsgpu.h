@@ -327,6 +327,28 @@ typedef struct fsg_atmosphere {
 } fsg_atmosphere;
 int fsg_set_atmosphere(fsg_ctx *ctx, int32_t atm, const fsg_atmosphere *a);
 
+/* Continuum opacity on the device: ContopCalc's loop over the grid (src/synth.f90:663-669,
+ * Xcont at Air2Vac of each wavelength, divided by ContOpacStd for Motype 1) for the sources
+ * of Xcont (src/contop.f90:267-314) that need only hydrogen, helium and electron
+ * populations.  ifop_mask bit s-1 switches source s of the reference's Ifop numbering; the
+ * device evaluates 2 H2PLOP, 3 HMINOP, 4 HRAYOP, 7 HEMIOP, 8 HERAOP, 12 ELECOP and 13 H2RAOP
+ * (other bits are refused: the host's Xcont keeps HOP, HE1OP, HE2OP, COOLOP, LUKEOP and
+ * HOTOP).  T, Xne and Rho are the context's (fsg_init / fsg_set_atmosphere); the populations
+ * are the EOS rows the sources read, as ContopSetup indexes them (src/synth.f90:608-628).
+ * add = 0 replaces the context's ContOpac with the device sources' sum; add = 1 adds the sum
+ * to it (the host's part, from Xcont with those Ifop flags off, given at fsg_init or
+ * fsg_set_atmosphere).  contop_out [nwl][ntau], if non-NULL, receives the resulting
+ * ContOpac.  Windows and formal solutions of earlier calls are dropped.  Ordinary contexts
+ * only. */
+typedef struct fsg_contop_eos {
+  const float *fpf_h1;     /* [ntau] FractionPF(IxH1, :)   H I  */
+  const float *fpf_h2;     /* [ntau] FractionPF(IxH2, :)   H II */
+  const float *frac_hmin;  /* [ntau] Fraction(IxHmin, :)   H-   */
+  const float *fpf_he1;    /* [ntau] FractionPF(IxHe1, :)  He I */
+  const float *frac_hmol;  /* [ntau] Fraction(IxHmol, :)   H2   */
+} fsg_contop_eos;
+int fsg_contop(fsg_ctx *ctx, const fsg_contop_eos *eos, int32_t ifop_mask, int32_t add, double *contop_out);
+
 /* Batch contexts: many model atmospheres synthesised in one launch sequence (the
  * differentiable batch of many small syntheses).  No counterpart in the reference, which
  * synthesises one atmosphere per run (src/fastspec.f90:241-461); per atmosphere the stages

@@ -500,3 +500,41 @@ def ref_convolgau(setup: Setup, s, fwhm: float):
     lib.ref_convolgau(out.shape[0], ptr(out, C.c_double), fwhm, setup.wmode, setup.wfirst, setup.wfirst + wave[-1],
                       setup.dstep)
     return out
+
+
+def ref_xcont(temp, xne, rho, pops: dict, sources, wl_vac, xna=None) -> np.ndarray:
+    """The reference's Xcont (src/contop.f90:267-314), compiled from its sources, at vacuum
+    wavelengths wl_vac with only the Ifop sources given switched on: Copac [n][ntau].  pops
+    holds the EOS rows (fpf_h1, fpf_h2, frac_hmin, fpf_he1, frac_hmol) that the harness puts
+    in its slots 1 H I, 2 H II, 3 H-, 4 He I, 5 H2 (He II, He III and the metals empty)."""
+    lib = ref_lib()
+    n = int(len(wl_vac))
+    ntau = int(len(temp))
+    fpf = np.zeros((ntau, 8), np.float32)
+    frac = np.zeros((ntau, 8), np.float32)
+    fpf[:, 0] = pops["fpf_h1"]
+    fpf[:, 1] = pops["fpf_h2"]
+    frac[:, 2] = pops["frac_hmin"]
+    fpf[:, 3] = pops["fpf_he1"]
+    frac[:, 4] = pops["frac_hmol"]
+    # Fraction and FractionPF of the other rows as the harness reads them: H I, H II and He I
+    # with their FractionPF, H- and H2 with their Fraction
+    frac[:, 0], frac[:, 1], frac[:, 3] = fpf[:, 0], fpf[:, 1], fpf[:, 3]
+    fpf[:, 2], fpf[:, 4] = frac[:, 2], frac[:, 4]
+    ifop = np.zeros(20, np.int32)
+    for s in sources:
+        ifop[int(s) - 1] = 1
+    out = np.zeros((n, ntau))
+    ar = lambda a: np.ascontiguousarray(a, np.float64)
+    t, ne, r = ar(temp), ar(xne), ar(rho)
+    na = ar(np.zeros(ntau) if xna is None else xna)
+    w = ar(wl_vac)
+    lib.ref_xcont(C.c_int(ntau), ptr(t, C.c_double), ptr(ne, C.c_double), ptr(na, C.c_double), ptr(r, C.c_double),
+                  ptr(np.ascontiguousarray(fpf), C.c_float), ptr(np.ascontiguousarray(frac), C.c_float),
+                  ptr(ifop, C.c_int32), C.c_int(n), ptr(w, C.c_double), ptr(out, C.c_double))
+    return out
+
+
+def ref_air2vac_array(wl) -> np.ndarray:
+    lib = ref_lib()
+    return np.array([lib.ref_air2vac(float(x)) for x in wl])

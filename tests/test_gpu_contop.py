@@ -1,0 +1,117 @@
+"""Continuum opacity on the device (fsg_contop) against the reference's own Xcont, compiled
+from its sources (oracle/_ref, src/contop.f90:267-314), on the configurations' grids: the
+plain configuration (5000-5100 A, solar), a slice of the cool-dwarf log grid (3200 K), the
+H- bound-free threshold in the infrared and the near-UV of a hot model.  Bound: 1e-12
+relative per (wavelength, depth), every source alone and all together; add mode, Motype 1
+and the refusal of sources the host keeps; and a synthesis that uses the device continuum
+against the oracle's with the reference continuum."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from fastspec_amd import workloads as W
+from fastspec_amd.abi import FSG_FP32, FSG_FP64
+from oracle_bridge import Oracle, ref_air2vac_array, ref_xcont
+
+pytestmark = [pytest.mark.gpu, pytest.mark.ref]
+
+SOURCES = (2, 3, 4, 7, 8, 12, 13)
+REL = 1e-12
+# H2PLOP's exponent is a quartic in ln(frequency) whose terms, ~1e4, cancel to ~-90: one ulp
+# of the logarithm (device and host libm may differ by one) moves H2+ opacity by ~2e-12
+# relative; it is a minor source, and the sums stay within REL
+REL_H2PLOP = 1e-11
+
+
+def _grids():
+    c1, _ = W.config_c1(10)
+    c3, _ = W.config_c3_setup(10000.0, 10020.0)
+    ir = W.build_setup(W.grey_atmosphere(4500.0, 4.5), 16300.0, 16600.0, 0.05, 0)
+    uv = W.build_setup(W.grey_atmosphere(8000.0, 4.0), 3000.0, 3100.0, 0.05, 0)
+    return dict(c1=c1, c3=c3, ir_edge=ir, uv=uv)
+
+
+GRIDS = _grids()
+
+
+def _ref(setup, pops, sources):
+    return ref_xcont(setup.temp, setup.xne, setup.rho, pops, sources, ref_air2vac_array(setup.wavelengths))
+
+
+def _gpu(setup, pops, sources, **kw):
+    from fastspec_amd.synth import FastSpecGPU
+    with FastSpecGPU(setup, precision=FSG_FP64) as g:
+        return g.ContopCalc(**pops, sources=sources, **kw)
+
+
+@pytest.mark.parametrize("name", sorted(GRIDS))
+def test_all_device_sources_match_reference(name):
+    s = GRIDS[name]
+    pops = W.continuum_populations(s)
+    got = _gpu(s, pops, SOURCES)
+    ref = _ref(s, pops, SOURCES)
+    err = float(np.max(np.abs(got - ref) / ref))
+    print(f"\nPARITY contop config={name} nwl={s.nwl} rel={err:.3e}", flush=True)
+    assert err <= REL
+
+
+@pytest.mark.parametrize("src", SOURCES)
+def test_each_source_matches_reference(src):
+    s = GRIDS["c1"]
+    pops = W.continuum_populations(s)
+    got = _gpu(s, pops, (src,))
+    ref = _ref(s, pops, (src,))
+    assert np.max(ref) > 0.0
+    assert np.all((got == 0.0) == (ref == 0.0))
+    nz = ref > 0.0
+    err = float(np.max(np.abs(got[nz] - ref[nz]) / ref[nz]))
+    print(f"\nPARITY contop source={src} rel={err:.3e}", flush=True)
+    assert err <= (REL_H2PLOP if src == 2 else REL)
+
+
+def test_add_mode_and_motype1():
+    s = GRIDS["c1"]
+    pops = W.continuum_populations(s)
+    ref = _ref(s, pops, SOURCES)
+    # the host's part (here: the workload's own continuum) plus the device's
+    got = _gpu(s, pops, SOURCES, add=True)
+    assert np.max(np.abs(got - (s.contop + ref)) / (s.contop + ref)) <= REL
+    # Motype 1: ContOpac in units of ContOpacStd
+    m1 = W.build_setup(W.grey_atmosphere(5777.0, 4.44), 5000.0, 5100.0, 0.01, 0, motype=1)
+    got1 = _gpu(m1, pops, SOURCES)
+    ref1 = _ref(m1, pops, SOURCES) / m1.contop_std[None, :]
+    assert np.max(np.abs(got1 - ref1) / ref1) <= REL
+
+
+def test_host_sources_refused():
+    from fastspec_amd._lib import FsgError
+    s = GRIDS["c1"]
+    pops = W.continuum_populations(s)
+    for src in (1, 5, 6, 9, 10, 11):
+        with pytest.raises(FsgError, match="not evaluated on the device"):
+            _gpu(s, pops, SOURCES + (src,))
+    with pytest.raises(FsgError, match="population"):
+        _gpu(s, dict(pops, frac_hmin=None), (3,))
+
+
+@pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])
+def test_synthesis_with_device_continuum(prec):
+    """ContopCalc then LineopCalc and the formal solution: the spectrum of the oracle run on
+    the reference's continuum."""
+    from fastspec_amd.synth import FastSpecGPU
+    setup, lines = W.config_c1(3000)
+    pops = W.continuum_populations(setup)
+    ref_cont = _ref(setup, pops, SOURCES)
+    orc = Oracle(dataclasses.replace(setup, contop=ref_cont))
+    orc.lineop(lines)
+    orc.radtran()
+    sfl, sfc = orc.flux()
+    orc.close()
+    with FastSpecGPU(setup, precision=prec) as g:
+        g.ContopCalc(**pops)
+        g.LineopCalc(lines)
+        g.RadTranCalc(intensities=False)
+        gl, gc = g.flux()
+    assert np.max(np.abs(gc - sfc) / sfc) <= 1e-9
+    assert np.max(np.abs(gl / gc - sfl / sfc)) <= (1e-8 if prec == FSG_FP64 else 1e-4)

@@ -1,0 +1,67 @@
+"""The harness around the reference's continuum opacity (oracle/ref_contop.f90: Xcont and
+ContopInit of src/contop.f90 compiled from the reference's sources), pinned by properties
+that a wrong slot, switch or unit in the harness would break:
+
+* Thomson scattering alone is 0.6653e-24 Xne / Rho at every wavelength;
+* each source is switched on by its own Ifop flag, and the sum of single-source runs is the
+  run with all of them;
+* H- free-free is proportional to Xne (bit for bit, the factor being 2), and H- bound-free
+  to the H- density below its threshold at 1.64 um and absent above it;
+* Rayleigh scattering by H I falls as lambda^-4 at long wavelengths.
+"""
+import numpy as np
+import pytest
+
+from fastspec_amd import workloads as W
+from oracle_bridge import ref_xcont
+
+pytestmark = pytest.mark.ref
+
+DEVICE_SOURCES = (2, 3, 4, 7, 8, 12, 13)
+
+
+@pytest.fixture(scope="module")
+def atm():
+    setup = W.build_setup(W.grey_atmosphere(5777.0, 4.44), 5000.0, 5001.0, 0.5, 0)
+    return setup, W.continuum_populations(setup)
+
+
+def test_thomson_alone(atm):
+    s, pops = atm
+    wl = np.array([3000.0, 5000.0, 20000.0])
+    out = ref_xcont(s.temp, s.xne, s.rho, pops, (12,), wl)
+    for row in out:
+        assert np.array_equal(row, 0.6653e-24 * s.xne / s.rho)
+
+
+def test_sources_are_separate_and_additive(atm):
+    s, pops = atm
+    wl = np.linspace(3200.0, 17000.0, 23)
+    parts = [ref_xcont(s.temp, s.xne, s.rho, pops, (src,), wl) for src in DEVICE_SOURCES]
+    for p in parts:
+        assert np.all(p >= 0.0) and np.max(p) > 0.0
+    total = ref_xcont(s.temp, s.xne, s.rho, pops, DEVICE_SOURCES, wl)
+    assert np.max(np.abs(sum(parts) - total) / total) <= 1e-14
+    # H- dominates the optical continuum of a solar model
+    hmin = parts[DEVICE_SOURCES.index(3)]
+    assert np.median(hmin[wl < 16000.0] / total[wl < 16000.0]) > 0.5
+
+
+def test_hminus_free_free_and_bound_free(atm):
+    s, pops = atm
+    wl = np.array([8000.0, 15000.0, 17000.0, 20000.0])
+    no_bf = dict(pops, frac_hmin=np.zeros_like(pops["frac_hmin"]))
+    ff1 = ref_xcont(s.temp, s.xne, s.rho, no_bf, (3,), wl)
+    ff2 = ref_xcont(s.temp, 2.0 * s.xne, s.rho, no_bf, (3,), wl)
+    assert np.array_equal(ff2, 2.0 * ff1)
+    one = ref_xcont(s.temp, s.xne, s.rho, pops, (3,), wl)
+    two = ref_xcont(s.temp, s.xne, s.rho, dict(pops, frac_hmin=2.0 * pops["frac_hmin"]), (3,), wl)
+    bf = two - one
+    assert np.all(bf[:2] > 0.0) and np.allclose(bf[:2], one[:2] - ff1[:2], rtol=1e-12)
+    assert np.array_equal(one[2:], ff1[2:])  # beyond the 1.6439 um threshold: free-free only
+
+
+def test_hydrogen_rayleigh_lambda4(atm):
+    s, pops = atm
+    sig = ref_xcont(s.temp, s.xne, s.rho, pops, (4,), np.array([40000.0, 80000.0]))
+    assert np.allclose(sig[0] / sig[1], 16.0, rtol=2e-3)
