@@ -475,6 +475,53 @@ def run_c3(steps: int, fp64: bool):
     return out
 
 
+def run_contop(setup):
+    """fsg_contop over the C2 grid: the seven continuum sources of Xcont the device evaluates
+    (H2+, H-, He-, Rayleigh by H I, He I and H2, Thomson), one lane per wavelength, 72 depths,
+    FP64, ContOpac left on the device.  The reference's Xcont (oracle/_ref, -O3, one thread:
+    its harness loops serially; ContopCalc is an OpenMP loop over wavelengths) is timed on
+    20,000 of the grid's wavelengths, for these sources and for all thirteen."""
+    from fastspec_amd import workloads as W
+    from fastspec_amd.abi import FSG_FP64
+    from fastspec_amd.synth import FastSpecGPU
+
+    pops = W.continuum_populations(setup)
+    g = FastSpecGPU(setup, precision=FSG_FP64)
+    g.ContopCalc(**pops, copy_out=False)
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.ContopCalc(**pops, copy_out=False)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    g.close()
+    n, nt = setup.nwl, setup.ntau
+    bytes_alg = n * (8 * nt + 8)  # ContOpac written, the grid read
+    res = {"ms": ms, "nwl": n, "ntau": nt, "sources": [2, 3, 4, 7, 8, 12, 13],
+           "points_per_s": n * nt / (ms * 1e-3),
+           "hbm_gbs_algorithmic": bytes_alg / (ms * 1e-3) / 1e9,
+           "note": "wall clock of fsg_contop (host set-up of the depth state and tables, one launch, "
+                   "synchronised); ContOpac stays in HBM"}
+    try:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_bridge
+        o3 = ROOT / "oracle" / "_ref" / "o3" / "libfsref.so"
+        if o3.exists():
+            oracle_bridge.REF_SO = o3
+            oracle_bridge._REF = None
+            wl = oracle_bridge.ref_air2vac_array(setup.wavelengths[::max(1, n // 20000)][:20000])
+            cpu = {}
+            for name, srcs in (("device_sources", (2, 3, 4, 7, 8, 12, 13)), ("all_13_sources", tuple(range(1, 14)))):
+                t1 = time.perf_counter()
+                oracle_bridge.ref_xcont(setup.temp, setup.xne, setup.rho, pops, srcs, wl)
+                us = (time.perf_counter() - t1) * 1e6 / len(wl)
+                cpu[name] = {"us_per_wavelength_1core": us, "full_grid_s_1core": us * n * 1e-6}
+            res["cpu_reference_xcont"] = cpu
+            res["speedup_vs_1core_device_sources"] = cpu["device_sources"]["full_grid_s_1core"] / (ms * 1e-3)
+    except Exception as e:  # the reference build is optional on the GPU host
+        res["cpu_reference_xcont"] = f"unavailable: {e}"
+    return res
+
+
 def host_cpu():
     """Core count and model of the host CPU."""
     model = ""
@@ -547,6 +594,7 @@ def main():
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--no-contop", action="store_true")
     ap.add_argument("--c3-fp64", action="store_true", help="also time the FP64 plain path on C3")
     ap.add_argument("--no-host", action="store_true", help="skip the host-to-host C2 timing")
     ap.add_argument("--only-c5", action="store_true", help="run only the differentiable batch (profiling)")
@@ -641,6 +689,10 @@ def main():
         log("C2 from host buffers to host buffers")
         out["c2_host_to_host"] = run_host_to_host(setup, lines, args.steps)
         log(json.dumps(out["c2_host_to_host"]))
+    if not args.no_contop:
+        log("continuum opacity on the C2 grid")
+        out["contop"] = run_contop(setup)
+        log(json.dumps(out["contop"]))
     if not args.no_c1:
         log("C1 plain configuration")
         out["c1_plain"] = run_c1(args.steps, args.warmup)

@@ -142,13 +142,14 @@ class FastSpecGPU:
     CONTOP_DEVICE_SOURCES = (2, 3, 4, 7, 8, 12, 13)  # H2PLOP HMINOP HRAYOP HEMIOP HERAOP ELECOP H2RAOP
 
     def ContopCalc(self, fpf_h1=None, fpf_h2=None, frac_hmin=None, fpf_he1=None, frac_hmol=None,
-                   sources=CONTOP_DEVICE_SOURCES, add: bool = False):
+                   sources=CONTOP_DEVICE_SOURCES, add: bool = False, copy_out: bool = True):
         """ContopCalc's loop (src/synth.f90:663-669) for the continuum sources the device
         evaluates (fsg_contop): sources are Ifop numbers among 2, 3, 4, 7, 8, 12, 13; the
         populations are the EOS rows [ntau] they read (FractionPF of H I, H II and He I,
         Fraction of H- and H2).  add=False replaces the context's ContOpac with their sum,
         add=True adds it to the ContOpac the context holds.  Returns the resulting ContOpac
-        [nwl][ntau], which the host copy of the set-up takes too."""
+        [nwl][ntau], which the host copy of the set-up takes too; copy_out=False leaves it on
+        the device only (returns None, and the host copy keeps its earlier ContOpac)."""
         import dataclasses
         from .abi import CContopEos
         n = self.setup.ntau
@@ -164,9 +165,10 @@ class FastSpecGPU:
         for src in sources:
             mask |= 1 << (int(src) - 1)
         eos = CContopEos(**{k: ptr(a, C.c_float) for k, a in arrs.items()})
-        out = np.zeros((self.setup.nwl, n))
+        out = np.zeros((self.setup.nwl, n)) if copy_out else None
         _lib.check(self._lib.fsg_contop(self.ctx, C.byref(eos), mask, int(bool(add)), ptr(out, C.c_double)))
-        self.setup = dataclasses.replace(self.setup, contop=out)
+        if copy_out:
+            self.setup = dataclasses.replace(self.setup, contop=out)
         return out
 
     contop_calc = ContopCalc
