@@ -598,6 +598,8 @@ def main():
     ap.add_argument("--c3-fp64", action="store_true", help="also time the FP64 plain path on C3")
     ap.add_argument("--no-host", action="store_true", help="skip the host-to-host C2 timing")
     ap.add_argument("--only-c5", action="store_true", help="run only the differentiable batch (profiling)")
+    ap.add_argument("--only", choices=["c1", "c3", "c4", "c5", "contop"],
+                    help="run one leg alone and print its result (profiling; not the benchmark line)")
     ap.add_argument("--c5-contexts", action="store_true",
                     help="differentiable batch with one context per atmosphere (the earlier design)")
     args = ap.parse_args()
@@ -611,8 +613,13 @@ def main():
     from fastspec_amd.abi import FSG_FP32, FSG_FP64
 
     if args.only_c5:  # the differentiable batch alone (for profiling); not the benchmark line
-        res = run_c5(args.steps, args.warmup, args.c5_contexts)
-        print(json.dumps({"c5_differentiable_batch": res}), flush=True)
+        args.only = "c5"
+    if args.only:  # one leg alone (for profiling); not the benchmark line
+        legs = {"c1": lambda: run_c1(args.steps, args.warmup), "c4": lambda: run_c4(args.steps, args.warmup),
+                "c3": lambda: run_c3(max(1, min(args.steps, 2)), args.c3_fp64),
+                "c5": lambda: run_c5(args.steps, args.warmup, args.c5_contexts),
+                "contop": lambda: run_contop(W.config_c2(nlines=10)[0])}
+        print(json.dumps({args.only: legs[args.only]()}), flush=True)
         return
 
     log("building the C2 workload")
