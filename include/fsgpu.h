@@ -228,10 +228,13 @@ int fsg_maglineop_dev(fsg_ctx *ctx, fsg_dlines *dl, int64_t nlines_lim, const do
  * gradient-based fitting, where gf changes between syntheses. */
 int fsg_lines_set_gf(fsg_dlines *dl, const float *gf, int32_t on_device);
 
-/* Line-list ingest from a sidecar directory: the /Lines group of a FastSpec HDF5 list as
- * raw little-endian column files <COLUMN>.bin (W f8, EL S<k>, ION i2, E, LGGF, GRAD, GQST,
- * GVDW f4; W, E, LGGF, GRAD for molecular lists) and a header.txt of key=value lines
- * (format=fastspec-lines-1, name, wl_vac, wmin, wmax, nl, hflag, columns=NAME:type,...).
+/* Line-list ingest.  dir names either a FastSpec HDF5 list file (recognised by its
+ * signature; its /Lines datasets are read through libhdf5, loaded at run time from
+ * FSG_HDF5_LIB or the usual names, the gzip+shuffle chunks inflated by the reader's threads)
+ * or a sidecar directory: the /Lines group as raw little-endian column files <COLUMN>.bin
+ * (W f8, EL S<k>, ION i2, E, LGGF, GRAD, GQST, GVDW f4; W, E, LGGF, GRAD for molecular
+ * lists) and a header.txt of key=value lines (format=fastspec-lines-1, name, wl_vac, wmin,
+ * wmax, nl, hflag, columns=NAME:type,...).  Both give the same lines bit for bit.
  * Replaces ReadHDF (src/ioutils.f90:545-1100) for atomic and molecular lists: whole-file
  * cull on WMIN/WMAX, Vac2Air when WL_VAC = 1, the sort check on the full W column, the
  * LowerBoundReal8/UpperBoundReal8 selection of [wstart, wend], the ion filter (ION <= 5)
