@@ -1056,10 +1056,12 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
 // lane and 64 points per wavefront such a line would keep a few lanes busy; here a workgroup
 // owns a tile of NARROW_TILE points and all depths, and each wavefront's lanes are
 // NARROW_TILE points x 8 depths (lane = 8 depth + point offset), so a window of a few points
-// fills most lanes.  Per-(line, depth) records come through vector loads that the lanes of
-// one depth share.  Every point of a window takes the general path of the core-zone kernel
-// (table in the box, one- or two-pole fraction outside: the reference's HVoigt in FP32), and
-// the lines of the tile are summed in the order of their bins with the same blocked Kahan
+// fills most lanes.  The per-(line, depth) records of eight candidates come through one
+// vector load (lane (depth, q) loads candidate q's record at its depth) and reach the lanes
+// of each depth by cross-lane permutes; the next eight are loaded while these are
+// evaluated.  Every point of a window takes the general path of the core-zone kernel (table
+// in the box, one- or two-pole fraction outside: the reference's HVoigt in FP32), and the
+// lines of the tile are summed in the order of their bins with the same blocked Kahan
 // compensation (groups of FOLD lines).
 __global__ void __launch_bounds__(1024) k_opacity32_narrow(DevSetup S, int ntiles, int nbins, const int2 *__restrict__ rng,
                                                            const CoefB *__restrict__ cb,
@@ -1070,6 +1072,7 @@ __global__ void __launch_bounds__(1024) k_opacity32_narrow(DevSetup S, int ntile
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int d = wave * 8 + (lane >> 3);
   const int dc = min(d, S.ntp - 1);
+  const int slot = lane & 7;  // the candidate (of eight) whose record this lane loads
   const int p0 = t * NARROW_TILE + 1, p1 = min(p0 + NARROW_TILE - 1, S.nwl);
   const int p = p0 + (lane & 7);
   const bool valid = p <= p1 && d < S.ntau;
@@ -1079,8 +1082,7 @@ __global__ void __launch_bounds__(1024) k_opacity32_narrow(DevSetup S, int ntile
   for (int b = 0; b < nbins; b++) {
     const int2 r = rng[(size_t)t * nbins + b];
     // candidates 64 at a time: each lane reads one candidate's window and centre, a ballot
-    // keeps those whose window meets the tile, and they are taken in candidate order with
-    // the next one's record loaded while the current one is evaluated
+    // keeps those whose window meets the tile, and they are taken in candidate order
     for (int base = r.x; base < r.y; base += 64) {
       const int jl = base + lane;
       int4 mm = make_int4(1, 0, 0, 0);
@@ -1089,32 +1091,67 @@ __global__ void __launch_bounds__(1024) k_opacity32_narrow(DevSetup S, int ntile
         mm = meta_s[jl];
         wl = wlc_s[jl];
       }
-      uint64_t todo = __builtin_amdgcn_ballot_w64(jl < r.y && mm.y >= p0 && mm.x <= p1);
-      if (!todo) continue;
-      int k = __builtin_ctzll(todo);
-      CoefB cn = cb[(size_t)(base + k) * S.ntp + dc];
-      while (true) {
-        const CoefB c = cn;
-        const int il = __builtin_amdgcn_readlane(mm.x, k), ir = __builtin_amdgcn_readlane(mm.y, k);
-        const double wck = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(wl), k),
-                                            __builtin_amdgcn_readlane(__double2loint(wl), k));
-        todo &= todo - 1;
-        const bool more = todo != 0;
-        const int kn = more ? __builtin_ctzll(todo) : k;
-        if (more) cn = cb[(size_t)(base + kn) * S.ntp + dc];
-        if (grp == FOLD) {
-          kahan_add(acc, cmp, part);
-          part = 0.f;
-          grp = 0;
+      uint64_t rest = __builtin_amdgcn_ballot_w64(jl < r.y && mm.y >= p0 && mm.x <= p1);
+      // eight candidates at a time: lane (depth, slot q) loads candidate q's record at its
+      // depth, so one load instruction brings eight candidates' records; the next group's
+      // loads are issued before the current group is evaluated
+      auto take8 = [](uint64_t &rs, int &n) {
+        uint64_t g = 0;
+        n = 0;
+        while (rs && n < 8) {
+          g |= rs & (~rs + 1);
+          rs &= rs - 1;
+          n++;
         }
-        grp++;
-        const bool act = valid && p >= il && p <= ir;
-        float v[1] = {(float)(Wp - wck) * c.vv}, out[1];
-        const bool a1[1] = {act};
-        contrib32_lane(S, c, v, a1, out);
-        part += out[0];
-        if (!more) break;
-        k = kn;
+        return g;
+      };
+      auto load_slot = [&](uint64_t g, int n) {
+        uint64_t u = g;
+        for (int q = 0; q < slot && q < n - 1; q++) u &= u - 1;
+        return cb[(size_t)(base + __builtin_ctzll(u)) * S.ntp + dc];
+      };
+      auto eval_group = [&](const CoefB &cur, uint64_t gq, int n) {
+        for (int q = 0; q < n; q++) {
+          const int k = __builtin_ctzll(gq);
+          gq &= gq - 1;
+          // candidate q's record at this lane's depth sits in lane (depth, q)
+          const int src = (lane & ~7) | q;
+          CoefB c;
+          c.vv = __shfl(cur.vv, src, 64);
+          c.a = __shfl(cur.a, src, 64);
+          c.a2 = __shfl(cur.a2, src, 64);
+          c.c1 = __shfl(cur.c1, src, 64);
+          c.ka = __shfl(cur.ka, src, 64);
+          c.cop = __shfl(cur.cop, src, 64);
+          c.ua = __shfl(cur.ua, src, 64);
+          c.ia = __shfl(cur.ia, src, 64);
+          const int il = __builtin_amdgcn_readlane(mm.x, k), ir = __builtin_amdgcn_readlane(mm.y, k);
+          const double wck = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(wl), k),
+                                              __builtin_amdgcn_readlane(__double2loint(wl), k));
+          if (grp == FOLD) {
+            kahan_add(acc, cmp, part);
+            part = 0.f;
+            grp = 0;
+          }
+          grp++;
+          const bool act = valid && p >= il && p <= ir;
+          float v[1] = {(float)(Wp - wck) * c.vv}, out[1];
+          const bool a1[1] = {act};
+          contrib32_lane(S, c, v, a1, out);
+          part += out[0];
+        }
+      };
+      int ng = 0;
+      uint64_t grpm = rest ? take8(rest, ng) : 0;
+      CoefB mine = grpm ? load_slot(grpm, ng) : CoefB{};
+      while (grpm) {
+        int ngn = 0;
+        const uint64_t grpn = rest ? take8(rest, ngn) : 0;
+        const CoefB cur = mine;
+        if (grpn) mine = load_slot(grpn, ngn);
+        eval_group(cur, grpm, ng);
+        grpm = grpn;
+        ng = ngn;
       }
     }
   }
