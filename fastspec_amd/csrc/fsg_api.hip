@@ -568,12 +568,19 @@ int fsg_init(const fsg_setup *s, fsg_ctx **ctx_out) {
   if (s->motype == 1) std::memcpy(cstd.data(), s->contop_std, sizeof(double) * N);
   rc |= upload(c, cstd.data(), N, &S.cstd, c->owned);
   // ContOpac(Ntau, Nwl) -> [ntau][nwl]: wavelength fastest for coalesced per-lambda lanes
-  {
-    std::vector<double> ct((size_t)N * Nw);
-    for (size_t iw = 0; iw < (size_t)Nw; iw++)
-      for (int k = 0; k < N; k++) ct[(size_t)k * Nw + iw] = s->contop[iw * N + k];
-    rc |= upload(c, ct.data(), ct.size(), &S.contT, c->owned);
+  // (copied as the host holds it, then transposed on the device)
+  if (!rc) {
+    void *ct = nullptr;
+    DBuf raw;
+    if (alloc_zero(c, sizeof(double) * (size_t)N * Nw, &ct) || raw.ensure(sizeof(double) * (size_t)N * Nw) != hipSuccess) {
+      fsg_finalize(c);
+      return fail("fsg_init: no device memory for ContOpac");
+    }
+    CKF(c, hipMemcpyAsync(raw.p, s->contop, sizeof(double) * (size_t)N * Nw, hipMemcpyHostToDevice, c->st));
+    launch_transpose_cont(Nw, N, raw.as<double>(), (double *)ct, c->st);
+    S.contT = (const double *)ct;
     CKF(c, hipStreamSynchronize(c->st));
+    raw.release();
   }
   // spherical models: the rays of SphereInit
   if (s->radius > 0.0 && s->nmu > 0) {
@@ -797,18 +804,19 @@ int set_atmosphere_one(fsg_ctx *c, const fsg_atmosphere *a) {
   CK(put(a->frac_hmol, c->fhmol, sizeof(float) * N));
   CK(put(a->contop_std, S.cstd, sizeof(double) * N));
   CK(put(a->he_gam, S.he_gam, sizeof(double) * (size_t)S.nhe * N));
-  std::vector<double> ct;
-  if (a->contop) {  // ContOpac(Ntau, Nwl) -> [ntau][nwl]
-    ct.resize(N * Nw);
-    for (size_t iw = 0; iw < Nw; iw++)
-      for (size_t k = 0; k < N; k++) ct[k * Nw + iw] = a->contop[iw * N + k];
-    CK(put(ct.data(), S.contT, sizeof(double) * ct.size()));
+  DBuf raw;
+  if (a->contop) {  // ContOpac(Ntau, Nwl) as the host holds it, transposed on the device
+    CK(raw.ensure(sizeof(double) * N * Nw));
+    CK(put(a->contop, raw.p, sizeof(double) * N * Nw));
+    launch_transpose_cont((int64_t)Nw, (int)N, raw.as<double>(), const_cast<double *>(S.contT), s);
   }
   // XlineInit's per-depth constants follow T, Xne and the perturber fractions
   if (a->temp || a->xne || a->frac_h1 || a->frac_he1 || a->frac_hmol)
     launch_xline_init(S.ntau, S.temp, S.xne, c->fh1, c->fhe1, c->fhmol, S.h2brd, const_cast<double *>(S.xln), s);
-  CK(hipGetLastError());
-  CK(hipStreamSynchronize(s));
+  const hipError_t e1 = hipGetLastError(), e2 = hipStreamSynchronize(s);
+  raw.release();
+  CK(e1);
+  CK(e2);
   return 0;
 }
 

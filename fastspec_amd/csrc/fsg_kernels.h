@@ -112,6 +112,8 @@ void convol_gauss_kernel(int nw, double fwhm, int wmode, double wfirst, double w
                          std::vector<double> &kern, int &h);
 void diskinteg_kernels(const ConvolSetup &cs, std::vector<double> &kern, std::vector<int> &rot_off,
                        std::vector<int> &rot_n, std::vector<int> &mac_off, int &mac_n);
+// ContOpac [nwl][ntau] (the host's layout, in device memory) -> [ntau][nwl]
+void launch_transpose_cont(int64_t nwl, int ntau, const double *src, double *dst, hipStream_t st);
 void launch_conv_rows(int nwl, int nrows, int maxk, const ConvRow *d_rows, const double *d_kern, hipStream_t st);
 
 // continuum opacity (fsg_contop.hip): the H- free-free table is CONT_HM_NW wavelengths x

@@ -1654,6 +1654,33 @@ void launch_mark_wide(const DevSetup &S, int64_t n, const double *wl, const int 
   if (n) k_mark_wide<<<nblk(n, 256), 256, 0, st>>>(S, n, wl, flag, il, ir, wlc_list, m, nstep, nwide);
 }
 
+// ContOpac(Ntau, Nwl) as the host holds it (depth fastest) into the device layout
+// [ntau][nwl] (wavelength fastest): 32 x 32 tiles through LDS, both sides coalesced
+__global__ void __launch_bounds__(256) k_transpose_cont(int64_t nwl, int ntau, const double *__restrict__ src,
+                                                        double *__restrict__ dst) {
+  __shared__ double tile[32][33];
+  const int64_t w0 = (int64_t)blockIdx.x * 32;
+  const int k0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int64_t w = w0 + r;
+    const int k = k0 + tx;
+    if (w < nwl && k < ntau) tile[r][tx] = src[w * ntau + k];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int k = k0 + r;
+    const int64_t w = w0 + tx;
+    if (w < nwl && k < ntau) dst[(size_t)k * nwl + w] = tile[tx][r];
+  }
+}
+
+void launch_transpose_cont(int64_t nwl, int ntau, const double *src, double *dst, hipStream_t st) {
+  if (nwl <= 0 || ntau <= 0) return;
+  const dim3 grid((unsigned)((nwl + 31) / 32), (unsigned)((ntau + 31) / 32));
+  k_transpose_cont<<<grid, 256, 0, st>>>(nwl, ntau, src, dst);
+}
+
 void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                    const int *flag, const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
                    int2 *zone_s, LineRec *lrec, double *wlc_s, int4 *meta_s, hipStream_t st) {
