@@ -482,11 +482,12 @@ def run_c3(steps: int, fp64: bool):
 
 
 def run_contop(setup):
-    """fsg_contop over the C2 grid: the seven continuum sources of Xcont the device evaluates
-    (H2+, H-, He-, Rayleigh by H I, He I and H2, Thomson), one lane per wavelength, 72 depths,
-    FP64, ContOpac left on the device.  The reference's Xcont (oracle/_ref, -O3, one thread:
-    its harness loops serially; ContopCalc is an OpenMP loop over wavelengths) is timed on
-    20,000 of the grid's wavelengths, for these sources and for all thirteen."""
+    """fsg_contop over the C2 grid: the ten continuum sources of Xcont the device evaluates
+    (H I, H2+, H-, He I, He II, He-, Rayleigh by H I, He I and H2, Thomson), one lane per
+    wavelength, 72 depths, FP64, ContOpac left on the device.  The reference's Xcont
+    (oracle/_ref, -O3, one thread: its harness loops serially; ContopCalc is an OpenMP loop
+    over wavelengths) is timed on 20,000 of the grid's wavelengths, for these sources and for
+    all thirteen."""
     from fastspec_amd import workloads as W
     from fastspec_amd.abi import FSG_FP64
     from fastspec_amd.synth import FastSpecGPU
@@ -502,7 +503,7 @@ def run_contop(setup):
     g.close()
     n, nt = setup.nwl, setup.ntau
     bytes_alg = n * (8 * nt + 8)  # ContOpac written, the grid read
-    res = {"ms": ms, "nwl": n, "ntau": nt, "sources": [2, 3, 4, 7, 8, 12, 13],
+    res = {"ms": ms, "nwl": n, "ntau": nt, "sources": list(FastSpecGPU.CONTOP_DEVICE_SOURCES),
            "points_per_s": n * nt / (ms * 1e-3),
            "hbm_gbs_algorithmic": bytes_alg / (ms * 1e-3) / 1e9,
            "note": "wall clock of fsg_contop (host set-up of the depth state and tables, one launch, "
@@ -516,7 +517,8 @@ def run_contop(setup):
             oracle_bridge._REF = None
             wl = oracle_bridge.ref_air2vac_array(setup.wavelengths[::max(1, n // 20000)][:20000])
             cpu = {}
-            for name, srcs in (("device_sources", (2, 3, 4, 7, 8, 12, 13)), ("all_13_sources", tuple(range(1, 14)))):
+            for name, srcs in (("device_sources", FastSpecGPU.CONTOP_DEVICE_SOURCES),
+                               ("all_13_sources", tuple(range(1, 14)))):
                 t1 = time.perf_counter()
                 oracle_bridge.ref_xcont(setup.temp, setup.xne, setup.rho, pops, srcs, wl)
                 us = (time.perf_counter() - t1) * 1e6 / len(wl)

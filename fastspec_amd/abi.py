@@ -83,9 +83,13 @@ class CAtmosphere(C.Structure):
     ]
 
 
+CONTOP_HOP_NOLEV = 1 << 20  # FSG_CONTOP_HOP_NOLEV: Ifop(1) = -1, HOP without levels 1-6
+
+
 class CContopEos(C.Structure):
     """fsg_contop_eos: the EOS rows the device's continuum sources read ([ntau] REAL(4))."""
-    _fields_ = [("fpf_h1", _fp), ("fpf_h2", _fp), ("frac_hmin", _fp), ("fpf_he1", _fp), ("frac_hmol", _fp)]
+    _fields_ = [("fpf_h1", _fp), ("fpf_h2", _fp), ("frac_hmin", _fp), ("fpf_he1", _fp), ("frac_hmol", _fp),
+                ("frac_he2", _fp), ("fpf_he2", _fp), ("fpf_he3", _fp)]
 
 
 class CStats(C.Structure):

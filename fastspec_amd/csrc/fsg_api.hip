@@ -23,6 +23,7 @@
 
 #include "../../include/fsgpu.h"
 #include "fsg_ingest.h"
+#include "fsg_contop_tables.h"
 #include "fsg_kernels.h"
 
 using namespace fsg;
@@ -871,18 +872,21 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
   g_err.clear();
   if (!c || !e) return fail("fsg_contop: NULL argument");
   if (c->S.batch) return fail("fsg_contop: a batch context takes each atmosphere's ContOpac through fsg_set_atmosphere");
-  const unsigned mask = (unsigned)ifop_mask;
-  // H2PLOP, HMINOP, HRAYOP, HEMIOP, HERAOP, ELECOP, H2RAOP (Ifop 2, 3, 4, 7, 8, 12, 13)
-  const unsigned on_device = (1u << 1) | (1u << 2) | (1u << 3) | (1u << 6) | (1u << 7) | (1u << 11) | (1u << 12);
+  const unsigned mask = (unsigned)ifop_mask & ~CONT_HOP_NOLEV;
+  if ((ifop_mask & FSG_CONTOP_HOP_NOLEV) && !(mask & 1u))
+    return fail("fsg_contop: FSG_CONTOP_HOP_NOLEV qualifies HOP (bit 0), which is not switched on");
+  // HOP, H2PLOP, HMINOP, HRAYOP, HE1OP, HE2OP, HEMIOP, HERAOP, ELECOP, H2RAOP (Ifop 1-8, 12, 13)
+  const unsigned on_device = 0xFFu | (1u << 11) | (1u << 12);
   if (mask & ~on_device) {
     std::string m = "fsg_contop: continuum source(s)";
     for (int b = 0; b < 32; b++)
       if ((mask & ~on_device) >> b & 1u) m += " " + std::to_string(b + 1);
     return fail(m + " are not evaluated on the device (the host's Xcont keeps them)");
   }
-  const bool h1 = mask & ((1u << 1) | (1u << 2) | (1u << 3)), he1 = mask & ((1u << 6) | (1u << 7));
-  if ((h1 && !e->fpf_h1) || ((mask >> 1 & 1u) && !e->fpf_h2) || ((mask >> 2 & 1u) && !e->frac_hmin) ||
-      (he1 && !e->fpf_he1) || ((mask >> 12 & 1u) && !e->frac_hmol))
+  auto on = [mask](int ifop) { return (mask >> (ifop - 1)) & 1u; };
+  if (((on(1) || on(2) || on(3) || on(4)) && !e->fpf_h1) || ((on(1) || on(2)) && !e->fpf_h2) ||
+      (on(3) && !e->frac_hmin) || ((on(5) || on(7) || on(8)) && !e->fpf_he1) || (on(13) && !e->frac_hmol) ||
+      (on(5) && !e->frac_he2) || (on(6) && (!e->fpf_he2 || !e->fpf_he3)))
     return fail("fsg_contop: a population the selected sources need is NULL");
   CK(hipSetDevice(c->dev));
   const DevSetup &S = c->S;
@@ -897,24 +901,72 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
   CK(hipStreamSynchronize(s));
   std::vector<double> tab;
   contop_tables(tab);
-  std::vector<double> blk(11 * N + tab.size(), 0.0);
+  std::vector<double> blk(CR_NROW * N + tab.size(), 0.0);
   auto col = [&](int r) { return blk.data() + (size_t)r * N; };
   const float *pops[5] = {e->fpf_h1, e->fpf_h2, e->frac_hmin, e->fpf_he1, e->frac_hmol};
   for (size_t k = 0; k < N; k++) {
     const double tk = kKbolt * T[k];
-    col(0)[k] = T[k];
-    col(1)[k] = kHc / tk;     // Hkt
-    col(2)[k] = tk * kErg2ev;  // Tkev
-    col(3)[k] = 5040.0 / T[k];
-    col(4)[k] = xne[k];
-    col(5)[k] = rho[k];
-    for (int q = 0; q < 5; q++) col(6 + q)[k] = pops[q] ? (double)pops[q][k] : 0.0;
+    col(CR_T)[k] = T[k];
+    col(CR_HKT)[k] = kHc / tk;
+    col(CR_TKEV)[k] = tk * kErg2ev;
+    col(CR_THETA)[k] = 5040.0 / T[k];
+    col(CR_XNE)[k] = xne[k];
+    col(CR_RHO)[k] = rho[k];
+    for (int q = 0; q < 5; q++) col(CR_FH1 + q)[k] = pops[q] ? (double)pops[q][k] : 0.0;
+    col(CR_TLOG)[k] = std::log(T[k]);
   }
-  std::copy(tab.begin(), tab.end(), blk.begin() + 11 * N);
+  // ContopInit's factors of HOP, HE1OP and HE2OP (src/contop.f90:377-470), in its order of
+  // operations with the host's libm; the EOS rows are REAL(4), so the logarithms of He I's
+  // FractionPF are single-precision ones, as the reference takes them
+  const double *Tkev = col(CR_TKEV);
+  if (on(1)) {
+    for (size_t j = 0; j < N; j++) {
+      const double nh1 = (double)e->fpf_h1[j];
+      for (int n = 1; n <= 8; n++)
+        col(CR_BOLTH + n - 1)[j] =
+            std::exp(-13.595 * (1.0 - 1.0 / (double)(n * n)) / Tkev[j]) * 2.0 * (double)(n * n) * nh1 / rho[j];
+      col(CR_FREETH)[j] = xne[j] * (double)e->fpf_h2[j] / (std::sqrt(T[j]) * rho[j]);
+      const double xr = nh1 / 13.595 * Tkev[j] / rho[j];
+      col(CR_BOLTEXH)[j] = std::exp(-13.427 / Tkev[j]) * xr;
+      col(CR_EXLIMH)[j] = std::exp(-13.595 / Tkev[j]) * xr;
+    }
+  }
+  if (on(5)) {
+    for (size_t j = 0; j < N; j++) {
+      const float fhe1 = e->fpf_he1[j];
+      const double lf = (double)std::log(fhe1), lr = std::log(rho[j]);
+      for (int n = 1; n <= 10; n++)
+        col(CR_BOLTHE1 + n - 1)[j] = std::exp(-kHe1Chi[n - 1] / Tkev[j] + lf - lr) * kHe1G[n - 1];
+      for (int n = 4; n <= 27; n++)
+        col(CR_BOLTNHE1 + n - 4)[j] =
+            std::exp(-24.587 * (1.0 - 1.0 / (double)(n * n)) / Tkev[j] + lf - lr) * 4.0 * (double)(n * n);
+      col(CR_FREETHE1)[j] = xne[j] * 1.0e-10 * (double)e->frac_he2[j] * 1.0e-10 / rho[j] / std::sqrt(T[j]) * 1.0e-10;
+      const double xrlog = std::log((double)fhe1 * (2.0 / 13.595) * Tkev[j] / rho[j]);
+      col(CR_BOLTEXHE1)[j] = std::exp(-23.730 / Tkev[j] + xrlog);
+      col(CR_EXLIMHE1)[j] = std::exp(-24.587 / Tkev[j] + xrlog);
+    }
+  }
+  if (on(6)) {
+    for (size_t j = 0; j < N; j++) {
+      const double fhe2 = (double)e->fpf_he2[j];
+      for (int n = 1; n <= 9; n++) {
+        const double bltarg = (54.403 - 54.403 / (double)(n * n)) / Tkev[j] + std::log(rho[j]);
+        col(CR_BOLTHE2 + n - 1)[j] =
+            (fhe2 == 0.0 || bltarg > 80.0) ? 0.0 : std::exp(-bltarg) * 2.0 * (double)(n * n) * fhe2;
+      }
+      col(CR_FREETHE2)[j] = xne[j] * (double)e->fpf_he3[j] / std::sqrt(T[j]) / rho[j];
+      const double xrlog = std::log(Tkev[j] / 13.595 / rho[j]);
+      const double bltlog = 53.859 / Tkev[j] - xrlog, exllog = 54.403 / Tkev[j] - xrlog;
+      col(CR_BOLTEXHE2)[j] = (fhe2 == 0.0 || bltlog > 80.0) ? 0.0 : fhe2 * std::exp(-bltlog);
+      col(CR_EXLIMHE2)[j] = (fhe2 == 0.0 || exllog > 80.0) ? 0.0 : fhe2 * std::exp(-exllog);
+    }
+  }
+  std::copy(tab.begin(), tab.end(), blk.begin() + CR_NROW * N);
   CK(c->ctab.ensure(sizeof(double) * blk.size()));
   double *dblk = c->ctab.as<double>();
   CK(hipMemcpyAsync(dblk, blk.data(), sizeof(double) * blk.size(), hipMemcpyHostToDevice, s));
-  launch_contop(S.nwl, S.ntau, S.wave, S.wfirst, dblk, dblk + 11 * N, S.cstd, S.motype, mask, add ? 1 : 0,
+  const unsigned kmask = mask | ((ifop_mask & FSG_CONTOP_HOP_NOLEV) ? CONT_HOP_NOLEV : 0u);
+  launch_contop(S.nwl, S.ntau, S.wave, S.wfirst, dblk, dblk + CR_NROW * N, S.cstd, S.motype, kmask, add ? 1 : 0,
                 const_cast<double *>(S.contT), s);
   CK(hipGetLastError());
   if (contop_out) {  // [ntau][nwl] -> ContOpac(Ntau, Nwl)

@@ -2,14 +2,21 @@
 // (src/synth.f90:663-669) for the sources of Xcont (src/contop.f90:267-314) that depend on
 // the hydrogen, helium and electron populations alone:
 //
+//   HOP     H I bound-free and free-free        src/contop.f90:1079-1100
 //   H2PLOP  H2+ bound-free and free-free        src/contop.f90:1102-1113
 //   HMINOP  H- bound-free and free-free         src/contop.f90:1128-1181
 //   HRAYOP  Rayleigh scattering by H I          src/contop.f90:1115-1124
 //   HEMIOP  He- free-free                       src/contop.f90:1518-1527
+//   HE1OP   He I bound-free and free-free       src/contop.f90:1187-1305
+//   HE2OP   He II bound-free and free-free      src/contop.f90:1488-1516
 //   HERAOP  Rayleigh scattering by He I         src/contop.f90:1529-1539
 //   ELECOP  Thomson scattering                  src/contop.f90:2764-2768
 //   H2RAOP  Rayleigh scattering by H2           src/contop.f90:2770-2780
 //   Linter, Map1 (HMINOP's interpolations)      src/contop.f90:500-601
+//   XKARZAS, COULFF (hydrogenic bound-free and   src/contop.f90:625-658, 695-1077
+//     free-free)
+//   CROSSHE, HE12S1S, HE12S3S, HE12P1P, HE12P3P  src/contop.f90:1309-1486
+//     (He I cross-sections)
 //
 // One lane per wavelength of the grid: the wavelength's own quantities (frequency, the H-
 // free-free table interpolated in wavelength for each of its 11 temperatures, the H-
@@ -25,10 +32,121 @@
 #include <cmath>
 #include <vector>
 
+#include "fsg_contop_tables.h"
 #include "fsg_device.h"
 #include "fsg_kernels.h"
 
 namespace fsg {
+
+constexpr double kClightA = kVc * 1e13;   // speed of light [A/s]
+constexpr double kClightCm = kVc * 1e5;   // speed of light [cm/s]
+constexpr int KL_NP = 29;                 // tabulated points per level of the Karzas & Latter tables
+
+// XKARZAS (src/contop.f90:695-1077) for levels n <= 15 (the callers use n <= 9): the
+// hydrogenic bound-free cross-section of level n (and orbital l, for l < n <= 6) at
+// frequency freq for an effective charge squared zeff2, interpolated linearly in
+// log10(cross-section) against log10(freq / zeff2) in the table of the level; 0 below the
+// threshold.  The bracketing point is the first tabulated frequency below the argument
+// (the tables descend); above the table's top it is the top interval.
+__device__ double xkarzas(double freq, double zeff2, int n, int l) {
+  const double flg = log10(freq / zeff2);
+  const double *fn = kKlFreqn + (n - 1) * KL_NP;
+  const double *x = (l >= n || n > 6) ? kKlXn + (n - 1) * KL_NP : kKlXl + (l * 6 + (n - 1)) * KL_NP;
+  if (flg < fn[KL_NP - 1]) return 0.0;
+  int i = 2;  // 1-based, as the reference's loop index
+  while (i <= KL_NP && !(flg > fn[i - 2])) i++;
+  i = max(i, 3);
+  const double X = (flg - fn[i - 2]) / (fn[i - 3] - fn[i - 2]) * (x[i - 3] - x[i - 2]) + x[i - 2];
+  return exp(X * 2.30258509299405) / zeff2;
+}
+
+// COULFF (src/contop.f90:625-658): the hydrogenic free-free Gaunt factor for charge nz (1 or
+// 2) at depth temperature ln T = tlog and ln(frequency) = freqlg, bilinear in the table of
+// log(gamma^2) and log(h nu / kT)
+__device__ double coulff(double tlog, double freqlg, int nz) {
+  const double z4log = nz == 1 ? 0.0 : 1.20412;
+  const double gamlog = 10.39638 - tlog / 1.15129 + z4log;
+  const int igam = max(min((int)(gamlog + 7.0), 10), 1);
+  const double hvktlg = (freqlg - tlog) / 1.15129 - 20.63764;
+  const int ihvkt = max(min((int)(hvktlg + 9.0), 11), 1);
+  const double P = gamlog - (igam - 7), Q = hvktlg - (ihvkt - 9);
+  auto A = [](int i, int j) { return kCoulffA[(i - 1) * 11 + (j - 1)]; };
+  return (1.0 - P) * ((1.0 - Q) * A(ihvkt, igam) + Q * A(ihvkt + 1, igam)) +
+         P * ((1.0 - Q) * A(ihvkt, igam + 1) + Q * A(ihvkt + 1, igam + 1));
+}
+
+// CROSSHE (src/contop.f90:1309-1355): He I 1s2 photoionization cross-section, linear in
+// wavelength in four tables (every 5, 2, 1 and 0.5 A)
+__device__ double crosshe(double freq0) {
+  if (freq0 < 5.945209e15) return 0.0;
+  const double w = kClightA / freq0;
+  int i;
+  if (w > 50.0) {
+    i = min(92, max(2, (int)(93.0 - (w - 50.0) / 5.0)));
+    return ((w - (double)(92 - i) * 5.0 - 50.0) / 5.0 * (kHeX505[i - 2] - kHeX505[i - 1]) + kHeX505[i - 1]) * 1.e-18;
+  }
+  if (w > 20.0) {
+    i = min(16, max(2, (int)(17.0 - (w - 20.0) / 2.0)));
+    return ((w - (double)(16 - i) * 2.0 - 20.0) / 2.0 * (kHeX50[i - 2] - kHeX50[i - 1]) + kHeX50[i - 1]) * 1.e-18;
+  }
+  if (w > 10.0) {
+    i = min(11, max(2, (int)(12.0 - (w - 10.0))));
+    return ((w - (double)(11 - i) - 10.0) * (kHeX20[i - 2] - kHeX20[i - 1]) + kHeX20[i - 1]) * 1.e-18;
+  }
+  i = min(21, max(2, (int)(22.0 - w / 0.5)));
+  return ((w - (double)(21 - i) * 0.5) / 0.5 * (kHeX10[i - 2] - kHeX10[i - 1]) + kHeX10[i - 1]) * 1.e-18;
+}
+
+// The Mathisen (1984) tables of HE12S1S, HE12S3S, HE12P1P, HE12P3P: log10 cross-section
+// linear in log10(frequency) over 16 descending points
+__device__ double he_table16(double freq0, const double *fq, const double *xs) {
+  const double flg = log10(freq0);
+  int i = 2;
+  while (i <= 16 && !(flg > fq[i - 1])) i++;
+  i = min(i, 16);
+  const double X = (flg - fq[i - 1]) / (fq[i - 2] - fq[i - 1]) * (xs[i - 2] - xs[i - 1]) + xs[i - 1];
+  return pow(10.0, X);
+}
+
+// HE12S1S, HE12S3S, HE12P1P, HE12P3P (src/contop.f90:1358-1486): the 1s2s and 1s2p
+// photoionization cross-sections (threshold wavenumber e0 [1/cm]); above 2.4 Ry the 2s and
+// 2p1P ones follow the resonance fits of the reference
+__device__ double he12(int which, double freq0) {
+  const double ryd = 109722.267;
+  switch (which) {
+    case 0: {  // 1s2s 1S
+      if (freq0 < 32033.214 * kClightCm) return 0.0;
+      if (freq0 > 2.4 * ryd * kClightCm) {
+        const double wn = freq0 / kClightCm, ek = (wn - 32033.214) / ryd;
+        const double eps = 2.0 * (ek - 2.612316) / 0.00322;
+        return 0.008175 * pow(484940.0 / wn, 2.71) * 8.067e-18 * ((eps + 76.21) * (eps + 76.21)) / (1.0 + eps * eps);
+      }
+      return he_table16(freq0, kHeFreq1s, kHeX1s);
+    }
+    case 1: {  // 1s2s 3S
+      if (freq0 < 38454.691 * kClightCm) return 0.0;
+      if (freq0 > 2.4 * ryd * kClightCm) {
+        const double wn = freq0 / kClightCm, ek = (wn - 38454.691) / ryd;
+        const double eps = 2.0 * (ek - 2.47898) / 0.000780;
+        return 0.01521 * pow(470310.0 / wn, 3.12) * 8.067e-18 * ((eps - 122.4) * (eps - 122.4)) / (1.0 + eps * eps);
+      }
+      return he_table16(freq0, kHeFreq3s, kHeX3s);
+    }
+    case 2: {  // 1s2p 1P
+      if (freq0 < 27175.76 * kClightCm) return 0.0;
+      if (freq0 > 2.4 * ryd * kClightCm) {
+        const double wn = freq0 / kClightCm, ek = (wn - 27175.76) / ryd;
+        const double e1s = 2.0 * (ek - 2.446534) / 0.01037, e1d = 2.0 * (ek - 2.59427) / 0.00538;
+        return 0.0009487 * pow(466750.0 / wn, 3.69) * 8.067e-18 *
+               ((e1s - 29.30) * (e1s - 29.30) / (1.0 + e1s * e1s) + (e1d + 172.4) * (e1d + 172.4) / (1.0 + e1d * e1d));
+      }
+      return he_table16(freq0, kHeFreq1p, kHeX1p);
+    }
+    default:  // 1s2p 3P
+      if (freq0 < 29223.753 * kClightCm) return 0.0;
+      return he_table16(freq0, kHeFreq3p, kHeX3p);
+  }
+}
 
 // Linter (src/contop.f90:500-516) at one abscissa: linear interpolation between the first
 // tabulated point above x and the one before it (extrapolation past either end); the
@@ -91,10 +209,11 @@ __global__ void __launch_bounds__(256) k_contop(int nwl, int ntau, const double 
                                                 double *__restrict__ contT) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nwl) return;
-  const double *T = dep, *Hkt = dep + ntau, *Tkev = dep + 2 * ntau, *Theta = dep + 3 * ntau;
-  const double *Xne = dep + 4 * ntau, *Rho = dep + 5 * ntau;
-  const double *Fh1 = dep + 6 * ntau, *Fh2 = dep + 7 * ntau, *Fhmin = dep + 8 * ntau;
-  const double *Fhe1 = dep + 9 * ntau, *Fhmol = dep + 10 * ntau;
+  auto row = [dep, ntau](int r) { return dep + (size_t)r * ntau; };
+  const double *T = row(CR_T), *Hkt = row(CR_HKT), *Tkev = row(CR_TKEV), *Theta = row(CR_THETA);
+  const double *Xne = row(CR_XNE), *Rho = row(CR_RHO);
+  const double *Fh1 = row(CR_FH1), *Fh2 = row(CR_FH2), *Fhmin = row(CR_FHMIN);
+  const double *Fhe1 = row(CR_FHE1), *Fhmol = row(CR_FHMOL), *Tlog = row(CR_TLOG);
   const double *hmwlog = tab, *hmfflog = tab + CONT_HM_NW;  // [22], [22][11]
   const double *thetaff = tab + CONT_HM_NW * (1 + CONT_HM_NT);  // [11]
   const double *wbf = thetaff + CONT_HM_NT, *bf = wbf + CONT_HMBF_N;  // [85], [85]
@@ -105,6 +224,71 @@ __global__ void __launch_bounds__(256) k_contop(int nwl, int ntau, const double 
   const bool on_h2p = (mask >> 1) & 1, on_hmin = (mask >> 2) & 1, on_hray = (mask >> 3) & 1;
   const bool on_hemi = (mask >> 6) & 1, on_hera = (mask >> 7) & 1, on_el = (mask >> 11) & 1;
   const bool on_h2ra = (mask >> 12) & 1;
+  const bool on_hop = mask & 1, on_he1 = (mask >> 4) & 1, on_he2 = (mask >> 5) & 1;
+  const bool hop_lev = !(mask & CONT_HOP_NOLEV);
+
+  // HOP: the bound-free cross-sections of levels 1-8 (Karzas & Latter, Z = 1)
+  double hcont[8], hop_c = 0.0, hop_cf = 0.0;
+  if (on_hop) {
+    for (int n = 1; n <= 8; n++) hcont[n - 1] = xkarzas(Freq, 1.0, n, n);
+    const double f3 = Freq * Freq * Freq;
+    hop_cf = 3.6919e8 / f3;
+    hop_c = 2.815e29 / f3;
+  }
+  // HE2OP: levels 1-9 of He II (Z^2 = 4)
+  double he2c[9], he2_c = 0.0, he2_cf = 0.0;
+  if (on_he2) {
+    for (int n = 1; n <= 9; n++) he2c[n - 1] = xkarzas(Freq, 4.0, n, n);
+    const double f5 = Freq * 1.0e-05, f3 = f5 * f5 * f5;
+    he2_cf = 3.6919e-07 / f3 * 4.0;
+    he2_c = 2.815e14 * 2.0 * 2.0 / f3;
+  }
+  // HE1OP: the ten explicit levels from the lowest one ionizable at this frequency, the
+  // photoionizations leaving He II in n = 2 and n = 3 added to their levels
+  double he1t[10], he1_c = 0.0, he1_cf = 0.0;
+  int he1min = 0;
+  if (on_he1) {
+    const double f10 = Freq * 1.0e-10, f3 = f10 * f10 * f10;
+    he1_cf = 3.6919e8 / f3;
+    he1_c = 2.815e-1 / f3;
+    const double hefreq[10] = {5.9452090e15, 1.1528440e15, 0.9603331e15, 0.8761076e15, 0.8147104e15,
+                               0.4519048e15, 0.4030971e15, 0.3821191e15, 0.3660215e15, 0.3627891e15};
+    for (int n = 1; n <= 10 && !he1min; n++)
+      if (hefreq[n - 1] <= Freq) he1min = n;
+    for (int n = 0; n < 10; n++) he1t[n] = 0.0;
+    if (he1min >= 1) {
+      const double zeff3[5] = {1.236439, 1.102898, 1.045499, 1.001427, 0.9926};
+      const int l3[5] = {0, 0, 1, 2, 1};
+      for (int n = he1min; n <= 10; n++) {
+        if (n == 1) he1t[0] = crosshe(Freq);
+        else if (n == 2) he1t[1] = he12(1, Freq);
+        else if (n == 3) he1t[2] = he12(0, Freq);
+        else if (n == 4) he1t[3] = he12(3, Freq);
+        else if (n == 5) he1t[4] = he12(2, Freq);
+        else he1t[n - 1] = xkarzas(Freq, zeff3[n - 6], 3, l3[n - 6]);
+      }
+      const double ryd = 109722.273 * 2.99792458e10;
+      // n = 2 levels, He II left in n = 2: 1s2p 1P, 1s2p 3P, 1s2s 1S, 1s2s 3S
+      const double e2 = 527490.06, lev2[4] = {171135.000, 169087.0, 166277.546, 159856.069};
+      const int to2[4] = {4, 3, 2, 1};
+      for (int q = 0; q < 4; q++) {
+        const double fh = (e2 - lev2[q]) * kClightCm;
+        if (!(Freq >= fh)) break;
+        he1t[to2[q]] = he1t[to2[q]] + xkarzas(Freq, fh / ryd, 1, 0);
+      }
+      // n = 3 levels, He II left in n = 3: 1s3p 1P, 1s3d, 1s3p 3P, 1s3s 1S, 1s3s 3S
+      const double e3 = 588451.59, lev3[5] = {186209.471, 186101.0, 185564.0, 184864.0, 183236.0};
+      const int to3[5] = {9, 8, 7, 6, 5};
+      for (int q = 0; q < 5; q++) {
+        const double fh = (e3 - lev3[q]) * kClightCm;
+        if (!(Freq >= fh)) break;
+        he1t[to3[q]] = he1t[to3[q]] + xkarzas(Freq, fh / ryd, 1, 0);
+      }
+    }
+  }
+  // hydrogenic levels n = 4-27 of He I, from 1.25408e16 Hz on (below 240 A): evaluated in
+  // the depth loop, where the rare frequencies that need them pay for them
+  const bool he1n = on_he1 && he1min >= 1 && Freq >= 1.25408e16 && Freq >= (588451.59 - 183236.0) * kClightCm;
 
   // H2PLOP
   const bool h2p = on_h2p && !(Freq > 3.28805e15);
@@ -150,7 +334,43 @@ __global__ void __launch_bounds__(256) k_contop(int nwl, int ntau, const double 
   for (int k = 0; k < ntau; k++) {
     const double ehvkt = exp(-Freq * Hkt[k]);
     const double stim = 1.0 - ehvkt;
+    double ahyd = 0.0, ahe1 = 0.0, ahe2 = 0.0;
     double ah2p = 0.0, ahmin = 0.0, ahemin = 0.0, sh = 0.0, she = 0.0, sel = 0.0, sh2 = 0.0;
+    if (on_hop) {
+      const double exlim = row(CR_EXLIMH)[k];
+      double ex = row(CR_BOLTEXH)[k];
+      if (Freq < 4.05933e13) ex = exlim / ehvkt;
+      double h = ((ex - exlim) * hop_c + hcont[6] * row(CR_BOLTH + 6)[k] + hcont[7] * row(CR_BOLTH + 7)[k] +
+                  coulff(Tlog[k], Freqlg, 1) * row(CR_FREETH)[k] * hop_cf) * stim;
+      if (hop_lev) {
+        double sm = 0.0;
+        for (int n = 0; n < 6; n++) sm = sm + hcont[n] * row(CR_BOLTH + n)[k];
+        h = h + sm * stim;
+      }
+      ahyd = h;
+    }
+    if (on_he1) {
+      const double exlim = row(CR_EXLIMHE1)[k];
+      double ex = row(CR_BOLTEXHE1)[k];
+      if (Freq < 2.055e14) ex = exlim / ehvkt;
+      double he1 = (ex - exlim) * he1_c;
+      if (he1min >= 1) {
+        for (int n = he1min; n <= 10; n++) he1 = he1 + he1t[n - 1] * row(CR_BOLTHE1 + n - 1)[k];
+        if (Freq >= 1.25408e16)
+          for (int n = 4; n <= 27; n++)
+            he1 = he1 + (he1n ? xkarzas(Freq, 4.0 - 3.0 / (double)(n * n), 1, 0) : 0.0) * row(CR_BOLTNHE1 + n - 4)[k];
+      }
+      ahe1 = (he1 + coulff(Tlog[k], Freqlg, 1) * row(CR_FREETHE1)[k] * he1_cf) * stim;
+    }
+    if (on_he2) {
+      const double exlim = row(CR_EXLIMHE2)[k];
+      double ex = row(CR_BOLTEXHE2)[k];
+      if (Freq < 1.31522e14) ex = exlim / ehvkt;
+      double he2 = (ex - exlim) * he2_c;
+      for (int n = 0; n < 9; n++) he2 = he2 + he2c[n] * row(CR_BOLTHE2 + n)[k];
+      he2 = (he2 + coulff(Tlog[k], Freqlg, 2) * he2_cf * row(CR_FREETHE2)[k]) * stim;
+      ahe2 = he2 < 1.0e-30 ? 0.0 : he2;
+    }
     if (h2p) ah2p = exp(-Es / Tkev[k] + Fr) * Fh1[k] * 2.0 * Fh2[k] / Rho[k] * stim;
     if (on_hmin) {
       const double fft = linter1(thetaff, fftt, 1, CONT_HM_NT, Theta[k]);
@@ -164,7 +384,7 @@ __global__ void __launch_bounds__(256) k_contop(int nwl, int ntau, const double 
     if (on_el) sel = 0.6653e-24 * Xne[k] / Rho[k];
     if (on_h2ra) sh2 = sig_h2 * Fhmol[k] / Rho[k];
     // Xcont's sum (the absorption sources, then the scattering ones)
-    double copac = ah2p + ahmin + ahemin;
+    double copac = ahyd + ah2p + ahmin + ahe1 + ahe2 + ahemin;
     copac = copac + sh + she + sel + sh2;
     if (motype == 1) copac = copac / cstd[k];
     const size_t o = (size_t)k * nwl + p;

@@ -119,9 +119,18 @@ void launch_conv_rows(int nwl, int nrows, int maxk, const ConvRow *d_rows, const
 // continuum opacity (fsg_contop.hip): the H- free-free table is CONT_HM_NW wavelengths x
 // CONT_HM_NT temperatures, the H- bound-free cross-section CONT_HMBF_N points; tab holds
 // ln(91.134 / lambda_k) [NW], ln(K_ff) [NW][NT], theta [NT], lambda_bf [N], sigma_bf [N];
-// dep holds T, hc/kT, kT [eV], 5040/T, Xne, Rho, FractionPF(H I), FractionPF(H II),
-// Fraction(H-), FractionPF(He I), Fraction(H2) [11][ntau]; mask bit s-1 switches source s
+// dep holds the rows below, each [ntau]: the depth state and populations, then ContopInit's
+// factors of HOP, HE1OP and HE2OP (src/contop.f90:377-470; BoltnHe1 for n = 4..27).  mask
+// bit s-1 switches source s; bit 20 is Ifop(1) = -1 (HOP without levels 1-6).
 constexpr int CONT_HM_NW = 22, CONT_HM_NT = 11, CONT_HMBF_N = 85;
+enum ContRow {
+  CR_T, CR_HKT, CR_TKEV, CR_THETA, CR_XNE, CR_RHO, CR_FH1, CR_FH2, CR_FHMIN, CR_FHE1, CR_FHMOL, CR_TLOG,
+  CR_BOLTH, CR_FREETH = CR_BOLTH + 8, CR_BOLTEXH, CR_EXLIMH,
+  CR_BOLTHE1, CR_BOLTNHE1 = CR_BOLTHE1 + 10, CR_FREETHE1 = CR_BOLTNHE1 + 24, CR_BOLTEXHE1, CR_EXLIMHE1,
+  CR_BOLTHE2, CR_FREETHE2 = CR_BOLTHE2 + 9, CR_BOLTEXHE2, CR_EXLIMHE2,
+  CR_NROW
+};
+constexpr unsigned CONT_HOP_NOLEV = 1u << 20;
 void launch_contop(int nwl, int ntau, const double *wave, double wfirst, const double *dep, const double *tab,
                    const double *cstd, int motype, unsigned mask, int add, double *contT, hipStream_t st);
 void contop_tables(std::vector<double> &tab);  // host: the table block of launch_contop

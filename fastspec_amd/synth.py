@@ -21,7 +21,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib
-from .abi import FSG_FP32, FSG_FP64, CStats, LineList, Setup, ptr
+from .abi import CONTOP_HOP_NOLEV, FSG_FP32, FSG_FP64, CStats, LineList, Setup, ptr
 
 # Contexts and device lists not yet released.  Interpreter teardown runs finalizers in no
 # set order and may run them after parts of the runtime are gone, so whatever the caller
@@ -139,23 +139,27 @@ class FastSpecGPU:
             self.setup = new
 
     # --- continuum opacity --------------------------------------------------------------
-    CONTOP_DEVICE_SOURCES = (2, 3, 4, 7, 8, 12, 13)  # H2PLOP HMINOP HRAYOP HEMIOP HERAOP ELECOP H2RAOP
+    # HOP H2PLOP HMINOP HRAYOP HE1OP HE2OP HEMIOP HERAOP ELECOP H2RAOP
+    CONTOP_DEVICE_SOURCES = (1, 2, 3, 4, 5, 6, 7, 8, 12, 13)
 
     def ContopCalc(self, fpf_h1=None, fpf_h2=None, frac_hmin=None, fpf_he1=None, frac_hmol=None,
-                   sources=CONTOP_DEVICE_SOURCES, add: bool = False, copy_out: bool = True):
+                   frac_he2=None, fpf_he2=None, fpf_he3=None, sources=CONTOP_DEVICE_SOURCES,
+                   hop_nolev: bool = False, add: bool = False, copy_out: bool = True):
         """ContopCalc's loop (src/synth.f90:663-669) for the continuum sources the device
-        evaluates (fsg_contop): sources are Ifop numbers among 2, 3, 4, 7, 8, 12, 13; the
-        populations are the EOS rows [ntau] they read (FractionPF of H I, H II and He I,
-        Fraction of H- and H2).  add=False replaces the context's ContOpac with their sum,
-        add=True adds it to the ContOpac the context holds.  Returns the resulting ContOpac
-        [nwl][ntau], which the host copy of the set-up takes too; copy_out=False leaves it on
-        the device only (returns None, and the host copy keeps its earlier ContOpac)."""
+        evaluates (fsg_contop): sources are Ifop numbers among 1-8, 12, 13; the populations
+        are the EOS rows [ntau] they read (FractionPF of H I, H II, He I, He II and He III,
+        Fraction of H-, H2 and He II).  hop_nolev is the reference's Ifop(1) = -1 (HOP without
+        levels 1-6, which a hydrogen line list supplies).  add=False replaces the context's
+        ContOpac with their sum, add=True adds it to the ContOpac the context holds.  Returns
+        the resulting ContOpac [nwl][ntau], which the host copy of the set-up takes too;
+        copy_out=False leaves it on the device only (returns None, and the host copy keeps its
+        earlier ContOpac)."""
         import dataclasses
         from .abi import CContopEos
         n = self.setup.ntau
         arrs = {}
         for k, v in dict(fpf_h1=fpf_h1, fpf_h2=fpf_h2, frac_hmin=frac_hmin, fpf_he1=fpf_he1,
-                         frac_hmol=frac_hmol).items():
+                         frac_hmol=frac_hmol, frac_he2=frac_he2, fpf_he2=fpf_he2, fpf_he3=fpf_he3).items():
             if v is not None:
                 a = np.ascontiguousarray(v, np.float32)
                 if a.shape != (n,):
@@ -164,6 +168,8 @@ class FastSpecGPU:
         mask = 0
         for src in sources:
             mask |= 1 << (int(src) - 1)
+        if hop_nolev:
+            mask |= CONTOP_HOP_NOLEV
         eos = CContopEos(**{k: ptr(a, C.c_float) for k, a in arrs.items()})
         out = np.zeros((self.setup.nwl, n)) if copy_out else None
         _lib.check(self._lib.fsg_contop(self.ctx, C.byref(eos), mask, int(bool(add)), ptr(out, C.c_double)))

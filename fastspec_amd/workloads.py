@@ -232,17 +232,23 @@ def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode:
 
 def continuum_populations(setup: Setup) -> dict:
     """The EOS rows the device's continuum sources read, for a set-up made by build_setup:
-    FractionPF of H I, H II and He I (their slots), Fraction of H2, and Fraction of H- from
-    the Saha equation against H I (binding 0.754 eV, statistical weights 1 and 2)."""
+    FractionPF of H I, H II, He I and He II (their slots), Fraction of H2 and He II (twice
+    its FractionPF, partition function 2), Fraction of H- from the Saha equation against H I
+    (binding 0.754 eV, statistical weights 1 and 2), and FractionPF of He III from the Saha
+    equation against He II (54.418 eV, partition functions 2 and 1)."""
     nsp = int(setup.meta["n_atomic"])  # H I, He I, H2, H II, He II follow the atomic slots
     ev_k = EV / KB
     t = setup.temp
     n_h1 = setup.frac_h1.astype(np.float64)
     c_saha = 4.1412e-16  # (h^2 / (2 pi m_e k))^(3/2) [cm^3 K^(3/2)]
     n_hmin = n_h1 * setup.xne * c_saha * t**-1.5 / 4.0 * np.exp(0.754 * ev_k / t)
+    fpf_he2 = setup.fraction_pf[:, nsp + 4].copy()
+    n_he2 = 2.0 * fpf_he2.astype(np.float64)
+    n_he3 = n_he2 * saha_ratio(t, setup.xne, 54.418, 2.0, 1.0)
     return dict(fpf_h1=setup.fraction_pf[:, nsp].copy(), fpf_h2=setup.fraction_pf[:, nsp + 3].copy(),
                 frac_hmin=n_hmin.astype(np.float32), fpf_he1=setup.fraction_pf[:, nsp + 1].copy(),
-                frac_hmol=setup.frac_hmol.copy())
+                frac_hmol=setup.frac_hmol.copy(), frac_he2=n_he2.astype(np.float32), fpf_he2=fpf_he2,
+                fpf_he3=n_he3.astype(np.float32))
 
 
 def _f32pow10(x):

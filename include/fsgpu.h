@@ -334,21 +334,28 @@ int fsg_set_atmosphere(fsg_ctx *ctx, int32_t atm, const fsg_atmosphere *a);
  * Xcont at Air2Vac of each wavelength, divided by ContOpacStd for Motype 1) for the sources
  * of Xcont (src/contop.f90:267-314) that need only hydrogen, helium and electron
  * populations.  ifop_mask bit s-1 switches source s of the reference's Ifop numbering; the
- * device evaluates 2 H2PLOP, 3 HMINOP, 4 HRAYOP, 7 HEMIOP, 8 HERAOP, 12 ELECOP and 13 H2RAOP
- * (other bits are refused: the host's Xcont keeps HOP, HE1OP, HE2OP, COOLOP, LUKEOP and
- * HOTOP).  T, Xne and Rho are the context's (fsg_init / fsg_set_atmosphere); the populations
- * are the EOS rows the sources read, as ContopSetup indexes them (src/synth.f90:608-628).
- * add = 0 replaces the context's ContOpac with the device sources' sum; add = 1 adds the sum
- * to it (the host's part, from Xcont with those Ifop flags off, given at fsg_init or
- * fsg_set_atmosphere).  contop_out [nwl][ntau], if non-NULL, receives the resulting
- * ContOpac.  Windows and formal solutions of earlier calls are dropped.  Ordinary contexts
- * only. */
+ * device evaluates 1 HOP, 2 H2PLOP, 3 HMINOP, 4 HRAYOP, 5 HE1OP, 6 HE2OP, 7 HEMIOP, 8 HERAOP,
+ * 12 ELECOP and 13 H2RAOP (bits 8-10, COOLOP, LUKEOP and HOTOP, are refused: the host's Xcont
+ * keeps them).  Bit FSG_CONTOP_HOP_NOLEV (with bit 0) is the reference's Ifop(1) = -1, set
+ * by ContopSetup when a hydrogen line list supplies the bound-free edges of levels 1-6
+ * (src/synth.f90:600): HOP then omits those levels.  T, Xne and Rho are the context's
+ * (fsg_init / fsg_set_atmosphere); the populations are the EOS rows the sources read, as
+ * ContopSetup indexes them (src/synth.f90:608-628); ContopInit's depth factors
+ * (src/contop.f90:319-492) are formed from them on the host.  add = 0 replaces the
+ * context's ContOpac with the device sources' sum; add = 1 adds the sum to it (the host's
+ * part, from Xcont with those Ifop flags off, given at fsg_init or fsg_set_atmosphere).
+ * contop_out [nwl][ntau], if non-NULL, receives the resulting ContOpac.  Windows and formal
+ * solutions of earlier calls are dropped.  Ordinary contexts only. */
+#define FSG_CONTOP_HOP_NOLEV (1 << 20)
 typedef struct fsg_contop_eos {
-  const float *fpf_h1;     /* [ntau] FractionPF(IxH1, :)   H I  */
-  const float *fpf_h2;     /* [ntau] FractionPF(IxH2, :)   H II */
-  const float *frac_hmin;  /* [ntau] Fraction(IxHmin, :)   H-   */
-  const float *fpf_he1;    /* [ntau] FractionPF(IxHe1, :)  He I */
-  const float *frac_hmol;  /* [ntau] Fraction(IxHmol, :)   H2   */
+  const float *fpf_h1;     /* [ntau] FractionPF(IxH1, :)   H I   (HOP, H2PLOP, HMINOP, HRAYOP) */
+  const float *fpf_h2;     /* [ntau] FractionPF(IxH2, :)   H II  (HOP, H2PLOP) */
+  const float *frac_hmin;  /* [ntau] Fraction(IxHmin, :)   H-    (HMINOP) */
+  const float *fpf_he1;    /* [ntau] FractionPF(IxHe1, :)  He I  (HE1OP, HEMIOP, HERAOP) */
+  const float *frac_hmol;  /* [ntau] Fraction(IxHmol, :)   H2    (H2RAOP) */
+  const float *frac_he2;   /* [ntau] Fraction(IxHe2, :)    He II (HE1OP free-free) */
+  const float *fpf_he2;    /* [ntau] FractionPF(IxHe2, :)  He II (HE2OP) */
+  const float *fpf_he3;    /* [ntau] FractionPF(IxHe3, :)  He III (HE2OP free-free) */
 } fsg_contop_eos;
 int fsg_contop(fsg_ctx *ctx, const fsg_contop_eos *eos, int32_t ifop_mask, int32_t add, double *contop_out);
 

@@ -502,11 +502,12 @@ def ref_convolgau(setup: Setup, s, fwhm: float):
     return out
 
 
-def ref_xcont(temp, xne, rho, pops: dict, sources, wl_vac, xna=None) -> np.ndarray:
+def ref_xcont(temp, xne, rho, pops: dict, sources, wl_vac, xna=None, hop_nolev=False) -> np.ndarray:
     """The reference's Xcont (src/contop.f90:267-314), compiled from its sources, at vacuum
-    wavelengths wl_vac with only the Ifop sources given switched on: Copac [n][ntau].  pops
-    holds the EOS rows (fpf_h1, fpf_h2, frac_hmin, fpf_he1, frac_hmol) that the harness puts
-    in its slots 1 H I, 2 H II, 3 H-, 4 He I, 5 H2 (He II, He III and the metals empty)."""
+    wavelengths wl_vac with only the Ifop sources given switched on (hop_nolev: Ifop(1) = -1):
+    Copac [n][ntau].  pops holds the EOS rows (fpf_h1, fpf_h2, frac_hmin, fpf_he1, frac_hmol
+    and, when present, frac_he2, fpf_he2, fpf_he3) that the harness puts in its slots 1 H I,
+    2 H II, 3 H-, 4 He I, 5 H2, 6 He II, 7 He III (the metals empty)."""
     lib = ref_lib()
     n = int(len(wl_vac))
     ntau = int(len(temp))
@@ -521,9 +522,14 @@ def ref_xcont(temp, xne, rho, pops: dict, sources, wl_vac, xna=None) -> np.ndarr
     # with their FractionPF, H- and H2 with their Fraction
     frac[:, 0], frac[:, 1], frac[:, 3] = fpf[:, 0], fpf[:, 1], fpf[:, 3]
     fpf[:, 2], fpf[:, 4] = frac[:, 2], frac[:, 4]
+    if "fpf_he2" in pops:
+        fpf[:, 5], frac[:, 5] = pops["fpf_he2"], pops["frac_he2"]
+        fpf[:, 6] = frac[:, 6] = pops["fpf_he3"]
     ifop = np.zeros(20, np.int32)
     for s in sources:
         ifop[int(s) - 1] = 1
+    if hop_nolev:
+        ifop[0] = -1
     out = np.zeros((n, ntau))
     ar = lambda a: np.ascontiguousarray(a, np.float64)
     t, ne, r = ar(temp), ar(xne), ar(rho)

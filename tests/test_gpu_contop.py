@@ -1,10 +1,13 @@
 """Continuum opacity on the device (fsg_contop) against the reference's own Xcont, compiled
 from its sources (oracle/_ref, src/contop.f90:267-314), on the configurations' grids: the
 plain configuration (5000-5100 A, solar), a slice of the cool-dwarf log grid (3200 K), the
-H- bound-free threshold in the infrared and the near-UV of a hot model.  Bound: 1e-12
-relative per (wavelength, depth), every source alone and all together; add mode, Motype 1
-and the refusal of sources the host keeps; and a synthesis that uses the device continuum
-against the oracle's with the reference continuum."""
+H- bound-free threshold in the infrared, the near-UV of an 8000 K model and of a 15000 K
+model (He II populated), and the far UV of the 15000 K model (220-950 A: the He II, He I and
+H I edges, and the hydrogenic He I levels below 240 A).  Bound: 1e-12 relative per
+(wavelength, depth), every source alone and all together; HOP without its levels 1-6
+(Ifop(1) = -1); add mode, Motype 1 and the refusal of sources the host keeps; and a
+synthesis that uses the device continuum against the oracle's with the reference
+continuum."""
 import dataclasses
 
 import numpy as np
@@ -16,7 +19,7 @@ from oracle_bridge import Oracle, ref_air2vac_array, ref_xcont
 
 pytestmark = [pytest.mark.gpu, pytest.mark.ref]
 
-SOURCES = (2, 3, 4, 7, 8, 12, 13)
+SOURCES = (1, 2, 3, 4, 5, 6, 7, 8, 12, 13)
 REL = 1e-12
 # H2PLOP's exponent is a quartic in ln(frequency) whose terms, ~1e4, cancel to ~-90: one ulp
 # of the logarithm (device and host libm may differ by one) moves H2+ opacity by ~2e-12
@@ -29,7 +32,9 @@ def _grids():
     c3, _ = W.config_c3_setup(10000.0, 10020.0)
     ir = W.build_setup(W.grey_atmosphere(4500.0, 4.5), 16300.0, 16600.0, 0.05, 0)
     uv = W.build_setup(W.grey_atmosphere(8000.0, 4.0), 3000.0, 3100.0, 0.05, 0)
-    return dict(c1=c1, c3=c3, ir_edge=ir, uv=uv)
+    hot = W.build_setup(W.grey_atmosphere(15000.0, 4.0), 3000.0, 3100.0, 0.05, 0)
+    fuv = W.build_setup(W.grey_atmosphere(15000.0, 4.0), 220.0, 950.0, 0.25, 0)
+    return dict(c1=c1, c3=c3, ir_edge=ir, uv=uv, hot=hot, fuv=fuv)
 
 
 GRIDS = _grids()
@@ -56,9 +61,14 @@ def test_all_device_sources_match_reference(name):
     assert err <= REL
 
 
-@pytest.mark.parametrize("src", SOURCES)
-def test_each_source_matches_reference(src):
-    s = GRIDS["c1"]
+# every source on the solar grid, the helium and hydrogen bound-free sources also on the hot
+# models (He II is empty in a solar model)
+EACH = [(src, "c1") for src in SOURCES if src != 6] + [(src, g) for src in (1, 5, 6) for g in ("hot", "fuv")]
+
+
+@pytest.mark.parametrize("src,grid", EACH)
+def test_each_source_matches_reference(src, grid):
+    s = GRIDS[grid]
     pops = W.continuum_populations(s)
     got = _gpu(s, pops, (src,))
     ref = _ref(s, pops, (src,))
@@ -66,8 +76,25 @@ def test_each_source_matches_reference(src):
     assert np.all((got == 0.0) == (ref == 0.0))
     nz = ref > 0.0
     err = float(np.max(np.abs(got[nz] - ref[nz]) / ref[nz]))
-    print(f"\nPARITY contop source={src} rel={err:.3e}", flush=True)
+    print(f"\nPARITY contop source={src} grid={grid} rel={err:.3e}", flush=True)
     assert err <= (REL_H2PLOP if src == 2 else REL)
+
+
+@pytest.mark.parametrize("grid", ["c1", "fuv"])
+def test_hop_without_levels_1_to_6(grid):
+    """Ifop(1) = -1 (a hydrogen line list supplies the edges of levels 1-6): less opacity
+    wherever those levels' edges lie redward, the reference's to 1e-12."""
+    s = GRIDS[grid]
+    pops = W.continuum_populations(s)
+    from fastspec_amd.synth import FastSpecGPU
+    with FastSpecGPU(s, precision=FSG_FP64) as g:
+        got = g.ContopCalc(**pops, sources=(1,), hop_nolev=True)
+    ref = ref_xcont(s.temp, s.xne, s.rho, pops, (1,), ref_air2vac_array(s.wavelengths), hop_nolev=True)
+    full = _ref(s, pops, (1,))
+    assert np.all(ref <= full) and np.any(ref < full)
+    err = float(np.max(np.abs(got - ref) / ref))
+    print(f"\nPARITY contop source=1 nolev grid={grid} rel={err:.3e}", flush=True)
+    assert err <= REL
 
 
 def test_add_mode_and_motype1():
@@ -88,11 +115,15 @@ def test_host_sources_refused():
     from fastspec_amd._lib import FsgError
     s = GRIDS["c1"]
     pops = W.continuum_populations(s)
-    for src in (1, 5, 6, 9, 10, 11):
+    for src in (9, 10, 11):
         with pytest.raises(FsgError, match="not evaluated on the device"):
             _gpu(s, pops, SOURCES + (src,))
     with pytest.raises(FsgError, match="population"):
         _gpu(s, dict(pops, frac_hmin=None), (3,))
+    with pytest.raises(FsgError, match="population"):
+        _gpu(s, dict(pops, fpf_he3=None), (6,))
+    with pytest.raises(FsgError, match="HOP"):
+        _gpu(s, pops, (2,), hop_nolev=True)
 
 
 @pytest.mark.parametrize("prec", [FSG_FP64, FSG_FP32])

@@ -65,3 +65,54 @@ def test_hydrogen_rayleigh_lambda4(atm):
     s, pops = atm
     sig = ref_xcont(s.temp, s.xne, s.rho, pops, (4,), np.array([40000.0, 80000.0]))
     assert np.allclose(sig[0] / sig[1], 16.0, rtol=2e-3)
+
+
+@pytest.fixture(scope="module")
+def hot():
+    setup = W.build_setup(W.grey_atmosphere(15000.0, 4.0), 3000.0, 3001.0, 0.5, 0)
+    return setup, W.continuum_populations(setup)
+
+
+ALL_DEVICE = (1, 2, 3, 4, 5, 6, 7, 8, 12, 13)
+
+
+def test_hydrogen_and_helium_sources_additive(hot):
+    """HOP, HE1OP and HE2OP sit in their own Ifop slots beside the others, on a 15000 K model
+    from the far UV (the He II, He I and H I edges) to the infrared."""
+    s, pops = hot
+    wl = np.concatenate([np.linspace(225.0, 950.0, 17), np.linspace(1200.0, 20000.0, 9)])
+    parts = [ref_xcont(s.temp, s.xne, s.rho, pops, (src,), wl) for src in ALL_DEVICE]
+    total = ref_xcont(s.temp, s.xne, s.rho, pops, ALL_DEVICE, wl)
+    assert np.max(np.abs(sum(parts) - total) / total) <= 1e-14
+    for src in (1, 5, 6):
+        assert np.max(parts[ALL_DEVICE.index(src)]) > 0.0
+
+
+def test_hop_scales_with_neutral_hydrogen_and_drops_levels(hot):
+    """Without H II (no free-free) HOP is proportional to FractionPF(H I), bit for bit for a
+    factor 2; Ifop(1) = -1 removes levels 1-6 only: the n = 1 edge at 911.8 A disappears."""
+    s, pops = hot
+    wl = np.array([300.0, 900.0, 925.0, 3000.0, 8000.0])
+    p0 = dict(pops, fpf_h2=np.zeros_like(pops["fpf_h2"]))
+    one = ref_xcont(s.temp, s.xne, s.rho, p0, (1,), wl)
+    two = ref_xcont(s.temp, s.xne, s.rho, dict(p0, fpf_h1=2.0 * p0["fpf_h1"]), (1,), wl)
+    assert np.array_equal(two, 2.0 * one)
+    full = ref_xcont(s.temp, s.xne, s.rho, pops, (1,), wl)
+    nolev = ref_xcont(s.temp, s.xne, s.rho, pops, (1,), wl, hop_nolev=True)
+    lyman = full - nolev
+    assert np.all(lyman[1] > 10.0 * lyman[2])  # blueward of the Lyman edge: the n = 1 level
+
+
+def test_helium_sources_read_their_populations(hot):
+    """HE2OP vanishes without He II and He III; HE1OP's free-free part is proportional to
+    Fraction(He II) (to rounding of the whole) and nothing else changes with it."""
+    s, pops = hot
+    wl = np.array([230.0, 500.0, 3000.0, 20000.0])
+    zero = np.zeros_like(pops["fpf_he2"])
+    assert not np.any(ref_xcont(s.temp, s.xne, s.rho, dict(pops, fpf_he2=zero, fpf_he3=zero), (6,), wl))
+    a = ref_xcont(s.temp, s.xne, s.rho, dict(pops, frac_he2=zero), (5,), wl)
+    b = ref_xcont(s.temp, s.xne, s.rho, pops, (5,), wl)
+    c = ref_xcont(s.temp, s.xne, s.rho, dict(pops, frac_he2=2.0 * pops["frac_he2"]), (5,), wl)
+    ff = b - a
+    assert np.all(ff >= 0.0) and np.max(ff) > 0.0
+    assert np.all(np.abs((c - a) - 2.0 * ff) <= 1e-13 * c)
