@@ -306,64 +306,55 @@ __global__ void __launch_bounds__(64) k_rt_vjp(DevSetup S, const LopT *lop, cons
 // reverse mode of the line opacity: per (line, depth) the sum over the line's window of
 // dL/dLineOpac x contribution, and the temperature derivative of the contributions
 // ------------------------------------------------------------------------------------
-// H(a, v) of the FP64 path with its partial derivatives: in the table box the bilinear
-// interpolant's slopes; outside it the continued fractions w(z) ~ i z / (sqrt(pi)(z^2 - 1/2))
-// (one pole) and i (z^2 - 1) / (sqrt(pi) z (z^2 - 3/2)) (two poles), H = Re w, z = |v| + i a,
-// dH/dv = Re w', dH/da = -Im w'.
-__device__ __forceinline__ double hvoigt64_d(const DevSetup &S, double A, int ia, double ua, double V, double &Hv,
-                                            double &Ha) {
-  const double av = fabs(V), sg = (V < 0.0) ? -1.0 : 1.0;
-  if (ia > 0 && av < 8.0) {
-    const double fv = av * S.idv;
+// H(a, v) of the table path with its partial derivatives, in FP32 (the reverse mode's
+// general path: gradients are needed to 1e-3, and the per-(line, depth) sums stay FP64): in
+// the table box the bilinear interpolant's slopes; outside it the continued fractions
+// w(z) ~ i z / (sqrt(pi)(z^2 - 1/2)) (one pole) and i (z^2 - 1) / (sqrt(pi) z (z^2 - 3/2))
+// (two poles), H = Re w, z = |v| + i a, dH/dv = Re w', dH/da = -Im w'; one reciprocal per
+// point.
+__device__ __forceinline__ float hvoigt32_d(const DevSetup &S, float A, int ia, float ua, float aida, float V,
+                                            float &Hv, float &Ha) {
+  const float av = fabsf(V), sg = (V < 0.0f) ? -1.0f : 1.0f;
+  const float idv = (float)S.idv;
+  if (ia > 0 && av < 8.0f) {
+    const float fv = av * idv;
     const int iv = (int)fv;
-    const double tv = fv - (double)iv;
-    const float *t0 = S.vtab + (size_t)(ia - 1) * TAB_LD;
-    const float *t1 = t0 + TAB_LD;
-    const double a0 = t0[iv], a1 = t0[iv + 1], b0 = t1[iv], b1 = t1[iv + 1];
-    const double c0 = a0 + tv * (a1 - a0), c1 = b0 + tv * (b1 - b0);
-    Hv = sg * S.idv * ((1.0 - ua) * (a1 - a0) + ua * (b1 - b0));
-    Ha = (c1 - c0) * S.aida[ia - 1];
+    const float tv = fv - (float)iv;
+    const float *t0 = S.vtab + (size_t)(ia - 1) * TAB_LD + iv;
+    const float a0 = t0[0], a1 = t0[1], b0 = t0[TAB_LD], b1 = t0[TAB_LD + 1];
+    const float c0 = a0 + tv * (a1 - a0), c1 = b0 + tv * (b1 - b0);
+    Hv = sg * idv * ((1.0f - ua) * (a1 - a0) + ua * (b1 - b0));
+    Ha = (c1 - c0) * aida;
     return c0 + ua * (c1 - c0);
   }
-  const double a2 = A * A, v2 = av * av, s = v2 + a2;
-  // complex z = av + i A
-  const double zr = av, zi = A;
-  const double z2r = v2 - a2, z2i = 2.0 * av * A;
-  double H, wr, wi;  // w' = wr + i wi
-  // one reciprocal per point: 1/|d|^2 with two Newton steps (full FP64 precision), the
-  // derivative's denominator being |d^2|^2 = |d|^4
-  auto inv = [](double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    r = r * (2.0 - x * r);
-    return r * (2.0 - x * r);
-  };
-  if (s >= 196.0) {
-    // d = z^2 - 1/2, |d|^2 = s^2 - v^2 + a^2 + 1/4
-    const double dr = z2r - 0.5, di = z2i;
-    const double rd = inv(dr * dr + di * di), rdd = rd * rd;
-    H = kSqrtPiInv * A * (s + 0.5) * rd;
-    // w' = -i K (z^2 + 1/2) / (z^2 - 1/2)^2
-    const double nr = z2r + 0.5, ni = z2i;
-    const double d2r = dr * dr - di * di, d2i = 2.0 * dr * di;
-    const double qr = (nr * d2r + ni * d2i) * rdd, qi = (ni * d2r - nr * d2i) * rdd;
-    wr = kSqrtPiInv * qi;
-    wi = -kSqrtPiInv * qr;
+  const float a2 = A * A, v2 = av * av, s = v2 + a2;
+  const float zr = av, zi = A;
+  const float z2r = v2 - a2, z2i = 2.0f * av * A;
+  const float kp = (float)kSqrtPiInv;
+  float H, wr, wi;
+  if (s >= 196.0f) {
+    const float dr = z2r - 0.5f, di = z2i;
+    const float rd = __builtin_amdgcn_rcpf(dr * dr + di * di), rdd = rd * rd;
+    H = kp * A * (s + 0.5f) * rd;
+    const float nr = z2r + 0.5f, ni = z2i;
+    const float d2r = dr * dr - di * di, d2i = 2.0f * dr * di;
+    const float qr = (nr * d2r + ni * d2i) * rdd, qi = (ni * d2r - nr * d2i) * rdd;
+    wr = kp * qi;
+    wi = -kp * qr;
   } else {
-    // g = z^3 - 3z/2 = z (z^2 - 3/2) = dr + i di
-    const double p = v2 - a2 - 1.5, q = 2.0 * av * A;
-    const double dr = av * p - A * q, di = av * q + A * p;
-    const double rg = inv(dr * dr + di * di), rgg = rg * rg;
-    H = kSqrtPiInv * ((p + 0.5) * di - q * dr) * rg;
-    // f = (z^2 - 1) / g; f' = (2z g - (z^2 - 1)(3z^2 - 3/2)) / g^2; w' = i K f'
-    const double gr = dr, gi = di;
-    const double t1r = 2.0 * (zr * gr - zi * gi), t1i = 2.0 * (zr * gi + zi * gr);
-    const double ar = z2r - 1.0, ai = z2i, br = 3.0 * z2r - 1.5, bi = 3.0 * z2i;
-    const double t2r = ar * br - ai * bi, t2i = ar * bi + ai * br;
-    const double nr = t1r - t2r, ni = t1i - t2i;
-    const double g2r = gr * gr - gi * gi, g2i = 2.0 * gr * gi;
-    const double fr = (nr * g2r + ni * g2i) * rgg, fi = (ni * g2r - nr * g2i) * rgg;
-    wr = -kSqrtPiInv * fi;
-    wi = kSqrtPiInv * fr;
+    const float p = v2 - a2 - 1.5f, q = 2.0f * av * A;
+    const float dr = av * p - A * q, di = av * q + A * p;
+    const float rg = __builtin_amdgcn_rcpf(dr * dr + di * di), rgg = rg * rg;
+    H = kp * ((p + 0.5f) * di - q * dr) * rg;
+    const float gr = dr, gi = di;
+    const float t1r = 2.0f * (zr * gr - zi * gi), t1i = 2.0f * (zr * gi + zi * gr);
+    const float ar = z2r - 1.0f, ai = z2i, br = 3.0f * z2r - 1.5f, bi = 3.0f * z2i;
+    const float t2r = ar * br - ai * bi, t2i = ar * bi + ai * br;
+    const float nr = t1r - t2r, ni = t1i - t2i;
+    const float g2r = gr * gr - gi * gi, g2i = 2.0f * gr * gi;
+    const float fr = (nr * g2r + ni * g2i) * rgg, fi = (ni * g2r - nr * g2i) * rgg;
+    wr = -kp * fi;
+    wi = kp * fr;
   }
   Hv = sg * wr;
   Ha = -wi;
@@ -448,15 +439,29 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
   }
 }
 
+// The far-wing test of the reverse mode for a line and a tile: the distance from the line
+// centre to the points it adds to in the tile (window and tile; 0 if the centre lies among
+// them, or for a line with a He I Stark shift).  At a depth whose series threshold dthr it
+// reaches, every such point is in the far wing.  k_lineop_vjp and k_lineop_vjp_mfma split
+// the (line, depth, tile) triples with this value and the comparison dmin >= dthr.
+__device__ __forceinline__ double tile_dmin(const DevSetup &S, const int4 m, double wc, int p0, int p1) {
+  const double Wlo = S.wave[max(m.x, p0) - 1], Whi = S.wave[min(m.y, p1) - 1];
+  return (m.w > 0) ? 0.0 : ((wc < Wlo) ? (Wlo - wc) : (wc > Whi ? (wc - Whi) : 0.0));
+}
+
 // One block per TILE-point wavelength tile, one wavefront per DPW depths (as the FP64
 // accumulation kernel).  For each candidate line the wave reduces sum_p gLop(d, p) c(d, p)
-// per depth into sl[line][d]; sum gLop dc/dT stays in registers per (lane, depth).
+// per depth into sl[line][d]; sum gLop dc/dT stays in registers per (lane, depth).  With
+// far_done set the depths in the far wing over the tile (tile_dmin >= dthr) are the
+// matrix-core kernel's: a wavefront then takes only the others, and skips a line none of
+// whose depths it has.
 __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int nbins, int rstride,
                                                      const int2 *__restrict__ rng, const Coef64 *__restrict__ coef,
                                                      const CoefV *__restrict__ coefv,
                                                      const double *__restrict__ wlc_s,
                                                      const int4 *__restrict__ meta_s, const double *__restrict__ glop,
-                                                     double *__restrict__ sl, double *__restrict__ gtemp) {
+                                                     double *__restrict__ sl, double *__restrict__ gtemp,
+                                                     int far_done) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
   const int N = S.ntau;
@@ -494,11 +499,13 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
       const Coef64 *c = coef + (size_t)j * S.ntp + d0;
       const CoefV *cv = coefv + (size_t)j * S.ntp + d0;
       const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
-      // distance from the line centre to the points the line adds to here (window and tile;
-      // 0 if the centre lies among them): at a depth whose series threshold it reaches, every
-      // such point is in the far wing (wave-uniform)
-      const double Wlo = S.wave[max(m.x, p0) - 1], Whi = S.wave[min(m.y, p1) - 1];
-      const double dmin = hs ? 0.0 : ((wc < Wlo) ? (Wlo - wc) : (wc > Whi ? (wc - Whi) : 0.0));
+      const double dmin = tile_dmin(S, m, wc, p0, p1);  // wave-uniform
+      if (far_done) {  // nothing left here when every depth of the wave is in the far wing
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < DPW; k++) any |= dmin < cv[k].dthr;
+        if (!any) continue;
+      }
       bool act[PPL];
       double dl[PPL], T1[PPL], T2[PPL], T3[PPL];
 #pragma unroll
@@ -519,7 +526,9 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
       for (int k = 0; k < DPW; k++) {
         const CoefV q3 = cv[k];
         double s = 0.0;
-        if (dmin >= q3.dthr) {
+        if (dmin >= q3.dthr && far_done) {
+          // the matrix-core kernel's
+        } else if (dmin >= q3.dthr) {
           // far wing at every point: the series in T with the precomputed coefficients
 #pragma unroll
           for (int q = 0; q < PPL; q++) {
@@ -529,16 +538,19 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
           }
         } else {
           const Coef64 ck = c[k];
+          const float fa = (float)ck.a, fua = (float)ck.ua, fcop = (float)ck.cop;
+          const float faida = ck.ia > 0 ? (float)S.aida[ck.ia - 1] : 0.0f;
+          const float p1 = (float)ck.pad1, p2 = (float)ck.pad2, p3 = (float)ck.pad3;
 #pragma unroll
           for (int q = 0; q < PPL; q++) {
             if (!act[q]) continue;
-            const double v = (hs ? dl[q] - hs[k] : dl[q]) * ck.vv;
-            double Hv, Ha;
-            const double H = hvoigt64_d(S, ck.a, ck.ia, ck.ua, v, Hv, Ha);
-            const double cval = H * ck.cop;
-            s += g[q][k] * cval;
+            const float v = (float)((hs ? dl[q] - hs[k] : dl[q]) * ck.vv);
+            float Hv, Ha;
+            const float H = hvoigt32_d(S, fa, ck.ia, fua, faida, v, Hv, Ha);
+            const float cval = H * fcop;
+            s += g[q][k] * (double)cval;
             // d c / dT = c d ln LinCop/dT + LinCop (Hv v d ln LinVvgt/dT + Ha d LinAvgt/dT)
-            tT[k] += g[q][k] * (cval * ck.pad1 + ck.cop * (Hv * v * ck.pad2 + Ha * ck.pad3));
+            tT[k] += g[q][k] * (double)(cval * p1 + fcop * (Hv * v * p2 + Ha * p3));
           }
         }
         sk[k] = s;
@@ -559,6 +571,129 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
     const double v = wave_sum(tT[k]);
     if (lane == 0 && d0 + k < N && v != 0.0) atomicAdd(&gtemp[d0 + k], v);
   }
+}
+
+typedef float mf4v __attribute__((ext_vector_type(4)));
+
+// Reverse mode of the far-wing series on the matrix cores.  For a tile of TILE points and a
+// group of 16 candidate lines the sums over the tile's points
+//     G_n[d][l] = sum_p gLop(d, p) T(l, p)^n,   n = 1..3,  T = 1/dlambda^2 (0 outside the window)
+// are a product of the tile's gLop (depths x points, FP32 in LDS) with X_n (points x lines,
+// computed per lane): v_mfma_f32_16x16x4f32 with A = 16 depths x 4 points, B = 4 points x 16
+// lines, and the tile's points as the contraction in TILE / 4 steps.  Where the line is in
+// its far wing at the depth over the tile (tile_dmin >= dthr, as k_lineop_vjp decides), the
+// per-(line, depth) sum is sum_n k_n G_n (added to sl) and the temperature derivative's share
+// sum_n m_n G_n (added to the depth's dL/dT); the other (line, depth) pairs are left to
+// k_lineop_vjp.  The four wavefronts take the tile's groups in turn; He I lines (Stark
+// shifted) are never in the far wing here.  Lane layout of v_mfma_f32_16x16x4f32: A[r][k] in
+// lane 16k + r, B[k][c] in lane 16k + c, D[4 (i / 16) + e][i % 16] in element e of lane i.
+template <int NQ>
+__global__ void __launch_bounds__(256) k_lineop_vjp_mfma(DevSetup S, int ntiles, int nbins, int rstride,
+                                                         const int2 *__restrict__ rng, const CoefV *__restrict__ coefv,
+                                                         const double *__restrict__ wlc_s,
+                                                         const int4 *__restrict__ meta_s,
+                                                         const double *__restrict__ glop, double *__restrict__ sl,
+                                                         double *__restrict__ gtemp) {
+  constexpr int LDG = TILE + 4;  // row stride of the gLop tile: the A reads of a step fall two lanes per bank
+  __shared__ float gt[NQ * 16][LDG];
+  __shared__ float ofs_s[TILE];
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  if (t >= ntiles) return;  // whole workgroups leave together (the barrier below)
+  const int N = S.ntau, nwl = S.nwl;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int c = lane & 15, kq = lane >> 4;  // line of the group (B column) / point of the step
+  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, nwl);
+  if (S.batch) gtemp += (size_t)S.pmap[p0 - 1] * N;  // the tile's atmosphere (batch contexts)
+  const double Wt0 = S.wave[p0 - 1];
+  // the tile's dL/dLineOpac (Motype 1: over ContOpacStd, as k_lineop_vjp scales it), zero
+  // past the grid and at the padding depths; the points' offsets from the first point
+  for (int i = threadIdx.x; i < NQ * 16 * TILE; i += blockDim.x) {
+    const int d = i / TILE, pi = i - d * TILE, p = p0 + pi;
+    float v = 0.0f;
+    if (d < N && p <= nwl) {
+      const double inv = (S.motype == 1) ? 1.0 / S.cstd[d] : 1.0;
+      v = (float)(glop[(size_t)d * nwl + p - 1] * inv);
+    }
+    gt[d][pi] = v;
+  }
+  for (int i = threadIdx.x; i < TILE; i += blockDim.x)
+    ofs_s[i] = (p0 + i <= nwl) ? (float)(S.wave[p0 + i - 1] - Wt0) : 0.0f;
+  __syncthreads();
+  double tT[NQ][4];
+#pragma unroll
+  for (int q = 0; q < NQ; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) tT[q][e] = 0.0;
+  int gidx = 0;  // the tile's groups of 16 candidates, in bin order; wavefront w takes w, w + 4, ...
+  for (int b = 0; b < nbins; b++) {
+    const int2 r = rng[(size_t)t * rstride + b];
+    for (int base = r.x; base < r.y; base += 16, gidx++) {
+      if ((gidx & 3) != wave) continue;
+      const int j = (base + c < r.y) ? base + c : -1;
+      int4 m = make_int4(1, 0, 0, 1);
+      double wc = 0.0;
+      if (j >= 0) {
+        m = meta_s[j];
+        wc = wlc_s[j];
+      }
+      const bool meets = j >= 0 && m.w == 0 && m.y >= p0 && m.x <= p1;
+      if (!__builtin_amdgcn_ballot_w64(meets)) continue;
+      const double dmin = meets ? tile_dmin(S, m, wc, p0, p1) : 0.0;
+      // B: T^n at (point, this lane's line); dl = (point - first point) + (first point -
+      // centre) in FP32, relative error below 2^-23 (1 + 2 tile width / |dl|), |dl| >= dthr
+      // wherever the sums are used
+      const float bs = (float)(Wt0 - wc);
+      mf4v C[3][NQ];
+#pragma unroll
+      for (int n = 0; n < 3; n++)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) C[n][q] = mf4v{0.f, 0.f, 0.f, 0.f};
+      for (int st = 0; st < TILE / 4; st++) {
+        const int pi = 4 * st + kq, p = p0 + pi;
+        const bool act = meets && p >= m.x && p <= m.y && p <= nwl;
+        const float x = ofs_s[pi] + bs;
+        const float T1 = act ? __builtin_amdgcn_rcpf(fmaxf(x * x, 1e-12f)) : 0.0f;
+        const float T2 = T1 * T1, T3 = T2 * T1;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+          const float a = gt[16 * q + c][4 * st + kq];
+          C[0][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, T1, C[0][q], 0, 0, 0);
+          C[1][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, T2, C[1][q], 0, 0, 0);
+          C[2][q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, T3, C[2][q], 0, 0, 0);
+        }
+      }
+      // this lane's (line c, depth 16 q + 4 kq + e) sums: the far-wing pairs only
+      if (meets) {
+        const CoefV *cv = coefv + (size_t)j * S.ntp;
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const int d = 16 * q + 4 * kq + e;
+            if (d >= N) continue;
+            const CoefV rec = cv[d];
+            if (!(dmin >= rec.dthr)) continue;
+            const double g1 = C[0][q][e], g2 = C[1][q][e], g3 = C[2][q][e];
+            const double sv = rec.k1 * g1 + rec.k2 * g2 + rec.k3 * g3;
+            if (sv != 0.0) atomicAdd(&sl[(size_t)j * S.ntp + d], sv);
+            tT[q][e] += rec.m1 * g1 + rec.m2 * g2 + rec.m3 * g3;
+          }
+      }
+    }
+  }
+  // dL/dT: the 16 lanes of a line column hold the same depths
+#pragma unroll
+  for (int q = 0; q < NQ; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      double v = tT[q][e];
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 4, 64);
+      v += __shfl_xor(v, 8, 64);
+      const int d = 16 * q + 4 * kq + e;
+      if (c == 0 && d < N && v != 0.0) atomicAdd(&gtemp[d], v);
+    }
 }
 
 // per line: dL/d log10 gf = ln 10 sum_d sl[line][d] (to the line's input index), and
@@ -618,8 +753,31 @@ void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, co
   // the 1024-thread bound keeps 128 registers and 4 wavefronts per SIMD, which outruns the
   // spill-free 138-register build at 3 (256 atmospheres of 2,760 points on average: 86 vs
   // 117 ms)
+  // the far-wing (line, depth, tile) triples on the matrix cores (up to 128 depths; the
+  // width bins, not the He I group, which is the last of the nbins groups)
+  const int nq = (S.ntau + 15) / 16;
+  int far_done = 0;
+  switch (nq) {
+#define FSG_VJP_MFMA(Q)                                                                                   \
+  case Q:                                                                                                \
+    k_lineop_vjp_mfma<Q><<<nb, 256, 0, st>>>(S, ntiles, nbins - 1, rstride, rng, coefv, wlc_s, meta_s, glop, sl, \
+                                             gtemp);                                                     \
+    far_done = 1;                                                                                        \
+    break;
+    FSG_VJP_MFMA(1)
+    FSG_VJP_MFMA(2)
+    FSG_VJP_MFMA(3)
+    FSG_VJP_MFMA(4)
+    FSG_VJP_MFMA(5)
+    FSG_VJP_MFMA(6)
+    FSG_VJP_MFMA(7)
+    FSG_VJP_MFMA(8)
+#undef FSG_VJP_MFMA
+    default:
+      break;
+  }
   k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, coefv, wlc_s, meta_s, glop,
-                                                     sl, gtemp);
+                                                     sl, gtemp, far_done);
 }
 
 void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *atm, const int *slot_s,
