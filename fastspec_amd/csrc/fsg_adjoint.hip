@@ -365,7 +365,7 @@ __device__ __forceinline__ float hvoigt32_d(const DevSetup &S, float A, int ia, 
 // d ln LinCop / dT, d ln LinVvgt / dT and d LinAvgt / dT in the record's spare words
 __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                              const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv, double *wlc_s,
-                             int4 *meta_s, int *slot_s) {
+                             int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
   const DevSetup S = line_setup(S0, L, order[j]);
@@ -437,6 +437,16 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
       cv[k] = q;
     }
   }
+  // the window's end wavelengths (the grid of the line's atmosphere: the same values as the
+  // concatenated grid of a batch holds) and each wavefront's largest threshold
+  const int4 mm = meta_s[j];
+  win[j] = VjpWin{S.wave[mm.x - S.goff - 1], S.wave[mm.y - S.goff - 1]};
+  const int nwv = S.ntp / DPW;
+  for (int w = 0; w < nwv; w++) {
+    double mx = 0.0;
+    for (int k = w * DPW; k < (w + 1) * DPW; k++) mx = fmax(mx, cv[k].dthr);
+    dmaxv[(size_t)j * nwv + w] = mx;
+  }
 }
 
 // The far-wing test of the reverse mode for a line and a tile: the distance from the line
@@ -444,9 +454,11 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
 // them, or for a line with a He I Stark shift).  At a depth whose series threshold dthr it
 // reaches, every such point is in the far wing.  k_lineop_vjp and k_lineop_vjp_mfma split
 // the (line, depth, tile) triples with this value and the comparison dmin >= dthr.
-__device__ __forceinline__ double tile_dmin(const DevSetup &S, const int4 m, double wc, int p0, int p1) {
-  const double Wlo = S.wave[max(m.x, p0) - 1], Whi = S.wave[min(m.y, p1) - 1];
-  return (m.w > 0) ? 0.0 : ((wc < Wlo) ? (Wlo - wc) : (wc > Whi ? (wc - Whi) : 0.0));
+// The window's and the tile's first and last wavelengths give those points: the grid rises
+// within an atmosphere, so the later first point and the earlier last point bound them.
+__device__ __forceinline__ double tile_dmin(const VjpWin w, int he, double wc, double Wt0, double Wt1) {
+  const double Wlo = fmax(w.wlo, Wt0), Whi = fmin(w.whi, Wt1);
+  return (he > 0) ? 0.0 : ((wc < Wlo) ? (Wlo - wc) : (wc > Whi ? (wc - Whi) : 0.0));
 }
 
 // One block per TILE-point wavelength tile, one wavefront per DPW depths (as the FP64
@@ -459,9 +471,10 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
                                                      const int2 *__restrict__ rng, const Coef64 *__restrict__ coef,
                                                      const CoefV *__restrict__ coefv,
                                                      const double *__restrict__ wlc_s,
-                                                     const int4 *__restrict__ meta_s, const double *__restrict__ glop,
-                                                     double *__restrict__ sl, double *__restrict__ gtemp,
-                                                     int far_done) {
+                                                     const int4 *__restrict__ meta_s,
+                                                     const VjpWin *__restrict__ win, const double *__restrict__ dmaxv,
+                                                     const double *__restrict__ glop, double *__restrict__ sl,
+                                                     double *__restrict__ gtemp, int far_done) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
   const int N = S.ntau;
@@ -490,30 +503,42 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
   // the tile's candidates are shared among gridDim.y workgroups (every output is an atomic
   // sum), so that small grids still fill the device
   const int slice = blockIdx.y, nslice = gridDim.y;
+  const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];
+  const int nwv = S.ntp / DPW;
   for (int b = 0; b < nbins; b++) {
     const int2 r = rng[(size_t)t * rstride + b];
+    if (r.x + slice >= r.y) continue;
+    // each candidate's meta, centre, window ends and the wave's largest threshold are
+    // loaded one candidate ahead
+    int jn = r.x + slice;
+    int4 nm = meta_s[jn];
+    double nwc = wlc_s[jn], ndx = dmaxv[(size_t)jn * nwv + wave];
+    VjpWin nw_ = win[jn];
     for (int j = r.x + slice; j < r.y; j += nslice) {
-      const int4 m = meta_s[j];
+      const int4 m = nm;
+      const double wc = nwc, dmx = ndx;
+      const VjpWin ww = nw_;
+      jn = min(j + nslice, r.y - 1);
+      nm = meta_s[jn];
+      nwc = wlc_s[jn];
+      ndx = dmaxv[(size_t)jn * nwv + wave];
+      nw_ = win[jn];
       if (m.y < p0 || m.x > p1) continue;
-      const double wc = wlc_s[j];
+      const double dmin = tile_dmin(ww, m.w, wc, Wt0, Wt1);  // wave-uniform
+      // nothing left here when every depth of the wave is in the far wing
+      if (far_done && dmin >= dmx) continue;
       const Coef64 *c = coef + (size_t)j * S.ntp + d0;
       const CoefV *cv = coefv + (size_t)j * S.ntp + d0;
       const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
-      const double dmin = tile_dmin(S, m, wc, p0, p1);  // wave-uniform
-      if (far_done) {  // nothing left here when every depth of the wave is in the far wing
-        bool any = false;
-#pragma unroll
-        for (int k = 0; k < DPW; k++) any |= dmin < cv[k].dthr;
-        if (!any) continue;
-      }
       bool act[PPL];
       double dl[PPL], T1[PPL], T2[PPL], T3[PPL];
 #pragma unroll
       for (int q = 0; q < PPL; q++) {
         act[q] = valid[q] && pp[q] >= m.x && pp[q] <= m.y;
         dl[q] = Wp[q] - wc;
-        // T = 1/dlambda^2, zero outside the window: the series adds nothing there
-        const double d2 = act[q] ? dl[q] * dl[q] : 1.0;
+        // T = 1/dlambda^2, zero outside the window: the series adds nothing there (unused
+        // when the series is the matrix-core kernel's)
+        const double d2 = (act[q] && !far_done) ? dl[q] * dl[q] : 1.0;
         double r = __builtin_amdgcn_rcp(d2);
         r = r * (2.0 - d2 * r);  // Newton steps to full FP64 precision
         r = r * (2.0 - d2 * r);
@@ -592,6 +617,7 @@ __global__ void __launch_bounds__(256) k_lineop_vjp_mfma(DevSetup S, int ntiles,
                                                          const int2 *__restrict__ rng, const CoefV *__restrict__ coefv,
                                                          const double *__restrict__ wlc_s,
                                                          const int4 *__restrict__ meta_s,
+                                                         const VjpWin *__restrict__ win,
                                                          const double *__restrict__ glop, double *__restrict__ sl,
                                                          double *__restrict__ gtemp) {
   constexpr int LDG = TILE + 4;  // row stride of the gLop tile: the A reads of a step fall two lanes per bank
@@ -604,7 +630,7 @@ __global__ void __launch_bounds__(256) k_lineop_vjp_mfma(DevSetup S, int ntiles,
   const int c = lane & 15, kq = lane >> 4;  // line of the group (B column) / point of the step
   const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, nwl);
   if (S.batch) gtemp += (size_t)S.pmap[p0 - 1] * N;  // the tile's atmosphere (batch contexts)
-  const double Wt0 = S.wave[p0 - 1];
+  const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];
   // the tile's dL/dLineOpac (Motype 1: over ContOpacStd, as k_lineop_vjp scales it), zero
   // past the grid and at the padding depths; the points' offsets from the first point
   for (int i = threadIdx.x; i < NQ * 16 * TILE; i += blockDim.x) {
@@ -632,13 +658,15 @@ __global__ void __launch_bounds__(256) k_lineop_vjp_mfma(DevSetup S, int ntiles,
       const int j = (base + c < r.y) ? base + c : -1;
       int4 m = make_int4(1, 0, 0, 1);
       double wc = 0.0;
+      VjpWin ww{0.0, 0.0};
       if (j >= 0) {
         m = meta_s[j];
         wc = wlc_s[j];
+        ww = win[j];
       }
       const bool meets = j >= 0 && m.w == 0 && m.y >= p0 && m.x <= p1;
       if (!__builtin_amdgcn_ballot_w64(meets)) continue;
-      const double dmin = meets ? tile_dmin(S, m, wc, p0, p1) : 0.0;
+      const double dmin = meets ? tile_dmin(ww, m.w, wc, Wt0, Wt1) : 0.0;
       // B: T^n at (point, this lane's line); dl = (point - first point) + (first point -
       // centre) in FP32, relative error below 2^-23 (1 + 2 tile width / |dl|), |dl| >= dthr
       // wherever the sums are used
@@ -734,15 +762,15 @@ void launch_rt_vjp(const DevSetup &S, const void *lop, int fp32, const double *g
 
 void launch_fill64_vjp(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                        const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv,
-                       double *wlc_s, int4 *meta_s, int *slot_s, hipStream_t st) {
+                       double *wlc_s, int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv, hipStream_t st) {
   if (nret)
     k_fill64_vjp<<<nb(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, coefv, wlc_s,
-                                                meta_s, slot_s);
+                                                meta_s, slot_s, win, dmaxv);
 }
 
 void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
-                       const CoefV *coefv, const double *wlc_s, const int4 *meta_s, const double *glop, double *sl,
-                       double *gtemp, hipStream_t st) {
+                       const CoefV *coefv, const double *wlc_s, const int4 *meta_s, const VjpWin *win,
+                       const double *dmaxv, const double *glop, double *sl, double *gtemp, hipStream_t st) {
   if (ntiles <= 0 || nbins <= 0) return;
   const int nw = S.ntp / DPW;
   const int nb = 8 * ((ntiles + 7) / 8);
@@ -760,8 +788,8 @@ void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, co
   switch (nq) {
 #define FSG_VJP_MFMA(Q)                                                                                   \
   case Q:                                                                                                \
-    k_lineop_vjp_mfma<Q><<<nb, 256, 0, st>>>(S, ntiles, nbins - 1, rstride, rng, coefv, wlc_s, meta_s, glop, sl, \
-                                             gtemp);                                                     \
+    k_lineop_vjp_mfma<Q><<<nb, 256, 0, st>>>(S, ntiles, nbins - 1, rstride, rng, coefv, wlc_s, meta_s, win, glop, \
+                                             sl, gtemp);                                                 \
     far_done = 1;                                                                                        \
     break;
     FSG_VJP_MFMA(1)
@@ -776,8 +804,8 @@ void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, co
     default:
       break;
   }
-  k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, coefv, wlc_s, meta_s, glop,
-                                                     sl, gtemp, far_done);
+  k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, coefv, wlc_s, meta_s, win,
+                                                     dmaxv, glop, sl, gtemp, far_done);
 }
 
 void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *atm, const int *slot_s,

@@ -2008,12 +2008,17 @@ int lineop_vjp_run(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
       std::vector<int64_t> goff, gcnt;
       std::vector<int> ghmax;
       group_tables(P, goff, gcnt, ghmax);
-      const size_t per =
-          S.ntp * (sizeof(Coef64) + sizeof(CoefV) + sizeof(double)) + sizeof(double) + sizeof(int4) + sizeof(int);
+      const int nwv = S.ntp / DPW;
+      const size_t per = S.ntp * (sizeof(Coef64) + sizeof(CoefV) + sizeof(double)) + sizeof(double) + sizeof(int4) +
+                         sizeof(int) + sizeof(VjpWin) + nwv * sizeof(double);
       const int64_t chunk = chunk_lines(c, P.nret, nlines_lim, per, 0);
-      // the chunk's Coef64 records, then its far-wing series records
-      CK(c->coef.ensure((size_t)chunk * S.ntp * (sizeof(Coef64) + sizeof(CoefV))));
+      // the chunk's Coef64 records, its far-wing series records, the window records and the
+      // wavefronts' largest thresholds
+      CK(c->coef.ensure((size_t)chunk * (S.ntp * (sizeof(Coef64) + sizeof(CoefV)) + sizeof(VjpWin) +
+                                         nwv * sizeof(double))));
       CoefV *coefv = reinterpret_cast<CoefV *>(c->coef.as<char>() + (size_t)chunk * S.ntp * sizeof(Coef64));
+      VjpWin *vwin = reinterpret_cast<VjpWin *>(coefv + (size_t)chunk * S.ntp);
+      double *dmaxv = reinterpret_cast<double *>(vwin + chunk);
       CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
       CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
       CK(c->vsl.ensure((size_t)chunk * S.ntp * sizeof(double)));
@@ -2030,12 +2035,12 @@ int lineop_vjp_run(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
         const int *order = c->idx2.as<int>() + s0;
         launch_fill64_vjp(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
                           c->ic.as<int>(), c->coef.as<Coef64>(), coefv, c->wlc.as<double>(), c->meta.as<int4>(),
-                          c->vslot.as<int>(), s);
+                          c->vslot.as<int>(), vwin, dmaxv, s);
         launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
                            c->meta.as<int4>(), c->rng.as<int2>(), s);
         CK(hipMemsetAsync(c->vsl.p, 0, (size_t)m * S.ntp * sizeof(double), s));
         launch_lineop_vjp(S, ntiles, ng, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), coefv, c->wlc.as<double>(),
-                          c->meta.as<int4>(), c->glop, c->vsl.as<double>(), dgt, s);
+                          c->meta.as<int4>(), vwin, dmaxv, c->glop, c->vsl.as<double>(), dgt, s);
         launch_lineop_vjp_scatter(S, m, order, dl->L.atm, c->vslot.as<int>(), c->vsl.as<double>(), dgf, dfpf, s);
       }
       CK(hipGetLastError());

@@ -91,6 +91,12 @@ struct alignas(64) CoefV {
   double k1, k2, k3, m1, m2, m3, dthr, pad;
 };
 
+// Per-line record of the reverse mode's far-wing test: the wavelengths of the first and last
+// points of the line's window, as the grid holds them.
+struct alignas(16) VjpWin {
+  double wlo, whi;
+};
+
 // Everything the kernels read from the set-up (device pointers + scalars).
 struct DevSetup {
   int ntau, ntp, nwl, nmu, wmode, motype, pf_ld, nslot, nspc, he1_slot, nhe, molbrd;
