@@ -365,7 +365,7 @@ __device__ __forceinline__ float hvoigt32_d(const DevSetup &S, float A, int ia, 
 // d ln LinCop / dT, d ln LinVvgt / dT and d LinAvgt / dT in the record's spare words
 __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                              const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv, double *wlc_s,
-                             int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv) {
+                             int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv, CoefR *coefr) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
   const DevSetup S = line_setup(S0, L, order[j]);
@@ -375,11 +375,13 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
   const int N = S.ntau;
   Coef64 *c = coef + (size_t)j * S.ntp;
   CoefV *cv = coefv + (size_t)j * S.ntp;
+  CoefR *cr = coefr + (size_t)j * S.ntp;
   for (int k = N; k < S.ntp; k++) {
     Coef64 r{};
     r.ia = 1;
     c[k] = r;
     cv[k] = CoefV{};  // padding depths: the series with zero coefficients everywhere
+    cr[k] = CoefR{};
   }
   for (int k = 0; k < N; k++) {
     double cop, vv, av, ua;
@@ -435,6 +437,18 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
       q.dthr = sqrt(fmax(196.0, 400.0 * be)) / vv * (1.0 + 1e-12);
       q.pad = 0.0;
       cv[k] = q;
+      CoefR g;
+      g.dthr = q.dthr;
+      g.vv = vv;
+      g.a = (float)av;
+      g.cop = (float)cop;
+      g.ua = (float)ua;
+      g.aida = ia > 0 ? (float)S.aida[ia - 1] : 0.0f;
+      g.p1 = (float)r.pad1;
+      g.p2 = (float)r.pad2;
+      g.p3 = (float)r.pad3;
+      g.ia = ia;
+      cr[k] = g;
     }
   }
   // the window's end wavelengths (the grid of the line's atmosphere: the same values as the
@@ -461,8 +475,8 @@ __device__ __forceinline__ double tile_dmin(const VjpWin w, int he, double wc, d
   return (he > 0) ? 0.0 : ((wc < Wlo) ? (Wlo - wc) : (wc > Whi ? (wc - Whi) : 0.0));
 }
 
-// One block per TILE-point wavelength tile, one wavefront per DPW depths (as the FP64
-// accumulation kernel).  For each candidate line the wave reduces sum_p gLop(d, p) c(d, p)
+// One block per VJP_TILE-point wavelength tile, one wavefront per DPW depths, VJP_PPL
+// points per lane.  For each candidate line the wave reduces sum_p gLop(d, p) c(d, p)
 // per depth into sl[line][d]; sum gLop dc/dT stays in registers per (lane, depth).  With
 // far_done set the depths in the far wing over the tile (tile_dmin >= dthr) are the
 // matrix-core kernel's: a wavefront then takes only the others, and skips a line none of
@@ -479,26 +493,26 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
   if (t >= ntiles) return;
   const int N = S.ntau;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, S.nwl);
+  const int p0 = t * VJP_TILE + 1, p1 = min(p0 + VJP_TILE - 1, S.nwl);
   if (S.batch) gtemp += (size_t)S.pmap[p0 - 1] * N;  // the tile's atmosphere (batch contexts)
-  int pp[PPL];
-  bool valid[PPL];
-  double Wp[PPL];
+  int pp[VJP_PPL];
+  bool valid[VJP_PPL];
+  double Wp[VJP_PPL];
 #pragma unroll
-  for (int q = 0; q < PPL; q++) {
+  for (int q = 0; q < VJP_PPL; q++) {
     pp[q] = p0 + lane + 64 * q;
     valid[q] = pp[q] <= S.nwl;
     Wp[q] = valid[q] ? S.wave[pp[q] - 1] : 0.0;
   }
   const int d0 = wave * DPW;
-  double g[PPL][DPW], tT[DPW];
+  double g[VJP_PPL][DPW], tT[DPW];
 #pragma unroll
   for (int k = 0; k < DPW; k++) {
     const int d = d0 + k;
     const double inv = (d < N && S.motype == 1) ? 1.0 / S.cstd[d] : 1.0;
     tT[k] = 0.0;
 #pragma unroll
-    for (int q = 0; q < PPL; q++) g[q][k] = (valid[q] && d < N) ? glop[(size_t)d * S.nwl + pp[q] - 1] * inv : 0.0;
+    for (int q = 0; q < VJP_PPL; q++) g[q][k] = (valid[q] && d < N) ? glop[(size_t)d * S.nwl + pp[q] - 1] * inv : 0.0;
   }
   // the tile's candidates are shared among gridDim.y workgroups (every output is an atomic
   // sum), so that small grids still fill the device
@@ -530,10 +544,10 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
       const Coef64 *c = coef + (size_t)j * S.ntp + d0;
       const CoefV *cv = coefv + (size_t)j * S.ntp + d0;
       const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
-      bool act[PPL];
-      double dl[PPL], T1[PPL], T2[PPL], T3[PPL];
+      bool act[VJP_PPL];
+      double dl[VJP_PPL], T1[VJP_PPL], T2[VJP_PPL], T3[VJP_PPL];
 #pragma unroll
-      for (int q = 0; q < PPL; q++) {
+      for (int q = 0; q < VJP_PPL; q++) {
         act[q] = valid[q] && pp[q] >= m.x && pp[q] <= m.y;
         dl[q] = Wp[q] - wc;
         // T = 1/dlambda^2, zero outside the window: the series adds nothing there (unused
@@ -556,7 +570,7 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
         } else if (dmin >= q3.dthr) {
           // far wing at every point: the series in T with the precomputed coefficients
 #pragma unroll
-          for (int q = 0; q < PPL; q++) {
+          for (int q = 0; q < VJP_PPL; q++) {
             const double x1 = g[q][k] * T1[q], x2 = g[q][k] * T2[q], x3 = g[q][k] * T3[q];
             s += q3.k1 * x1 + q3.k2 * x2 + q3.k3 * x3;
             tT[k] += q3.m1 * x1 + q3.m2 * x2 + q3.m3 * x3;
@@ -567,7 +581,7 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
           const float faida = ck.ia > 0 ? (float)S.aida[ck.ia - 1] : 0.0f;
           const float p1 = (float)ck.pad1, p2 = (float)ck.pad2, p3 = (float)ck.pad3;
 #pragma unroll
-          for (int q = 0; q < PPL; q++) {
+          for (int q = 0; q < VJP_PPL; q++) {
             if (!act[q]) continue;
             const float v = (float)((hs ? dl[q] - hs[k] : dl[q]) * ck.vv);
             float Hv, Ha;
@@ -598,14 +612,136 @@ __global__ void __launch_bounds__(1024) k_lineop_vjp(DevSetup S, int ntiles, int
   }
 }
 
+// The general (line, depth, tile) triples of the reverse mode when the far-wing ones are the
+// matrix-core kernel's: as k_lineop_vjp, but a wavefront reads the nine CoefR records of its
+// depths with one load per lane (lane k: depth k) as soon as it takes a line, finds its
+// general depths by one ballot (dmin < dthr, the test of k_lineop_vjp_mfma) and hands each
+// general depth's record to all lanes by lane reads.
+__global__ void __launch_bounds__(1024) k_lineop_vjp_gen(DevSetup S, int ntiles, int nbins, int rstride,
+                                                         const int2 *__restrict__ rng, const CoefR *__restrict__ coefr,
+                                                         const double *__restrict__ wlc_s,
+                                                         const int4 *__restrict__ meta_s,
+                                                         const VjpWin *__restrict__ win,
+                                                         const double *__restrict__ dmaxv,
+                                                         const double *__restrict__ glop, double *__restrict__ sl,
+                                                         double *__restrict__ gtemp) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  if (t >= ntiles) return;
+  const int N = S.ntau;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int p0 = t * VJP_TILE + 1, p1 = min(p0 + VJP_TILE - 1, S.nwl);
+  if (S.batch) gtemp += (size_t)S.pmap[p0 - 1] * N;  // the tile's atmosphere (batch contexts)
+  int pp[VJP_PPL];
+  bool valid[VJP_PPL];
+  double Wp[VJP_PPL];
+#pragma unroll
+  for (int q = 0; q < VJP_PPL; q++) {
+    pp[q] = p0 + lane + 64 * q;
+    valid[q] = pp[q] <= S.nwl;
+    Wp[q] = valid[q] ? S.wave[pp[q] - 1] : 0.0;
+  }
+  const int d0 = wave * DPW;
+  double g[VJP_PPL][DPW], tT[DPW];
+#pragma unroll
+  for (int k = 0; k < DPW; k++) {
+    const int d = d0 + k;
+    const double inv = (d < N && S.motype == 1) ? 1.0 / S.cstd[d] : 1.0;
+    tT[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < VJP_PPL; q++) g[q][k] = (valid[q] && d < N) ? glop[(size_t)d * S.nwl + pp[q] - 1] * inv : 0.0;
+  }
+  const int slice = blockIdx.y, nslice = gridDim.y;
+  const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];
+  const int nwv = S.ntp / DPW;
+  const int kl = lane < DPW ? lane : 0;  // the depth whose record this lane reads
+  for (int b = 0; b < nbins; b++) {
+    const int2 r = rng[(size_t)t * rstride + b];
+    if (r.x + slice >= r.y) continue;
+    // each candidate's meta, centre, window ends and the wave's largest threshold are
+    // loaded one candidate ahead
+    int jn = r.x + slice;
+    int4 nm = meta_s[jn];
+    double nwc = wlc_s[jn], ndx = dmaxv[(size_t)jn * nwv + wave];
+    VjpWin nw_ = win[jn];
+    for (int j = r.x + slice; j < r.y; j += nslice) {
+      const int4 m = nm;
+      const double wc = nwc, dmx = ndx;
+      const VjpWin ww = nw_;
+      jn = min(j + nslice, r.y - 1);
+      nm = meta_s[jn];
+      nwc = wlc_s[jn];
+      ndx = dmaxv[(size_t)jn * nwv + wave];
+      nw_ = win[jn];
+      if (m.y < p0 || m.x > p1) continue;
+      const double dmin = tile_dmin(ww, m.w, wc, Wt0, Wt1);  // wave-uniform
+      if (dmin >= dmx) continue;  // every depth of the wave in the far wing
+      const CoefR rec = coefr[(size_t)j * S.ntp + d0 + kl];
+      const uint64_t gen = __builtin_amdgcn_ballot_w64(lane < DPW && dmin < rec.dthr);
+      const double *hs = (m.w > 0) ? S.he_shf + (size_t)(m.w - 1) * S.ntp + d0 : nullptr;
+      bool act[VJP_PPL];
+      double dl[VJP_PPL];
+#pragma unroll
+      for (int q = 0; q < VJP_PPL; q++) {
+        act[q] = valid[q] && pp[q] >= m.x && pp[q] <= m.y;
+        dl[q] = Wp[q] - wc;
+      }
+      double sk[DPW];
+#pragma unroll
+      for (int k = 0; k < DPW; k++) {
+        double s = 0.0;
+        if ((gen >> k) & 1) {
+          // depth k's record from lane k
+          const double vv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(rec.vv), k),
+                                             __builtin_amdgcn_readlane(__double2loint(rec.vv), k));
+          const float fa = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.a), k));
+          const float fcop = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.cop), k));
+          const float fua = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.ua), k));
+          const float faida = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.aida), k));
+          const float p1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.p1), k));
+          const float p2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.p2), k));
+          const float p3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rec.p3), k));
+          const int ia = __builtin_amdgcn_readlane(rec.ia, k);
+#pragma unroll
+          for (int q = 0; q < VJP_PPL; q++) {
+            if (!act[q]) continue;
+            const float v = (float)((hs ? dl[q] - hs[k] : dl[q]) * vv);
+            float Hv, Ha;
+            const float H = hvoigt32_d(S, fa, ia, fua, faida, v, Hv, Ha);
+            const float cval = H * fcop;
+            s += g[q][k] * (double)cval;
+            // d c / dT = c d ln LinCop/dT + LinCop (Hv v d ln LinVvgt/dT + Ha d LinAvgt/dT)
+            tT[k] += g[q][k] * (double)(cval * p1 + fcop * (Hv * v * p2 + Ha * p3));
+          }
+        }
+        sk[k] = s;
+      }
+      static_assert(DPW == 9, "eight depths in one reduction and one more");
+      double s8[8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) s8[k] = sk[k];
+      const double tt = wave_sum8(s8, lane);
+      if ((lane & 7) == 0 && tt != 0.0) atomicAdd(&sl[(size_t)j * S.ntp + d0 + (lane >> 3)], tt);
+      if ((gen >> 8) & 1) {
+        const double t8 = wave_sum(sk[8]);
+        if (lane == 0 && t8 != 0.0) atomicAdd(&sl[(size_t)j * S.ntp + d0 + 8], t8);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < DPW; k++) {
+    const double v = wave_sum(tT[k]);
+    if (lane == 0 && d0 + k < N && v != 0.0) atomicAdd(&gtemp[d0 + k], v);
+  }
+}
+
 typedef float mf4v __attribute__((ext_vector_type(4)));
 
-// Reverse mode of the far-wing series on the matrix cores.  For a tile of TILE points and a
+// Reverse mode of the far-wing series on the matrix cores.  For a tile of VJP_TILE points and a
 // group of 16 candidate lines the sums over the tile's points
 //     G_n[d][l] = sum_p gLop(d, p) T(l, p)^n,   n = 1..3,  T = 1/dlambda^2 (0 outside the window)
 // are a product of the tile's gLop (depths x points, FP32 in LDS) with X_n (points x lines,
 // computed per lane): v_mfma_f32_16x16x4f32 with A = 16 depths x 4 points, B = 4 points x 16
-// lines, and the tile's points as the contraction in TILE / 4 steps.  Where the line is in
+// lines, and the tile's points as the contraction in VJP_TILE / 4 steps.  Where the line is in
 // its far wing at the depth over the tile (tile_dmin >= dthr, as k_lineop_vjp decides), the
 // per-(line, depth) sum is sum_n k_n G_n (added to sl) and the temperature derivative's share
 // sum_n m_n G_n (added to the depth's dL/dT); the other (line, depth) pairs are left to
@@ -620,21 +756,21 @@ __global__ void __launch_bounds__(256) k_lineop_vjp_mfma(DevSetup S, int ntiles,
                                                          const VjpWin *__restrict__ win,
                                                          const double *__restrict__ glop, double *__restrict__ sl,
                                                          double *__restrict__ gtemp) {
-  constexpr int LDG = TILE + 4;  // row stride of the gLop tile: the A reads of a step fall two lanes per bank
+  constexpr int LDG = VJP_TILE + 4;  // row stride of the gLop tile: the A reads of a step fall two lanes per bank
   __shared__ float gt[NQ * 16][LDG];
-  __shared__ float ofs_s[TILE];
+  __shared__ float ofs_s[VJP_TILE];
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;  // whole workgroups leave together (the barrier below)
   const int N = S.ntau, nwl = S.nwl;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int c = lane & 15, kq = lane >> 4;  // line of the group (B column) / point of the step
-  const int p0 = t * TILE + 1, p1 = min(p0 + TILE - 1, nwl);
+  const int p0 = t * VJP_TILE + 1, p1 = min(p0 + VJP_TILE - 1, nwl);
   if (S.batch) gtemp += (size_t)S.pmap[p0 - 1] * N;  // the tile's atmosphere (batch contexts)
   const double Wt0 = S.wave[p0 - 1], Wt1 = S.wave[p1 - 1];
   // the tile's dL/dLineOpac (Motype 1: over ContOpacStd, as k_lineop_vjp scales it), zero
   // past the grid and at the padding depths; the points' offsets from the first point
-  for (int i = threadIdx.x; i < NQ * 16 * TILE; i += blockDim.x) {
-    const int d = i / TILE, pi = i - d * TILE, p = p0 + pi;
+  for (int i = threadIdx.x; i < NQ * 16 * VJP_TILE; i += blockDim.x) {
+    const int d = i / VJP_TILE, pi = i - d * VJP_TILE, p = p0 + pi;
     float v = 0.0f;
     if (d < N && p <= nwl) {
       const double inv = (S.motype == 1) ? 1.0 / S.cstd[d] : 1.0;
@@ -642,7 +778,7 @@ __global__ void __launch_bounds__(256) k_lineop_vjp_mfma(DevSetup S, int ntiles,
     }
     gt[d][pi] = v;
   }
-  for (int i = threadIdx.x; i < TILE; i += blockDim.x)
+  for (int i = threadIdx.x; i < VJP_TILE; i += blockDim.x)
     ofs_s[i] = (p0 + i <= nwl) ? (float)(S.wave[p0 + i - 1] - Wt0) : 0.0f;
   __syncthreads();
   double tT[NQ][4];
@@ -676,7 +812,7 @@ __global__ void __launch_bounds__(256) k_lineop_vjp_mfma(DevSetup S, int ntiles,
       for (int n = 0; n < 3; n++)
 #pragma unroll
         for (int q = 0; q < NQ; q++) C[n][q] = mf4v{0.f, 0.f, 0.f, 0.f};
-      for (int st = 0; st < TILE / 4; st++) {
+      for (int st = 0; st < VJP_TILE / 4; st++) {
         const int pi = 4 * st + kq, p = p0 + pi;
         const bool act = meets && p >= m.x && p <= m.y && p <= nwl;
         const float x = ofs_s[pi] + bs;
@@ -762,15 +898,17 @@ void launch_rt_vjp(const DevSetup &S, const void *lop, int fp32, const double *g
 
 void launch_fill64_vjp(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                        const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv,
-                       double *wlc_s, int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv, hipStream_t st) {
+                       double *wlc_s, int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv, CoefR *coefr,
+                       hipStream_t st) {
   if (nret)
     k_fill64_vjp<<<nb(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, coefv, wlc_s,
-                                                meta_s, slot_s, win, dmaxv);
+                                                meta_s, slot_s, win, dmaxv, coefr);
 }
 
 void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
-                       const CoefV *coefv, const double *wlc_s, const int4 *meta_s, const VjpWin *win,
-                       const double *dmaxv, const double *glop, double *sl, double *gtemp, hipStream_t st) {
+                       const CoefV *coefv, const CoefR *coefr, const double *wlc_s, const int4 *meta_s,
+                       const VjpWin *win, const double *dmaxv, const double *glop, double *sl, double *gtemp,
+                       hipStream_t st) {
   if (ntiles <= 0 || nbins <= 0) return;
   const int nw = S.ntp / DPW;
   const int nb = 8 * ((ntiles + 7) / 8);
@@ -804,8 +942,12 @@ void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, co
     default:
       break;
   }
-  k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, coefv, wlc_s, meta_s, win,
-                                                     dmaxv, glop, sl, gtemp, far_done);
+  if (far_done)
+    k_lineop_vjp_gen<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coefr, wlc_s, meta_s, win,
+                                                           dmaxv, glop, sl, gtemp);
+  else
+    k_lineop_vjp<<<dim3(nb, nslice), 64 * nw, 0, st>>>(S, ntiles, nbins, rstride, rng, coef, coefv, wlc_s, meta_s, win,
+                                                       dmaxv, glop, sl, gtemp, 0);
 }
 
 void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *atm, const int *slot_s,

@@ -2009,21 +2009,22 @@ int lineop_vjp_run(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
       std::vector<int> ghmax;
       group_tables(P, goff, gcnt, ghmax);
       const int nwv = S.ntp / DPW;
-      const size_t per = S.ntp * (sizeof(Coef64) + sizeof(CoefV) + sizeof(double)) + sizeof(double) + sizeof(int4) +
-                         sizeof(int) + sizeof(VjpWin) + nwv * sizeof(double);
+      const size_t per = S.ntp * (sizeof(Coef64) + sizeof(CoefV) + sizeof(CoefR) + sizeof(double)) + sizeof(double) +
+                         sizeof(int4) + sizeof(int) + sizeof(VjpWin) + nwv * sizeof(double);
       const int64_t chunk = chunk_lines(c, P.nret, nlines_lim, per, 0);
-      // the chunk's Coef64 records, its far-wing series records, the window records and the
-      // wavefronts' largest thresholds
-      CK(c->coef.ensure((size_t)chunk * (S.ntp * (sizeof(Coef64) + sizeof(CoefV)) + sizeof(VjpWin) +
+      // the chunk's Coef64 records, its far-wing series records, its general records, the
+      // window records and the wavefronts' largest thresholds
+      CK(c->coef.ensure((size_t)chunk * (S.ntp * (sizeof(Coef64) + sizeof(CoefV) + sizeof(CoefR)) + sizeof(VjpWin) +
                                          nwv * sizeof(double))));
       CoefV *coefv = reinterpret_cast<CoefV *>(c->coef.as<char>() + (size_t)chunk * S.ntp * sizeof(Coef64));
-      VjpWin *vwin = reinterpret_cast<VjpWin *>(coefv + (size_t)chunk * S.ntp);
+      CoefR *coefr = reinterpret_cast<CoefR *>(coefv + (size_t)chunk * S.ntp);
+      VjpWin *vwin = reinterpret_cast<VjpWin *>(coefr + (size_t)chunk * S.ntp);
       double *dmaxv = reinterpret_cast<double *>(vwin + chunk);
       CK(c->wlc.ensure((size_t)chunk * sizeof(double)));
       CK(c->meta.ensure((size_t)chunk * sizeof(int4)));
       CK(c->vsl.ensure((size_t)chunk * S.ntp * sizeof(double)));
       CK(c->vslot.ensure((size_t)chunk * sizeof(int)));
-      const int ntiles = (S.nwl + TILE - 1) / TILE;
+      const int ntiles = (S.nwl + VJP_TILE - 1) / VJP_TILE;
       CK(c->rng.ensure((size_t)ntiles * ng * sizeof(int2)));
       std::vector<std::vector<int>> tables;
       for (int64_t s0 = 0; s0 < P.nret; s0 += chunk) {
@@ -2035,12 +2036,12 @@ int lineop_vjp_run(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, double *glogg
         const int *order = c->idx2.as<int>() + s0;
         launch_fill64_vjp(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
                           c->ic.as<int>(), c->coef.as<Coef64>(), coefv, c->wlc.as<double>(), c->meta.as<int4>(),
-                          c->vslot.as<int>(), vwin, dmaxv, s);
-        launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
+                          c->vslot.as<int>(), vwin, dmaxv, coefr, s);
+        launch_tile_ranges(ntiles, VJP_TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
                            c->meta.as<int4>(), c->rng.as<int2>(), s);
         CK(hipMemsetAsync(c->vsl.p, 0, (size_t)m * S.ntp * sizeof(double), s));
-        launch_lineop_vjp(S, ntiles, ng, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), coefv, c->wlc.as<double>(),
-                          c->meta.as<int4>(), vwin, dmaxv, c->glop, c->vsl.as<double>(), dgt, s);
+        launch_lineop_vjp(S, ntiles, ng, ng, c->rng.as<int2>(), c->coef.as<Coef64>(), coefv, coefr,
+                          c->wlc.as<double>(), c->meta.as<int4>(), vwin, dmaxv, c->glop, c->vsl.as<double>(), dgt, s);
         launch_lineop_vjp_scatter(S, m, order, dl->L.atm, c->vslot.as<int>(), c->vsl.as<double>(), dgf, dfpf, s);
       }
       CK(hipGetLastError());

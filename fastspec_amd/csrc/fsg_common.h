@@ -91,6 +91,17 @@ struct alignas(64) CoefV {
   double k1, k2, k3, m1, m2, m3, dthr, pad;
 };
 
+// Per-(line, depth) record of the reverse mode's general path, in the precision it is
+// evaluated in (FP32), with the far-wing threshold dthr (FP64, the value of CoefV) that decides
+// which kernel takes the depth.  48 bytes: one wavefront loads nine of them with three
+// 16-byte loads, one record per lane.
+struct alignas(16) CoefR {
+  double dthr, vv;
+  float a, cop, ua, aida, p1, p2, p3;  // p1..p3: d ln LinCop/dT, d ln LinVvgt/dT, d LinAvgt/dT
+  int ia;
+};
+static_assert(sizeof(CoefR) == 48, "three 16-byte loads per record");
+
 // Per-line record of the reverse mode's far-wing test: the wavelengths of the first and last
 // points of the line's window, as the grid holds them.
 struct alignas(16) VjpWin {

@@ -88,14 +88,21 @@ void launch_stokes(const DevSetup &S, const void *lop, const void *mlop, int fp3
 size_t rt_vjp_scratch_per_point(const DevSetup &S);
 void launch_rt_vjp(const DevSetup &S, const void *lop, int fp32, const double *gsfl, const double *gsfc, int p_lo,
                    int np, double *scr, double *glop, double *gcont, double *gtemp, hipStream_t st);
+// the reverse mode's tiles: 128 points, two per lane (64-point tiles leave fewer points to
+// the general path but double the matrix-core kernel's per-tile work: 112 against 108.5 ms
+// for the differentiable batch's forward + adjoint)
+constexpr int VJP_PPL = 2;
+constexpr int VJP_TILE = 64 * VJP_PPL;
 // the reverse mode's records of a chunk; win [line] and dmaxv [line][ntp / DPW] (the largest
 // far-wing threshold of each wavefront's depths) feed the test that splits the two kernels
 void launch_fill64_vjp(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                        const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv,
-                       double *wlc_s, int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv, hipStream_t st);
+                       double *wlc_s, int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv, CoefR *coefr,
+                       hipStream_t st);
 void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,
-                       const CoefV *coefv, const double *wlc_s, const int4 *meta_s, const VjpWin *win,
-                       const double *dmaxv, const double *glop, double *sl, double *gtemp, hipStream_t st);
+                       const CoefV *coefv, const CoefR *coefr, const double *wlc_s, const int4 *meta_s,
+                       const VjpWin *win, const double *dmaxv, const double *glop, double *sl, double *gtemp,
+                       hipStream_t st);
 void launch_lineop_vjp_scatter(const DevSetup &S, int64_t nret, const int *order, const int *atm, const int *slot_s,
                                const double *sl, double *g_loggf, double *g_fpf, hipStream_t st);
 
