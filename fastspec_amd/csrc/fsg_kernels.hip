@@ -916,8 +916,9 @@ __global__ void __launch_bounds__(256) k_opacity32_mfma(DevSetup S, int ntiles, 
 // inside its window, at the points of zone and window, every depth.  Per (line, depth) the
 // far-wing expansion still applies where the whole tile lies beyond dthr (straight-line FMA
 // code); the other depths run the Voigt table / continued fractions (contrib32).  Same layout
-// as k_opacity32 (a tile of 64 CP points, one wavefront per DPW depths, CP points per lane,
-// per-(line, depth) records through scalar loads).  The candidates come bin by bin from the
+// as k_opacity32 (a tile of 64 CP points, one wavefront per DPW depths, CP points per lane);
+// a line's records of the wavefront's DPW depths come in one vector load (lane k holds depth
+// k) and reach all lanes by lane reads.  The candidates come bin by bin from the
 // lines sorted by core-zone half width (zrec), NZB - 1 bins per tile in rng.
 template <int CP>
 __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles, const int2 *__restrict__ rng,
@@ -926,7 +927,7 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
                                                          float *__restrict__ lop) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   if (t >= ntiles) return;
-  const int N = S.ntau, nwv = S.ntp / DPW;
+  const int N = S.ntau;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   constexpr int TC = 64 * CP;  // points per tile
   const int p0 = t * TC + 1, p1 = min(p0 + TC - 1, S.nwl);
@@ -979,22 +980,16 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
       bool act[CP];
 #pragma unroll
       for (int q = 0; q < CP; q++) act[q] = valid[q] && pp[q] >= lo && pp[q] <= hi;
-      const f4v *cp = reinterpret_cast<const f4v *>(ca + (size_t)j * S.ntp + d0);
-      f4v rr[DPW];
-#pragma unroll
-      for (int k = 0; k < DPW; k++) rr[k] = cp[k];
-      // depths of this wave that leave the far wing somewhere in the tile (bit k)
-      unsigned gen = 0;
-      if (dmb < __float_as_uint(dmaxw[(size_t)j * nwv + wave])) {
-#pragma unroll
-        for (int k = 0; k < DPW; k++)
-          if (dmb < __float_as_uint(rr[k][3])) {
-            gen |= 1u << k;
-            rr[k][0] = rr[k][1] = rr[k][2] = 0.0f;
-          }
-      }
-      // far wing: straight-line FMA code over all depths (zero coefficients at the general
-      // depths), skipped when every depth of the wave takes the general path (wave-uniform);
+      // the records of the wavefront's depths with one load per lane (lane k: depth k), the
+      // depths that leave the far wing somewhere in the tile by one ballot, and each depth's
+      // record handed to all lanes by lane reads
+      const int kl = lane < DPW ? lane : 0;
+      const f4v ra = reinterpret_cast<const f4v *>(ca + (size_t)j * S.ntp + d0)[kl];  // k0, k1, k2, dthr
+      const CoefB rb = cb[(size_t)j * S.ntp + d0 + kl];
+      const unsigned gen =
+          (unsigned)__builtin_amdgcn_ballot_w64(lane < DPW && dmb < __float_as_uint(ra[3]));  // bit k: depth k
+      // far wing: straight-line FMA code over the depths that stay in it for the whole tile,
+      // skipped when every depth of the wave takes the general path (wave-uniform);
       // dl = (point - first point) + (first point - centre), both rounded to FP32
       if (gen != (1u << DPW) - 1u) {
         const float bs = (float)(Wt0 - wc);
@@ -1005,29 +1000,38 @@ __global__ void __launch_bounds__(1024) k_opacity32_core(DevSetup S, int ntiles,
           T2[q] = act[q] ? __builtin_amdgcn_rcpf(fmaxf(xx * xx, 1e-30f)) : 0.0f;
         }
 #pragma unroll
-        for (int k = 0; k < DPW; k++)
+        for (int k = 0; k < DPW; k++) {
+          if ((gen >> k) & 1u) continue;
+          const float k0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ra[0]), k));
+          const float k1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ra[1]), k));
+          const float k2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ra[2]), k));
 #pragma unroll
           for (int q = 0; q < CP; q++) {
             const float xx = T2[q];
-            part[q][k] += xx * fmaf(fmaf(rr[k][2], xx, rr[k][1]), xx, rr[k][0]);
+            part[q][k] += xx * fmaf(fmaf(k2, xx, k1), xx, k0);
           }
+        }
       }
       if (gen) {
-        // the wave's general-path records all in one batch of scalar loads
-        const CoefB *cbp = cb + (size_t)j * S.ntp + d0;
-        CoefB cc[DPW];
-#pragma unroll
-        for (int k = 0; k < DPW; k++) cc[k] = cbp[k];
         float dl[CP];
 #pragma unroll
         for (int q = 0; q < CP; q++) dl[q] = (float)(Wp[q] - wc);
 #pragma unroll
         for (int k = 0; k < DPW; k++) {
-          if (!(gen & (1u << k))) continue;
+          if (!((gen >> k) & 1u)) continue;
+          CoefB c;
+          c.vv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rb.vv), k));
+          c.a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rb.a), k));
+          c.a2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rb.a2), k));
+          c.c1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rb.c1), k));
+          c.ka = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rb.ka), k));
+          c.cop = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rb.cop), k));
+          c.ua = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rb.ua), k));
+          c.ia = __builtin_amdgcn_readlane(rb.ia, k);
           float v[CP], r[CP];
 #pragma unroll
-          for (int q = 0; q < CP; q++) v[q] = dl[q] * cc[k].vv;
-          contrib32_lane(S, cc[k], v, act, r);
+          for (int q = 0; q < CP; q++) v[q] = dl[q] * c.vv;
+          contrib32_lane(S, c, v, act, r);
 #pragma unroll
           for (int q = 0; q < CP; q++) part[q][k] += r[q];
         }
