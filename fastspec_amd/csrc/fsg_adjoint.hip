@@ -102,8 +102,9 @@ __device__ __forceinline__ CentGrad cent_der_grad(double Dx, double Dx1, double 
 // checkpoint of the reverse sweep).  Reverse, from the top down: adjoints of Y (per mu), U,
 // Sd, Deltau, kappa and S, each emitted as soon as no later step can add to it.
 // Scratch rows (stride ld, one column per lane): Yin [(N-1) M], Deltau [N], Sd [N], S [N];
-// the adjoints of S go to gsrc (rows of stride ld).
-template <typename Col>
+// the adjoints of S go to gsrc (rows of stride ld).  MM bounds the angle count (the per-angle
+// registers of a lane).
+template <int MM, typename Col>
 __device__ void rt_vjp_column(const DevSetup &S, const Col &col, double gF, const double *imu, double C1, double C2,
                               double *scr, size_t ld, double *gkap, size_t gstride, double *gsrc) {
   const int N = S.ntau, M = S.nmu;
@@ -119,7 +120,7 @@ __device__ void rt_vjp_column(const DevSetup &S, const Col &col, double gF, cons
     double dt_im1 = tau_step_mu(S.rhox, 1.0, i - 1, (i >= 2) ? col.k(i - 2) : 0.0, col.k(i - 1), col.k(i));
     double Sd_ip1 = (S_ip1 - S_i) / dt_i;
     sds[(size_t)(N - 1) * ld] = Sd_ip1;
-    double Y[MAXMU];
+    double Y[MM];
     for (int m = 0; m < M; m++) Y[m] = S_ip1;
     for (; i >= 0; i--) {
       const double Sd_i = (i == 0) ? (S_ip1 - S_i) / dt_i : cent_der_i(dt_im1, dt_i, S_im1, S_i, S_ip1);
@@ -144,7 +145,7 @@ __device__ void rt_vjp_column(const DevSetup &S, const Col &col, double gF, cons
     }
   }
   // ---- reverse sweep
-  double yb[MAXMU];
+  double yb[MM];
   for (int m = 0; m < M; m++) yb[m] = gF * S.wmu[m];
   double sb_prev = 0.0, sb_cur = 0.0, sb_next = 0.0;  // S adjoints at i-1, i, i+1
   double sdb_cur = 0.0, sdb_next = 0.0;               // Sd adjoints at i, i+1
@@ -261,7 +262,7 @@ __device__ __forceinline__ double wave_sum8(const double (&s)[8], int lane) {
 // One lane per wavelength point of [p_lo, p_lo + np): the line column (cont + LineOpac,
 // adjoint gsfl) and the continuum column (adjoint gsfc).  Outputs dL/dLineOpac and
 // dL/dContOpac [ntau][nwl] and the source-function part of dL/dT [ntau] (accumulated).
-template <typename LopT>
+template <typename LopT, int MM>
 __global__ void __launch_bounds__(64) k_rt_vjp(DevSetup S, const LopT *lop, const double *gsfl, const double *gsfc,
                                                int p_lo, int np, double *scr, double *glop, double *gcont,
                                                double *gtemp) {
@@ -275,7 +276,7 @@ __global__ void __launch_bounds__(64) k_rt_vjp(DevSetup S, const LopT *lop, cons
   point_setup(S, p);
   const double Wl = S.wave[p] + S.wfirst;
   const double C1 = kPlkC1 / Wl, C2 = exp(50.7649141 - 5.0 * log(Wl));
-  double imu[MAXMU];
+  double imu[MM];
   for (int m = 0; m < M; m++) imu[m] = 1.0 / S.xmu[m];
   // scratch: the checkpoint rows, then the S adjoints of the line and continuum columns
   const size_t ld = (size_t)np;
@@ -285,9 +286,9 @@ __global__ void __launch_bounds__(64) k_rt_vjp(DevSetup S, const LopT *lop, cons
     // kappa adjoints go straight to the outputs; the line column's are added to the
     // continuum's, since ContOpac enters both columns
     GridCol<LopT> line{S.contT, lop, S.nwl, p, true};
-    rt_vjp_column(S, line, gsfl[p], imu, C1, C2, scr_l, ld, glop + p, (size_t)S.nwl, sL);
+    rt_vjp_column<MM>(S, line, gsfl[p], imu, C1, C2, scr_l, ld, glop + p, (size_t)S.nwl, sL);
     GridCol<LopT> cont{S.contT, lop, S.nwl, p, false};
-    rt_vjp_column(S, cont, gsfc[p], imu, C1, C2, scr_l, ld, gcont + p, (size_t)S.nwl, sC);
+    rt_vjp_column<MM>(S, cont, gsfc[p], imu, C1, C2, scr_l, ld, gcont + p, (size_t)S.nwl, sC);
     for (int k = 0; k < N; k++) gcont[(size_t)k * S.nwl + p] += glop[(size_t)k * S.nwl + p];
   }
   // dL/dT from the source function: (S-bar_L + S-bar_C) dS/dT, dS/dT = S (C1/T^2)(1 + S/C2)
@@ -365,7 +366,7 @@ __device__ __forceinline__ float hvoigt32_d(const DevSetup &S, float A, int ia, 
 // d ln LinCop / dT, d ln LinVvgt / dT and d LinAvgt / dT in the record's spare words
 __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
                              const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv, double *wlc_s,
-                             int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv, CoefR *coefr) {
+                             int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv, CoefR *coefr, int want64) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= nret) return;
   const DevSetup S = line_setup(S0, L, order[j]);
@@ -377,9 +378,11 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
   CoefV *cv = coefv + (size_t)j * S.ntp;
   CoefR *cr = coefr + (size_t)j * S.ntp;
   for (int k = N; k < S.ntp; k++) {
-    Coef64 r{};
-    r.ia = 1;
-    c[k] = r;
+    if (want64) {
+      Coef64 r{};
+      r.ia = 1;
+      c[k] = r;
+    }
     cv[k] = CoefV{};  // padding depths: the series with zero coefficients everywhere
     cr[k] = CoefR{};
   }
@@ -414,7 +417,7 @@ __global__ void k_fill64_vjp(DevSetup S0, DevLines L, int list_type, int64_t nre
     r.pad1 = dlnEfract + dlnXstim - dlnDopl;           // d ln LinCop / dT
     r.pad2 = -dlnDopl;                                 // d ln LinVvgt / dT
     r.pad3 = dGam / (Dndopl * kXlnC4Pi) - av * dlnDopl;  // d LinAvgt / dT
-    c[k] = r;
+    if (want64) c[k] = r;
     // far-wing series H = a/sqrt(pi) (u + f1 u^2 + f2 u^3), u = 1/v^2, of the one-pole
     // fraction (the FP32 path's expansion), valid to 1e-7 relative where v^2 >= 400 (a^2 + 1/2):
     //   c     = LinCop H
@@ -889,20 +892,34 @@ size_t rt_vjp_scratch_per_point(const DevSetup &S) {
 void launch_rt_vjp(const DevSetup &S, const void *lop, int fp32, const double *gsfl, const double *gsfc, int p_lo,
                    int np, double *scr, double *glop, double *gcont, double *gtemp, hipStream_t st) {
   if (np <= 0) return;
-  if (fp32)
-    k_rt_vjp<float><<<nb(np, 64), 64, 0, st>>>(S, (const float *)lop, gsfl, gsfc, p_lo, np, scr, glop, gcont, gtemp);
-  else
-    k_rt_vjp<double><<<nb(np, 64), 64, 0, st>>>(S, (const double *)lop, gsfl, gsfc, p_lo, np, scr, glop, gcont,
-                                               gtemp);
+  // up to 8 angles (the usual disk-integration sets): half the per-angle registers (172
+  // VGPRs; forcing 3 wavefronts per SIMD moves the angle arrays to scratch and is slower)
+  if (fp32) {
+    if (S.nmu <= 8)
+      k_rt_vjp<float, 8><<<nb(np, 64), 64, 0, st>>>(S, (const float *)lop, gsfl, gsfc, p_lo, np, scr, glop, gcont,
+                                                    gtemp);
+    else
+      k_rt_vjp<float, MAXMU><<<nb(np, 64), 64, 0, st>>>(S, (const float *)lop, gsfl, gsfc, p_lo, np, scr, glop,
+                                                        gcont, gtemp);
+  } else {
+    if (S.nmu <= 8)
+      k_rt_vjp<double, 8><<<nb(np, 64), 64, 0, st>>>(S, (const double *)lop, gsfl, gsfc, p_lo, np, scr, glop,
+                                                     gcont, gtemp);
+    else
+      k_rt_vjp<double, MAXMU><<<nb(np, 64), 64, 0, st>>>(S, (const double *)lop, gsfl, gsfc, p_lo, np, scr, glop,
+                                                         gcont, gtemp);
+  }
 }
 
 void launch_fill64_vjp(const DevSetup &S, const DevLines &L, int list_type, int64_t nret, const int *order,
                        const int *flag, const int *il, const int *ir, const int *ic, Coef64 *coef, CoefV *coefv,
                        double *wlc_s, int4 *meta_s, int *slot_s, VjpWin *win, double *dmaxv, CoefR *coefr,
                        hipStream_t st) {
+  // the Coef64 records feed only the kernel that runs past the matrix cores' 128 depths
+  const int want64 = (S.ntau + 15) / 16 > MFMA_MAX_NQ;
   if (nret)
     k_fill64_vjp<<<nb(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, coef, coefv, wlc_s,
-                                                meta_s, slot_s, win, dmaxv, coefr);
+                                                meta_s, slot_s, win, dmaxv, coefr, want64);
 }
 
 void launch_lineop_vjp(const DevSetup &S, int ntiles, int nbins, int rstride, const int2 *rng, const Coef64 *coef,

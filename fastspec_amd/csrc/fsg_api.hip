@@ -1933,8 +1933,10 @@ int flux_vjp_run(fsg_ctx *c, const double *gsfl, const double *gsfc, double *glo
   CK(hipMemcpyAsync(dgc, gsfc, sizeof(double) * Nw, kin, s));
   CK(hipMemsetAsync(dgt, 0, sizeof(double) * NB * N, s));
   const size_t per = rt_vjp_scratch_per_point(S);
-  // chunks of whole wavefronts (a batch's atmospheres start at multiples of 64 points)
-  const int64_t chunk = std::min<int64_t>((int64_t)Nw, std::max<int64_t>(64, (int64_t)((size_t)1 << 30) / (int64_t)per) & ~(int64_t)63);
+  // chunks of whole wavefronts (a batch's atmospheres start at multiples of 64 points) within
+  // 8 GiB of checkpoint scratch: 1.2 million points of 72 depths and 7 angles in one launch,
+  // so that the wavefronts of the last round do not leave most of the chip idle
+  const int64_t chunk = std::min<int64_t>((int64_t)Nw, std::max<int64_t>(64, (int64_t)((size_t)8 << 30) / (int64_t)per) & ~(int64_t)63);
   CK(c->vscr.ensure(per * (size_t)chunk));
   for (int64_t p0 = 0; p0 < (int64_t)Nw; p0 += chunk) {
     const int np = (int)std::min<int64_t>(chunk, (int64_t)Nw - p0);
