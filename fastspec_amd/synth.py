@@ -557,12 +557,21 @@ class FastSpecBatch(FastSpecGPU):
 
     radtran_calc = RadTranCalc
 
-    def flux(self):
-        """(SfL, SfC) per atmosphere: two lists of [nwl] arrays."""
+    def flux(self, joined: bool = False):
+        """(SfL, SfC) per atmosphere: two lists of [nwl] arrays; joined=True: the two arrays
+        over the concatenated grid (join's layout, padding points included)."""
         sfl = np.zeros(self.nwl_total)
         sfc = np.zeros(self.nwl_total)
         _lib.check(self._lib.fsg_flux(self.ctx, ptr(sfl, C.c_double), ptr(sfc, C.c_double)))
+        if joined:
+            return sfl, sfc
         return self.split(sfl), self.split(sfc)
+
+    def _joined(self, g):
+        """A per-atmosphere list, or an array already over the concatenated grid."""
+        if isinstance(g, np.ndarray) and g.shape == (self.nwl_total,):
+            return np.ascontiguousarray(g, np.float64)
+        return np.ascontiguousarray(self.join(g))
 
     def get_lineop(self):
         out = np.zeros((self.nwl_total, self.setup.ntau))
@@ -570,11 +579,12 @@ class FastSpecBatch(FastSpecGPU):
         return self.split(out)
 
     def flux_vjp(self, gsfl, gsfc, want_opac: bool = False):
-        """From per-atmosphere dL/dSfL, dL/dSfC: per-atmosphere dL/dLineOpac and dL/dContOpac
-        (or None) and the source-function part of dL/dT [nb][ntau]."""
+        """From per-atmosphere dL/dSfL, dL/dSfC (lists, or arrays over the concatenated grid):
+        per-atmosphere dL/dLineOpac and dL/dContOpac (or None) and the source-function part of
+        dL/dT [nb][ntau]."""
         n = self.setup.ntau
-        a = np.ascontiguousarray(self.join(gsfl))
-        b = np.ascontiguousarray(self.join(gsfc))
+        a = self._joined(gsfl)
+        b = self._joined(gsfc)
         glop = np.zeros((self.nwl_total, n)) if want_opac else None
         gcont = np.zeros((self.nwl_total, n)) if want_opac else None
         gt = np.zeros((self.nb, n))

@@ -60,6 +60,32 @@ def _batch(lines, models, prec, weights=None):
     return out
 
 
+def test_joined_arrays_match_lists(c5_small):
+    """flux(joined=True) is flux()'s lists in the concatenated layout, and flux_vjp takes the
+    gradients in either form (the atomic sums of dL/dT may add in another order)."""
+    lines, models, weights = c5_small
+    setups = [s for s, _ in models[:3]]
+    with FastSpecBatch(setups, precision=FSG_FP32) as b:
+        shared = b.upload(lines)
+        bl = b.batch_lines(shared, [idx for _, idx in models[:3]])
+        b.lineop_dev(bl)
+        b.RadTranCalc(intensities=False)
+        sfl, sfc = b.flux()
+        jl, jc = b.flux(joined=True)
+        assert jl.shape == (b.nwl_total,) and jc.shape == (b.nwl_total,)
+        for k, su in enumerate(setups):
+            assert np.array_equal(jl[b.off[k]:b.off[k] + su.nwl], sfl[k])
+            assert np.array_equal(jc[b.off[k]:b.off[k] + su.nwl], sfc[k])
+        gl = [a / c for a, c in zip(weights[:3], sfc)]
+        gc = [-a * l / c**2 for a, l, c in zip(weights[:3], sfl, sfc)]
+        _, _, gt1 = b.flux_vjp(gl, gc)
+        _, _, gt2 = b.flux_vjp(b.join(gl), b.join(gc))
+        assert np.count_nonzero(gt1) > 0
+        np.testing.assert_allclose(gt2, gt1, rtol=1e-12, atol=0.0)
+        bl.free()
+        shared.free()
+
+
 def test_layout(c5_small):
     lines, models, _ = c5_small
     out = _batch(lines, models[:3], FSG_FP64)
