@@ -89,7 +89,9 @@ CONTOP_HOP_NOLEV = 1 << 20  # FSG_CONTOP_HOP_NOLEV: Ifop(1) = -1, HOP without le
 class CContopEos(C.Structure):
     """fsg_contop_eos: the EOS rows the device's continuum sources read ([ntau] REAL(4))."""
     _fields_ = [("fpf_h1", _fp), ("fpf_h2", _fp), ("frac_hmin", _fp), ("fpf_he1", _fp), ("frac_hmol", _fp),
-                ("frac_he2", _fp), ("fpf_he2", _fp), ("fpf_he3", _fp)]
+                ("frac_he2", _fp), ("fpf_he2", _fp), ("fpf_he3", _fp),
+                ("fpf_c1", _fp), ("fpf_mg1", _fp), ("fpf_al1", _fp), ("fpf_si1", _fp), ("fpf_fe1", _fp),
+                ("fpf_ch", _fp), ("fpf_oh", _fp), ("frac_nh", _fp), ("frac_h1", _fp), ("frac_he1", _fp)]
 
 
 class CStats(C.Structure):

@@ -190,6 +190,7 @@ struct fsg_ctx {
   // fsg_set_atmosphere)
   const float *fh1 = nullptr, *fhe1 = nullptr, *fhmol = nullptr;
   DBuf ctab;  // fsg_contop: per-depth state and the H- tables
+  DBuf cxs;   // fsg_contop: COOLOP's per-wavelength cross-sections of C I, Mg I and Si I
 };
 
 struct fsg_dlines {
@@ -671,7 +672,7 @@ int fsg_finalize(fsg_ctx *c) {
   for (void *p : c->owned) (void)hipFree(p);
   for (DBuf *b : {&c->flag, &c->il, &c->ir, &c->ic, &c->key, &c->key2, &c->idx, &c->idx2, &c->coef, &c->wlc,
                   &c->meta, &c->rng, &c->rng2, &c->tmp, &c->pos, &c->wlcl, &c->rflag, &c->comp, &c->vscr,
-                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm, &c->zrec, &c->rngn, &c->fcap, &c->foff, &c->ftmp, &c->flist, &c->ctab, &c->cv_a,
+                  &c->vgcont, &c->vsl, &c->vslot, &c->vout, &c->zkey, &c->zkey2, &c->zval, &c->zperm, &c->zrec, &c->rngn, &c->fcap, &c->foff, &c->ftmp, &c->flist, &c->ctab, &c->cxs, &c->cv_a,
                   &c->cv_b, &c->cv_kern, &c->cv_rows, &c->cv_w})
     b->release();
   if (c->st2) (void)hipStreamSynchronize(c->st2);
@@ -875,8 +876,9 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
   const unsigned mask = (unsigned)ifop_mask & ~CONT_HOP_NOLEV;
   if ((ifop_mask & FSG_CONTOP_HOP_NOLEV) && !(mask & 1u))
     return fail("fsg_contop: FSG_CONTOP_HOP_NOLEV qualifies HOP (bit 0), which is not switched on");
-  // HOP, H2PLOP, HMINOP, HRAYOP, HE1OP, HE2OP, HEMIOP, HERAOP, ELECOP, H2RAOP (Ifop 1-8, 12, 13)
-  const unsigned on_device = 0xFFu | (1u << 11) | (1u << 12);
+  // HOP, H2PLOP, HMINOP, HRAYOP, HE1OP, HE2OP, HEMIOP, HERAOP, COOLOP, ELECOP, H2RAOP (Ifop 1-9,
+  // 12, 13)
+  const unsigned on_device = 0x1FFu | (1u << 11) | (1u << 12);
   if (mask & ~on_device) {
     std::string m = "fsg_contop: continuum source(s)";
     for (int b = 0; b < 32; b++)
@@ -888,6 +890,13 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
       (on(3) && !e->frac_hmin) || ((on(5) || on(7) || on(8)) && !e->fpf_he1) || (on(13) && !e->frac_hmol) ||
       (on(5) && !e->frac_he2) || (on(6) && (!e->fpf_he2 || !e->fpf_he3)))
     return fail("fsg_contop: a population the selected sources need is NULL");
+  const float *const cool_pops[10] = {e->fpf_c1, e->fpf_mg1, e->fpf_al1, e->fpf_si1, e->fpf_fe1,
+                                      e->fpf_ch,  e->fpf_oh,  e->frac_nh, e->frac_h1, e->frac_he1};
+  if (on(9)) {
+    if (!e->frac_hmol) return fail("fsg_contop: COOLOP needs frac_hmol");
+    for (const float *q : cool_pops)
+      if (!q) return fail("fsg_contop: COOLOP needs the C I, Mg I, Al I, Si I, Fe I, CH, OH, NH, H I and He I rows");
+  }
   CK(hipSetDevice(c->dev));
   const DevSetup &S = c->S;
   const size_t N = (size_t)S.ntau, Nw = (size_t)S.nwl;
@@ -961,6 +970,8 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
       col(CR_EXLIMHE2)[j] = (fhe2 == 0.0 || exllog > 80.0) ? 0.0 : fhe2 * std::exp(-exllog);
     }
   }
+  // ContopInit's factors of COOLOP's members (src/contop.f90:337-420)
+  if (on(9)) coolop_rows((int)N, cool_pops, blk.data());
   std::copy(tab.begin(), tab.end(), blk.begin() + CR_NROW * N);
   CK(c->ctab.ensure(sizeof(double) * blk.size()));
   double *dblk = c->ctab.as<double>();
@@ -968,6 +979,11 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
   const unsigned kmask = mask | ((ifop_mask & FSG_CONTOP_HOP_NOLEV) ? CONT_HOP_NOLEV : 0u);
   launch_contop(S.nwl, S.ntau, S.wave, S.wfirst, dblk, dblk + CR_NROW * N, S.cstd, S.motype, kmask, add ? 1 : 0,
                 const_cast<double *>(S.contT), s);
+  if (on(9)) {
+    CK(c->cxs.ensure(sizeof(double) * coolop_scratch_per_point() * Nw));
+    launch_coolop(S.nwl, S.ntau, S.wave, S.wfirst, dblk, dblk + CR_NROW * N, S.cstd, S.motype, c->cxs.as<double>(),
+                  const_cast<double *>(S.contT), s);
+  }
   CK(hipGetLastError());
   if (contop_out) {  // [ntau][nwl] -> ContOpac(Ntau, Nwl)
     std::vector<double> h(N * Nw);

@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "fsg_contop_tables.h"
+#include "fsg_coolop_tables.h"
 #include "fsg_device.h"
 #include "fsg_kernels.h"
 
@@ -398,6 +399,336 @@ void launch_contop(int nwl, int ntau, const double *wave, double wfirst, const d
 }
 
 // ------------------------------------------------------------------------------------
+// COOLOP (src/contop.f90:1544-1569): C I, Mg I, Si I, Al I and Fe I bound-free, CH and OH
+// photodissociation, H2-H2 / H2-He / H2-H collision-induced absorption, H2- free-free and
+// NH photodissociation, nothing above the H I Lyman edge.
+// ------------------------------------------------------------------------------------
+constexpr double kCoolRyd = 109732.298;
+// series limits and edges of the hydrogenic pseudo-continua of C I, Mg I and Si I
+// (src/contop.f90:72-77)
+constexpr double kPsElim[3] = {90820.42, 61671.02, 65747.55};
+constexpr double kPsEdge[3] = {kCoolRyd / 16.0, kCoolRyd / 25.0, kCoolRyd / 25.0};
+constexpr int COOL_NX = COOL_NC1 + COOL_NMG1 + COOL_NSI1;  // the three atoms' cross-sections per lane
+
+__device__ __forceinline__ double pow10d(double x) { return pow(10.0, x); }
+__device__ __forceinline__ double exp10d(double x) { return exp(x * 2.302585092994046); }
+
+// C1OP's cross-sections at this frequency (src/contop.f90:1575-1652), X[i * ld]
+__device__ void c1op_x(double Freq, double Waveno, double *X, int ld) {
+  const int lryd[14] = {2, 2, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 0, 0};
+  for (int i = 0; i < COOL_NC1; i++) X[i * ld] = 0.0;
+  // 2s2 2p3d and 2p3p / 2p3s Rydberg levels, to C II 2P average
+  double Elim = 90862.70;
+  for (int i = 0; i < 14; i++) {
+    if (Waveno < Elim - kC1Elev[i]) break;
+    X[i * ld] = xkarzas(Freq, 9.0 / kCoolRyd * (Elim - kC1Elev[i]), 3, lryd[i]);
+  }
+  // 2s2 2p2 levels to C II 2P1/2 (weight 1/3) and to 2P3/2 (weight 2/3): Luo & Pradhan
+  // fits with the resonances of Burke & Taylor
+  for (int pass = 0; pass < 2; pass++) {
+    Elim = pass == 0 ? 90820.42 : 90820.42 + 63.42;
+    if (Waveno < Elim - kC1Elev[14]) continue;
+    const double xs0 = pow10d(-16.80 - (Waveno - Elim + kC1Elev[14]) / 3.00 / kCoolRyd);
+    double eps = (Waveno - 97700.0) * 2.0 / 2743.0;
+    const double xs1 = (68.e-18 * eps + 118.e-18) / (eps * eps + 1.0);
+    X[14 * ld] = pass == 0 ? (xs0 + xs1) / 3.0 : X[14 * ld] + (xs0 + xs1) * 2.0 / 3.0;
+    if (Waveno < Elim - kC1Elev[15]) continue;
+    const double xd0 = pow10d(-16.80 - (Waveno - Elim + kC1Elev[15]) / 3.00 / kCoolRyd);
+    eps = (Waveno - 93917.0) * 2.0 / 9230.0;
+    const double xd1 = (22.e-18 * eps + 26.e-18) / (eps * eps + 1.0);
+    eps = (Waveno - 111130.0) * 2.0 / 2743.0;
+    const double xd2 = (-10.5e-18 * eps + 46.e-18) / (eps * eps + 1.0);
+    X[15 * ld] = pass == 0 ? (xd0 + xd1 + xd2) / 3.0 : X[15 * ld] + (xd0 + xd1 + xd2) * 2.0 / 3.0;
+    for (int q = 16; q <= 18; q++) {  // 3P2, 3P1, 3P0
+      if (Waveno < Elim - kC1Elev[q]) break;
+      const double x = pow10d(-16.80 - (Waveno - Elim + kC1Elev[q]) / 3.00 / kCoolRyd);
+      X[q * ld] = pass == 0 ? x / 3.0 : X[q * ld] + x * 2.0 / 3.0;
+    }
+  }
+  // 2s2p3 levels to C II 2s2p2 4P
+  Elim = 90820.42 + 43003.3;
+  for (int i = 19; i < COOL_NC1; i++) {
+    if (Waveno < Elim - kC1Elev[i]) break;
+    X[i * ld] = xkarzas(Freq, 4.0 / kCoolRyd * (Elim - kC1Elev[i]), 2, 1) * 3.0;
+  }
+}
+
+// MG1OP's cross-sections (src/contop.f90:1672-1727): Karzas & Latter for the 3s4l levels,
+// power-law fits for the lower ones
+__device__ void mg1op_x(double Freq, double Waveno, double *X, int ld) {
+  const double Elim = 61671.02;
+  for (int i = 0; i < COOL_NMG1; i++) X[i * ld] = 0.0;
+  const int lk[5] = {3, 3, 2, 2, 1};
+  for (int i = 0; i < 15; i++) {
+    if (Waveno < Elim - kMg1Elev[i]) break;
+    double x;
+    if (i < 5) x = xkarzas(Freq, 16.0 / kCoolRyd * (Elim - kMg1Elev[i]), 4, lk[i]);
+    else if (i == 5) x = 25.e-18 * pow(13713.986 / Waveno, 2.7);
+    else if (i == 6) x = 33.8e-18 * pow(13823.223 / Waveno, 2.8);
+    else if (i == 7) x = 45.e-18 * pow(15267.955 / Waveno, 2.7);
+    else if (i == 8) x = 0.43e-18 * pow(18167.687 / Waveno, 2.6);
+    else if (i == 9) x = 2.1e-18 * pow(20473.617 / Waveno, 2.6);
+    else if (i == 10) x = 16.e-18 * pow(26619.756 / Waveno, 2.1) - 7.8e-18 * pow(26619.756 / Waveno, 9.5);
+    else if (i <= 13) x = fmax(20.e-18 * pow(39759.842 / Waveno, 2.7), 40.e-18 * pow(39759.842 / Waveno, 14.0));
+    else x = 1.1e-18 * pow((Elim - kMg1Elev[14]) / Waveno, 10.0);
+    X[i * ld] = x;
+  }
+}
+
+// SI1OP's cross-sections (src/contop.f90:1763-1866): Karzas & Latter for the Rydberg and
+// 3s3p3 levels, fits to Nahar & Pradhan for the 3s2 3p2 ones
+__device__ void si1op_x(double Freq, double Waveno, double *X, int ld) {
+  const int nryd[22] = {4, 4, 4, 4, 4, 4, 4, 4, 3, 3, 3, 4, 3, 4, 3, 4, 4, 4, 3, 4, 4, 4};
+  const int lryd[22] = {2, 3, 2, 2, 2, 3, 2, 2, 2, 2, 2, 1, 2, 1, 2, 1, 1, 1, 2, 1, 0, 0};
+  for (int i = 0; i < COOL_NSI1; i++) X[i * ld] = 0.0;
+  double Elim = 65939.18;
+  for (int i = 0; i < 22; i++) {
+    if (Waveno < Elim - kSi1Elev[i]) break;
+    const double zeff2 = (double)(nryd[i] * nryd[i]) / kCoolRyd * (Elim - kSi1Elev[i]);
+    X[i * ld] = xkarzas(Freq, zeff2, nryd[i], lryd[i]);
+  }
+  // 3s2 3p2 levels to Si II 2P1/2 (weight 1/3) and 2P3/2 (weight 2/3)
+  for (int pass = 0; pass < 2; pass++) {
+    Elim = pass == 0 ? 65747.55 : 65747.55 + 287.45;
+    if (Waveno < Elim - kSi1Elev[22]) continue;
+    double eps = (Waveno - 70000.0) * 2.0 / 6500.0;
+    double reson1 = (97.e-18 * eps + 94.e-18) / (eps * eps + 1.0);
+    double x = 37.e-18 * pow(50353.180 / Waveno, 2.40) + reson1;
+    X[22 * ld] = pass == 0 ? x / 3.0 : X[22 * ld] + x * 2.0 / 3.0;
+    if (Waveno < Elim - kSi1Elev[23]) continue;
+    eps = (Waveno - 78600.0) * 2.0 / 13000.0;
+    reson1 = (-10.e-18 * eps + 77.e-18) / (eps * eps + 1.0);
+    x = 24.5e-18 * pow(59448.700 / Waveno, 1.85) + reson1;
+    X[23 * ld] = pass == 0 ? x / 3.0 : X[23 * ld] + x * 2.0 / 3.0;
+    for (int q = 24; q <= 26; q++) {  // 3P2, 3P1, 3P0
+      if (Waveno < Elim - kSi1Elev[q]) break;
+      const double y = Waveno <= 74000.0 ? 72.e-18 * pow(65524.393 / Waveno, 1.90)
+                                         : 93.e-18 * pow(65524.393 / Waveno, 4.00);
+      X[q * ld] = pass == 0 ? y / 3.0 : X[q * ld] + y * 2.0 / 3.0;
+    }
+  }
+  // 3s3p3 levels to Si II 3s3p2 4P
+  Elim = 65747.5 + 42824.35;
+  for (int i = 27; i < COOL_NSI1; i++) {
+    if (Waveno < Elim - kSi1Elev[i]) break;
+    X[i * ld] = xkarzas(Freq, 9.0 / kCoolRyd * (Elim - kSi1Elev[i]), 3, 1) * 3.0;
+  }
+}
+
+// One lane per wavelength as in k_contop; the lane's 73 cross-sections of C I, Mg I and
+// Si I go to scratch xs [73][nwl] (a wavefront's columns stay in L1 while it runs down the
+// depths) and the molecular tables are interpolated per depth as the reference
+// interpolates them (powers of ten as exp(x ln 10), within 1e-14 of the reference's pow at
+// the tables' exponents).  Without LDS, three wavefronts share a SIMD.  A level no lane of the wavefront can ionize adds
+// exact zeros to the reference's sums: the depth loop takes only the levels of the
+// wavefront's masks (of the 73, and of Fe I's 48, whose cross-sections it forms as it goes).
+// Acool is added to contT (divided by ContOpacStd for Motype 1).
+__global__ void __launch_bounds__(64, 3) k_coolop(int nwl, int ntau, const double *__restrict__ wave, double wfirst,
+                                                  const double *__restrict__ dep, const double *__restrict__ tab,
+                                                  const double *__restrict__ cstd, int motype, double *__restrict__ xs,
+                                                  double *__restrict__ contT) {
+  const int lane = threadIdx.x;
+  const int p = blockIdx.x * blockDim.x + lane;
+  if (p >= nwl) return;
+  auto row = [dep, ntau](int r) { return dep + (size_t)r * ntau; };
+  const double clight = kVc * 1e13;
+  const double Freq = clight / air2vac(wave[p] + wfirst);
+  if (Freq > 3.28805e15) return;  // no cool-star opacity above the H I Lyman edge
+  const double Waveno = Freq / kClightCm;
+  const double Freq3 = 2.815e29 / Freq / Freq / Freq;
+  double *X = xs + p;  // X[i * nwl]
+  c1op_x(Freq, Waveno, X, nwl);
+  mg1op_x(Freq, Waveno, X + (size_t)COOL_NC1 * nwl, nwl);
+  si1op_x(Freq, Waveno, X + (size_t)(COOL_NC1 + COOL_NMG1) * nwl, nwl);
+  // the levels some lane of the wavefront can ionize (bit i of mx: X level i; of mfe: Fe I
+  // level i); the lanes that returned above take no part
+  uint64_t mx[2] = {0, 0}, mfe = 0;
+  for (int i = 0; i < COOL_NX; i++)
+    if (__builtin_amdgcn_ballot_w64(X[(size_t)i * nwl] != 0.0)) mx[i >> 6] |= 1ull << (i & 63);
+  for (int i = 0; i < COOL_NFE1; i++)
+    if (__builtin_amdgcn_ballot_w64(!(Waveno < 21000.0) && kFe1Wno[i] < Waveno)) mfe |= 1ull << i;
+  bool blue[3];
+  for (int n = 0; n < 3; n++) blue[n] = kPsElim[n] - kPsEdge[n] >= kPsElim[n] - Waveno;
+  // AL1OP (src/contop.f90:1747-1758): the 3p 2P3/2 and 2P1/2 components
+  double xal1 = 0.0;
+  {
+    const double Elim = 48278.37;
+    if (!(Waveno < Elim - 112.061)) {
+      xal1 = 6.5e-17 * pow((Elim - 112.061) / Waveno, 5.0) * 4.0;
+      if (!(Waveno < Elim)) xal1 = xal1 + 6.5e-17 * pow(Elim / Waveno, 5.0) * 2.0;
+    }
+  }
+  // FE1OP (src/contop.f90:1879-1904): nothing below 21000 cm^-1
+  const bool fe1 = !(Waveno < 21000.0);
+  // CHOP and OHOP (src/contop.f90:1909-2099, 2104-2331): the energy interval of the tables
+  const double Evolt = Waveno / 8065.479;
+  const int nch = (int)(Evolt * 10.0);
+  const bool ch = !(nch < 20 || nch >= 105);
+  const double ench = (double)nch * 0.1;
+  const int noh = (int)(Evolt * 10.0 - 20.0);
+  const bool oh = !(noh <= 0 || noh >= 130);
+  const double enoh = (double)noh * 0.1 + 2.0;
+  // H2COLLOP (src/contop.f90:2341-2372): the wavenumber interval
+  const bool cia = !(Waveno < 50.0 || Waveno > 50.0 + 50.0 * (double)(400 - 1));
+  const int nu = cia ? min((int)((Waveno - 50.0) / 50.0) + 1, 400 - 1) : 1;
+  const double delnu = (Waveno - 50.0) / 50.0 - (double)(nu - 1);
+  const bool h2h = Waveno <= 10000.0;
+  // H2MINOP (src/contop.f90:2379-2396): the interval of the Bell table, ends extended
+  const double *h2mwlog = tab + CONT_TAB_H2M;
+  const double wlog = log(clight / Freq);
+  int ih = 1;
+  while (ih < 17 && wlog > h2mwlog[ih]) ih++;
+  const double dwh = (wlog - h2mwlog[ih - 1]) / (h2mwlog[ih] - h2mwlog[ih - 1]);
+  // NHOP (src/contop.f90:2403-2418): the wavelength interval of the NH table
+  const double wl = clight / Freq;
+  const bool nh = !(wl < 910.0 || wl > 910.0 + 2.0 * (double)(2146 - 1));
+  const int inh = nh ? min((int)((wl - 910.0) / 2.0) + 1, 2146 - 1) : 1;
+  const double dwn = (wl - 910.0) / 2.0 - (double)(inh - 1);
+
+  const double *T = row(CR_T), *Hkt = row(CR_HKT), *Hckt = row(CR_HCKT), *Tk = row(CR_TK);
+  const double *Xne = row(CR_XNE), *Rho = row(CR_RHO);
+  for (int j = 0; j < ntau; j++) {
+    const double stim = 1.0 - exp(-Freq * Hkt[j]);
+    const double hckt = Hckt[j], rho = Rho[j], t = T[j];
+    // C1OP, MG1OP, SI1OP: the pseudo-continuum, then the explicit levels
+    double aatom[3];
+    const int base[3] = {0, COOL_NC1, COOL_NC1 + COOL_NMG1}, nlev[3] = {COOL_NC1, COOL_NMG1, COOL_NSI1};
+    const int brow[3] = {CR_BOLTC1, CR_BOLTMG1, CR_BOLTSI1}, frow[3] = {CR_FC1, CR_FMG1, CR_FSI1};
+    const double gps[3] = {6.0, 2.0, 6.0};
+    for (int s = 0; s < 3; s++) {
+      double e1 = row(CR_PSEXP1 + s)[j];
+      if (!blue[s]) e1 = exp(-(kPsElim[s] - Waveno) * hckt);
+      double h = Freq3 * gps[s] / (kCoolRyd * hckt) * (e1 - row(CR_PSEXP0 + s)[j]);
+      for (int i = 0; i < nlev[s]; i++) {
+        const int b = base[s] + i;
+        if ((mx[b >> 6] >> (b & 63)) & 1ull) h = h + X[(size_t)b * nwl] * row(brow[s] + i)[j];
+      }
+      aatom[s] = h * row(frow[s])[j] * stim / rho;
+    }
+    // CHOP, OHOP
+    double ach = 0.0, aoh = 0.0;
+    if ((ch || oh) && !(t >= 9000.0)) {
+      const int ip = max((int)((t - 1000.0) / 200.0 + 1.0), 1);
+      const double tp = (double)ip * 200.0 + 800.0;
+      const int it = max((int)((t - 2000.0) / 500.0 + 1.0), 1);
+      const double tn = (double)it * 500.0 + 1500.0;
+      if (ch) {
+        const double part = kPartCH[ip - 1] + (kPartCH[ip] - kPartCH[ip - 1]) * (t - tp) / 200.0;
+        const double *c0 = kCrossCH + (size_t)(nch - 1) * 15, *c1 = c0 + 15;
+        const double x0 = c0[it - 1] + (c1[it - 1] - c0[it - 1]) * (Evolt - ench) / 0.1;
+        const double x1 = c0[it] + (c1[it] - c0[it]) * (Evolt - ench) / 0.1;
+        const double chop0 = exp((x0 + (x1 - x0) * (t - tn) / 500.0) * 2.30258509299405) * part;
+        ach = chop0 * row(CR_FCH)[j] * stim / rho;
+      }
+      if (oh) {
+        const double part = kPartOH[ip - 1] + (kPartOH[ip] - kPartOH[ip - 1]) * (t - tp) / 200.0;
+        const double *c0 = kCrossOH + (size_t)(noh - 1) * 15, *c1 = c0 + 15;
+        const double x0 = c0[it - 1] + (c1[it - 1] - c0[it - 1]) * (Evolt - enoh) / 0.1;
+        const double x1 = c0[it] + (c1[it] - c0[it]) * (Evolt - enoh) / 0.1;
+        const double ohop0 = exp((x0 + (x1 - x0) * (t - tn) / 500.0) * 2.30258509299405) * part;
+        aoh = ohop0 * row(CR_FOH)[j] * stim / rho;
+      }
+    }
+    // H2COLLOP: the tables interpolated in wavenumber at the two temperatures around T
+    double ah2coll = 0.0;
+    if (cia) {
+      const int it = max(1, min(13 - 1, (int)((t - 1000.0) / 500.0) + 1));
+      const double delt = fmax(0.0, fmin(1.0, (t - 1000.0) / 500.0 - (double)(it - 1)));
+      auto at = [nu, delnu](const double *tb, int i) {
+        return tb[(size_t)(nu - 1) * 13 + (i - 1)] * (1.0 - delnu) + tb[(size_t)nu * 13 + (i - 1)] * delnu;
+      };
+      const double xh2h2 = at(kCiaH2h2, it) * (1.0 - delt) + at(kCiaH2h2, it + 1) * delt;
+      const double xh2he = at(kCiaH2he, it) * (1.0 - delt) + at(kCiaH2he, it + 1) * delt;
+      const double xnh2 = row(CR_FHMOL)[j];
+      ah2coll = exp10d(xh2h2) * xnh2 + exp10d(xh2he) * row(CR_FRHE1)[j];
+      if (h2h) {
+        const double xh2h = at(kCiaH2h, it) * (1.0 - delt) + at(kCiaH2h, it + 1) * delt;
+        ah2coll = ah2coll + exp10d(xh2h) * row(CR_FRH1)[j];
+      }
+      ah2coll = ah2coll * xnh2 / rho;
+    }
+    // H2MINOP
+    const double ah2min = exp(row(CR_H2MLK + ih - 1)[j] * (1.0 - dwh) + row(CR_H2MLK + ih)[j] * dwh) * Xne[j] *
+                          Tk[j] * row(CR_FHMOL)[j] / rho;
+    // NHOP: the cross-sections at this depth's temperature at the two table wavelengths
+    double anh = 0.0;
+    const double nhnrho = row(CR_NHNRHO)[j];
+    if (nh && nhnrho != 0.0) {
+      const int n = (int)row(CR_NHN)[j];
+      const double w = row(CR_NHW)[j];
+      const double *l0 = kNhLog + (size_t)(inh - 1) * 15, *l1 = l0 + 15;
+      const double s0 = exp10d(l0[n - 1] + (l0[n] - l0[n - 1]) * w);
+      const double s1 = exp10d(l1[n - 1] + (l1[n] - l1[n - 1]) * w);
+      anh = (s0 + (s1 - s0) * dwn) * nhnrho * stim;
+    }
+    // FE1OP: SUM(Xsect * BoltFe1(:, j)) over the levels of the mask
+    double afe1 = 0.0;
+    for (uint64_t m = mfe; m; m &= m - 1) {
+      const int i = __builtin_ctzll(m);
+      const double x = (kFe1Wno[i] + 3000.0 - Waveno) / kFe1Wno[i] / 0.1, x2 = x * x;
+      const double xsect = (fe1 && kFe1Wno[i] < Waveno) ? 3.e-18 / (1.0 + x2 * x2) : 0.0;
+      afe1 = afe1 + xsect * row(CR_BOLTFE1 + i)[j];
+    }
+    double acool = aatom[0] + aatom[1] + aatom[2] + ach + aoh + ah2coll + ah2min + anh +
+                   (xal1 * row(CR_FAL1)[j] + afe1 * row(CR_FFE1)[j]) * stim / rho;
+    if (motype == 1) acool = acool / cstd[j];
+    const size_t o = (size_t)j * nwl + p;
+    contT[o] = contT[o] + acool;
+  }
+}
+
+void launch_coolop(int nwl, int ntau, const double *wave, double wfirst, const double *dep, const double *tab,
+                   const double *cstd, int motype, double *xs, double *contT, hipStream_t st) {
+  k_coolop<<<(nwl + 63) / 64, 64, 0, st>>>(nwl, ntau, wave, wfirst, dep, tab, cstd, motype, xs, contT);
+}
+
+// ContopInit's factors of COOLOP's members (src/contop.f90:337-420) and the populations
+// COOLOP reads, into the rows of dep, in ContopInit's order of operations with the host's
+// libm.  T, Hkt, Rho from the rows already filled; pops: FractionPF of C I, Mg I, Al I,
+// Si I, Fe I, CH and OH, then Fraction of NH, H I and He I.
+void coolop_rows(int ntau, const float *const pops[10], double *dep) {
+  const size_t N = (size_t)ntau;
+  auto col = [dep, N](int r) { return dep + (size_t)r * N; };
+  const double *T = col(CR_T), *Hkt = col(CR_HKT), *Rho = col(CR_RHO);
+  for (size_t j = 0; j < N; j++) {
+    const double hckt = kClightCm * Hkt[j];
+    col(CR_HCKT)[j] = hckt;
+    col(CR_TK)[j] = kKbolt * T[j];
+    for (int i = 0; i < COOL_NC1; i++) col(CR_BOLTC1 + i)[j] = kC1Glev[i] * std::exp(-kC1Elev[i] * hckt);
+    for (int i = 0; i < COOL_NMG1; i++) col(CR_BOLTMG1 + i)[j] = kMg1Glev[i] * std::exp(-kMg1Elev[i] * hckt);
+    for (int i = 0; i < COOL_NSI1; i++) col(CR_BOLTSI1 + i)[j] = kSi1Glev[i] * std::exp(-kSi1Elev[i] * hckt);
+    for (int i = 0; i < COOL_NFE1; i++) col(CR_BOLTFE1 + i)[j] = kFe1G[i] * std::exp(-kFe1E[i] * kClightCm * Hkt[j]);
+    for (int n = 0; n < 3; n++) {
+      col(CR_PSEXP0 + n)[j] = std::exp(-kPsElim[n] * hckt);
+      col(CR_PSEXP1 + n)[j] = std::exp(-(kPsElim[n] - kPsEdge[n]) * hckt);
+    }
+    // ln K of the Bell H2- table at this depth: linear in ln theta, theta clamped to the table
+    const double thlog = std::log(std::fmin(std::fmax(5040.4 / T[j], kH2mTheta[0]), kH2mTheta[7]));
+    int n = 1;
+    while (n < 7 && thlog > std::log(kH2mTheta[n])) n++;
+    const double dth = (thlog - std::log(kH2mTheta[n - 1])) / (std::log(kH2mTheta[n]) - std::log(kH2mTheta[n - 1]));
+    for (int i = 0; i < CONT_H2M_NW; i++)
+      col(CR_H2MLK + i)[j] = std::log(kH2mFf[i * 8 + n - 1] * 1.e-26) * (1.0 - dth) +
+                             std::log(kH2mFf[i * 8 + n] * 1.e-26) * dth;
+    // the NH table's temperature interval (1-based) and weight in 1/T; none outside it
+    col(CR_NHN)[j] = 1.0;
+    col(CR_NHW)[j] = 0.0;
+    col(CR_NHNRHO)[j] = 0.0;
+    if (!(T[j] < kNhT[0] || T[j] > kNhT[14])) {
+      int m = 1;
+      while (m < 15 - 1 && T[j] > kNhT[m]) m++;
+      col(CR_NHN)[j] = (double)m;
+      col(CR_NHW)[j] = (1.0 / T[j] - 1.0 / kNhT[m - 1]) / (1.0 / kNhT[m] - 1.0 / kNhT[m - 1]);
+      col(CR_NHNRHO)[j] = (double)pops[7][j] / Rho[j];
+    }
+    const int prow[7] = {CR_FC1, CR_FMG1, CR_FAL1, CR_FSI1, CR_FFE1, CR_FCH, CR_FOH};
+    for (int q = 0; q < 7; q++) col(prow[q])[j] = (double)pops[q][j];
+    col(CR_FRH1)[j] = (double)pops[8][j];
+    col(CR_FRHE1)[j] = (double)pops[9][j];
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // host side: HMINOP's tables (src/contop.f90:183-208 and 1133-1162; Bell & Berrington 1987
 // free-free absorption, 1e-26 cm^4/dyn per H atom and unit electron pressure, at 22
 // wavelengths [um] x 11 temperatures theta = 5040/T; Wishart 1979 bound-free cross-section,
@@ -467,7 +798,8 @@ const double kHmBfSigma[85] = {
 }  // namespace
 
 void contop_tables(std::vector<double> &tab) {
-  tab.assign((size_t)CONT_HM_NW * (1 + CONT_HM_NT) + CONT_HM_NT + 2 * CONT_HMBF_N, 0.0);
+  tab.assign((size_t)CONT_TAB_H2M + CONT_H2M_NW, 0.0);
+  for (int i = 0; i < CONT_H2M_NW; i++) tab[CONT_TAB_H2M + i] = std::log(kH2mWave[i]);
   double *w = tab.data(), *ff = w + CONT_HM_NW, *th = ff + CONT_HM_NW * CONT_HM_NT;
   double *wb = th + CONT_HM_NT, *sb = wb + CONT_HMBF_N;
   for (int i = 0; i < CONT_HM_NW; i++) w[i] = std::log(91.134 / kHmWavek[i]);

@@ -127,22 +127,42 @@ void launch_conv_rows(int nwl, int nrows, int maxk, const ConvRow *d_rows, const
 
 // continuum opacity (fsg_contop.hip): the H- free-free table is CONT_HM_NW wavelengths x
 // CONT_HM_NT temperatures, the H- bound-free cross-section CONT_HMBF_N points; tab holds
-// ln(91.134 / lambda_k) [NW], ln(K_ff) [NW][NT], theta [NT], lambda_bf [N], sigma_bf [N];
-// dep holds the rows below, each [ntau]: the depth state and populations, then ContopInit's
-// factors of HOP, HE1OP and HE2OP (src/contop.f90:377-470; BoltnHe1 for n = 4..27).  mask
-// bit s-1 switches source s; bit 20 is Ifop(1) = -1 (HOP without levels 1-6).
-constexpr int CONT_HM_NW = 22, CONT_HM_NT = 11, CONT_HMBF_N = 85;
+// ln(91.134 / lambda_k) [NW], ln(K_ff) [NW][NT], theta [NT], lambda_bf [N], sigma_bf [N], then
+// ln(lambda) of the H2- free-free table [CONT_H2M_NW] (from CONT_TAB_H2M on); dep holds the
+// rows below, each [ntau]: the depth state and populations, then ContopInit's factors of HOP,
+// HE1OP and HE2OP (src/contop.f90:377-470; BoltnHe1 for n = 4..27) and of COOLOP's members
+// (src/contop.f90:337-420: hc/kT in cm, kT, the level Boltzmann factors of C I, Mg I, Si I and
+// Fe I, the pseudo-continuum exponentials of C I, Mg I and Si I, ln K of the H2- table at each
+// depth, the NH table's temperature interval and weight and Fraction(NH) / Rho) and the
+// populations COOLOP reads.  mask bit s-1 switches source s; bit 20 is Ifop(1) = -1 (HOP
+// without levels 1-6).
+constexpr int CONT_HM_NW = 22, CONT_HM_NT = 11, CONT_HMBF_N = 85, CONT_H2M_NW = 18;
+constexpr int CONT_TAB_H2M = CONT_HM_NW + CONT_HM_NW * CONT_HM_NT + CONT_HM_NT + 2 * CONT_HMBF_N;
+constexpr int COOL_NC1 = 25, COOL_NMG1 = 15, COOL_NSI1 = 33, COOL_NFE1 = 48;
 enum ContRow {
   CR_T, CR_HKT, CR_TKEV, CR_THETA, CR_XNE, CR_RHO, CR_FH1, CR_FH2, CR_FHMIN, CR_FHE1, CR_FHMOL, CR_TLOG,
   CR_BOLTH, CR_FREETH = CR_BOLTH + 8, CR_BOLTEXH, CR_EXLIMH,
   CR_BOLTHE1, CR_BOLTNHE1 = CR_BOLTHE1 + 10, CR_FREETHE1 = CR_BOLTNHE1 + 24, CR_BOLTEXHE1, CR_EXLIMHE1,
   CR_BOLTHE2, CR_FREETHE2 = CR_BOLTHE2 + 9, CR_BOLTEXHE2, CR_EXLIMHE2,
+  CR_HCKT, CR_TK,
+  CR_BOLTC1, CR_BOLTMG1 = CR_BOLTC1 + COOL_NC1, CR_BOLTSI1 = CR_BOLTMG1 + COOL_NMG1,
+  CR_BOLTFE1 = CR_BOLTSI1 + COOL_NSI1, CR_PSEXP0 = CR_BOLTFE1 + COOL_NFE1, CR_PSEXP1 = CR_PSEXP0 + 3,
+  CR_H2MLK = CR_PSEXP1 + 3, CR_NHN = CR_H2MLK + CONT_H2M_NW, CR_NHW, CR_NHNRHO,
+  CR_FC1, CR_FMG1, CR_FAL1, CR_FSI1, CR_FFE1, CR_FCH, CR_FOH, CR_FRH1, CR_FRHE1,
   CR_NROW
 };
 constexpr unsigned CONT_HOP_NOLEV = 1u << 20;
 void launch_contop(int nwl, int ntau, const double *wave, double wfirst, const double *dep, const double *tab,
                    const double *cstd, int motype, unsigned mask, int add, double *contT, hipStream_t st);
+// COOLOP (Ifop 9) added into contT after launch_contop: dep and tab as above; xs holds
+// coolop_scratch_per_point() doubles per wavelength (the cross-sections of C I, Mg I, Si I)
+void launch_coolop(int nwl, int ntau, const double *wave, double wfirst, const double *dep, const double *tab,
+                   const double *cstd, int motype, double *xs, double *contT, hipStream_t st);
+constexpr int coolop_scratch_per_point() { return COOL_NC1 + COOL_NMG1 + COOL_NSI1; }
 void contop_tables(std::vector<double> &tab);  // host: the table block of launch_contop
+// host: COOLOP's rows of dep (T, Hkt and Rho rows filled first); pops: FractionPF of C I,
+// Mg I, Al I, Si I, Fe I, CH, OH, then Fraction of NH, H I, He I, each [ntau]
+void coolop_rows(int ntau, const float *const pops[10], double *dep);
 void launch_mu_sum(int nwl, int nmu, const double *rl, const double *rc, const double *w, double *sfl, double *sfc,
                    hipStream_t st);
 

@@ -139,27 +139,29 @@ class FastSpecGPU:
             self.setup = new
 
     # --- continuum opacity --------------------------------------------------------------
-    # HOP H2PLOP HMINOP HRAYOP HE1OP HE2OP HEMIOP HERAOP ELECOP H2RAOP
-    CONTOP_DEVICE_SOURCES = (1, 2, 3, 4, 5, 6, 7, 8, 12, 13)
+    # HOP H2PLOP HMINOP HRAYOP HE1OP HE2OP HEMIOP HERAOP COOLOP ELECOP H2RAOP
+    CONTOP_DEVICE_SOURCES = (1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 13)
 
-    def ContopCalc(self, fpf_h1=None, fpf_h2=None, frac_hmin=None, fpf_he1=None, frac_hmol=None,
-                   frac_he2=None, fpf_he2=None, fpf_he3=None, sources=CONTOP_DEVICE_SOURCES,
-                   hop_nolev: bool = False, add: bool = False, copy_out: bool = True):
+    def ContopCalc(self, sources=CONTOP_DEVICE_SOURCES, hop_nolev: bool = False, add: bool = False,
+                   copy_out: bool = True, **pops):
         """ContopCalc's loop (src/synth.f90:663-669) for the continuum sources the device
-        evaluates (fsg_contop): sources are Ifop numbers among 1-8, 12, 13; the populations
-        are the EOS rows [ntau] they read (FractionPF of H I, H II, He I, He II and He III,
-        Fraction of H-, H2 and He II).  hop_nolev is the reference's Ifop(1) = -1 (HOP without
-        levels 1-6, which a hydrogen line list supplies).  add=False replaces the context's
-        ContOpac with their sum, add=True adds it to the ContOpac the context holds.  Returns
-        the resulting ContOpac [nwl][ntau], which the host copy of the set-up takes too;
-        copy_out=False leaves it on the device only (returns None, and the host copy keeps its
-        earlier ContOpac)."""
+        evaluates (fsg_contop): sources are Ifop numbers among 1-9, 12, 13; pops are the EOS
+        rows [ntau] they read, by the names of fsg_contop_eos (FractionPF of H I, H II, He I,
+        He II, He III, C I, Mg I, Al I, Si I, Fe I, CH and OH; Fraction of H-, H2, He II, NH,
+        H I and He I).  hop_nolev is the reference's Ifop(1) = -1 (HOP without levels 1-6,
+        which a hydrogen line list supplies).  add=False replaces the context's ContOpac with
+        their sum, add=True adds it to the ContOpac the context holds.  Returns the resulting
+        ContOpac [nwl][ntau], which the host copy of the set-up takes too; copy_out=False
+        leaves it on the device only (returns None, and the host copy keeps its earlier
+        ContOpac)."""
         import dataclasses
         from .abi import CContopEos
         n = self.setup.ntau
+        names = {f for f, _ in CContopEos._fields_}
         arrs = {}
-        for k, v in dict(fpf_h1=fpf_h1, fpf_h2=fpf_h2, frac_hmin=frac_hmin, fpf_he1=fpf_he1,
-                         frac_hmol=frac_hmol, frac_he2=frac_he2, fpf_he2=fpf_he2, fpf_he3=fpf_he3).items():
+        for k, v in pops.items():
+            if k not in names:
+                raise TypeError(f"ContopCalc: unknown population {k!r}")
             if v is not None:
                 a = np.ascontiguousarray(v, np.float32)
                 if a.shape != (n,):

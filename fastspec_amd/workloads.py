@@ -230,12 +230,22 @@ def build_setup(atm: Atmosphere, wstart: float, wend: float, step: float, wmode:
     )
 
 
+# (name, ionisation energy [eV], log abundance, U0, U1) of the atoms COOLOP reads
+COOLOP_ATOMS = [("c1", 11.260, -3.57, 9.3, 5.9), ("mg1", 7.646, -4.40, 1.0, 2.0), ("al1", 5.986, -5.55, 5.9, 1.0),
+                ("si1", 8.152, -4.49, 9.4, 5.7), ("fe1", 7.902, -4.50, 25.0, 40.0)]
+# (name, log abundance at full association, dissociation energy [eV], T0 [K], U) of its
+# molecules; NH's row is a Fraction (U = 1)
+COOLOP_MOLECULES = [("ch", -7.5, 3.47, 3000.0, 200.0), ("oh", -7.0, 4.39, 3000.0, 150.0),
+                    ("nh", -8.0, 3.47, 3000.0, 1.0)]
+
+
 def continuum_populations(setup: Setup) -> dict:
     """The EOS rows the device's continuum sources read, for a set-up made by build_setup:
     FractionPF of H I, H II, He I and He II (their slots), Fraction of H2 and He II (twice
     its FractionPF, partition function 2), Fraction of H- from the Saha equation against H I
-    (binding 0.754 eV, statistical weights 1 and 2), and FractionPF of He III from the Saha
-    equation against He II (54.418 eV, partition functions 2 and 1)."""
+    (binding 0.754 eV, statistical weights 1 and 2), FractionPF of He III from the Saha
+    equation against He II (54.418 eV, partition functions 2 and 1), and COOLOP's rows
+    (COOLOP_ATOMS, COOLOP_MOLECULES, Fraction of H I and He I)."""
     nsp = int(setup.meta["n_atomic"])  # H I, He I, H2, H II, He II follow the atomic slots
     ev_k = EV / KB
     t = setup.temp
@@ -245,10 +255,23 @@ def continuum_populations(setup: Setup) -> dict:
     fpf_he2 = setup.fraction_pf[:, nsp + 4].copy()
     n_he2 = 2.0 * fpf_he2.astype(np.float64)
     n_he3 = n_he2 * saha_ratio(t, setup.xne, 54.418, 2.0, 1.0)
-    return dict(fpf_h1=setup.fraction_pf[:, nsp].copy(), fpf_h2=setup.fraction_pf[:, nsp + 3].copy(),
+    pops = dict(fpf_h1=setup.fraction_pf[:, nsp].copy(), fpf_h2=setup.fraction_pf[:, nsp + 3].copy(),
                 frac_hmin=n_hmin.astype(np.float32), fpf_he1=setup.fraction_pf[:, nsp + 1].copy(),
                 frac_hmol=setup.frac_hmol.copy(), frac_he2=n_he2.astype(np.float32), fpf_he2=fpf_he2,
                 fpf_he3=n_he3.astype(np.float32))
+    # COOLOP's rows: neutral C, Mg, Al, Si and Fe from the Saha equation (solar abundances,
+    # constant partition functions U0, U1), CH, OH and NH from the dissociation balance of the
+    # molecular slots, and the number densities of H I and He I
+    n_h = n_h1 + setup.fraction_pf[:, nsp + 3].astype(np.float64)
+    for name, chi, logab, u0, u1 in COOLOP_ATOMS:
+        n0 = 10.0**logab * n_h / (1.0 + saha_ratio(t, setup.xne, chi, u0, u1))
+        pops["fpf_" + name] = (n0 / u0).astype(np.float32)
+    for name, logab, d0, t0, u in COOLOP_MOLECULES:
+        n_m = 10.0**logab * n_h * np.minimum(1.0, np.exp(d0 * ev_k * (1.0 / t - 1.0 / t0)))
+        pops[("frac_" if name == "nh" else "fpf_") + name] = (n_m / u).astype(np.float32)
+    pops["frac_h1"] = setup.frac_h1.astype(np.float32)
+    pops["frac_he1"] = setup.frac_he1.astype(np.float32)
+    return pops
 
 
 def _f32pow10(x):

@@ -333,10 +333,10 @@ int fsg_set_atmosphere(fsg_ctx *ctx, int32_t atm, const fsg_atmosphere *a);
 /* Continuum opacity on the device: ContopCalc's loop over the grid (src/synth.f90:663-669,
  * Xcont at Air2Vac of each wavelength, divided by ContOpacStd for Motype 1) for the sources
  * of Xcont (src/contop.f90:267-314) that need only hydrogen, helium and electron
- * populations.  ifop_mask bit s-1 switches source s of the reference's Ifop numbering; the
- * device evaluates 1 HOP, 2 H2PLOP, 3 HMINOP, 4 HRAYOP, 5 HE1OP, 6 HE2OP, 7 HEMIOP, 8 HERAOP,
- * 12 ELECOP and 13 H2RAOP (bits 8-10, COOLOP, LUKEOP and HOTOP, are refused: the host's Xcont
- * keeps them).  Bit FSG_CONTOP_HOP_NOLEV (with bit 0) is the reference's Ifop(1) = -1, set
+ * populations, and COOLOP's atoms and molecules.  ifop_mask bit s-1 switches source s of the
+ * reference's Ifop numbering; the device evaluates 1 HOP, 2 H2PLOP, 3 HMINOP, 4 HRAYOP, 5 HE1OP,
+ * 6 HE2OP, 7 HEMIOP, 8 HERAOP, 9 COOLOP, 12 ELECOP and 13 H2RAOP (bits 9 and 10, LUKEOP and
+ * HOTOP, are refused: the host's Xcont keeps them).  Bit FSG_CONTOP_HOP_NOLEV (with bit 0) is the reference's Ifop(1) = -1, set
  * by ContopSetup when a hydrogen line list supplies the bound-free edges of levels 1-6
  * (src/synth.f90:600): HOP then omits those levels.  T, Xne and Rho are the context's
  * (fsg_init / fsg_set_atmosphere); the populations are the EOS rows the sources read, as
@@ -356,6 +356,17 @@ typedef struct fsg_contop_eos {
   const float *frac_he2;   /* [ntau] Fraction(IxHe2, :)    He II (HE1OP free-free) */
   const float *fpf_he2;    /* [ntau] FractionPF(IxHe2, :)  He II (HE2OP) */
   const float *fpf_he3;    /* [ntau] FractionPF(IxHe3, :)  He III (HE2OP free-free) */
+  /* COOLOP (src/contop.f90:1544-1569) */
+  const float *fpf_c1;     /* [ntau] FractionPF(IxC1, :)   C I   (C1OP) */
+  const float *fpf_mg1;    /* [ntau] FractionPF(IxMg1, :)  Mg I  (MG1OP) */
+  const float *fpf_al1;    /* [ntau] FractionPF(IxAl1, :)  Al I  (AL1OP) */
+  const float *fpf_si1;    /* [ntau] FractionPF(IxSi1, :)  Si I  (SI1OP) */
+  const float *fpf_fe1;    /* [ntau] FractionPF(IxFe1, :)  Fe I  (FE1OP) */
+  const float *fpf_ch;     /* [ntau] FractionPF(IxCH, :)   CH    (CHOP) */
+  const float *fpf_oh;     /* [ntau] FractionPF(IxOH, :)   OH    (OHOP) */
+  const float *frac_nh;    /* [ntau] Fraction(IxNH, :)     NH    (NHOP) */
+  const float *frac_h1;    /* [ntau] Fraction(IxH1, :)     H I   (H2COLLOP) */
+  const float *frac_he1;   /* [ntau] Fraction(IxHe1, :)    He I  (H2COLLOP; with frac_hmol, H2COLLOP and H2MINOP) */
 } fsg_contop_eos;
 int fsg_contop(fsg_ctx *ctx, const fsg_contop_eos *eos, int32_t ifop_mask, int32_t add, double *contop_out);
 

@@ -16,13 +16,14 @@ MODULE RefContop
 
 CONTAINS
 
-  ! EOS slots of the table: 1 H I, 2 H II, 3 H-, 4 He I, 5 H2, 6 He II, 7 He III, 8 every
-  ! other species (empty).  fpf, frac: FractionPF and Fraction (8, ntau).  ifop_(20): the
-  ! source switches (OpFlags).  out(ntau, n): Copac at each wavelength wl (vacuum, A).
+  ! EOS slots of the table: 1 H I, 2 H II, 3 H-, 4 He I, 5 H2, 6 He II, 7 He III, 8 C I,
+  ! 9 Mg I, 10 Al I, 11 Si I, 12 Fe I, 13 CH, 14 OH, 15 NH, 16 every other species (empty).
+  ! fpf, frac: FractionPF and Fraction (16, ntau).  ifop_(20): the source switches (OpFlags).
+  ! out(ntau, n): Copac at each wavelength wl (vacuum, A).
   SUBROUTINE ref_xcont(ntau_, temp, xne, xna, rho, fpf, frac, ifop_, n, wl, out) BIND(C, NAME='ref_xcont')
     INTEGER(C_INT), VALUE :: ntau_, n
     REAL(C_DOUBLE), INTENT(IN) :: temp(ntau_), xne(ntau_), xna(ntau_), rho(ntau_), wl(n)
-    REAL(C_FLOAT), INTENT(IN) :: fpf(8, ntau_), frac(8, ntau_)
+    REAL(C_FLOAT), INTENT(IN) :: fpf(16, ntau_), frac(16, ntau_)
     INTEGER(C_INT), INTENT(IN) :: ifop_(20)
     REAL(C_DOUBLE), INTENT(OUT) :: out(ntau_, n)
     REAL(8) :: Copac(ntau_)
@@ -31,7 +32,7 @@ CONTAINS
     Ntau = ntau_
     IF (ALLOCATED(Fraction)) DEALLOCATE(Fraction)
     IF (ALLOCATED(FractionPF)) DEALLOCATE(FractionPF)
-    ALLOCATE(Fraction(8, Ntau), FractionPF(8, Ntau))
+    ALLOCATE(Fraction(16, Ntau), FractionPF(16, Ntau))
     Fraction = frac
     FractionPF = fpf
     IF (ALLOCATED(CT)) DEALLOCATE(CT, CTlog, CTkev, CTk, CHkt, CEhvkt, CStim, CXne, CXna, CRho)
@@ -47,9 +48,9 @@ CONTAINS
     CHkt = Hc / CTk
     Ifop = ifop_
     IxH1 = 1; IxH2 = 2; IxHmin = 3; IxHe1 = 4; IxHmol = 5; IxHe2 = 6; IxHe3 = 7
-    IxC1 = 8; IxC2 = 8; IxN1 = 8; IxO1 = 8; IxMg1 = 8; IxMg2 = 8; IxAl1 = 8; IxSi1 = 8; IxSi2 = 8
-    IxCa2 = 8; IxFe1 = 8; IxCH = 8; IxOH = 8; IxNH = 8
-    IxHot = 8
+    IxC1 = 8; IxMg1 = 9; IxAl1 = 10; IxSi1 = 11; IxFe1 = 12; IxCH = 13; IxOH = 14; IxNH = 15
+    IxC2 = 16; IxN1 = 16; IxO1 = 16; IxMg2 = 16; IxSi2 = 16; IxCa2 = 16
+    IxHot = 16
     CALL ContopInit
     DO i = 1, n
       CALL Xcont(wl(i), Copac)

@@ -505,26 +505,38 @@ def ref_convolgau(setup: Setup, s, fwhm: float):
 def ref_xcont(temp, xne, rho, pops: dict, sources, wl_vac, xna=None, hop_nolev=False) -> np.ndarray:
     """The reference's Xcont (src/contop.f90:267-314), compiled from its sources, at vacuum
     wavelengths wl_vac with only the Ifop sources given switched on (hop_nolev: Ifop(1) = -1):
-    Copac [n][ntau].  pops holds the EOS rows (fpf_h1, fpf_h2, frac_hmin, fpf_he1, frac_hmol
-    and, when present, frac_he2, fpf_he2, fpf_he3) that the harness puts in its slots 1 H I,
-    2 H II, 3 H-, 4 He I, 5 H2, 6 He II, 7 He III (the metals empty)."""
+    Copac [n][ntau].  pops holds the EOS rows (fsg_contop_eos's names) that the harness puts
+    in its slots 1 H I, 2 H II, 3 H-, 4 He I, 5 H2, 6 He II, 7 He III, 8 C I, 9 Mg I, 10 Al I,
+    11 Si I, 12 Fe I, 13 CH, 14 OH, 15 NH (16, the other species, empty); rows not given stay
+    zero."""
     lib = ref_lib()
     n = int(len(wl_vac))
     ntau = int(len(temp))
-    fpf = np.zeros((ntau, 8), np.float32)
-    frac = np.zeros((ntau, 8), np.float32)
+    fpf = np.zeros((ntau, 16), np.float32)
+    frac = np.zeros((ntau, 16), np.float32)
     fpf[:, 0] = pops["fpf_h1"]
     fpf[:, 1] = pops["fpf_h2"]
     frac[:, 2] = pops["frac_hmin"]
     fpf[:, 3] = pops["fpf_he1"]
     frac[:, 4] = pops["frac_hmol"]
     # Fraction and FractionPF of the other rows as the harness reads them: H I, H II and He I
-    # with their FractionPF, H- and H2 with their Fraction
+    # with their FractionPF (Fraction of H I and He I, which H2COLLOP reads, when given), H-
+    # and H2 with their Fraction
     frac[:, 0], frac[:, 1], frac[:, 3] = fpf[:, 0], fpf[:, 1], fpf[:, 3]
     fpf[:, 2], fpf[:, 4] = frac[:, 2], frac[:, 4]
+    if "frac_h1" in pops:
+        frac[:, 0] = pops["frac_h1"]
+    if "frac_he1" in pops:
+        frac[:, 3] = pops["frac_he1"]
     if "fpf_he2" in pops:
         fpf[:, 5], frac[:, 5] = pops["fpf_he2"], pops["frac_he2"]
         fpf[:, 6] = frac[:, 6] = pops["fpf_he3"]
+    for slot, key in ((7, "fpf_c1"), (8, "fpf_mg1"), (9, "fpf_al1"), (10, "fpf_si1"), (11, "fpf_fe1"),
+                      (12, "fpf_ch"), (13, "fpf_oh")):
+        if key in pops:
+            fpf[:, slot] = frac[:, slot] = pops[key]
+    if "frac_nh" in pops:
+        fpf[:, 14] = frac[:, 14] = pops["frac_nh"]
     ifop = np.zeros(20, np.int32)
     for s in sources:
         ifop[int(s) - 1] = 1

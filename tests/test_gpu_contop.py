@@ -2,8 +2,11 @@
 from its sources (oracle/_ref, src/contop.f90:267-314), on the configurations' grids: the
 plain configuration (5000-5100 A, solar), a slice of the cool-dwarf log grid (3200 K), the
 H- bound-free threshold in the infrared, the near-UV of an 8000 K model and of a 15000 K
-model (He II populated), and the far UV of the 15000 K model (220-950 A: the He II, He I and
-H I edges, and the hydrogenic He I levels below 240 A).  Bound: 1e-12 relative per
+model (He II populated), the far UV of the 15000 K model (220-950 A: the He II, He I and
+H I edges, and the hydrogenic He I levels below 240 A), and COOLOP's ranges on cool models:
+the C I, Si I, Mg I and Al I edges in the far UV of a 3500 K model, Fe I, CH, OH and NH in
+its visible, and the collision-induced and H2- free-free absorption in the red and infrared
+of a 3000 K model.  Bound: 1e-12 relative per
 (wavelength, depth), every source alone and all together; HOP without its levels 1-6
 (Ifop(1) = -1); add mode, Motype 1 and the refusal of sources the host keeps; and a
 synthesis that uses the device continuum against the oracle's with the reference
@@ -19,7 +22,7 @@ from oracle_bridge import Oracle, ref_air2vac_array, ref_xcont
 
 pytestmark = [pytest.mark.gpu, pytest.mark.ref]
 
-SOURCES = (1, 2, 3, 4, 5, 6, 7, 8, 12, 13)
+SOURCES = (1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 13)
 REL = 1e-12
 # H2PLOP's exponent is a quartic in ln(frequency) whose terms, ~1e4, cancel to ~-90: one ulp
 # of the logarithm (device and host libm may differ by one) moves H2+ opacity by ~2e-12
@@ -34,7 +37,12 @@ def _grids():
     uv = W.build_setup(W.grey_atmosphere(8000.0, 4.0), 3000.0, 3100.0, 0.05, 0)
     hot = W.build_setup(W.grey_atmosphere(15000.0, 4.0), 3000.0, 3100.0, 0.05, 0)
     fuv = W.build_setup(W.grey_atmosphere(15000.0, 4.0), 220.0, 950.0, 0.25, 0)
-    return dict(c1=c1, c3=c3, ir_edge=ir, uv=uv, hot=hot, fuv=fuv)
+    cool_fuv = W.build_setup(W.grey_atmosphere(3500.0, 4.5), 900.0, 2100.0, 0.4, 0)
+    cool_vis = W.build_setup(W.grey_atmosphere(3500.0, 4.5), 4000.0, 6500.0, 1.0, 0)
+    cool_red = W.build_setup(W.grey_atmosphere(3000.0, 5.0), 6500.0, 12000.0, 2.0, 0)
+    cool_ir = W.build_setup(W.grey_atmosphere(3000.0, 5.0), 12000.0, 200000.0, 50.0, 0)
+    return dict(c1=c1, c3=c3, ir_edge=ir, uv=uv, hot=hot, fuv=fuv, cool_fuv=cool_fuv, cool_vis=cool_vis,
+                cool_red=cool_red, cool_ir=cool_ir)
 
 
 GRIDS = _grids()
@@ -62,8 +70,10 @@ def test_all_device_sources_match_reference(name):
 
 
 # every source on the solar grid, the helium and hydrogen bound-free sources also on the hot
-# models (He II is empty in a solar model)
-EACH = [(src, "c1") for src in SOURCES if src != 6] + [(src, g) for src in (1, 5, 6) for g in ("hot", "fuv")]
+# models (He II is empty in a solar model), COOLOP on the cool grids and on the hot near UV
+# (CH and OH above 9000 K and NH above 10000 K add nothing)
+EACH = ([(src, "c1") for src in SOURCES if src != 6] + [(src, g) for src in (1, 5, 6) for g in ("hot", "fuv")] +
+        [(9, g) for g in ("cool_fuv", "cool_vis", "cool_red", "cool_ir", "uv", "c3")])
 
 
 @pytest.mark.parametrize("src,grid", EACH)
@@ -115,9 +125,12 @@ def test_host_sources_refused():
     from fastspec_amd._lib import FsgError
     s = GRIDS["c1"]
     pops = W.continuum_populations(s)
-    for src in (9, 10, 11):
+    for src in (10, 11):
         with pytest.raises(FsgError, match="not evaluated on the device"):
             _gpu(s, pops, SOURCES + (src,))
+    for key in ("fpf_c1", "fpf_ch", "frac_nh", "frac_he1"):
+        with pytest.raises(FsgError, match="COOLOP needs"):
+            _gpu(s, dict(pops, **{key: None}), (9,))
     with pytest.raises(FsgError, match="population"):
         _gpu(s, dict(pops, frac_hmin=None), (3,))
     with pytest.raises(FsgError, match="population"):
