@@ -486,9 +486,10 @@ def run_c3(steps: int, fp64: bool):
 
 
 def run_contop(setup):
-    """fsg_contop over the C2 grid: the ten continuum sources of Xcont the device evaluates
-    (H I, H2+, H-, He I, He II, He-, Rayleigh by H I, He I and H2, Thomson), one lane per
-    wavelength, 72 depths, FP64, ContOpac left on the device.  The reference's Xcont
+    """fsg_contop over the C2 grid: the eleven continuum sources of Xcont the device
+    evaluates (H I, H2+, H-, He I, He II, He-, COOLOP's metals and molecules, Rayleigh by
+    H I, He I and H2, Thomson), one lane per wavelength, 72 depths, FP64, ContOpac left on
+    the device.  The reference's Xcont
     (oracle/_ref, -O3, one thread: its harness loops serially; ContopCalc is an OpenMP loop
     over wavelengths) is timed on 20,000 of the grid's wavelengths, for these sources and for
     all thirteen."""
