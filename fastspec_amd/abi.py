@@ -91,7 +91,9 @@ class CContopEos(C.Structure):
     _fields_ = [("fpf_h1", _fp), ("fpf_h2", _fp), ("frac_hmin", _fp), ("fpf_he1", _fp), ("frac_hmol", _fp),
                 ("frac_he2", _fp), ("fpf_he2", _fp), ("fpf_he3", _fp),
                 ("fpf_c1", _fp), ("fpf_mg1", _fp), ("fpf_al1", _fp), ("fpf_si1", _fp), ("fpf_fe1", _fp),
-                ("fpf_ch", _fp), ("fpf_oh", _fp), ("frac_nh", _fp), ("frac_h1", _fp), ("frac_he1", _fp)]
+                ("fpf_ch", _fp), ("fpf_oh", _fp), ("frac_nh", _fp), ("frac_h1", _fp), ("frac_he1", _fp),
+                ("fpf_c2", _fp), ("fpf_mg2", _fp), ("fpf_si2", _fp), ("fpf_n1", _fp), ("fpf_o1", _fp),
+                ("fpf_ca2", _fp), ("hot_frac", _fp), ("hot_fpf", _fp)]
 
 
 class CStats(C.Structure):

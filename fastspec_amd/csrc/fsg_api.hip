@@ -876,9 +876,8 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
   const unsigned mask = (unsigned)ifop_mask & ~CONT_HOP_NOLEV;
   if ((ifop_mask & FSG_CONTOP_HOP_NOLEV) && !(mask & 1u))
     return fail("fsg_contop: FSG_CONTOP_HOP_NOLEV qualifies HOP (bit 0), which is not switched on");
-  // HOP, H2PLOP, HMINOP, HRAYOP, HE1OP, HE2OP, HEMIOP, HERAOP, COOLOP, ELECOP, H2RAOP (Ifop 1-9,
-  // 12, 13)
-  const unsigned on_device = 0x1FFu | (1u << 11) | (1u << 12);
+  // all thirteen sources of Xcont (Ifop 1-13)
+  const unsigned on_device = 0x1FFFu;
   if (mask & ~on_device) {
     std::string m = "fsg_contop: continuum source(s)";
     for (int b = 0; b < 32; b++)
@@ -897,6 +896,12 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
     for (const float *q : cool_pops)
       if (!q) return fail("fsg_contop: COOLOP needs the C I, Mg I, Al I, Si I, Fe I, CH, OH, NH, H I and He I rows");
   }
+  const float *const luke_pops[6] = {e->fpf_c2, e->fpf_mg2, e->fpf_si2, e->fpf_n1, e->fpf_o1, e->fpf_ca2};
+  if (on(10))
+    for (const float *q : luke_pops)
+      if (!q) return fail("fsg_contop: LUKEOP needs the C II, Mg II, Si II, N I, O I and Ca II rows");
+  if (on(11) && (!e->hot_frac || !e->hot_fpf))
+    return fail("fsg_contop: HOTOP needs hot_frac and hot_fpf");
   CK(hipSetDevice(c->dev));
   const DevSetup &S = c->S;
   const size_t N = (size_t)S.ntau, Nw = (size_t)S.nwl;
@@ -972,6 +977,9 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
   }
   // ContopInit's factors of COOLOP's members (src/contop.f90:337-420)
   if (on(9)) coolop_rows((int)N, cool_pops, blk.data());
+  // and of LUKEOP's members and HOTOP (src/contop.f90:337-366, 462-490)
+  if (on(10)) lukeop_rows((int)N, luke_pops, blk.data());
+  if (on(11)) hotop_rows((int)N, e->hot_frac, e->hot_fpf, blk.data());
   std::copy(tab.begin(), tab.end(), blk.begin() + CR_NROW * N);
   CK(c->ctab.ensure(sizeof(double) * blk.size()));
   double *dblk = c->ctab.as<double>();
@@ -979,11 +987,14 @@ int fsg_contop(fsg_ctx *c, const fsg_contop_eos *e, int32_t ifop_mask, int32_t a
   const unsigned kmask = mask | ((ifop_mask & FSG_CONTOP_HOP_NOLEV) ? CONT_HOP_NOLEV : 0u);
   launch_contop(S.nwl, S.ntau, S.wave, S.wfirst, dblk, dblk + CR_NROW * N, S.cstd, S.motype, kmask, add ? 1 : 0,
                 const_cast<double *>(S.contT), s);
-  if (on(9)) {
-    CK(c->cxs.ensure(sizeof(double) * coolop_scratch_per_point() * Nw));
+  if (on(9) || on(10) || on(11))
+    CK(c->cxs.ensure(sizeof(double) * std::max(coolop_scratch_per_point(), luke_scratch_per_point()) * Nw));
+  if (on(9))
     launch_coolop(S.nwl, S.ntau, S.wave, S.wfirst, dblk, dblk + CR_NROW * N, S.cstd, S.motype, c->cxs.as<double>(),
                   const_cast<double *>(S.contT), s);
-  }
+  if (on(10) || on(11))
+    launch_lukehot(S.nwl, S.ntau, S.wave, S.wfirst, dblk, dblk + CR_NROW * N, S.cstd, S.motype, mask,
+                   c->cxs.as<double>(), const_cast<double *>(S.contT), s);
   CK(hipGetLastError());
   if (contop_out) {  // [ntau][nwl] -> ContOpac(Ntau, Nwl)
     std::vector<double> h(N * Nw);

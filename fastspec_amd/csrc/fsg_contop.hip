@@ -34,6 +34,7 @@
 
 #include "fsg_contop_tables.h"
 #include "fsg_coolop_tables.h"
+#include "fsg_ionop_tables.h"
 #include "fsg_device.h"
 #include "fsg_kernels.h"
 
@@ -49,14 +50,14 @@ constexpr int KL_NP = 29;                 // tabulated points per level of the K
 // log10(cross-section) against log10(freq / zeff2) in the table of the level; 0 below the
 // threshold.  The bracketing point is the first tabulated frequency below the argument
 // (the tables descend); above the table's top it is the top interval.
-__device__ double xkarzas(double freq, double zeff2, int n, int l) {
+__host__ __device__ double xkarzas(double freq, double zeff2, int n, int l) {
   const double flg = log10(freq / zeff2);
   const double *fn = kKlFreqn + (n - 1) * KL_NP;
   const double *x = (l >= n || n > 6) ? kKlXn + (n - 1) * KL_NP : kKlXl + (l * 6 + (n - 1)) * KL_NP;
   if (flg < fn[KL_NP - 1]) return 0.0;
   int i = 2;  // 1-based, as the reference's loop index
   while (i <= KL_NP && !(flg > fn[i - 2])) i++;
-  i = max(i, 3);
+  i = i < 3 ? 3 : i;
   const double X = (flg - fn[i - 2]) / (fn[i - 3] - fn[i - 2]) * (x[i - 3] - x[i - 2]) + x[i - 2];
   return exp(X * 2.30258509299405) / zeff2;
 }
@@ -65,7 +66,8 @@ __device__ double xkarzas(double freq, double zeff2, int n, int l) {
 // 2) at depth temperature ln T = tlog and ln(frequency) = freqlg, bilinear in the table of
 // log(gamma^2) and log(h nu / kT)
 __device__ double coulff(double tlog, double freqlg, int nz) {
-  const double z4log = nz == 1 ? 0.0 : 1.20412;
+  const double z4 [6] = {0.0, 1.20412, 1.90849, 2.40824, 2.79588, 3.11261};  // log10 of Z^4
+  const double z4log = z4[nz - 1];
   const double gamlog = 10.39638 - tlog / 1.15129 + z4log;
   const int igam = max(min((int)(gamlog + 7.0), 10), 1);
   const double hvktlg = (freqlg - tlog) / 1.15129 - 20.63764;
@@ -404,10 +406,11 @@ void launch_contop(int nwl, int ntau, const double *wave, double wfirst, const d
 // NH photodissociation, nothing above the H I Lyman edge.
 // ------------------------------------------------------------------------------------
 constexpr double kCoolRyd = 109732.298;
-// series limits and edges of the hydrogenic pseudo-continua of C I, Mg I and Si I
-// (src/contop.f90:72-77)
-constexpr double kPsElim[3] = {90820.42, 61671.02, 65747.55};
-constexpr double kPsEdge[3] = {kCoolRyd / 16.0, kCoolRyd / 25.0, kCoolRyd / 25.0};
+// series limits and edges of the hydrogenic pseudo-continua of C I, Mg I, Si I, C II (two)
+// and Mg II (src/contop.f90:72-77)
+constexpr double kPsElim[6] = {90820.42, 61671.02, 65747.55, 196664.7, 196664.7 + 52367.06, 121267.61};
+constexpr double kPsEdge[6] = {kCoolRyd / 16.0, kCoolRyd / 25.0, kCoolRyd / 25.0,
+                               kCoolRyd * 4.0 / 36.0, kCoolRyd * 4.0 / 16.0, kCoolRyd * 4.0 / 64.0};
 constexpr int COOL_NX = COOL_NC1 + COOL_NMG1 + COOL_NSI1;  // the three atoms' cross-sections per lane
 
 __device__ __forceinline__ double pow10d(double x) { return pow(10.0, x); }
@@ -729,6 +732,269 @@ void coolop_rows(int ntau, const float *const pops[10], double *dep) {
 }
 
 // ------------------------------------------------------------------------------------
+// LUKEOP (src/contop.f90:2422-2440): C II, Mg II and Si II bound-free, N I, O I and Ca II by
+// Seaton's fits; HOTOP (src/contop.f90:2729-2763): free-free of the ions of C, N, O, Ne, Mg,
+// Si, S and Fe and the bound-free edges of C II-IV, N II-V, O II-VI and Ne I-VI.
+// ------------------------------------------------------------------------------------
+// SEATON (src/contop.f90:494-498)
+__device__ __forceinline__ double seaton(double freq, double freq0, double xsect, double power, double a) {
+  const double r = freq0 / freq;
+  return xsect * (a + (1.0 - a) * r) * sqrt(pow(r, (double)(int)(2.0 * power + 0.01)));
+}
+
+// C2OP's cross-sections (src/contop.f90:2447-2484), X[i * ld] for the 34 levels
+__device__ void c2op_x(double Freq, double Waveno, double *X, size_t ld) {
+  for (int i = 0; i < LUKE_NC2; i++) X[i * ld] = 0.0;
+  double Elim = 196664.7;  // 2s2 nl (n = 5, 4, 3) to C III 2s2 1S
+  for (int i = 0; i < 12; i++) {
+    if (Waveno < Elim - kC2Elev[i]) break;
+    const double zeff2 = (double)(kC2Nlv1[i] * kC2Nlv1[i]) / kCoolRyd * (Elim - kC2Elev[i]);
+    X[i * ld] = xkarzas(Freq, zeff2, kC2Nlv1[i], kC2Llv1[i]);
+  }
+  Elim = 196664.7 + 52367.06;  // 2s2p 3l to C III 2s2p 3P
+  for (int i = 13; i < 27; i++) {
+    if (Waveno < Elim - kC2Elev[i]) break;
+    X[i * ld] = xkarzas(Freq, 9.0 / kCoolRyd * (Elim - kC2Elev[i]), 3, kC2Llv2[i - 13]);
+  }
+  Elim = 196664.7 + 137425.70;  // 2p3 to C III 2p2 3P
+  for (int i = 31; i < 34; i++) {
+    if (Waveno < Elim - kC2Elev[i]) break;
+    X[i * ld] = xkarzas(Freq, 4.0 / kCoolRyd * (Elim - kC2Elev[i]), 2, 1) * 3.0;
+  }
+}
+
+// MG2OP's cross-sections (src/contop.f90:2531-2553), X[i * ld] for the 14 levels
+__device__ void mg2op_x(double Freq, double Waveno, double *X, size_t ld) {
+  const double Elim = 121267.61;
+  for (int i = 0; i < 13; i++) {
+    X[i * ld] = 0.0;
+    if (Waveno < Elim - kMg2Elev[i]) continue;
+    const double zeff2 = (double)(kMg2Nlv[i] * kMg2Nlv[i]) / kCoolRyd * (Elim - kMg2Elev[i]);
+    X[i * ld] = xkarzas(Freq, zeff2, kMg2Nlv[i], kMg2Llv[i]);
+  }
+  X[13 * ld] = 0.0;
+  if (Waveno >= Elim - kMg2Elev[13]) {
+    const double r = (Elim - kMg2Elev[13]) / Waveno;
+    X[13 * ld] = 0.14e-18 * (6.700 * pow(r, 4.0) - 5.700 * pow(r, 5.0));
+  }
+}
+
+// One lane per wavelength: LUKEOP's and HOTOP's sums added into contT, in Xcont's order
+// (Aluke, then Ahot).  C II's and Mg II's 48 cross-sections go to scratch xs [48][nwl] with a
+// ballot per level as in k_coolop; the Si II table comes from tab; HOTOP takes its edges in
+// the reference's order, each one's addition depending on the sum before it.
+__global__ void __launch_bounds__(64, 3) k_lukehot(int nwl, int ntau, const double *__restrict__ wave, double wfirst,
+                                                   const double *__restrict__ dep, const double *__restrict__ tab,
+                                                   const double *__restrict__ cstd, int motype, unsigned mask,
+                                                   double *__restrict__ xs, double *__restrict__ contT) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nwl) return;
+  auto row = [dep, ntau](int r) { return dep + (size_t)r * ntau; };
+  const bool luke = (mask >> 9) & 1, hot = (mask >> 10) & 1;
+  const double clight = kVc * 1e13;
+  const double Freq = clight / air2vac(wave[p] + wfirst);
+  const double Freqlg = log(Freq);
+  const double Waveno = Freq / kClightCm;
+  constexpr int NL = LUKE_NC2 + LUKE_NMG2;
+  double *X = xs + p;  // X[i * nwl]
+  uint64_t mx = 0;
+  double x853 = 0.0, x1020 = 0.0, x1130 = 0.0, xo1 = 0.0, x1044 = 0.0, x1218 = 0.0, x1420 = 0.0;
+  bool blue[3] = {false, false, false}, si2tab = false;
+  int iw = 1;
+  double wfrac = 0.0, freq3 = 0.0;
+  if (luke) {
+    c2op_x(Freq, Waveno, X, (size_t)nwl);
+    mg2op_x(Freq, Waveno, X + (size_t)LUKE_NC2 * nwl, (size_t)nwl);
+    for (int i = 0; i < NL; i++)
+      if (__builtin_amdgcn_ballot_w64(X[(size_t)i * nwl] != 0.0)) mx |= 1ull << i;
+    for (int n = 0; n < 3; n++) blue[n] = kPsElim[3 + n] - kPsEdge[3 + n] >= kPsElim[3 + n] - Waveno;
+    freq3 = 2.815e29 / Freq / Freq / Freq * 16.0;
+    // N1OP, O1OP, CA2OP (src/contop.f90:2505-2529, 2708-2727)
+    if (Freq >= 3.517915e15) x853 = seaton(Freq, 3.517915e15, 1.142e-17, 2.0, 4.29);
+    if (Freq >= 2.941534e15) x1020 = seaton(Freq, 2.941534e15, 4.410e-18, 1.5, 3.85);
+    if (Freq >= 2.653317e15) x1130 = seaton(Freq, 2.653317e15, 4.200e-18, 1.5, 4.34);
+    if (Freq >= 3.28805e15) xo1 = 9.0 * seaton(Freq, 3.28805e15, 2.94e-18, 1.0, 2.66);
+    if (Freq >= 2.870454e15) {
+      double x = 2.870454e15 / Freq;
+      x = x * x * x;
+      x1044 = 1.08e-19 * x;
+    }
+    if (Freq >= 2.460127e15) x1218 = 1.64e-17 * sqrt(2.460127e15 / Freq);
+    if (Freq >= 2.110779e15) x1420 = seaton(Freq, 2.110779e15, 4.13e-18, 3.0, 0.69);
+    // SI2OP (src/contop.f90:2572-2596): the table's wavenumber interval
+    si2tab = Waveno >= 12192.48;
+    if (si2tab) {
+      iw = max(min((int)(Waveno * 0.001), CONT_SI2_NW - 1), 1);
+      wfrac = (Waveno - (double)iw * 1000.0) / 1000.0;
+    }
+  }
+  const double *si2 = tab + CONT_TAB_SI2;  // [nu][k]
+  const double *T = row(CR_T), *Hkt = row(CR_HKT), *Hckt = row(CR_HCKT), *Tlog = row(CR_TLOG);
+  const double *Xne = row(CR_XNE), *Rho = row(CR_RHO);
+  for (int j = 0; j < ntau; j++) {
+    const double ehvkt = exp(-Freq * Hkt[j]);
+    const double stim = 1.0 - ehvkt;
+    const double rho = Rho[j];
+    double add = 0.0;
+    if (luke) {
+      const double hckt = Hckt[j];
+      // C2OP: the 2s2 nl (n >= 6) and 2s2p nl (n >= 4) pseudo-continua, then the levels
+      double e1 = row(CR_PSEXP1 + 3)[j];
+      if (!blue[0]) e1 = exp(-(kPsElim[3] - Waveno) * hckt);
+      double h = freq3 * 1.0 / (kCoolRyd * 4.0 * hckt) * (e1 - row(CR_PSEXP0 + 3)[j]);
+      e1 = row(CR_PSEXP1 + 4)[j];
+      if (!blue[1]) e1 = exp(-(kPsElim[4] - Waveno) * hckt);
+      h = h + freq3 * 9.0 / (kCoolRyd * 4.0 * hckt) * (e1 - row(CR_PSEXP0 + 4)[j]);
+      for (int i = 0; i < LUKE_NC2; i++)
+        if ((mx >> i) & 1ull) h = h + X[(size_t)i * nwl] * row(CR_BOLTC2 + i)[j];
+      const double ac2 = h * row(CR_FC2)[j] * stim / rho;
+      // MG2OP: the n >= 8 pseudo-continuum, then the levels
+      e1 = row(CR_PSEXP1 + 5)[j];
+      if (!blue[2]) e1 = exp(-(kPsElim[5] - Waveno) * hckt);
+      h = freq3 / (kCoolRyd * 4.0 * hckt) * (e1 - row(CR_PSEXP0 + 5)[j]);
+      for (int i = 0; i < LUKE_NMG2; i++)
+        if ((mx >> (LUKE_NC2 + i)) & 1ull) h = h + X[(size_t)(LUKE_NC2 + i) * nwl] * row(CR_BOLTMG2 + i)[j];
+      const double amg2 = h * row(CR_FMG2)[j] * stim / rho;
+      // SI2OP: the table, or below it the high levels' free-bound limit
+      double asi2;
+      if (si2tab) {
+        const int it = (int)row(CR_SI2IT)[j];
+        const double tf = row(CR_SI2TF)[j];
+        const double *a = si2 + (size_t)(iw - 1) * CONT_SI2_NT, *b = a + CONT_SI2_NT;
+        const double hs = (a[it - 1] * (1.0 - tf) + a[it] * tf) * (1.0 - wfrac) + (b[it - 1] * (1.0 - tf) + b[it] * tf) * wfrac;
+        asi2 = exp(hs) * row(CR_FSI2)[j] * stim / rho;
+      } else {
+        const double hs = freq3 * (1.0 / ehvkt - 1.0) * row(CR_SI2BN)[j];
+        asi2 = hs * row(CR_FSI2)[j] * stim / rho;
+      }
+      const double an1 = x853 * 4.0 + x1020 * row(CR_BOLTN1 + 1)[j] + x1130 * row(CR_BOLTN1)[j];
+      const double aca2 = x1044 + x1218 * row(CR_BOLTCA2)[j] + x1420 * row(CR_BOLTCA2 + 1)[j];
+      add = ac2 + amg2 + asi2 + (an1 * row(CR_FN1)[j] + xo1 * row(CR_FO1)[j] + aca2 * row(CR_FCA2)[j]) * stim / rho;
+    }
+    if (hot) {
+      const double tlog = Tlog[j];
+      double free = 0.0;
+      for (int z = 1; z <= 5; z++) {
+        const double t = coulff(tlog, Freqlg, z) * (double)(z * z) * row(CR_HOTFF + z - 1)[j];
+        free = z == 1 ? t : free + t;
+      }
+      double ahot = free * 3.6919e8 / Freq / Freq / Freq * Xne[j] / sqrt(T[j]);
+      for (int i = 0; i < HOT_NEDGE; i++) {
+        const double *a = kHotA + 7 * i;
+        if (Freq < a[0]) continue;
+        const double r = a[0] / Freq;
+        const double xsect = a[1] * (a[2] + r - a[2] * r) * sqrt(pow(r, (double)(int)a[3]));
+        const double xx = xsect * row(CR_XNFP + (int)a[6] - 1)[j] * a[4];
+        if (xx > ahot / 100.0) ahot = ahot + xx / row(CR_HOTEXP + i)[j];
+      }
+      ahot = ahot * stim / rho;
+      add = add + ahot;
+    }
+    if (motype == 1) add = add / cstd[j];
+    const size_t o = (size_t)j * nwl + p;
+    contT[o] = contT[o] + add;
+  }
+}
+
+void launch_lukehot(int nwl, int ntau, const double *wave, double wfirst, const double *dep, const double *tab,
+                    const double *cstd, int motype, unsigned mask, double *xs, double *contT, hipStream_t st) {
+  k_lukehot<<<(nwl + 63) / 64, 64, 0, st>>>(nwl, ntau, wave, wfirst, dep, tab, cstd, motype, mask, xs, contT);
+}
+
+// ContopInit's factors of LUKEOP's members (src/contop.f90:337-366): C II and Mg II level
+// Boltzmann factors, N I and Ca II ones, the pseudo-continuum exponentials of C II and Mg II,
+// and the Si II table's temperature interval, weight and long-wavelength factor
+void lukeop_rows(int ntau, const float *const pops[6], double *dep) {
+  const size_t N = (size_t)ntau;
+  auto col = [dep, N](int r) { return dep + (size_t)r * N; };
+  const double *Hkt = col(CR_HKT), *Tkev = col(CR_TKEV), *Tlog = col(CR_TLOG);
+  for (size_t j = 0; j < N; j++) {
+    const double hckt = kClightCm * Hkt[j];
+    col(CR_HCKT)[j] = hckt;
+    for (int i = 0; i < LUKE_NC2; i++) col(CR_BOLTC2 + i)[j] = kC2Glev[i] * std::exp(-kC2Elev[i] * hckt);
+    for (int i = 0; i < LUKE_NMG2; i++) col(CR_BOLTMG2 + i)[j] = kMg2Glev[i] * std::exp(-kMg2Elev[i] * hckt);
+    col(CR_BOLTN1)[j] = 6.0 * std::exp(-3.575 / Tkev[j]);
+    col(CR_BOLTN1 + 1)[j] = 10.0 * std::exp(-2.384 / Tkev[j]);
+    col(CR_BOLTCA2)[j] = 10.0 * std::exp(-1.697 / Tkev[j]);
+    col(CR_BOLTCA2 + 1)[j] = 6.0 * std::exp(-3.142 / Tkev[j]);
+    for (int n = 3; n < 6; n++) {
+      col(CR_PSEXP0 + n)[j] = std::exp(-kPsElim[n] * hckt);
+      col(CR_PSEXP1 + n)[j] = std::exp(-(kPsElim[n] - kPsEdge[n]) * hckt);
+    }
+    const double tlog10 = Tlog[j] / 2.30258509299405;
+    const int it = std::max(std::min((int)((tlog10 - 3.48) / 0.02), 50), 1);
+    col(CR_SI2IT)[j] = (double)it;
+    col(CR_SI2TF)[j] = (tlog10 - 3.48 - (double)it * 0.02) / 0.02;
+    col(CR_SI2BN)[j] = (1.0 * std::exp(-131838.4 * hckt) + 9.0 * std::exp(-184563.09 * hckt)) / (109732.298 * 4.0 * hckt);
+    const int prow[6] = {CR_FC2, CR_FMG2, CR_FSI2, CR_FN1, CR_FO1, CR_FCA2};
+    for (int q = 0; q < 6; q++) col(prow[q])[j] = (double)pops[q][j];
+  }
+}
+
+// ContopInit's factors of HOTOP (src/contop.f90:462-490): exp(E / kT) of the edges, the
+// free-free sums of the ions of charge 1-5 (HOTOP's sums over the elements, in its order),
+// FractionPF of the edges' species
+void hotop_rows(int ntau, const float *frac, const float *fpf, double *dep) {
+  const size_t N = (size_t)ntau;
+  auto col = [dep, N](int r) { return dep + (size_t)r * N; };
+  const double *Tkev = col(CR_TKEV);
+  for (size_t j = 0; j < N; j++) {
+    for (int i = 0; i < HOT_NEDGE; i++) col(CR_HOTEXP + i)[j] = std::exp(kHotA[7 * i + 5] / Tkev[j]);
+    const float *f = frac + j * 48;  // [element][stage]
+    for (int st = 2; st <= 6; st++) {
+      const int nel = st == 6 ? 7 : 8;  // no Fe VI
+      double sm = (double)f[st - 1];
+      for (int k = 1; k < nel; k++) sm = sm + (double)f[k * 6 + st - 1];
+      col(CR_HOTFF + st - 2)[j] = sm;
+    }
+    for (int q = 0; q < HOT_NFPF; q++) col(CR_XNFP + q)[j] = (double)fpf[j * HOT_NFPF + q];
+  }
+}
+
+// SI2OPINIT (src/contop.f90:2606-2706) on the host, as the reference builds it: ln of the
+// Si II opacity at 200 wavenumbers (1000 cm^-1 steps) and 51 temperatures (log T 3.50-4.50),
+// [nu][k]
+static void si2_table(double *out) {
+  const double ryd = 109732.298;
+  double zeff2[44], hck[51], b3s2[51], b3s3p[51], bolt[51][44];
+  for (int i = 0; i < 44; i++) zeff2[i] = (double)(kSi2Nlev[i] * kSi2Nlev[i]) / ryd * kSi2Tlev[i];
+  for (int k = 1; k <= 51; k++) {
+    const double ttab = std::pow(10.0, 3.48 + (double)k * 0.02);
+    hck[k - 1] = 6.6256e-27 * 2.99792458e10 / 1.38054e-16 / ttab;
+    b3s2[k - 1] = std::exp(-kSi2Elim[0] * hck[k - 1]);
+    b3s3p[k - 1] = std::exp(-kSi2Elim[11] * hck[k - 1]);
+    for (int i = 0; i < 44; i++) bolt[k - 1][i] = kSi2Glev[i] * std::exp(-kSi2Elev[i] * hck[k - 1]);
+  }
+  for (int nu = 1; nu <= CONT_SI2_NW; nu++) {
+    const double wno = (double)nu * 1000.0, fr = wno * 2.99792458e10;
+    const double f3 = 2.815e29 / fr / fr / fr * 16.0;
+    double x[44] = {0.0};
+    for (int i = 0; i < 11; i++)
+      if (!(wno < kSi2Tlev[i])) x[i] = xkarzas(fr, zeff2[i], kSi2Nlev[i], kSi2Llev[i]);
+    for (int i = 11; i < 37; i++) {
+      if (wno < kSi2Tlev[i]) break;
+      x[i] = xkarzas(fr, zeff2[i], kSi2Nlev[i], kSi2Llev[i]);
+    }
+    for (int i = 37; i < 41; i++) {
+      if (wno < kSi2Tlev[i]) break;
+      x[i] = xkarzas(fr, zeff2[i], kSi2Nlev[i], kSi2Llev[i]) * 2.0;
+    }
+    for (int i = 41; i < 44; i++) {
+      if (wno < kSi2Tlev[i]) break;
+      x[i] = xkarzas(fr, zeff2[i], kSi2Nlev[i], kSi2Llev[i]) * 3.0;
+    }
+    for (int k = 0; k < 51; k++) {
+      double h = f3 * kSi2Glev[44] / (ryd * 4.0 * hck[k]) *
+                 (std::exp(-std::fmax(kSi2Elev[44], kSi2Elim[44] - wno) * hck[k]) - b3s2[k]);
+      h = h + f3 * kSi2Glev[45] / (ryd * 4.0 * hck[k]) *
+                  (std::exp(-std::fmax(kSi2Elev[45], kSi2Elim[45] - wno) * hck[k]) - b3s3p[k]);
+      for (int i = 0; i < 44; i++) h = h + x[i] * bolt[k][i];
+      out[(size_t)(nu - 1) * CONT_SI2_NT + k] = std::log(std::fmax(h, 1.e-300));
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // host side: HMINOP's tables (src/contop.f90:183-208 and 1133-1162; Bell & Berrington 1987
 // free-free absorption, 1e-26 cm^4/dyn per H atom and unit electron pressure, at 22
 // wavelengths [um] x 11 temperatures theta = 5040/T; Wishart 1979 bound-free cross-section,
@@ -798,8 +1064,14 @@ const double kHmBfSigma[85] = {
 }  // namespace
 
 void contop_tables(std::vector<double> &tab) {
-  tab.assign((size_t)CONT_TAB_H2M + CONT_H2M_NW, 0.0);
+  tab.assign((size_t)CONT_TAB_SI2 + CONT_SI2_NW * CONT_SI2_NT, 0.0);
   for (int i = 0; i < CONT_H2M_NW; i++) tab[CONT_TAB_H2M + i] = std::log(kH2mWave[i]);
+  static std::vector<double> si2;  // atmosphere-independent: built once per process
+  if (si2.empty()) {
+    si2.resize((size_t)CONT_SI2_NW * CONT_SI2_NT);
+    si2_table(si2.data());
+  }
+  std::copy(si2.begin(), si2.end(), tab.begin() + CONT_TAB_SI2);
   double *w = tab.data(), *ff = w + CONT_HM_NW, *th = ff + CONT_HM_NW * CONT_HM_NT;
   double *wb = th + CONT_HM_NT, *sb = wb + CONT_HMBF_N;
   for (int i = 0; i < CONT_HM_NW; i++) w[i] = std::log(91.134 / kHmWavek[i]);

@@ -138,7 +138,10 @@ void launch_conv_rows(int nwl, int nrows, int maxk, const ConvRow *d_rows, const
 // without levels 1-6).
 constexpr int CONT_HM_NW = 22, CONT_HM_NT = 11, CONT_HMBF_N = 85, CONT_H2M_NW = 18;
 constexpr int CONT_TAB_H2M = CONT_HM_NW + CONT_HM_NW * CONT_HM_NT + CONT_HM_NT + 2 * CONT_HMBF_N;
+// then the Si II log-opacity table of SI2OPINIT [200 wavenumbers][51 temperatures]
+constexpr int CONT_SI2_NW = 200, CONT_SI2_NT = 51, CONT_TAB_SI2 = CONT_TAB_H2M + CONT_H2M_NW;
 constexpr int COOL_NC1 = 25, COOL_NMG1 = 15, COOL_NSI1 = 33, COOL_NFE1 = 48;
+constexpr int LUKE_NC2 = 34, LUKE_NMG2 = 14, HOT_NEDGE = 60, HOT_NFPF = 21;
 enum ContRow {
   CR_T, CR_HKT, CR_TKEV, CR_THETA, CR_XNE, CR_RHO, CR_FH1, CR_FH2, CR_FHMIN, CR_FHE1, CR_FHMOL, CR_TLOG,
   CR_BOLTH, CR_FREETH = CR_BOLTH + 8, CR_BOLTEXH, CR_EXLIMH,
@@ -146,10 +149,17 @@ enum ContRow {
   CR_BOLTHE2, CR_FREETHE2 = CR_BOLTHE2 + 9, CR_BOLTEXHE2, CR_EXLIMHE2,
   CR_HCKT, CR_TK,
   CR_BOLTC1, CR_BOLTMG1 = CR_BOLTC1 + COOL_NC1, CR_BOLTSI1 = CR_BOLTMG1 + COOL_NMG1,
-  CR_BOLTFE1 = CR_BOLTSI1 + COOL_NSI1, CR_PSEXP0 = CR_BOLTFE1 + COOL_NFE1, CR_PSEXP1 = CR_PSEXP0 + 3,
-  CR_H2MLK = CR_PSEXP1 + 3, CR_NHN = CR_H2MLK + CONT_H2M_NW, CR_NHW, CR_NHNRHO,
+  CR_BOLTFE1 = CR_BOLTSI1 + COOL_NSI1, CR_PSEXP0 = CR_BOLTFE1 + COOL_NFE1, CR_PSEXP1 = CR_PSEXP0 + 6,
+  CR_H2MLK = CR_PSEXP1 + 6, CR_NHN = CR_H2MLK + CONT_H2M_NW, CR_NHW, CR_NHNRHO,
   CR_FC1, CR_FMG1, CR_FAL1, CR_FSI1, CR_FFE1, CR_FCH, CR_FOH, CR_FRH1, CR_FRHE1,
-  CR_NROW
+  // LUKEOP: level Boltzmann factors of C II, Mg II, N I, Ca II, the Si II table's temperature
+  // interval, weight and long-wavelength factor, the populations
+  CR_BOLTC2, CR_BOLTMG2 = CR_BOLTC2 + LUKE_NC2, CR_BOLTN1 = CR_BOLTMG2 + LUKE_NMG2, CR_BOLTCA2 = CR_BOLTN1 + 2,
+  CR_SI2IT = CR_BOLTCA2 + 2, CR_SI2TF, CR_SI2BN, CR_FC2, CR_FMG2, CR_FSI2, CR_FN1, CR_FO1, CR_FCA2,
+  // HOTOP: exp(excitation / kT) of the 60 edges, the free-free sums of the ions of charge 1-5,
+  // FractionPF of C I-IV, N I-V, O I-VI, Ne I-VI
+  CR_HOTEXP, CR_HOTFF = CR_HOTEXP + HOT_NEDGE, CR_XNFP = CR_HOTFF + 5,
+  CR_NROW = CR_XNFP + HOT_NFPF
 };
 constexpr unsigned CONT_HOP_NOLEV = 1u << 20;
 void launch_contop(int nwl, int ntau, const double *wave, double wfirst, const double *dep, const double *tab,
@@ -163,6 +173,17 @@ void contop_tables(std::vector<double> &tab);  // host: the table block of launc
 // host: COOLOP's rows of dep (T, Hkt and Rho rows filled first); pops: FractionPF of C I,
 // Mg I, Al I, Si I, Fe I, CH, OH, then Fraction of NH, H I, He I, each [ntau]
 void coolop_rows(int ntau, const float *const pops[10], double *dep);
+// host: LUKEOP's rows (after coolop_rows' depth state); pops: FractionPF of C II, Mg II,
+// Si II, N I, O I, Ca II, each [ntau]
+void lukeop_rows(int ntau, const float *const pops[6], double *dep);
+// host: HOTOP's rows; frac [ntau][8][6] Fraction of stages I-VI of C, N, O, Ne, Mg, Si, S,
+// Fe; fpf [ntau][21] FractionPF of C I-IV, N I-V, O I-VI, Ne I-VI
+void hotop_rows(int ntau, const float *frac, const float *fpf, double *dep);
+// LUKEOP (Ifop 10) and HOTOP (Ifop 11) added into contT: mask bit 9 and 10 as launch_contop's;
+// xs holds luke_scratch_per_point() doubles per wavelength
+void launch_lukehot(int nwl, int ntau, const double *wave, double wfirst, const double *dep, const double *tab,
+                    const double *cstd, int motype, unsigned mask, double *xs, double *contT, hipStream_t st);
+constexpr int luke_scratch_per_point() { return LUKE_NC2 + LUKE_NMG2; }
 void launch_mu_sum(int nwl, int nmu, const double *rl, const double *rc, const double *w, double *sfl, double *sfc,
                    hipStream_t st);
 

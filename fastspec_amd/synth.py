@@ -139,16 +139,17 @@ class FastSpecGPU:
             self.setup = new
 
     # --- continuum opacity --------------------------------------------------------------
-    # HOP H2PLOP HMINOP HRAYOP HE1OP HE2OP HEMIOP HERAOP COOLOP ELECOP H2RAOP
-    CONTOP_DEVICE_SOURCES = (1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 13)
+    # HOP H2PLOP HMINOP HRAYOP HE1OP HE2OP HEMIOP HERAOP COOLOP LUKEOP HOTOP ELECOP H2RAOP
+    CONTOP_DEVICE_SOURCES = tuple(range(1, 14))
 
     def ContopCalc(self, sources=CONTOP_DEVICE_SOURCES, hop_nolev: bool = False, add: bool = False,
                    copy_out: bool = True, **pops):
         """ContopCalc's loop (src/synth.f90:663-669) for the continuum sources the device
-        evaluates (fsg_contop): sources are Ifop numbers among 1-9, 12, 13; pops are the EOS
-        rows [ntau] they read, by the names of fsg_contop_eos (FractionPF of H I, H II, He I,
-        He II, He III, C I, Mg I, Al I, Si I, Fe I, CH and OH; Fraction of H-, H2, He II, NH,
-        H I and He I).  hop_nolev is the reference's Ifop(1) = -1 (HOP without levels 1-6,
+        evaluates (fsg_contop): sources are Ifop numbers 1-13; pops are the EOS rows [ntau]
+        they read, by the names of fsg_contop_eos (FractionPF of H I, H II, He I, He II,
+        He III, C I, Mg I, Al I, Si I, Fe I, CH, OH, C II, Mg II, Si II, N I, O I and Ca II;
+        Fraction of H-, H2, He II, NH, H I and He I; HOTOP's hot_frac [ntau][8][6] and
+        hot_fpf [ntau][21]).  hop_nolev is the reference's Ifop(1) = -1 (HOP without levels 1-6,
         which a hydrogen line list supplies).  add=False replaces the context's ContOpac with
         their sum, add=True adds it to the ContOpac the context holds.  Returns the resulting
         ContOpac [nwl][ntau], which the host copy of the set-up takes too; copy_out=False
@@ -164,8 +165,9 @@ class FastSpecGPU:
                 raise TypeError(f"ContopCalc: unknown population {k!r}")
             if v is not None:
                 a = np.ascontiguousarray(v, np.float32)
-                if a.shape != (n,):
-                    raise ValueError(f"ContopCalc: {k} must have ntau values")
+                shape = {"hot_frac": (n, 8, 6), "hot_fpf": (n, 21)}.get(k, (n,))
+                if a.shape != shape:
+                    raise ValueError(f"ContopCalc: {k} must have the shape {shape}")
                 arrs[k] = a
         mask = 0
         for src in sources:

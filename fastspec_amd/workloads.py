@@ -239,13 +239,28 @@ COOLOP_MOLECULES = [("ch", -7.5, 3.47, 3000.0, 200.0), ("oh", -7.0, 4.39, 3000.0
                     ("nh", -8.0, 3.47, 3000.0, 1.0)]
 
 
+# (name, ionisation energies of stages I-V [eV], log abundance) of HOTOP's elements in its
+# order (C, N, O, Ne, Mg, Si, S, Fe), and Ca for LUKEOP's Ca II
+HOT_ELEMENTS = [("c", (11.260, 24.383, 47.888, 64.494, 392.09), -3.57),
+                ("n", (14.534, 29.601, 47.449, 77.473, 97.890), -4.17),
+                ("o", (13.618, 35.121, 54.936, 77.414, 113.90), -3.31),
+                ("ne", (21.565, 40.963, 63.45, 97.12, 126.21), -4.07),
+                ("mg", (7.646, 15.035, 80.144, 109.27, 141.27), -4.40),
+                ("si", (8.152, 16.346, 33.493, 45.142, 166.77), -4.49),
+                ("s", (10.360, 23.338, 34.79, 47.222, 72.594), -4.88),
+                ("fe", (7.902, 16.199, 30.652, 54.91, 75.0), -4.50),
+                ("ca", (6.113, 11.872, 50.913, 67.27, 84.5), -5.66)]
+HOT_PF = np.array([6.0, 5.0, 4.0, 3.0, 2.0, 1.0])
+
+
 def continuum_populations(setup: Setup) -> dict:
     """The EOS rows the device's continuum sources read, for a set-up made by build_setup:
     FractionPF of H I, H II, He I and He II (their slots), Fraction of H2 and He II (twice
     its FractionPF, partition function 2), Fraction of H- from the Saha equation against H I
     (binding 0.754 eV, statistical weights 1 and 2), FractionPF of He III from the Saha
-    equation against He II (54.418 eV, partition functions 2 and 1), and COOLOP's rows
-    (COOLOP_ATOMS, COOLOP_MOLECULES, Fraction of H I and He I)."""
+    equation against He II (54.418 eV, partition functions 2 and 1), COOLOP's rows
+    (COOLOP_ATOMS, COOLOP_MOLECULES, Fraction of H I and He I), and LUKEOP's and HOTOP's
+    (HOT_ELEMENTS' ionisation stages)."""
     nsp = int(setup.meta["n_atomic"])  # H I, He I, H2, H II, He II follow the atomic slots
     ev_k = EV / KB
     t = setup.temp
@@ -271,7 +286,28 @@ def continuum_populations(setup: Setup) -> dict:
         pops[("frac_" if name == "nh" else "fpf_") + name] = (n_m / u).astype(np.float32)
     pops["frac_h1"] = setup.frac_h1.astype(np.float32)
     pops["frac_he1"] = setup.frac_he1.astype(np.float32)
+    # LUKEOP's and HOTOP's rows: stages I-VI of each element from a Saha chain with constant
+    # partition functions (HOT_PF)
+    stages = {name: ion_stages(t, setup.xne, n_h, chis, logab) for name, chis, logab in HOT_ELEMENTS}
+    hot = np.stack([stages[name] for name, _, _ in HOT_ELEMENTS if name != "ca"], axis=1)  # [ntau][8][6]
+    pops["hot_frac"] = hot.astype(np.float32)
+    nst = (4, 5, 6, 6)  # C I-IV, N I-V, O I-VI, Ne I-VI
+    pops["hot_fpf"] = np.concatenate([hot[:, k, :m] / HOT_PF[:m] for k, m in enumerate(nst)], axis=1).astype(np.float32)
+    for key, (name, st) in dict(fpf_c2=("c", 1), fpf_mg2=("mg", 1), fpf_si2=("si", 1), fpf_n1=("n", 0),
+                                fpf_o1=("o", 0), fpf_ca2=("ca", 1)).items():
+        pops[key] = (stages[name][:, st] / HOT_PF[st]).astype(np.float32)
     return pops
+
+
+def ion_stages(t, ne, n_h, chis, logab):
+    """Number densities of stages I-VI [ntau][6] of an element of abundance logab from the
+    Saha equation with the ionisation energies chis (eV) and partition functions HOT_PF."""
+    n = np.zeros((t.shape[0], 6))
+    n[:, 0] = 1.0
+    for i, chi in enumerate(chis[:5]):
+        n[:, i + 1] = n[:, i] * saha_ratio(t, ne, chi, HOT_PF[i], HOT_PF[i + 1])
+    n /= n.sum(axis=1, keepdims=True)
+    return n * (10.0**logab * n_h)[:, None]
 
 
 def _f32pow10(x):

@@ -333,10 +333,10 @@ int fsg_set_atmosphere(fsg_ctx *ctx, int32_t atm, const fsg_atmosphere *a);
 /* Continuum opacity on the device: ContopCalc's loop over the grid (src/synth.f90:663-669,
  * Xcont at Air2Vac of each wavelength, divided by ContOpacStd for Motype 1) for the sources
  * of Xcont (src/contop.f90:267-314) that need only hydrogen, helium and electron
- * populations, and COOLOP's atoms and molecules.  ifop_mask bit s-1 switches source s of the
- * reference's Ifop numbering; the device evaluates 1 HOP, 2 H2PLOP, 3 HMINOP, 4 HRAYOP, 5 HE1OP,
- * 6 HE2OP, 7 HEMIOP, 8 HERAOP, 9 COOLOP, 12 ELECOP and 13 H2RAOP (bits 9 and 10, LUKEOP and
- * HOTOP, are refused: the host's Xcont keeps them).  Bit FSG_CONTOP_HOP_NOLEV (with bit 0) is the reference's Ifop(1) = -1, set
+ * populations, COOLOP's atoms and molecules, LUKEOP's and HOTOP's ions: all thirteen.
+ * ifop_mask bit s-1 switches source s of the reference's Ifop numbering: 1 HOP, 2 H2PLOP,
+ * 3 HMINOP, 4 HRAYOP, 5 HE1OP, 6 HE2OP, 7 HEMIOP, 8 HERAOP, 9 COOLOP, 10 LUKEOP, 11 HOTOP,
+ * 12 ELECOP, 13 H2RAOP.  Bit FSG_CONTOP_HOP_NOLEV (with bit 0) is the reference's Ifop(1) = -1, set
  * by ContopSetup when a hydrogen line list supplies the bound-free edges of levels 1-6
  * (src/synth.f90:600): HOP then omits those levels.  T, Xne and Rho are the context's
  * (fsg_init / fsg_set_atmosphere); the populations are the EOS rows the sources read, as
@@ -367,6 +367,17 @@ typedef struct fsg_contop_eos {
   const float *frac_nh;    /* [ntau] Fraction(IxNH, :)     NH    (NHOP) */
   const float *frac_h1;    /* [ntau] Fraction(IxH1, :)     H I   (H2COLLOP) */
   const float *frac_he1;   /* [ntau] Fraction(IxHe1, :)    He I  (H2COLLOP; with frac_hmol, H2COLLOP and H2MINOP) */
+  /* LUKEOP (src/contop.f90:2422-2440) */
+  const float *fpf_c2;     /* [ntau] FractionPF(IxC2, :)   C II  (C2OP) */
+  const float *fpf_mg2;    /* [ntau] FractionPF(IxMg2, :)  Mg II (MG2OP) */
+  const float *fpf_si2;    /* [ntau] FractionPF(IxSi2, :)  Si II (SI2OP) */
+  const float *fpf_n1;     /* [ntau] FractionPF(IxN1, :)   N I   (N1OP) */
+  const float *fpf_o1;     /* [ntau] FractionPF(IxO1, :)   O I   (O1OP) */
+  const float *fpf_ca2;    /* [ntau] FractionPF(IxCa2, :)  Ca II (CA2OP) */
+  /* HOTOP (src/contop.f90:2729-2763): the rows ContopInit takes through IxHot
+   * (src/contop.f90:471-490) */
+  const float *hot_frac;   /* [ntau][8][6] Fraction of stages I-VI of C, N, O, Ne, Mg, Si, S, Fe */
+  const float *hot_fpf;    /* [ntau][21] FractionPF of C I-IV, N I-V, O I-VI, Ne I-VI */
 } fsg_contop_eos;
 int fsg_contop(fsg_ctx *ctx, const fsg_contop_eos *eos, int32_t ifop_mask, int32_t add, double *contop_out);
 

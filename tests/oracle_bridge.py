@@ -507,13 +507,15 @@ def ref_xcont(temp, xne, rho, pops: dict, sources, wl_vac, xna=None, hop_nolev=F
     wavelengths wl_vac with only the Ifop sources given switched on (hop_nolev: Ifop(1) = -1):
     Copac [n][ntau].  pops holds the EOS rows (fsg_contop_eos's names) that the harness puts
     in its slots 1 H I, 2 H II, 3 H-, 4 He I, 5 H2, 6 He II, 7 He III, 8 C I, 9 Mg I, 10 Al I,
-    11 Si I, 12 Fe I, 13 CH, 14 OH, 15 NH (16, the other species, empty); rows not given stay
-    zero."""
+    11 Si I, 12 Fe I, 13 CH, 14 OH, 15 NH (16, the other species, empty), 17 C II, 18 Mg II,
+    19 Si II, 20 N I, 21 O I, 22 Ca II and 23-70 HOTOP's stages I-VI of C, N, O, Ne, Mg, Si,
+    S, Fe (hot_frac [ntau][8][6]; hot_fpf [ntau][21] the FractionPF of C I-IV, N I-V, O I-VI,
+    Ne I-VI); rows not given stay zero."""
     lib = ref_lib()
     n = int(len(wl_vac))
     ntau = int(len(temp))
-    fpf = np.zeros((ntau, 16), np.float32)
-    frac = np.zeros((ntau, 16), np.float32)
+    fpf = np.zeros((ntau, 70), np.float32)
+    frac = np.zeros((ntau, 70), np.float32)
     fpf[:, 0] = pops["fpf_h1"]
     fpf[:, 1] = pops["fpf_h2"]
     frac[:, 2] = pops["frac_hmin"]
@@ -537,6 +539,17 @@ def ref_xcont(temp, xne, rho, pops: dict, sources, wl_vac, xna=None, hop_nolev=F
             fpf[:, slot] = frac[:, slot] = pops[key]
     if "frac_nh" in pops:
         fpf[:, 14] = frac[:, 14] = pops["frac_nh"]
+    for slot, key in ((16, "fpf_c2"), (17, "fpf_mg2"), (18, "fpf_si2"), (19, "fpf_n1"), (20, "fpf_o1"),
+                      (21, "fpf_ca2")):
+        if key in pops:
+            fpf[:, slot] = frac[:, slot] = pops[key]
+    if "hot_frac" in pops:
+        hf = np.asarray(pops["hot_frac"], np.float32).reshape(ntau, 48)
+        frac[:, 22:70] = hf
+        fpf[:, 22:70] = hf
+        # the FractionPF rows the bound-free edges read: C I-IV, N I-V, O I-VI, Ne I-VI
+        cols = [22 + 6 * k + i for k, nst in enumerate((4, 5, 6, 6)) for i in range(nst)]
+        fpf[:, cols] = np.asarray(pops["hot_fpf"], np.float32).reshape(ntau, 21)
     ifop = np.zeros(20, np.int32)
     for s in sources:
         ifop[int(s) - 1] = 1

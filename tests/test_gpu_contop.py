@@ -6,7 +6,8 @@ model (He II populated), the far UV of the 15000 K model (220-950 A: the He II, 
 H I edges, and the hydrogenic He I levels below 240 A), and COOLOP's ranges on cool models:
 the C I, Si I, Mg I and Al I edges in the far UV of a 3500 K model, Fe I, CH, OH and NH in
 its visible, and the collision-induced and H2- free-free absorption in the red and infrared
-of a 3000 K model.  Bound: 1e-12 relative per
+of a 3000 K model; LUKEOP's and HOTOP's ions on the solar, hot and far-UV grids.  Bound:
+1e-12 relative per
 (wavelength, depth), every source alone and all together; HOP without its levels 1-6
 (Ifop(1) = -1); add mode, Motype 1 and the refusal of sources the host keeps; and a
 synthesis that uses the device continuum against the oracle's with the reference
@@ -22,7 +23,7 @@ from oracle_bridge import Oracle, ref_air2vac_array, ref_xcont
 
 pytestmark = [pytest.mark.gpu, pytest.mark.ref]
 
-SOURCES = (1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 13)
+SOURCES = tuple(range(1, 14))
 REL = 1e-12
 # H2PLOP's exponent is a quartic in ln(frequency) whose terms, ~1e4, cancel to ~-90: one ulp
 # of the logarithm (device and host libm may differ by one) moves H2+ opacity by ~2e-12
@@ -73,7 +74,8 @@ def test_all_device_sources_match_reference(name):
 # models (He II is empty in a solar model), COOLOP on the cool grids and on the hot near UV
 # (CH and OH above 9000 K and NH above 10000 K add nothing)
 EACH = ([(src, "c1") for src in SOURCES if src != 6] + [(src, g) for src in (1, 5, 6) for g in ("hot", "fuv")] +
-        [(9, g) for g in ("cool_fuv", "cool_vis", "cool_red", "cool_ir", "uv", "c3")])
+        [(9, g) for g in ("cool_fuv", "cool_vis", "cool_red", "cool_ir", "uv", "c3")] +
+        [(src, g) for src in (10, 11) for g in ("c1", "uv", "hot", "fuv", "cool_fuv", "ir_edge")])
 
 
 @pytest.mark.parametrize("src,grid", EACH)
@@ -121,16 +123,21 @@ def test_add_mode_and_motype1():
     assert np.max(np.abs(got1 - ref1) / ref1) <= REL
 
 
-def test_host_sources_refused():
+def test_missing_populations_refused():
     from fastspec_amd._lib import FsgError
     s = GRIDS["c1"]
     pops = W.continuum_populations(s)
-    for src in (10, 11):
-        with pytest.raises(FsgError, match="not evaluated on the device"):
-            _gpu(s, pops, SOURCES + (src,))
+    with pytest.raises(FsgError, match="not evaluated on the device"):
+        _gpu(s, pops, SOURCES + (14,))
     for key in ("fpf_c1", "fpf_ch", "frac_nh", "frac_he1"):
         with pytest.raises(FsgError, match="COOLOP needs"):
             _gpu(s, dict(pops, **{key: None}), (9,))
+    for key in ("fpf_c2", "fpf_si2", "fpf_ca2"):
+        with pytest.raises(FsgError, match="LUKEOP needs"):
+            _gpu(s, dict(pops, **{key: None}), (10,))
+    for key in ("hot_frac", "hot_fpf"):
+        with pytest.raises(FsgError, match="HOTOP needs"):
+            _gpu(s, dict(pops, **{key: None}), (11,))
     with pytest.raises(FsgError, match="population"):
         _gpu(s, dict(pops, frac_hmin=None), (3,))
     with pytest.raises(FsgError, match="population"):
