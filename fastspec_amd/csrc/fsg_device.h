@@ -234,15 +234,21 @@ __device__ __forceinline__ int wave_idx(const DevSetup &S, double W) {
   return (int)fmin(fmax(X, 1.0), (double)(S.nwl - 1));
 }
 
-// XCD-aware tile order: consecutive block ids go round-robin over the 8 XCDs, so map
-// them so that each XCD works on a contiguous run of tiles (shared lines stay in its L2).
-// The grid holds 8 * per blocks (per = ceil(ntiles / 8)), so bid -> x * per + r is a
-// bijection onto [0, 8 * per) and every tile below ntiles is visited exactly once.
+// XCD-aware tile order: consecutive block ids go round-robin over the 8 XCDs.  Each XCD
+// takes runs of XCD_RUN consecutive tiles (lines shared by neighbouring tiles stay in its
+// L2), and the runs go round-robin over the XCDs, so that work concentrated in part of the
+// grid (a block of a bin-sorted list covers a slice of the wavelengths) still spreads over
+// all of them.  The grid holds 8 * per blocks; the rows of complete runs map run by run, the
+// rest contiguously per XCD: a bijection onto [0, 8 * per), every tile below ntiles visited
+// exactly once.
+constexpr int XCD_RUN = 16;
 __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
   const int NX = 8;
-  int per = nblk / NX;
-  int x = bid % NX, r = bid / NX;
-  return x * per + r;
+  const int per = nblk / NX;
+  const int x = bid % NX, r = bid / NX;
+  const int full = (per / XCD_RUN) * XCD_RUN;
+  if (r < full) return ((r / XCD_RUN) * NX + x) * XCD_RUN + r % XCD_RUN;
+  return full * NX + x * (per - full) + (r - full);
 }
 
 __device__ __forceinline__ void kahan_add(float &acc, float &cmp, float val) {
