@@ -51,7 +51,7 @@ def measured_traffic():
     try:
         for line in TRAFFIC_MD.read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
+                return float(line.split(":", 1)[1].split()[0])
     except OSError:
         pass
     return None

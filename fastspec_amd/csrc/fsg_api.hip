@@ -1577,6 +1577,8 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
       int2 *zone = reinterpret_cast<int2 *>(carve((size_t)m * sizeof(int2)));
       LineRec *lrec = reinterpret_cast<LineRec *>(carve((size_t)(m + 1) * sizeof(LineRec)));  // + an empty one
       const int *order = c->idx2.as<int>() + s0;
+      // chunk positions below mnarrow hold the lines of the narrow bins
+      const int64_t mnarrow = nbn > 0 ? std::min<int64_t>(std::max<int64_t>(goff[nbn] - s0, 0), m) : 0;
       if (c->precision == FSG_FP32)
         launch_fill32(S, dl->L, dl->list_type, m, order, c->flag.as<int>(), c->il.as<int>(), c->ir.as<int>(),
                       c->ic.as<int>(), ca, cb, dmaxw, zone, lrec, c->wlc.as<double>(), c->meta.as<int4>(), s);
@@ -1585,8 +1587,6 @@ int fsg_lineop_dev(fsg_ctx *c, fsg_dlines *dl, int64_t nlines_lim, fsg_lineop_st
                       c->ic.as<int>(), c->coef.as<Coef64>(), c->wlc.as<double>(), c->meta.as<int4>(), s);
       launch_tile_ranges(ntiles, TILE, ng, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
                          c->meta.as<int4>(), c->rng.as<int2>(), s);
-      // chunk positions below mnarrow hold the lines of the narrow bins
-      const int64_t mnarrow = nbn > 0 ? std::min<int64_t>(std::max<int64_t>(goff[nbn] - s0, 0), m) : 0;
       if (mnarrow > 0)
         launch_tile_ranges(ntiles_n, NARROW_TILE, nbn, S.nwl, c->binoff, c->binoff + ng, c->binoff + 2 * ng,
                            c->meta.as<int4>(), c->rngn.as<int2>(), s);

@@ -180,14 +180,23 @@ __global__ void k_iota(int *v, int64_t n) {
   if (i < n) v[i] = (int)i;
 }
 
-// Sum over retained lines of window pixels x Ntau: the metric's evaluation count
-__global__ void k_count_evals(int64_t n, const int *flag, const int *il, const int *ir, int ntau,
-                              unsigned long long *acc) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Sum over retained lines of window pixels x Ntau: the metric's evaluation count.  A
+// grid-stride loop with one atomic per workgroup (a single counter takes every atomic, so
+// one per wavefront over 10^8 lines would serialise on it).
+constexpr int COUNT_BLOCKS = 2048;
+__global__ void __launch_bounds__(256) k_count_evals(int64_t n, const int *flag, const int *il, const int *ir, int ntau,
+                                                     unsigned long long *acc) {
+  __shared__ unsigned long long part[4];
   unsigned long long v = 0;
-  if (i < n && !(flag[i] & 1)) v = (unsigned long long)(ir[i] - il[i] + 1) * (unsigned long long)ntau;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (!(flag[i] & 1)) v += (unsigned long long)(ir[i] - il[i] + 1) * (unsigned long long)ntau;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(acc, v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(acc, t);
+  }
 }
 
 // MarkWideLines criterion (src/synth.f90:709-744) over the (packed or full) list of
@@ -237,87 +246,101 @@ __global__ void k_retained_flag(int64_t n, const int *flag, int *out) {
 }
 
 // ------------------------------------------------------------------------------------
-// coefficient fill: one lane per retained line in bin-sorted order
+// coefficient fill: one wavefront per 64 retained lines in bin-sorted order.  Lanes take the
+// lines for XlineSetup and the core-zone search, and the flattened (line, depth) pairs for
+// the records, so the records of consecutive lines go out as one contiguous stream.
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) k_fill32(DevSetup S0, DevLines L, int list_type, int64_t nret, const int *order, const int *flag,
-                         const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
-                         int2 *zone_s, LineRec *lrec, double *wlc_s, int4 *meta_s) {
-  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nret) return;
-  const DevSetup S = line_setup(S0, L, order[j]);
-  XC xc;
-  line_meta(S, L, list_type, j, order, flag, il, ir, ic, xc, wlc_s, meta_s);
-  const int N = S.ntau;
-  CoefA *A = ca + (size_t)j * S.ntp;
-  CoefB *B = cb + (size_t)j * S.ntp;
-  for (int k = N; k < S.ntp; k++) {  // padding depths contribute exactly zero on either path
+constexpr int FILL_LINES = 64;
+__global__ void __launch_bounds__(64) k_fill32(DevSetup S0, DevLines L, int list_type, int64_t nret,
+                         const int *order, const int *flag, const int *il, const int *ir, const int *ic, CoefA *ca,
+                         CoefB *cb, float *dmaxw, int2 *zone_s, LineRec *lrec, double *wlc_s, int4 *meta_s) {
+  __shared__ XC sxc[FILL_LINES];
+  __shared__ unsigned sgm[FILL_LINES * 16];  // per depth group (ntau <= 144: at most 16): largest dthr (bits)
+  const int lane = threadIdx.x;
+  const int64_t j0 = (int64_t)blockIdx.x * FILL_LINES;
+  const int nl = (int)min((int64_t)FILL_LINES, nret - j0);
+  const int ntp = S0.ntp, N = S0.ntau, G = ntp / DPW;  // one depth count for every line of a batch
+  for (int i = lane; i < FILL_LINES * G; i += 64) sgm[i] = 0u;
+  if (lane < nl) {
+    const int64_t j = j0 + lane;
+    const DevSetup S = line_setup(S0, L, order[j]);
+    line_meta(S, L, list_type, j, order, flag, il, ir, ic, sxc[lane], wlc_s, meta_s);
+  }
+  __syncthreads();
+  CoefA *A = ca + (size_t)j0 * ntp;
+  CoefB *B = cb + (size_t)j0 * ntp;
+  const int np = nl * ntp;
+  for (int p = lane; p < np; p += 64) {
+    const int m = p / ntp, k = p - m * ntp;
     CoefA a{};
     CoefB b{};
-    b.a2 = 1e4f;
-    b.c1 = 1e4f + 0.25f;
-    A[k] = a;
-    B[k] = b;
+    if (k < N) {
+      const DevSetup S = line_setup(S0, L, order[j0 + m]);
+      const XC &xc = sxc[m];
+      double cop, vv, av, ua;
+      int ia;
+      xline_depth(S, k, xc, cop, vv, av);
+      voigt_astate(S, av, ia, ua);
+      const double a2 = av * av, K = cop * av * kSqrtPiInv;
+      const double al = 2.0 * a2 - 1.0, be = a2 + 0.5;
+      // series of (1 + be u)/(1 + al u + be^2 u^2): f_n = -al f_{n-1} - be^2 f_{n-2}
+      const double f1 = be - al;
+      const double f2 = -al * f1 - be * be;
+      const double w = 1.0 / (vv * vv);
+      a.k0 = (float)(K * w);
+      a.k1 = (float)(K * f1 * w * w);
+      a.k2 = (float)(K * f2 * w * w * w);
+      a.dthr = (float)(sqrt(fmax(196.0, 400.0 * be)) / vv * (1.0 + 1e-4));
+      b.vv = (float)vv;
+      b.a = (float)av;
+      b.a2 = (float)a2;
+      b.c1 = (float)(a2 + 0.25);
+      b.ka = (float)K;
+      b.cop = (float)cop;
+      b.ua = (float)ua;
+      b.ia = ia;
+      // the thresholds are finite and >= 0, so their bit patterns order as the values do
+      // (a NaN is skipped, as fmaxf would)
+      if (a.dthr > 0.0f) atomicMax(&sgm[p / DPW], __float_as_uint(a.dthr));
+    } else {  // padding depths contribute exactly zero on either path
+      b.a2 = 1e4f;
+      b.c1 = 1e4f + 0.25f;
+    }
+    A[p] = a;
+    B[p] = b;
   }
-  for (int k = 0; k < N; k++) {
-    double cop, vv, av, ua;
-    int ia;
-    xline_depth(S, k, xc, cop, vv, av);
-    voigt_astate(S, av, ia, ua);
-    const double a2 = av * av, K = cop * av * kSqrtPiInv;
-    const double al = 2.0 * a2 - 1.0, be = a2 + 0.5;
-    // series of (1 + be u)/(1 + al u + be^2 u^2): f_n = -al f_{n-1} - be^2 f_{n-2}
-    const double f1 = be - al;
-    const double f2 = -al * f1 - be * be;
-    const double w = 1.0 / (vv * vv);
-    CoefA a;
-    a.k0 = (float)(K * w);
-    a.k1 = (float)(K * f1 * w * w);
-    a.k2 = (float)(K * f2 * w * w * w);
-    a.dthr = (float)(sqrt(fmax(196.0, 400.0 * be)) / vv * (1.0 + 1e-4));
-    CoefB b;
-    b.vv = (float)vv;
-    b.a = (float)av;
-    b.a2 = (float)a2;
-    b.c1 = (float)(a2 + 0.25);
-    b.ka = (float)K;
-    b.cop = (float)cop;
-    b.ua = (float)ua;
-    b.ia = ia;
-    A[k] = a;
-    B[k] = b;
-  }
+  __syncthreads();
   // per wavefront depth group: the largest far-wing threshold (padding depths: 0)
+  for (int i = lane; i < nl * G; i += 64) dmaxw[(size_t)j0 * G + i] = __uint_as_float(sgm[i]);
+  if (lane >= nl) return;
+  const int64_t j = j0 + lane;
+  const DevSetup S = line_setup(S0, L, order[j]);
   float zmax = 0.0f;
-  for (int w = 0; w < S.ntp / DPW; w++) {
-    float mx = 0.0f;
-    for (int k = w * DPW; k < (w + 1) * DPW; k++) mx = fmaxf(mx, A[k].dthr);
-    dmaxw[(size_t)j * (S.ntp / DPW) + w] = mx;
-    zmax = fmaxf(zmax, mx);
-  }
+  for (int w = 0; w < G; w++) zmax = fmaxf(zmax, __uint_as_float(sgm[lane * G + w]));
   // core zone of the line: the grid points with |lambda - lambda_c| < max_k dthr (1-based,
   // empty when zl > zr); outside it the far-wing expansion holds at every depth
   const double wc = wlc_s[j], zlo = wc - (double)zmax, zhi = wc + (double)zmax;
-  int a = 0, e = S.nwl;
-  while (a < e) {  // first point above zlo
-    const int mid = (a + e) >> 1;
+  int lo = 0, e = S.nwl;
+  while (lo < e) {  // first point above zlo
+    const int mid = (lo + e) >> 1;
     if (S.wave[mid] > zlo) e = mid;
-    else a = mid + 1;
+    else lo = mid + 1;
   }
-  const int zl = a + 1;
-  a = 0;
+  const int zl = lo + 1;
+  lo = 0;
   e = S.nwl;
-  while (a < e) {  // first point at or above zhi
-    const int mid = (a + e) >> 1;
+  while (lo < e) {  // first point at or above zhi
+    const int mid = (lo + e) >> 1;
     if (S.wave[mid] >= zhi) e = mid;
-    else a = mid + 1;
+    else lo = mid + 1;
   }
-  zone_s[j] = make_int2(zl + S.goff, a + S.goff);
+  zone_s[j] = make_int2(zl + S.goff, lo + S.goff);
   const int4 mm = meta_s[j];
   LineRec r;
   r.il = mm.x;
   r.ir = mm.y;
   r.zl = zl + S.goff;
-  r.zr = a + S.goff;
+  r.zr = lo + S.goff;
   r.wc = wc;
   r.pad0 = r.pad1 = 0;
   lrec[j] = r;
@@ -1641,7 +1664,7 @@ void launch_iota(int *v, int64_t n, hipStream_t st) {
 
 void launch_count_evals(int64_t n, const int *flag, const int *il, const int *ir, int ntau,
                         unsigned long long *acc, hipStream_t st) {
-  if (n) k_count_evals<<<nblk(n, 256), 256, 0, st>>>(n, flag, il, ir, ntau, acc);
+  if (n) k_count_evals<<<(unsigned)std::min<int64_t>(COUNT_BLOCKS, nblk(n, 256)), 256, 0, st>>>(n, flag, il, ir, ntau, acc);
 }
 
 void launch_retained_flag(int64_t n, const int *flag, int *out, hipStream_t st) {
@@ -1689,7 +1712,7 @@ void launch_fill32(const DevSetup &S, const DevLines &L, int list_type, int64_t 
                    const int *flag, const int *il, const int *ir, const int *ic, CoefA *ca, CoefB *cb, float *dmaxw,
                    int2 *zone_s, LineRec *lrec, double *wlc_s, int4 *meta_s, hipStream_t st) {
   if (nret)
-    k_fill32<<<nblk(nret, 128), 128, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, ca, cb, dmaxw, zone_s,
+    k_fill32<<<nblk(nret, FILL_LINES), 64, 0, st>>>(S, L, list_type, nret, order, flag, il, ir, ic, ca, cb, dmaxw, zone_s,
                                               lrec, wlc_s, meta_s);
 }
 
