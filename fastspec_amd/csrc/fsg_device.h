@@ -240,8 +240,9 @@ __device__ __forceinline__ int wave_idx(const DevSetup &S, double W) {
 // grid (a block of a bin-sorted list covers a slice of the wavelengths) still spreads over
 // all of them.  The grid holds 8 * per blocks; the rows of complete runs map run by run, the
 // rest contiguously per XCD: a bijection onto [0, 8 * per), every tile below ntiles visited
-// exactly once.
-constexpr int XCD_RUN = 16;
+// exactly once.  Runs of 4 to 64 tiles take the same time; the longest keeps the most
+// lines of a core zone (several 64-point tiles wide) within one L2.
+constexpr int XCD_RUN = 64;
 __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
   const int NX = 8;
   const int per = nblk / NX;
